@@ -1,0 +1,350 @@
+"""ctypes binding of ``include/idp_amd.h`` (the C ABI of libidp_amd.so).
+
+The HIP library is mandatory: importing this module raises if the shared
+library has not been built; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libidp_amd.so")
+
+IDP_OK = 0
+IDP_E_INVALID = -1
+IDP_E_HIP = -2
+IDP_E_NONPOS_DIST = -3
+IDP_E_NOT_SPD = -4
+IDP_E_CCD_ZERO = -5
+IDP_E_CAPACITY = -6
+IDP_E_INVALID_START = -7
+IDP_E_NO_GPU = -8
+
+_STATUS = {
+    IDP_E_INVALID: "invalid argument",
+    IDP_E_HIP: "HIP runtime error",
+    IDP_E_NONPOS_DIST: "non-positive distance in a barrier term",
+    IDP_E_NOT_SPD: "linear solve breakdown (Hessian not SPD)",
+    IDP_E_CCD_ZERO: "zero time of impact",
+    IDP_E_CAPACITY: "output capacity too small",
+    IDP_E_INVALID_START: "degenerate tet at the start of the step",
+    IDP_E_NO_GPU: "no HIP device",
+}
+
+EXPORTED = [
+    "idp_ctx_create", "idp_ctx_destroy", "idp_last_error", "idp_device_count", "idp_fem_init", "idp_surface",
+    "idp_initialize_oipc", "idp_set_mesh", "idp_set_dbc", "idp_set_state", "idp_get_state", "idp_snapshot_state",
+    "idp_restore_state", "idp_surface_sizes", "idp_advance_one_step", "idp_constraint_set", "idp_get_constraints",
+    "idp_barrier_energy", "idp_barrier_gradient", "idp_barrier_hessian_local", "idp_min_dist2", "idp_ccd_stepsize",
+    "idp_elastic_energy", "idp_elastic_gradient", "idp_elastic_hessian_local", "idp_newton_direction", "idp_get_bsr",
+]
+
+
+class IdpError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"{_STATUS.get(code, 'error')} (status {code}){': ' + msg if msg else ''}")
+        self.code = code
+
+
+class SolverOpts(C.Structure):
+    _fields_ = [("pcg_rtol", C.c_double), ("pcg_maxit", C.c_int), ("max_newton", C.c_int)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [
+        ("newton_iters", C.c_int), ("line_search_trials", C.c_int), ("constraint_sets", C.c_int),
+        ("gd_iters", C.c_int), ("pcg_iters", C.c_long), ("n_constraints", C.c_long), ("ccd_candidates", C.c_long),
+        ("last_alpha", C.c_double), ("first_ccd_alpha", C.c_double), ("last_res", C.c_double),
+        ("last_relres", C.c_double), ("min_dist2", C.c_double), ("E", C.c_double), ("kappa", C.c_double),
+        ("ms_total", C.c_double), ("ms_constraint_set", C.c_double), ("ms_ccd", C.c_double),
+        ("ms_energy", C.c_double), ("ms_gradient", C.c_double), ("ms_hessian", C.c_double),
+        ("ms_solve", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"libidp_amd.so not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.idp_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+        L.idp_ctx_destroy.argtypes = [vp]
+        L.idp_last_error.argtypes = [vp]
+        L.idp_last_error.restype = C.c_char_p
+        L.idp_device_count.argtypes = [ip]
+        L.idp_fem_init.argtypes = [C.c_int, dp, C.c_int, ip, C.c_double, C.c_double, C.c_double, dp, dp, dp, dp, dp]
+        L.idp_surface.argtypes = [C.c_int, dp, C.c_int, ip, ip, ip, ip, ip, ip, ip]
+        L.idp_initialize_oipc.argtypes = [C.c_double, C.c_int, dp, C.c_double, dp, dp]
+        L.idp_set_mesh.argtypes = [vp, C.c_int, dp, C.c_int, ip, dp, dp, dp, dp, dp]
+        L.idp_set_dbc.argtypes = [vp, C.c_int, ip]
+        L.idp_set_state.argtypes = [vp, dp, dp]
+        L.idp_get_state.argtypes = [vp, dp, dp]
+        L.idp_snapshot_state.argtypes = [vp]
+        L.idp_restore_state.argtypes = [vp]
+        L.idp_surface_sizes.argtypes = [vp, ip, ip, ip]
+        L.idp_advance_one_step.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_double, dp, C.c_double,
+                                           C.POINTER(SolverOpts), C.POINTER(StepStats)]
+        L.idp_constraint_set.argtypes = [vp, dp, C.c_double, C.c_double, C.POINTER(C.c_long)]
+        L.idp_get_constraints.argtypes = [vp, ip, C.c_long]
+        L.idp_barrier_energy.argtypes = [vp, dp, C.c_long, ip, C.c_double, C.c_double, C.c_double, dp]
+        L.idp_barrier_gradient.argtypes = [vp, dp, C.c_long, ip, C.c_double, C.c_double, C.c_double, dp]
+        L.idp_barrier_hessian_local.argtypes = [vp, dp, C.c_long, ip, C.c_double, C.c_double, C.c_double, C.c_int, dp,
+                                                ip]
+        L.idp_min_dist2.argtypes = [vp, dp, C.c_long, ip, C.c_double, dp, dp]
+        L.idp_ccd_stepsize.argtypes = [vp, dp, dp, C.c_double, C.c_double, dp, C.POINTER(C.c_long)]
+        L.idp_elastic_energy.argtypes = [vp, dp, C.c_double, dp, ip]
+        L.idp_elastic_gradient.argtypes = [vp, dp, C.c_double, dp]
+        L.idp_elastic_hessian_local.argtypes = [vp, dp, C.c_double, C.c_int, dp]
+        L.idp_newton_direction.argtypes = [vp, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                           C.POINTER(SolverOpts), dp, dp, dp, ip]
+        L.idp_get_bsr.argtypes = [vp, C.POINTER(C.c_long), ip, ip, dp, C.c_long]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(dp)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(ip)
+
+
+def _out_d(n):
+    a = np.zeros(n, np.float64)
+    return a, a.ctypes.data_as(dp)
+
+
+def _out_i(n):
+    a = np.zeros(n, np.int32)
+    return a, a.ctypes.data_as(ip)
+
+
+def device_count():
+    n = C.c_int()
+    lib().idp_device_count(C.byref(n))
+    return n.value
+
+
+def fem_init(X, tets, density, E, nu):
+    """Compute_Vol_And_Inv_Basis + Lame parameters + lumped mass (host)."""
+    X, Xp = _d(X)
+    tets, tp = _i(tets)
+    nV, nT = X.size // 3, tets.size // 4
+    vol, vp = _out_d(nT)
+    IB, ibp = _out_d(9 * nT)
+    lam, lp = _out_d(nT)
+    mu, mp = _out_d(nT)
+    mass, msp = _out_d(nV)
+    rc = lib().idp_fem_init(nV, Xp, nT, tp, density, E, nu, vp, ibp, lp, mp, msp)
+    if rc != IDP_OK:
+        raise IdpError(rc)
+    return vol, IB, lam, mu, mass
+
+
+def surface(X, tets):
+    X, Xp = _d(X)
+    tets, tp = _i(tets)
+    nV, nT = X.size // 3, tets.size // 4
+    bn, bnp = _out_i(nV)
+    be, bep = _out_i(12 * nT)
+    bt, btp = _out_i(12 * nT)
+    nb, ne, nf = C.c_int(), C.c_int(), C.c_int()
+    rc = lib().idp_surface(nV, Xp, nT, tp, C.byref(nb), C.byref(ne), C.byref(nf), bnp, bep, btp)
+    if rc != IDP_OK:
+        raise IdpError(rc)
+    return bn[: nb.value].copy(), be[: 2 * ne.value].reshape(-1, 2).copy(), bt[: 3 * nf.value].reshape(-1, 3).copy()
+
+
+def initialize_oipc(dhat, mass, stiff_mult=1.0):
+    mass, mp = _d(mass)
+    k, kp = _out_d(3)
+    d2 = C.c_double()
+    rc = lib().idp_initialize_oipc(dhat, mass.size, mp, stiff_mult, kp, C.byref(d2))
+    if rc != IDP_OK:
+        raise IdpError(rc)
+    return d2.value, k
+
+
+class Context:
+    """One device context: mesh, state and work buffers on one HIP stream."""
+
+    def __init__(self, device=0):
+        self._h = C.c_void_p()
+        rc = lib().idp_ctx_create(device, C.byref(self._h))
+        if rc != IDP_OK:
+            raise IdpError(rc)
+        self.nV = 0
+
+    def close(self):
+        if self._h:
+            lib().idp_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc < 0:
+            raise IdpError(rc, lib().idp_last_error(self._h).decode())
+        return rc
+
+    def set_mesh(self, X_rest, tets, IB, vol, lam, mu, mass):
+        X, Xp = _d(X_rest)
+        t, tp = _i(tets)
+        ib, ibp = _d(IB)
+        v, vp = _d(vol)
+        l, lp = _d(lam)
+        m, mp = _d(mu)
+        ms, msp = _d(mass)
+        self.nV = X.size // 3
+        self._check(lib().idp_set_mesh(self._h, self.nV, Xp, t.size // 4, tp, ibp, vp, lp, mp, msp))
+
+    def set_dbc(self, idx):
+        a, ap = _i(idx)
+        self._check(lib().idp_set_dbc(self._h, a.size, ap))
+
+    def surface_sizes(self):
+        nb, ne, nf = C.c_int(), C.c_int(), C.c_int()
+        self._check(lib().idp_surface_sizes(self._h, C.byref(nb), C.byref(ne), C.byref(nf)))
+        return nb.value, ne.value, nf.value
+
+    def set_state(self, X=None, v=None):
+        Xa, Xp = _d(X) if X is not None else (None, None)
+        va, vp = _d(v) if v is not None else (None, None)
+        self._check(lib().idp_set_state(self._h, Xp, vp))
+
+    def get_state(self):
+        X, Xp = _out_d(3 * self.nV)
+        v, vp = _out_d(3 * self.nV)
+        self._check(lib().idp_get_state(self._h, Xp, vp))
+        return X.reshape(-1, 3), v.reshape(-1, 3)
+
+    def snapshot(self):
+        self._check(lib().idp_snapshot_state(self._h))
+
+    def restore(self):
+        self._check(lib().idp_restore_state(self._h))
+
+    def advance(self, h, tol, dHat2, kappa, thickness=0.0, with_collision=True, pcg_rtol=1e-8, pcg_maxit=10000,
+                max_newton=0):
+        """Advance_One_Step_IE_Discrete_Shell; kappa (3,) is updated in place."""
+        kappa_arr = np.asarray(kappa, dtype=np.float64)
+        if not kappa_arr.flags["C_CONTIGUOUS"]:
+            raise ValueError("kappa must be contiguous float64")
+        opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton)
+        st = StepStats()
+        it = self._check(lib().idp_advance_one_step(self._h, h, tol, int(with_collision), dHat2,
+                                                    kappa_arr.ctypes.data_as(dp), thickness, C.byref(opts),
+                                                    C.byref(st)))
+        return it, st.as_dict()
+
+    # ---- stage entry points -------------------------------------------------
+    def constraint_set(self, X, dHat2, thickness=0.0):
+        X, Xp = _d(X)
+        n = C.c_long()
+        self._check(lib().idp_constraint_set(self._h, Xp, dHat2, thickness, C.byref(n)))
+        cs, csp = _out_i(4 * max(n.value, 1))
+        self._check(lib().idp_get_constraints(self._h, csp, n.value))
+        return cs[: 4 * n.value].reshape(-1, 4)
+
+    def barrier_energy(self, X, cs, dHat2, kappa, thickness=0.0):
+        X, Xp = _d(X)
+        cs, csp = _i(cs)
+        E = C.c_double()
+        self._check(lib().idp_barrier_energy(self._h, Xp, cs.size // 4, csp, dHat2, kappa, thickness, C.byref(E)))
+        return E.value
+
+    def barrier_gradient(self, X, cs, dHat2, kappa, thickness=0.0):
+        X, Xp = _d(X)
+        cs, csp = _i(cs)
+        g, gp = _out_d(X.size)
+        self._check(lib().idp_barrier_gradient(self._h, Xp, cs.size // 4, csp, dHat2, kappa, thickness, gp))
+        return g
+
+    def barrier_hessian_local(self, X, cs, dHat2, kappa, thickness=0.0, project=True):
+        X, Xp = _d(X)
+        cs, csp = _i(cs)
+        n = cs.size // 4
+        H, Hp = _out_d(144 * max(n, 1))
+        vid, vp = _out_i(4 * max(n, 1))
+        self._check(lib().idp_barrier_hessian_local(self._h, Xp, n, csp, dHat2, kappa, thickness, int(project), Hp,
+                                                    vp))
+        return H[: 144 * n].reshape(n, 144), vid[: 4 * n].reshape(n, 4)
+
+    def min_dist2(self, X, cs, thickness=0.0):
+        X, Xp = _d(X)
+        cs, csp = _i(cs)
+        n = cs.size // 4
+        d, dpp = _out_d(max(n, 1))
+        m = C.c_double()
+        self._check(lib().idp_min_dist2(self._h, Xp, n, csp, thickness, dpp, C.byref(m)))
+        return d[:n], m.value
+
+    def ccd(self, X, p, alpha, thickness=0.0):
+        X, Xp = _d(X)
+        p, pp = _d(p)
+        out = C.c_double()
+        nc = C.c_long()
+        self._check(lib().idp_ccd_stepsize(self._h, Xp, pp, thickness, alpha, C.byref(out), C.byref(nc)))
+        return out.value, nc.value
+
+    def elastic_energy(self, X, w):
+        X, Xp = _d(X)
+        E = C.c_double()
+        ok = C.c_int()
+        self._check(lib().idp_elastic_energy(self._h, Xp, w, C.byref(E), C.byref(ok)))
+        return E.value if ok.value else None
+
+    def elastic_gradient(self, X, w):
+        X, Xp = _d(X)
+        g, gp = _out_d(X.size)
+        self._check(lib().idp_elastic_gradient(self._h, Xp, w, gp))
+        return g
+
+    def elastic_hessian_local(self, X, w, nT, project=True):
+        X, Xp = _d(X)
+        H, Hp = _out_d(144 * nT)
+        self._check(lib().idp_elastic_hessian_local(self._h, Xp, w, int(project), Hp))
+        return H.reshape(nT, 144)
+
+    def newton_direction(self, X, xtilde, h, dHat2, kappa, thickness=0.0, pcg_rtol=1e-10, pcg_maxit=20000):
+        X, Xp = _d(X)
+        xt, xtp = _d(xtilde)
+        g, gp = _out_d(X.size)
+        sol, sp = _out_d(X.size)
+        rel = C.c_double()
+        it = C.c_int()
+        opts = SolverOpts(pcg_rtol, pcg_maxit, 0)
+        self._check(lib().idp_newton_direction(self._h, Xp, xtp, h, dHat2, kappa, thickness, C.byref(opts), gp, sp,
+                                               C.byref(rel), C.byref(it)))
+        return g, sol, rel.value, it.value
+
+    def get_bsr(self):
+        nnzb = C.c_long()
+        rc = lib().idp_get_bsr(self._h, C.byref(nnzb), None, None, None, 0)
+        if rc not in (IDP_OK, IDP_E_CAPACITY):
+            self._check(rc)
+        rp, rpp = _out_i(self.nV + 1)
+        col, cp = _out_i(max(nnzb.value, 1))
+        val, vp = _out_d(9 * max(nnzb.value, 1))
+        self._check(lib().idp_get_bsr(self._h, C.byref(nnzb), rpp, cp, vp, nnzb.value))
+        return rp, col[: nnzb.value], val[: 9 * nnzb.value].reshape(-1, 3, 3)

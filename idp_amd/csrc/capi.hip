@@ -1,0 +1,1019 @@
+// C ABI and host orchestration of the Newton-with-barrier time step.
+//   Advance_One_Step_IE_Discrete_Shell   Library/FEM/Shell/IMPLICIT_EULER.h:152-891
+//   Line_Search                          Library/FEM/Shell/IMPLICIT_EULER.h:10-149
+// State stays on the device for the whole step; the host drives control flow
+// (line-search halving, kappa update, stopping rule) from scalars read back
+// once per stage.
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "linalg.hpp"
+
+using namespace idp;
+
+struct idp_ctx {
+    int device = 0;
+    hipStream_t s = nullptr;
+    std::string err;
+    MeshDev m;
+    ContactWork cw;
+    LinWork L;
+    bool have_mesh = false;
+    int ndbc = 0;
+    // state and work vectors (3 nV)
+    DBuf<double> x, v, xt, xn, xprev, sol, rhs, g, tmpv, snap_x, snap_v, hx, hp;
+    DBuf<double> gl, gb, Hl, Hb;  // per-tet / per-constraint local terms
+    DBuf<double> part_e, part_i, part_b;
+    DBuf<int> flags;
+    DBuf<unsigned long long> dmin;
+    DBuf<double> dist2, dist2_prev;
+    DBuf<int4> cs_prev;
+    long ncs_prev = 0;
+    double snap_kappa = 0;
+    bool snap_valid = false;
+};
+
+namespace {
+
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <class F>
+int guarded(idp_ctx* c, F&& f)
+{
+    if (!c) return IDP_E_INVALID;
+    try {
+        return f();
+    } catch (const HipError& e) {
+        c->err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        c->err = e.what();
+        return IDP_E_INVALID;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host set-up helpers
+// ---------------------------------------------------------------------------
+inline double det3h(const double* m)
+{
+    return m[0] * (m[4] * m[8] - m[7] * m[5]) - m[3] * (m[1] * m[8] - m[7] * m[2]) +
+           m[6] * (m[1] * m[5] - m[4] * m[2]);
+}
+
+// surface primitives (MESHIO.h:486-545, 769-835): triangles are the tet faces
+// (0,2,1),(0,3,2),(0,1,3),(1,2,3) without an oppositely oriented twin, in
+// lexicographic order of the stored vertex triple; edges take the orientation
+// of their first boundary triangle and are listed lexicographically; nodes
+// are the vertices with nonzero accumulated boundary area, ascending.
+void surface_host(int nV, const double* X, int nT, const int* tets, std::vector<int>& bn,
+                  std::vector<std::array<int, 2>>& be, std::vector<std::array<int, 3>>& bt)
+{
+    struct Face {
+        int s[3];  // sorted vertex set
+        int o[3];  // stored orientation
+        int par;   // parity of o relative to s (0 even)
+    };
+    std::vector<Face> f;
+    f.reserve(4 * (size_t)nT);
+    const int loc[4][3] = {{0, 2, 1}, {0, 3, 2}, {0, 1, 3}, {1, 2, 3}};
+    for (int t = 0; t < nT; ++t)
+        for (auto& l : loc) {
+            Face F;
+            for (int k = 0; k < 3; ++k) F.o[k] = tets[4 * t + l[k]];
+            int a = F.o[0], b = F.o[1], c = F.o[2];
+            int s0 = std::min(a, std::min(b, c)), s2 = std::max(a, std::max(b, c)), s1 = a + b + c - s0 - s2;
+            F.s[0] = s0;
+            F.s[1] = s1;
+            F.s[2] = s2;
+            // even iff (a,b,c) is a rotation of (s0,s1,s2)
+            bool even = (a == s0 && b == s1) || (b == s0 && c == s1) || (c == s0 && a == s1);
+            F.par = even ? 0 : 1;
+            f.push_back(F);
+        }
+    std::sort(f.begin(), f.end(), [](const Face& x, const Face& y) {
+        if (x.s[0] != y.s[0]) return x.s[0] < y.s[0];
+        if (x.s[1] != y.s[1]) return x.s[1] < y.s[1];
+        return x.s[2] < y.s[2];
+    });
+    bt.clear();
+    for (size_t i = 0; i < f.size();) {
+        size_t j = i;
+        bool has[2] = {false, false};
+        while (j < f.size() && f[j].s[0] == f[i].s[0] && f[j].s[1] == f[i].s[1] && f[j].s[2] == f[i].s[2]) {
+            has[f[j].par] = true;
+            ++j;
+        }
+        for (size_t k = i; k < j; ++k)
+            if (!has[f[k].par ^ 1]) bt.push_back({f[k].o[0], f[k].o[1], f[k].o[2]});
+        i = j;
+    }
+    std::sort(bt.begin(), bt.end());
+    bt.erase(std::unique(bt.begin(), bt.end()), bt.end());
+    // edges: first-seen orientation
+    std::unordered_map<long long, std::array<int, 2>> em;
+    em.reserve(bt.size() * 2);
+    std::vector<double> area(nV, 0.0);
+    for (auto& t : bt) {
+        const int pr[3][2] = {{t[0], t[1]}, {t[1], t[2]}, {t[2], t[0]}};
+        for (auto& p : pr) {
+            long long key = (long long)std::min(p[0], p[1]) * nV + std::max(p[0], p[1]);
+            if (!em.count(key)) em[key] = {p[0], p[1]};
+        }
+        double e1[3], e2[3];
+        for (int d = 0; d < 3; ++d) {
+            e1[d] = X[3 * t[1] + d] - X[3 * t[0] + d];
+            e2[d] = X[3 * t[2] + d] - X[3 * t[0] + d];
+        }
+        double c0 = e1[1] * e2[2] - e1[2] * e2[1], c1 = e1[2] * e2[0] - e1[0] * e2[2], c2 = e1[0] * e2[1] - e1[1] * e2[0];
+        double A = 0.5 * std::sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+        for (int k = 0; k < 3; ++k) area[t[k]] += A / 3;
+    }
+    be.clear();
+    be.reserve(em.size());
+    for (auto& kv : em) be.push_back(kv.second);
+    std::sort(be.begin(), be.end());
+    bn.clear();
+    for (int v = 0; v < nV; ++v)
+        if (area[v] != 0.0) bn.push_back(v);
+}
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+struct EnergyOut {
+    bool valid;
+    bool dist_ok;
+    double E;
+};
+
+// Compute_IncPotential: elastic (w = h^2) + inertia + barrier over the
+// context's current constraint set
+EnergyOut energy(idp_ctx* c, const double* X, bool with_barrier, double h, double dHat2, double kappa,
+                 double thickness)
+{
+    auto s = c->s;
+    auto& m = c->m;
+    int* fl = c->flags.ensure(4, s);
+    IDP_HIP(hipMemsetAsync(fl, 0, 2 * sizeof(int), s));
+    int nbt = tet_energy_blocks(m.nT), nbi = inertia_energy_blocks(m.nV);
+    double* pe = c->part_e.ensure(nbt + 1, s);
+    double* pi = c->part_i.ensure(nbi + 1, s);
+    launch_tet_energy(s, m, X, h * h, pe, fl);
+    launch_inertia_energy(s, m.nV, X, c->xt.p, m.mass.p, pi);
+    long ncs = c->cw.ncs;
+    double Eb = 0.0;
+    bool bar = with_barrier && ncs > 0;
+    int nbb = bar ? barrier_energy_blocks(ncs) : 0;
+    double* pb = c->part_b.ensure(nbb + 1, s);
+    if (bar) {
+        double th2 = thickness * thickness;
+        double dh = dHat2 + 2 * std::sqrt(dHat2) * thickness;
+        launch_barrier_energy(s, ncs, (const int*)c->cw.cs.p, X, m.X0.p, dh, kappa, th2, pb, fl + 1);
+    }
+    int hf[2];
+    IDP_HIP(hipMemcpyAsync(hf, fl, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    IDP_HIP(hipStreamSynchronize(s));
+    EnergyOut o;
+    o.valid = hf[0] == 0;
+    o.dist_ok = hf[1] == 0;
+    o.E = 0;
+    if (!o.valid) return o;
+    double Ee = dev_sum_partials(c->L, s, nbt, pe);
+    double Ei = dev_sum_partials(c->L, s, nbi, pi);
+    if (bar) Eb = dev_sum_partials(c->L, s, nbb, pb);
+    o.E = Ee + Ei + Eb;
+    return o;
+}
+
+double min_dist2_dev(idp_ctx* c, const double* X, long n, const int* cs, double* dist2)
+{
+    auto s = c->s;
+    unsigned long long* dm = c->dmin.ensure(1, s);
+    unsigned long long init = 0x7ff0000000000000ull;  // +inf
+    IDP_HIP(hipMemcpyAsync(dm, &init, sizeof(init), hipMemcpyHostToDevice, s));
+    launch_min_dist2(s, n, cs, X, dist2, dm);
+    unsigned long long r;
+    IDP_HIP(hipMemcpyAsync(&r, dm, sizeof(r), hipMemcpyDeviceToHost, s));
+    IDP_HIP(hipStreamSynchronize(s));
+    double d;
+    std::memcpy(&d, &r, sizeof(d));
+    return d;
+}
+
+// gradient at X with the current constraint set (structure must be built)
+void gradient(idp_ctx* c, const double* X, double h, double dHat2, double kappa, double thickness, bool with_barrier,
+              double* g, double* rhs)
+{
+    auto s = c->s;
+    auto& m = c->m;
+    double* gl = c->gl.ensure(12 * (size_t)m.nT, s);
+    launch_tet_grad(s, m, X, h * h, gl);
+    const double* gb = nullptr;
+    long ncs = c->cw.ncs;
+    if (with_barrier && ncs) {
+        int* fl = c->flags.ensure(4, s);
+        IDP_HIP(hipMemsetAsync(fl + 2, 0, sizeof(int), s));
+        double th2 = thickness * thickness, dh = dHat2 + 2 * std::sqrt(dHat2) * thickness;
+        double* b = c->gb.ensure(12 * (size_t)ncs, s);
+        launch_barrier_grad(s, ncs, (const int*)c->cw.cs.p, X, m.X0.p, dh, kappa, th2, b, fl + 2);
+        gb = b;
+    }
+    contact_gradient_gather(c->L, s, m, X, c->xt.p, gl, gb, g, rhs);
+}
+
+void hessian(idp_ctx* c, const double* X, double h, double dHat2, double kappa, double thickness, bool with_barrier)
+{
+    auto s = c->s;
+    auto& m = c->m;
+    double* Hl = c->Hl.ensure(144 * (size_t)m.nT, s);
+    launch_tet_hess(s, m, X, h * h, 1, Hl);
+    const double* Hb = nullptr;
+    long ncs = c->cw.ncs;
+    if (with_barrier && ncs) {
+        int* fl = c->flags.ensure(4, s);
+        double th2 = thickness * thickness, dh = dHat2 + 2 * std::sqrt(dHat2) * thickness;
+        double* b = c->Hb.ensure(144 * (size_t)ncs, s);
+        launch_barrier_hess(s, ncs, (const int*)c->cw.cs.p, X, m.X0.p, dh, kappa, th2, 1, 1, b, nullptr, fl + 3);
+        Hb = b;
+    }
+    assemble(c->L, s, m, Hl, Hb, c->ndbc > 0);
+}
+
+void set_constraints_from_host(idp_ctx* c, long n, const int* cs)
+{
+    c->cw.cs.upload((const int4*)cs, n, c->s);
+    c->cw.ncs = n;
+}
+
+const double* stage_x(idp_ctx* c, const double* X)
+{
+    if (X) {
+        c->hx.upload(X, 3 * (size_t)c->m.nV, c->s);
+        return c->hx.p;
+    }
+    return c->x.p;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int idp_device_count(int* n)
+{
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    if (n) *n = c;
+    return IDP_OK;
+}
+
+int idp_ctx_create(int device, idp_ctx** out)
+{
+    if (!out) return IDP_E_INVALID;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return IDP_E_NO_GPU;
+    if (device < 0 || device >= n) return IDP_E_INVALID;
+    idp_ctx* c = new idp_ctx();
+    c->device = device;
+    try {
+        IDP_HIP(hipSetDevice(device));
+        IDP_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+        barrier_init_constants();
+    } catch (const HipError& e) {
+        delete c;
+        return e.code;
+    }
+    *out = c;
+    return IDP_OK;
+}
+
+int idp_ctx_destroy(idp_ctx* c)
+{
+    if (!c) return IDP_E_INVALID;
+    (void)hipSetDevice(c->device);
+    if (c->s) {
+        (void)hipStreamSynchronize(c->s);
+        (void)hipStreamDestroy(c->s);
+    }
+    delete c;
+    return IDP_OK;
+}
+
+const char* idp_last_error(const idp_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int idp_fem_init(int nV, const double* X, int nT, const int* tets, double density, double E, double nu,
+                 double* vol, double* IB, double* lambda, double* mu, double* mass)
+{
+    if (nV <= 0 || nT <= 0 || !X || !tets) return IDP_E_INVALID;
+    const double L = E * nu / ((1 + nu) * (1 - 2 * nu));
+    const double M = E / (2 * (1 + nu));
+    for (int v = 0; v < nV; ++v) mass[v] = 0.0;
+    for (int t = 0; t < nT; ++t) {
+        const int* e = tets + 4 * t;
+        double D[9];
+        for (int k = 0; k < 3; ++k)
+            for (int i = 0; i < 3; ++i) D[3 * i + k] = X[3 * e[k + 1] + i] - X[3 * e[0] + i];
+        double det = det3h(D);
+        if (det == 0.0) return IDP_E_INVALID;  // MATRIX::invert exits on a singular basis
+        vol[t] = det / 6.0;
+        double c[9];
+        c[0] = D[4] * D[8] - D[5] * D[7];
+        c[1] = D[5] * D[6] - D[3] * D[8];
+        c[2] = D[3] * D[7] - D[4] * D[6];
+        c[3] = D[2] * D[7] - D[1] * D[8];
+        c[4] = D[0] * D[8] - D[2] * D[6];
+        c[5] = D[1] * D[6] - D[0] * D[7];
+        c[6] = D[1] * D[5] - D[2] * D[4];
+        c[7] = D[2] * D[3] - D[0] * D[5];
+        c[8] = D[0] * D[4] - D[1] * D[3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) IB[9 * t + 3 * i + j] = c[3 * j + i] / det;
+        lambda[t] = L;
+        mu[t] = M;
+        double me = vol[t] * density / 4;
+        for (int k = 0; k < 4; ++k) mass[e[k]] += me;
+    }
+    return IDP_OK;
+}
+
+int idp_surface(int nV, const double* X, int nT, const int* tets, int* nb, int* ne, int* nf, int* bnode, int* bedge,
+                int* btri)
+{
+    if (nV <= 0 || nT <= 0) return IDP_E_INVALID;
+    std::vector<int> bn;
+    std::vector<std::array<int, 2>> be;
+    std::vector<std::array<int, 3>> bt;
+    surface_host(nV, X, nT, tets, bn, be, bt);
+    *nb = (int)bn.size();
+    *ne = (int)be.size();
+    *nf = (int)bt.size();
+    if (bnode) std::memcpy(bnode, bn.data(), bn.size() * sizeof(int));
+    if (bedge)
+        for (size_t i = 0; i < be.size(); ++i) {
+            bedge[2 * i] = be[i][0];
+            bedge[2 * i + 1] = be[i][1];
+        }
+    if (btri)
+        for (size_t i = 0; i < bt.size(); ++i)
+            for (int k = 0; k < 3; ++k) btri[3 * i + k] = bt[i][k];
+    return IDP_OK;
+}
+
+int idp_initialize_oipc(double dhat, int nV, const double* mass, double stiff_mult, double* kappa3, double* dHat2)
+{
+    if (nV <= 0 || !mass || !kappa3) return IDP_E_INVALID;
+    const double d2 = dhat * dhat;
+    const double dd = 1.0e-16, t = dd - d2;
+    const double Hb = (-2.0 * std::log(dd / d2) - 4.0 * t / dd + t * t / (dd * dd));
+    double msum = 0;
+    for (int v = 0; v < nV; ++v) msum += mass[v];
+    const double k0 = stiff_mult * 1.0e11 * (msum / nV) * 3 / (4.0e-16 * Hb);
+    kappa3[0] = k0;
+    kappa3[1] = 100 * k0;
+    kappa3[2] = 0.0;
+    if (dHat2) *dHat2 = d2;
+    return IDP_OK;
+}
+
+int idp_set_mesh(idp_ctx* c, int nV, const double* X_rest, int nT, const int* tets, const double* IB,
+                 const double* vol, const double* lambda, const double* mu, const double* mass)
+{
+    return guarded(c, [&]() -> int {
+        if (nV <= 0 || nT <= 0 || !X_rest || !tets || !IB || !vol || !lambda || !mu || !mass) return IDP_E_INVALID;
+        for (long i = 0; i < 4L * nT; ++i)
+            if (tets[i] < 0 || tets[i] >= nV) return IDP_E_INVALID;
+        IDP_HIP(hipSetDevice(c->device));
+        auto s = c->s;
+        MeshDev& m = c->m;
+        m.nV = nV;
+        m.nT = nT;
+        m.X0.upload(X_rest, 3 * (size_t)nV, s);
+        m.mass.upload(mass, nV, s);
+        m.tets.upload(tets, 4 * (size_t)nT, s);
+        m.IB.upload(IB, 9 * (size_t)nT, s);
+        m.vol.upload(vol, nT, s);
+        m.lam.upload(lambda, nT, s);
+        m.mu.upload(mu, nT, s);
+        std::vector<unsigned char> dbc(nV, 0);
+        m.dbc.upload(dbc.data(), nV, s);
+        c->ndbc = 0;
+        // surface
+        std::vector<int> bn;
+        std::vector<std::array<int, 2>> be;
+        std::vector<std::array<int, 3>> bt;
+        surface_host(nV, X_rest, nT, tets, bn, be, bt);
+        m.nb = (int)bn.size();
+        m.ne = (int)be.size();
+        m.nf = (int)bt.size();
+        m.bn.upload(bn.data(), bn.size(), s);
+        m.be.upload((const int*)be.data(), 2 * be.size(), s);
+        m.bt.upload((const int*)bt.data(), 3 * bt.size(), s);
+        std::vector<int> v2b(nV, -1);
+        for (int i = 0; i < m.nb; ++i) v2b[bn[i]] = i;
+        m.v2b.upload(v2b.data(), nV, s);
+        // static elastic pattern: bucket records by row, sort each row by (col, src)
+        std::vector<int> rcount(nV + 1, 0);
+        for (int t = 0; t < nT; ++t)
+            for (int a = 0; a < 4; ++a) rcount[tets[4 * t + a] + 1] += 4;
+        for (int v = 0; v < nV; ++v) rcount[v + 1] += rcount[v];
+        std::vector<std::pair<int, int>> rec(16 * (size_t)nT);  // (col, src)
+        std::vector<int> fill(rcount.begin(), rcount.end() - 1);
+        for (int t = 0; t < nT; ++t)
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b)
+                    rec[fill[tets[4 * t + a]]++] = {tets[4 * t + b], t * 16 + a * 4 + b};
+        std::vector<int> rowptr(nV + 1, 0), col, sptr, src, diag(nV, -1), rowof;
+        col.reserve(27 * (size_t)nV);
+        src.reserve(16 * (size_t)nT);
+        sptr.push_back(0);
+        for (int v = 0; v < nV; ++v) {
+            std::sort(rec.begin() + rcount[v], rec.begin() + rcount[v + 1]);
+            for (int k = rcount[v]; k < rcount[v + 1];) {
+                int j = k;
+                while (j < rcount[v + 1] && rec[j].first == rec[k].first) src.push_back(rec[j++].second);
+                if (rec[k].first == v) diag[v] = (int)col.size();
+                col.push_back(rec[k].first);
+                rowof.push_back(v);
+                sptr.push_back((int)src.size());
+                k = j;
+            }
+            rowptr[v + 1] = (int)col.size();
+            if (diag[v] < 0) return IDP_E_INVALID;  // vertex not in any tet
+        }
+        m.nnzb_e = (long)col.size();
+        m.e_rowptr.upload(rowptr.data(), rowptr.size(), s);
+        m.e_col.upload(col.data(), col.size(), s);
+        m.e_src_ptr.upload(sptr.data(), sptr.size(), s);
+        m.e_src.upload(src.data(), src.size(), s);
+        m.e_diag.upload(diag.data(), diag.size(), s);
+        init_erow(c->L, s, m, rowof);
+        // state
+        c->x.upload(X_rest, 3 * (size_t)nV, s);
+        std::vector<double> zero(3 * (size_t)nV, 0.0);
+        c->v.upload(zero.data(), zero.size(), s);
+        c->cw.ncs = 0;
+        c->ncs_prev = 0;
+        IDP_HIP(hipStreamSynchronize(s));
+        c->have_mesh = true;
+        c->snap_valid = false;
+        return IDP_OK;
+    });
+}
+
+int idp_set_dbc(idp_ctx* c, int n, const int* idx)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh || n < 0) return IDP_E_INVALID;
+        std::vector<unsigned char> d(c->m.nV, 0);
+        for (int i = 0; i < n; ++i) {
+            if (idx[i] < 0 || idx[i] >= c->m.nV) return IDP_E_INVALID;
+            d[idx[i]] = 1;
+        }
+        c->ndbc = 0;
+        for (auto b : d) c->ndbc += b;
+        c->m.dbc.upload(d.data(), d.size(), c->s);
+        IDP_HIP(hipStreamSynchronize(c->s));
+        return IDP_OK;
+    });
+}
+
+int idp_surface_sizes(idp_ctx* c, int* nb, int* ne, int* nf)
+{
+    if (!c || !c->have_mesh) return IDP_E_INVALID;
+    *nb = c->m.nb;
+    *ne = c->m.ne;
+    *nf = c->m.nf;
+    return IDP_OK;
+}
+
+int idp_set_state(idp_ctx* c, const double* X, const double* v)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh) return IDP_E_INVALID;
+        size_t n = 3 * (size_t)c->m.nV;
+        if (X) c->x.upload(X, n, c->s);
+        if (v) c->v.upload(v, n, c->s);
+        IDP_HIP(hipStreamSynchronize(c->s));
+        return IDP_OK;
+    });
+}
+
+int idp_get_state(idp_ctx* c, double* X, double* v)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh) return IDP_E_INVALID;
+        size_t n = 3 * (size_t)c->m.nV;
+        if (X) c->x.download(X, n, c->s);
+        if (v) c->v.download(v, n, c->s);
+        IDP_HIP(hipStreamSynchronize(c->s));
+        return IDP_OK;
+    });
+}
+
+int idp_snapshot_state(idp_ctx* c)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh) return IDP_E_INVALID;
+        size_t n = 3 * (size_t)c->m.nV;
+        c->snap_x.ensure(n, c->s);
+        c->snap_v.ensure(n, c->s);
+        IDP_HIP(hipMemcpyAsync(c->snap_x.p, c->x.p, n * 8, hipMemcpyDeviceToDevice, c->s));
+        IDP_HIP(hipMemcpyAsync(c->snap_v.p, c->v.p, n * 8, hipMemcpyDeviceToDevice, c->s));
+        IDP_HIP(hipStreamSynchronize(c->s));
+        c->snap_valid = true;
+        return IDP_OK;
+    });
+}
+
+int idp_restore_state(idp_ctx* c)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh || !c->snap_valid) return IDP_E_INVALID;
+        size_t n = 3 * (size_t)c->m.nV;
+        IDP_HIP(hipMemcpyAsync(c->x.p, c->snap_x.p, n * 8, hipMemcpyDeviceToDevice, c->s));
+        IDP_HIP(hipMemcpyAsync(c->v.p, c->snap_v.p, n * 8, hipMemcpyDeviceToDevice, c->s));
+        IDP_HIP(hipStreamSynchronize(c->s));
+        return IDP_OK;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// the time step
+// ---------------------------------------------------------------------------
+int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_collision, double dHat2, double* kappa3,
+                         double thickness, const idp_solver_opts* opts, idp_step_stats* st)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh || !kappa3 || !(h > 0)) return IDP_E_INVALID;
+        IDP_HIP(hipSetDevice(c->device));
+        const double t_start = now_ms();
+        idp_step_stats S;
+        std::memset(&S, 0, sizeof(S));
+        const double rtol = (opts && opts->pcg_rtol > 0) ? opts->pcg_rtol : 1e-8;
+        const int maxit = (opts && opts->pcg_maxit > 0) ? opts->pcg_maxit : 10000;
+        const int max_newton = opts ? opts->max_newton : 0;
+        auto s = c->s;
+        MeshDev& m = c->m;
+        const long n3 = 3L * m.nV;
+        const bool coll = with_collision != 0;
+        double* x = c->x.p;
+        double* xn = c->xn.ensure(n3, s);
+        double* xt = c->xt.ensure(n3, s);
+        double* xprev = c->xprev.ensure(n3, s);
+        double* sol = c->sol.ensure(n3, s);
+        double* rhs = c->rhs.ensure(n3, s);
+        double* g = c->g.ensure(n3, s);
+        // Xn = X; Xtilde = X + h v (zero body force, IMPLICIT_EULER.h:196-217)
+        IDP_HIP(hipMemcpyAsync(xn, x, n3 * 8, hipMemcpyDeviceToDevice, s));
+        dev_axpy(s, n3, x, h, c->v.p, xt);
+        double kappa = kappa3[0];
+        auto t0 = now_ms();
+        auto cset = [&](const double* X) {
+            double ta = now_ms();
+            contact_constraint_set(c->cw, s, m, X, dHat2, thickness);
+            S.constraint_sets++;
+            S.ms_constraint_set += now_ms() - ta;
+        };
+        if (coll) cset(x);
+        t0 = now_ms();
+        EnergyOut e0 = energy(c, x, coll, h, dHat2, kappa, thickness);
+        S.ms_energy += now_ms() - t0;
+        if (!e0.valid) return IDP_E_INVALID_START;
+        if (!e0.dist_ok) return IDP_E_NONPOS_DIST;
+        double Eprev = e0.E;
+        std::deque<double> res;
+        bool useGD = false;
+        int PNIter = 0;
+        double L2 = 0;
+        DBuf<int4>& csp = c->cs_prev;
+        c->ncs_prev = 0;
+        do {
+            // ---- gradient and Hessian at x with the current constraint set
+            t0 = now_ms();
+            if (coll) contact_structure(c->L, s, m.nV, c->cw.ncs, (const int*)c->cw.cs.p);
+            else c->L.ncb = 0;
+            gradient(c, x, h, dHat2, kappa, thickness, coll, g, rhs);
+            IDP_HIP(hipStreamSynchronize(s));
+            S.ms_gradient += now_ms() - t0;
+            if (!useGD) {
+                t0 = now_ms();
+                hessian(c, x, h, dHat2, kappa, thickness, coll);
+                IDP_HIP(hipStreamSynchronize(s));
+                S.ms_hessian += now_ms() - t0;
+                t0 = now_ms();
+                double rel = 0;
+                int it = pcg_solve(c->L, s, m, rhs, sol, rtol, maxit, &rel, 8);
+                S.ms_solve += now_ms() - t0;
+                if (it < 0) {
+                    useGD = true;  // DIRECT_SOLVER.h:42-44 -> gradient descent (IMPLICIT_EULER.h:531-551)
+                    IDP_HIP(hipMemcpyAsync(sol, rhs, n3 * 8, hipMemcpyDeviceToDevice, s));
+                } else {
+                    S.pcg_iters += it;
+                    S.last_relres = rel;
+                }
+            } else {
+                IDP_HIP(hipMemcpyAsync(sol, rhs, n3 * 8, hipMemcpyDeviceToDevice, s));
+            }
+            if (useGD) S.gd_iters++;
+
+            // ---- Line_Search
+            IDP_HIP(hipMemcpyAsync(xprev, x, n3 * 8, hipMemcpyDeviceToDevice, s));
+            double alpha = 1.0, E = 0.0;
+            bool valid;
+            int guard = 0;
+            do {
+                if (++guard > 4000) return IDP_E_INVALID;
+                dev_axpy(s, n3, xprev, alpha, sol, x);
+                t0 = now_ms();
+                EnergyOut eo = energy(c, x, false, h, dHat2, kappa, thickness);
+                S.ms_energy += now_ms() - t0;
+                valid = eo.valid;
+                if (!valid) alpha /= 2.0;
+            } while (!valid);
+            IDP_HIP(hipMemcpyAsync(x, xprev, n3 * 8, hipMemcpyDeviceToDevice, s));
+            if (coll) {
+                t0 = now_ms();
+                long nc = 0;
+                alpha = contact_ccd(c->cw, s, m, x, sol, thickness, alpha, &nc);
+                S.ms_ccd += now_ms() - t0;
+                S.ccd_candidates += nc;
+            }
+            const double feasibleAlpha = alpha;
+            if (PNIter == 0) S.first_ccd_alpha = alpha;
+            guard = 0;
+            do {
+                if (++guard > 4000) return IDP_E_INVALID;  // alpha underflow: x no longer moves
+                dev_axpy(s, n3, xprev, alpha, sol, x);
+                t0 = now_ms();
+                EnergyOut eo = energy(c, x, false, h, dHat2, kappa, thickness);
+                S.ms_energy += now_ms() - t0;
+                valid = eo.valid;
+                E = eo.E;
+                if (valid) {
+                    if (coll) {
+                        cset(x);
+                        if (c->cw.ncs) {
+                            double md = min_dist2_dev(c, x, c->cw.ncs, (const int*)c->cw.cs.p, nullptr) -
+                                        thickness * thickness;
+                            if (md <= 0) {
+                                alpha /= 2;
+                                E = Eprev + 1;
+                                S.line_search_trials++;
+                                continue;
+                            }
+                        }
+                    }
+                    t0 = now_ms();
+                    eo = energy(c, x, coll, h, dHat2, kappa, thickness);
+                    S.ms_energy += now_ms() - t0;
+                    valid = eo.valid;
+                    E = eo.E;
+                    if (!eo.dist_ok) return IDP_E_NONPOS_DIST;
+                }
+                alpha /= 2.0;
+                S.line_search_trials++;
+            } while (E > Eprev || !valid);
+            Eprev = E;
+            S.last_alpha = alpha * 2;
+
+            // ---- kappa update (IMPLICIT_EULER.h:566-595)
+            if (coll && c->ncs_prev) {
+                long np = c->ncs_prev;
+                double* cur = c->tmpv.ensure(np, s);
+                min_dist2_dev(c, x, np, (const int*)csp.p, cur);
+                std::vector<double> hc(np), hp(np);
+                IDP_HIP(hipMemcpyAsync(hc.data(), cur, np * 8, hipMemcpyDeviceToHost, s));
+                IDP_HIP(hipMemcpyAsync(hp.data(), c->dist2_prev.p, np * 8, hipMemcpyDeviceToHost, s));
+                IDP_HIP(hipStreamSynchronize(s));
+                bool upd = false;
+                for (long i = 0; i < np; ++i)
+                    if (hp[i] < 1e-18 && hc[i] < hp[i]) {
+                        upd = true;
+                        break;
+                    }
+                if (upd && kappa < kappa3[1]) {
+                    kappa *= 2;
+                    kappa3[0] = kappa;
+                    EnergyOut eo = energy(c, x, coll, h, dHat2, kappa, thickness);
+                    Eprev = eo.E;
+                }
+            }
+            if (coll) {
+                long n = c->cw.ncs;
+                double* d2 = c->dist2_prev.ensure(n + 1, s);
+                if (n) S.min_dist2 = min_dist2_dev(c, x, n, (const int*)c->cw.cs.p, d2) - thickness * thickness;
+                csp.ensure(n + 1, s);
+                if (n) IDP_HIP(hipMemcpyAsync(csp.p, c->cw.cs.p, n * sizeof(int4), hipMemcpyDeviceToDevice, s));
+                c->ncs_prev = n;
+            }
+
+            // ---- stopping rule (IMPLICIT_EULER.h:647-676, 759, 855)
+            double ss = dev_dot(c->L, s, n3, sol, sol);
+            L2 = std::sqrt(ss / (m.nV - c->ndbc)) / h;
+            ++PNIter;
+            res.push_back(L2);
+            if (res.size() > 3) res.pop_front();
+            L2 = *std::max_element(res.begin(), res.end());
+            if (useGD) L2 = newton_tol * 10;
+            useGD = (alpha * 2 < 1e-8 && feasibleAlpha > 1e-8);
+            S.newton_iters = PNIter;
+            S.last_res = L2;
+            S.E = Eprev;
+            if (max_newton > 0 && PNIter >= max_newton) break;
+        } while (res.size() < 3 || L2 > newton_tol);
+
+        // v = (x - xn) / h
+        dev_velocity(s, n3, x, xn, h, c->v.p);
+        IDP_HIP(hipStreamSynchronize(s));
+        S.n_constraints = c->cw.ncs;
+        S.kappa = kappa;
+        S.ms_total = now_ms() - t_start;
+        if (st) *st = S;
+        return PNIter;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// stage entry points
+// ---------------------------------------------------------------------------
+int idp_constraint_set(idp_ctx* c, const double* X, double dHat2, double thickness, long* n)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh) return IDP_E_INVALID;
+        const double* xd = stage_x(c, X);
+        long k = contact_constraint_set(c->cw, c->s, c->m, xd, dHat2, thickness);
+        IDP_HIP(hipStreamSynchronize(c->s));
+        if (n) *n = k;
+        return IDP_OK;
+    });
+}
+
+int idp_get_constraints(idp_ctx* c, int* cs, long cap)
+{
+    return guarded(c, [&]() -> int {
+        if (cap < c->cw.ncs) return IDP_E_CAPACITY;
+        c->cw.cs.download((int4*)cs, c->cw.ncs, c->s);
+        IDP_HIP(hipStreamSynchronize(c->s));
+        return IDP_OK;
+    });
+}
+
+int idp_barrier_energy(idp_ctx* c, const double* X, long n, const int* cs, double dHat2, double kappa, double thickness,
+                       double* E)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh) return IDP_E_INVALID;
+        const double* xd = stage_x(c, X);
+        set_constraints_from_host(c, n, cs);
+        int* fl = c->flags.ensure(4, c->s);
+        IDP_HIP(hipMemsetAsync(fl, 0, sizeof(int), c->s));
+        int nb = barrier_energy_blocks(n);
+        double* pb = c->part_b.ensure(nb + 1, c->s);
+        double dh = dHat2 + 2 * std::sqrt(dHat2) * thickness;
+        if (n) launch_barrier_energy(c->s, n, (const int*)c->cw.cs.p, xd, c->m.X0.p, dh, kappa, thickness * thickness,
+                                     pb, fl);
+        int hf = 0;
+        IDP_HIP(hipMemcpyAsync(&hf, fl, sizeof(int), hipMemcpyDeviceToHost, c->s));
+        IDP_HIP(hipStreamSynchronize(c->s));
+        if (hf) return IDP_E_NONPOS_DIST;
+        *E = n ? dev_sum_partials(c->L, c->s, nb, pb) : 0.0;
+        return IDP_OK;
+    });
+}
+
+int idp_barrier_gradient(idp_ctx* c, const double* X, long n, const int* cs, double dHat2, double kappa,
+                         double thickness, double* g)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh) return IDP_E_INVALID;
+        auto s = c->s;
+        const double* xd = stage_x(c, X);
+        set_constraints_from_host(c, n, cs);
+        contact_structure(c->L, s, c->m.nV, n, (const int*)c->cw.cs.p);
+        int* fl = c->flags.ensure(4, s);
+        IDP_HIP(hipMemsetAsync(fl, 0, sizeof(int), s));
+        double dh = dHat2 + 2 * std::sqrt(dHat2) * thickness;
+        double* b = c->gb.ensure(12 * (size_t)std::max(n, 1L), s);
+        if (n) launch_barrier_grad(s, n, (const int*)c->cw.cs.p, xd, c->m.X0.p, dh, kappa, thickness * thickness, b, fl);
+        double* gd = c->g.ensure(3 * (size_t)c->m.nV, s);
+        contact_gradient_gather(c->L, s, c->m, xd, nullptr, nullptr, n ? b : nullptr, gd, nullptr);
+        int hf = 0;
+        IDP_HIP(hipMemcpyAsync(&hf, fl, sizeof(int), hipMemcpyDeviceToHost, s));
+        c->g.download(g, 3 * (size_t)c->m.nV, s);
+        IDP_HIP(hipStreamSynchronize(s));
+        return hf ? IDP_E_NONPOS_DIST : IDP_OK;
+    });
+}
+
+int idp_barrier_hessian_local(idp_ctx* c, const double* X, long n, const int* cs, double dHat2, double kappa,
+                              double thickness, int project, double* H, int* vid)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh) return IDP_E_INVALID;
+        auto s = c->s;
+        const double* xd = stage_x(c, X);
+        set_constraints_from_host(c, n, cs);
+        int* fl = c->flags.ensure(4, s);
+        IDP_HIP(hipMemsetAsync(fl, 0, sizeof(int), s));
+        double dh = dHat2 + 2 * std::sqrt(dHat2) * thickness;
+        double* b = c->Hb.ensure(144 * (size_t)std::max(n, 1L), s);
+        DBuf<int> dv;
+        int* dvid = dv.ensure(4 * (size_t)std::max(n, 1L), s);
+        if (n)
+            launch_barrier_hess(s, n, (const int*)c->cw.cs.p, xd, c->m.X0.p, dh, kappa, thickness * thickness, project,
+                                0, b, dvid, fl);
+        int hf = 0;
+        IDP_HIP(hipMemcpyAsync(&hf, fl, sizeof(int), hipMemcpyDeviceToHost, s));
+        if (n) {
+            IDP_HIP(hipMemcpyAsync(H, b, 144 * 8 * (size_t)n, hipMemcpyDeviceToHost, s));
+            IDP_HIP(hipMemcpyAsync(vid, dvid, 4 * 4 * (size_t)n, hipMemcpyDeviceToHost, s));
+        }
+        IDP_HIP(hipStreamSynchronize(s));
+        return hf ? IDP_E_NONPOS_DIST : IDP_OK;
+    });
+}
+
+int idp_min_dist2(idp_ctx* c, const double* X, long n, const int* cs, double thickness, double* dist2,
+                  double* min_dist2)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh) return IDP_E_INVALID;
+        auto s = c->s;
+        const double* xd = stage_x(c, X);
+        set_constraints_from_host(c, n, cs);
+        double* d = c->tmpv.ensure(std::max(n, 1L), s);
+        double md = n ? min_dist2_dev(c, xd, n, (const int*)c->cw.cs.p, d) : INFINITY;
+        if (dist2 && n) {
+            IDP_HIP(hipMemcpyAsync(dist2, d, 8 * (size_t)n, hipMemcpyDeviceToHost, s));
+            IDP_HIP(hipStreamSynchronize(s));
+        }
+        if (min_dist2) *min_dist2 = md - thickness * thickness;
+        return IDP_OK;
+    });
+}
+
+int idp_ccd_stepsize(idp_ctx* c, const double* X, const double* p, double thickness, double alpha_in,
+                     double* alpha_out, long* n_candidates)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh || !p || !alpha_out) return IDP_E_INVALID;
+        auto s = c->s;
+        const double* xd = stage_x(c, X);
+        c->hp.upload(p, 3 * (size_t)c->m.nV, s);
+        long nc = 0;
+        *alpha_out = contact_ccd(c->cw, s, c->m, xd, c->hp.p, thickness, alpha_in, &nc);
+        if (n_candidates) *n_candidates = nc;
+        return IDP_OK;
+    });
+}
+
+int idp_elastic_energy(idp_ctx* c, const double* X, double w, double* E, int* valid)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh) return IDP_E_INVALID;
+        auto s = c->s;
+        const double* xd = stage_x(c, X);
+        int* fl = c->flags.ensure(4, s);
+        IDP_HIP(hipMemsetAsync(fl, 0, sizeof(int), s));
+        int nb = tet_energy_blocks(c->m.nT);
+        double* pe = c->part_e.ensure(nb + 1, s);
+        launch_tet_energy(s, c->m, xd, w, pe, fl);
+        int hf = 0;
+        IDP_HIP(hipMemcpyAsync(&hf, fl, sizeof(int), hipMemcpyDeviceToHost, s));
+        IDP_HIP(hipStreamSynchronize(s));
+        if (valid) *valid = hf ? 0 : 1;
+        if (E) *E = hf ? 0.0 : dev_sum_partials(c->L, s, nb, pe);
+        return IDP_OK;
+    });
+}
+
+int idp_elastic_gradient(idp_ctx* c, const double* X, double w, double* g)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh) return IDP_E_INVALID;
+        auto s = c->s;
+        const double* xd = stage_x(c, X);
+        double* gl = c->gl.ensure(12 * (size_t)c->m.nT, s);
+        launch_tet_grad(s, c->m, xd, w, gl);
+        c->L.ncb = 0;
+        double* gd = c->g.ensure(3 * (size_t)c->m.nV, s);
+        contact_gradient_gather(c->L, s, c->m, xd, nullptr, gl, nullptr, gd, nullptr);
+        c->g.download(g, 3 * (size_t)c->m.nV, s);
+        IDP_HIP(hipStreamSynchronize(s));
+        return IDP_OK;
+    });
+}
+
+int idp_elastic_hessian_local(idp_ctx* c, const double* X, double w, int project, double* H)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh) return IDP_E_INVALID;
+        auto s = c->s;
+        const double* xd = stage_x(c, X);
+        double* Hl = c->Hl.ensure(144 * (size_t)c->m.nT, s);
+        launch_tet_hess_dense(s, c->m, xd, w, project, Hl);
+        IDP_HIP(hipMemcpyAsync(H, Hl, 144 * 8 * (size_t)c->m.nT, hipMemcpyDeviceToHost, s));
+        IDP_HIP(hipStreamSynchronize(s));
+        return IDP_OK;
+    });
+}
+
+int idp_newton_direction(idp_ctx* c, const double* X, const double* xtilde, double h, double dHat2, double kappa,
+                         double thickness, const idp_solver_opts* opts, double* g, double* sol, double* relres,
+                         int* iters)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh || !xtilde) return IDP_E_INVALID;
+        auto s = c->s;
+        auto& m = c->m;
+        size_t n3 = 3 * (size_t)m.nV;
+        const double* xd = stage_x(c, X);
+        c->xt.upload(xtilde, n3, s);
+        contact_constraint_set(c->cw, s, m, xd, dHat2, thickness);
+        contact_structure(c->L, s, m.nV, c->cw.ncs, (const int*)c->cw.cs.p);
+        double* gd = c->g.ensure(n3, s);
+        double* rhs = c->rhs.ensure(n3, s);
+        double* sd = c->sol.ensure(n3, s);
+        gradient(c, xd, h, dHat2, kappa, thickness, true, gd, rhs);
+        hessian(c, xd, h, dHat2, kappa, thickness, true);
+        double rtol = (opts && opts->pcg_rtol > 0) ? opts->pcg_rtol : 1e-8;
+        int maxit = (opts && opts->pcg_maxit > 0) ? opts->pcg_maxit : 10000;
+        double rel = 0;
+        int it = pcg_solve(c->L, s, m, rhs, sd, rtol, maxit, &rel, 8);
+        if (g) c->g.download(g, n3, s);
+        if (sol) c->sol.download(sol, n3, s);
+        IDP_HIP(hipStreamSynchronize(s));
+        if (relres) *relres = rel;
+        if (iters) *iters = it;
+        return it < 0 ? IDP_E_NOT_SPD : IDP_OK;
+    });
+}
+
+int idp_get_bsr(idp_ctx* c, long* nnzb, int* rowptr, int* col, double* val, long cap)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh) return IDP_E_INVALID;
+        auto s = c->s;
+        auto& m = c->m;
+        auto& L = c->L;
+        // union of E and C patterns, row by row on the host
+        std::vector<int> erp(m.nV + 1), ecol(m.nnzb_e), crp(m.nV + 1, 0), ccol(L.ncb);
+        std::vector<double> ev(9 * m.nnzb_e), cv(9 * L.ncb);
+        m.e_rowptr.download(erp.data(), m.nV + 1, s);
+        m.e_col.download(ecol.data(), m.nnzb_e, s);
+        L.evals.download(ev.data(), 9 * m.nnzb_e, s);
+        if (L.ncb) {
+            L.crp.download(crp.data(), m.nV + 1, s);
+            L.ccol.download(ccol.data(), L.ncb, s);
+            L.cvals.download(cv.data(), 9 * L.ncb, s);
+        }
+        IDP_HIP(hipStreamSynchronize(s));
+        std::vector<int> orp(1, 0), oc;
+        std::vector<double> ov;
+        for (int r = 0; r < m.nV; ++r) {
+            std::vector<std::pair<int, const double*>> ent;
+            for (int k = erp[r]; k < erp[r + 1]; ++k) ent.push_back({ecol[k], &ev[9 * (size_t)k]});
+            if (L.ncb)
+                for (int k = crp[r]; k < crp[r + 1]; ++k) ent.push_back({ccol[k], &cv[9 * (size_t)k]});
+            std::stable_sort(ent.begin(), ent.end(),
+                             [](const std::pair<int, const double*>& a, const std::pair<int, const double*>& b) {
+                                 return a.first < b.first;
+                             });
+            for (size_t i = 0; i < ent.size();) {
+                size_t j = i;
+                double acc[9] = {0};
+                while (j < ent.size() && ent[j].first == ent[i].first) {
+                    for (int k = 0; k < 9; ++k) acc[k] += ent[j].second[k];
+                    ++j;
+                }
+                oc.push_back(ent[i].first);
+                ov.insert(ov.end(), acc, acc + 9);
+                i = j;
+            }
+            orp.push_back((int)oc.size());
+        }
+        *nnzb = (long)oc.size();
+        if (cap < (long)oc.size()) return IDP_E_CAPACITY;
+        std::memcpy(rowptr, orp.data(), orp.size() * sizeof(int));
+        std::memcpy(col, oc.data(), oc.size() * sizeof(int));
+        std::memcpy(val, ov.data(), ov.size() * sizeof(double));
+        return IDP_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,37 @@
+// Host-side interface of the BSR assembly and PCG (linalg.hip).
+#pragma once
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace idp {
+
+struct LinWork {
+    DBuf<unsigned char> tmp;
+    // contact structure of the current constraint set
+    DBuf<int> vid, nv, crow, ccol, cdiag, crp;
+    DBuf<unsigned> u0, u1, seg, rv0, rv1;
+    DBuf<unsigned long long> key0, key1;
+    long ncb = 0, nrec = 0, ncs = 0;
+    // values
+    DBuf<int> erow;  // row of every elastic block
+    DBuf<double> evals, cvals;
+    // PCG
+    DBuf<double> pinv, r, z, p, q, part0, part1, part2, part3, scal, scal2;
+    DBuf<int> err;
+};
+
+void contact_structure(LinWork& L, hipStream_t s, int nV, long ncs, const int* cs);
+void contact_gradient_gather(LinWork& L, hipStream_t s, const MeshDev& m, const double* X, const double* Xt,
+                             const double* gl, const double* gb, double* g, double* rhs);
+void assemble(LinWork& L, hipStream_t s, const MeshDev& m, const double* Hl, const double* Hb, bool any_dbc);
+void init_erow(LinWork& L, hipStream_t s, const MeshDev& m, const std::vector<int>& rowof);
+int pcg_solve(LinWork& L, hipStream_t s, const MeshDev& m, const double* b, double* x, double rtol, int maxit,
+              double* relres, int check_every);
+double dev_dot(LinWork& L, hipStream_t s, long n, const double* a, const double* b);
+void dev_velocity(hipStream_t s, long n, const double* x, const double* xn, double h, double* v);
+void dev_axpy(hipStream_t s, long n, const double* a, double sc, const double* b, double* x);
+double dev_sum_partials(LinWork& L, hipStream_t s, int npart, const double* part);
+void launch_tet_hess_dense(hipStream_t s, const MeshDev& m, const double* X, double w, int project, double* Hl);
+
+}  // namespace idp

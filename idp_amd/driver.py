@@ -1,0 +1,43 @@
+"""Tet animation-fix driver on top of the C ABI.
+
+Mirrors the calls the reference's Python driver makes for a tet scene
+(``Python/Drivers/FEMDiscreteShellBase.py:214-221, 239-243, 297-361``):
+``initialize_added_objects`` (volumes, inverse bases, Lame parameters, lumped
+mass), ``initialize_OIPC`` (dhat^2 and kappa), the target expressed as the
+velocity ``(x_target - x)/h`` (``MeshIO.Load_Velocity_X0``) and
+``Advance_One_Step_IE_Hinge`` per step.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import capi
+
+
+class TetAnimFix:
+    def __init__(self, scene, device=0, pcg_rtol=1e-8, kappa_mult=1.0):
+        self.scene = scene
+        X = np.ascontiguousarray(scene.X, dtype=np.float64)
+        tets = np.ascontiguousarray(scene.tets, dtype=np.int32)
+        self.vol, self.IB, self.lam, self.mu, self.mass = capi.fem_init(X, tets, scene.density, scene.E, scene.nu)
+        self.ctx = capi.Context(device)
+        self.ctx.set_mesh(X, tets, self.IB, self.vol, self.lam, self.mu, self.mass)
+        self.dHat2, self.kappa = capi.initialize_oipc(scene.dhat, self.mass, kappa_mult)
+        self.h = scene.h
+        self.tol = scene.newton_tol
+        self.pcg_rtol = pcg_rtol
+        self.ctx.set_state(X, scene.velocity)
+        self.pn_iters = 0
+
+    def set_target(self, target):
+        X, _ = self.ctx.get_state()
+        self.ctx.set_state(None, (np.asarray(target) - X) / self.h)
+
+    def advance_one_time_step(self, max_newton=0):
+        it, st = self.ctx.advance(self.h, self.tol, self.dHat2, self.kappa, 0.0, True, self.pcg_rtol,
+                                  max_newton=max_newton)
+        self.pn_iters += it
+        return it, st
+
+    def state(self):
+        return self.ctx.get_state()

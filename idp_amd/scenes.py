@@ -1,0 +1,179 @@
+"""Synthetic tet scenes for the animation-fix workloads.
+
+Every mesh is a block of unit cells, each split into the 6-tet Kuhn template of
+``Projects/FEM/input/cube.vtk:5-21`` (all six tets share the 0-6 diagonal, so
+translated copies are conforming).  Targets are analytic maps of the rest
+positions; the solver receives them as the velocity ``v = (x_target - x)/h``,
+the ``Load_Velocity_X0`` semantics of ``Library/Utils/MESHIO.h:987-1005``.
+
+Material and solver constants follow the reference animation-fix driver
+``Projects/FEMShell/16_fix_char_seq.py:27-37`` (dt 0.04, density 1000,
+E 100, nu 0.4, zero gravity, kappa multiplier 1), with dhat and the Newton
+tolerance relative to the rest bounding-box diagonal as the workloads state.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+# cube.vtk corner offsets (point order of the file) and its six tets
+_CORNERS = np.array(
+    [[0, 0, 0], [1, 0, 0], [1, 0, 1], [0, 0, 1], [0, 1, 0], [1, 1, 0], [1, 1, 1], [0, 1, 1]], dtype=np.int64
+)
+_KUHN = np.array(
+    [[0, 4, 6, 5], [3, 6, 2, 0], [0, 4, 7, 6], [3, 6, 0, 7], [2, 0, 6, 1], [6, 0, 5, 1]], dtype=np.int64
+)
+
+DT = 0.04
+DENSITY = 1000.0
+YOUNGS = 100.0
+POISSON = 0.4
+
+
+@dataclasses.dataclass
+class Scene:
+    name: str
+    X: np.ndarray  # (V,3) float64 start positions (= rest)
+    tets: np.ndarray  # (T,4) int32
+    target: np.ndarray  # (V,3) float64
+    h: float = DT
+    density: float = DENSITY
+    E: float = YOUNGS
+    nu: float = POISSON
+    dhat_rel: float = 1e-3
+    tol_rel: float = 1e-4
+
+    @property
+    def diag(self) -> float:
+        return float(np.linalg.norm(self.X.max(0) - self.X.min(0)))
+
+    @property
+    def dhat(self) -> float:
+        return self.dhat_rel * self.diag
+
+    @property
+    def newton_tol(self) -> float:
+        return self.tol_rel * self.diag
+
+    @property
+    def velocity(self) -> np.ndarray:
+        return (self.target - self.X) / self.h
+
+
+def kuhn_block(nx: int, ny: int, nz: int, spacing: float):
+    """Regular (nx+1)(ny+1)(nz+1) grid, vertex id = i + (nx+1)(j + (ny+1)k)."""
+    ii, jj, kk = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    X = np.stack([ii.ravel(order="F"), jj.ravel(order="F"), kk.ravel(order="F")], 1).astype(np.float64)
+    X *= spacing
+    ci, cj, ck = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    cells = np.stack([ci.ravel(order="F"), cj.ravel(order="F"), ck.ravel(order="F")], 1)
+    corner = cells[:, None, :] + _CORNERS[None, :, :]  # (C,8,3)
+    cid = corner[..., 0] + (nx + 1) * (corner[..., 1] + (ny + 1) * corner[..., 2])
+    tets = cid[:, _KUHN].reshape(-1, 4)
+    return X, tets.astype(np.int32)
+
+
+def tet_volumes(X, tets):
+    a = X[tets[:, 1]] - X[tets[:, 0]]
+    b = X[tets[:, 2]] - X[tets[:, 0]]
+    c = X[tets[:, 3]] - X[tets[:, 0]]
+    return np.einsum("ij,ij->i", np.cross(a, b), c) / 6.0
+
+
+def _jitter(X, sigma, seed):
+    rng = np.random.default_rng(seed)
+    return X + rng.normal(0.0, sigma, X.shape)
+
+
+def _bend_with_legs(X, x_mid, y_mid, radius, angle_deg):
+    """Bend about the z axis: the middle of the bar (arc length radius*angle)
+    wraps onto a circle of the given centre-line radius, the rest continues as
+    straight tangent legs, so a bend beyond ~180 degrees makes the legs cross."""
+    s = X[:, 0] - x_mid
+    t = X[:, 1] - y_mid
+    half = math.radians(angle_deg) / 2
+    a = radius * half
+    out = X.copy()
+    arc = np.abs(s) <= a
+    phi = np.where(arc, s / radius, np.sign(s) * half)
+    rr = radius - t
+    bx = rr * np.sin(phi)
+    by = radius - rr * np.cos(phi)
+    leg = np.where(arc, 0.0, np.abs(s) - a)
+    sg = np.sign(s)
+    out[:, 0] = bx + leg * sg * np.cos(phi)
+    out[:, 1] = by + leg * sg * np.sin(phi)
+    return out
+
+
+def bar_bend(nx=20, ny=4, nz=4, spacing=0.05, radius=0.15, angle_deg=220.0, jitter=0.0, seed=0,
+             name="bar_bend_220"):
+    """Bar of nx*ny*nz cells bent about a circle of the given radius."""
+    X, tets = kuhn_block(nx, ny, nz, spacing)
+    if jitter > 0:
+        X = _jitter(X, jitter * spacing, seed)
+    L, W = nx * spacing, ny * spacing
+    target = _bend_with_legs(X, L / 2, W / 2, radius, angle_deg)
+    return Scene(name, X, tets, target)
+
+
+def bar_twist_fold(nx=64, ny=16, nz=16, spacing=0.05, twists=2.0, fold_radius=0.35, jitter=0.05, seed=1,
+                   name="bar_twist2_fold180"):
+    """Two full twists about the long axis, then a 180 degree fold at mid-length."""
+    X, tets = kuhn_block(nx, ny, nz, spacing)
+    X = _jitter(X, jitter * spacing, seed)
+    L, W, D = nx * spacing, ny * spacing, nz * spacing
+    ang = 2 * math.pi * twists * X[:, 0] / L
+    y, z = X[:, 1] - W / 2, X[:, 2] - D / 2
+    Y = np.stack([X[:, 0], W / 2 + np.cos(ang) * y - np.sin(ang) * z, D / 2 + np.sin(ang) * y + np.cos(ang) * z], 1)
+    target = _bend_with_legs(Y, L / 2, W / 2, fold_radius, 180.0)
+    return Scene(name, X, tets, target)
+
+
+def sinusoid_target(X, side, n_modes, amp_range, wl_range, rng):
+    disp = np.zeros_like(X)
+    for _ in range(n_modes):
+        k = rng.normal(size=3)
+        k /= np.linalg.norm(k)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        A = rng.uniform(*amp_range) * side
+        lam = rng.uniform(*wl_range) * side
+        ph = rng.uniform(0, 2 * math.pi)
+        disp += A * np.sin(2 * math.pi * (X @ k) / lam + ph)[:, None] * d[None, :]
+    return X + disp
+
+
+def cube_sinusoid(n, spacing=0.05, jitter=0.03, seed=0, n_modes=4, amp=(0.3, 0.6), wl=(0.5, 1.0)):
+    X, tets = kuhn_block(n, n, n, spacing)
+    rng = np.random.default_rng(seed)
+    X = X + rng.normal(0.0, jitter * spacing, X.shape)
+    target = sinusoid_target(X, n * spacing, n_modes, amp, wl, rng)
+    return Scene(f"cube{n}_sin{n_modes}", X, tets, target)
+
+
+def config1():
+    """20x4x4 cells, spacing 0.05, 220 degree bend about radius 0.15 (ends overlap)."""
+    return bar_bend()
+
+
+def config2():
+    """64x16x16 cells, jitter 0.05*spacing, two twists + 180 degree fold."""
+    return bar_twist_fold()
+
+
+def config5():
+    """160x48x48 cells, jitter 0.02*spacing, crumple of 12 sinusoidal modes."""
+    X, tets = kuhn_block(160, 48, 48, 0.05)
+    rng = np.random.default_rng(5)
+    X = X + rng.normal(0.0, 0.02 * 0.05, X.shape)
+    width = 48 * 0.05
+    target = sinusoid_target(X, width, 12, (0.1, 0.4), (0.5, 1.0), rng)
+    return Scene("bar160x48x48_crumple12", X, tets, target)
+
+
+def small_bend(nx=8, ny=2, nz=2, spacing=0.05, angle_deg=250.0, radius=0.06):
+    """Tiny bar used by the smoke check and fast parity tests."""
+    return bar_bend(nx, ny, nz, spacing, radius, angle_deg, name=f"bar{nx}x{ny}x{nz}_bend{int(angle_deg)}")

@@ -1,0 +1,144 @@
+/*
+ * idp_amd.h - C ABI of the MI355X (gfx950) Newton-with-barrier inner loop.
+ *
+ * Plain C types only: pointers, sizes, status codes.  Host arrays are
+ * borrowed for the duration of a call; device state lives in an opaque
+ * context (one HIP stream each, not thread-safe; use one context per thread).
+ *
+ * Layouts (same as the reference's flattened Storage objects):
+ *   positions, velocities, gradients, directions: 3*nV doubles, vertex-major
+ *   (x0,y0,z0,x1,...), i.e. sol/rhs of Library/FEM/Shell/IMPLICIT_EULER.h
+ *   tets: 4*nT ints; IB: 9*nT doubles, row-major 3x3 inverse rest basis
+ *   constraints: 4 ints each, the encoding of Library/FEM/IPC.h:19-740
+ *     (-p-1,t0,t1,t2) PT   (-p-1,e0,e1,-m) PE x m   (-p-1,q,-1,-m) PP x m
+ *     (a0,a1,b0,b1) EE   (a0,a1,-b0-1,b1) / (p,e0,e1,-o-1) / (p,q,-a-1,-b-1)
+ *     mollified EE / PE / PP
+ *   local Hessians: 144 doubles per constraint (nv*3 square, row-major,
+ *     rows ordered like the reference's triplet emission) plus 4 vertex ids.
+ */
+#ifndef IDP_AMD_H
+#define IDP_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (the reference exits or prints; see each entry point) */
+#define IDP_OK 0
+#define IDP_E_INVALID (-1)      /* bad argument / no mesh */
+#define IDP_E_HIP (-2)          /* HIP runtime error */
+#define IDP_E_NONPOS_DIST (-3)  /* dist2 <= 0 in a barrier term: IPC.h:773-776 exit(-1) */
+#define IDP_E_NOT_SPD (-4)      /* linear solve breakdown: DIRECT_SOLVER.h:42-44 returns false */
+#define IDP_E_CCD_ZERO (-5)     /* zero time of impact: IPC.h:2014-2031 exit(-1) */
+#define IDP_E_CAPACITY (-6)     /* output array too small; *n holds the size needed */
+#define IDP_E_INVALID_START (-7)/* degenerate tet (det F <= 0) at the start of a step */
+#define IDP_E_NO_GPU (-8)       /* no HIP device */
+
+typedef struct idp_ctx idp_ctx;
+
+typedef struct idp_solver_opts {
+    double pcg_rtol;  /* relative residual of the Newton-direction solve (default 1e-8) */
+    int pcg_maxit;    /* default 10000 */
+    int max_newton;   /* 0 = until the reference stopping rule holds */
+} idp_solver_opts;
+
+typedef struct idp_step_stats {
+    int newton_iters;
+    int line_search_trials;
+    int constraint_sets;
+    int gd_iters;
+    long pcg_iters;
+    long n_constraints;
+    long ccd_candidates;
+    double last_alpha, first_ccd_alpha, last_res, last_relres, min_dist2, E;
+    double kappa;
+    /* stage wall time in ms (stream-synchronous timings) */
+    double ms_total, ms_constraint_set, ms_ccd, ms_energy, ms_gradient, ms_hessian, ms_solve;
+} idp_step_stats;
+
+/* ---- context ---------------------------------------------------------- */
+int idp_ctx_create(int device, idp_ctx** out);
+int idp_ctx_destroy(idp_ctx* ctx);
+const char* idp_last_error(const idp_ctx* ctx);
+int idp_device_count(int* n);
+
+/* ---- host-side set-up helpers (no device work) ------------------------- */
+/* Compute_Vol_And_Inv_Basis (Library/FEM/DEFORMATION_GRADIENT.h:39-76),
+ * FIXED_COROTATED::All_Append_FEM Lame parameters (Library/Physics/FIXED_COROTATED.h:56-80),
+ * Compute_Mass_And_Init_Velocity_NoAlloc lumped mass (Library/FEM/ELEM_TO_NODE.h:11-47). */
+int idp_fem_init(int nV, const double* X, int nT, const int* tets, double density, double E, double nu,
+                 double* vol, double* IB, double* lambda, double* mu, double* mass);
+/* Find_Surface_TriMesh<T,false> + Find_Surface_Primitives_And_Compute_Area
+ * (Library/Utils/MESHIO.h:486-545, 769-835).  bnode >= nV ints, btri >= 12*nT,
+ * bedge >= 12*nT. */
+int idp_surface(int nV, const double* X, int nT, const int* tets, int* nb, int* ne, int* nf, int* bnode,
+                int* bedge, int* btri);
+/* Initialize_EIPC<double,false> as bound to Initialize_OIPC
+ * (Library/FEM/Shell/DISCRETE_SHELL.h:555-577, 1105). kappa3 = {kappa, kappa_max, 0}. */
+int idp_initialize_oipc(double dhat, int nV, const double* mass, double stiff_mult, double* kappa3,
+                        double* dHat2);
+
+/* ---- device mesh and state -------------------------------------------- */
+/* Uploads topology, rest positions (nodeAttr.x0: mollifier thresholds),
+ * tetAttr.IB, tetElasticityAttr vol/lambda/mu and nodeAttr.m, extracts the
+ * surface primitives and precomputes the static elastic BSR pattern. */
+int idp_set_mesh(idp_ctx* ctx, int nV, const double* X_rest, int nT, const int* tets, const double* IB,
+                 const double* vol, const double* lambda, const double* mu, const double* mass);
+/* DBC vertex list (VECTOR_STORAGE DBC of the reference, fixed targets = current x). */
+int idp_set_dbc(idp_ctx* ctx, int n, const int* idx);
+int idp_set_state(idp_ctx* ctx, const double* X, const double* v);
+int idp_get_state(idp_ctx* ctx, double* X, double* v);
+/* device-to-device save/restore of (x, v, kappa) for repeated solves */
+int idp_snapshot_state(idp_ctx* ctx);
+int idp_restore_state(idp_ctx* ctx);
+int idp_surface_sizes(idp_ctx* ctx, int* nb, int* ne, int* nf);
+
+/* ---- the hot path ------------------------------------------------------ */
+/* Advance_One_Step_IE_Discrete_Shell<double,3,false,false> for tet meshes
+ * (Library/FEM/Shell/IMPLICIT_EULER.h:152-891, bound as
+ * FEM.DiscreteShell.Advance_One_Step_IE_Hinge at DISCRETE_SHELL.h:1110):
+ * projected Newton with the IPC barrier, ACCD-filtered backtracking line
+ * search, kappa doubling and the max-of-last-3 residual stop.  Mutates the
+ * context's x, v and kappa3[0] in place (like X, nodeAttr.v and kappaVec).
+ * Returns the Newton iteration count (>= 0) or a negative status. */
+int idp_advance_one_step(idp_ctx* ctx, double h, double newton_tol, int with_collision, double dHat2,
+                         double* kappa3, double thickness, const idp_solver_opts* opts, idp_step_stats* stats);
+
+/* ---- stage entry points (parity tests and callers with their own loop) --- */
+/* Compute_Constraint_Set (Library/FEM/IPC.h:19-740) at X (host, 3*nV);
+ * *n receives the constraint count; fetch with idp_get_constraints. */
+int idp_constraint_set(idp_ctx* ctx, const double* X, double dHat2, double thickness, long* n);
+int idp_get_constraints(idp_ctx* ctx, int* cs, long cap);
+/* Compute_Barrier (IPC.h:742-941): E = sum of barrier terms (not +=). */
+int idp_barrier_energy(idp_ctx* ctx, const double* X, long n, const int* cs, double dHat2, double kappa,
+                       double thickness, double* E);
+/* Compute_Barrier_Gradient (IPC.h:943-1256): g (3*nV) is overwritten. */
+int idp_barrier_gradient(idp_ctx* ctx, const double* X, long n, const int* cs, double dHat2, double kappa,
+                         double thickness, double* g);
+/* Compute_Barrier_Hessian (IPC.h:1258-1731) per constraint, PSD-projected
+ * when project != 0 (makePD, Library/Math/UTILS.h:9-27). */
+int idp_barrier_hessian_local(idp_ctx* ctx, const double* X, long n, const int* cs, double dHat2, double kappa,
+                              double thickness, int project, double* H, int* vid);
+/* Compute_Min_Dist2 (IPC.h:2246-2388) */
+int idp_min_dist2(idp_ctx* ctx, const double* X, long n, const int* cs, double thickness, double* dist2,
+                  double* min_dist2);
+/* Compute_Intersection_Free_StepSize (IPC.h:1879-2244) */
+int idp_ccd_stepsize(idp_ctx* ctx, const double* X, const double* p, double thickness, double alpha_in,
+                     double* alpha_out, long* n_candidates);
+/* Neo-Hookean tet terms with weight w = h^2 (INC_POTENTIAL.h:98-109, 228-229, 355-363) */
+int idp_elastic_energy(idp_ctx* ctx, const double* X, double w, double* E, int* valid);
+int idp_elastic_gradient(idp_ctx* ctx, const double* X, double w, double* g);
+int idp_elastic_hessian_local(idp_ctx* ctx, const double* X, double w, int project, double* H);
+/* One Newton direction at X: assemble the projected Hessian (elastic + mass +
+ * barrier for the constraint set at X) into BSR, apply the DBC projection and
+ * solve H sol = -g with block-Jacobi PCG.  xtilde is the inertia target. */
+int idp_newton_direction(idp_ctx* ctx, const double* X, const double* xtilde, double h, double dHat2,
+                         double kappa, double thickness, const idp_solver_opts* opts, double* g, double* sol,
+                         double* relres, int* iters);
+/* BSR of the last assembled system: sizes, then arrays (rowptr nV+1, col nnzb, val 9*nnzb) */
+int idp_get_bsr(idp_ctx* ctx, long* nnzb, int* rowptr, int* col, double* val, long cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

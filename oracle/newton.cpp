@@ -1,0 +1,431 @@
+// Oracle (test infrastructure only): restatement of the projected-Newton
+// time step with IPC barrier for tet meshes,
+//   Advance_One_Step_IE_Discrete_Shell  Library/FEM/Shell/IMPLICIT_EULER.h:152-891
+//   Line_Search                         Library/FEM/Shell/IMPLICIT_EULER.h:10-149
+//   Compute_IncPotential{,_Gradient,_Hessian}  Library/FEM/Shell/INC_POTENTIAL.h:15-395
+//   CSR assembly + DBC projection       Library/Math/CSR_MATRIX.h:49-56, 130-141
+// The linear solve restates the direct solve (DIRECT_SOLVER.h:14-89) with a
+// block-Jacobi preconditioned CG run to a tight relative residual (1e-12 by
+// default); the direction therefore agrees with a direct factorization to
+// that residual.
+#include "oracle.h"
+#include "geom.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <deque>
+#include <vector>
+
+using namespace orc;
+
+namespace {
+
+struct Blk {
+    long key;  // row*nV + col
+    double v[9];
+};
+
+struct Sys {
+    int nV;
+    std::vector<int> rowptr, col;
+    std::vector<double> val;  // 9 per block, row-major 3x3
+};
+
+void add_local(std::vector<Blk>& out, int nv, const int* vid, const double* H, int nV)
+{
+    int N = 3 * nv;
+    for (int a = 0; a < nv; ++a)
+        for (int b = 0; b < nv; ++b) {
+            Blk k;
+            k.key = (long)vid[a] * nV + vid[b];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) k.v[3 * i + j] = H[(3 * a + i) * N + 3 * b + j];
+            out.push_back(k);
+        }
+}
+
+void build_sys(std::vector<Blk>& blks, int nV, Sys& S)
+{
+    std::stable_sort(blks.begin(), blks.end(), [](const Blk& a, const Blk& b) { return a.key < b.key; });
+    S.nV = nV;
+    S.rowptr.assign(nV + 1, 0);
+    S.col.clear();
+    S.val.clear();
+    for (size_t i = 0; i < blks.size();) {
+        size_t j = i;
+        double acc[9] = {0};
+        while (j < blks.size() && blks[j].key == blks[i].key) {
+            for (int k = 0; k < 9; ++k) acc[k] += blks[j].v[k];
+            ++j;
+        }
+        int r = (int)(blks[i].key / nV), c = (int)(blks[i].key % nV);
+        S.rowptr[r + 1]++;
+        S.col.push_back(c);
+        S.val.insert(S.val.end(), acc, acc + 9);
+        i = j;
+    }
+    for (int r = 0; r < nV; ++r) S.rowptr[r + 1] += S.rowptr[r];
+}
+
+void spmv(const Sys& S, const double* x, double* y)
+{
+#pragma omp parallel for schedule(static)
+    for (int r = 0; r < S.nV; ++r) {
+        double a0 = 0, a1 = 0, a2 = 0;
+        for (int k = S.rowptr[r]; k < S.rowptr[r + 1]; ++k) {
+            const double* b = S.val.data() + 9 * (size_t)k;
+            const double* xc = x + 3 * (size_t)S.col[k];
+            a0 += b[0] * xc[0] + b[1] * xc[1] + b[2] * xc[2];
+            a1 += b[3] * xc[0] + b[4] * xc[1] + b[5] * xc[2];
+            a2 += b[6] * xc[0] + b[7] * xc[1] + b[8] * xc[2];
+        }
+        y[3 * r] = a0;
+        y[3 * r + 1] = a1;
+        y[3 * r + 2] = a2;
+    }
+}
+
+bool inv3(const double* a, double* o)
+{
+    double c0 = a[4] * a[8] - a[5] * a[7], c1 = a[5] * a[6] - a[3] * a[8], c2 = a[3] * a[7] - a[4] * a[6];
+    double det = a[0] * c0 + a[1] * c1 + a[2] * c2;
+    if (det == 0) return false;
+    double id = 1.0 / det;
+    o[0] = c0 * id;
+    o[3] = c1 * id;
+    o[6] = c2 * id;
+    o[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+    o[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+    o[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+    o[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+    o[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+    o[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+    return true;
+}
+
+// Block-Jacobi PCG; returns iterations (negative on breakdown)
+int pcg(const Sys& S, const double* b, double* x, double rtol, int maxit, double* relres)
+{
+    size_t n = 3 * (size_t)S.nV;
+    std::vector<double> Dinv(9 * (size_t)S.nV, 0.0), r(n), z(n), p(n), q(n);
+    for (int v = 0; v < S.nV; ++v)
+        for (int k = S.rowptr[v]; k < S.rowptr[v + 1]; ++k)
+            if (S.col[k] == v)
+                if (!inv3(S.val.data() + 9 * (size_t)k, Dinv.data() + 9 * (size_t)v)) return -1;
+    auto prec = [&](const double* in, double* out) {
+#pragma omp parallel for
+        for (int v = 0; v < S.nV; ++v) {
+            const double* D = Dinv.data() + 9 * (size_t)v;
+            for (int i = 0; i < 3; ++i)
+                out[3 * v + i] = D[3 * i] * in[3 * v] + D[3 * i + 1] * in[3 * v + 1] + D[3 * i + 2] * in[3 * v + 2];
+        }
+    };
+    auto dotp = [&](const double* a, const double* c) {
+        double s = 0;
+#pragma omp parallel for reduction(+ : s)
+        for (long i = 0; i < (long)n; ++i) s += a[i] * c[i];
+        return s;
+    };
+    std::fill(x, x + n, 0.0);
+    for (size_t i = 0; i < n; ++i) r[i] = b[i];
+    double bn = std::sqrt(dotp(b, b));
+    if (bn == 0) {
+        *relres = 0;
+        return 0;
+    }
+    prec(r.data(), z.data());
+    p = z;
+    double rz = dotp(r.data(), z.data());
+    int it = 0;
+    double rn = bn;
+    for (; it < maxit; ++it) {
+        rn = std::sqrt(dotp(r.data(), r.data()));
+        if (rn <= rtol * bn) break;
+        spmv(S, p.data(), q.data());
+        double pq = dotp(p.data(), q.data());
+        if (pq <= 0) return -2;
+        double al = rz / pq;
+#pragma omp parallel for
+        for (long i = 0; i < (long)n; ++i) {
+            x[i] += al * p[i];
+            r[i] -= al * q[i];
+        }
+        prec(r.data(), z.data());
+        double rz2 = dotp(r.data(), z.data());
+        double be = rz2 / rz;
+        rz = rz2;
+#pragma omp parallel for
+        for (long i = 0; i < (long)n; ++i) p[i] = z[i] + be * p[i];
+    }
+    *relres = rn / bn;
+    return it;
+}
+
+struct Ctx {
+    int nV, nT, nb, ne, nf;
+    const double* X0;
+    const double* mass;
+    const int* tets;
+    const double *IB, *vol, *lam, *mu;
+    const int *bnode, *bedge, *btri;
+    std::vector<unsigned char> dbc;
+    double h, dHat2, thickness;
+    double* kappa;
+    bool withCollision;
+    std::vector<double> Xtilde;
+    std::vector<int> cs;
+    long ncs = 0;
+    orc_stats* st;
+    double t_cs = 0, t_ccd = 0, t_solve = 0, t_hess = 0, t_grad = 0, t_E = 0;
+};
+
+double now()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void compute_cs(Ctx& c, const double* X)
+{
+    double t0 = now();
+    long cap = std::max<long>(1024, c.ncs * 2 + 1024);
+    for (;;) {
+        c.cs.resize(4 * cap);
+        long n = orc_constraint_set(c.nV, X, c.X0, c.dbc.data(), c.nb, c.bnode, c.ne, c.bedge, c.nf, c.btri,
+                                    c.dHat2, c.thickness, 1, c.cs.data(), cap);
+        if (n >= 0) {
+            c.ncs = n;
+            break;
+        }
+        cap = -n;
+    }
+    c.t_cs += now() - t0;
+    if (c.st) c.st->n_constraint_sets++;
+}
+
+// Compute_IncPotential (tet + inertia + barrier); returns validity
+bool energy(Ctx& c, const double* X, bool withBarrier, double& E)
+{
+    double t0 = now();
+    double Ee = 0;
+    if (!orc_elastic_energy(X, c.nT, c.tets, c.IB, c.vol, c.lam, c.mu, c.h * c.h, &Ee)) {
+        c.t_E += now() - t0;
+        return false;
+    }
+    double Ei = 0;
+    for (int v = 0; v < c.nV; ++v)
+        for (int d = 0; d < 3; ++d) {
+            double dx = X[3 * v + d] - c.Xtilde[3 * v + d];
+            Ei += (c.mass[v] * dx) * dx;
+        }
+    E = Ee + 0.5 * Ei;
+    if (withBarrier && c.ncs) {
+        double Eb = 0;
+        if (orc_barrier_energy(X, c.X0, c.ncs, c.cs.data(), c.dHat2, c.kappa[0], c.thickness, &Eb, nullptr))
+            return false;
+        E += Eb;
+    }
+    c.t_E += now() - t0;
+    return true;
+}
+
+}  // namespace
+
+extern "C" int orc_advance_one_step(int nV, double* X, const double* X0, double* vel, const double* mass,
+                                    int nT, const int* tets, const double* IB, const double* vol,
+                                    const double* lam, const double* mu, int nb, const int* bnode, int ne,
+                                    const int* bedge, int nf, const int* btri, int nDBC, const int* dbc_idx,
+                                    double h, double NewtonTol, int withCollision, double dHat2, double* kappa,
+                                    double thickness, double lin_rtol, int max_newton, orc_stats* st)
+{
+    Ctx c;
+    c.nV = nV; c.nT = nT; c.nb = nb; c.ne = ne; c.nf = nf;
+    c.X0 = X0; c.mass = mass; c.tets = tets; c.IB = IB; c.vol = vol; c.lam = lam; c.mu = mu;
+    c.bnode = bnode; c.bedge = bedge; c.btri = btri;
+    c.dbc.assign(nV, 0);
+    for (int i = 0; i < nDBC; ++i) c.dbc[dbc_idx[i]] = 1;
+    c.h = h; c.dHat2 = dHat2; c.thickness = thickness; c.kappa = kappa; c.withCollision = withCollision != 0;
+    c.st = st;
+    if (st) std::memset(st, 0, sizeof(*st));
+    const size_t n3 = 3 * (size_t)nV;
+    std::vector<double> Xn(X, X + n3);
+    c.Xtilde.resize(n3);
+    // IMPLICIT_EULER.h:196-217 with zero body force: x~ = x + h v
+    for (size_t i = 0; i < n3; ++i) c.Xtilde[i] = X[i] + (h * vel[i] + h * h * 0.0);
+
+    if (c.withCollision) compute_cs(c, X);
+    double Eprev;
+    if (!energy(c, X, c.withCollision, Eprev)) return ORC_E_INVALID_START;
+    std::deque<double> resRecord;
+    std::vector<double> g(n3), rhs(n3), sol(n3), Xprev(n3), dist2, dist2_prev;
+    std::vector<int> cs_prev;
+    long ncs_prev = 0;
+    std::vector<double> Hl;
+    std::vector<int> vl;
+    bool useGD = false;
+    int PNIter = 0;
+    double L2Norm = 0;
+    do {
+        // gradient (INC_POTENTIAL.h:154-275)
+        double t0 = now();
+        std::fill(g.begin(), g.end(), 0.0);
+        orc_elastic_gradient(X, nT, tets, IB, vol, lam, mu, h * h, g.data());
+        for (int v = 0; v < nV; ++v)
+            for (int d = 0; d < 3; ++d) g[3 * v + d] += mass[v] * (X[3 * v + d] - c.Xtilde[3 * v + d]);
+        if (c.withCollision && c.ncs)
+            orc_barrier_gradient(X, X0, c.ncs, c.cs.data(), dHat2, kappa[0], thickness, nV, g.data());
+        for (int v = 0; v < nV; ++v)
+            if (c.dbc[v]) g[3 * v] = g[3 * v + 1] = g[3 * v + 2] = 0;
+        for (size_t i = 0; i < n3; ++i) rhs[i] = -g[i];
+        c.t_grad += now() - t0;
+
+        if (!useGD) {
+            t0 = now();
+            std::vector<Blk> blks;
+            blks.reserve(16 * (size_t)nT + nV + 16 * c.ncs);
+            std::vector<double> He(144 * (size_t)nT);
+            orc_elastic_hessian_local(X, nT, tets, IB, vol, lam, mu, h * h, 1, He.data());
+            for (int t = 0; t < nT; ++t) add_local(blks, 4, tets + 4 * t, He.data() + 144 * (size_t)t, nV);
+            if (c.withCollision && c.ncs) {
+                Hl.resize(144 * c.ncs);
+                vl.resize(4 * c.ncs);
+                orc_barrier_hessian_local(X, X0, c.ncs, c.cs.data(), dHat2, kappa[0], thickness, 1, Hl.data(),
+                                          vl.data());
+                for (long i = 0; i < c.ncs; ++i) {
+                    int nv = 0;
+                    while (nv < 4 && vl[4 * i + nv] >= 0) ++nv;
+                    add_local(blks, nv, vl.data() + 4 * i, Hl.data() + 144 * i, nV);
+                }
+            }
+            for (int v = 0; v < nV; ++v) {
+                Blk k;
+                k.key = (long)v * nV + v;
+                std::memset(k.v, 0, sizeof(k.v));
+                k.v[0] = k.v[4] = k.v[8] = mass[v];
+                blks.push_back(k);
+            }
+            Sys S;
+            build_sys(blks, nV, S);
+            // Project_DBC: rows/cols of DBC nodes -> identity
+            for (int r = 0; r < nV; ++r)
+                for (int k = S.rowptr[r]; k < S.rowptr[r + 1]; ++k)
+                    if (c.dbc[r] || c.dbc[S.col[k]])
+                        for (int i = 0; i < 3; ++i)
+                            for (int j = 0; j < 3; ++j) S.val[9 * (size_t)k + 3 * i + j] = (r == S.col[k] && i == j);
+            c.t_hess += now() - t0;
+            t0 = now();
+            double relres = 0;
+            int it = pcg(S, rhs.data(), sol.data(), lin_rtol, 100000, &relres);
+            c.t_solve += now() - t0;
+            if (st) {
+                st->linear_iters += std::max(it, 0);
+                st->last_relres = relres;
+            }
+            if (it < 0) {
+                useGD = true;
+                sol = rhs;
+            }
+        } else {
+            sol = rhs;
+        }
+
+        // Line_Search (IMPLICIT_EULER.h:10-149)
+        std::copy(X, X + n3, Xprev.begin());
+        double alpha = 1.0, E = 0;
+        bool valid;
+        do {
+            for (size_t i = 0; i < n3; ++i) X[i] = Xprev[i] + alpha * sol[i];
+            valid = energy(c, X, false, E);
+            if (!valid) alpha /= 2.0;
+        } while (!valid);
+        std::copy(Xprev.begin(), Xprev.end(), X);
+        if (c.withCollision) {
+            t0 = now();
+            std::vector<unsigned char> dbcm = c.dbc;
+            orc_ccd_stepsize(nV, X, sol.data(), dbcm.data(), nb, bnode, ne, bedge, nf, btri, thickness, alpha,
+                             &alpha, nullptr);
+            c.t_ccd += now() - t0;
+        }
+        double feasibleAlpha = alpha;
+        if (st && st->n_newton == 0) st->first_ccd_alpha = alpha;
+        do {
+            for (size_t i = 0; i < n3; ++i) X[i] = Xprev[i] + alpha * sol[i];
+            valid = energy(c, X, false, E);
+            if (valid) {
+                if (c.withCollision) {
+                    compute_cs(c, X);
+                    if (c.ncs) {
+                        double md;
+                        orc_min_dist2(X, c.ncs, c.cs.data(), thickness, nullptr, &md);
+                        if (md <= 0) {
+                            alpha /= 2;
+                            E = Eprev + 1;
+                            continue;
+                        }
+                    }
+                }
+                valid = energy(c, X, c.withCollision, E);
+            }
+            alpha /= 2.0;
+            if (st) st->line_search_trials++;
+        } while (E > Eprev || !valid);
+        Eprev = E;
+        if (st) st->last_alpha = alpha * 2;
+
+        // kappa update (IMPLICIT_EULER.h:566-595)
+        if (ncs_prev) {
+            std::vector<double> cur(ncs_prev);
+            double md;
+            orc_min_dist2(X, ncs_prev, cs_prev.data(), thickness, cur.data(), &md);
+            bool upd = false;
+            for (long i = 0; i < ncs_prev; ++i)
+                if (dist2_prev[i] < 1e-18 && cur[i] < dist2_prev[i]) {
+                    upd = true;
+                    break;
+                }
+            if (upd && kappa[0] < kappa[1]) {
+                kappa[0] *= 2;
+                energy(c, X, c.withCollision, Eprev);
+            }
+        }
+        dist2.assign(c.ncs, 0.0);
+        if (c.ncs) {
+            double md;
+            orc_min_dist2(X, c.ncs, c.cs.data(), thickness, dist2.data(), &md);
+            if (st) st->min_dist2 = md;
+        }
+        cs_prev.assign(c.cs.begin(), c.cs.begin() + 4 * c.ncs);
+        ncs_prev = c.ncs;
+        dist2_prev = dist2;
+
+        // stopping criterion (IMPLICIT_EULER.h:647-676)
+        L2Norm = 0;
+        for (size_t i = 0; i < n3; ++i) L2Norm += sol[i] * sol[i];
+        L2Norm = std::sqrt(L2Norm / (nV - nDBC)) / h;
+        ++PNIter;
+        resRecord.push_back(L2Norm);
+        if (resRecord.size() > 3) resRecord.pop_front();
+        L2Norm = *std::max_element(resRecord.begin(), resRecord.end());
+        if (useGD) L2Norm = NewtonTol * 10;
+        if (alpha * 2 < 1e-8 && feasibleAlpha > 1e-8) useGD = true;
+        else useGD = false;
+        if (st) {
+            st->n_newton = PNIter;
+            st->n_constraints = c.ncs;
+            st->last_res = L2Norm;
+            st->E = Eprev;
+        }
+        if (max_newton > 0 && PNIter >= max_newton) break;
+    } while (resRecord.size() < 3 || L2Norm > NewtonTol);
+
+    for (size_t i = 0; i < n3; ++i) vel[i] = (X[i] - Xn[i]) / h;
+    if (st) {
+        st->t_constraint_set = c.t_cs;
+        st->t_ccd = c.t_ccd;
+        st->t_solve = c.t_solve;
+        st->t_hessian = c.t_hess;
+        st->t_gradient = c.t_grad;
+        st->t_energy = c.t_E;
+    }
+    return PNIter;
+}

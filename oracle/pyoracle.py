@@ -1,0 +1,229 @@
+"""ctypes loader for the oracle (test infrastructure only).
+
+Used by tests/, ``__graft_entry__.smoke()`` and the cpu_baseline leg of
+bench.py to check or time the CPU restatement; the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("n_newton", C.c_int),
+        ("line_search_trials", C.c_int),
+        ("n_constraint_sets", C.c_int),
+        ("linear_iters", C.c_long),
+        ("n_constraints", C.c_long),
+        ("last_alpha", C.c_double),
+        ("first_ccd_alpha", C.c_double),
+        ("last_res", C.c_double),
+        ("last_relres", C.c_double),
+        ("min_dist2", C.c_double),
+        ("E", C.c_double),
+        ("t_constraint_set", C.c_double),
+        ("t_ccd", C.c_double),
+        ("t_solve", C.c_double),
+        ("t_hessian", C.c_double),
+        ("t_gradient", C.c_double),
+        ("t_energy", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        L.orc_surface.argtypes = [C.c_int, dp, C.c_int, ip, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                  C.POINTER(C.c_int), ip, ip, ip]
+        L.orc_constraint_set.restype = C.c_long
+        L.orc_constraint_set.argtypes = [C.c_int, dp, dp, u8p, C.c_int, ip, C.c_int, ip, C.c_int, ip, C.c_double,
+                                         C.c_double, C.c_int, ip, C.c_long]
+        L.orc_barrier_energy.argtypes = [dp, dp, C.c_long, ip, C.c_double, C.c_double, C.c_double,
+                                         C.POINTER(C.c_double), C.c_void_p]
+        L.orc_barrier_gradient.argtypes = [dp, dp, C.c_long, ip, C.c_double, C.c_double, C.c_double, C.c_int, dp]
+        L.orc_barrier_hessian_local.argtypes = [dp, dp, C.c_long, ip, C.c_double, C.c_double, C.c_double,
+                                                C.c_int, dp, ip]
+        L.orc_min_dist2.argtypes = [dp, C.c_long, ip, C.c_double, C.c_void_p, C.POINTER(C.c_double)]
+        L.orc_ccd_stepsize.argtypes = [C.c_int, dp, dp, u8p, C.c_int, ip, C.c_int, ip, C.c_int, ip, C.c_double,
+                                       C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_long)]
+        L.orc_geom_probe.argtypes = [C.c_int, dp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.orc_fem_init.argtypes = [C.c_int, dp, C.c_int, ip, C.c_double, C.c_double, C.c_double, dp, dp, dp, dp,
+                                   dp]
+        L.orc_elastic_energy.argtypes = [dp, C.c_int, ip, dp, dp, dp, dp, C.c_double, C.POINTER(C.c_double)]
+        L.orc_elastic_gradient.argtypes = [dp, C.c_int, ip, dp, dp, dp, dp, C.c_double, dp]
+        L.orc_elastic_gradient.restype = None
+        L.orc_elastic_hessian_local.argtypes = [dp, C.c_int, ip, dp, dp, dp, dp, C.c_double, C.c_int, dp]
+        L.orc_elastic_hessian_local.restype = None
+        L.orc_advance_one_step.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int, ip,
+                                           C.c_int, ip, C.c_int, ip, C.c_int, ip, C.c_double, C.c_double,
+                                           C.c_int, C.c_double, dp, C.c_double, C.c_double, C.c_int,
+                                           C.POINTER(Stats)]
+        _lib = L
+    return _lib
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ravel()
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32).ravel()
+
+
+def surface(X, tets):
+    X = _f(X)
+    tets = _i(tets)
+    nV, nT = X.size // 3, tets.size // 4
+    bn = np.zeros(nV, np.int32)
+    bt = np.zeros(3 * 4 * nT, np.int32)
+    be = np.zeros(2 * 6 * nT, np.int32)
+    nb, ne, nf = C.c_int(), C.c_int(), C.c_int()
+    lib().orc_surface(nV, X, nT, tets, C.byref(nb), C.byref(ne), C.byref(nf), bn, be, bt)
+    return bn[: nb.value].copy(), be[: 2 * ne.value].reshape(-1, 2).copy(), bt[: 3 * nf.value].reshape(-1, 3).copy()
+
+
+class Mesh:
+    """Host-side bundle of what the reference keeps in its Storage objects."""
+
+    def __init__(self, X, tets, density, E, nu):
+        self.X = _f(X).copy()
+        self.X0 = self.X.copy()
+        self.tets = _i(tets)
+        self.nV, self.nT = self.X.size // 3, self.tets.size // 4
+        self.vol = np.zeros(self.nT)
+        self.IB = np.zeros(9 * self.nT)
+        self.lam = np.zeros(self.nT)
+        self.mu = np.zeros(self.nT)
+        self.mass = np.zeros(self.nV)
+        rc = lib().orc_fem_init(self.nV, self.X, self.nT, self.tets, density, E, nu, self.vol, self.IB, self.lam,
+                                self.mu, self.mass)
+        if rc != 0:
+            raise RuntimeError(f"orc_fem_init failed: {rc}")
+        self.bnode, self.bedge, self.btri = surface(self.X, self.tets)
+        self.dbc = np.zeros(self.nV, np.uint8)
+
+    def constraint_set(self, X, dHat2, thickness=0.0, use_hash=True):
+        X = _f(X)
+        cap = 1 << 16
+        while True:
+            out = np.zeros(4 * cap, np.int32)
+            n = lib().orc_constraint_set(self.nV, X, self.X0, self.dbc, self.bnode.size, self.bnode,
+                                         self.bedge.shape[0], _i(self.bedge), self.btri.shape[0], _i(self.btri),
+                                         dHat2, thickness, int(use_hash), out, cap)
+            if n >= 0:
+                return out[: 4 * n].reshape(-1, 4).copy()
+            cap = -n
+
+    def barrier_energy(self, X, cs, dHat2, kappa, thickness=0.0):
+        E = C.c_double()
+        rc = lib().orc_barrier_energy(_f(X), self.X0, cs.shape[0], _i(cs), dHat2, kappa, thickness, C.byref(E), None)
+        if rc:
+            raise RuntimeError("non-positive distance")
+        return E.value
+
+    def barrier_gradient(self, X, cs, dHat2, kappa, thickness=0.0):
+        g = np.zeros(3 * self.nV)
+        rc = lib().orc_barrier_gradient(_f(X), self.X0, cs.shape[0], _i(cs), dHat2, kappa, thickness, self.nV, g)
+        if rc:
+            raise RuntimeError("non-positive distance")
+        return g
+
+    def barrier_hessian_local(self, X, cs, dHat2, kappa, thickness=0.0, project=True):
+        n = cs.shape[0]
+        H = np.zeros(144 * n)
+        vid = np.zeros(4 * n, np.int32)
+        rc = lib().orc_barrier_hessian_local(_f(X), self.X0, n, _i(cs), dHat2, kappa, thickness, int(project), H,
+                                             vid)
+        if rc:
+            raise RuntimeError("non-positive distance")
+        return H.reshape(n, 144), vid.reshape(n, 4)
+
+    def min_dist2(self, X, cs, thickness=0.0):
+        d = np.zeros(cs.shape[0])
+        m = C.c_double()
+        lib().orc_min_dist2(_f(X), cs.shape[0], _i(cs), thickness, d.ctypes.data_as(C.c_void_p), C.byref(m))
+        return d, m.value
+
+    def ccd(self, X, p, alpha, thickness=0.0):
+        out = C.c_double()
+        nc = C.c_long()
+        lib().orc_ccd_stepsize(self.nV, _f(X), _f(p), self.dbc, self.bnode.size, self.bnode, self.bedge.shape[0],
+                               _i(self.bedge), self.btri.shape[0], _i(self.btri), thickness, alpha, C.byref(out),
+                               C.byref(nc))
+        return out.value, nc.value
+
+    def elastic_energy(self, X, w):
+        E = C.c_double()
+        ok = lib().orc_elastic_energy(_f(X), self.nT, self.tets, self.IB, self.vol, self.lam, self.mu, w, C.byref(E))
+        return (E.value if ok else None)
+
+    def elastic_gradient(self, X, w):
+        g = np.zeros(3 * self.nV)
+        lib().orc_elastic_gradient(_f(X), self.nT, self.tets, self.IB, self.vol, self.lam, self.mu, w, g)
+        return g
+
+    def elastic_hessian_local(self, X, w, project=True):
+        H = np.zeros(144 * self.nT)
+        lib().orc_elastic_hessian_local(_f(X), self.nT, self.tets, self.IB, self.vol, self.lam, self.mu, w,
+                                        int(project), H)
+        return H.reshape(self.nT, 144)
+
+    def advance(self, X, vel, h, tol, dHat2, kappa, thickness=0.0, lin_rtol=1e-12, max_newton=0,
+                with_collision=True, dbc_idx=None):
+        X = _f(X).copy()
+        vel = _f(vel).copy()
+        kap = np.ascontiguousarray(kappa, dtype=np.float64).copy()
+        dbc_idx = np.zeros(0, np.int32) if dbc_idx is None else _i(dbc_idx)
+        st = Stats()
+        it = lib().orc_advance_one_step(self.nV, X, self.X0, vel, self.mass, self.nT, self.tets, self.IB, self.vol,
+                                        self.lam, self.mu, self.bnode.size, self.bnode, self.bedge.shape[0],
+                                        _i(self.bedge), self.btri.shape[0], _i(self.btri), dbc_idx.size, dbc_idx,
+                                        h, tol, int(with_collision), dHat2, kap, thickness, lin_rtol, max_newton,
+                                        C.byref(st))
+        return it, X.reshape(-1, 3), vel.reshape(-1, 3), kap, st.as_dict()
+
+
+def geom_probe(kind, pts):
+    """(type, unclassified squared distance) of a PT / EE / PE configuration."""
+    x = np.zeros(12)
+    pts = np.asarray(pts, dtype=np.float64).ravel()
+    x[: pts.size] = pts
+    t = C.c_int()
+    d = C.c_double()
+    lib().orc_geom_probe(int(kind), x, C.byref(t), C.byref(d))
+    return t.value, d.value
+
+
+def barrier_d2b(d, dh, k):
+    t = d - dh
+    return k * (-2.0 * np.log(d / dh) - 4.0 * t / d + t * t / (d * d))
+
+
+def initialize_oipc(dhat, mass, stiff_mult=1.0):
+    """Initialize_EIPC<double,false> (Library/FEM/Shell/DISCRETE_SHELL.h:555-577)."""
+    dHat2 = dhat * dhat
+    Hb = barrier_d2b(1.0e-16, dHat2, 1.0)
+    k0 = stiff_mult * 1.0e11 * float(np.mean(mass)) * 3 / (4.0e-16 * Hb)
+    return dHat2, np.array([k0, 100 * k0, 0.0])
