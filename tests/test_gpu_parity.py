@@ -1,0 +1,200 @@
+"""Parity of the HIP path with the oracle, through the C ABI (MI355X).
+
+Bars: bit-exact pair sets and CCD step sizes; energies and gradients within
+1e-10 relative; local Hessians within 1e-10 of their max entry (1e-9 after
+the PSD clamp); Newton direction within the PCG relative residual of a
+direct sparse solve; final positions within 1e-6 of the bounding-box
+diagonal; every accepted iterate intersection free.
+"""
+import numpy as np
+import pytest
+
+from idp_amd import capi, driver, scenes
+from oracle import pyoracle
+from tests import exact
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _canon(cs):
+    return sorted(tuple(int(x) for x in c) for c in cs)
+
+
+class Case:
+    def __init__(self, sc):
+        self.sc = sc
+        self.m = pyoracle.Mesh(sc.X, sc.tets, sc.density, sc.E, sc.nu)
+        self.dHat2, self.kappa = pyoracle.initialize_oipc(sc.dhat, self.m.mass)
+        it, Xo, vo, ko, st = self.m.advance(sc.X, sc.velocity, sc.h, sc.newton_tol, self.dHat2, self.kappa)
+        self.it_o, self.Xo, self.st_o = it, Xo.ravel(), st
+        vol, IB, lam, mu, mass = capi.fem_init(sc.X, sc.tets, sc.density, sc.E, sc.nu)
+        say(f"[{sc.name}] context + set_mesh")
+        self.ctx = capi.Context(0)
+        self.ctx.set_mesh(sc.X, sc.tets, IB, vol, lam, mu, mass)
+
+
+@pytest.fixture(scope="module", params=["small", "config1"])
+def case(request):
+    sc = scenes.small_bend() if request.param == "small" else scenes.config1()
+    return Case(sc)
+
+
+def _states(c):
+    sc = c.sc
+    return [("solution", c.Xo), ("halfway", sc.X.ravel() + 0.5 * (c.Xo - sc.X.ravel()))]
+
+
+def test_surface_sizes(case):
+    say(f"[{case.sc.name}] surface sizes")
+    assert case.ctx.surface_sizes() == (case.m.bnode.size, case.m.bedge.shape[0], case.m.btri.shape[0])
+
+
+@pytest.mark.parametrize("radius_mult", [1, 25])
+def test_constraint_set_bit_exact(case, radius_mult):
+    for tag, X in _states(case):
+        d2 = case.dHat2 * radius_mult
+        say(f"[{case.sc.name}] constraint set at {tag} r2x{radius_mult}")
+        g = case.ctx.constraint_set(X, d2)
+        o = case.m.constraint_set(X, d2)
+        assert _canon(g) == _canon(o)
+
+
+def test_barrier_energy_gradient_hessian(case):
+    X = case.Xo
+    cs = case.m.constraint_set(X, case.dHat2)
+    assert len(cs) > 0
+    k = case.kappa[0]
+    say(f"[{case.sc.name}] barrier energy/gradient/hessian on {len(cs)} constraints")
+    Eg = case.ctx.barrier_energy(X, cs, case.dHat2, k)
+    Eo = case.m.barrier_energy(X, cs, case.dHat2, k)
+    assert abs(Eg - Eo) <= 1e-10 * abs(Eo)
+    gg = case.ctx.barrier_gradient(X, cs, case.dHat2, k)
+    go = case.m.barrier_gradient(X, cs, case.dHat2, k)
+    assert np.linalg.norm(gg - go) <= 1e-10 * np.linalg.norm(go)
+    for project, tol in ((False, 1e-10), (True, 1e-9)):
+        Hg, vg = case.ctx.barrier_hessian_local(X, cs, case.dHat2, k, project=project)
+        Ho, vo = case.m.barrier_hessian_local(X, cs, case.dHat2, k, project=project)
+        assert np.array_equal(vg, vo)
+        scale = np.abs(Ho).max(1)
+        assert np.all(np.abs(Hg - Ho).max(1) <= tol * scale)
+
+
+def test_min_dist2(case):
+    X = case.Xo
+    cs = case.m.constraint_set(X, case.dHat2)
+    say(f"[{case.sc.name}] min dist2")
+    dg, mg = case.ctx.min_dist2(X, cs)
+    do, mo = case.m.min_dist2(X, cs)
+    assert np.array_equal(dg, do) and mg == mo
+
+
+def test_elastic_terms(case):
+    w = case.sc.h ** 2
+    nT = case.sc.tets.shape[0]
+    for tag, X in _states(case):
+        say(f"[{case.sc.name}] elastic terms at {tag}")
+        Eg, Eo = case.ctx.elastic_energy(X, w), case.m.elastic_energy(X, w)
+        assert abs(Eg - Eo) <= 1e-10 * abs(Eo)
+        gg, go = case.ctx.elastic_gradient(X, w), case.m.elastic_gradient(X, w)
+        assert np.linalg.norm(gg - go) <= 1e-10 * np.linalg.norm(go)
+        Hg = case.ctx.elastic_hessian_local(X, w, nT)
+        Ho = case.m.elastic_hessian_local(X, w)
+        assert np.all(np.abs(Hg - Ho).max(1) <= 1e-9 * np.abs(Ho).max(1))
+
+
+def test_elastic_invalid_state_detected(case):
+    X = case.sc.X.copy().ravel()
+    t = case.sc.tets[0]
+    X[3 * t[0]: 3 * t[0] + 3] = X[3 * t[1]: 3 * t[1] + 3]  # collapse a tet: det F = 0
+    say(f"[{case.sc.name}] degenerate tet detection")
+    assert case.ctx.elastic_energy(X, 1.0) is None
+    assert case.m.elastic_energy(X, 1.0) is None
+
+
+def test_ccd_bit_exact(case):
+    sc = case.sc
+    for tag, X, p in [("rest->target", sc.X.ravel(), (sc.target - sc.X).ravel()),
+                      ("solution->target", case.Xo, sc.target.ravel() - case.Xo),
+                      ("rest->half", sc.X.ravel(), 0.5 * (sc.target - sc.X).ravel())]:
+        say(f"[{sc.name}] CCD {tag}")
+        ag, _ = case.ctx.ccd(X, p, 1.0)
+        ao, _ = case.m.ccd(X, p, 1.0)
+        assert ag == ao
+        assert 0 < ag <= 1.0
+
+
+def test_newton_direction_vs_direct_solve(case):
+    import scipy.sparse as sps
+    import scipy.sparse.linalg as spla
+
+    sc = case.sc
+    X = sc.X.ravel() + 0.5 * (case.Xo - sc.X.ravel())
+    xt = sc.target.ravel()
+    say(f"[{sc.name}] newton direction")
+    g, sol, rel, it = case.ctx.newton_direction(X, xt, sc.h, case.dHat2, case.kappa[0], pcg_rtol=1e-12)
+    assert it > 0 and rel <= 1e-12
+    rp, col, val = case.ctx.get_bsr()
+    n = rp.size - 1
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    A = sps.bsr_matrix((val, col, rp), shape=(3 * n, 3 * n)).tocsc()
+    assert abs(A - A.T).max() <= 1e-12 * abs(A).max()
+    d = spla.spsolve(A, -g)
+    assert np.linalg.norm(sol - d) <= 1e-8 * np.linalg.norm(d)
+    # gradient = oracle elastic + inertia + barrier
+    w = sc.h ** 2
+    m = case.m
+    cs = m.constraint_set(X, case.dHat2)
+    go = m.elastic_gradient(X, w) + np.repeat(m.mass, 3) * (X - xt)
+    if len(cs):
+        go += m.barrier_gradient(X, cs, case.dHat2, case.kappa[0])
+    assert np.linalg.norm(g - go) <= 1e-10 * np.linalg.norm(go)
+    del rows
+
+
+def test_advance_matches_oracle(case):
+    sc = case.sc
+    say(f"[{sc.name}] advance one step (oracle newton {case.it_o})")
+    d = driver.TetAnimFix(sc, pcg_rtol=1e-12)
+    it, st = d.advance_one_time_step()
+    X, v = d.state()
+    err = np.abs(X.ravel() - case.Xo).max() / sc.diag
+    say(f"[{sc.name}] gpu newton {it}, max err/diag {err:.3e}, constraints {st['n_constraints']}")
+    assert err < 1e-6
+    assert not exact.surface_self_intersects(X, case.m.bedge, case.m.btri)
+    cs = case.m.constraint_set(X.ravel(), case.dHat2)
+    assert case.m.min_dist2(X.ravel(), cs)[1] > 0
+    assert np.allclose(v, (X - sc.X) / sc.h, rtol=0, atol=1e-12 * np.abs(v).max())
+
+
+def test_repeat_is_bitwise_deterministic():
+    sc = scenes.small_bend()
+    say("[determinism] two identical solves")
+    outs = []
+    for _ in range(2):
+        d = driver.TetAnimFix(sc)
+        it, st = d.advance_one_time_step()
+        X, _ = d.state()
+        outs.append((it, X.copy(), st["pcg_iters"]))
+    assert outs[0][0] == outs[1][0] and outs[0][2] == outs[1][2]
+    assert np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_snapshot_restore():
+    sc = scenes.small_bend()
+    d = driver.TetAnimFix(sc)
+    d.ctx.snapshot()
+    k0 = d.kappa.copy()
+    say("[snapshot] solve, restore, solve")
+    d.advance_one_time_step()
+    X1, _ = d.state()
+    d.ctx.restore()
+    d.kappa[:] = k0
+    X0, v0 = d.state()
+    assert np.array_equal(X0, sc.X)
+    d.advance_one_time_step()
+    X2, _ = d.state()
+    assert np.array_equal(X1, X2)
