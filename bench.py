@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Throughput of the Newton-with-barrier step on one MI355X.
+
+One step is one complete ``Advance_One_Step_IE_Discrete_Shell`` solve
+(Library/FEM/Shell/IMPLICIT_EULER.h:152-891) of the animation-fix workload:
+the 64x16x16-cell Kuhn bar (98,304 tets, 18,785 vertices, seeded rest jitter)
+driven towards two twists plus a 180-degree fold, FP64, dhat = 1e-3 and
+Newton tolerance 1e-4 of the bounding-box diagonal.  Every timed solve starts
+from the same device-resident state (restored device-to-device), so the
+inputs are in HBM when the clock starts.
+
+Printed: one JSON line with solves/s, p50/p99 solve latency, the roofline
+position of the dominant kernel (device time from HIP events recorded on the
+library's stream around each launch, algorithmic bytes or flops from the
+kernel's work model) and the oracle's time on this host's cores for a bounded
+sample of the same workload.
+
+``--max-newton K`` truncates each solve to K Newton iterations: used only for
+profiler runs (the JSON line then names the truncation).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK = 8.0e12        # B/s, MI355X HBM3E
+FP64_VEC_PEAK = 78.6e12  # FLOP/s, MI355X FP64
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(name):
+    from idp_amd import scenes
+
+    if name == "bar_twist_fold":
+        return scenes.config2()
+    if name == "bar_bend":
+        return scenes.config1()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def read_traffic(path):
+    """HBM bytes per PCG iteration from a committed counter summary, or None."""
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"hbm_bytes_per_pcg_iteration:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(sc, gpu_newton, seconds, rtol):
+    """The oracle's Newton loop on this host's cores for a bounded sample.
+
+    Runs the first M Newton iterations of the same solve (M grown until about
+    `seconds` of CPU time), takes seconds per Newton iteration, and scales to
+    one solve with the GPU solve's Newton-iteration count.  Early iterations
+    carry fewer contacts than late ones, so the estimate favours the CPU.
+    """
+    from oracle import pyoracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    m = pyoracle.Mesh(sc.X, sc.tets, sc.density, sc.E, sc.nu)
+    dHat2, kappa = pyoracle.initialize_oipc(sc.dhat, m.mass)
+    M, t = 1, 0.0
+    while True:
+        t0 = time.perf_counter()
+        it, *_ = m.advance(sc.X, sc.velocity, sc.h, sc.newton_tol, dHat2, kappa, lin_rtol=rtol, max_newton=M)
+        t = time.perf_counter() - t0
+        if t >= seconds / 4 or it < M or M >= 64:
+            break
+        M *= 2
+    per_iter = t / max(it, 1)
+    return {
+        "value": 1.0 / (per_iter * gpu_newton),
+        "unit": "solves/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"oracle (C++/OpenMP restatement) on the same scene: first {it} Newton iterations in "
+                  f"{t:.1f} s ({per_iter:.3f} s/iteration), scaled by the GPU solve's {gpu_newton} Newton iterations",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--workload", default="bar_twist_fold", choices=["bar_twist_fold", "bar_bend"])
+    ap.add_argument("--pcg-rtol", type=float, default=1e-8)
+    ap.add_argument("--max-newton", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic", default=os.path.join(HERE, "profiles", "pcg_traffic_bar_twist_fold.txt"))
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
+              "the sharded layout is described in the design notes, not launched", file=sys.stderr)
+        sys.exit(2)
+
+    from idp_amd import capi, driver
+
+    sc = workload(a.workload)
+    log(f"[bench] workload {sc.name}: {sc.X.shape[0]} vertices, {sc.tets.shape[0]} tets")
+    d = driver.TetAnimFix(sc, pcg_rtol=a.pcg_rtol)
+    k0 = d.kappa.copy()
+    d.ctx.snapshot()
+
+    def solve(max_newton):
+        d.ctx.restore()
+        d.kappa[:] = k0
+        t0 = time.perf_counter()
+        # returns after the context's stream has drained (stats and kappa are copied back)
+        it, st = d.advance_one_time_step(max_newton=max_newton)
+        return it, st, time.perf_counter() - t0
+
+    # buffer priming (device allocations grow on first use): a 2-iteration solve
+    solve(2)
+    for w in range(a.warmup):
+        it, st, t = solve(a.max_newton)
+        log(f"[bench] warmup {w}: {it} Newton iterations, {t:.2f} s")
+
+    d.ctx.profile(True)
+    lat, its, pcg, pairs = [], [], [], 0
+    t_start = time.perf_counter()
+    for k in range(a.steps):
+        it, st, t = solve(a.max_newton)
+        lat.append(t)
+        its.append(it)
+        pcg.append(st["pcg_iters"])
+        log(f"[bench] step {k}: {it} Newton iterations, {st['pcg_iters']} PCG iterations, "
+            f"{st['n_constraints']} constraints, {t:.2f} s")
+    total = time.perf_counter() - t_start
+    prof = d.ctx.profile_read()
+    d.ctx.profile(False)
+
+    # dominant kernel: largest device time among the families with a work model
+    fams = {k: prof[k] for k in ("pcg", "barrier_hess", "tet_hess", "assembly")}
+    dom = max(fams, key=lambda k: fams[k]["ms"])
+    f = fams[dom]
+    launch_s = f["ms"] * 1e-3 / max(f["launches"], 1)
+    per_launch = f["work"] / max(f["launches"], 1)
+    if dom == "barrier_hess" or dom == "tet_hess":
+        roof = {"kernel": dom, "bound": "fp64-vector", "achieved": per_launch / launch_s / 1e12,
+                "peak": FP64_VEC_PEAK / 1e12, "unit": "TFLOP/s"}
+        roof["frac"] = roof["achieved"] / roof["peak"]
+        roof["traffic"] = None
+    else:
+        roof = {"kernel": dom, "bound": "hbm", "achieved": per_launch / launch_s / 1e9, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s"}
+        roof["frac"] = roof["achieved"] / roof["peak"]
+        tpi = read_traffic(a.traffic) if dom == "pcg" else None
+        roof["traffic"] = tpi * sum(pcg) / max(f["launches"], 1) if tpi else None
+    roof["avg_launch_ms"] = launch_s * 1e3
+    roof["algorithmic_per_launch"] = per_launch
+
+    kern = {}
+    for k in capi.K_FAMILIES:
+        e = prof[k]
+        kern[k] = {"ms": round(e["ms"], 3), "launches": e["launches"]}
+    bh = prof["barrier_hess"]
+    if bh["ms"] > 0:
+        kern["barrier_hess"]["tflops"] = bh["work"] / (bh["ms"] * 1e-3) / 1e12
+    for k in ("pcg", "assembly"):
+        if prof[k]["ms"] > 0:
+            kern[k]["gbs"] = prof[k]["work"] / (prof[k]["ms"] * 1e-3) / 1e9
+    pairs = int(sum(prof["barrier_pairs"]))
+
+    base = None
+    if not a.no_cpu_baseline:
+        log(f"[bench] cpu baseline (oracle, ~{a.cpu_seconds:.0f} s)")
+        base = cpu_baseline(sc, int(np.median(its)), a.cpu_seconds, a.pcg_rtol)
+
+    lat_ms = np.array(lat) * 1e3
+    out = {
+        "metric": "solves/sec + p50/p99 latency at fixed accuracy; barrier pair-evals/sec vs plain",
+        "value": a.steps / total,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": total * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": sc.name, "vertices": int(sc.X.shape[0]), "tets": int(sc.tets.shape[0]),
+                   "pcg_rtol": a.pcg_rtol, "newton_iters": its, "pcg_iters": pcg,
+                   "max_newton": a.max_newton or None},
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+        "barrier_hessian_pair_evals_per_s": pairs / total,
+        "roofline": roof,
+        "kernels": kern,
+        "cpu_baseline": base,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -40,7 +40,10 @@ EXPORTED = [
     "idp_restore_state", "idp_surface_sizes", "idp_advance_one_step", "idp_constraint_set", "idp_get_constraints",
     "idp_barrier_energy", "idp_barrier_gradient", "idp_barrier_hessian_local", "idp_min_dist2", "idp_ccd_stepsize",
     "idp_elastic_energy", "idp_elastic_gradient", "idp_elastic_hessian_local", "idp_newton_direction", "idp_get_bsr",
+    "idp_profile_enable", "idp_profile_read",
 ]
+
+K_FAMILIES = ["pcg", "barrier_hess", "tet_hess", "assembly", "constraint_set", "ccd", "energy", "gradient"]
 
 
 class IdpError(RuntimeError):
@@ -50,7 +53,7 @@ class IdpError(RuntimeError):
 
 
 class SolverOpts(C.Structure):
-    _fields_ = [("pcg_rtol", C.c_double), ("pcg_maxit", C.c_int), ("max_newton", C.c_int)]
+    _fields_ = [("pcg_rtol", C.c_double), ("pcg_maxit", C.c_int), ("max_newton", C.c_int), ("verbose", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -66,6 +69,19 @@ class StepStats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Profile(C.Structure):
+    _fields_ = [("ms", C.c_double * 16), ("launches", C.c_long * 16), ("work", C.c_double * 16),
+                ("barrier_pairs", C.c_long * 8), ("barrier_clamped", C.c_long * 8)]
+
+    def as_dict(self):
+        out = {}
+        for i, name in enumerate(K_FAMILIES):
+            out[name] = dict(ms=self.ms[i], launches=self.launches[i], work=self.work[i])
+        out["barrier_pairs"] = list(self.barrier_pairs)[:7]
+        out["barrier_clamped"] = list(self.barrier_clamped)[:7]
+        return out
 
 
 _lib = None
@@ -111,6 +127,8 @@ def lib():
         L.idp_newton_direction.argtypes = [vp, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double,
                                            C.POINTER(SolverOpts), dp, dp, dp, ip]
         L.idp_get_bsr.argtypes = [vp, C.POINTER(C.c_long), ip, ip, dp, C.c_long]
+        L.idp_profile_enable.argtypes = [vp, C.c_int]
+        L.idp_profile_read.argtypes = [vp, C.POINTER(Profile)]
         _lib = L
     return _lib
 
@@ -238,6 +256,14 @@ class Context:
         self._check(lib().idp_get_state(self._h, Xp, vp))
         return X.reshape(-1, 3), v.reshape(-1, 3)
 
+    def profile(self, on=True):
+        self._check(lib().idp_profile_enable(self._h, int(on)))
+
+    def profile_read(self):
+        p = Profile()
+        self._check(lib().idp_profile_read(self._h, C.byref(p)))
+        return p.as_dict()
+
     def snapshot(self):
         self._check(lib().idp_snapshot_state(self._h))
 
@@ -245,12 +271,12 @@ class Context:
         self._check(lib().idp_restore_state(self._h))
 
     def advance(self, h, tol, dHat2, kappa, thickness=0.0, with_collision=True, pcg_rtol=1e-8, pcg_maxit=10000,
-                max_newton=0):
+                max_newton=0, verbose=0):
         """Advance_One_Step_IE_Discrete_Shell; kappa (3,) is updated in place."""
         kappa_arr = np.asarray(kappa, dtype=np.float64)
         if not kappa_arr.flags["C_CONTIGUOUS"]:
             raise ValueError("kappa must be contiguous float64")
-        opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton)
+        opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, verbose)
         st = StepStats()
         it = self._check(lib().idp_advance_one_step(self._h, h, tol, int(with_collision), dHat2,
                                                     kappa_arr.ctypes.data_as(dp), thickness, C.byref(opts),
@@ -333,7 +359,7 @@ class Context:
         sol, sp = _out_d(X.size)
         rel = C.c_double()
         it = C.c_int()
-        opts = SolverOpts(pcg_rtol, pcg_maxit, 0)
+        opts = SolverOpts(pcg_rtol, pcg_maxit, 0, 0)
         self._check(lib().idp_newton_direction(self._h, Xp, xtp, h, dHat2, kappa, thickness, C.byref(opts), gp, sp,
                                                C.byref(rel), C.byref(it)))
         return g, sol, rel.value, it.value

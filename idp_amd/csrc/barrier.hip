@@ -497,7 +497,7 @@ __device__ inline void jacobi(double* A, double* W)
 
 // makePD in the reduced space; H (ld 9, M blocks) is replaced by its projection.
 template <int KIND>
-__device__ inline void project_reduced(double* H)
+__device__ inline bool project_reduced(double* H)
 {
 #pragma clang fp contract(fast)
     constexpr int M = KT<KIND>::M;
@@ -530,7 +530,7 @@ __device__ inline void project_reduced(double* H)
     double lmin = K[0];
 #pragma unroll
     for (int i = 1; i < N; ++i) lmin = fmin(lmin, K[i * 9 + i]);
-    if (lmin >= 0.0) return;  // UTILS.h:12-15: nothing to clamp
+    if (lmin >= 0.0) return false;  // UTILS.h:12-15: nothing to clamp
     double lam[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) lam[i] = fmax(K[i * 9 + i], 0.0);
@@ -564,6 +564,7 @@ __device__ inline void project_reduced(double* H)
                     }
                     H[(3 * a + i) * 9 + 3 * b + j] = acc;
                 }
+    return true;
 }
 
 // vertex gradient g_x = B^T g_y (3*NV values)
@@ -642,7 +643,7 @@ __device__ inline void grad_body(const Decoded& k, const double* X, const double
 
 template <int KIND>
 __device__ inline void hess_body(const Decoded& k, const double* X, const double* X0, double dh, double kappa,
-                                 double th2, int project_spd, int mode, double* out, int* err)
+                                 double th2, int project_spd, int mode, double* out, int* err, int* counts)
 {
     constexpr int NV = KT<KIND>::NV;
     double g[9], H[81], e;
@@ -651,7 +652,12 @@ __device__ inline void hess_body(const Decoded& k, const double* X, const double
         for (int j = 0; j < 144; ++j) out[j] = 0.0;
         return;
     }
-    if (project_spd) project_reduced<KIND>(H);
+    bool clamped = false;
+    if (project_spd) clamped = project_reduced<KIND>(H);
+    if (counts) {
+        atomicAdd(counts + 2 * KIND, 1);
+        if (clamped) atomicAdd(counts + 2 * KIND + 1, 1);
+    }
     double blk[9];
 #pragma unroll
     for (int a = 0; a < NV; ++a)
@@ -703,12 +709,12 @@ __global__ void __launch_bounds__(64) k_barrier_hess(long n, const int4* __restr
                                                      const double* __restrict__ X0, double dh, double kappa,
                                                      double th2, int project_spd, int mode,
                                                      double* __restrict__ out, int* __restrict__ vid,
-                                                     int* __restrict__ err)
+                                                     int* __restrict__ err, int* __restrict__ counts)
 {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Decoded k = decode(cs[i]);
-    IDP_KIND_SWITCH(k.kind, hess_body, k, X, X0, dh, kappa, th2, project_spd, mode, out + 144 * i, err)
+    IDP_KIND_SWITCH(k.kind, hess_body, k, X, X0, dh, kappa, th2, project_spd, mode, out + 144 * i, err, counts)
     if (vid)
         for (int j = 0; j < 4; ++j) vid[4 * i + j] = (j < k.nv) ? k.v[j] : -1;
 }
@@ -746,10 +752,11 @@ void launch_barrier_grad(hipStream_t s, long n, const int* cs, const double* X, 
 }
 
 void launch_barrier_hess(hipStream_t s, long n, const int* cs, const double* X, const double* X0, double dh,
-                         double kappa, double th2, int project_spd, int mode, double* out, int* vid, int* err)
+                         double kappa, double th2, int project_spd, int mode, double* out, int* vid, int* err,
+                         int* counts)
 {
     IDP_LAUNCH(k_barrier_hess, grid_for(n, 64), 64, 0, s, n, (const int4*)cs, X, X0, dh, kappa, th2, project_spd,
-               mode, out, vid, err);
+               mode, out, vid, err, counts);
 }
 
 void launch_constraint_vids(hipStream_t s, long n, const int* cs, int* vid, int* nv)

@@ -40,6 +40,8 @@ struct idp_ctx {
     long ncs_prev = 0;
     double snap_kappa = 0;
     bool snap_valid = false;
+    Profiler prof;
+    DBuf<int> bcounts;
 };
 
 namespace {
@@ -168,6 +170,7 @@ EnergyOut energy(idp_ctx* c, const double* X, bool with_barrier, double h, doubl
     auto& m = c->m;
     int* fl = c->flags.ensure(4, s);
     IDP_HIP(hipMemsetAsync(fl, 0, 2 * sizeof(int), s));
+    const int tok = c->prof.begin(IDP_K_ENERGY);
     int nbt = tet_energy_blocks(m.nT), nbi = inertia_energy_blocks(m.nV);
     double* pe = c->part_e.ensure(nbt + 1, s);
     double* pi = c->part_i.ensure(nbi + 1, s);
@@ -183,6 +186,7 @@ EnergyOut energy(idp_ctx* c, const double* X, bool with_barrier, double h, doubl
         double dh = dHat2 + 2 * std::sqrt(dHat2) * thickness;
         launch_barrier_energy(s, ncs, (const int*)c->cw.cs.p, X, m.X0.p, dh, kappa, th2, pb, fl + 1);
     }
+    c->prof.end(IDP_K_ENERGY, tok);
     int hf[2];
     IDP_HIP(hipMemcpyAsync(hf, fl, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
     IDP_HIP(hipStreamSynchronize(s));
@@ -220,6 +224,7 @@ void gradient(idp_ctx* c, const double* X, double h, double dHat2, double kappa,
     auto s = c->s;
     auto& m = c->m;
     double* gl = c->gl.ensure(12 * (size_t)m.nT, s);
+    const int tok = c->prof.begin(IDP_K_GRADIENT);
     launch_tet_grad(s, m, X, h * h, gl);
     const double* gb = nullptr;
     long ncs = c->cw.ncs;
@@ -232,6 +237,7 @@ void gradient(idp_ctx* c, const double* X, double h, double dHat2, double kappa,
         gb = b;
     }
     contact_gradient_gather(c->L, s, m, X, c->xt.p, gl, gb, g, rhs);
+    c->prof.end(IDP_K_GRADIENT, tok);
 }
 
 void hessian(idp_ctx* c, const double* X, double h, double dHat2, double kappa, double thickness, bool with_barrier)
@@ -239,14 +245,19 @@ void hessian(idp_ctx* c, const double* X, double h, double dHat2, double kappa, 
     auto s = c->s;
     auto& m = c->m;
     double* Hl = c->Hl.ensure(144 * (size_t)m.nT, s);
+    int tok = c->prof.begin(IDP_K_TET_HESS);
     launch_tet_hess(s, m, X, h * h, 1, Hl);
+    c->prof.end(IDP_K_TET_HESS, tok, 6500.0 * m.nT);
     const double* Hb = nullptr;
     long ncs = c->cw.ncs;
     if (with_barrier && ncs) {
         int* fl = c->flags.ensure(4, s);
         double th2 = thickness * thickness, dh = dHat2 + 2 * std::sqrt(dHat2) * thickness;
         double* b = c->Hb.ensure(144 * (size_t)ncs, s);
-        launch_barrier_hess(s, ncs, (const int*)c->cw.cs.p, X, m.X0.p, dh, kappa, th2, 1, 1, b, nullptr, fl + 3);
+        tok = c->prof.begin(IDP_K_BARRIER_HESS);
+        launch_barrier_hess(s, ncs, (const int*)c->cw.cs.p, X, m.X0.p, dh, kappa, th2, 1, 1, b, nullptr, fl + 3,
+                            c->prof.on ? c->bcounts.p : nullptr);
+        c->prof.end(IDP_K_BARRIER_HESS, tok);
         Hb = b;
     }
     assemble(c->L, s, m, Hl, Hb, c->ndbc > 0);
@@ -294,6 +305,9 @@ int idp_ctx_create(int device, idp_ctx** out)
         IDP_HIP(hipSetDevice(device));
         IDP_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
         barrier_init_constants();
+        c->prof.s = c->s;
+        c->cw.prof = &c->prof;
+        c->L.prof = &c->prof;
     } catch (const HipError& e) {
         delete c;
         return e.code;
@@ -315,6 +329,56 @@ int idp_ctx_destroy(idp_ctx* c)
 }
 
 const char* idp_last_error(const idp_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int idp_profile_enable(idp_ctx* c, int on)
+{
+    return guarded(c, [&]() -> int {
+        c->prof.reset();
+        c->prof.on = on != 0;
+        int* b = c->bcounts.ensure(16, c->s);
+        IDP_HIP(hipMemsetAsync(b, 0, 16 * sizeof(int), c->s));
+        IDP_HIP(hipStreamSynchronize(c->s));
+        return IDP_OK;
+    });
+}
+
+// algorithmic flops of one barrier Hessian evaluation: derivative terms, the
+// n^2 outer products, a 9n^3 symmetric eigendecomposition and, when clamped,
+// a 2n^3 reconstruction (n = 12 for PT/EE and mollified kinds, 9 PE, 6 PP)
+static double barrier_hess_flops(int kind, bool clamped)
+{
+    static const double fdh[7] = {1500, 1200, 450, 60, 1200, 1200, 1200};
+    static const int nn[7] = {12, 12, 9, 6, 12, 12, 12};
+    const double n = nn[kind];
+    return fdh[kind] + 3 * n * n + 9 * n * n * n + (clamped ? 2 * n * n * n : 0.0);
+}
+
+int idp_profile_read(idp_ctx* c, idp_profile* out)
+{
+    return guarded(c, [&]() -> int {
+        if (!out) return IDP_E_INVALID;
+        c->prof.collect();
+        int cnt[16] = {0};
+        if (c->bcounts.p) {
+            IDP_HIP(hipMemcpyAsync(cnt, c->bcounts.p, sizeof(cnt), hipMemcpyDeviceToHost, c->s));
+            IDP_HIP(hipStreamSynchronize(c->s));
+        }
+        std::memset(out, 0, sizeof(*out));
+        for (int i = 0; i < IDP_K_COUNT; ++i) {
+            out->ms[i] = c->prof.ms[i];
+            out->launches[i] = c->prof.launches[i];
+            out->work[i] = c->prof.work[i];
+        }
+        double fl = 0;
+        for (int k = 0; k < 7; ++k) {
+            out->barrier_pairs[k] = cnt[2 * k];
+            out->barrier_clamped[k] = cnt[2 * k + 1];
+            fl += (cnt[2 * k] - cnt[2 * k + 1]) * barrier_hess_flops(k, false) + cnt[2 * k + 1] * barrier_hess_flops(k, true);
+        }
+        out->work[IDP_K_BARRIER_HESS] = fl;
+        return IDP_OK;
+    });
+}
 
 int idp_fem_init(int nV, const double* X, int nT, const int* tets, double density, double E, double nu,
                  double* vol, double* IB, double* lambda, double* mu, double* mass)
@@ -567,6 +631,7 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
         const double rtol = (opts && opts->pcg_rtol > 0) ? opts->pcg_rtol : 1e-8;
         const int maxit = (opts && opts->pcg_maxit > 0) ? opts->pcg_maxit : 10000;
         const int max_newton = opts ? opts->max_newton : 0;
+        const int verbose = opts ? opts->verbose : 0;
         auto s = c->s;
         MeshDev& m = c->m;
         const long n3 = 3L * m.nV;
@@ -585,7 +650,9 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
         auto t0 = now_ms();
         auto cset = [&](const double* X) {
             double ta = now_ms();
+            const int tk = c->prof.begin(IDP_K_CONSTRAINT_SET);
             contact_constraint_set(c->cw, s, m, X, dHat2, thickness);
+            c->prof.end(IDP_K_CONSTRAINT_SET, tk);
             S.constraint_sets++;
             S.ms_constraint_set += now_ms() - ta;
         };
@@ -617,7 +684,7 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
                 S.ms_hessian += now_ms() - t0;
                 t0 = now_ms();
                 double rel = 0;
-                int it = pcg_solve(c->L, s, m, rhs, sol, rtol, maxit, &rel, 8);
+                int it = pcg_solve_persistent(c->L, s, m, rhs, sol, rtol, maxit, &rel);
                 S.ms_solve += now_ms() - t0;
                 if (it < 0) {
                     useGD = true;  // DIRECT_SOLVER.h:42-44 -> gradient descent (IMPLICIT_EULER.h:531-551)
@@ -732,6 +799,10 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
             if (useGD) L2 = newton_tol * 10;
             useGD = (alpha * 2 < 1e-8 && feasibleAlpha > 1e-8);
             S.newton_iters = PNIter;
+            if (verbose > 0 && PNIter % verbose == 0)
+                std::fprintf(stderr, "[idp] newton %d res %.3e tol %.3e alpha %.3e feasible %.3e ncs %ld E %.6e kappa %.3e t %.1f ms\n",
+                             PNIter, L2, newton_tol, alpha * 2, feasibleAlpha, c->cw.ncs, Eprev, kappa,
+                             now_ms() - t_start);
             S.last_res = L2;
             S.E = Eprev;
             if (max_newton > 0 && PNIter >= max_newton) break;
@@ -742,6 +813,7 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
         IDP_HIP(hipStreamSynchronize(s));
         S.n_constraints = c->cw.ncs;
         S.kappa = kappa;
+        c->prof.collect();
         S.ms_total = now_ms() - t_start;
         if (st) *st = S;
         return PNIter;
@@ -954,7 +1026,7 @@ int idp_newton_direction(idp_ctx* c, const double* X, const double* xtilde, doub
         double rtol = (opts && opts->pcg_rtol > 0) ? opts->pcg_rtol : 1e-8;
         int maxit = (opts && opts->pcg_maxit > 0) ? opts->pcg_maxit : 10000;
         double rel = 0;
-        int it = pcg_solve(c->L, s, m, rhs, sd, rtol, maxit, &rel, 8);
+        int it = pcg_solve_persistent(c->L, s, m, rhs, sd, rtol, maxit, &rel);
         if (g) c->g.download(g, n3, s);
         if (sol) c->sol.download(sol, n3, s);
         IDP_HIP(hipStreamSynchronize(s));

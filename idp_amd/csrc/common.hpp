@@ -80,6 +80,78 @@ struct DBuf {
 };
 
 // ---------------------------------------------------------------------------
+// Optional per-family device timing with HIP events on one stream.
+// ---------------------------------------------------------------------------
+struct Profiler {
+    bool on = false;
+    hipStream_t s = nullptr;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<int, std::pair<int, int>>> open;  // family, (start, end) event ids
+    size_t next = 0;
+    double ms[IDP_K_COUNT] = {0};
+    long launches[IDP_K_COUNT] = {0};
+    double work[IDP_K_COUNT] = {0};
+    long pairs[8] = {0}, clamped[8] = {0};
+    int cur_start = -1;
+    ~Profiler()
+    {
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+    int ev()
+    {
+        if (next == pool.size()) {
+            hipEvent_t e;
+            IDP_HIP(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        return (int)next++;
+    }
+    void reset()
+    {
+        collect();
+        for (int i = 0; i < IDP_K_COUNT; ++i) ms[i] = work[i] = 0, launches[i] = 0;
+        for (int i = 0; i < 8; ++i) pairs[i] = clamped[i] = 0;
+    }
+    // returns a token for end()
+    int begin(int fam)
+    {
+        if (!on) return -1;
+        int a = ev();
+        IDP_HIP(hipEventRecord(pool[a], s));
+        return a;
+    }
+    void end(int fam, int tok, double w = 0.0)
+    {
+        if (!on || tok < 0) return;
+        int b = ev();
+        IDP_HIP(hipEventRecord(pool[b], s));
+        open.push_back({fam, {tok, b}});
+        launches[fam] += 1;
+        work[fam] += w;
+        if (open.size() > 4096) collect();
+    }
+    void add_work(int fam, double w)
+    {
+        if (on) work[fam] += w;
+    }
+    void collect()
+    {
+        if (open.empty()) {
+            next = 0;
+            return;
+        }
+        IDP_HIP(hipStreamSynchronize(s));
+        for (auto& o : open) {
+            float t = 0.f;
+            IDP_HIP(hipEventElapsedTime(&t, pool[o.second.first], pool[o.second.second]));
+            ms[o.first] += t;
+        }
+        open.clear();
+        next = 0;
+    }
+};
+
+// ---------------------------------------------------------------------------
 // Deterministic reductions.  Every sum is taken in a fixed order for a fixed
 // problem size: block b reduces a fixed index range through a fixed LDS tree,
 // and a single block then reduces the per-block partials the same way.  No

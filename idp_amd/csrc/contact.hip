@@ -896,7 +896,9 @@ double contact_ccd(ContactWork& w, hipStream_t s, const MeshDev& m, const double
     unsigned long long abits;
     std::memcpy(&abits, &alpha, sizeof(abits));
     IDP_HIP(hipMemcpyAsync(amin, &abits, sizeof(abits), hipMemcpyHostToDevice, s));
+    const int tok = w.prof ? w.prof->begin(IDP_K_CCD) : -1;
     IDP_LAUNCH(k_accd, grid_for(npt + nee, 64), 64, 0, s, npt, nee, cand, A, 0.1, amin);
+    if (w.prof) w.prof->end(IDP_K_CCD, tok, (double)npt + nee);
     IDP_HIP(hipMemcpyAsync(&abits, amin, sizeof(abits), hipMemcpyDeviceToHost, s));
     IDP_HIP(hipStreamSynchronize(s));
     std::memcpy(&alpha, &abits, sizeof(alpha));

@@ -48,6 +48,7 @@ struct ContactWork {
     DBuf<unsigned long long> dk0, dk1;
     DBuf<int2> cand;
     long ncs = 0;
+    Profiler* prof = nullptr;
 };
 
 // barrier.hip
@@ -60,7 +61,8 @@ void launch_min_dist2(hipStream_t s, long n, const int* cs, const double* X, dou
 void launch_barrier_grad(hipStream_t s, long n, const int* cs, const double* X, const double* X0, double dh,
                          double kappa, double th2, double* gout, int* err);
 void launch_barrier_hess(hipStream_t s, long n, const int* cs, const double* X, const double* X0, double dh,
-                         double kappa, double th2, int project_spd, int mode, double* out, int* vid, int* err);
+                         double kappa, double th2, int project_spd, int mode, double* out, int* vid, int* err,
+                         int* counts = nullptr);
 void launch_constraint_vids(hipStream_t s, long n, const int* cs, int* vid, int* nv);
 
 // contact.hip

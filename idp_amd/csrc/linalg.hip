@@ -15,6 +15,7 @@
 #include "kernels.hpp"
 #include "linalg.hpp"
 
+#include <hip/hip_cooperative_groups.h>
 #include <hipcub/hipcub.hpp>
 
 namespace idp {
@@ -387,6 +388,169 @@ __global__ void __launch_bounds__(RB) k_pcg_init_scalars(int npart, const double
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Persistent PCG: the whole iteration loop in one cooperative launch.
+// Two grid barriers per iteration: the SpMV of the new direction reads
+// p_new_j = z_j + beta p_old_j of neighbouring rows on the fly (p ping-pongs
+// between two buffers), so q = A p_new is exact and no residual drift builds
+// up.  Every block re-reduces the per-block partials in the same order, so
+// all blocks take identical, deterministic decisions.
+// ---------------------------------------------------------------------------
+struct PcgArgs {
+    int nV;
+    const int* erp;
+    const int* ecol;
+    const double* eval;
+    const int* crp;  // null when there are no contact blocks
+    const int* ccol;
+    const double* cval;
+    const double* b;
+    const double* Pinv;
+    double* x;
+    double* r;
+    double* z;
+    double* p0;
+    double* p1;
+    double* q;
+    double* part_a;
+    double* part_b;
+    double* part_c;
+    double* out;  // [0] iterations, [1] |r|^2, [2] |b|^2, [3] breakdown
+    double rtol2;
+    int maxit;
+};
+
+__device__ inline void row_spmv_dir(int v, const int* __restrict__ rp, const int* __restrict__ col,
+                                    const double* __restrict__ val, const double* __restrict__ z,
+                                    const double* __restrict__ po, double beta, double& y0, double& y1, double& y2)
+{
+    for (int k = rp[v]; k < rp[v + 1]; ++k) {
+        const double* bk = val + 9 * (size_t)k;
+        const size_t c = 3 * (size_t)col[k];
+        const double x0 = z[c] + beta * po[c], x1 = z[c + 1] + beta * po[c + 1], x2 = z[c + 2] + beta * po[c + 2];
+        y0 += bk[0] * x0 + bk[1] * x1 + bk[2] * x2;
+        y1 += bk[3] * x0 + bk[4] * x1 + bk[5] * x2;
+        y2 += bk[6] * x0 + bk[7] * x1 + bk[8] * x2;
+    }
+}
+
+__global__ void __launch_bounds__(RB) k_pcg_persistent(PcgArgs A)
+{
+    namespace cgr = cooperative_groups;
+    cgr::grid_group grid = cgr::this_grid();
+    __shared__ double sh[RB];
+    const int nb = gridDim.x;
+    const int stride = nb * RB;
+    // init: x = 0, r = b, z = P r, p_old = 0 (so p = z with beta = 0)
+    double drz = 0.0, drr = 0.0;
+    for (int v = blockIdx.x * RB + threadIdx.x; v < A.nV; v += stride) {
+        double rv[3] = {A.b[3 * v], A.b[3 * v + 1], A.b[3 * v + 2]};
+        const double* P = A.Pinv + 9 * (size_t)v;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            A.x[3 * v + i] = 0.0;
+            A.r[3 * v + i] = rv[i];
+            A.p0[3 * v + i] = 0.0;
+            double zi = P[3 * i] * rv[0] + P[3 * i + 1] * rv[1] + P[3 * i + 2] * rv[2];
+            A.z[3 * v + i] = zi;
+            drz += rv[i] * zi;
+            drr += rv[i] * rv[i];
+        }
+    }
+    {
+        double a = block_sum<RB>(drz, sh), c = block_sum<RB>(drr, sh);
+        if (threadIdx.x == 0) {
+            A.part_b[blockIdx.x] = a;
+            A.part_c[blockIdx.x] = c;
+        }
+    }
+    grid.sync();
+    double rz = sum_partials_block(A.part_b, nb, sh);
+    const double bb = sum_partials_block(A.part_c, nb, sh);
+    const double stop2 = A.rtol2 * bb;
+    double beta = 0.0, rr = bb;
+    int it = 0, status = 0;
+    double* po = A.p0;
+    double* pn = A.p1;
+    if (bb > 0.0) {
+        for (; it < A.maxit; ++it) {
+            // phase 1: p_new = z + beta p_old ; q = A p_new ; partial p.q
+            double dpq = 0.0;
+            for (int v = blockIdx.x * RB + threadIdx.x; v < A.nV; v += stride) {
+                double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+                row_spmv_dir(v, A.erp, A.ecol, A.eval, A.z, po, beta, y0, y1, y2);
+                if (A.crp) row_spmv_dir(v, A.crp, A.ccol, A.cval, A.z, po, beta, y0, y1, y2);
+                const double p0v = A.z[3 * v] + beta * po[3 * v], p1v = A.z[3 * v + 1] + beta * po[3 * v + 1],
+                             p2v = A.z[3 * v + 2] + beta * po[3 * v + 2];
+                pn[3 * v] = p0v;
+                pn[3 * v + 1] = p1v;
+                pn[3 * v + 2] = p2v;
+                A.q[3 * v] = y0;
+                A.q[3 * v + 1] = y1;
+                A.q[3 * v + 2] = y2;
+                dpq += p0v * y0 + p1v * y1 + p2v * y2;
+            }
+            {
+                double a = block_sum<RB>(dpq, sh);
+                if (threadIdx.x == 0) A.part_a[blockIdx.x] = a;
+            }
+            grid.sync();
+            const double pq = sum_partials_block(A.part_a, nb, sh);
+            if (!(pq > 0.0)) {
+                status = 1;
+                break;
+            }
+            const double alpha = rz / pq;
+            // phase 2: x += alpha p; r -= alpha q; z = P r; partials r.z, r.r
+            drz = 0.0;
+            drr = 0.0;
+            for (int v = blockIdx.x * RB + threadIdx.x; v < A.nV; v += stride) {
+                double rv[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    A.x[3 * v + i] += alpha * pn[3 * v + i];
+                    rv[i] = A.r[3 * v + i] - alpha * A.q[3 * v + i];
+                    A.r[3 * v + i] = rv[i];
+                }
+                const double* P = A.Pinv + 9 * (size_t)v;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    double zi = P[3 * i] * rv[0] + P[3 * i + 1] * rv[1] + P[3 * i + 2] * rv[2];
+                    A.z[3 * v + i] = zi;
+                    drz += rv[i] * zi;
+                    drr += rv[i] * rv[i];
+                }
+            }
+            {
+                double a = block_sum<RB>(drz, sh), c = block_sum<RB>(drr, sh);
+                if (threadIdx.x == 0) {
+                    A.part_b[blockIdx.x] = a;
+                    A.part_c[blockIdx.x] = c;
+                }
+            }
+            grid.sync();
+            const double rzn = sum_partials_block(A.part_b, nb, sh);
+            rr = sum_partials_block(A.part_c, nb, sh);
+            beta = rzn / rz;
+            rz = rzn;
+            double* t = po;
+            po = pn;
+            pn = t;
+            if (rr <= stop2) {
+                ++it;
+                break;
+            }
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        A.out[0] = (double)it;
+        A.out[1] = rr;
+        A.out[2] = bb;
+        A.out[3] = (double)status;
+    }
+}
+
 // generic deterministic sum of partials -> out[0]
 __global__ void __launch_bounds__(RB) k_sum_partials(int npart, const double* __restrict__ part,
                                                      double* __restrict__ out)
@@ -511,6 +675,9 @@ void contact_gradient_gather(LinWork& L, hipStream_t s, const MeshDev& m, const 
 
 void assemble(LinWork& L, hipStream_t s, const MeshDev& m, const double* Hl, const double* Hb, bool any_dbc)
 {
+    // algorithmic bytes: 80 per contribution (9 doubles + key), 76 per block, rowptr
+    const double ncontrib = 16.0 * m.nT + (double)L.nrec + m.nV;
+    const int tok = L.prof ? L.prof->begin(IDP_K_ASSEMBLY) : -1;
     double* ev = L.evals.ensure(9 * (size_t)m.nnzb_e, s);
     IDP_LAUNCH(k_elastic_values, grid_for(m.nnzb_e, 256), 256, 0, s, m.nnzb_e, L.erow.p, m.e_col.p, m.e_src_ptr.p,
                m.e_src.p, Hl, m.mass.p, ev);
@@ -524,6 +691,8 @@ void assemble(LinWork& L, hipStream_t s, const MeshDev& m, const double* Hl, con
             IDP_LAUNCH(k_project_dbc, grid_for(L.ncb, 256), 256, 0, s, (long)L.ncb, L.crow.p, L.ccol.p, m.dbc.p, 0,
                        L.cvals.p);
     }
+    if (L.prof)
+        L.prof->end(IDP_K_ASSEMBLY, tok, 80.0 * ncontrib + 76.0 * ((double)m.nnzb_e + L.ncb) + 4.0 * (m.nV + 1));
 }
 
 void init_erow(LinWork& L, hipStream_t s, const MeshDev& m, const std::vector<int>& rowof)
@@ -584,6 +753,69 @@ int pcg_solve(LinWork& L, hipStream_t s, const MeshDev& m, const double* b, doub
     IDP_HIP(hipStreamSynchronize(s));
     *relres = std::sqrt(hs[2] / bn2);
     return it;
+}
+
+
+int pcg_solve_persistent(LinWork& L, hipStream_t s, const MeshDev& m, const double* b, double* x, double rtol,
+                         int maxit, double* relres)
+{
+    const int nV = m.nV;
+    static int max_blocks = -1;
+    if (max_blocks < 0) {
+        int dev = 0, ncu = 0, per = 0;
+        IDP_HIP(hipGetDevice(&dev));
+        IDP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        IDP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pcg_persistent, RB, 0));
+        max_blocks = std::max(1, ncu * std::max(per, 1));
+    }
+    const int nblk = std::max(1, std::min(max_blocks, grid_for(nV, RB)));
+    double* Pinv = L.pinv.ensure(9 * (size_t)nV, s);
+    int* err = L.err.ensure(1, s);
+    IDP_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
+    const bool haveC = L.ncb > 0;
+    IDP_LAUNCH(k_block_jacobi, grid_for(nV, 256), 256, 0, s, nV, m.e_diag.p, L.evals.p,
+               haveC ? L.cdiag.p : (const int*)nullptr, L.cvals.p, Pinv, err);
+    PcgArgs A;
+    A.nV = nV;
+    A.erp = m.e_rowptr.p;
+    A.ecol = m.e_col.p;
+    A.eval = L.evals.p;
+    A.crp = haveC ? L.crp.p : nullptr;
+    A.ccol = L.ccol.p;
+    A.cval = L.cvals.p;
+    A.b = b;
+    A.Pinv = Pinv;
+    A.x = x;
+    A.r = L.r.ensure(3 * (size_t)nV, s);
+    A.z = L.z.ensure(3 * (size_t)nV, s);
+    A.p0 = L.p.ensure(3 * (size_t)nV, s);
+    A.p1 = L.p1.ensure(3 * (size_t)nV, s);
+    A.q = L.q.ensure(3 * (size_t)nV, s);
+    A.part_a = L.part0.ensure(nblk, s);
+    A.part_b = L.part1.ensure(nblk, s);
+    A.part_c = L.part2.ensure(nblk, s);
+    A.out = L.scal.ensure(8, s);
+    A.rtol2 = rtol * rtol;
+    A.maxit = maxit;
+    void* args[] = {&A};
+    const int tok = L.prof ? L.prof->begin(IDP_K_PCG) : -1;
+    IDP_HIP(hipLaunchCooperativeKernel((const void*)k_pcg_persistent, dim3(nblk), dim3(RB), args, 0, s));
+    if (L.prof) L.prof->end(IDP_K_PCG, tok);
+    double hs[4];
+    int herr = 0;
+    IDP_HIP(hipMemcpyAsync(hs, A.out, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+    IDP_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
+    IDP_HIP(hipStreamSynchronize(s));
+    if (herr) return -1;
+    if (hs[3] != 0.0) return -2;
+    *relres = hs[2] > 0 ? std::sqrt(hs[1] / hs[2]) : 0.0;
+    // algorithmic bytes per PCG iteration: BSR SpMV (76 nnzb + 4 (V+1) + 48 V),
+    // block-Jacobi apply (72 V + 48 V), two dots and three axpys (8 * 24 V)
+    if (L.prof) {
+        const double nnzb = (double)m.nnzb_e + (double)L.ncb;
+        L.prof->add_work(IDP_K_PCG, hs[0] * (76.0 * nnzb + 364.0 * nV + 4.0));
+    }
+    return (int)hs[0];
 }
 
 double dev_dot(LinWork& L, hipStream_t s, long n, const double* a, const double* b)

@@ -17,8 +17,9 @@ struct LinWork {
     DBuf<int> erow;  // row of every elastic block
     DBuf<double> evals, cvals;
     // PCG
-    DBuf<double> pinv, r, z, p, q, part0, part1, part2, part3, scal, scal2;
+    DBuf<double> pinv, r, z, p, p1, q, part0, part1, part2, part3, scal, scal2;
     DBuf<int> err;
+    Profiler* prof = nullptr;
 };
 
 void contact_structure(LinWork& L, hipStream_t s, int nV, long ncs, const int* cs);
@@ -28,6 +29,8 @@ void assemble(LinWork& L, hipStream_t s, const MeshDev& m, const double* Hl, con
 void init_erow(LinWork& L, hipStream_t s, const MeshDev& m, const std::vector<int>& rowof);
 int pcg_solve(LinWork& L, hipStream_t s, const MeshDev& m, const double* b, double* x, double rtol, int maxit,
               double* relres, int check_every);
+int pcg_solve_persistent(LinWork& L, hipStream_t s, const MeshDev& m, const double* b, double* x, double rtol,
+                         int maxit, double* relres);
 double dev_dot(LinWork& L, hipStream_t s, long n, const double* a, const double* b);
 void dev_velocity(hipStream_t s, long n, const double* x, const double* xn, double h, double* v);
 void dev_axpy(hipStream_t s, long n, const double* a, double sc, const double* b, double* x);

@@ -33,9 +33,9 @@ class TetAnimFix:
         X, _ = self.ctx.get_state()
         self.ctx.set_state(None, (np.asarray(target) - X) / self.h)
 
-    def advance_one_time_step(self, max_newton=0):
+    def advance_one_time_step(self, max_newton=0, verbose=0):
         it, st = self.ctx.advance(self.h, self.tol, self.dHat2, self.kappa, 0.0, True, self.pcg_rtol,
-                                  max_newton=max_newton)
+                                  max_newton=max_newton, verbose=verbose)
         self.pn_iters += it
         return it, st
 

@@ -40,6 +40,7 @@ typedef struct idp_solver_opts {
     double pcg_rtol;  /* relative residual of the Newton-direction solve (default 1e-8) */
     int pcg_maxit;    /* default 10000 */
     int max_newton;   /* 0 = until the reference stopping rule holds */
+    int verbose;      /* > 0: print a progress line to stderr every `verbose` Newton iterations */
 } idp_solver_opts;
 
 typedef struct idp_step_stats {
@@ -56,11 +57,37 @@ typedef struct idp_step_stats {
     double ms_total, ms_constraint_set, ms_ccd, ms_energy, ms_gradient, ms_hessian, ms_solve;
 } idp_step_stats;
 
+/* Kernel-family profile: device time from HIP events recorded on the
+ * context's stream around each launch, plus the algorithmic work of those
+ * launches (bytes for HBM-bound families, flops for compute-bound ones). */
+enum {
+    IDP_K_PCG = 0,          /* persistent block-Jacobi PCG (bytes) */
+    IDP_K_BARRIER_HESS = 1, /* barrier Hessian + PSD projection (flops) */
+    IDP_K_TET_HESS = 2,     /* tet projected 12x12 (flops) */
+    IDP_K_ASSEMBLY = 3,     /* elastic + contact BSR value assembly (bytes) */
+    IDP_K_CONSTRAINT_SET = 4, /* whole contact-set stage (no work count) */
+    IDP_K_CCD = 5,          /* ACCD kernel over candidate pairs (pair count) */
+    IDP_K_ENERGY = 6,       /* tet + inertia + barrier energy kernels */
+    IDP_K_GRADIENT = 7,     /* tet + barrier gradient and vertex gather */
+    IDP_K_COUNT = 16
+};
+typedef struct idp_profile {
+    double ms[IDP_K_COUNT];
+    long launches[IDP_K_COUNT];
+    double work[IDP_K_COUNT];
+    long barrier_pairs[8];   /* Hessian evaluations per constraint kind */
+    long barrier_clamped[8]; /* ... of which needed the PSD clamp */
+} idp_profile;
+
 /* ---- context ---------------------------------------------------------- */
 int idp_ctx_create(int device, idp_ctx** out);
 int idp_ctx_destroy(idp_ctx* ctx);
 const char* idp_last_error(const idp_ctx* ctx);
 int idp_device_count(int* n);
+/* Profiling (no counterpart in the reference, which only has wall-clock
+ * TIMER_FLAG scopes): on != 0 resets the counters and starts recording. */
+int idp_profile_enable(idp_ctx* ctx, int on);
+int idp_profile_read(idp_ctx* ctx, idp_profile* out);
 
 /* ---- host-side set-up helpers (no device work) ------------------------- */
 /* Compute_Vol_And_Inv_Basis (Library/FEM/DEFORMATION_GRADIENT.h:39-76),
