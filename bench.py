@@ -50,11 +50,12 @@ def workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def read_traffic(path):
-    """HBM bytes per PCG iteration from a committed counter summary, or None."""
+def read_traffic(path, kernel):
+    """HBM bytes per dispatch of `kernel` from a committed counter summary
+    (a line "traffic_bytes_per_dispatch <kernel>: <bytes>"), or None."""
     if not os.path.exists(path):
         return None
-    m = re.search(r"hbm_bytes_per_pcg_iteration:\s*([0-9.eE+]+)", open(path).read())
+    m = re.search(r"traffic_bytes_per_dispatch\s+" + re.escape(kernel) + r":\s*([0-9.eE+]+)", open(path).read())
     return float(m.group(1)) if m else None
 
 
@@ -91,6 +92,28 @@ def cpu_baseline(sc, gpu_newton, seconds, rtol):
     }
 
 
+def copy_calibration(mib):
+    """One device-to-device copy of `mib` MiB (the runtime's blit kernel):
+    the known-size reference for scaling FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = int(mib) << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(C.byref(ptr), C.c_size_t(n)) != 0:
+            raise SystemExit("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    rc = hip.hipMemcpy(b, a, C.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    t = time.perf_counter() - t0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": n, "rc": rc, "seconds": t}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,12 +124,17 @@ def main():
     ap.add_argument("--max-newton", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--traffic", default=os.path.join(HERE, "profiles", "pcg_traffic_bar_twist_fold.txt"))
+    ap.add_argument("--traffic", default=os.path.join(HERE, "profiles", "traffic_bar_twist_fold.txt"))
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
               "the sharded layout is described in the design notes, not launched", file=sys.stderr)
         sys.exit(2)
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
 
     from idp_amd import capi, driver
 
@@ -145,13 +173,14 @@ def main():
     d.ctx.profile(False)
 
     # dominant kernel: largest device time among the families with a work model
-    fams = {k: prof[k] for k in ("pcg", "barrier_hess", "tet_hess", "assembly")}
+    fams = {k: prof[k] for k in ("pcg_spmv", "pcg_update", "barrier_hess", "tet_hess", "assembly")}
     dom = max(fams, key=lambda k: fams[k]["ms"])
     f = fams[dom]
     launch_s = f["ms"] * 1e-3 / max(f["launches"], 1)
     per_launch = f["work"] / max(f["launches"], 1)
     if dom == "barrier_hess" or dom == "tet_hess":
-        roof = {"kernel": dom, "bound": "fp64-vector", "achieved": per_launch / launch_s / 1e12,
+        # FP64 compute bound: MI355X's FP64 vector and FP64 matrix peaks are both 78.6 TFLOP/s
+        roof = {"kernel": dom, "bound": "mfma", "achieved": per_launch / launch_s / 1e12,
                 "peak": FP64_VEC_PEAK / 1e12, "unit": "TFLOP/s"}
         roof["frac"] = roof["achieved"] / roof["peak"]
         roof["traffic"] = None
@@ -159,8 +188,7 @@ def main():
         roof = {"kernel": dom, "bound": "hbm", "achieved": per_launch / launch_s / 1e9, "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s"}
         roof["frac"] = roof["achieved"] / roof["peak"]
-        tpi = read_traffic(a.traffic) if dom == "pcg" else None
-        roof["traffic"] = tpi * sum(pcg) / max(f["launches"], 1) if tpi else None
+        roof["traffic"] = read_traffic(a.traffic, dom)
     roof["avg_launch_ms"] = launch_s * 1e3
     roof["algorithmic_per_launch"] = per_launch
 
@@ -171,7 +199,7 @@ def main():
     bh = prof["barrier_hess"]
     if bh["ms"] > 0:
         kern["barrier_hess"]["tflops"] = bh["work"] / (bh["ms"] * 1e-3) / 1e12
-    for k in ("pcg", "assembly"):
+    for k in ("pcg", "pcg_spmv", "pcg_update", "assembly"):
         if prof[k]["ms"] > 0:
             kern[k]["gbs"] = prof[k]["work"] / (prof[k]["ms"] * 1e-3) / 1e9
     pairs = int(sum(prof["barrier_pairs"]))

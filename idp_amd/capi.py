@@ -43,7 +43,8 @@ EXPORTED = [
     "idp_profile_enable", "idp_profile_read",
 ]
 
-K_FAMILIES = ["pcg", "barrier_hess", "tet_hess", "assembly", "constraint_set", "ccd", "energy", "gradient"]
+K_FAMILIES = ["pcg", "barrier_hess", "tet_hess", "assembly", "constraint_set", "ccd", "energy", "gradient",
+              "pcg_spmv", "pcg_update"]
 
 
 class IdpError(RuntimeError):

@@ -12,7 +12,8 @@
 // then H_x = B^T H_y B.  The PSD clamp of H_x is computed exactly in the
 // reduced space: with C = (b b^T)^(1/2) (x) I3, the nonzero spectrum of
 // H_x equals that of K = C H_y C, and makePD(H_x) = B^T C^-1 makePD(K) C^-1 B.
-// K is diagonalised by cyclic Jacobi in registers, one constraint per lane.
+// K is diagonalised by cyclic Jacobi, one constraint per lane (packed K in
+// registers, eigenvectors in LDS).
 #include "common.hpp"
 #include "geom_dev.hpp"
 #include "kernels.hpp"
@@ -244,22 +245,33 @@ template <> struct KT<K_MEE> { static constexpr int M = 3, NV = 4; static conste
 template <> struct KT<K_MPE> { static constexpr int M = 3, NV = 4; static constexpr bool MOLL = true; };
 template <> struct KT<K_MPP> { static constexpr int M = 3, NV = 4; static constexpr bool MOLL = true; };
 
-__device__ inline void skew_add(double* H, int r0, int c0, d3 w, double s)
+// Symmetric matrices are held as packed upper triangles: entry (i, j) of an
+// N x N matrix at sx<N>(i, j).  All loops over entries are fully unrolled, so
+// every index is a compile-time constant and the arrays stay in registers.
+template <int N>
+__device__ constexpr int sx(int i, int j)
 {
-    // H[r0.., c0..] += s * [w]x  ([w]x z = w x z)
-    H[(r0 + 0) * 9 + c0 + 1] += -s * w.z;
-    H[(r0 + 0) * 9 + c0 + 2] += s * w.y;
-    H[(r0 + 1) * 9 + c0 + 0] += s * w.z;
-    H[(r0 + 1) * 9 + c0 + 2] += -s * w.x;
-    H[(r0 + 2) * 9 + c0 + 0] += -s * w.y;
-    H[(r0 + 2) * 9 + c0 + 1] += s * w.x;
+    return (i <= j) ? (i * (2 * N - i + 1)) / 2 + (j - i) : (j * (2 * N - j + 1)) / 2 + (i - j);
 }
 
-// q = |u x v|^2 over y = (a,u,v): gradient and Hessian (a-blocks zero)
-__device__ inline void q_derivs(d3 u, d3 v, double* gq, double* Hq)
+// entry (r, c) of [w]x  ([w]x z = w x z)
+__device__ inline double skew_entry(int r, int c, d3 w)
+{
+    if (r == 0 && c == 1) return -w.z;
+    if (r == 0 && c == 2) return w.y;
+    if (r == 1 && c == 0) return w.z;
+    if (r == 1 && c == 2) return -w.x;
+    if (r == 2 && c == 0) return -w.y;
+    if (r == 2 && c == 1) return w.x;
+    return 0.0;
+}
+
+__device__ inline double comp(d3 w, int i) { return i == 0 ? w.x : (i == 1 ? w.y : w.z); }
+
+// q = |u x v|^2 over y = (a,u,v): gradient (a-block zero)
+__device__ inline void q_grad(d3 u, d3 v, double uu, double vv, double uv, double* gq)
 {
 #pragma clang fp contract(fast)
-    const double uu = sqn(u), vv = sqn(v), uv = dot(u, v);
     gq[0] = gq[1] = gq[2] = 0.0;
     gq[3] = 2 * vv * u.x - 2 * uv * v.x;
     gq[4] = 2 * vv * u.y - 2 * uv * v.y;
@@ -267,101 +279,176 @@ __device__ inline void q_derivs(d3 u, d3 v, double* gq, double* Hq)
     gq[6] = 2 * uu * v.x - 2 * uv * u.x;
     gq[7] = 2 * uu * v.y - 2 * uv * u.y;
     gq[8] = 2 * uu * v.z - 2 * uv * u.z;
-    const double uc[3] = {u.x, u.y, u.z}, vc[3] = {v.x, v.y, v.z};
-#pragma unroll
-    for (int i = 0; i < 81; ++i) Hq[i] = 0.0;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            const double id = (i == j) ? 1.0 : 0.0;
-            Hq[(3 + i) * 9 + 3 + j] = 2 * vv * id - 2 * vc[i] * vc[j];
-            Hq[(6 + i) * 9 + 6 + j] = 2 * uu * id - 2 * uc[i] * uc[j];
-            const double huv = 4 * uc[i] * vc[j] - 2 * vc[i] * uc[j] - 2 * uv * id;
-            Hq[(3 + i) * 9 + 6 + j] = huv;
-            Hq[(6 + j) * 9 + 3 + i] = huv;
-        }
 }
 
-// plane form d = s^2/q, s = a.(u x v), q = |u x v|^2 over y=(a,u,v)
-template <bool WANT_H>
-__device__ inline void plane_derivs(d3 a, d3 u, d3 v, double* gd, double* Hd)
+// entry (i, j) of the Hessian of q (a-blocks zero)
+__device__ inline double q_hess(int i, int j, d3 u, d3 v, double uu, double vv, double uv)
 {
 #pragma clang fp contract(fast)
-    double gq[9], Hq[81];
-    q_derivs(u, v, gq, Hq);
-    const d3 n = cross(u, v);
-    const double s = dot(a, n);
-    const double q = sqn(n);
-    const d3 gsu = cross(v, a), gsv = cross(a, u);
-    const double gs[9] = {n.x, n.y, n.z, gsu.x, gsu.y, gsu.z, gsv.x, gsv.y, gsv.z};
-    const double iq = 1.0 / q;
-    const double c1 = 2 * s * iq;        // d(d)/ds
-    const double c2 = -s * s * iq * iq;  // d(d)/dq
-#pragma unroll
-    for (int i = 0; i < 9; ++i) gd[i] = c1 * gs[i] + c2 * gq[i];
-    if (!WANT_H) return;
-    const double css = 2 * iq;
-    const double csq = -2 * s * iq * iq;
-    const double cqq = 2 * s * s * iq * iq * iq;
-#pragma unroll
-    for (int i = 0; i < 9; ++i)
-#pragma unroll
-        for (int j = 0; j < 9; ++j)
-            Hd[i * 9 + j] = css * gs[i] * gs[j] + csq * (gs[i] * gq[j] + gq[i] * gs[j]) + cqq * gq[i] * gq[j] +
-                            c2 * Hq[i * 9 + j];
-    // + (2s/q) Hess s : blocks [a,u]=-[v]x [u,a]=[v]x [a,v]=[u]x [v,a]=-[u]x [u,v]=-[a]x [v,u]=[a]x
-    skew_add(Hd, 0, 3, v, -c1);
-    skew_add(Hd, 3, 0, v, c1);
-    skew_add(Hd, 0, 6, u, c1);
-    skew_add(Hd, 6, 0, u, -c1);
-    skew_add(Hd, 3, 6, a, -c1);
-    skew_add(Hd, 6, 3, a, c1);
+    const int bi = i / 3, bj = j / 3, ii = i % 3, jj = j % 3;
+    const double id = (ii == jj) ? 1.0 : 0.0;
+    if (bi == 1 && bj == 1) return 2 * vv * id - 2 * comp(v, ii) * comp(v, jj);
+    if (bi == 2 && bj == 2) return 2 * uu * id - 2 * comp(u, ii) * comp(u, jj);
+    if (bi == 1 && bj == 2) return 4 * comp(u, ii) * comp(v, jj) - 2 * comp(v, ii) * comp(u, jj) - 2 * uv * id;
+    if (bi == 2 && bj == 1) return 4 * comp(u, jj) * comp(v, ii) - 2 * comp(v, jj) * comp(u, ii) - 2 * uv * id;
+    return 0.0;
 }
 
-// point-edge form d = |a|^2 - (a.w)^2/|w|^2, blocks IA, IW of a 9-wide layout
-template <bool WANT_H, int IA, int IW>
-__device__ inline void pe_derivs(d3 a, d3 w, double* gd, double* Hd)
-{
+// plane form d = s^2/q, s = a.(u x v), q = |u x v|^2 over y=(a,u,v):
+// gradient on construction, Hessian entries on demand (no stored matrix)
+struct PlaneForm {
+    d3 a, u, v;
+    double gs[9], gq[9];
+    double uu, vv, uv, c1, c2, css, csq, cqq;
+    __device__ PlaneForm(d3 a_, d3 u_, d3 v_, double* gd) : a(a_), u(u_), v(v_)
+    {
 #pragma clang fp contract(fast)
-    const double c = dot(a, w), ww = sqn(w), iw2 = 1.0 / ww;
-    const double ac[3] = {a.x, a.y, a.z}, wc[3] = {w.x, w.y, w.z};
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        gd[IA + i] = 2 * ac[i] - 2 * c * iw2 * wc[i];
-        gd[IW + i] = -2 * c * iw2 * ac[i] + 2 * c * c * iw2 * iw2 * wc[i];
+        uu = sqn(u);
+        vv = sqn(v);
+        uv = dot(u, v);
+        q_grad(u, v, uu, vv, uv, gq);
+        const d3 n = cross(u, v);
+        const double s = dot(a, n);
+        const double q = sqn(n);
+        const d3 gsu = cross(v, a), gsv = cross(a, u);
+        gs[0] = n.x; gs[1] = n.y; gs[2] = n.z;
+        gs[3] = gsu.x; gs[4] = gsu.y; gs[5] = gsu.z;
+        gs[6] = gsv.x; gs[7] = gsv.y; gs[8] = gsv.z;
+        const double iq = 1.0 / q;
+        c1 = 2 * s * iq;        // d(d)/ds
+        c2 = -s * s * iq * iq;  // d(d)/dq
+        for (int i = 0; i < 9; ++i) gd[i] = c1 * gs[i] + c2 * gq[i];
+        css = 2 * iq;
+        csq = -2 * s * iq * iq;
+        cqq = 2 * s * s * iq * iq * iq;
     }
-    if (!WANT_H) return;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            const double id = (i == j) ? 1.0 : 0.0;
-            Hd[(IA + i) * 9 + IA + j] = 2 * id - 2 * iw2 * wc[i] * wc[j];
-            const double haw = -2 * iw2 * wc[i] * ac[j] + 4 * c * iw2 * iw2 * wc[i] * wc[j] - 2 * c * iw2 * id;
-            Hd[(IA + i) * 9 + IW + j] = haw;
-            Hd[(IW + j) * 9 + IA + i] = haw;
-            Hd[(IW + i) * 9 + IW + j] = -2 * iw2 * ac[i] * ac[j] +
-                                        4 * c * iw2 * iw2 * (ac[i] * wc[j] + wc[i] * ac[j]) -
-                                        8 * c * c * iw2 * iw2 * iw2 * wc[i] * wc[j] + 2 * c * c * iw2 * iw2 * id;
+    __device__ double h(int i, int j) const
+    {
+#pragma clang fp contract(fast)
+        double r = css * gs[i] * gs[j] + csq * (gs[i] * gq[j] + gq[i] * gs[j]) + cqq * gq[i] * gq[j] +
+                   c2 * q_hess(i, j, u, v, uu, vv, uv);
+        // + (2s/q) Hess s: [a,u] = -[v]x, [a,v] = [u]x, [u,v] = -[a]x and transposes
+        const int bi = i / 3, bj = j / 3;
+        if (bi == 0 && bj == 1) r += -c1 * skew_entry(i % 3, j % 3, v);
+        if (bi == 1 && bj == 0) r += c1 * skew_entry(i % 3, j % 3, v);
+        if (bi == 0 && bj == 2) r += c1 * skew_entry(i % 3, j % 3, u);
+        if (bi == 2 && bj == 0) r += -c1 * skew_entry(i % 3, j % 3, u);
+        if (bi == 1 && bj == 2) r += -c1 * skew_entry(i % 3, j % 3, a);
+        if (bi == 2 && bj == 1) r += c1 * skew_entry(i % 3, j % 3, a);
+        return r;
+    }
+};
+
+// point-edge form d = |a|^2 - (a.w)^2/|w|^2 in blocks IA < IW of a 9-wide layout
+template <int IA, int IW>
+struct PEForm {
+    double ac[3], wc[3], c, iw2;
+    __device__ PEForm(d3 a, d3 w, double* gd)
+    {
+#pragma clang fp contract(fast)
+        c = dot(a, w);
+        iw2 = 1.0 / sqn(w);
+        ac[0] = a.x; ac[1] = a.y; ac[2] = a.z;
+        wc[0] = w.x; wc[1] = w.y; wc[2] = w.z;
+        for (int i = 0; i < 3; ++i) {
+            gd[IA + i] = 2 * ac[i] - 2 * c * iw2 * wc[i];
+            gd[IW + i] = -2 * c * iw2 * ac[i] + 2 * c * c * iw2 * iw2 * wc[i];
         }
+    }
+    __device__ double aw(int i, int j) const
+    {
+#pragma clang fp contract(fast)
+        const double id = (i == j) ? 1.0 : 0.0;
+        return -2 * iw2 * wc[i] * ac[j] + 4 * c * iw2 * iw2 * wc[i] * wc[j] - 2 * c * iw2 * id;
+    }
+    __device__ double h(int i, int j) const
+    {
+#pragma clang fp contract(fast)
+        const bool ia = i >= IA && i < IA + 3, iw = i >= IW && i < IW + 3;
+        const bool ja = j >= IA && j < IA + 3, jw = j >= IW && j < IW + 3;
+        if (ia && ja) {
+            const int r = i - IA, s = j - IA;
+            return 2 * (r == s ? 1.0 : 0.0) - 2 * iw2 * wc[r] * wc[s];
+        }
+        if (iw && jw) {
+            const int r = i - IW, s = j - IW;
+            return -2 * iw2 * ac[r] * ac[s] + 4 * c * iw2 * iw2 * (ac[r] * wc[s] + wc[r] * ac[s]) -
+                   8 * c * c * iw2 * iw2 * iw2 * wc[r] * wc[s] + 2 * c * c * iw2 * iw2 * (r == s ? 1.0 : 0.0);
+        }
+        if (ia && jw) return aw(i - IA, j - IW);
+        if (iw && ja) return aw(j - IA, i - IW);
+        return 0.0;
+    }
+};
+
+// point-point d = |a|^2
+struct PPForm {
+    __device__ PPForm(d3 a, double* gd)
+    {
+        gd[0] = 2 * a.x;
+        gd[1] = 2 * a.y;
+        gd[2] = 2 * a.z;
+    }
+    __device__ double h(int i, int j) const { return (i == j && i < 3) ? 2.0 : 0.0; }
+};
+
+// Gradient g (3M) and, when WANT_H, Hessian H (3M x 3M, packed sx<3M>) in y
+// of f = mult * b(d - th2) [* e(q)].  Returns false when d - th2 <= 0.
+template <int KIND, bool WANT_H, class Form>
+__device__ inline void assemble_reduced(const Form& F, const double* gd, const Decoded& k, const double* X,
+                                        const double* X0, double b, double b1, double b2, d3 y1, d3 y2, double* g,
+                                        double* H, double& energy)
+{
+#pragma clang fp contract(fast)
+    constexpr int N = 3 * KT<KIND>::M;
+    if (!KT<KIND>::MOLL) {
+        const double m = k.mult;
+        energy = (m > 1.0) ? b * m : b;
+#pragma unroll
+        for (int i = 0; i < N; ++i) g[i] = m * b1 * gd[i];
+        if (WANT_H)
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+#pragma unroll
+                for (int j = i; j < N; ++j) H[sx<N>(i, j)] = m * (b2 * gd[i] * gd[j] + b1 * F.h(i, j));
+        return;
+    }
+    double q, eps;
+    const double e = mollifier_value(k, X, X0, q, eps);
+    energy = b * e;
+    if (!(q < eps)) {  // e == 1, grad e == 0
+#pragma unroll
+        for (int i = 0; i < 9; ++i) g[i] = b1 * gd[i];
+        if (WANT_H)
+#pragma unroll
+            for (int i = 0; i < 9; ++i)
+#pragma unroll
+                for (int j = i; j < 9; ++j) H[sx<9>(i, j)] = b2 * gd[i] * gd[j] + b1 * F.h(i, j);
+        return;
+    }
+    const double de = 2.0 / eps * (1.0 - q / eps);
+    const double d2e = -2.0 / (eps * eps);
+    const double uu = sqn(y1), vv = sqn(y2), uv = dot(y1, y2);
+    double gq[9];
+    q_grad(y1, y2, uu, vv, uv, gq);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) g[i] = e * b1 * gd[i] + b * de * gq[i];
+    if (WANT_H)
+#pragma unroll
+        for (int i = 0; i < 9; ++i)
+#pragma unroll
+            for (int j = i; j < 9; ++j) {
+                const double He = de * q_hess(i, j, y1, y2, uu, vv, uv) + d2e * gq[i] * gq[j];
+                H[sx<9>(i, j)] = e * b2 * gd[i] * gd[j] + e * b1 * F.h(i, j) +
+                                 b1 * de * (gd[i] * gq[j] + gq[i] * gd[j]) + b * He;
+            }
 }
 
-// Gradient g (3M) and, when WANT_H, Hessian H (3M x 3M, ld 9) in y of
-// f = mult * b(d - th2) [* e(q)].  Returns false when d - th2 <= 0.
 template <int KIND, bool WANT_H>
 __device__ inline bool reduced_derivs(const Decoded& k, const double* X, const double* X0, double dh,
                                       double kappa, double th2, double* g, double* H, double& energy)
 {
 #pragma clang fp contract(fast)
-    constexpr int M = KT<KIND>::M;
-    constexpr int N = 3 * M;
-    double gd[9], Hd[81];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) gd[i] = 0.0;
-    if (WANT_H)
-#pragma unroll
-        for (int i = 0; i < 81; ++i) Hd[i] = 0.0;
     d3 y0, y1, y2;
     if (KIND == K_PT) {
         d3 p = ld3(X, k.v[0]), t0 = ld3(X, k.v[1]);
@@ -378,56 +465,25 @@ __device__ inline bool reduced_derivs(const Decoded& k, const double* X, const d
     // the distance value exactly as the energy kernel evaluates it
     const double d = constraint_d2(k, X) - th2;
     if (!(d > 0)) return false;
-    if (KIND == K_PT || KIND == K_EE || KIND == K_MEE) plane_derivs<WANT_H>(y0, y1, y2, gd, Hd);
-    else if (KIND == K_PE) pe_derivs<WANT_H, 0, 3>(y0, y1, gd, Hd);
-    else if (KIND == K_MPE) pe_derivs<WANT_H, 0, 6>(y0, y2, gd, Hd);
-    else {  // PP and mollified PP: d = |a|^2
-        gd[0] = 2 * y0.x; gd[1] = 2 * y0.y; gd[2] = 2 * y0.z;
-        if (WANT_H) Hd[0] = Hd[10] = Hd[20] = 2.0;
-    }
     const double b = barrier_b(d, dh, kappa);
     const double b1 = barrier_db(d, dh, kappa);
     const double b2 = WANT_H ? barrier_d2b(d, dh, kappa) : 0.0;
-    if (!KT<KIND>::MOLL) {
-        const double m = k.mult;
-        energy = (m > 1.0) ? b * m : b;
+    double gd[9];
 #pragma unroll
-        for (int i = 0; i < N; ++i) g[i] = m * b1 * gd[i];
-        if (WANT_H)
-#pragma unroll
-            for (int i = 0; i < N; ++i)
-#pragma unroll
-                for (int j = 0; j < N; ++j) H[i * 9 + j] = m * (b2 * gd[i] * gd[j] + b1 * Hd[i * 9 + j]);
-        return true;
+    for (int i = 0; i < 9; ++i) gd[i] = 0.0;
+    if (KIND == K_PT || KIND == K_EE || KIND == K_MEE) {
+        PlaneForm F(y0, y1, y2, gd);
+        assemble_reduced<KIND, WANT_H>(F, gd, k, X, X0, b, b1, b2, y1, y2, g, H, energy);
+    } else if (KIND == K_PE) {
+        PEForm<0, 3> F(y0, y1, gd);
+        assemble_reduced<KIND, WANT_H>(F, gd, k, X, X0, b, b1, b2, y1, y2, g, H, energy);
+    } else if (KIND == K_MPE) {
+        PEForm<0, 6> F(y0, y2, gd);
+        assemble_reduced<KIND, WANT_H>(F, gd, k, X, X0, b, b1, b2, y1, y2, g, H, energy);
+    } else {
+        PPForm F(y0, gd);
+        assemble_reduced<KIND, WANT_H>(F, gd, k, X, X0, b, b1, b2, y1, y2, g, H, energy);
     }
-    double q, eps;
-    const double e = mollifier_value(k, X, X0, q, eps);
-    energy = b * e;
-    if (!(q < eps)) {  // e == 1, grad e == 0
-#pragma unroll
-        for (int i = 0; i < 9; ++i) g[i] = b1 * gd[i];
-        if (WANT_H)
-#pragma unroll
-            for (int i = 0; i < 9; ++i)
-#pragma unroll
-                for (int j = 0; j < 9; ++j) H[i * 9 + j] = b2 * gd[i] * gd[j] + b1 * Hd[i * 9 + j];
-        return true;
-    }
-    const double de = 2.0 / eps * (1.0 - q / eps);
-    const double d2e = -2.0 / (eps * eps);
-    double gq[9], Hq[81];
-    q_derivs(y1, y2, gq, Hq);
-#pragma unroll
-    for (int i = 0; i < 9; ++i) g[i] = e * b1 * gd[i] + b * de * gq[i];
-    if (WANT_H)
-#pragma unroll
-        for (int i = 0; i < 9; ++i)
-#pragma unroll
-            for (int j = 0; j < 9; ++j) {
-                const double He = de * Hq[i * 9 + j] + d2e * gq[i] * gq[j];
-                H[i * 9 + j] = e * b2 * gd[i] * gd[j] + e * b1 * Hd[i * 9 + j] +
-                               b1 * de * (gd[i] * gq[j] + gq[i] * gd[j]) + b * He;
-            }
     return true;
 }
 
@@ -440,8 +496,12 @@ __device__ inline const double* bmat()
 }
 
 // ---------------------------------------------------------------------------
-// Jacobi eigen-decomposition of a symmetric N x N matrix (ld 9) in registers.
+// Cyclic Jacobi eigen-decomposition of a symmetric N x N matrix.  A is held in
+// registers as the packed upper triangle; the eigenvector matrix W lives in
+// LDS (element (k, p) of this lane at W[(9 k + p) * WS]).
 // ---------------------------------------------------------------------------
+constexpr int WS = 64;  // lanes per block of the Hessian kernel = LDS stride
+
 template <int N>
 __device__ inline void jacobi(double* A, double* W)
 {
@@ -449,23 +509,24 @@ __device__ inline void jacobi(double* A, double* W)
 #pragma unroll
     for (int i = 0; i < N; ++i)
 #pragma unroll
-        for (int j = 0; j < N; ++j) W[i * 9 + j] = (i == j) ? 1.0 : 0.0;
+        for (int j = 0; j < N; ++j) W[(i * 9 + j) * WS] = (i == j) ? 1.0 : 0.0;
+#pragma nounroll
     for (int sweep = 0; sweep < 16; ++sweep) {
         double off = 0.0, dg = 0.0;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-            dg += A[i * 9 + i] * A[i * 9 + i];
+            dg += A[sx<N>(i, i)] * A[sx<N>(i, i)];
 #pragma unroll
-            for (int j = i + 1; j < N; ++j) off += A[i * 9 + j] * A[i * 9 + j];
+            for (int j = i + 1; j < N; ++j) off += A[sx<N>(i, j)] * A[sx<N>(i, j)];
         }
         if (!(off > 1e-34 * dg)) break;
 #pragma unroll
         for (int p = 0; p < N - 1; ++p)
 #pragma unroll
             for (int q = p + 1; q < N; ++q) {
-                const double apq = A[p * 9 + q];
+                const double apq = A[sx<N>(p, q)];
                 if (apq != 0.0) {
-                    const double theta = (A[q * 9 + q] - A[p * 9 + p]) / (2.0 * apq);
+                    const double theta = (A[sx<N>(q, q)] - A[sx<N>(p, p)]) / (2.0 * apq);
                     const double at = fabs(theta);
                     const double t = (at > 1e150) ? 0.5 / theta
                                                   : copysign(1.0, theta) / (at + sqrt(theta * theta + 1.0));
@@ -473,96 +534,101 @@ __device__ inline void jacobi(double* A, double* W)
 #pragma unroll
                     for (int k2 = 0; k2 < N; ++k2) {
                         if (k2 == p || k2 == q) continue;
-                        const double akp = A[k2 * 9 + p], akq = A[k2 * 9 + q];
-                        const double np = c * akp - s * akq, nq = s * akp + c * akq;
-                        A[k2 * 9 + p] = np;
-                        A[p * 9 + k2] = np;
-                        A[k2 * 9 + q] = nq;
-                        A[q * 9 + k2] = nq;
+                        const double akp = A[sx<N>(k2, p)], akq = A[sx<N>(k2, q)];
+                        A[sx<N>(k2, p)] = c * akp - s * akq;
+                        A[sx<N>(k2, q)] = s * akp + c * akq;
                     }
-                    A[p * 9 + p] = A[p * 9 + p] - t * apq;
-                    A[q * 9 + q] = A[q * 9 + q] + t * apq;
-                    A[p * 9 + q] = 0.0;
-                    A[q * 9 + p] = 0.0;
+                    A[sx<N>(p, p)] = A[sx<N>(p, p)] - t * apq;
+                    A[sx<N>(q, q)] = A[sx<N>(q, q)] + t * apq;
+                    A[sx<N>(p, q)] = 0.0;
 #pragma unroll
                     for (int k2 = 0; k2 < N; ++k2) {
-                        const double wkp = W[k2 * 9 + p], wkq = W[k2 * 9 + q];
-                        W[k2 * 9 + p] = c * wkp - s * wkq;
-                        W[k2 * 9 + q] = s * wkp + c * wkq;
+                        const double wkp = W[(k2 * 9 + p) * WS], wkq = W[(k2 * 9 + q) * WS];
+                        W[(k2 * 9 + p) * WS] = c * wkp - s * wkq;
+                        W[(k2 * 9 + q) * WS] = s * wkp + c * wkq;
                     }
                 }
+                // keep each rotation's LDS traffic in place (no hoisting of later
+                // rotations' loads, which only raises register pressure)
+                __builtin_amdgcn_sched_barrier(0);
             }
     }
 }
 
-// makePD in the reduced space; H (ld 9, M blocks) is replaced by its projection.
+// makePD in the reduced space; H (packed sx<3M>) is replaced by its
+// projection.  W: this lane's LDS eigenvector workspace.
 template <int KIND>
-__device__ inline bool project_reduced(double* H)
+__device__ inline bool project_reduced(double* H, double* W)
 {
 #pragma clang fp contract(fast)
     constexpr int M = KT<KIND>::M;
     constexpr int N = 3 * M;
     const double* S = (M == 3) ? ((KIND == K_PT) ? c_S3pt : c_S3ee) : c_S2;
     const double* Si = (M == 3) ? ((KIND == K_PT) ? c_Si3pt : c_Si3ee) : c_Si2;
-    double K[81], W[81];
-    // K = C H C, C = S (x) I3  (for M == 1, C = sqrt(2) I)
+    double K[N * (N + 1) / 2];
+    // K = C H C, C = S (x) I3  (for M == 1, C = sqrt(2) I); upper triangle
 #pragma unroll
     for (int a = 0; a < M; ++a)
 #pragma unroll
-        for (int b = 0; b < M; ++b)
+        for (int b = a; b < M; ++b)
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
                 for (int j = 0; j < 3; ++j) {
+                    if (a == b && j < i) continue;
                     double acc = 0.0;
                     if (M == 1) {
-                        acc = 2.0 * H[i * 9 + j];
+                        acc = 2.0 * H[sx<N>(i, j)];
                     } else {
 #pragma unroll
                         for (int g = 0; g < M; ++g)
 #pragma unroll
                             for (int h = 0; h < M; ++h)
-                                acc += S[a * M + g] * H[(3 * g + i) * 9 + 3 * h + j] * S[h * M + b];
+                                acc += S[a * M + g] * H[sx<N>(3 * g + i, 3 * h + j)] * S[h * M + b];
                     }
-                    K[(3 * a + i) * 9 + 3 * b + j] = acc;
+                    K[sx<N>(3 * a + i, 3 * b + j)] = acc;
                 }
     jacobi<N>(K, W);
-    double lmin = K[0];
+    double lmin = K[sx<N>(0, 0)];
 #pragma unroll
-    for (int i = 1; i < N; ++i) lmin = fmin(lmin, K[i * 9 + i]);
+    for (int i = 1; i < N; ++i) lmin = fmin(lmin, K[sx<N>(i, i)]);
     if (lmin >= 0.0) return false;  // UTILS.h:12-15: nothing to clamp
     double lam[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) lam[i] = fmax(K[i * 9 + i], 0.0);
+    for (int i = 0; i < N; ++i) lam[i] = fmax(K[sx<N>(i, i)], 0.0);
 #pragma unroll
-    for (int i = 0; i < N; ++i)
+    for (int i = 0; i < N; ++i) {
+        double wi[N];
+#pragma unroll
+        for (int k2 = 0; k2 < N; ++k2) wi[k2] = W[(i * 9 + k2) * WS] * lam[k2];
 #pragma unroll
         for (int j = i; j < N; ++j) {
             double acc = 0.0;
 #pragma unroll
-            for (int k2 = 0; k2 < N; ++k2) acc += W[i * 9 + k2] * lam[k2] * W[j * 9 + k2];
-            K[i * 9 + j] = acc;
-            K[j * 9 + i] = acc;
+            for (int k2 = 0; k2 < N; ++k2) acc += wi[k2] * W[(j * 9 + k2) * WS];
+            K[sx<N>(i, j)] = acc;
         }
+    }
 #pragma unroll
     for (int a = 0; a < M; ++a)
 #pragma unroll
-        for (int b = 0; b < M; ++b)
+        for (int b = a; b < M; ++b)
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
                 for (int j = 0; j < 3; ++j) {
+                    if (a == b && j < i) continue;
                     double acc = 0.0;
                     if (M == 1) {
-                        acc = 0.5 * K[i * 9 + j];
+                        acc = 0.5 * K[sx<N>(i, j)];
                     } else {
 #pragma unroll
                         for (int g = 0; g < M; ++g)
 #pragma unroll
                             for (int h = 0; h < M; ++h)
-                                acc += Si[a * M + g] * K[(3 * g + i) * 9 + 3 * h + j] * Si[h * M + b];
+                                acc += Si[a * M + g] * K[sx<N>(3 * g + i, 3 * h + j)] * Si[h * M + b];
                     }
-                    H[(3 * a + i) * 9 + 3 * b + j] = acc;
+                    H[sx<N>(3 * a + i, 3 * b + j)] = acc;
                 }
     return true;
 }
@@ -603,7 +669,7 @@ __device__ inline void expand_block(const double* H, int vi, int vj, double* blk
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int j = 0; j < 3; ++j) blk[3 * i + j] = s * H[i * 9 + j];
+            for (int j = 0; j < 3; ++j) blk[3 * i + j] = s * H[sx<3>(i, j)];
         return;
     }
     const double* B = bmat<KIND>();
@@ -618,7 +684,7 @@ __device__ inline void expand_block(const double* H, int vi, int vj, double* blk
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
-                for (int j = 0; j < 3; ++j) blk[3 * i + j] += s * H[(3 * a + i) * 9 + 3 * b + j];
+                for (int j = 0; j < 3; ++j) blk[3 * i + j] += s * H[sx<3 * M>(3 * a + i, 3 * b + j)];
         }
     }
 }
@@ -643,25 +709,26 @@ __device__ inline void grad_body(const Decoded& k, const double* X, const double
 
 template <int KIND>
 __device__ inline void hess_body(const Decoded& k, const double* X, const double* X0, double dh, double kappa,
-                                 double th2, int project_spd, int mode, double* out, int* err, int* counts)
+                                 double th2, int project_spd, int mode, double* out, int* err, int* counts,
+                                 double* W)
 {
     constexpr int NV = KT<KIND>::NV;
-    double g[9], H[81], e;
+    double g[9], H[45], e;
     if (!reduced_derivs<KIND, true>(k, X, X0, dh, kappa, th2, g, H, e)) {
         atomicOr(err, 1);
         for (int j = 0; j < 144; ++j) out[j] = 0.0;
         return;
     }
     bool clamped = false;
-    if (project_spd) clamped = project_reduced<KIND>(H);
+    if (project_spd) clamped = project_reduced<KIND>(H, W);
     if (counts) {
         atomicAdd(counts + 2 * KIND, 1);
         if (clamped) atomicAdd(counts + 2 * KIND + 1, 1);
     }
     double blk[9];
-#pragma unroll
+#pragma nounroll
     for (int a = 0; a < NV; ++a)
-#pragma unroll
+#pragma nounroll
         for (int b = 0; b < NV; ++b) {
             expand_block<KIND>(H, a, b, blk);
             if (mode == 0) {
@@ -704,17 +771,19 @@ __global__ void __launch_bounds__(256) k_barrier_grad(long n, const int4* __rest
 // Hessian: mode 0 writes the dense (3nv)^2 local matrix (144 per constraint,
 // row-major, vertex order of decode); mode 1 writes nv*nv vertex blocks of 9
 // (block (a,b) at 9*(a*nv+b)).
-__global__ void __launch_bounds__(64) k_barrier_hess(long n, const int4* __restrict__ cs,
+__global__ void __launch_bounds__(WS) k_barrier_hess(long n, const int4* __restrict__ cs,
                                                      const double* __restrict__ X,
                                                      const double* __restrict__ X0, double dh, double kappa,
                                                      double th2, int project_spd, int mode,
                                                      double* __restrict__ out, int* __restrict__ vid,
                                                      int* __restrict__ err, int* __restrict__ counts)
 {
+    __shared__ double wsh[81 * WS];
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Decoded k = decode(cs[i]);
-    IDP_KIND_SWITCH(k.kind, hess_body, k, X, X0, dh, kappa, th2, project_spd, mode, out + 144 * i, err, counts)
+    IDP_KIND_SWITCH(k.kind, hess_body, k, X, X0, dh, kappa, th2, project_spd, mode, out + 144 * i, err, counts,
+                    wsh + threadIdx.x)
     if (vid)
         for (int j = 0; j < 4; ++j) vid[4 * i + j] = (j < k.nv) ? k.v[j] : -1;
 }
@@ -755,7 +824,7 @@ void launch_barrier_hess(hipStream_t s, long n, const int* cs, const double* X, 
                          double kappa, double th2, int project_spd, int mode, double* out, int* vid, int* err,
                          int* counts)
 {
-    IDP_LAUNCH(k_barrier_hess, grid_for(n, 64), 64, 0, s, n, (const int4*)cs, X, X0, dh, kappa, th2, project_spd,
+    IDP_LAUNCH(k_barrier_hess, grid_for(n, WS), WS, 0, s, n, (const int4*)cs, X, X0, dh, kappa, th2, project_spd,
                mode, out, vid, err, counts);
 }
 

@@ -80,19 +80,23 @@ struct DBuf {
 };
 
 // ---------------------------------------------------------------------------
-// Optional per-family device timing with HIP events on one stream.
+// Optional per-family device timing with HIP events on one stream.  Records
+// stay open until collect(); a caller may discard records it tagged (e.g. the
+// kernels a chunked solver launched past its stopping point).
 // ---------------------------------------------------------------------------
 struct Profiler {
+    struct Rec {
+        int fam, a, b, tag;
+        double work;
+    };
     bool on = false;
     hipStream_t s = nullptr;
     std::vector<hipEvent_t> pool;
-    std::vector<std::pair<int, std::pair<int, int>>> open;  // family, (start, end) event ids
+    std::vector<Rec> open;
     size_t next = 0;
     double ms[IDP_K_COUNT] = {0};
     long launches[IDP_K_COUNT] = {0};
     double work[IDP_K_COUNT] = {0};
-    long pairs[8] = {0}, clamped[8] = {0};
-    int cur_start = -1;
     ~Profiler()
     {
         for (auto e : pool) (void)hipEventDestroy(e);
@@ -110,7 +114,6 @@ struct Profiler {
     {
         collect();
         for (int i = 0; i < IDP_K_COUNT; ++i) ms[i] = work[i] = 0, launches[i] = 0;
-        for (int i = 0; i < 8; ++i) pairs[i] = clamped[i] = 0;
     }
     // returns a token for end()
     int begin(int fam)
@@ -120,19 +123,28 @@ struct Profiler {
         IDP_HIP(hipEventRecord(pool[a], s));
         return a;
     }
-    void end(int fam, int tok, double w = 0.0)
+    void end(int fam, int tok, double w = 0.0, int tag = 0)
     {
         if (!on || tok < 0) return;
         int b = ev();
         IDP_HIP(hipEventRecord(pool[b], s));
-        open.push_back({fam, {tok, b}});
-        launches[fam] += 1;
-        work[fam] += w;
-        if (open.size() > 4096) collect();
+        open.push_back({fam, tok, b, tag, w});
+    }
+    void discard_from(int fam, int tag)
+    {
+        size_t k = 0;
+        for (size_t i = 0; i < open.size(); ++i)
+            if (!(open[i].fam == fam && open[i].tag >= tag)) open[k++] = open[i];
+        open.resize(k);
     }
     void add_work(int fam, double w)
     {
         if (on) work[fam] += w;
+    }
+    // safe point: collect when many records are open
+    void checkpoint()
+    {
+        if (open.size() > 16384) collect();
     }
     void collect()
     {
@@ -143,8 +155,10 @@ struct Profiler {
         IDP_HIP(hipStreamSynchronize(s));
         for (auto& o : open) {
             float t = 0.f;
-            IDP_HIP(hipEventElapsedTime(&t, pool[o.second.first], pool[o.second.second]));
-            ms[o.first] += t;
+            IDP_HIP(hipEventElapsedTime(&t, pool[o.a], pool[o.b]));
+            ms[o.fam] += t;
+            launches[o.fam] += 1;
+            work[o.fam] += o.work;
         }
         open.clear();
         next = 0;
@@ -176,12 +190,18 @@ __device__ inline double block_sum(double v, double* sh)
 }
 
 // Sum of n partials in a fixed order, computed redundantly by every block that
-// needs it (so a dependent kernel needs no extra launch).
-__device__ inline double sum_partials_block(const double* part, int n, double* sh)
+// needs it (so a dependent kernel needs no extra launch).  B = block size.
+template <int B>
+__device__ inline double sum_partials_block_n(const double* part, int n, double* sh)
 {
     double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += part[i];
-    return block_sum<RB>(acc, sh);
+    for (int i = threadIdx.x; i < n; i += B) acc += part[i];
+    return block_sum<B>(acc, sh);
+}
+
+__device__ inline double sum_partials_block(const double* part, int n, double* sh)
+{
+    return sum_partials_block_n<RB>(part, n, sh);
 }
 
 }  // namespace idp

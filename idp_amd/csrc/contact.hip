@@ -6,16 +6,18 @@
 //
 // Broad phase.  The reference hashes primitives into a uniform voxel grid
 // (voxel = mean boundary edge length) and pairs primitives that share a
-// voxel.  Here every primitive gets an integer voxel box; boxes are expanded
-// into (voxel key, primitive) records, radix-sorted by key, and each query
-// primitive walks the voxels of its own box, binary-searching the sorted
-// keys.  A pair found in several shared voxels is kept only in the lowest
-// shared voxel (componentwise max of the two box minima), so no hash set is
-// needed.  The narrow phase then applies the reference's FP64 filters in the
-// reference's order (incidence, all-DBC, AABB test, type, distance), so the
-// emitted set is the reference's.  Counts are produced first, then an
-// exclusive scan places every query's output; the output order is therefore
-// deterministic.
+// voxel.  Here every triangle and edge gets an integer voxel box; boxes are
+// expanded into (2 * voxel + type, primitive) records and radix-sorted, and a
+// dense [start, end) table per record key (cleared after each use) gives the
+// primitives of a voxel in O(1).  Each query (boundary vertex or edge) is
+// handled by a group of lanes that split the voxels of its box.  A pair found
+// in several shared voxels is kept only in the lowest shared voxel
+// (componentwise max of the two box minima), so no hash set is needed.  The
+// narrow phase applies the reference's FP64 filters in the reference's order
+// (incidence, all-DBC, AABB test, type, distance), so the emitted set is the
+// reference's.  Outputs are appended with atomic counters and then sorted on
+// integer keys, so the constraint order is deterministic; CCD candidates need
+// no order (the step is a minimum).
 #include "common.hpp"
 #include "geom_dev.hpp"
 #include "kernels.hpp"
@@ -74,39 +76,50 @@ __device__ inline unsigned key_of(const GridDev& g, int x, int y, int z)
     return (unsigned)x + (unsigned)g.cnt[0] * ((unsigned)y + (unsigned)g.cnt[1] * (unsigned)z);
 }
 
-// chunked partial sums of boundary edge lengths (chunks of 256, sequential)
-__global__ void k_edge_len_parts(int ne, const int2* __restrict__ E, const double* __restrict__ X,
-                                 double* __restrict__ parts)
+// Sum of a chunk of 256 consecutive values, added sequentially in index
+// order (the order of the CPU restatement): the block computes the values in
+// parallel into LDS, one lane adds them up.
+__device__ inline double chunk_sum_lds(double v, double* sh, int nvalid)
 {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    long b = (long)c * 256;
-    if (b >= ne) return;
-    long e = b + 256 < ne ? b + 256 : ne;
+    sh[threadIdx.x] = v;
+    __syncthreads();
     double s = 0.0;
-    for (long i = b; i < e; ++i) {
+    if (threadIdx.x == 0)
+        for (int i = 0; i < nvalid; ++i) s += sh[i];
+    return s;
+}
+
+// chunked partial sums of boundary edge lengths (one block of 256 per chunk)
+__global__ void __launch_bounds__(256) k_edge_len_parts(int ne, const int2* __restrict__ E,
+                                                        const double* __restrict__ X, double* __restrict__ parts)
+{
+    __shared__ double sh[256];
+    const long b = (long)blockIdx.x * 256, i = b + threadIdx.x;
+    double len = 0.0;
+    if (i < ne) {
         int2 ed = E[i];
-        s += sqrt(sqn(ld3(X, ed.x) - ld3(X, ed.y)));
+        len = sqrt(sqn(ld3(X, ed.x) - ld3(X, ed.y)));
     }
-    parts[c] = s;
+    const double s = chunk_sum_lds(len, sh, (int)min(256L, ne - b));
+    if (threadIdx.x == 0) parts[blockIdx.x] = s;
 }
 
 // chunked partial sums of |p| over boundary nodes (x, y, z in order)
-__global__ void k_psize_parts(int nb, const int* __restrict__ bn, const double* __restrict__ p,
-                              double* __restrict__ parts)
+__global__ void __launch_bounds__(256) k_psize_parts(int nb, const int* __restrict__ bn,
+                                                     const double* __restrict__ p, double* __restrict__ parts)
 {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    long b = (long)c * 256, n = 3L * nb;
-    if (b >= n) return;
-    long e = b + 256 < n ? b + 256 : n;
-    double s = 0.0;
-    for (long i = b; i < e; ++i) s += fabs(p[3 * bn[i / 3] + (i % 3)]);
-    parts[c] = s;
+    __shared__ double sh[256];
+    const long n = 3L * nb, b = (long)blockIdx.x * 256, i = b + threadIdx.x;
+    const double a = i < n ? fabs(p[3 * bn[i / 3] + (i % 3)]) : 0.0;
+    const double s = chunk_sum_lds(a, sh, (int)min(256L, n - b));
+    if (threadIdx.x == 0) parts[blockIdx.x] = s;
 }
 
-// min / max of positions: per-block partials, exact (order independent)
+// min / max of positions (and of x + alpha p when p is given): per-block
+// partials, exact (order independent)
 __global__ void __launch_bounds__(RB) k_minmax_parts(long n, const int* __restrict__ idx,
                                                      const double* __restrict__ X, const double* __restrict__ p,
-                                                     double alpha, double* __restrict__ parts)
+                                                     const double* __restrict__ alpha_p, double* __restrict__ parts)
 {
     __shared__ double sh[6][RB];
     long i = (long)blockIdx.x * RB + threadIdx.x;
@@ -118,7 +131,7 @@ __global__ void __launch_bounds__(RB) k_minmax_parts(long n, const int* __restri
             mn[d] = fmin(mn[d], a);
             mx[d] = fmax(mx[d], a);
             if (p) {
-                double b = X[3 * v + d] + alpha * p[3 * v + d];
+                double b = X[3 * v + d] + *alpha_p * p[3 * v + d];
                 mn[d] = fmin(mn[d], b);
                 mx[d] = fmax(mx[d], b);
             }
@@ -157,9 +170,10 @@ __global__ void __launch_bounds__(RB) k_minmax_final(int np, const double* __res
 // ---------------------------------------------------------------------------
 // boxes and cell records
 // ---------------------------------------------------------------------------
-// vertex boxes: static -> point voxel; swept -> voxel box of [min(x, x+a p), max(..)] -/+ th/2
+// vertex boxes: static -> point voxel; swept -> voxel box of
+// [min(x, x + a p), max(..)] -/+ th/2 with a read from device memory
 __global__ void k_vertex_boxes(int nb, const int* __restrict__ bn, const double* __restrict__ X,
-                               const double* __restrict__ p, double alpha, double halfth,
+                               const double* __restrict__ p, const double* __restrict__ alpha_p, double halfth,
                                const GridDev* __restrict__ gp, int* __restrict__ vbox /* 6 per boundary node */)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -170,7 +184,7 @@ __global__ void k_vertex_boxes(int nb, const int* __restrict__ bn, const double*
     for (int d = 0; d < 3; ++d) {
         double x0 = X[3 * v + d];
         if (p) {
-            double x1 = x0 + alpha * p[3 * v + d];
+            double x1 = x0 + *alpha_p * p[3 * v + d];
             a[d] = fmin(x0, x1) - halfth;
             b[d] = fmax(x0, x1) + halfth;
         } else {
@@ -215,12 +229,16 @@ __global__ void k_prim_boxes(int n, const int* __restrict__ prim, const int* __r
     cnt[i] = c;
 }
 
-__global__ void k_emit_cells(int n, const int* __restrict__ pbox, const unsigned* __restrict__ off,
+// (2 * voxel + type, primitive) records; primitives [0, nsplit) are type 0
+// (triangles), [nsplit, n) type 1 (edges, numbered from 0)
+__global__ void k_emit_cells(int n, int nsplit, const int* __restrict__ pbox, const unsigned* __restrict__ off,
                              const GridDev* __restrict__ gp, unsigned* __restrict__ keys, int* __restrict__ vals)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const GridDev g = *gp;
+    const unsigned type = i >= nsplit ? 1u : 0u;
+    const int id = i - (type ? nsplit : 0);
     int l[3], h[3];
     for (int d = 0; d < 3; ++d) {
         l[d] = max(pbox[6 * i + d], 0);
@@ -230,8 +248,8 @@ __global__ void k_emit_cells(int n, const int* __restrict__ pbox, const unsigned
     for (int z = l[2]; z <= h[2]; ++z)
         for (int y = l[1]; y <= h[1]; ++y)
             for (int x = l[0]; x <= h[0]; ++x) {
-                keys[o] = key_of(g, x, y, z);
-                vals[o] = i;
+                keys[o] = 2u * key_of(g, x, y, z) + type;
+                vals[o] = id;
                 ++o;
             }
 }
@@ -247,6 +265,76 @@ __device__ inline unsigned lower_bound_u(const unsigned* a, unsigned n, unsigned
     return lo;
 }
 
+// Sorted records plus, when the grid is small enough, a dense [start, end)
+// table per record key (all zero between calls: entries are cleared after use).
+struct CellTable {
+    const unsigned* keys;
+    const int* vals;
+    const unsigned* start;  // null: binary search in keys
+    const unsigned* end;
+    unsigned n;
+};
+
+__device__ inline void cell_range(const CellTable& T, unsigned key, unsigned& s, unsigned& e)
+{
+    if (T.start) {
+        s = T.start[key];
+        e = T.end[key];
+    } else {
+        s = lower_bound_u(T.keys, T.n, key);
+        e = lower_bound_u(T.keys, T.n, key + 1);
+    }
+}
+
+// sets (clear = 0) or resets (clear = 1) the table entries of the keys present
+__global__ void k_cell_bounds(unsigned n, const unsigned* __restrict__ keys, unsigned* __restrict__ start,
+                              unsigned* __restrict__ end, int clear)
+{
+    unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned k = keys[i];
+    if (i == 0 || keys[i - 1] != k) start[k] = clear ? 0u : i;
+    if (i == n - 1 || keys[i + 1] != k) end[k] = clear ? 0u : i + 1;
+}
+
+__global__ void k_fill_u32(size_t n, unsigned* __restrict__ a, unsigned v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// Voxels of a query box split over the G lanes of a group (lane j takes the
+// linear voxel indices j, j+G, ...).  Every stored primitive of the requested
+// type whose box overlaps the query box is visited exactly once, in the
+// lowest shared voxel (componentwise max of the two box minima, clamped).
+template <int G, class F>
+__device__ inline void walk_lanes(const GridDev& g, const CellTable& T, const int* __restrict__ pbox_t, int type,
+                                  const int* ql, const int* qh, int lane, F&& visit)
+{
+    int l[3], n[3];
+    for (int d = 0; d < 3; ++d) {
+        l[d] = max(ql[d], 0);
+        n[d] = min(qh[d], g.cnt[d] - 1) - l[d] + 1;
+        if (n[d] <= 0) return;
+    }
+    const long nvox = (long)n[0] * n[1] * n[2];
+    for (long li = lane; li < nvox; li += G) {
+        const int x = l[0] + (int)(li % n[0]);
+        const long r = li / n[0];
+        const int y = l[1] + (int)(r % n[1]);
+        const int z = l[2] + (int)(r / n[1]);
+        unsigned s, e;
+        cell_range(T, 2u * key_of(g, x, y, z) + (unsigned)type, s, e);
+        for (unsigned k = s; k < e; ++k) {
+            const int f = T.vals[k];
+            const int* pb = pbox_t + 6L * f;
+            if (max(max(ql[0], pb[0]), 0) != x || max(max(ql[1], pb[1]), 0) != y || max(max(ql[2], pb[2]), 0) != z)
+                continue;
+            visit(f);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // narrow phase of the constraint set
 // ---------------------------------------------------------------------------
@@ -257,24 +345,47 @@ struct CSArgs {
     const int* bn;
     const int2* be;
     const int3* bt;
-    const int* v2b;
+    const int* pbox;  // triangles [0, nf), then edges
     const GridDev* g;
-    const unsigned* ckeys;  // sorted cell keys
-    const int* cvals;       // primitive per record
-    unsigned ncell;
-    const int* pbox;        // box of every stored primitive (tri or edge)
+    int nf;
     double r, r2;
 };
 
-__device__ inline void emit(int4* out, unsigned& n, int a, int b, int c, int d)
+// Outputs are appended with atomic counters and put in a fixed order
+// afterwards by sorting on their keys: pass tuples on (vertex, triangle) for
+// PT and (edge, edge) for EE, PP/PE duplicates on the merge key.
+struct CSOut {
+    unsigned* cnt;  // [0] pass, [1] dup
+    unsigned long long* pkey;
+    int4* pval;
+    unsigned long long* dkey;
+    unsigned cap_p, cap_d;
+    unsigned long long nf, ne, V;
+};
+
+__device__ inline void out_pass(const CSOut& O, unsigned long long key, int a, int b, int c, int d)
 {
-    if (out) out[n] = make_int4(a, b, c, d);
-    ++n;
+    const unsigned i = atomicAdd(O.cnt, 1u);
+    if (i < O.cap_p) {
+        O.pkey[i] = key;
+        O.pval[i] = make_int4(a, b, c, d);
+    }
+}
+
+// PP / PE duplicate (-p-1, q, e1 or -1, -1): key of the (s0, s1, s2) map entry
+// in std::map order
+__device__ inline void out_dup(const CSOut& O, int a, int b, int c)
+{
+    const unsigned i = atomicAdd(O.cnt + 1, 1u);
+    if (i < O.cap_d) {
+        const unsigned long long s0 = (unsigned long long)(a + (long long)O.V);  // a in [-V,-1]
+        O.dkey[i] = (s0 * O.V + (unsigned long long)b) * (O.V + 1) + (unsigned long long)(c + 1);
+    }
 }
 
 // Classifies one point-triangle pair (IPC.h:158-268).  Returns the number of
 // tuples written (0/1) into pass (PT) or dup (PP/PE) streams.
-__device__ inline void pt_narrow(const CSArgs& A, int vI, int f, int4* pass, unsigned& np, int4* dup, unsigned& nd)
+__device__ inline void pt_narrow(const CSArgs& A, int vI, int f, const CSOut& O)
 {
     int3 t = A.bt[f];
     if (vI == t.x || vI == t.y || vI == t.z) return;
@@ -283,19 +394,18 @@ __device__ inline void pt_narrow(const CSArgs& A, int vI, int f, int4* pass, uns
     if (!pt_cd_broadphase(p, t0, t1, t2, A.r)) return;
     double d;
     switch (pt_type(p, t0, t1, t2)) {
-    case 0: d = d2_pp(p, t0); if (d < A.r2) emit(dup, nd, -vI - 1, t.x, -1, -1); break;
-    case 1: d = d2_pp(p, t1); if (d < A.r2) emit(dup, nd, -vI - 1, t.y, -1, -1); break;
-    case 2: d = d2_pp(p, t2); if (d < A.r2) emit(dup, nd, -vI - 1, t.z, -1, -1); break;
-    case 3: d = d2_pe(p, t0, t1); if (d < A.r2) emit(dup, nd, -vI - 1, t.x, t.y, -1); break;
-    case 4: d = d2_pe(p, t1, t2); if (d < A.r2) emit(dup, nd, -vI - 1, t.y, t.z, -1); break;
-    case 5: d = d2_pe(p, t2, t0); if (d < A.r2) emit(dup, nd, -vI - 1, t.z, t.x, -1); break;
-    default: d = d2_pt(p, t0, t1, t2); if (d < A.r2) emit(pass, np, -vI - 1, t.x, t.y, t.z); break;
+    case 0: d = d2_pp(p, t0); if (d < A.r2) out_dup(O, -vI - 1, t.x, -1); break;
+    case 1: d = d2_pp(p, t1); if (d < A.r2) out_dup(O, -vI - 1, t.y, -1); break;
+    case 2: d = d2_pp(p, t2); if (d < A.r2) out_dup(O, -vI - 1, t.z, -1); break;
+    case 3: d = d2_pe(p, t0, t1); if (d < A.r2) out_dup(O, -vI - 1, t.x, t.y); break;
+    case 4: d = d2_pe(p, t1, t2); if (d < A.r2) out_dup(O, -vI - 1, t.y, t.z); break;
+    case 5: d = d2_pe(p, t2, t0); if (d < A.r2) out_dup(O, -vI - 1, t.z, t.x); break;
+    default: d = d2_pt(p, t0, t1, t2); if (d < A.r2) out_pass(O, (unsigned long long)vI * O.nf + f, -vI - 1, t.x, t.y, t.z); break;
     }
 }
 
 // Classifies one edge-edge pair (IPC.h:371-568)
-__device__ inline void ee_narrow(const CSArgs& A, int eI, int eJ, int4* pass, unsigned& np, int4* dup,
-                                 unsigned& nd)
+__device__ inline void ee_narrow(const CSArgs& A, int eI, int eJ, const CSOut& O)
 {
     int2 a = A.be[eI], b = A.be[eJ];
     if (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y || eI > eJ) return;
@@ -305,164 +415,109 @@ __device__ inline void ee_narrow(const CSArgs& A, int eI, int eJ, int4* pass, un
     const double cr = ee_cross_sqn(a0, a1, b0, b1);
     const double eps = mollifier_eps(ld3(A.X0, a.x), ld3(A.X0, a.y), ld3(A.X0, b.x), ld3(A.X0, b.y));
     const bool m = cr < eps;
+    const unsigned long long ekey = O.V * O.nf + (unsigned long long)eI * O.ne + eJ;
     double d;
     switch (ee_type(a0, a1, b0, b1)) {
     case 0:
         d = d2_pp(a0, b0);
         if (d < A.r2) {
-            if (m) emit(pass, np, a.x, b.x, -a.y - 1, -b.y - 1);
-            else emit(dup, nd, -a.x - 1, b.x, -1, -1);
+            if (m) out_pass(O, ekey, a.x, b.x, -a.y - 1, -b.y - 1);
+            else out_dup(O, -a.x - 1, b.x, -1);
         }
         break;
     case 1:
         d = d2_pp(a0, b1);
         if (d < A.r2) {
-            if (m) emit(pass, np, a.x, b.y, -a.y - 1, -b.x - 1);
-            else emit(dup, nd, -a.x - 1, b.y, -1, -1);
+            if (m) out_pass(O, ekey, a.x, b.y, -a.y - 1, -b.x - 1);
+            else out_dup(O, -a.x - 1, b.y, -1);
         }
         break;
     case 2:
         d = d2_pe(a0, b0, b1);
         if (d < A.r2) {
-            if (m) emit(pass, np, a.x, b.x, b.y, -a.y - 1);
-            else emit(dup, nd, -a.x - 1, b.x, b.y, -1);
+            if (m) out_pass(O, ekey, a.x, b.x, b.y, -a.y - 1);
+            else out_dup(O, -a.x - 1, b.x, b.y);
         }
         break;
     case 3:
         d = d2_pp(a1, b0);
         if (d < A.r2) {
-            if (m) emit(pass, np, a.y, b.x, -a.x - 1, -b.y - 1);
-            else emit(dup, nd, -a.y - 1, b.x, -1, -1);
+            if (m) out_pass(O, ekey, a.y, b.x, -a.x - 1, -b.y - 1);
+            else out_dup(O, -a.y - 1, b.x, -1);
         }
         break;
     case 4:
         d = d2_pp(a1, b1);
         if (d < A.r2) {
-            if (m) emit(pass, np, a.y, b.y, -a.x - 1, -b.x - 1);
-            else emit(dup, nd, -a.y - 1, b.y, -1, -1);
+            if (m) out_pass(O, ekey, a.y, b.y, -a.x - 1, -b.x - 1);
+            else out_dup(O, -a.y - 1, b.y, -1);
         }
         break;
     case 5:
         d = d2_pe(a1, b0, b1);
         if (d < A.r2) {
-            if (m) emit(pass, np, a.y, b.x, b.y, -a.x - 1);
-            else emit(dup, nd, -a.y - 1, b.x, b.y, -1);
+            if (m) out_pass(O, ekey, a.y, b.x, b.y, -a.x - 1);
+            else out_dup(O, -a.y - 1, b.x, b.y);
         }
         break;
     case 6:
         d = d2_pe(b0, a0, a1);
         if (d < A.r2) {
-            if (m) emit(pass, np, b.x, a.x, a.y, -b.y - 1);
-            else emit(dup, nd, -b.x - 1, a.x, a.y, -1);
+            if (m) out_pass(O, ekey, b.x, a.x, a.y, -b.y - 1);
+            else out_dup(O, -b.x - 1, a.x, a.y);
         }
         break;
     case 7:
         d = d2_pe(b1, a0, a1);
         if (d < A.r2) {
-            if (m) emit(pass, np, b.y, a.x, a.y, -b.x - 1);
-            else emit(dup, nd, -b.y - 1, a.x, a.y, -1);
+            if (m) out_pass(O, ekey, b.y, a.x, a.y, -b.x - 1);
+            else out_dup(O, -b.y - 1, a.x, a.y);
         }
         break;
     default:
         d = d2_ee(a0, a1, b0, b1);
         if (d < A.r2) {
-            if (m) emit(pass, np, a.x, a.y, -b.x - 1, b.y);
-            else emit(pass, np, a.x, a.y, b.x, b.y);
+            if (m) out_pass(O, ekey, a.x, a.y, -b.x - 1, b.y);
+            else out_pass(O, ekey, a.x, a.y, b.x, b.y);
         }
         break;
     }
 }
 
-// Walk the voxels of a query box; for every stored primitive in a voxel whose
-// box overlaps, visit it once (in the lowest shared voxel).
-template <class F>
-__device__ inline void walk_box(const CSArgs& A, const int* ql, const int* qh, F&& visit)
+// Constraint-set queries: q < nb are boundary vertices against triangles
+// (SPATIAL_HASH.h:215-241, box [p - r, p + r]); the rest are boundary edges
+// against edges eJ > eI (SPATIAL_HASH.h:243-291, edge box inflated by r).
+// G lanes per query.
+template <int G>
+__global__ void __launch_bounds__(256) k_cs_walk(int nb, int ne, CSArgs A, CellTable T, CSOut O)
 {
+    const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long q = tid / G;
+    const int lane = (int)(tid % G);
+    if (q >= (long)nb + ne) return;
     const GridDev& g = *A.g;
-    int l[3], h[3];
-    for (int d = 0; d < 3; ++d) {
-        l[d] = max(ql[d], 0);
-        h[d] = min(qh[d], g.cnt[d] - 1);
-    }
-    for (int z = l[2]; z <= h[2]; ++z)
-        for (int y = l[1]; y <= h[1]; ++y)
-            for (int x = l[0]; x <= h[0]; ++x) {
-                unsigned key = key_of(g, x, y, z);
-                unsigned s = lower_bound_u(A.ckeys, A.ncell, key);
-                for (unsigned k = s; k < A.ncell && A.ckeys[k] == key; ++k) {
-                    int f = A.cvals[k];
-                    const int* pb = A.pbox + 6 * f;
-                    // canonical voxel: componentwise max of the box minima (clamped)
-                    int cx = max(max(ql[0], pb[0]), 0), cy = max(max(ql[1], pb[1]), 0), cz = max(max(ql[2], pb[2]), 0);
-                    if (cx != x || cy != y || cz != z) continue;
-                    visit(f);
-                }
-            }
-}
-
-// PT queries: one thread per boundary vertex.  pass == nullptr: count only.
-__global__ void k_cs_pt(int nb, CSArgs A, const unsigned* __restrict__ offP, const unsigned* __restrict__ offD,
-                        int4* __restrict__ pass, int4* __restrict__ dup, unsigned* __restrict__ cntP,
-                        unsigned* __restrict__ cntD)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nb) return;
-    const GridDev& g = *A.g;
-    int vI = A.bn[i];
-    d3 p = ld3(A.X, vI);
     int ql[3], qh[3];
-    // SPATIAL_HASH.h:215-241: query box of [p - r, p + r]
-    locate(g, p.x - A.r, p.y - A.r, p.z - A.r, ql);
-    locate(g, p.x + A.r, p.y + A.r, p.z + A.r, qh);
-    unsigned np = 0, nd = 0;
-    int4* po = pass ? pass + offP[i] : nullptr;
-    int4* dO = pass ? dup + offD[i] : nullptr;
-    walk_box(A, ql, qh, [&](int f) { pt_narrow(A, vI, f, po, np, dO, nd); });
-    if (!pass) {
-        cntP[i] = np;
-        cntD[i] = nd;
-    }
-}
-
-// EE queries: one thread per boundary edge; stored edges eJ > eI only.
-__global__ void k_cs_ee(int ne, CSArgs A, const unsigned* __restrict__ offP, const unsigned* __restrict__ offD,
-                        int4* __restrict__ pass, int4* __restrict__ dup, unsigned* __restrict__ cntP,
-                        unsigned* __restrict__ cntD)
-{
-    int eI = blockIdx.x * blockDim.x + threadIdx.x;
-    if (eI >= ne) return;
-    const GridDev& g = *A.g;
-    int2 e = A.be[eI];
-    d3 a = ld3(A.X, e.x), b = ld3(A.X, e.y);
-    int ql[3], qh[3];
-    // SPATIAL_HASH.h:243-291: box of the edge inflated by r
-    locate(g, fmin(a.x, b.x) - A.r, fmin(a.y, b.y) - A.r, fmin(a.z, b.z) - A.r, ql);
-    locate(g, fmax(a.x, b.x) + A.r, fmax(a.y, b.y) + A.r, fmax(a.z, b.z) + A.r, qh);
-    unsigned np = 0, nd = 0;
-    int4* po = pass ? pass + offP[eI] : nullptr;
-    int4* dO = pass ? dup + offD[eI] : nullptr;
-    walk_box(A, ql, qh, [&](int f) {
-        if (f > eI) ee_narrow(A, eI, f, po, np, dO, nd);
-    });
-    if (!pass) {
-        cntP[eI] = np;
-        cntD[eI] = nd;
+    if (q < nb) {
+        const int vI = A.bn[q];
+        const d3 p = ld3(A.X, vI);
+        locate(g, p.x - A.r, p.y - A.r, p.z - A.r, ql);
+        locate(g, p.x + A.r, p.y + A.r, p.z + A.r, qh);
+        walk_lanes<G>(g, T, A.pbox, 0, ql, qh, lane, [&](int f) { pt_narrow(A, vI, f, O); });
+    } else {
+        const int eI = (int)(q - nb);
+        const int2 e = A.be[eI];
+        const d3 a = ld3(A.X, e.x), b = ld3(A.X, e.y);
+        locate(g, fmin(a.x, b.x) - A.r, fmin(a.y, b.y) - A.r, fmin(a.z, b.z) - A.r, ql);
+        locate(g, fmax(a.x, b.x) + A.r, fmax(a.y, b.y) + A.r, fmax(a.z, b.z) + A.r, qh);
+        walk_lanes<G>(g, T, A.pbox + 6L * A.nf, 1, ql, qh, lane, [&](int f) {
+            if (f > eI) ee_narrow(A, eI, f, O);
+        });
     }
 }
 
 // ---------------------------------------------------------------------------
 // merge of PP/PE duplicates (IPC.h:599-654): key order = std::map order
 // ---------------------------------------------------------------------------
-__global__ void k_dup_keys(unsigned n, const int4* __restrict__ dup, unsigned long long V,
-                           unsigned long long* __restrict__ keys)
-{
-    unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int4 c = dup[i];
-    unsigned long long s0 = (unsigned long long)(c.x + (long long)V);  // c.x in [-V,-1]
-    keys[i] = (s0 * V + (unsigned long long)c.y) * (V + 1) + (unsigned long long)(c.z + 1);
-}
-
 __global__ void k_dup_heads(unsigned n, const unsigned long long* __restrict__ keys, unsigned* __restrict__ head)
 {
     unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -497,109 +552,189 @@ struct CCDArgs {
     const int2* be;
     const int3* bt;
     const int* vbox;  // per boundary node
-    const int* pbox;  // per stored primitive
+    const int* pbox;  // triangles [0, nf), then edges
     const GridDev* g;
-    const unsigned* ckeys;
-    const int* cvals;
-    unsigned ncell;
+    int nf;
     double th;
 };
 
-template <class F>
-__device__ inline void walk_box_ccd(const CCDArgs& A, const int* ql, const int* qh, F&& visit)
+// Candidate pairs that survive the swept AABB test, appended in any order:
+// PT as (vertex, triangle) from the front of the buffer (counter cnt[0]), EE
+// as (-eI-1, eJ) from the back (counter cnt[1]).  The step size is the minimum
+// over candidates, so the order does not matter.
+template <int G>
+__global__ void __launch_bounds__(256) k_ccd_walk(int nb, int ne, CCDArgs A, CellTable T,
+                                                  unsigned* __restrict__ cnt, int2* __restrict__ cand, unsigned cap)
 {
+    const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long q = tid / G;
+    const int lane = (int)(tid % G);
+    if (q >= (long)nb + ne) return;
     const GridDev& g = *A.g;
-    int l[3], h[3];
-    for (int d = 0; d < 3; ++d) {
-        l[d] = max(ql[d], 0);
-        h[d] = min(qh[d], g.cnt[d] - 1);
+    if (q < nb) {
+        const int v = A.bn[q];
+        const d3 p = ld3(A.X, v), dp = ld3(A.P, v);
+        walk_lanes<G>(g, T, A.pbox, 0, A.vbox + 6 * q, A.vbox + 6 * q + 3, lane, [&](int f) {
+            int3 t = A.bt[f];
+            if (v == t.x || v == t.y || v == t.z) return;
+            if (A.dbc[v] && A.dbc[t.x] && A.dbc[t.y] && A.dbc[t.z]) return;
+            if (!pt_ccd_broadphase(p, ld3(A.X, t.x), ld3(A.X, t.y), ld3(A.X, t.z), dp, ld3(A.P, t.x),
+                                   ld3(A.P, t.y), ld3(A.P, t.z), A.th))
+                return;
+            const unsigned i = atomicAdd(cnt, 1u);
+            if (i < cap) cand[i] = make_int2(v, f);  // PT from the front
+        });
+    } else {
+        const int eI = (int)(q - nb);
+        const int2 a = A.be[eI];
+        const d3 a0 = ld3(A.X, a.x), a1 = ld3(A.X, a.y), da0 = ld3(A.P, a.x), da1 = ld3(A.P, a.y);
+        const int* pe = A.pbox + 6L * A.nf;
+        const int* eb = pe + 6L * eI;
+        walk_lanes<G>(g, T, pe, 1, eb, eb + 3, lane, [&](int f) {
+            if (f <= eI) return;
+            int2 b = A.be[f];
+            if (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y) return;
+            if (A.dbc[a.x] && A.dbc[a.y] && A.dbc[b.x] && A.dbc[b.y]) return;
+            if (!ee_ccd_broadphase(a0, a1, ld3(A.X, b.x), ld3(A.X, b.y), da0, da1, ld3(A.P, b.x), ld3(A.P, b.y),
+                                   A.th))
+                return;
+            const unsigned i = atomicAdd(cnt + 1, 1u);
+            if (i < cap) cand[cap - 1 - i] = make_int2(-eI - 1, f);  // EE from the back
+        });
     }
-    for (int z = l[2]; z <= h[2]; ++z)
-        for (int y = l[1]; y <= h[1]; ++y)
-            for (int x = l[0]; x <= h[0]; ++x) {
-                unsigned key = key_of(g, x, y, z);
-                unsigned s = lower_bound_u(A.ckeys, A.ncell, key);
-                for (unsigned k = s; k < A.ncell && A.ckeys[k] == key; ++k) {
-                    int f = A.cvals[k];
-                    const int* pb = A.pbox + 6 * f;
-                    int cx = max(max(ql[0], pb[0]), 0), cy = max(max(ql[1], pb[1]), 0), cz = max(max(ql[2], pb[2]), 0);
-                    if (cx != x || cy != y || cz != z) continue;
-                    visit(f);
+}
+
+// ACCD over the candidates as a work queue.  Each lane carries one pair
+// through the additive CCD iteration of CCD.h:280-395 (same arithmetic as
+// accd_pt / accd_ee, step by step); a lane whose pair ends (hit, passed the
+// bound, or zero motion) takes the next pair, one atomic per wavefront for
+// all idle lanes, so long and short pairs do not serialise a wavefront.
+// PT candidates are queued before EE ones.  The bound is the running
+// minimum, refreshed every 16 steps: a pair that cannot go below it cannot
+// change the minimum, and a pair that does reaches the same time of impact
+// whatever bound it started with, so the result is the minimum over pairs.
+struct AccdLane {
+    d3 x[4], dx[4];
+    double maxDisp, d2, dc, f, gap, toc, tmax;
+    int guard;
+    bool ee;
+};
+
+__device__ inline double accd_dist(const AccdLane& L)
+{
+    return L.ee ? d2_ee_unclassified(L.x[0], L.x[1], L.x[2], L.x[3])
+                : d2_pt_unclassified(L.x[0], L.x[1], L.x[2], L.x[3]);
+}
+
+// EE: fall back to the closest endpoint pair when the line distance is inside
+// the thickness (CCD.h:350-356)
+__device__ inline void accd_fix_ee(AccdLane& L, double th)
+{
+    if (L.ee && L.f <= 0) {
+        L.d2 = fmin(fmin(sqn(L.x[0] - L.x[2]), sqn(L.x[0] - L.x[3])), fmin(sqn(L.x[1] - L.x[2]), sqn(L.x[1] - L.x[3])));
+        L.f = L.d2 - th * th;
+    }
+}
+
+// returns false when the pair has no motion (no hit)
+__device__ inline bool accd_init(AccdLane& L, int2 c, const CCDArgs& A, double eta, double tmax)
+{
+    const double th = A.th;
+    if (c.x >= 0) {
+        const int3 t = A.bt[c.y];
+        L.ee = false;
+        L.x[0] = ld3(A.X, c.x); L.x[1] = ld3(A.X, t.x); L.x[2] = ld3(A.X, t.y); L.x[3] = ld3(A.X, t.z);
+        d3 dp = ld3(A.P, c.x), dt0 = ld3(A.P, t.x), dt1 = ld3(A.P, t.y), dt2 = ld3(A.P, t.z);
+        const d3 mov = 0.25 * (((dt0 + dt1) + dt2) + dp);
+        dt0 = dt0 - mov; dt1 = dt1 - mov; dt2 = dt2 - mov; dp = dp - mov;
+        L.dx[0] = dp; L.dx[1] = dt0; L.dx[2] = dt1; L.dx[3] = dt2;
+        const double m2 = fmax(sqn(dt0), fmax(sqn(dt1), sqn(dt2)));
+        L.maxDisp = sqrt(sqn(dp)) + sqrt(m2);
+    } else {
+        const int2 a = A.be[-c.x - 1], b = A.be[c.y];
+        L.ee = true;
+        L.x[0] = ld3(A.X, a.x); L.x[1] = ld3(A.X, a.y); L.x[2] = ld3(A.X, b.x); L.x[3] = ld3(A.X, b.y);
+        d3 da0 = ld3(A.P, a.x), da1 = ld3(A.P, a.y), db0 = ld3(A.P, b.x), db1 = ld3(A.P, b.y);
+        const d3 mov = 0.25 * (((da0 + da1) + db0) + db1);
+        da0 = da0 - mov; da1 = da1 - mov; db0 = db0 - mov; db1 = db1 - mov;
+        L.dx[0] = da0; L.dx[1] = da1; L.dx[2] = db0; L.dx[3] = db1;
+        L.maxDisp = sqrt(fmax(sqn(da0), sqn(da1))) + sqrt(fmax(sqn(db0), sqn(db1)));
+    }
+    if (L.maxDisp == 0) return false;
+    L.d2 = accd_dist(L);
+    L.f = L.d2 - th * th;
+    accd_fix_ee(L, th);
+    L.dc = sqrt(L.d2);
+    L.gap = eta * L.f / (L.dc + th);
+    L.toc = 0;
+    L.tmax = tmax;
+    L.guard = 0;
+    return true;
+}
+
+// one additive step; returns 0 running, 1 hit (toc), 2 no hit
+__device__ inline int accd_step(AccdLane& L, double eta, double th)
+{
+    const double lb = (1 - eta) * L.f / ((L.dc + th) * L.maxDisp);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) L.x[k] = L.x[k] + lb * L.dx[k];
+    L.d2 = accd_dist(L);
+    L.f = L.d2 - th * th;
+    accd_fix_ee(L, th);
+    L.dc = sqrt(L.d2);
+    if (L.toc != 0.0 && (L.f / (L.dc + th) < L.gap)) return 1;
+    L.toc += lb;
+    if (L.toc > L.tmax) return 2;
+    if (++L.guard >= (1 << 20)) return 1;
+    return 0;
+}
+
+__global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, unsigned cap,
+                                              const int2* __restrict__ cand, CCDArgs A, double eta,
+                                              unsigned long long* __restrict__ amin, unsigned* __restrict__ next)
+{
+    const unsigned npt = min(cnt[0], cap), nee = min(cnt[1], cap - npt);
+    const unsigned n = npt + nee;
+    const int lane = threadIdx.x & 63;
+    AccdLane L;
+    bool active = false, finished = false;
+    int steps = 0;
+    for (;;) {
+        const unsigned long long idle = __ballot(!active && !finished);
+        if (idle) {
+            const int leader = __ffsll((long long)idle) - 1;
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(next, (unsigned)__popcll(idle));
+            base = __shfl(base, leader);
+            if (!active && !finished) {
+                const unsigned j = base + (unsigned)__popcll(idle & ((1ull << lane) - 1));
+                if (j >= n) {
+                    finished = true;
+                } else {
+                    const int2 c = j < npt ? cand[j] : cand[cap - 1 - (j - npt)];
+                    const double tmax = __longlong_as_double((long long)*((volatile unsigned long long*)amin));
+                    active = accd_init(L, c, A, eta, tmax);
                 }
             }
-}
-
-// Candidate pairs that survive the swept AABB test: count (out == nullptr) / write
-__global__ void k_ccd_pt_cand(int nb, CCDArgs A, const unsigned* __restrict__ off, int2* __restrict__ out,
-                              unsigned* __restrict__ cnt)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nb) return;
-    int v = A.bn[i];
-    d3 p = ld3(A.X, v), dp = ld3(A.P, v);
-    unsigned n = 0;
-    int2* o = out ? out + off[i] : nullptr;
-    walk_box_ccd(A, A.vbox + 6 * i, A.vbox + 6 * i + 3, [&](int f) {
-        int3 t = A.bt[f];
-        if (v == t.x || v == t.y || v == t.z) return;
-        if (A.dbc[v] && A.dbc[t.x] && A.dbc[t.y] && A.dbc[t.z]) return;
-        if (!pt_ccd_broadphase(p, ld3(A.X, t.x), ld3(A.X, t.y), ld3(A.X, t.z), dp, ld3(A.P, t.x), ld3(A.P, t.y),
-                               ld3(A.P, t.z), A.th))
-            return;
-        if (o) o[n] = make_int2(v, f);
-        ++n;
-    });
-    if (!out) cnt[i] = n;
-}
-
-__global__ void k_ccd_ee_cand(int ne, CCDArgs A, const unsigned* __restrict__ off, int2* __restrict__ out,
-                              unsigned* __restrict__ cnt)
-{
-    int eI = blockIdx.x * blockDim.x + threadIdx.x;
-    if (eI >= ne) return;
-    int2 a = A.be[eI];
-    d3 a0 = ld3(A.X, a.x), a1 = ld3(A.X, a.y), da0 = ld3(A.P, a.x), da1 = ld3(A.P, a.y);
-    unsigned n = 0;
-    int2* o = out ? out + off[eI] : nullptr;
-    const int* eb = A.pbox + 6 * eI;
-    walk_box_ccd(A, eb, eb + 3, [&](int f) {
-        if (f <= eI) return;
-        int2 b = A.be[f];
-        if (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y) return;
-        if (A.dbc[a.x] && A.dbc[a.y] && A.dbc[b.x] && A.dbc[b.y]) return;
-        if (!ee_ccd_broadphase(a0, a1, ld3(A.X, b.x), ld3(A.X, b.y), da0, da1, ld3(A.P, b.x), ld3(A.P, b.y), A.th))
-            return;
-        if (o) o[n] = make_int2(eI, f);
-        ++n;
-    });
-    if (!out) cnt[eI] = n;
-}
-
-// ACCD over candidate pairs; toc bound = running minimum (result independent
-// of evaluation order: the minimum over pairs of their time of impact).
-__global__ void k_accd(unsigned npt, unsigned nee, const int2* __restrict__ cand, CCDArgs A, double eta,
-                       unsigned long long* __restrict__ amin)
-{
-    unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npt + nee) return;
-    int2 c = cand[i];
-    double toc = __longlong_as_double((long long)*((volatile unsigned long long*)amin));
-    bool hit;
-    if (i < npt) {
-        int3 t = A.bt[c.y];
-        hit = accd_pt(ld3(A.X, c.x), ld3(A.X, t.x), ld3(A.X, t.y), ld3(A.X, t.z), ld3(A.P, c.x), ld3(A.P, t.x),
-                      ld3(A.P, t.y), ld3(A.P, t.z), eta, A.th, toc);
-    } else {
-        int2 a = A.be[c.x], b = A.be[c.y];
-        hit = accd_ee(ld3(A.X, a.x), ld3(A.X, a.y), ld3(A.X, b.x), ld3(A.X, b.y), ld3(A.P, a.x), ld3(A.P, a.y),
-                      ld3(A.P, b.x), ld3(A.P, b.y), eta, A.th, toc);
+        }
+        if (__ballot(active) == 0) {
+            if (__ballot(!finished) == 0) break;
+            continue;
+        }
+        if (active) {
+            const int r = accd_step(L, eta, A.th);
+            if (r == 1) atomicMin(amin, (unsigned long long)__double_as_longlong(L.toc));
+            if (r != 0) active = false;
+            else if ((++steps & 15) == 0)
+                L.tmax = fmin(L.tmax, __longlong_as_double((long long)*((volatile unsigned long long*)amin)));
+        }
     }
-    if (hit) atomicMin(amin, (unsigned long long)__double_as_longlong(toc));
 }
 
-// span cap (SPATIAL_HASH.h:466-482), one thread
+// span cap (SPATIAL_HASH.h:466-482), one thread; also seeds the ACCD bound
 __global__ void k_ccd_cap(const double* __restrict__ eparts, int nep, long ne, const double* __restrict__ pparts,
-                          int npp, long nb, double alpha_in, double* __restrict__ alpha_out)
+                          int npp, long nb, double alpha_in, double* __restrict__ alpha_out,
+                          unsigned long long* __restrict__ amin, unsigned* __restrict__ cnt)
 {
     double vs = 0.0, ps = 0.0;
     for (int c = 0; c < nep; ++c) vs += eparts[c];
@@ -610,12 +745,29 @@ __global__ void k_ccd_cap(const double* __restrict__ eparts, int nep, long ne, c
     const double span = a * ps / vs;
     if (span > 1) a /= span;
     *alpha_out = a;
+    *amin = (unsigned long long)__double_as_longlong(a);
+    cnt[0] = cnt[1] = cnt[2] = 0u;
+}
+
+__global__ void k_reset_ccd(const double* __restrict__ alpha, unsigned long long* __restrict__ amin,
+                            unsigned* __restrict__ cnt)
+{
+    *amin = (unsigned long long)__double_as_longlong(*alpha);
+    cnt[0] = cnt[1] = cnt[2] = 0u;
+}
+
+__global__ void k_zero_u32(unsigned* __restrict__ a, int n)
+{
+    if ((int)threadIdx.x < n) a[threadIdx.x] = 0u;
 }
 
 // ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
 namespace {
+
+constexpr int WALK_G = 8;                        // lanes per query
+constexpr unsigned long long DENSE_MAX = 1ull << 25;  // max record keys of a dense cell table
 
 template <class T>
 void excl_scan(hipStream_t s, DBuf<unsigned char>& tmp, const T* in, T* out, int n)
@@ -660,40 +812,67 @@ int bits_for(unsigned long long maxv)
     return b;
 }
 
-// Sorted (voxel key, primitive) records of the boxes in pbox (count n).
-unsigned build_cells(ContactWork& w, hipStream_t s, int n, const GridDev* g)
+int walk_blocks(long nq) { return grid_for(nq * WALK_G, 256); }
+
+// Boxes of the triangles and edges from the vertex boxes, their sorted
+// (2 * voxel + type, primitive) records and the cell table.  One host
+// synchronisation (record count and grid size).
+CellTable build_cells(ContactWork& w, hipStream_t s, const MeshDev& m, const GridDev* g, const int* vbox)
 {
-    if (n == 0) return 0;
+    const int n = m.nf + m.ne;
+    int* pbox = w.pbox.ensure(6 * (size_t)std::max(n, 1), s);
     unsigned* cnt = w.u0.ensure(n + 1, s);
     unsigned* off = w.u1.ensure(n + 1, s);
-    // counts already written by k_prim_boxes into u0; total = scan
-    IDP_HIP(hipMemsetAsync(cnt + n, 0, sizeof(unsigned), s));
+    IDP_LAUNCH(k_prim_boxes<3>, grid_for(m.nf, 256), 256, 0, s, m.nf, m.bt.p, m.v2b.p, vbox, g, pbox, cnt);
+    IDP_LAUNCH(k_prim_boxes<2>, grid_for(m.ne, 256), 256, 0, s, m.ne, m.be.p, m.v2b.p, vbox, g,
+               pbox + 6 * (size_t)m.nf, cnt + m.nf);
+    IDP_LAUNCH(k_zero_u32, 1, 64, 0, s, cnt + n, 1);
     excl_scan(s, w.tmp, cnt, off, n + 1);
-    unsigned total = read_u(s, off + n);
-    if (total == 0) return 0;
-    unsigned* k0 = w.k0.ensure(total, s);
-    int* v0 = w.v0.ensure(total, s);
-    IDP_LAUNCH(k_emit_cells, grid_for(n, 256), 256, 0, s, n, w.pbox.p, off, g, k0, v0);
-    unsigned* k1 = w.k1.ensure(total, s);
-    int* v1 = w.v1.ensure(total, s);
-    GridDev gh;
-    IDP_HIP(hipMemcpyAsync(&gh, g, sizeof(gh), hipMemcpyDeviceToHost, s));
+    struct {
+        unsigned total;
+        GridDev g;
+    } h;
+    IDP_HIP(hipMemcpyAsync(&h.total, off + n, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    IDP_HIP(hipMemcpyAsync(&h.g, g, sizeof(GridDev), hipMemcpyDeviceToHost, s));
     IDP_HIP(hipStreamSynchronize(s));
-    unsigned long long maxkey = (unsigned long long)gh.cnt[0] * gh.cnt[1] * gh.cnt[2];
-    sort_pairs(s, w.tmp, k0, k1, v0, v1, total, bits_for(maxkey));
-    return total;
+    CellTable T{};
+    T.n = h.total;
+    if (h.total == 0) return T;
+    unsigned* k0 = w.k0.ensure(h.total, s);
+    int* v0 = w.v0.ensure(h.total, s);
+    IDP_LAUNCH(k_emit_cells, grid_for(n, 256), 256, 0, s, n, m.nf, pbox, off, g, k0, v0);
+    unsigned* k1 = w.k1.ensure(h.total, s);
+    int* v1 = w.v1.ensure(h.total, s);
+    const unsigned long long nkeys = 2ull * h.g.cnt[0] * h.g.cnt[1] * h.g.cnt[2];
+    sort_pairs(s, w.tmp, k0, k1, v0, v1, h.total, bits_for(nkeys));
+    T.keys = k1;
+    T.vals = v1;
+    if (nkeys <= DENSE_MAX) {
+        if (w.cstart.cap < nkeys || !w.cstart.p) {
+            w.cstart.ensure(nkeys, s);
+            w.cend.ensure(nkeys, s);
+            IDP_LAUNCH(k_fill_u32, grid_for((long)w.cstart.cap, 256), 256, 0, s, w.cstart.cap, w.cstart.p, 0u);
+            IDP_LAUNCH(k_fill_u32, grid_for((long)w.cend.cap, 256), 256, 0, s, w.cend.cap, w.cend.p, 0u);
+        }
+        IDP_LAUNCH(k_cell_bounds, grid_for(h.total, 256), 256, 0, s, h.total, k1, w.cstart.p, w.cend.p, 0);
+        T.start = w.cstart.p;
+        T.end = w.cend.p;
+    }
+    return T;
+}
+
+void clear_cells(ContactWork& w, hipStream_t s, const CellTable& T)
+{
+    if (T.start && T.n)
+        IDP_LAUNCH(k_cell_bounds, grid_for(T.n, 256), 256, 0, s, T.n, T.keys, w.cstart.p, w.cend.p, 1);
 }
 
 void grid_from(ContactWork& w, hipStream_t s, const MeshDev& m, const double* X, const int* idx, long nidx,
-               const double* p, double alpha, double pad)
+               const double* p, const double* alpha_p, double pad, const double* ep, int nep)
 {
-    // mean boundary edge length (chunked sum) and min/max over the given vertices
-    int nep = grid_for(m.ne, 256);
-    double* ep = w.dparts.ensure(nep + 1, s);
-    IDP_LAUNCH(k_edge_len_parts, grid_for(nep, 64), 64, 0, s, m.ne, (const int2*)m.be.p, X, ep);
     int nmm = grid_for(nidx, RB);
     double* mm = w.mmparts.ensure(6 * (size_t)nmm + 6, s);
-    IDP_LAUNCH(k_minmax_parts, nmm, RB, 0, s, nidx, idx, X, p, alpha, mm);
+    IDP_LAUNCH(k_minmax_parts, nmm, RB, 0, s, nidx, idx, X, p, alpha_p, mm);
     IDP_LAUNCH(k_minmax_final, 1, RB, 0, s, nmm, mm, mm + 6 * (size_t)nmm);
     GridDev* g = w.grid.ensure(1, s);
     IDP_LAUNCH(k_grid_setup, 1, 1, 0, s, g, ep, nep, (long)m.ne, mm + 6 * (size_t)nmm, pad);
@@ -701,18 +880,25 @@ void grid_from(ContactWork& w, hipStream_t s, const MeshDev& m, const double* X,
 
 }  // namespace
 
-// Constraint set at X (device); result in w.cs (count returned)
+// Constraint set at X (device); result in w.cs (count returned).  Order:
+// PT pass tuples by (vertex, triangle), EE pass tuples by (edge, edge), then
+// the merged PP/PE entries in map order.
 long contact_constraint_set(ContactWork& w, hipStream_t s, const MeshDev& m, const double* X, double dHat2,
                             double thickness)
 {
     const double r = sqrt(dHat2) + thickness;  // IPC.h:51-52
     const double r2 = r * r;
+    w.ncs = 0;
     if (m.nb == 0) return 0;
-    grid_from(w, s, m, X, nullptr, m.nV, nullptr, 0.0, 0.0);
+    const int nep = grid_for(m.ne, 256);
+    double* ep = w.dparts.ensure(nep + 1, s);
+    IDP_LAUNCH(k_edge_len_parts, nep, 256, 0, s, m.ne, (const int2*)m.be.p, X, ep);
+    grid_from(w, s, m, X, nullptr, m.nV, nullptr, nullptr, 0.0, ep, nep);
     const GridDev* g = w.grid.p;
     int* vbox = w.vbox.ensure(6 * (size_t)m.nb, s);
-    IDP_LAUNCH(k_vertex_boxes, grid_for(m.nb, 256), 256, 0, s, m.nb, m.bn.p, X, (const double*)nullptr, 0.0, 0.0,
-               g, vbox);
+    IDP_LAUNCH(k_vertex_boxes, grid_for(m.nb, 256), 256, 0, s, m.nb, m.bn.p, X, (const double*)nullptr,
+               (const double*)nullptr, 0.0, g, vbox);
+    CellTable T = build_cells(w, s, m, g, vbox);
 
     CSArgs A;
     A.X = X;
@@ -721,98 +907,66 @@ long contact_constraint_set(ContactWork& w, hipStream_t s, const MeshDev& m, con
     A.bn = m.bn.p;
     A.be = (const int2*)m.be.p;
     A.bt = (const int3*)m.bt.p;
-    A.v2b = m.v2b.p;
+    A.pbox = w.pbox.p;
     A.g = g;
+    A.nf = m.nf;
     A.r = r;
     A.r2 = r2;
-
-    // ---- PT: triangles in cells, query per boundary vertex
-    unsigned nPTp = 0, nPTd = 0, nEEp = 0, nEEd = 0;
-    DBuf<int4>& pass = w.pass;
-    DBuf<int4>& dup = w.dup;
-    unsigned* cntP = w.cP.ensure(m.nb + m.ne + 2, s);
-    unsigned* cntD = w.cD.ensure(m.nb + m.ne + 2, s);
-    unsigned* offP = w.oP.ensure(m.nb + m.ne + 2, s);
-    unsigned* offD = w.oD.ensure(m.nb + m.ne + 2, s);
-    {
-        w.pbox.ensure(6 * (size_t)std::max(m.nf, 1), s);
-        w.u0.ensure(m.nf + 1, s);
-        IDP_LAUNCH(k_prim_boxes<3>, grid_for(m.nf, 256), 256, 0, s, m.nf, m.bt.p, m.v2b.p, vbox, g, w.pbox.p, w.u0.p);
-        unsigned ncell = build_cells(w, s, m.nf, g);
-        A.ckeys = w.k1.p;
-        A.cvals = w.v1.p;
-        A.ncell = ncell;
-        A.pbox = w.pbox.p;
-        IDP_LAUNCH(k_cs_pt, grid_for(m.nb, 128), 128, 0, s, m.nb, A, (const unsigned*)nullptr,
-                   (const unsigned*)nullptr, (int4*)nullptr, (int4*)nullptr, cntP, cntD);
-        IDP_HIP(hipMemsetAsync(cntP + m.nb, 0, sizeof(unsigned), s));
-        IDP_HIP(hipMemsetAsync(cntD + m.nb, 0, sizeof(unsigned), s));
-        excl_scan(s, w.tmp, cntP, offP, m.nb + 1);
-        excl_scan(s, w.tmp, cntD, offD, m.nb + 1);
-        nPTp = read_u(s, offP + m.nb);
-        nPTd = read_u(s, offD + m.nb);
+    CSOut O;
+    O.nf = (unsigned long long)std::max(m.nf, 1);
+    O.ne = (unsigned long long)std::max(m.ne, 1);
+    O.V = (unsigned long long)m.nV;
+    unsigned* cnt = w.cnt.ensure(4, s);
+    if (w.cap_pass == 0) w.cap_pass = 4u * (m.nb + m.ne) + 1024;
+    if (w.cap_dup == 0) w.cap_dup = 4u * (m.nb + m.ne) + 1024;
+    unsigned hc[2] = {0, 0};
+    if (T.n) {
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            O.cnt = cnt;
+            O.pkey = w.pk0.ensure(w.cap_pass, s);
+            O.pval = w.pass.ensure(w.cap_pass, s);
+            O.dkey = w.dk0.ensure(w.cap_dup, s);
+            O.cap_p = (unsigned)std::min(w.pk0.cap, w.pass.cap);
+            O.cap_d = (unsigned)w.dk0.cap;
+            IDP_LAUNCH(k_zero_u32, 1, 64, 0, s, cnt, 2);
+            IDP_LAUNCH(k_cs_walk<WALK_G>, walk_blocks((long)m.nb + m.ne), 256, 0, s, m.nb, m.ne, A, T, O);
+            IDP_HIP(hipMemcpyAsync(hc, cnt, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+            IDP_HIP(hipStreamSynchronize(s));
+            if (hc[0] <= O.cap_p && hc[1] <= O.cap_d) break;
+            w.cap_pass = std::max<size_t>(w.cap_pass, hc[0] + hc[0] / 4 + 1024);
+            w.cap_dup = std::max<size_t>(w.cap_dup, hc[1] + hc[1] / 4 + 1024);
+            if (attempt == 1) throw HipError(IDP_E_CAPACITY, "constraint-set output overflow");
+        }
+        clear_cells(w, s, T);
     }
-    // output layout: [PT pass | EE pass] and [PT dup | EE dup]
-    // (PT results are written after EE counts are known)
-    unsigned* cntP2 = cntP + m.nb + 1;
-    unsigned* cntD2 = cntD + m.nb + 1;
-    unsigned* offP2 = offP + m.nb + 1;
-    unsigned* offD2 = offD + m.nb + 1;
-    CSArgs B = A;
-    {
-        // write PT results now (buffers sized after the count)
-        pass.ensure(nPTp + 1, s);
-        dup.ensure(nPTd + 1, s);
-        IDP_LAUNCH(k_cs_pt, grid_for(m.nb, 128), 128, 0, s, m.nb, A, offP, offD, pass.p, dup.p, cntP, cntD);
-        // ---- EE: edges in cells, query per edge
-        w.pbox.ensure(6 * (size_t)std::max(m.ne, 1), s);
-        w.u0.ensure(m.ne + 1, s);
-        IDP_LAUNCH(k_prim_boxes<2>, grid_for(m.ne, 256), 256, 0, s, m.ne, m.be.p, m.v2b.p, vbox, g, w.pbox.p,
-                   w.u0.p);
-        unsigned ncell = build_cells(w, s, m.ne, g);
-        B.ckeys = w.k1.p;
-        B.cvals = w.v1.p;
-        B.ncell = ncell;
-        B.pbox = w.pbox.p;
-        IDP_LAUNCH(k_cs_ee, grid_for(m.ne, 128), 128, 0, s, m.ne, B, (const unsigned*)nullptr,
-                   (const unsigned*)nullptr, (int4*)nullptr, (int4*)nullptr, cntP2, cntD2);
-        IDP_HIP(hipMemsetAsync(cntP2 + m.ne, 0, sizeof(unsigned), s));
-        IDP_HIP(hipMemsetAsync(cntD2 + m.ne, 0, sizeof(unsigned), s));
-        excl_scan(s, w.tmp, cntP2, offP2, m.ne + 1);
-        excl_scan(s, w.tmp, cntD2, offD2, m.ne + 1);
-        nEEp = read_u(s, offP2 + m.ne);
-        nEEd = read_u(s, offD2 + m.ne);
+    const unsigned npass = hc[0], nd = hc[1];
+    const unsigned long long V = O.V;
+    // ---- order the pass tuples
+    if (npass) {
+        unsigned long long* pk1 = w.pk1.ensure(npass, s);
+        int4* pv1 = w.dup.ensure(npass, s);
+        const unsigned long long maxkey = V * O.nf + O.ne * O.ne;
+        sort_pairs(s, w.tmp, w.pk0.p, pk1, w.pass.p, pv1, npass, bits_for(maxkey));
     }
-    // the PT pass tuples must stay where they are; grow with keep
-    pass.ensure(nPTp + nEEp + 1, s, true);
-    dup.ensure(nPTd + nEEd + 1, s, true);
-    IDP_LAUNCH(k_cs_ee, grid_for(m.ne, 128), 128, 0, s, m.ne, B, offP2, offD2, pass.p + nPTp, dup.p + nPTd, cntP2,
-               cntD2);
-
-    // ---- merge duplicates
-    unsigned nd = nPTd + nEEd;
+    // ---- merge duplicates (IPC.h:599-654)
     unsigned nmerged = 0;
-    unsigned long long V = (unsigned long long)m.nV;
     if (nd) {
-        unsigned long long* k0 = w.dk0.ensure(nd, s);
         unsigned long long* k1 = w.dk1.ensure(nd, s);
-        IDP_LAUNCH(k_dup_keys, grid_for(nd, 256), 256, 0, s, nd, dup.p, V, k0);
-        sort_keys(s, w.tmp, k0, k1, nd, bits_for(V * V * (V + 1)));
+        sort_keys(s, w.tmp, w.dk0.p, k1, nd, bits_for(V * V * (V + 1)));
         unsigned* head = w.u0.ensure(nd + 1, s);
         unsigned* hs = w.u1.ensure(nd + 1, s);
         IDP_LAUNCH(k_dup_heads, grid_for(nd, 256), 256, 0, s, nd, k1, head);
-        IDP_HIP(hipMemsetAsync(head + nd, 0, sizeof(unsigned), s));
+        IDP_LAUNCH(k_zero_u32, 1, 64, 0, s, head + nd, 1);
         excl_scan(s, w.tmp, head, hs, nd + 1);
         nmerged = read_u(s, hs + nd);
         w.merged.ensure(nmerged + 1, s);
         IDP_LAUNCH(k_dup_write, grid_for(nd, 256), 256, 0, s, nd, k1, hs, V, w.merged.p);
     }
-    long total = (long)nPTp + nEEp + nmerged;
+    long total = (long)npass + nmerged;
     int4* cs = w.cs.ensure(total + 1, s);
-    if (nPTp + nEEp)
-        IDP_HIP(hipMemcpyAsync(cs, pass.p, sizeof(int4) * (nPTp + nEEp), hipMemcpyDeviceToDevice, s));
+    if (npass) IDP_HIP(hipMemcpyAsync(cs, w.dup.p, sizeof(int4) * npass, hipMemcpyDeviceToDevice, s));
     if (nmerged)
-        IDP_HIP(hipMemcpyAsync(cs + nPTp + nEEp, w.merged.p, sizeof(int4) * nmerged, hipMemcpyDeviceToDevice, s));
+        IDP_HIP(hipMemcpyAsync(cs + npass, w.merged.p, sizeof(int4) * nmerged, hipMemcpyDeviceToDevice, s));
     w.ncs = total;
     return total;
 }
@@ -821,32 +975,26 @@ long contact_constraint_set(ContactWork& w, hipStream_t s, const MeshDev& m, con
 double contact_ccd(ContactWork& w, hipStream_t s, const MeshDev& m, const double* X, const double* P,
                    double thickness, double alpha_in, long* ncand_out)
 {
+    if (ncand_out) *ncand_out = 0;
     if (m.nb == 0) return alpha_in;
-    // span cap (needs mean edge length and mean |p|)
-    int nep = grid_for(m.ne, 256);
-    int npp = grid_for(3L * m.nb, 256);
+    // span cap (mean edge length and mean |p|), kept on the device
+    const int nep = grid_for(m.ne, 256);
+    const int npp = grid_for(3L * m.nb, 256);
     double* ep = w.dparts.ensure(nep + npp + 2, s);
     double* pp = ep + nep + 1;
-    IDP_LAUNCH(k_edge_len_parts, grid_for(nep, 64), 64, 0, s, m.ne, (const int2*)m.be.p, X, ep);
-    IDP_LAUNCH(k_psize_parts, grid_for(npp, 64), 64, 0, s, m.nb, m.bn.p, P, pp);
+    IDP_LAUNCH(k_edge_len_parts, nep, 256, 0, s, m.ne, (const int2*)m.be.p, X, ep);
+    IDP_LAUNCH(k_psize_parts, npp, 256, 0, s, m.nb, m.bn.p, P, pp);
     double* dalpha = w.scal.ensure(4, s);
-    IDP_LAUNCH(k_ccd_cap, 1, 1, 0, s, ep, nep, (long)m.ne, pp, npp, (long)m.nb, alpha_in, dalpha);
-    double alpha;
-    IDP_HIP(hipMemcpyAsync(&alpha, dalpha, sizeof(double), hipMemcpyDeviceToHost, s));
-    IDP_HIP(hipStreamSynchronize(s));
+    unsigned long long* amin = (unsigned long long*)w.scal.p + 1;
+    unsigned* cnt = w.cnt.ensure(4, s);
+    IDP_LAUNCH(k_ccd_cap, 1, 1, 0, s, ep, nep, (long)m.ne, pp, npp, (long)m.nb, alpha_in, dalpha, amin, cnt);
 
-    // swept grid over boundary nodes at [x, x + alpha p] -/+ th/2
-    // (SPATIAL_HASH.h:484-511); mean edge length recomputed inside grid_from
-    int nmm = grid_for(m.nb, RB);
-    double* mm = w.mmparts.ensure(6 * (size_t)nmm + 6, s);
-    IDP_LAUNCH(k_minmax_parts, nmm, RB, 0, s, (long)m.nb, m.bn.p, X, P, alpha, mm);
-    IDP_LAUNCH(k_minmax_final, 1, RB, 0, s, nmm, mm, mm + 6 * (size_t)nmm);
-    GridDev* g = w.grid.ensure(1, s);
-    double* ep2 = w.dparts2.ensure(nep + 1, s);
-    IDP_LAUNCH(k_edge_len_parts, grid_for(nep, 64), 64, 0, s, m.ne, (const int2*)m.be.p, X, ep2);
-    IDP_LAUNCH(k_grid_setup, 1, 1, 0, s, g, ep2, nep, (long)m.ne, mm + 6 * (size_t)nmm, thickness / 2);
+    // swept grid over boundary nodes at [x, x + alpha p] -/+ th/2 (SPATIAL_HASH.h:484-511)
+    grid_from(w, s, m, X, m.bn.p, m.nb, P, dalpha, thickness / 2, ep, nep);
+    const GridDev* g = w.grid.p;
     int* vbox = w.vbox.ensure(6 * (size_t)m.nb, s);
-    IDP_LAUNCH(k_vertex_boxes, grid_for(m.nb, 256), 256, 0, s, m.nb, m.bn.p, X, P, alpha, thickness / 2, g, vbox);
+    IDP_LAUNCH(k_vertex_boxes, grid_for(m.nb, 256), 256, 0, s, m.nb, m.bn.p, X, P, dalpha, thickness / 2, g, vbox);
+    CellTable T = build_cells(w, s, m, g, vbox);
 
     CCDArgs A;
     A.X = X;
@@ -856,53 +1004,39 @@ double contact_ccd(ContactWork& w, hipStream_t s, const MeshDev& m, const double
     A.be = (const int2*)m.be.p;
     A.bt = (const int3*)m.bt.p;
     A.vbox = vbox;
+    A.pbox = w.pbox.p;
     A.g = g;
+    A.nf = m.nf;
     A.th = thickness;
-
-    unsigned* cnt = w.cP.ensure(std::max(m.nb, m.ne) + 2, s);
-    unsigned* off = w.oP.ensure(std::max(m.nb, m.ne) + 2, s);
-    // PT candidates
-    w.pbox.ensure(6 * (size_t)std::max(m.nf, 1), s);
-    w.u0.ensure(m.nf + 1, s);
-    IDP_LAUNCH(k_prim_boxes<3>, grid_for(m.nf, 256), 256, 0, s, m.nf, m.bt.p, m.v2b.p, vbox, g, w.pbox.p, w.u0.p);
-    unsigned ncell = build_cells(w, s, m.nf, g);
-    A.ckeys = w.k1.p;
-    A.cvals = w.v1.p;
-    A.ncell = ncell;
-    A.pbox = w.pbox.p;
-    IDP_LAUNCH(k_ccd_pt_cand, grid_for(m.nb, 128), 128, 0, s, m.nb, A, (const unsigned*)nullptr, (int2*)nullptr, cnt);
-    IDP_HIP(hipMemsetAsync(cnt + m.nb, 0, sizeof(unsigned), s));
-    excl_scan(s, w.tmp, cnt, off, m.nb + 1);
-    unsigned npt = read_u(s, off + m.nb);
-    int2* cand = w.cand.ensure(npt + 1, s);
-    IDP_LAUNCH(k_ccd_pt_cand, grid_for(m.nb, 128), 128, 0, s, m.nb, A, off, cand, cnt);
-    // EE candidates
-    w.pbox.ensure(6 * (size_t)std::max(m.ne, 1), s);
-    w.u0.ensure(m.ne + 1, s);
-    IDP_LAUNCH(k_prim_boxes<2>, grid_for(m.ne, 256), 256, 0, s, m.ne, m.be.p, m.v2b.p, vbox, g, w.pbox.p, w.u0.p);
-    ncell = build_cells(w, s, m.ne, g);
-    A.ckeys = w.k1.p;
-    A.cvals = w.v1.p;
-    A.ncell = ncell;
-    A.pbox = w.pbox.p;
-    IDP_LAUNCH(k_ccd_ee_cand, grid_for(m.ne, 128), 128, 0, s, m.ne, A, (const unsigned*)nullptr, (int2*)nullptr, cnt);
-    IDP_HIP(hipMemsetAsync(cnt + m.ne, 0, sizeof(unsigned), s));
-    excl_scan(s, w.tmp, cnt, off, m.ne + 1);
-    unsigned nee = read_u(s, off + m.ne);
-    cand = w.cand.ensure(npt + nee + 1, s, true);
-    IDP_LAUNCH(k_ccd_ee_cand, grid_for(m.ne, 128), 128, 0, s, m.ne, A, off, cand + npt, cnt);
-    // ACCD
-    unsigned long long* amin = (unsigned long long*)w.scal.p + 1;
-    unsigned long long abits;
-    std::memcpy(&abits, &alpha, sizeof(abits));
-    IDP_HIP(hipMemcpyAsync(amin, &abits, sizeof(abits), hipMemcpyHostToDevice, s));
-    const int tok = w.prof ? w.prof->begin(IDP_K_CCD) : -1;
-    IDP_LAUNCH(k_accd, grid_for(npt + nee, 64), 64, 0, s, npt, nee, cand, A, 0.1, amin);
-    if (w.prof) w.prof->end(IDP_K_CCD, tok, (double)npt + nee);
-    IDP_HIP(hipMemcpyAsync(&abits, amin, sizeof(abits), hipMemcpyDeviceToHost, s));
-    IDP_HIP(hipStreamSynchronize(s));
-    std::memcpy(&alpha, &abits, sizeof(alpha));
-    if (ncand_out) *ncand_out = (long)npt + nee;
+    if (w.cap_cand == 0) w.cap_cand = 16u * (m.nb + m.ne) + 4096;
+    struct {
+        unsigned long long abits;
+        unsigned n[2];
+    } h{0, {0, 0}};
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        int2* cand = w.cand.ensure(w.cap_cand, s);
+        const unsigned cap = (unsigned)w.cand.cap;
+        if (attempt) IDP_LAUNCH(k_reset_ccd, 1, 1, 0, s, dalpha, amin, cnt);
+        if (T.n)
+            IDP_LAUNCH(k_ccd_walk<WALK_G>, walk_blocks((long)m.nb + m.ne), 256, 0, s, m.nb, m.ne, A, T, cnt, cand,
+                       cap);
+        const int tok = w.prof ? w.prof->begin(IDP_K_CCD) : -1;
+        // a work queue needs no more lanes than the chip holds at once
+        IDP_LAUNCH(k_accd, 2048, 256, 0, s, cnt, cap, cand, A, 0.1, amin, cnt + 2);
+        if (w.prof) w.prof->end(IDP_K_CCD, tok);
+        IDP_HIP(hipMemcpyAsync(&h.abits, amin, sizeof(h.abits), hipMemcpyDeviceToHost, s));
+        IDP_HIP(hipMemcpyAsync(h.n, cnt, sizeof(h.n), hipMemcpyDeviceToHost, s));
+        IDP_HIP(hipStreamSynchronize(s));
+        const size_t total = (size_t)h.n[0] + h.n[1];
+        if (total <= cap) break;
+        w.cap_cand = std::max<size_t>(w.cap_cand, total + total / 4 + 4096);
+        if (attempt == 1) throw HipError(IDP_E_CAPACITY, "CCD candidate overflow");
+    }
+    clear_cells(w, s, T);
+    if (w.prof) w.prof->add_work(IDP_K_CCD, (double)h.n[0] + h.n[1]);
+    if (ncand_out) *ncand_out = (long)h.n[0] + h.n[1];
+    double alpha;
+    std::memcpy(&alpha, &h.abits, sizeof(alpha));
     return alpha;
 }
 

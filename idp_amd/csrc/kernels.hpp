@@ -45,8 +45,10 @@ struct ContactWork {
     DBuf<unsigned> u0, u1, cP, cD, oP, oD, k0, k1;
     DBuf<int> v0, v1;
     DBuf<int4> pass, dup, merged, cs;
-    DBuf<unsigned long long> dk0, dk1;
+    DBuf<unsigned long long> dk0, dk1, pk0, pk1;
+    DBuf<unsigned> cstart, cend, cnt;  // dense cell table, atomic counters
     DBuf<int2> cand;
+    size_t cap_pass = 0, cap_dup = 0, cap_cand = 0;
     long ncs = 0;
     Profiler* prof = nullptr;
 };

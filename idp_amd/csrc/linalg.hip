@@ -15,7 +15,6 @@
 #include "kernels.hpp"
 #include "linalg.hpp"
 
-#include <hip/hip_cooperative_groups.h>
 #include <hipcub/hipcub.hpp>
 
 namespace idp {
@@ -237,165 +236,21 @@ __global__ void k_block_jacobi(int nV, const int* __restrict__ ediag, const doub
     o[8] = (a[0] * a[4] - a[1] * a[3]) * id;
 }
 
-__device__ inline void row_spmv(int v, const int* __restrict__ rp, const int* __restrict__ col,
-                                const double* __restrict__ val, const double* __restrict__ x, double& y0,
-                                double& y1, double& y2)
-{
-    for (int k = rp[v]; k < rp[v + 1]; ++k) {
-        const double* b = val + 9 * (size_t)k;
-        const double* xc = x + 3 * (size_t)col[k];
-        const double x0 = xc[0], x1 = xc[1], x2 = xc[2];
-        y0 += b[0] * x0 + b[1] * x1 + b[2] * x2;
-        y1 += b[3] * x0 + b[4] * x1 + b[5] * x2;
-        y2 += b[6] * x0 + b[7] * x1 + b[8] * x2;
-    }
-}
-
-// q = (E + C) p, partial sums of p.q
-__global__ void __launch_bounds__(RB) k_spmv_dot(int nV, const int* __restrict__ erp, const int* __restrict__ ecol,
-                                                 const double* __restrict__ eval, const int* __restrict__ crp,
-                                                 const int* __restrict__ ccol, const double* __restrict__ cval,
-                                                 const double* __restrict__ p, double* __restrict__ q,
-                                                 double* __restrict__ part)
-{
-    __shared__ double sh[RB];
-    int v = blockIdx.x * RB + threadIdx.x;
-    double d = 0.0;
-    if (v < nV) {
-        double y0 = 0.0, y1 = 0.0, y2 = 0.0;
-        row_spmv(v, erp, ecol, eval, p, y0, y1, y2);
-        if (crp) row_spmv(v, crp, ccol, cval, p, y0, y1, y2);
-        q[3 * v] = y0;
-        q[3 * v + 1] = y1;
-        q[3 * v + 2] = y2;
-        d = p[3 * v] * y0 + p[3 * v + 1] * y1 + p[3 * v + 2] * y2;
-    }
-    double s = block_sum<RB>(d, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
-
-// scalars layout: [0] rz (even iters) [1] rz (odd) [2] rr [3] breakdown flag
-// alpha = rz / (p.q); x += alpha p; r -= alpha q; z = P r; partials of r.z, r.r
-__global__ void __launch_bounds__(RB) k_pcg_update(int nV, int npart, const double* __restrict__ part_pq,
-                                                   const double* __restrict__ scal, int par,
-                                                   const double* __restrict__ p, const double* __restrict__ q,
-                                                   const double* __restrict__ Pinv, double* __restrict__ x,
-                                                   double* __restrict__ r, double* __restrict__ z,
-                                                   double* __restrict__ part_rz, double* __restrict__ part_rr)
-{
-    __shared__ double sh[RB];
-    const double pq = sum_partials_block(part_pq, npart, sh);
-    const double rz = scal[par];
-    const double alpha = (pq > 0) ? rz / pq : 0.0;
-    int v = blockIdx.x * RB + threadIdx.x;
-    double drz = 0.0, drr = 0.0;
-    if (v < nV) {
-        double rv[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            x[3 * v + i] += alpha * p[3 * v + i];
-            rv[i] = r[3 * v + i] - alpha * q[3 * v + i];
-            r[3 * v + i] = rv[i];
-        }
-        const double* P = Pinv + 9 * (size_t)v;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            double zi = P[3 * i] * rv[0] + P[3 * i + 1] * rv[1] + P[3 * i + 2] * rv[2];
-            z[3 * v + i] = zi;
-            drz += rv[i] * zi;
-            drr += rv[i] * rv[i];
-        }
-    }
-    double a = block_sum<RB>(drz, sh);
-    double b = block_sum<RB>(drr, sh);
-    if (threadIdx.x == 0) {
-        part_rz[blockIdx.x] = a;
-        part_rr[blockIdx.x] = b;
-    }
-}
-
-// beta = rz_new / rz_old; p = z + beta p; block 0 stores rz_new, rr and the
-// breakdown flag (p.q <= 0)
-__global__ void __launch_bounds__(RB) k_pcg_direction(int nV, int npart, const double* __restrict__ part_rz,
-                                                      const double* __restrict__ part_rr,
-                                                      const double* __restrict__ part_pq, double* __restrict__ scal,
-                                                      int par, const double* __restrict__ z, double* __restrict__ p)
-{
-    __shared__ double sh[RB];
-    const double rzn = sum_partials_block(part_rz, npart, sh);
-    const double rzo = scal[par];
-    const double beta = rzo != 0.0 ? rzn / rzo : 0.0;
-    int v = blockIdx.x * RB + threadIdx.x;
-    if (v < nV)
-#pragma unroll
-        for (int i = 0; i < 3; ++i) p[3 * v + i] = z[3 * v + i] + beta * p[3 * v + i];
-    if (blockIdx.x == 0) {
-        const double rr = sum_partials_block(part_rr, npart, sh);
-        const double pq = sum_partials_block(part_pq, npart, sh);
-        if (threadIdx.x == 0) {
-            scal[par ^ 1] = rzn;
-            scal[2] = rr;
-            if (!(pq > 0)) scal[3] = 1.0;
-        }
-    }
-}
-
-// initial: x = 0, r = b, z = P b, p = z, partials of r.z and r.r
-__global__ void __launch_bounds__(RB) k_pcg_init(int nV, const double* __restrict__ b,
-                                                 const double* __restrict__ Pinv, double* __restrict__ x,
-                                                 double* __restrict__ r, double* __restrict__ z,
-                                                 double* __restrict__ p, double* __restrict__ part_rz,
-                                                 double* __restrict__ part_rr)
-{
-    __shared__ double sh[RB];
-    int v = blockIdx.x * RB + threadIdx.x;
-    double drz = 0.0, drr = 0.0;
-    if (v < nV) {
-        double rv[3] = {b[3 * v], b[3 * v + 1], b[3 * v + 2]};
-        const double* P = Pinv + 9 * (size_t)v;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            x[3 * v + i] = 0.0;
-            r[3 * v + i] = rv[i];
-            double zi = P[3 * i] * rv[0] + P[3 * i + 1] * rv[1] + P[3 * i + 2] * rv[2];
-            z[3 * v + i] = zi;
-            p[3 * v + i] = zi;
-            drz += rv[i] * zi;
-            drr += rv[i] * rv[i];
-        }
-    }
-    double a = block_sum<RB>(drz, sh);
-    double c = block_sum<RB>(drr, sh);
-    if (threadIdx.x == 0) {
-        part_rz[blockIdx.x] = a;
-        part_rr[blockIdx.x] = c;
-    }
-}
-
-__global__ void __launch_bounds__(RB) k_pcg_init_scalars(int npart, const double* __restrict__ part_rz,
-                                                         const double* __restrict__ part_rr,
-                                                         double* __restrict__ scal)
-{
-    __shared__ double sh[RB];
-    const double rz = sum_partials_block(part_rz, npart, sh);
-    const double rr = sum_partials_block(part_rr, npart, sh);
-    if (threadIdx.x == 0) {
-        scal[0] = rz;
-        scal[1] = rz;
-        scal[2] = rr;
-        scal[3] = 0.0;
-        scal[4] = rr;  // |b|^2
-    }
-}
-
-
 // ---------------------------------------------------------------------------
-// Persistent PCG: the whole iteration loop in one cooperative launch.
-// Two grid barriers per iteration: the SpMV of the new direction reads
-// p_new_j = z_j + beta p_old_j of neighbouring rows on the fly (p ping-pongs
-// between two buffers), so q = A p_new is exact and no residual drift builds
-// up.  Every block re-reduces the per-block partials in the same order, so
-// all blocks take identical, deterministic decisions.
+// Block-Jacobi PCG as two kernels per iteration, enqueued in chunks with no
+// host synchronisation inside a chunk.  A grid-wide barrier inside one
+// persistent kernel costs a device-scope release/acquire per block (the
+// eight XCD L2s are not coherent for ordinary device memory), which grows
+// with the block count; a kernel boundary does that once.
+//  * k_pcg_spmv(it): every block re-reduces the previous update's partials
+//    (r.z, r.r) in a fixed order, decides convergence identically, forms
+//    p_it = z + beta p_(it-1) on the fly and computes q = A p_it with a group
+//    of G lanes per block row (xor-butterfly row reduction), plus partial p.q.
+//  * k_pcg_update(it): alpha = r.z / p.q; x += alpha p; r -= alpha q; z = P r;
+//    partials of the new r.z and r.r.
+// p and the r.z / r.r partials ping-pong on the iteration parity.  When the
+// residual test holds (or on breakdown), the deciding kernel sets a flag and
+// every later kernel returns at once, so the host may enqueue past the end.
 // ---------------------------------------------------------------------------
 struct PcgArgs {
     int nV;
@@ -410,144 +265,165 @@ struct PcgArgs {
     double* x;
     double* r;
     double* z;
-    double* p0;
-    double* p1;
+    double* pbuf;     // 2 x 3V
     double* q;
-    double* part_a;
-    double* part_b;
-    double* part_c;
-    double* out;  // [0] iterations, [1] |r|^2, [2] |b|^2, [3] breakdown
+    double* part_pq;  // nbA
+    double* part_rz;  // 2 x nbB
+    double* part_rr;  // 2 x nbB
+    int nbA, nbB;
+    int* flag;        // 0 running, 1 stopped (converged or maxit), 2 breakdown
+    double* scal;     // [0] |b|^2, [1] final |r|^2, [2] iterations
     double rtol2;
     int maxit;
 };
 
-__device__ inline void row_spmv_dir(int v, const int* __restrict__ rp, const int* __restrict__ col,
-                                    const double* __restrict__ val, const double* __restrict__ z,
-                                    const double* __restrict__ po, double beta, double& y0, double& y1, double& y2)
+constexpr int PCG_SPMV_BLOCK = 1024;
+
+template <int G>
+__device__ inline void group_row_spmv(int v, int lane, const int* __restrict__ rp, const int* __restrict__ col,
+                                      const double* __restrict__ val, const double* __restrict__ z,
+                                      const double* __restrict__ po, double beta, double& y0, double& y1, double& y2)
 {
-    for (int k = rp[v]; k < rp[v + 1]; ++k) {
+    const int k1 = rp[v + 1];
+    for (int k = rp[v] + lane; k < k1; k += G) {
         const double* bk = val + 9 * (size_t)k;
         const size_t c = 3 * (size_t)col[k];
-        const double x0 = z[c] + beta * po[c], x1 = z[c + 1] + beta * po[c + 1], x2 = z[c + 2] + beta * po[c + 2];
+        double x0 = z[c], x1 = z[c + 1], x2 = z[c + 2];
+        if (po) {
+            x0 += beta * po[c];
+            x1 += beta * po[c + 1];
+            x2 += beta * po[c + 2];
+        }
         y0 += bk[0] * x0 + bk[1] * x1 + bk[2] * x2;
         y1 += bk[3] * x0 + bk[4] * x1 + bk[5] * x2;
         y2 += bk[6] * x0 + bk[7] * x1 + bk[8] * x2;
     }
 }
 
-__global__ void __launch_bounds__(RB) k_pcg_persistent(PcgArgs A)
+// x = 0, r = b, z = P b, partials of r.z and r.r (parity 0)
+__global__ void __launch_bounds__(RB) k_pcg_init(PcgArgs A)
 {
-    namespace cgr = cooperative_groups;
-    cgr::grid_group grid = cgr::this_grid();
     __shared__ double sh[RB];
-    const int nb = gridDim.x;
-    const int stride = nb * RB;
-    // init: x = 0, r = b, z = P r, p_old = 0 (so p = z with beta = 0)
+    const int v = blockIdx.x * RB + threadIdx.x;
     double drz = 0.0, drr = 0.0;
-    for (int v = blockIdx.x * RB + threadIdx.x; v < A.nV; v += stride) {
-        double rv[3] = {A.b[3 * v], A.b[3 * v + 1], A.b[3 * v + 2]};
+    if (v < A.nV) {
+        const double rv[3] = {A.b[3 * v], A.b[3 * v + 1], A.b[3 * v + 2]};
         const double* P = A.Pinv + 9 * (size_t)v;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             A.x[3 * v + i] = 0.0;
             A.r[3 * v + i] = rv[i];
-            A.p0[3 * v + i] = 0.0;
-            double zi = P[3 * i] * rv[0] + P[3 * i + 1] * rv[1] + P[3 * i + 2] * rv[2];
+            const double zi = P[3 * i] * rv[0] + P[3 * i + 1] * rv[1] + P[3 * i + 2] * rv[2];
             A.z[3 * v + i] = zi;
             drz += rv[i] * zi;
             drr += rv[i] * rv[i];
         }
     }
-    {
-        double a = block_sum<RB>(drz, sh), c = block_sum<RB>(drr, sh);
-        if (threadIdx.x == 0) {
-            A.part_b[blockIdx.x] = a;
-            A.part_c[blockIdx.x] = c;
+    const double a = block_sum<RB>(drz, sh), c = block_sum<RB>(drr, sh);
+    if (threadIdx.x == 0) {
+        A.part_rz[blockIdx.x] = a;
+        A.part_rr[blockIdx.x] = c;
+        if (blockIdx.x == 0) A.flag[0] = 0;
+    }
+}
+
+template <int G>
+__global__ void __launch_bounds__(PCG_SPMV_BLOCK) k_pcg_spmv(PcgArgs A, int it)
+{
+    __shared__ double sh[PCG_SPMV_BLOCK];
+    __shared__ int sflag;
+    if (threadIdx.x == 0) sflag = *(volatile int*)A.flag;
+    __syncthreads();
+    if (sflag != 0) return;  // block-uniform
+    const int par = it & 1;
+    const double rz = sum_partials_block_n<PCG_SPMV_BLOCK>(A.part_rz + par * A.nbB, A.nbB, sh);
+    const double rr = sum_partials_block_n<PCG_SPMV_BLOCK>(A.part_rr + par * A.nbB, A.nbB, sh);
+    const double bb = it == 0 ? rr : A.scal[0];
+    if (!(bb > 0.0) || (it > 0 && rr <= A.rtol2 * bb) || it >= A.maxit) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            A.flag[0] = 1;
+            A.scal[0] = bb;
+            A.scal[1] = rr;
+            A.scal[2] = (double)it;
+        }
+        return;
+    }
+    if (it == 0 && blockIdx.x == 0 && threadIdx.x == 0) A.scal[0] = bb;
+    double beta = 0.0;
+    const double* po = nullptr;
+    if (it > 0) {
+        beta = rz / sum_partials_block_n<PCG_SPMV_BLOCK>(A.part_rz + (par ^ 1) * A.nbB, A.nbB, sh);
+        po = A.pbuf + (size_t)(par ^ 1) * 3 * A.nV;
+    }
+    double* pn = A.pbuf + (size_t)par * 3 * A.nV;
+    const int lane = threadIdx.x & (G - 1);
+    const int v = (blockIdx.x * PCG_SPMV_BLOCK + threadIdx.x) / G;
+    double dpq = 0.0;
+    if (v < A.nV) {
+        double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+        group_row_spmv<G>(v, lane, A.erp, A.ecol, A.eval, A.z, po, beta, y0, y1, y2);
+        if (A.crp) group_row_spmv<G>(v, lane, A.crp, A.ccol, A.cval, A.z, po, beta, y0, y1, y2);
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) {
+            y0 += __shfl_xor(y0, o, G);
+            y1 += __shfl_xor(y1, o, G);
+            y2 += __shfl_xor(y2, o, G);
+        }
+        if (lane < 3) {
+            const double yl = lane == 0 ? y0 : (lane == 1 ? y1 : y2);
+            const size_t i = 3 * (size_t)v + lane;
+            const double pv = po ? A.z[i] + beta * po[i] : A.z[i];
+            pn[i] = pv;
+            A.q[i] = yl;
+            dpq = pv * yl;
         }
     }
-    grid.sync();
-    double rz = sum_partials_block(A.part_b, nb, sh);
-    const double bb = sum_partials_block(A.part_c, nb, sh);
-    const double stop2 = A.rtol2 * bb;
-    double beta = 0.0, rr = bb;
-    int it = 0, status = 0;
-    double* po = A.p0;
-    double* pn = A.p1;
-    if (bb > 0.0) {
-        for (; it < A.maxit; ++it) {
-            // phase 1: p_new = z + beta p_old ; q = A p_new ; partial p.q
-            double dpq = 0.0;
-            for (int v = blockIdx.x * RB + threadIdx.x; v < A.nV; v += stride) {
-                double y0 = 0.0, y1 = 0.0, y2 = 0.0;
-                row_spmv_dir(v, A.erp, A.ecol, A.eval, A.z, po, beta, y0, y1, y2);
-                if (A.crp) row_spmv_dir(v, A.crp, A.ccol, A.cval, A.z, po, beta, y0, y1, y2);
-                const double p0v = A.z[3 * v] + beta * po[3 * v], p1v = A.z[3 * v + 1] + beta * po[3 * v + 1],
-                             p2v = A.z[3 * v + 2] + beta * po[3 * v + 2];
-                pn[3 * v] = p0v;
-                pn[3 * v + 1] = p1v;
-                pn[3 * v + 2] = p2v;
-                A.q[3 * v] = y0;
-                A.q[3 * v + 1] = y1;
-                A.q[3 * v + 2] = y2;
-                dpq += p0v * y0 + p1v * y1 + p2v * y2;
-            }
-            {
-                double a = block_sum<RB>(dpq, sh);
-                if (threadIdx.x == 0) A.part_a[blockIdx.x] = a;
-            }
-            grid.sync();
-            const double pq = sum_partials_block(A.part_a, nb, sh);
-            if (!(pq > 0.0)) {
-                status = 1;
-                break;
-            }
-            const double alpha = rz / pq;
-            // phase 2: x += alpha p; r -= alpha q; z = P r; partials r.z, r.r
-            drz = 0.0;
-            drr = 0.0;
-            for (int v = blockIdx.x * RB + threadIdx.x; v < A.nV; v += stride) {
-                double rv[3];
+    const double a = block_sum<PCG_SPMV_BLOCK>(dpq, sh);
+    if (threadIdx.x == 0) A.part_pq[blockIdx.x] = a;
+}
+
+__global__ void __launch_bounds__(RB) k_pcg_update(PcgArgs A, int it)
+{
+    __shared__ double sh[RB];
+    __shared__ int sflag;
+    if (threadIdx.x == 0) sflag = *(volatile int*)A.flag;
+    __syncthreads();
+    if (sflag != 0) return;  // block-uniform
+    const int par = it & 1;
+    const double pq = sum_partials_block(A.part_pq, A.nbA, sh);
+    const double rz = sum_partials_block(A.part_rz + par * A.nbB, A.nbB, sh);
+    if (!(pq > 0.0)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            A.flag[0] = 2;
+            A.scal[2] = (double)it;
+        }
+        return;
+    }
+    const double alpha = rz / pq;
+    const double* p = A.pbuf + (size_t)par * 3 * A.nV;
+    const int v = blockIdx.x * RB + threadIdx.x;
+    double drz = 0.0, drr = 0.0;
+    if (v < A.nV) {
+        double rv[3];
 #pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    A.x[3 * v + i] += alpha * pn[3 * v + i];
-                    rv[i] = A.r[3 * v + i] - alpha * A.q[3 * v + i];
-                    A.r[3 * v + i] = rv[i];
-                }
-                const double* P = A.Pinv + 9 * (size_t)v;
+        for (int i = 0; i < 3; ++i) {
+            A.x[3 * v + i] += alpha * p[3 * v + i];
+            rv[i] = A.r[3 * v + i] - alpha * A.q[3 * v + i];
+            A.r[3 * v + i] = rv[i];
+        }
+        const double* P = A.Pinv + 9 * (size_t)v;
 #pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    double zi = P[3 * i] * rv[0] + P[3 * i + 1] * rv[1] + P[3 * i + 2] * rv[2];
-                    A.z[3 * v + i] = zi;
-                    drz += rv[i] * zi;
-                    drr += rv[i] * rv[i];
-                }
-            }
-            {
-                double a = block_sum<RB>(drz, sh), c = block_sum<RB>(drr, sh);
-                if (threadIdx.x == 0) {
-                    A.part_b[blockIdx.x] = a;
-                    A.part_c[blockIdx.x] = c;
-                }
-            }
-            grid.sync();
-            const double rzn = sum_partials_block(A.part_b, nb, sh);
-            rr = sum_partials_block(A.part_c, nb, sh);
-            beta = rzn / rz;
-            rz = rzn;
-            double* t = po;
-            po = pn;
-            pn = t;
-            if (rr <= stop2) {
-                ++it;
-                break;
-            }
+        for (int i = 0; i < 3; ++i) {
+            const double zi = P[3 * i] * rv[0] + P[3 * i + 1] * rv[1] + P[3 * i + 2] * rv[2];
+            A.z[3 * v + i] = zi;
+            drz += rv[i] * zi;
+            drr += rv[i] * rv[i];
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        A.out[0] = (double)it;
-        A.out[1] = rr;
-        A.out[2] = bb;
-        A.out[3] = (double)status;
+    const double a = block_sum<RB>(drz, sh), c = block_sum<RB>(drr, sh);
+    if (threadIdx.x == 0) {
+        A.part_rz[(par ^ 1) * A.nbB + blockIdx.x] = a;
+        A.part_rr[(par ^ 1) * A.nbB + blockIdx.x] = c;
     }
 }
 
@@ -700,79 +576,19 @@ void init_erow(LinWork& L, hipStream_t s, const MeshDev& m, const std::vector<in
     L.erow.upload(rowof.data(), rowof.size(), s);
 }
 
-int pcg_solve(LinWork& L, hipStream_t s, const MeshDev& m, const double* b, double* x, double rtol, int maxit,
-              double* relres, int check_every)
-{
-    const int nV = m.nV;
-    const int nblk = grid_for(nV, RB);
-    double* Pinv = L.pinv.ensure(9 * (size_t)nV, s);
-    double* r = L.r.ensure(3 * (size_t)nV, s);
-    double* z = L.z.ensure(3 * (size_t)nV, s);
-    double* p = L.p.ensure(3 * (size_t)nV, s);
-    double* q = L.q.ensure(3 * (size_t)nV, s);
-    double* ppq = L.part0.ensure(nblk, s);
-    double* prz = L.part1.ensure(nblk, s);
-    double* prr = L.part2.ensure(nblk, s);
-    double* scal = L.scal.ensure(8, s);
-    int* err = L.err.ensure(1, s);
-    IDP_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
-    const bool haveC = L.ncb > 0;
-    IDP_LAUNCH(k_block_jacobi, grid_for(nV, 256), 256, 0, s, nV, m.e_diag.p, L.evals.p,
-               haveC ? L.cdiag.p : (const int*)nullptr, L.cvals.p, Pinv, err);
-    IDP_LAUNCH(k_pcg_init, nblk, RB, 0, s, nV, b, Pinv, x, r, z, p, prz, prr);
-    IDP_LAUNCH(k_pcg_init_scalars, 1, RB, 0, s, nblk, prz, prr, scal);
-    double hs[5];
-    IDP_HIP(hipMemcpyAsync(hs, scal, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
-    int herr = 0;
-    IDP_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
-    IDP_HIP(hipStreamSynchronize(s));
-    if (herr) return -1;
-    const double bn2 = hs[4];
-    if (bn2 == 0.0) {
-        *relres = 0.0;
-        return 0;
-    }
-    const double stop2 = rtol * rtol * bn2;
-    int it = 0;
-    int par = 0;
-    while (it < maxit) {
-        IDP_LAUNCH(k_spmv_dot, nblk, RB, 0, s, nV, m.e_rowptr.p, m.e_col.p, L.evals.p,
-                   haveC ? L.crp.p : (const int*)nullptr, L.ccol.p, L.cvals.p, p, q, ppq);
-        IDP_LAUNCH(k_pcg_update, nblk, RB, 0, s, nV, nblk, ppq, scal, par, p, q, Pinv, x, r, z, prz, prr);
-        IDP_LAUNCH(k_pcg_direction, nblk, RB, 0, s, nV, nblk, prz, prr, ppq, scal, par, z, p);
-        par ^= 1;
-        ++it;
-        if (it % check_every == 0 || it == maxit) {
-            IDP_HIP(hipMemcpyAsync(hs, scal, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
-            IDP_HIP(hipStreamSynchronize(s));
-            if (hs[3] != 0.0) return -2;
-            if (hs[2] <= stop2) break;
-        }
-    }
-    IDP_HIP(hipMemcpyAsync(hs, scal, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
-    IDP_HIP(hipStreamSynchronize(s));
-    *relres = std::sqrt(hs[2] / bn2);
-    return it;
-}
-
-
 int pcg_solve_persistent(LinWork& L, hipStream_t s, const MeshDev& m, const double* b, double* x, double rtol,
                          int maxit, double* relres)
 {
     const int nV = m.nV;
-    static int max_blocks = -1;
-    if (max_blocks < 0) {
-        int dev = 0, ncu = 0, per = 0;
-        IDP_HIP(hipGetDevice(&dev));
-        IDP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        IDP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pcg_persistent, RB, 0));
-        max_blocks = std::max(1, ncu * std::max(per, 1));
-    }
-    const int nblk = std::max(1, std::min(max_blocks, grid_for(nV, RB)));
+    const double nnzb = (double)m.nnzb_e + (double)L.ncb;
+    // lanes per block row from the mean row length (a tet-mesh row holds ~15 blocks)
+    const double avg = nnzb / std::max(nV, 1);
+    const int G = avg <= 8.0 ? 4 : (avg <= 24.0 ? 8 : 16);
     double* Pinv = L.pinv.ensure(9 * (size_t)nV, s);
-    int* err = L.err.ensure(1, s);
+    int* err = L.err.ensure(4, s);
     IDP_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
     const bool haveC = L.ncb > 0;
+    const int tok = L.prof ? L.prof->begin(IDP_K_PCG) : -1;
     IDP_LAUNCH(k_block_jacobi, grid_for(nV, 256), 256, 0, s, nV, m.e_diag.p, L.evals.p,
                haveC ? L.cdiag.p : (const int*)nullptr, L.cvals.p, Pinv, err);
     PcgArgs A;
@@ -788,34 +604,67 @@ int pcg_solve_persistent(LinWork& L, hipStream_t s, const MeshDev& m, const doub
     A.x = x;
     A.r = L.r.ensure(3 * (size_t)nV, s);
     A.z = L.z.ensure(3 * (size_t)nV, s);
-    A.p0 = L.p.ensure(3 * (size_t)nV, s);
-    A.p1 = L.p1.ensure(3 * (size_t)nV, s);
+    A.pbuf = L.p.ensure(6 * (size_t)nV, s);
     A.q = L.q.ensure(3 * (size_t)nV, s);
-    A.part_a = L.part0.ensure(nblk, s);
-    A.part_b = L.part1.ensure(nblk, s);
-    A.part_c = L.part2.ensure(nblk, s);
-    A.out = L.scal.ensure(8, s);
+    A.nbA = std::max(1, grid_for((long)nV * G, PCG_SPMV_BLOCK));
+    A.nbB = std::max(1, grid_for(nV, RB));
+    A.part_pq = L.part0.ensure(A.nbA, s);
+    A.part_rz = L.part1.ensure(2 * (size_t)A.nbB, s);
+    A.part_rr = L.part2.ensure(2 * (size_t)A.nbB, s);
+    A.flag = err + 1;
+    A.scal = L.scal.ensure(8, s);
     A.rtol2 = rtol * rtol;
     A.maxit = maxit;
-    void* args[] = {&A};
-    const int tok = L.prof ? L.prof->begin(IDP_K_PCG) : -1;
-    IDP_HIP(hipLaunchCooperativeKernel((const void*)k_pcg_persistent, dim3(nblk), dim3(RB), args, 0, s));
-    if (L.prof) L.prof->end(IDP_K_PCG, tok);
-    double hs[4];
-    int herr = 0;
-    IDP_HIP(hipMemcpyAsync(hs, A.out, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
-    IDP_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
-    IDP_HIP(hipStreamSynchronize(s));
-    if (herr) return -1;
-    if (hs[3] != 0.0) return -2;
-    *relres = hs[2] > 0 ? std::sqrt(hs[1] / hs[2]) : 0.0;
-    // algorithmic bytes per PCG iteration: BSR SpMV (76 nnzb + 4 (V+1) + 48 V),
-    // block-Jacobi apply (72 V + 48 V), two dots and three axpys (8 * 24 V)
-    if (L.prof) {
-        const double nnzb = (double)m.nnzb_e + (double)L.ncb;
-        L.prof->add_work(IDP_K_PCG, hs[0] * (76.0 * nnzb + 364.0 * nV + 4.0));
+    IDP_LAUNCH(k_pcg_init, A.nbB, RB, 0, s, A);
+    struct {
+        int err, flag;
+    } hf{0, 0};
+    double hs[3] = {0, 0, 0};
+    // algorithmic bytes of one SpMV launch (blocks + columns, two row-pointer
+    // arrays, z and p_old read, p_new and q written) and of one update launch
+    // (x, p, r, q, P^-1 read; x, r, z written)
+    const double spmv_bytes = 76.0 * nnzb + 8.0 * (nV + 1) + 96.0 * nV;
+    const double upd_bytes = 240.0 * nV;
+    Profiler* P = L.prof;
+    int it = 0, chunk = 8;
+    for (;;) {
+        for (int j = 0; j < chunk && it <= maxit; ++j, ++it) {
+            // per-kernel events on one iteration in 8 (an event pair between
+            // every launch would itself widen the launch gaps)
+            const bool sample = P && (it & 7) == 3;
+            int t = sample ? P->begin(IDP_K_PCG_SPMV) : -1;
+            if (G == 4) IDP_LAUNCH(k_pcg_spmv<4>, A.nbA, PCG_SPMV_BLOCK, 0, s, A, it);
+            else if (G == 8) IDP_LAUNCH(k_pcg_spmv<8>, A.nbA, PCG_SPMV_BLOCK, 0, s, A, it);
+            else IDP_LAUNCH(k_pcg_spmv<16>, A.nbA, PCG_SPMV_BLOCK, 0, s, A, it);
+            if (sample) P->end(IDP_K_PCG_SPMV, t, spmv_bytes, it);
+            t = sample ? P->begin(IDP_K_PCG_UPDATE) : -1;
+            IDP_LAUNCH(k_pcg_update, A.nbB, RB, 0, s, A, it);
+            if (sample) P->end(IDP_K_PCG_UPDATE, t, upd_bytes, it);
+        }
+        IDP_HIP(hipMemcpyAsync(&hf, err, sizeof(hf), hipMemcpyDeviceToHost, s));
+        IDP_HIP(hipMemcpyAsync(hs, A.scal, sizeof(hs), hipMemcpyDeviceToHost, s));
+        IDP_HIP(hipStreamSynchronize(s));
+        if (hf.err || hf.flag || it > maxit) break;
+        chunk = std::min(chunk * 2, 256);
     }
-    return (int)hs[0];
+    if (P) P->end(IDP_K_PCG, tok);
+    if (hf.err) return -1;
+    if (hf.flag == 0) throw HipError(IDP_E_HIP, "PCG did not stop");
+    const int iters = (int)hs[2];
+    if (P) {
+        // launches at or after the stopping iteration did no SpMV/update work
+        P->discard_from(IDP_K_PCG_SPMV, iters);
+        P->discard_from(IDP_K_PCG_UPDATE, iters);
+        P->checkpoint();
+    }
+    if (hf.flag == 2) return -2;
+    *relres = hs[0] > 0 ? std::sqrt(hs[1] / hs[0]) : 0.0;
+    // algorithmic bytes per PCG iteration (each array touched once):
+    //   SpMV: blocks + columns 76 nnzb, two row pointers 8 (V+1),
+    //         read z, p_old, write p_new, q: 96 V
+    //   update: read x, p_new, r, q, P^-1 (96 V + 72 V), write x, r, z (72 V)
+    if (P) P->add_work(IDP_K_PCG, iters * (spmv_bytes + upd_bytes));
+    return iters;
 }
 
 double dev_dot(LinWork& L, hipStream_t s, long n, const double* a, const double* b)

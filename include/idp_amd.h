@@ -61,7 +61,7 @@ typedef struct idp_step_stats {
  * context's stream around each launch, plus the algorithmic work of those
  * launches (bytes for HBM-bound families, flops for compute-bound ones). */
 enum {
-    IDP_K_PCG = 0,          /* persistent block-Jacobi PCG (bytes) */
+    IDP_K_PCG = 0,          /* whole block-Jacobi PCG solve, launch gaps included (bytes) */
     IDP_K_BARRIER_HESS = 1, /* barrier Hessian + PSD projection (flops) */
     IDP_K_TET_HESS = 2,     /* tet projected 12x12 (flops) */
     IDP_K_ASSEMBLY = 3,     /* elastic + contact BSR value assembly (bytes) */
@@ -69,6 +69,8 @@ enum {
     IDP_K_CCD = 5,          /* ACCD kernel over candidate pairs (pair count) */
     IDP_K_ENERGY = 6,       /* tet + inertia + barrier energy kernels */
     IDP_K_GRADIENT = 7,     /* tet + barrier gradient and vertex gather */
+    IDP_K_PCG_SPMV = 8,     /* PCG SpMV kernel, sampled on one iteration in 8 (bytes) */
+    IDP_K_PCG_UPDATE = 9,   /* PCG vector-update kernel, sampled likewise (bytes) */
     IDP_K_COUNT = 16
 };
 typedef struct idp_profile {

@@ -229,6 +229,45 @@ bool energy(Ctx& c, const double* X, bool withBarrier, double& E)
     return true;
 }
 
+// Projected Hessian of the incremental potential at X (INC_POTENTIAL.h:278-395):
+// tet blocks + barrier blocks of the current constraint set + mass, summed
+// per (row, col) in a stable key order, then Project_DBC.
+void assemble_sys(Ctx& c, const double* X, Sys& S)
+{
+    const int nV = c.nV, nT = c.nT;
+    const int* tets = c.tets;
+    std::vector<Blk> blks;
+    blks.reserve(16 * (size_t)nT + nV + 16 * c.ncs);
+    std::vector<double> He(144 * (size_t)nT);
+    orc_elastic_hessian_local(X, nT, tets, c.IB, c.vol, c.lam, c.mu, c.h * c.h, 1, He.data());
+    for (int t = 0; t < nT; ++t) add_local(blks, 4, tets + 4 * t, He.data() + 144 * (size_t)t, nV);
+    if (c.withCollision && c.ncs) {
+        std::vector<double> Hl(144 * c.ncs);
+        std::vector<int> vl(4 * c.ncs);
+        orc_barrier_hessian_local(X, c.X0, c.ncs, c.cs.data(), c.dHat2, c.kappa[0], c.thickness, 1, Hl.data(),
+                                  vl.data());
+        for (long i = 0; i < c.ncs; ++i) {
+            int nv = 0;
+            while (nv < 4 && vl[4 * i + nv] >= 0) ++nv;
+            add_local(blks, nv, vl.data() + 4 * i, Hl.data() + 144 * i, nV);
+        }
+    }
+    for (int v = 0; v < nV; ++v) {
+        Blk k;
+        k.key = (long)v * nV + v;
+        std::memset(k.v, 0, sizeof(k.v));
+        k.v[0] = k.v[4] = k.v[8] = c.mass[v];
+        blks.push_back(k);
+    }
+    build_sys(blks, nV, S);
+    // Project_DBC: rows/cols of DBC nodes -> identity
+    for (int r = 0; r < nV; ++r)
+        for (int k = S.rowptr[r]; k < S.rowptr[r + 1]; ++k)
+            if (c.dbc[r] || c.dbc[S.col[k]])
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) S.val[9 * (size_t)k + 3 * i + j] = (r == S.col[k] && i == j);
+}
+
 }  // namespace
 
 extern "C" int orc_advance_one_step(int nV, double* X, const double* X0, double* vel, const double* mass,
@@ -260,8 +299,6 @@ extern "C" int orc_advance_one_step(int nV, double* X, const double* X0, double*
     std::vector<double> g(n3), rhs(n3), sol(n3), Xprev(n3), dist2, dist2_prev;
     std::vector<int> cs_prev;
     long ncs_prev = 0;
-    std::vector<double> Hl;
-    std::vector<int> vl;
     bool useGD = false;
     int PNIter = 0;
     double L2Norm = 0;
@@ -281,37 +318,8 @@ extern "C" int orc_advance_one_step(int nV, double* X, const double* X0, double*
 
         if (!useGD) {
             t0 = now();
-            std::vector<Blk> blks;
-            blks.reserve(16 * (size_t)nT + nV + 16 * c.ncs);
-            std::vector<double> He(144 * (size_t)nT);
-            orc_elastic_hessian_local(X, nT, tets, IB, vol, lam, mu, h * h, 1, He.data());
-            for (int t = 0; t < nT; ++t) add_local(blks, 4, tets + 4 * t, He.data() + 144 * (size_t)t, nV);
-            if (c.withCollision && c.ncs) {
-                Hl.resize(144 * c.ncs);
-                vl.resize(4 * c.ncs);
-                orc_barrier_hessian_local(X, X0, c.ncs, c.cs.data(), dHat2, kappa[0], thickness, 1, Hl.data(),
-                                          vl.data());
-                for (long i = 0; i < c.ncs; ++i) {
-                    int nv = 0;
-                    while (nv < 4 && vl[4 * i + nv] >= 0) ++nv;
-                    add_local(blks, nv, vl.data() + 4 * i, Hl.data() + 144 * i, nV);
-                }
-            }
-            for (int v = 0; v < nV; ++v) {
-                Blk k;
-                k.key = (long)v * nV + v;
-                std::memset(k.v, 0, sizeof(k.v));
-                k.v[0] = k.v[4] = k.v[8] = mass[v];
-                blks.push_back(k);
-            }
             Sys S;
-            build_sys(blks, nV, S);
-            // Project_DBC: rows/cols of DBC nodes -> identity
-            for (int r = 0; r < nV; ++r)
-                for (int k = S.rowptr[r]; k < S.rowptr[r + 1]; ++k)
-                    if (c.dbc[r] || c.dbc[S.col[k]])
-                        for (int i = 0; i < 3; ++i)
-                            for (int j = 0; j < 3; ++j) S.val[9 * (size_t)k + 3 * i + j] = (r == S.col[k] && i == j);
+            assemble_sys(c, X, S);
             c.t_hess += now() - t0;
             t0 = now();
             double relres = 0;
@@ -428,4 +436,42 @@ extern "C" int orc_advance_one_step(int nV, double* X, const double* X0, double*
         st->t_energy = c.t_E;
     }
     return PNIter;
+}
+
+// The Newton system at X: the projected Hessian (BSR, 3x3 blocks) and the
+// gradient of the incremental potential with inertia target Xtilde, for the
+// constraint set at X.  Returns nnzb, or -nnzb when cap is too small.
+extern "C" long orc_newton_system(int nV, const double* X, const double* Xtilde, const double* X0,
+                                  const double* mass, int nT, const int* tets, const double* IB, const double* vol,
+                                  const double* lam, const double* mu, int nb, const int* bnode, int ne,
+                                  const int* bedge, int nf, const int* btri, int nDBC, const int* dbc_idx, double h,
+                                  double dHat2, double kappa, double thickness, int* rowptr, int* col, double* val,
+                                  double* g, long cap)
+{
+    Ctx c;
+    c.nV = nV; c.nT = nT; c.nb = nb; c.ne = ne; c.nf = nf;
+    c.X0 = X0; c.mass = mass; c.tets = tets; c.IB = IB; c.vol = vol; c.lam = lam; c.mu = mu;
+    c.bnode = bnode; c.bedge = bedge; c.btri = btri;
+    c.dbc.assign(nV, 0);
+    for (int i = 0; i < nDBC; ++i) c.dbc[dbc_idx[i]] = 1;
+    double kap[3] = {kappa, kappa, 0};
+    c.h = h; c.dHat2 = dHat2; c.thickness = thickness; c.kappa = kap; c.withCollision = true;
+    c.st = nullptr;
+    c.Xtilde.assign(Xtilde, Xtilde + 3 * (size_t)nV);
+    compute_cs(c, X);
+    Sys S;
+    assemble_sys(c, X, S);
+    const long nnzb = (long)S.col.size();
+    if (nnzb > cap) return -nnzb;
+    std::copy(S.rowptr.begin(), S.rowptr.end(), rowptr);
+    std::copy(S.col.begin(), S.col.end(), col);
+    std::copy(S.val.begin(), S.val.end(), val);
+    std::fill(g, g + 3 * (size_t)nV, 0.0);
+    orc_elastic_gradient(X, nT, tets, IB, vol, lam, mu, h * h, g);
+    for (int v = 0; v < nV; ++v)
+        for (int d = 0; d < 3; ++d) g[3 * v + d] += mass[v] * (X[3 * v + d] - Xtilde[3 * v + d]);
+    if (c.ncs) orc_barrier_gradient(X, X0, c.ncs, c.cs.data(), dHat2, kappa, thickness, nV, g);
+    for (int v = 0; v < nV; ++v)
+        if (c.dbc[v]) g[3 * v] = g[3 * v + 1] = g[3 * v + 2] = 0;
+    return nnzb;
 }

@@ -43,6 +43,11 @@ void orc_elastic_gradient(const double* X, int nT, const int* tets, const double
                           const double* lam, const double* mu, double w, double* g);
 void orc_elastic_hessian_local(const double* X, int nT, const int* tets, const double* IB, const double* vol,
                                const double* lam, const double* mu, double w, int projectSPD, double* H);
+long orc_newton_system(int nV, const double* X, const double* Xtilde, const double* X0, const double* mass, int nT,
+                       const int* tets, const double* IB, const double* vol, const double* lam, const double* mu,
+                       int nb, const int* bnode, int ne, const int* bedge, int nf, const int* btri, int nDBC,
+                       const int* dbc_idx, double h, double dHat2, double kappa, double thickness, int* rowptr,
+                       int* col, double* val, double* g, long cap);
 int orc_advance_one_step(int nV, double* X, const double* X0, double* vel, const double* mass, int nT,
                          const int* tets, const double* IB, const double* vol, const double* lam,
                          const double* mu, int nb, const int* bnode, int ne, const int* bedge, int nf,

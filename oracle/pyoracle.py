@@ -80,6 +80,10 @@ def lib():
                                            C.c_int, ip, C.c_int, ip, C.c_int, ip, C.c_double, C.c_double,
                                            C.c_int, C.c_double, dp, C.c_double, C.c_double, C.c_int,
                                            C.POINTER(Stats)]
+        L.orc_newton_system.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int, ip, C.c_int,
+                                        ip, C.c_int, ip, C.c_int, ip, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, ip, ip, dp, dp, C.c_long]
+        L.orc_newton_system.restype = C.c_long
         _lib = L
     return _lib
 
@@ -189,6 +193,25 @@ class Mesh:
         lib().orc_elastic_hessian_local(_f(X), self.nT, self.tets, self.IB, self.vol, self.lam, self.mu, w,
                                         int(project), H)
         return H.reshape(self.nT, 144)
+
+    def newton_system(self, X, Xtilde, h, dHat2, kappa, thickness=0.0, dbc_idx=None):
+        """(rowptr, col, val (nnzb,3,3), g) of the projected Newton system at X."""
+        X = _f(X)
+        Xt = _f(Xtilde)
+        dbc_idx = np.zeros(0, np.int32) if dbc_idx is None else _i(dbc_idx)
+        cap = 64 * self.nV + 1024
+        while True:
+            rp = np.zeros(self.nV + 1, np.int32)
+            col = np.zeros(cap, np.int32)
+            val = np.zeros(9 * cap)
+            g = np.zeros(3 * self.nV)
+            n = lib().orc_newton_system(self.nV, X, Xt, self.X0, self.mass, self.nT, self.tets, self.IB, self.vol,
+                                        self.lam, self.mu, self.bnode.size, self.bnode, self.bedge.shape[0],
+                                        _i(self.bedge), self.btri.shape[0], _i(self.btri), dbc_idx.size, dbc_idx,
+                                        h, dHat2, kappa, thickness, rp, col, val, g, cap)
+            if n >= 0:
+                return rp, col[:n], val[: 9 * n].reshape(n, 3, 3), g
+            cap = -n
 
     def advance(self, X, vel, h, tol, dHat2, kappa, thickness=0.0, lin_rtol=1e-12, max_newton=0,
                 with_collision=True, dbc_idx=None):
