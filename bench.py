@@ -114,12 +114,220 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": n, "rc": rc, "seconds": t}), flush=True)
 
 
+def bench_batch(a):
+    """256 independent cube solves batched in one context vs the plain
+    one-mesh-at-a-time path (timed on an evenly spaced sample of instances)."""
+    from idp_amd import driver, scenes
+
+    scs = scenes.config3()
+    log(f"[bench] cubes256: {sum(x.X.shape[0] for x in scs)} vertices, {sum(x.tets.shape[0] for x in scs)} tets")
+    b = driver.BatchAnimFix(scs, pcg_rtol=a.pcg_rtol)
+    k0 = b.kappa.copy()
+    b.ctx.snapshot()
+
+    def solve(max_newton):
+        b.ctx.restore()
+        b.kappa[:] = k0
+        t0 = time.perf_counter()
+        its, st = b.advance_one_time_step(max_newton=max_newton)
+        return its, st, time.perf_counter() - t0
+
+    solve(2)  # buffer priming
+    for w in range(a.warmup):
+        solve(a.max_newton)
+    lat, newton = [], []
+    t_start = time.perf_counter()
+    for k in range(a.steps):
+        its, st, t = solve(a.max_newton)
+        lat.append(t)
+        newton.append(int(its.max()))
+        log(f"[bench] batch step {k}: {len(scs)} solves in {t:.2f} s, Newton max {its.max()} mean {its.mean():.1f}")
+    total = time.perf_counter() - t_start
+    batched = a.steps * len(scs) / total
+    # plain path: the same instances one at a time
+    idx = list(range(0, len(scs), max(1, len(scs) // a.plain_sample)))[: a.plain_sample]
+    tp = 0.0
+    for i in idx:
+        d = driver.TetAnimFix(scs[i], pcg_rtol=a.pcg_rtol)
+        d.ctx.snapshot()
+        d.advance_one_time_step(max_newton=2)  # same priming as the batch
+        d.ctx.restore()
+        d.kappa[:] = k0[i]
+        t0 = time.perf_counter()
+        d.advance_one_time_step(max_newton=a.max_newton)
+        tp += time.perf_counter() - t0
+    plain = len(idx) / tp
+    log(f"[bench] plain path: {len(idx)} instances in {tp:.2f} s")
+    lat_ms = np.array(lat) * 1e3
+    out = {
+        "metric": "solves/sec + p50/p99 latency at fixed accuracy; barrier pair-evals/sec vs plain",
+        "value": batched,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": total * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "cubes256_batched", "instances": len(scs), "pcg_rtol": a.pcg_rtol,
+                   "newton_max": newton},
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+        "plain_path": {"solves_per_s": plain, "sample": f"{len(idx)} evenly spaced instances, one at a time"},
+        "batched_vs_plain": batched / plain,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def serve_sequences(frames):
+    """Animation-fix sequences: bars of 8x4x4 (6), 32x8x8 (4) and 64x16x16 (2)
+    cells bent from 0 to 270 degrees over `frames` frames (bend radius 0.15 of
+    the bar length).  Returns [(rest scene, [target of frame 1..frames])]."""
+    from idp_amd import scenes
+
+    out = []
+    for nx, ny, nz, count in ((8, 4, 4, 6), (32, 8, 8, 4), (64, 16, 16, 2)):
+        for _ in range(count):
+            L, W = nx * 0.05, ny * 0.05
+            mid = np.array([L / 2, W / 2, 0.0])  # the bend map is centred on the bar's middle
+            tg = [scenes.bar_bend(nx, ny, nz, 0.05, 0.15 * L, 270.0 * k / frames).target + mid
+                  for k in range(1, frames + 1)]
+            out.append((scenes.bar_bend(nx, ny, nz, 0.05, 0.15 * L, 0.0), tg))
+    return out
+
+
+def bench_serve(a):
+    """Continuous batching of animation-fix frames arriving as a Poisson stream:
+    one slot per sequence (the sequence's mesh stays on the device and every
+    frame warm-starts from the previous frame's solution); a frame starts as
+    soon as it has arrived and its predecessor has finished, while the other
+    slots are mid-solve.  The plain path serves the same stream one frame at a
+    time (FIFO, service times measured with the one-mesh path)."""
+    from idp_amd import capi, driver
+
+    seqs = serve_sequences(a.frames)
+    S, F = len(seqs), a.frames
+    rest = [sc for sc, _ in seqs]
+    # admission by HBM budget: every sequence gets a slot when its estimated
+    # device footprint fits in the budget (all of them on one MI355X)
+    per_slot = [2000.0 * sc.X.shape[0] + 3000.0 * sc.tets.shape[0] for sc in rest]  # bytes, generous
+    budget = 64e9
+    assert sum(per_slot) < budget
+    b = driver.BatchAnimFix(rest, pcg_rtol=a.pcg_rtol)
+    ctx = b.ctx
+    ctx.serve_configure(b.h, pcg_rtol=a.pcg_rtol)
+    k_init = b.kappa.copy()
+    h = b.h
+
+    def run(rate, seed):
+        rng = np.random.default_rng(seed)
+        order = [(s, k) for k in range(F) for s in range(S)]
+        arrive = np.cumsum(rng.exponential(1.0 / rate, len(order)))
+        arr = {o: t for o, t in zip(order, arrive)}
+        state = [rest[s].X.copy() for s in range(S)]
+        kap = k_init.copy()
+        nxt = [0] * S          # next frame per sequence
+        busy = [False] * S
+        lat, t0 = [], time.perf_counter()
+        done = 0
+        while done < S * F:
+            now = time.perf_counter() - t0
+            ready = [s for s in range(S) if not busy[s] and nxt[s] < F and arr[(s, nxt[s])] <= now]
+            if ready:
+                X = [state[s] for s in ready]
+                V = [(seqs[s][1][nxt[s]] - state[s]) / h for s in ready]
+                ctx.serve_start(ready, X, V, b.tol[ready], b.dHat2[ready], kap[ready])
+                for s in ready:
+                    busy[s] = True
+            if any(busy):
+                n, fin = ctx.serve_iterate()
+                tf = time.perf_counter() - t0
+                for s in np.nonzero(fin)[0]:
+                    Xs, _, k3, _ = ctx.serve_result(int(s))
+                    state[s] = Xs
+                    kap[s, 0] = k3[0]
+                    lat.append(tf - arr[(int(s), nxt[s])])
+                    nxt[s] += 1
+                    busy[s] = False
+                    done += 1
+            else:
+                waits = [arr[(s, nxt[s])] for s in range(S) if nxt[s] < F]
+                time.sleep(max(0.0, min(waits) - (time.perf_counter() - t0)))
+        span = time.perf_counter() - t0
+        return S * F / span, np.array(lat) * 1e3
+
+    # warm-up pass (device buffers reach their working sizes)
+    run(1e6, 0)
+    results = {}
+    for rate in [float(r) for r in a.rates.split(",")]:
+        thr, lat = run(rate, 1)
+        results[rate] = {"solves_per_s": thr, "p50_ms": float(np.percentile(lat, 50)),
+                         "p99_ms": float(np.percentile(lat, 99))}
+        log(f"[bench] serve rate {rate}: {thr:.2f} solves/s, p50 {results[rate]['p50_ms']:.1f} ms, "
+            f"p99 {results[rate]['p99_ms']:.1f} ms")
+    # plain path: one frame at a time, FIFO over the same arrival stream
+    svc = {}
+    for s in range(S):
+        d = driver.TetAnimFix(rest[s], pcg_rtol=a.pcg_rtol)
+        d.advance_one_time_step(max_newton=1)  # buffer priming
+        d.ctx.set_state(rest[s].X, np.zeros_like(rest[s].X))
+        d.kappa[:] = k_init[s]
+        X = rest[s].X
+        for k in range(F):
+            d.ctx.set_state(X, (seqs[s][1][k] - X) / h)
+            t0 = time.perf_counter()
+            d.advance_one_time_step()
+            svc[(s, k)] = time.perf_counter() - t0
+            X, _ = d.state()
+    plain = {}
+    for rate in results:
+        rng = np.random.default_rng(1)
+        order = [(s, k) for k in range(F) for s in range(S)]
+        arrive = np.cumsum(rng.exponential(1.0 / rate, len(order)))
+        t, lat = 0.0, []
+        for o, ta in zip(order, arrive):
+            t = max(t, ta) + svc[o]
+            lat.append(t - ta)
+        lat = np.array(lat) * 1e3
+        plain[rate] = {"solves_per_s": len(order) / t, "p50_ms": float(np.percentile(lat, 50)),
+                       "p99_ms": float(np.percentile(lat, 99))}
+    top = max(results)
+    out = {
+        "metric": "solves/sec + p50/p99 latency at fixed accuracy; barrier pair-evals/sec vs plain",
+        "value": results[top]["solves_per_s"],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": S * F,
+        "warmup": S * F,
+        "ms_per_step": 1e3 / results[top]["solves_per_s"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "serve_bar_frames", "sequences": S, "frames": F, "slots": S,
+                   "rates": list(results), "pcg_rtol": a.pcg_rtol},
+        "latency_ms": {"p50": results[top]["p50_ms"], "p99": results[top]["p99_ms"]},
+        "served": results,
+        "plain_path": plain,
+        "served_vs_plain": results[top]["solves_per_s"] / plain[top]["solves_per_s"],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--workload", default="bar_twist_fold", choices=["bar_twist_fold", "bar_bend"])
+    ap.add_argument("--workload", default="bar_twist_fold",
+                    choices=["bar_twist_fold", "bar_bend", "cubes256", "serve"])
+    ap.add_argument("--frames", type=int, default=10, help="serve: frames per sequence (0 to 270 degrees)")
+    ap.add_argument("--rates", default="5,20,100,200", help="serve: Poisson arrival rates (req/s)")
+    ap.add_argument("--plain-sample", type=int, default=16,
+                    help="cubes256: instances solved one at a time for the plain-path comparison")
     ap.add_argument("--pcg-rtol", type=float, default=1e-8)
     ap.add_argument("--max-newton", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -138,6 +346,10 @@ def main():
 
     from idp_amd import capi, driver
 
+    if a.workload == "cubes256":
+        return bench_batch(a)
+    if a.workload == "serve":
+        return bench_serve(a)
     sc = workload(a.workload)
     log(f"[bench] workload {sc.name}: {sc.X.shape[0]} vertices, {sc.tets.shape[0]} tets")
     d = driver.TetAnimFix(sc, pcg_rtol=a.pcg_rtol)
@@ -203,6 +415,10 @@ def main():
         if prof[k]["ms"] > 0:
             kern[k]["gbs"] = prof[k]["work"] / (prof[k]["ms"] * 1e-3) / 1e9
     pairs = int(sum(prof["barrier_pairs"]))
+    kern["barrier_hess"]["pairs"] = pairs
+    kern["barrier_hess"]["clamped"] = int(sum(prof["barrier_clamped"]))
+    kern["ccd"]["steps"] = prof["ccd_steps"]
+    kern["ccd"]["max_steps"] = prof["ccd_max_steps"]
 
     base = None
     if not a.no_cpu_baseline:

@@ -40,7 +40,8 @@ EXPORTED = [
     "idp_restore_state", "idp_surface_sizes", "idp_advance_one_step", "idp_constraint_set", "idp_get_constraints",
     "idp_barrier_energy", "idp_barrier_gradient", "idp_barrier_hessian_local", "idp_min_dist2", "idp_ccd_stepsize",
     "idp_elastic_energy", "idp_elastic_gradient", "idp_elastic_hessian_local", "idp_newton_direction", "idp_get_bsr",
-    "idp_profile_enable", "idp_profile_read",
+    "idp_profile_enable", "idp_profile_read", "idp_set_mesh_batch", "idp_batch_size", "idp_advance_batch",
+    "idp_serve_configure", "idp_serve_start", "idp_serve_iterate", "idp_serve_result",
 ]
 
 K_FAMILIES = ["pcg", "barrier_hess", "tet_hess", "assembly", "constraint_set", "ccd", "energy", "gradient",
@@ -74,7 +75,8 @@ class StepStats(C.Structure):
 
 class Profile(C.Structure):
     _fields_ = [("ms", C.c_double * 16), ("launches", C.c_long * 16), ("work", C.c_double * 16),
-                ("barrier_pairs", C.c_long * 8), ("barrier_clamped", C.c_long * 8)]
+                ("barrier_pairs", C.c_long * 8), ("barrier_clamped", C.c_long * 8), ("ccd_steps", C.c_double),
+                ("ccd_max_steps", C.c_double)]
 
     def as_dict(self):
         out = {}
@@ -82,6 +84,8 @@ class Profile(C.Structure):
             out[name] = dict(ms=self.ms[i], launches=self.launches[i], work=self.work[i])
         out["barrier_pairs"] = list(self.barrier_pairs)[:7]
         out["barrier_clamped"] = list(self.barrier_clamped)[:7]
+        out["ccd_steps"] = self.ccd_steps
+        out["ccd_max_steps"] = self.ccd_max_steps
         return out
 
 
@@ -128,6 +132,14 @@ def lib():
         L.idp_newton_direction.argtypes = [vp, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double,
                                            C.POINTER(SolverOpts), dp, dp, dp, ip]
         L.idp_get_bsr.argtypes = [vp, C.POINTER(C.c_long), ip, ip, dp, C.c_long]
+        L.idp_set_mesh_batch.argtypes = [vp, C.c_int, ip, ip, dp, ip, dp, dp, dp, dp, dp]
+        L.idp_batch_size.argtypes = [vp, ip]
+        L.idp_advance_batch.argtypes = [vp, C.c_double, dp, C.c_int, dp, dp, C.c_double, C.POINTER(SolverOpts), ip,
+                                        C.POINTER(StepStats)]
+        L.idp_serve_configure.argtypes = [vp, C.c_double, C.c_int, C.c_double, C.POINTER(SolverOpts)]
+        L.idp_serve_start.argtypes = [vp, C.c_int, ip, dp, dp, dp, dp, dp]
+        L.idp_serve_iterate.argtypes = [vp, ip, ip]
+        L.idp_serve_result.argtypes = [vp, C.c_int, dp, dp, dp, C.POINTER(StepStats)]
         L.idp_profile_enable.argtypes = [vp, C.c_int]
         L.idp_profile_read.argtypes = [vp, C.POINTER(Profile)]
         _lib = L
@@ -236,6 +248,73 @@ class Context:
         ms, msp = _d(mass)
         self.nV = X.size // 3
         self._check(lib().idp_set_mesh(self._h, self.nV, Xp, t.size // 4, tp, ibp, vp, lp, mp, msp))
+
+    def set_mesh_batch(self, meshes):
+        """meshes: list of (X_rest (V,3), tets (T,4) local ids, IB, vol, lam, mu, mass)."""
+        nV = np.array([np.asarray(m[0]).reshape(-1, 3).shape[0] for m in meshes], np.int32)
+        nT = np.array([np.asarray(m[1]).reshape(-1, 4).shape[0] for m in meshes], np.int32)
+        cat = [np.ascontiguousarray(np.concatenate([np.asarray(m[k], dtype=np.float64 if k != 1 else np.int32)
+                                                    .ravel() for m in meshes])) for k in range(7)]
+        X, Xp = _d(cat[0])
+        t, tp = _i(cat[1])
+        ib, ibp = _d(cat[2])
+        v, vp = _d(cat[3])
+        l, lp = _d(cat[4])
+        m, mp = _d(cat[5])
+        ms, msp = _d(cat[6])
+        self.nV = int(nV.sum())
+        self.batch_nV = nV
+        self._check(lib().idp_set_mesh_batch(self._h, len(meshes), nV.ctypes.data_as(ip), nT.ctypes.data_as(ip), Xp,
+                                             tp, ibp, vp, lp, mp, msp))
+
+    def advance_batch(self, h, tol, dHat2, kappa3, thickness=0.0, with_collision=True, pcg_rtol=1e-8,
+                      pcg_maxit=10000, max_newton=0):
+        """One step of every instance; kappa3 (B,3) is updated in place.
+        Returns (newton iterations (B,), list of per-instance stats dicts)."""
+        B = len(self.batch_nV)
+        tol = np.ascontiguousarray(tol, dtype=np.float64)
+        d2 = np.ascontiguousarray(dHat2, dtype=np.float64)
+        if not (kappa3.flags["C_CONTIGUOUS"] and kappa3.dtype == np.float64 and kappa3.size == 3 * B):
+            raise ValueError("kappa3 must be a contiguous float64 (B,3) array")
+        its = np.zeros(B, np.int32)
+        st = (StepStats * B)()
+        opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, 0)
+        self._check(lib().idp_advance_batch(self._h, h, tol.ctypes.data_as(dp), int(with_collision),
+                                            d2.ctypes.data_as(dp), kappa3.ctypes.data_as(dp), thickness,
+                                            C.byref(opts), its.ctypes.data_as(ip), st))
+        return its, [x.as_dict() for x in st]
+
+    # ---- continuous batching over the slots of a batch mesh ----------------
+    def serve_configure(self, h, with_collision=True, thickness=0.0, pcg_rtol=1e-8, pcg_maxit=10000, max_newton=0):
+        opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, 0)
+        self._check(lib().idp_serve_configure(self._h, h, int(with_collision), thickness, C.byref(opts)))
+
+    def serve_start(self, slots, X, v, tol, dHat2, kappa3):
+        slots = np.ascontiguousarray(slots, dtype=np.int32)
+        Xa, Xp = _d(np.concatenate([np.asarray(x).ravel() for x in X]))
+        va, vp = _d(np.concatenate([np.asarray(x).ravel() for x in v]))
+        t = np.ascontiguousarray(tol, dtype=np.float64)
+        d2 = np.ascontiguousarray(dHat2, dtype=np.float64)
+        k3 = np.ascontiguousarray(kappa3, dtype=np.float64).ravel()
+        self._check(lib().idp_serve_start(self._h, slots.size, slots.ctypes.data_as(ip), Xp, vp,
+                                          t.ctypes.data_as(dp), d2.ctypes.data_as(dp), k3.ctypes.data_as(dp)))
+
+    def serve_iterate(self):
+        """One Newton iteration of all busy slots: (busy count, finished mask)."""
+        B = len(self.batch_nV)
+        fin = np.zeros(B, np.int32)
+        n = C.c_int()
+        self._check(lib().idp_serve_iterate(self._h, C.byref(n), fin.ctypes.data_as(ip)))
+        return n.value, fin.astype(bool)
+
+    def serve_result(self, slot):
+        nv = int(self.batch_nV[slot])
+        X, Xp = _out_d(3 * nv)
+        v, vp = _out_d(3 * nv)
+        k3, kp = _out_d(3)
+        st = StepStats()
+        self._check(lib().idp_serve_result(self._h, slot, Xp, vp, kp, C.byref(st)))
+        return X.reshape(-1, 3), v.reshape(-1, 3), k3, st.as_dict()
 
     def set_dbc(self, idx):
         a, ap = _i(idx)

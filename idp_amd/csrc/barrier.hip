@@ -190,22 +190,22 @@ __device__ inline double mollifier_value(const Decoded& k, const double* X, cons
 // ---------------------------------------------------------------------------
 // energy and minimum distance
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(RB) k_barrier_energy(long n, const int4* __restrict__ cs,
-                                                       const double* __restrict__ X,
-                                                       const double* __restrict__ X0, double dh, double kappa,
-                                                       double th2, double* __restrict__ part,
-                                                       int* __restrict__ err)
+__global__ void __launch_bounds__(RB) k_barrier_energy(const int* __restrict__ lo, const int* __restrict__ hi,
+                                                       const int4* __restrict__ cs, const double* __restrict__ X,
+                                                       const double* __restrict__ X0, BarrierParams bp,
+                                                       double* __restrict__ part, int* __restrict__ err)
 {
     __shared__ double sh[RB];
-    long i = (long)blockIdx.x * RB + threadIdx.x;
+    const int i = lo[blockIdx.x] + threadIdx.x;
     double e = 0.0;
-    if (i < n) {
+    if (i < hi[blockIdx.x]) {
         Decoded k = decode(cs[i]);
-        double d = constraint_d2(k, X) - th2;
+        const int inst = bp.inst_of(k.v[0]);
+        double d = constraint_d2(k, X) - bp.th2[inst];
         if (d <= 0) {
-            atomicOr(err, 1);
+            atomicOr(err + inst, 1);
         } else {
-            e = barrier_b(d, dh, kappa);
+            e = barrier_b(d, bp.dh[inst], bp.kappa[inst]);
             if (k.kind >= K_MEE) {
                 double q, eps;
                 e *= mollifier_value(k, X, X0, q, eps);
@@ -218,18 +218,20 @@ __global__ void __launch_bounds__(RB) k_barrier_energy(long n, const int4* __res
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// dmin: one running minimum per instance (vinst null: a single one)
 __global__ void __launch_bounds__(RB) k_min_dist2(long n, const int4* __restrict__ cs,
-                                                  const double* __restrict__ X, double* __restrict__ dist2,
-                                                  unsigned long long* __restrict__ dmin)
+                                                  const double* __restrict__ X, const int* __restrict__ vinst,
+                                                  double* __restrict__ dist2, unsigned long long* __restrict__ dmin)
 {
     long i = (long)blockIdx.x * RB + threadIdx.x;
     if (i >= n) return;
     Decoded k = decode(cs[i]);
     double d = constraint_d2(k, X);
     if (dist2) dist2[i] = d;
+    unsigned long long* m = dmin + (vinst ? vinst[k.v[0]] : 0);
     // squared distances are >= 0, so their IEEE bit patterns order like integers
-    if (d >= 0) atomicMin(dmin, (unsigned long long)__double_as_longlong(d));
-    else atomicMin(dmin, 0ull);
+    if (d >= 0) atomicMin(m, (unsigned long long)__double_as_longlong(d));
+    else atomicMin(m, 0ull);
 }
 
 // ---------------------------------------------------------------------------
@@ -759,13 +761,15 @@ __device__ inline void hess_body(const Decoded& k, const double* X, const double
 // gradient: per-constraint vertex contributions (12 doubles; ids from decode)
 __global__ void __launch_bounds__(256) k_barrier_grad(long n, const int4* __restrict__ cs,
                                                       const double* __restrict__ X,
-                                                      const double* __restrict__ X0, double dh, double kappa,
-                                                      double th2, double* __restrict__ gout, int* __restrict__ err)
+                                                      const double* __restrict__ X0, BarrierParams bp,
+                                                      double* __restrict__ gout, int* __restrict__ err)
 {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Decoded k = decode(cs[i]);
-    IDP_KIND_SWITCH(k.kind, grad_body, k, X, X0, dh, kappa, th2, gout + 12 * i, err)
+    const int inst = bp.inst_of(k.v[0]);
+    IDP_KIND_SWITCH(k.kind, grad_body, k, X, X0, bp.dh[inst], bp.kappa[inst], bp.th2[inst], gout + 12 * i,
+                    err + inst)
 }
 
 // Hessian: mode 0 writes the dense (3nv)^2 local matrix (144 per constraint,
@@ -773,8 +777,8 @@ __global__ void __launch_bounds__(256) k_barrier_grad(long n, const int4* __rest
 // (block (a,b) at 9*(a*nv+b)).
 __global__ void __launch_bounds__(WS) k_barrier_hess(long n, const int4* __restrict__ cs,
                                                      const double* __restrict__ X,
-                                                     const double* __restrict__ X0, double dh, double kappa,
-                                                     double th2, int project_spd, int mode,
+                                                     const double* __restrict__ X0, BarrierParams bp,
+                                                     int project_spd, int mode,
                                                      double* __restrict__ out, int* __restrict__ vid,
                                                      int* __restrict__ err, int* __restrict__ counts)
 {
@@ -782,7 +786,9 @@ __global__ void __launch_bounds__(WS) k_barrier_hess(long n, const int4* __restr
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Decoded k = decode(cs[i]);
-    IDP_KIND_SWITCH(k.kind, hess_body, k, X, X0, dh, kappa, th2, project_spd, mode, out + 144 * i, err, counts,
+    const int inst = bp.inst_of(k.v[0]);
+    IDP_KIND_SWITCH(k.kind, hess_body, k, X, X0, bp.dh[inst], bp.kappa[inst], bp.th2[inst], project_spd, mode,
+                    out + 144 * i, err + inst, counts,
                     wsh + threadIdx.x)
     if (vid)
         for (int j = 0; j < 4; ++j) vid[4 * i + j] = (j < k.nv) ? k.v[j] : -1;
@@ -801,31 +807,30 @@ __global__ void k_constraint_vids(long n, const int4* __restrict__ cs, int* __re
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
-void launch_barrier_energy(hipStream_t s, long n, const int* cs, const double* X, const double* X0, double dh,
-                           double kappa, double th2, double* part, int* err)
+void launch_barrier_energy(hipStream_t s, int nblk, const int* lo, const int* hi, const int* cs, const double* X,
+                           const double* X0, const BarrierParams& bp, double* part, int* err)
 {
-    IDP_LAUNCH(k_barrier_energy, grid_for(n, RB), RB, 0, s, n, (const int4*)cs, X, X0, dh, kappa, th2, part, err);
+    IDP_LAUNCH(k_barrier_energy, nblk, RB, 0, s, lo, hi, (const int4*)cs, X, X0, bp, part, err);
 }
-int barrier_energy_blocks(long n) { return grid_for(n, RB); }
 
-void launch_min_dist2(hipStream_t s, long n, const int* cs, const double* X, double* dist2,
+void launch_min_dist2(hipStream_t s, long n, const int* cs, const double* X, const int* vinst, double* dist2,
                       unsigned long long* dmin)
 {
-    IDP_LAUNCH(k_min_dist2, grid_for(n, RB), RB, 0, s, n, (const int4*)cs, X, dist2, dmin);
+    IDP_LAUNCH(k_min_dist2, grid_for(n, RB), RB, 0, s, n, (const int4*)cs, X, vinst, dist2, dmin);
 }
 
-void launch_barrier_grad(hipStream_t s, long n, const int* cs, const double* X, const double* X0, double dh,
-                         double kappa, double th2, double* gout, int* err)
+void launch_barrier_grad(hipStream_t s, long n, const int* cs, const double* X, const double* X0,
+                         const BarrierParams& bp, double* gout, int* err)
 {
-    IDP_LAUNCH(k_barrier_grad, grid_for(n, 256), 256, 0, s, n, (const int4*)cs, X, X0, dh, kappa, th2, gout, err);
+    IDP_LAUNCH(k_barrier_grad, grid_for(n, 256), 256, 0, s, n, (const int4*)cs, X, X0, bp, gout, err);
 }
 
-void launch_barrier_hess(hipStream_t s, long n, const int* cs, const double* X, const double* X0, double dh,
-                         double kappa, double th2, int project_spd, int mode, double* out, int* vid, int* err,
+void launch_barrier_hess(hipStream_t s, long n, const int* cs, const double* X, const double* X0,
+                         const BarrierParams& bp, int project_spd, int mode, double* out, int* vid, int* err,
                          int* counts)
 {
-    IDP_LAUNCH(k_barrier_hess, grid_for(n, WS), WS, 0, s, n, (const int4*)cs, X, X0, dh, kappa, th2, project_spd,
-               mode, out, vid, err, counts);
+    IDP_LAUNCH(k_barrier_hess, grid_for(n, WS), WS, 0, s, n, (const int4*)cs, X, X0, bp, project_spd, mode, out,
+               vid, err, counts);
 }
 
 void launch_constraint_vids(hipStream_t s, long n, const int* cs, int* vid, int* nv)

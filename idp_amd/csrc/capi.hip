@@ -20,6 +20,8 @@
 
 using namespace idp;
 
+struct BatchEngine;
+
 struct idp_ctx {
     int device = 0;
     hipStream_t s = nullptr;
@@ -42,6 +44,10 @@ struct idp_ctx {
     bool snap_valid = false;
     Profiler prof;
     DBuf<int> bcounts;
+    SegDev cseg;             // constraint blocks per instance (energy)
+    DBuf<double> bpar;       // per instance: dh[B], kappa[B], th2[B]
+    DBuf<double> esum;       // per-instance partial-sum results
+    BatchEngine* eng = nullptr;  // batched / served solves
 };
 
 namespace {
@@ -161,45 +167,95 @@ struct EnergyOut {
     double E;
 };
 
-// Compute_IncPotential: elastic (w = h^2) + inertia + barrier over the
-// context's current constraint set
-EnergyOut energy(idp_ctx* c, const double* X, bool with_barrier, double h, double dHat2, double kappa,
-                 double thickness)
+// per-instance sums of block partials: one block per instance, the partials of
+// instance i at [pbeg[i], pbeg[i+1]) added in the fixed order of
+// sum_partials_block (with one instance: the plain sum over all partials)
+__global__ void __launch_bounds__(RB) k_seg_sums(const int* __restrict__ pbeg, const double* __restrict__ part,
+                                                 double* __restrict__ out)
+{
+    __shared__ double sh[RB];
+    const int i = blockIdx.x;
+    const double s = sum_partials_block(part + pbeg[i], pbeg[i + 1] - pbeg[i], sh);
+    if (threadIdx.x == 0) out[i] = s;
+}
+
+BarrierParams bparams(idp_ctx* c)
+{
+    const int B = c->m.inst.B;
+    return BarrierParams{c->bpar.p, c->bpar.p + B, c->bpar.p + 2 * B, B > 1 ? c->m.inst.vinst.p : nullptr};
+}
+
+// per-instance barrier parameters from dHat2, kappa, thickness (host, B each)
+void set_bparams(idp_ctx* c, const double* dHat2, const double* kappa, const double* thickness)
+{
+    const int B = c->m.inst.B;
+    std::vector<double> h(3 * B);
+    for (int i = 0; i < B; ++i) {
+        h[i] = dHat2[i] + 2 * std::sqrt(dHat2[i]) * thickness[i];
+        h[B + i] = kappa[i];
+        h[2 * B + i] = thickness[i] * thickness[i];
+    }
+    c->bpar.upload(h.data(), h.size(), c->s);
+}
+
+void set_bparams1(idp_ctx* c, double dHat2, double kappa, double thickness)
+{
+    set_bparams(c, &dHat2, &kappa, &thickness);
+}
+
+// constraint block table from the per-instance constraint ranges
+void update_cseg(idp_ctx* c)
+{
+    if (c->cw.h_cbeg.size() != (size_t)c->m.inst.B + 1) c->cw.h_cbeg = {0, c->cw.ncs};
+    c->cseg.build(c->cw.h_cbeg, RB, c->s);
+}
+
+// Compute_IncPotential per instance: elastic (w = h^2) + inertia + barrier
+// over the current constraint set (barrier parameters from c->bpar)
+void energies(idp_ctx* c, const double* X, bool with_barrier, double h, std::vector<EnergyOut>& out)
 {
     auto s = c->s;
     auto& m = c->m;
-    int* fl = c->flags.ensure(4, s);
-    IDP_HIP(hipMemsetAsync(fl, 0, 2 * sizeof(int), s));
+    const InstDev& I = m.inst;
+    const int B = I.B;
+    int* fl = c->flags.ensure(2 * (size_t)B + 4, s);
+    IDP_HIP(hipMemsetAsync(fl, 0, 2 * B * sizeof(int), s));
     const int tok = c->prof.begin(IDP_K_ENERGY);
-    int nbt = tet_energy_blocks(m.nT), nbi = inertia_energy_blocks(m.nV);
-    double* pe = c->part_e.ensure(nbt + 1, s);
-    double* pi = c->part_i.ensure(nbi + 1, s);
-    launch_tet_energy(s, m, X, h * h, pe, fl);
-    launch_inertia_energy(s, m.nV, X, c->xt.p, m.mass.p, pi);
-    long ncs = c->cw.ncs;
-    double Eb = 0.0;
-    bool bar = with_barrier && ncs > 0;
-    int nbb = bar ? barrier_energy_blocks(ncs) : 0;
-    double* pb = c->part_b.ensure(nbb + 1, s);
+    double* pe = c->part_e.ensure(I.tblk.nblk + 1, s);
+    double* pi = c->part_i.ensure(I.vblk.nblk + 1, s);
+    launch_tet_energy(s, m, I.tblk, X, h * h, pe, fl);
+    launch_inertia_energy(s, I.vblk, X, c->xt.p, m.mass.p, pi);
+    const bool bar = with_barrier && c->cw.ncs > 0;
+    double* es = c->esum.ensure(3 * (size_t)B, s);
+    IDP_LAUNCH(k_seg_sums, B, RB, 0, s, I.tblk.pbeg.p, pe, es);
+    IDP_LAUNCH(k_seg_sums, B, RB, 0, s, I.vblk.pbeg.p, pi, es + B);
     if (bar) {
-        double th2 = thickness * thickness;
-        double dh = dHat2 + 2 * std::sqrt(dHat2) * thickness;
-        launch_barrier_energy(s, ncs, (const int*)c->cw.cs.p, X, m.X0.p, dh, kappa, th2, pb, fl + 1);
+        double* pb = c->part_b.ensure(c->cseg.nblk + 1, s);
+        launch_barrier_energy(s, c->cseg.nblk, c->cseg.lo.p, c->cseg.hi.p, (const int*)c->cw.cs.p, X, m.X0.p,
+                              bparams(c), pb, fl + B);
+        IDP_LAUNCH(k_seg_sums, B, RB, 0, s, c->cseg.pbeg.p, pb, es + 2 * B);
     }
     c->prof.end(IDP_K_ENERGY, tok);
-    int hf[2];
-    IDP_HIP(hipMemcpyAsync(hf, fl, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    std::vector<int> hf(2 * B);
+    std::vector<double> he(3 * B, 0.0);
+    IDP_HIP(hipMemcpyAsync(hf.data(), fl, 2 * B * sizeof(int), hipMemcpyDeviceToHost, s));
+    IDP_HIP(hipMemcpyAsync(he.data(), es, (bar ? 3 : 2) * B * sizeof(double), hipMemcpyDeviceToHost, s));
     IDP_HIP(hipStreamSynchronize(s));
-    EnergyOut o;
-    o.valid = hf[0] == 0;
-    o.dist_ok = hf[1] == 0;
-    o.E = 0;
-    if (!o.valid) return o;
-    double Ee = dev_sum_partials(c->L, s, nbt, pe);
-    double Ei = dev_sum_partials(c->L, s, nbi, pi);
-    if (bar) Eb = dev_sum_partials(c->L, s, nbb, pb);
-    o.E = Ee + Ei + Eb;
-    return o;
+    out.resize(B);
+    for (int i = 0; i < B; ++i) {
+        EnergyOut& o = out[i];
+        o.valid = hf[i] == 0;
+        o.dist_ok = hf[B + i] == 0;
+        o.E = o.valid ? he[i] + he[B + i] + (bar ? he[2 * B + i] : 0.0) : 0.0;
+    }
+}
+
+// one instance (plain mesh)
+EnergyOut energy(idp_ctx* c, const double* X, bool with_barrier, double h)
+{
+    std::vector<EnergyOut> o;
+    energies(c, X, with_barrier, h, o);
+    return o[0];
 }
 
 double min_dist2_dev(idp_ctx* c, const double* X, long n, const int* cs, double* dist2)
@@ -208,7 +264,7 @@ double min_dist2_dev(idp_ctx* c, const double* X, long n, const int* cs, double*
     unsigned long long* dm = c->dmin.ensure(1, s);
     unsigned long long init = 0x7ff0000000000000ull;  // +inf
     IDP_HIP(hipMemcpyAsync(dm, &init, sizeof(init), hipMemcpyHostToDevice, s));
-    launch_min_dist2(s, n, cs, X, dist2, dm);
+    launch_min_dist2(s, n, cs, X, nullptr, dist2, dm);
     unsigned long long r;
     IDP_HIP(hipMemcpyAsync(&r, dm, sizeof(r), hipMemcpyDeviceToHost, s));
     IDP_HIP(hipStreamSynchronize(s));
@@ -218,8 +274,7 @@ double min_dist2_dev(idp_ctx* c, const double* X, long n, const int* cs, double*
 }
 
 // gradient at X with the current constraint set (structure must be built)
-void gradient(idp_ctx* c, const double* X, double h, double dHat2, double kappa, double thickness, bool with_barrier,
-              double* g, double* rhs)
+void gradient(idp_ctx* c, const double* X, double h, bool with_barrier, double* g, double* rhs)
 {
     auto s = c->s;
     auto& m = c->m;
@@ -229,18 +284,18 @@ void gradient(idp_ctx* c, const double* X, double h, double dHat2, double kappa,
     const double* gb = nullptr;
     long ncs = c->cw.ncs;
     if (with_barrier && ncs) {
-        int* fl = c->flags.ensure(4, s);
-        IDP_HIP(hipMemsetAsync(fl + 2, 0, sizeof(int), s));
-        double th2 = thickness * thickness, dh = dHat2 + 2 * std::sqrt(dHat2) * thickness;
+        const int B = m.inst.B;
+        int* fl = c->flags.ensure(4 * (size_t)B + 4, s) + 2 * B;
+        IDP_HIP(hipMemsetAsync(fl, 0, B * sizeof(int), s));
         double* b = c->gb.ensure(12 * (size_t)ncs, s);
-        launch_barrier_grad(s, ncs, (const int*)c->cw.cs.p, X, m.X0.p, dh, kappa, th2, b, fl + 2);
+        launch_barrier_grad(s, ncs, (const int*)c->cw.cs.p, X, m.X0.p, bparams(c), b, fl);
         gb = b;
     }
     contact_gradient_gather(c->L, s, m, X, c->xt.p, gl, gb, g, rhs);
     c->prof.end(IDP_K_GRADIENT, tok);
 }
 
-void hessian(idp_ctx* c, const double* X, double h, double dHat2, double kappa, double thickness, bool with_barrier)
+void hessian(idp_ctx* c, const double* X, double h, bool with_barrier)
 {
     auto s = c->s;
     auto& m = c->m;
@@ -251,11 +306,11 @@ void hessian(idp_ctx* c, const double* X, double h, double dHat2, double kappa, 
     const double* Hb = nullptr;
     long ncs = c->cw.ncs;
     if (with_barrier && ncs) {
-        int* fl = c->flags.ensure(4, s);
-        double th2 = thickness * thickness, dh = dHat2 + 2 * std::sqrt(dHat2) * thickness;
+        const int B = m.inst.B;
+        int* fl = c->flags.ensure(4 * (size_t)B + 4, s) + 3 * B;
         double* b = c->Hb.ensure(144 * (size_t)ncs, s);
         tok = c->prof.begin(IDP_K_BARRIER_HESS);
-        launch_barrier_hess(s, ncs, (const int*)c->cw.cs.p, X, m.X0.p, dh, kappa, th2, 1, 1, b, nullptr, fl + 3,
+        launch_barrier_hess(s, ncs, (const int*)c->cw.cs.p, X, m.X0.p, bparams(c), 1, 1, b, nullptr, fl,
                             c->prof.on ? c->bcounts.p : nullptr);
         c->prof.end(IDP_K_BARRIER_HESS, tok);
         Hb = b;
@@ -267,6 +322,8 @@ void set_constraints_from_host(idp_ctx* c, long n, const int* cs)
 {
     c->cw.cs.upload((const int4*)cs, n, c->s);
     c->cw.ncs = n;
+    c->cw.h_cbeg = {0, n};
+    update_cseg(c);
 }
 
 const double* stage_x(idp_ctx* c, const double* X)
@@ -276,6 +333,569 @@ const double* stage_x(idp_ctx* c, const double* X)
         return c->hx.p;
     }
     return c->x.p;
+}
+
+// ---------------------------------------------------------------------------
+// per-instance vector helpers of the batched Newton loop (mask: instances
+// that take part; the arithmetic per element is that of the plain loop)
+// ---------------------------------------------------------------------------
+__global__ void k_axpy_inst(int nV, const int* __restrict__ vinst, const unsigned char* __restrict__ mask,
+                            const double* __restrict__ a, const double* __restrict__ alpha,
+                            const double* __restrict__ b, double* __restrict__ x)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nV) return;
+    const int i = vinst[v];
+    if (mask && !mask[i]) return;
+    const double al = alpha[i];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) x[3 * v + d] = a[3 * v + d] + al * b[3 * v + d];
+}
+
+__global__ void k_copy_inst(int nV, const int* __restrict__ vinst, const unsigned char* __restrict__ mask,
+                            const double* __restrict__ src, double* __restrict__ dst)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nV) return;
+    if (mask && !mask[vinst[v]]) return;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) dst[3 * v + d] = src[3 * v + d];
+}
+
+__global__ void k_velocity_inst(int nV, const int* __restrict__ vinst, const unsigned char* __restrict__ mask,
+                                const double* __restrict__ x, const double* __restrict__ xn, double h,
+                                double* __restrict__ v)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nV) return;
+    if (mask && !mask[vinst[u]]) return;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) v[3 * u + d] = (x[3 * u + d] - xn[3 * u + d]) / h;
+}
+
+// partial sums of a.b over blocks [lo, hi) (one instance per block)
+__global__ void __launch_bounds__(RB) k_dot_parts_seg(const int* __restrict__ lo, const int* __restrict__ hi,
+                                                      const double* __restrict__ a, const double* __restrict__ b,
+                                                      double* __restrict__ part)
+{
+    __shared__ double sh[RB];
+    const int i = lo[blockIdx.x] + threadIdx.x;
+    const double d = (i < hi[blockIdx.x]) ? a[i] * b[i] : 0.0;
+    const double s = block_sum<RB>(d, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// kappa test (IMPLICIT_EULER.h:566-595) per instance: a pair of the previous
+// constraint set whose squared distance was below 1e-18 and shrank
+__global__ void k_kappa_test(long n, const int4* __restrict__ cs, const int* __restrict__ vinst,
+                             const double* __restrict__ cur, const double* __restrict__ prev, int* __restrict__ flag)
+{
+    const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    if (prev[j] < 1e-18 && cur[j] < prev[j]) {
+        const int4 c = cs[j];
+        const int v = c.x >= 0 ? c.x : -c.x - 1;  // every encoding has a vertex of its instance in slot 0
+        flag[vinst[v]] = 1;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Batched Newton engine: every stage runs over all active instances at once;
+// each instance keeps the state of its own Newton loop (IMPLICIT_EULER.h:
+// 152-891) and takes part in a stage only when its own loop would run it.
+// start() begins solves in some instances while others are mid-solve, and
+// iterate() advances every active instance by one Newton iteration, so
+// instances can join and leave between iterations (continuous batching).
+// ---------------------------------------------------------------------------
+struct BatchEngine {
+    bool configured = false;
+    double h = 0, thickness = 0;
+    bool coll = true;
+    double rtol = 1e-8;
+    int maxit = 10000, max_newton = 0;
+    int B = 0, n_active = 0;
+    std::vector<double> kappa, kmax, tol, dHat2, th, Eprev, alpha, feasible, E;
+    std::vector<std::deque<double>> res;
+    std::vector<char> useGD, active, finished_now;
+    std::vector<int> PN, ndbc;
+    std::vector<idp_step_stats> S;
+    std::vector<unsigned char> mask;
+    DBuf<unsigned char> dmask, dmask2;
+    DBuf<double> dalpha;
+    std::vector<EnergyOut> eo;
+
+    int configure(idp_ctx* c, double h_, bool coll_, double th_, const idp_solver_opts* opts)
+    {
+        B = c->m.inst.B;
+        stream = c->s;
+        h = h_;
+        coll = coll_;
+        thickness = th_;
+        rtol = (opts && opts->pcg_rtol > 0) ? opts->pcg_rtol : 1e-8;
+        maxit = (opts && opts->pcg_maxit > 0) ? opts->pcg_maxit : 10000;
+        max_newton = opts ? opts->max_newton : 0;
+        kappa.assign(B, 0);
+        kmax.assign(B, 0);
+        tol.assign(B, 0);
+        dHat2.assign(B, 1e-6);
+        th.assign(B, thickness);
+        Eprev.assign(B, 0);
+        alpha.assign(B, 1);
+        feasible.assign(B, 1);
+        E.assign(B, 0);
+        res.assign(B, {});
+        useGD.assign(B, 0);
+        active.assign(B, 0);
+        finished_now.assign(B, 0);
+        PN.assign(B, 0);
+        S.assign(B, idp_step_stats{});
+        mask.assign(B, 0);
+        n_active = 0;
+        const long n3 = 3L * c->m.nV;
+        auto s = c->s;
+        c->xn.ensure(n3, s);
+        c->xt.ensure(n3, s);
+        c->xprev.ensure(n3, s);
+        c->sol.ensure(n3, s);
+        c->rhs.ensure(n3, s);
+        c->g.ensure(n3, s);
+        ndbc.assign(B, 0);
+        std::vector<unsigned char> hd(c->m.nV);
+        c->m.dbc.download(hd.data(), c->m.nV, s);
+        IDP_HIP(hipStreamSynchronize(s));
+        const InstDev& I = c->m.inst;
+        for (int i = 0; i < B; ++i)
+            for (long v = I.vbeg[i]; v < I.vbeg[i + 1]; ++v) ndbc[i] += hd[v];
+        c->ncs_prev = 0;
+        c->cw.h_cbeg.assign(B + 1, 0);
+        c->cw.ncs = 0;
+        update_cseg(c);
+        configured = true;
+        return IDP_OK;
+    }
+
+    hipStream_t stream = nullptr;
+    const unsigned char* up_mask() { return dmask.upload_get(mask.data(), B, stream); }
+
+    // Begin solves in the listed instances (their x and v are set): x_n = x,
+    // x~ = x + h v, constraint set and energy at x.  Arrays are indexed by
+    // instance (B entries); kappa3 holds 3 per instance.
+    int start(idp_ctx* c, const std::vector<int>& list, const double* tol_, const double* dHat2_,
+              const double* kappa3, bool by_instance = true)
+    {
+        auto s = c->s;
+        MeshDev& m = c->m;
+        const InstDev& I = m.inst;
+        (void)by_instance;
+        std::fill(mask.begin(), mask.end(), 0);
+        for (int i : list) {
+            mask[i] = 1;
+            active[i] = 1;
+            ++n_active;
+            tol[i] = tol_[i];
+            dHat2[i] = dHat2_[i];
+            kappa[i] = kappa3[3 * i];
+            kmax[i] = kappa3[3 * i + 1];
+            res[i].clear();
+            useGD[i] = 0;
+            PN[i] = 0;
+            std::memset(&S[i], 0, sizeof(idp_step_stats));
+        }
+        set_bparams(c, dHat2.data(), kappa.data(), th.data());
+        const unsigned char* dm = dmask.upload_get(mask.data(), B, s);
+        const int vgrid = grid_for(m.nV, 256);
+        // Xn = X; Xtilde = X + h v (zero body force, IMPLICIT_EULER.h:196-217)
+        IDP_LAUNCH(k_copy_inst, vgrid, 256, 0, s, m.nV, I.vinst.p, dm, c->x.p, c->xn.p);
+        std::vector<double> hv(B, h);
+        IDP_LAUNCH(k_axpy_inst, vgrid, 256, 0, s, m.nV, I.vinst.p, dm, c->x.p, dalpha.upload_get(hv.data(), B, s),
+                   c->v.p, c->xt.p);
+        if (coll) cset(c, dm);
+        energies(c, c->x.p, coll, h, eo);
+        for (int i : list) {
+            if (!eo[i].valid) return IDP_E_INVALID_START;
+            if (!eo[i].dist_ok) return IDP_E_NONPOS_DIST;
+            Eprev[i] = eo[i].E;
+        }
+        return IDP_OK;
+    }
+
+    void cset(idp_ctx* c, const unsigned char* dm)
+    {
+        double ta = now_ms();
+        const int tk = c->prof.begin(IDP_K_CONSTRAINT_SET);
+        contact_constraint_set(c->cw, c->s, c->m, c->x.p, dHat2.data(), thickness, dm);
+        update_cseg(c);
+        c->prof.end(IDP_K_CONSTRAINT_SET, tk);
+        const double dt = now_ms() - ta;
+        for (int i = 0; i < B; ++i)
+            if (mask[i]) {
+                S[i].constraint_sets++;
+                S[i].ms_constraint_set += dt;
+            }
+    }
+
+    // per-instance min dist^2 over the current constraint set
+    void min_dist(idp_ctx* c, double* d2out, std::vector<double>& md)
+    {
+        auto s = c->s;
+        unsigned long long* dm = c->dmin.ensure(B, s);
+        std::vector<unsigned long long> init(B, 0x7ff0000000000000ull);
+        IDP_HIP(hipMemcpyAsync(dm, init.data(), B * 8, hipMemcpyHostToDevice, s));
+        if (c->cw.ncs)
+            launch_min_dist2(s, c->cw.ncs, (const int*)c->cw.cs.p, c->x.p, c->m.inst.vinst.p, d2out, dm);
+        std::vector<unsigned long long> r(B);
+        IDP_HIP(hipMemcpyAsync(r.data(), dm, B * 8, hipMemcpyDeviceToHost, s));
+        IDP_HIP(hipStreamSynchronize(s));
+        md.resize(B);
+        for (int i = 0; i < B; ++i) std::memcpy(&md[i], &r[i], 8);
+    }
+
+    bool has_constraints(idp_ctx* c, int i) const
+    {
+        return c->cw.h_cbeg.size() == (size_t)B + 1 && c->cw.h_cbeg[i + 1] > c->cw.h_cbeg[i];
+    }
+
+    // one Newton iteration of every active instance
+    int iterate(idp_ctx* c)
+    {
+        auto s = c->s;
+        MeshDev& m = c->m;
+        const InstDev& I = m.inst;
+        const int vgrid = grid_for(m.nV, 256);
+        double* x = c->x.p;
+        double* xprev = c->xprev.p;
+        double* sol = c->sol.p;
+        double* rhs = c->rhs.p;
+        double* g = c->g.p;
+        std::fill(finished_now.begin(), finished_now.end(), 0);
+        for (int i = 0; i < B; ++i) mask[i] = active[i];
+        const unsigned char* dact = up_mask();
+        // ---- gradient and Hessian at x with the current constraint sets
+        double t0 = now_ms();
+        if (coll) contact_structure(c->L, s, m.nV, c->cw.ncs, (const int*)c->cw.cs.p);
+        else c->L.ncb = 0;
+        gradient(c, x, h, coll, g, rhs);
+        bool anyN = false;
+        for (int i = 0; i < B; ++i) anyN |= active[i] && !useGD[i];
+        std::vector<int> its(B, 0);
+        std::vector<double> rel(B, 0.0);
+        if (anyN) {
+            hessian(c, x, h, coll);
+            std::vector<unsigned char> sm(B);
+            for (int i = 0; i < B; ++i) sm[i] = active[i] && !useGD[i];
+            pcg_solve_batch(c->L, s, m, rhs, sol, rtol, maxit, dmask2.upload_get(sm.data(), B, s), its.data(),
+                            rel.data());
+            for (int i = 0; i < B; ++i) {
+                if (!sm[i]) continue;
+                if (its[i] < 0) useGD[i] = 1;  // DIRECT_SOLVER.h:42-44 -> gradient descent
+                else {
+                    S[i].pcg_iters += its[i];
+                    S[i].last_relres = rel[i];
+                }
+            }
+        }
+        for (int i = 0; i < B; ++i) mask[i] = active[i] && useGD[i];
+        IDP_LAUNCH(k_copy_inst, vgrid, 256, 0, s, m.nV, I.vinst.p, up_mask(), rhs, sol);
+        const double tsolve = now_ms() - t0;
+        for (int i = 0; i < B; ++i) {
+            if (!active[i]) continue;
+            if (useGD[i]) S[i].gd_iters++;
+            S[i].ms_solve += tsolve;
+        }
+        // ---- Line_Search, part 1: halve until no tet inverts
+        for (int i = 0; i < B; ++i) mask[i] = active[i];
+        dact = up_mask();
+        IDP_LAUNCH(k_copy_inst, vgrid, 256, 0, s, m.nV, I.vinst.p, dact, x, xprev);
+        for (int i = 0; i < B; ++i) alpha[i] = 1.0;
+        std::vector<unsigned char> srch(active.begin(), active.end());
+        for (int guard = 0;; ++guard) {
+            if (guard > 4000) return IDP_E_INVALID;
+            bool any = false;
+            for (int i = 0; i < B; ++i) any |= srch[i] != 0;
+            if (!any) break;
+            mask = srch;
+            IDP_LAUNCH(k_axpy_inst, vgrid, 256, 0, s, m.nV, I.vinst.p, up_mask(), xprev,
+                       dalpha.upload_get(alpha.data(), B, s), sol, x);
+            energies(c, x, false, h, eo);
+            for (int i = 0; i < B; ++i)
+                if (srch[i]) {
+                    if (eo[i].valid) srch[i] = 0;
+                    else alpha[i] /= 2.0;
+                }
+        }
+        for (int i = 0; i < B; ++i) mask[i] = active[i];
+        dact = up_mask();
+        IDP_LAUNCH(k_copy_inst, vgrid, 256, 0, s, m.nV, I.vinst.p, dact, xprev, x);
+        if (coll) {
+            t0 = now_ms();
+            long nc = 0;
+            std::vector<double> a2(alpha);
+            contact_ccd_batch(c->cw, s, m, x, sol, thickness, a2.data(), dact, &nc);
+            const double tc = now_ms() - t0;
+            for (int i = 0; i < B; ++i)
+                if (active[i]) {
+                    alpha[i] = a2[i];
+                    S[i].ms_ccd += tc;
+                    if (n_active == 1) S[i].ccd_candidates += nc;
+                }
+        }
+        for (int i = 0; i < B; ++i)
+            if (active[i]) {
+                feasible[i] = alpha[i];
+                if (PN[i] == 0) S[i].first_ccd_alpha = alpha[i];
+            }
+        // ---- Line_Search, part 2: backtracking on energy with the
+        // minimum-distance safeguard (IMPLICIT_EULER.h:102-146)
+        srch.assign(active.begin(), active.end());
+        for (int guard = 0;; ++guard) {
+            bool any = false;
+            for (int i = 0; i < B; ++i) any |= srch[i] != 0;
+            if (!any) break;
+            if (guard > 4000) return IDP_E_INVALID;  // alpha underflow: x no longer moves
+            mask = srch;
+            IDP_LAUNCH(k_axpy_inst, vgrid, 256, 0, s, m.nV, I.vinst.p, up_mask(), xprev,
+                       dalpha.upload_get(alpha.data(), B, s), sol, x);
+            energies(c, x, false, h, eo);
+            std::vector<unsigned char> valid(B, 0), retry(B, 0);
+            bool anyv = false;
+            for (int i = 0; i < B; ++i)
+                if (srch[i]) {
+                    valid[i] = eo[i].valid;
+                    E[i] = eo[i].E;
+                    anyv |= valid[i] != 0;
+                }
+            if (anyv && coll) {
+                mask = valid;
+                cset(c, up_mask());
+                std::vector<double> md;
+                min_dist(c, nullptr, md);
+                for (int i = 0; i < B; ++i)
+                    if (valid[i] && has_constraints(c, i) && md[i] - thickness * thickness <= 0) {
+                        alpha[i] /= 2;
+                        E[i] = Eprev[i] + 1;
+                        S[i].line_search_trials++;
+                        retry[i] = 1;
+                    }
+            }
+            bool anyb = false;
+            for (int i = 0; i < B; ++i) anyb |= valid[i] && !retry[i];
+            if (anyb) {
+                energies(c, x, coll, h, eo);
+                for (int i = 0; i < B; ++i)
+                    if (valid[i] && !retry[i]) {
+                        valid[i] = eo[i].valid;
+                        E[i] = eo[i].E;
+                        if (!eo[i].dist_ok) return IDP_E_NONPOS_DIST;
+                    }
+            }
+            for (int i = 0; i < B; ++i) {
+                if (!srch[i] || retry[i]) continue;
+                alpha[i] /= 2.0;
+                S[i].line_search_trials++;
+                if (!(E[i] > Eprev[i] || !valid[i])) srch[i] = 0;  // accepted
+            }
+        }
+        for (int i = 0; i < B; ++i)
+            if (active[i]) {
+                Eprev[i] = E[i];
+                S[i].last_alpha = alpha[i] * 2;
+            }
+        // ---- kappa update (IMPLICIT_EULER.h:566-595), instances past their
+        // first iteration (the previous constraint set is theirs)
+        if (coll && c->ncs_prev) {
+            const long np = c->ncs_prev;
+            double* cur = c->tmpv.ensure(np, s);
+            unsigned long long* dm = c->dmin.ensure(B, s);
+            int* kflag = c->flags.ensure(4 * (size_t)B + 4, s);
+            launch_min_dist2(s, np, (const int*)c->cs_prev.p, x, I.vinst.p, cur, dm);
+            IDP_HIP(hipMemsetAsync(kflag, 0, B * sizeof(int), s));
+            IDP_LAUNCH(k_kappa_test, grid_for(np, 256), 256, 0, s, np, c->cs_prev.p, I.vinst.p, cur,
+                       c->dist2_prev.p, kflag);
+            std::vector<int> hk(B);
+            IDP_HIP(hipMemcpyAsync(hk.data(), kflag, B * sizeof(int), hipMemcpyDeviceToHost, s));
+            IDP_HIP(hipStreamSynchronize(s));
+            bool anyk = false;
+            for (int i = 0; i < B; ++i)
+                if (active[i] && PN[i] > 0 && hk[i] && kappa[i] < kmax[i]) {
+                    kappa[i] *= 2;
+                    anyk = true;
+                } else {
+                    hk[i] = 0;
+                }
+            if (anyk) {
+                set_bparams(c, dHat2.data(), kappa.data(), th.data());
+                energies(c, x, coll, h, eo);
+                for (int i = 0; i < B; ++i)
+                    if (hk[i]) Eprev[i] = eo[i].E;
+            }
+        }
+        if (coll) {
+            const long n = c->cw.ncs;
+            double* d2 = c->dist2_prev.ensure(n + 1, s);
+            std::vector<double> md;
+            min_dist(c, d2, md);
+            for (int i = 0; i < B; ++i)
+                if (active[i] && has_constraints(c, i)) S[i].min_dist2 = md[i] - thickness * thickness;
+            c->cs_prev.ensure(n + 1, s);
+            if (n) IDP_HIP(hipMemcpyAsync(c->cs_prev.p, c->cw.cs.p, n * sizeof(int4), hipMemcpyDeviceToDevice, s));
+            c->ncs_prev = n;
+        }
+        // ---- stopping rule (IMPLICIT_EULER.h:647-676, 759, 855)
+        double* pd = c->part_i.ensure(I.dblk.nblk + 1, s);
+        double* es = c->esum.ensure(3 * (size_t)B, s);
+        IDP_LAUNCH(k_dot_parts_seg, I.dblk.nblk, RB, 0, s, I.dblk.lo.p, I.dblk.hi.p, sol, sol, pd);
+        IDP_LAUNCH(k_seg_sums, B, RB, 0, s, I.dblk.pbeg.p, pd, es);
+        std::vector<double> ss(B);
+        IDP_HIP(hipMemcpyAsync(ss.data(), es, B * sizeof(double), hipMemcpyDeviceToHost, s));
+        IDP_HIP(hipStreamSynchronize(s));
+        std::fill(mask.begin(), mask.end(), 0);
+        bool anyf = false;
+        for (int i = 0; i < B; ++i) {
+            if (!active[i]) continue;
+            const long nv = I.vbeg[i + 1] - I.vbeg[i];
+            double L2 = std::sqrt(ss[i] / (nv - ndbc[i])) / h;
+            ++PN[i];
+            res[i].push_back(L2);
+            if (res[i].size() > 3) res[i].pop_front();
+            L2 = *std::max_element(res[i].begin(), res[i].end());
+            if (useGD[i]) L2 = tol[i] * 10;
+            useGD[i] = (alpha[i] * 2 < 1e-8 && feasible[i] > 1e-8);
+            S[i].newton_iters = PN[i];
+            S[i].last_res = L2;
+            S[i].E = Eprev[i];
+            S[i].kappa = kappa[i];
+            S[i].n_constraints = has_constraints(c, i) ? c->cw.h_cbeg[i + 1] - c->cw.h_cbeg[i] : 0;
+            const bool stop = (res[i].size() >= 3 && L2 <= tol[i]) || (max_newton > 0 && PN[i] >= max_newton);
+            if (stop) {
+                active[i] = 0;
+                finished_now[i] = 1;
+                mask[i] = 1;
+                anyf = true;
+                --n_active;
+            }
+        }
+        // v = (x - xn) / h for the instances that finished
+        if (anyf) {
+            std::vector<double> ih(B, 1.0 / h);
+            IDP_LAUNCH(k_velocity_inst, vgrid, 256, 0, s, m.nV, I.vinst.p, up_mask(), x, c->xn.p, h, c->v.p);
+            IDP_HIP(hipStreamSynchronize(s));
+        }
+        c->prof.checkpoint();
+        return IDP_OK;
+    }
+};
+
+namespace {
+
+// Uploads a mesh of one or more instances (vertex ranges vbeg, tet ranges
+// tbeg; tets in global vertex ids, instance-major).
+int setup_mesh(idp_ctx* c, int nV, const double* X_rest, int nT, const int* tets, const double* IB,
+               const double* vol, const double* lambda, const double* mu, const double* mass,
+               const std::vector<long>& vbeg, const std::vector<long>& tbeg)
+{
+        if (nV <= 0 || nT <= 0 || !X_rest || !tets || !IB || !vol || !lambda || !mu || !mass) return IDP_E_INVALID;
+        for (long i = 0; i < 4L * nT; ++i)
+            if (tets[i] < 0 || tets[i] >= nV) return IDP_E_INVALID;
+        IDP_HIP(hipSetDevice(c->device));
+        auto s = c->s;
+        MeshDev& m = c->m;
+        m.nV = nV;
+        m.nT = nT;
+        m.X0.upload(X_rest, 3 * (size_t)nV, s);
+        m.mass.upload(mass, nV, s);
+        m.tets.upload(tets, 4 * (size_t)nT, s);
+        m.IB.upload(IB, 9 * (size_t)nT, s);
+        m.vol.upload(vol, nT, s);
+        m.lam.upload(lambda, nT, s);
+        m.mu.upload(mu, nT, s);
+        std::vector<unsigned char> dbc(nV, 0);
+        m.dbc.upload(dbc.data(), nV, s);
+        c->ndbc = 0;
+        // surface
+        std::vector<int> bn;
+        std::vector<std::array<int, 2>> be;
+        std::vector<std::array<int, 3>> bt;
+        surface_host(nV, X_rest, nT, tets, bn, be, bt);
+        m.nb = (int)bn.size();
+        m.ne = (int)be.size();
+        m.nf = (int)bt.size();
+        m.bn.upload(bn.data(), bn.size(), s);
+        m.be.upload((const int*)be.data(), 2 * be.size(), s);
+        m.bt.upload((const int*)bt.data(), 3 * bt.size(), s);
+        std::vector<int> v2b(nV, -1);
+        for (int i = 0; i < m.nb; ++i) v2b[bn[i]] = i;
+        m.v2b.upload(v2b.data(), nV, s);
+        // static elastic pattern: bucket records by row, sort each row by (col, src)
+        std::vector<int> rcount(nV + 1, 0);
+        for (int t = 0; t < nT; ++t)
+            for (int a = 0; a < 4; ++a) rcount[tets[4 * t + a] + 1] += 4;
+        for (int v = 0; v < nV; ++v) rcount[v + 1] += rcount[v];
+        std::vector<std::pair<int, int>> rec(16 * (size_t)nT);  // (col, src)
+        std::vector<int> fill(rcount.begin(), rcount.end() - 1);
+        for (int t = 0; t < nT; ++t)
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b)
+                    rec[fill[tets[4 * t + a]]++] = {tets[4 * t + b], t * 16 + a * 4 + b};
+        std::vector<int> rowptr(nV + 1, 0), col, sptr, src, diag(nV, -1), rowof;
+        col.reserve(27 * (size_t)nV);
+        src.reserve(16 * (size_t)nT);
+        sptr.push_back(0);
+        for (int v = 0; v < nV; ++v) {
+            std::sort(rec.begin() + rcount[v], rec.begin() + rcount[v + 1]);
+            for (int k = rcount[v]; k < rcount[v + 1];) {
+                int j = k;
+                while (j < rcount[v + 1] && rec[j].first == rec[k].first) src.push_back(rec[j++].second);
+                if (rec[k].first == v) diag[v] = (int)col.size();
+                col.push_back(rec[k].first);
+                rowof.push_back(v);
+                sptr.push_back((int)src.size());
+                k = j;
+            }
+            rowptr[v + 1] = (int)col.size();
+            if (diag[v] < 0) return IDP_E_INVALID;  // vertex not in any tet
+        }
+        m.nnzb_e = (long)col.size();
+        m.e_rowptr.upload(rowptr.data(), rowptr.size(), s);
+        m.e_col.upload(col.data(), col.size(), s);
+        m.e_src_ptr.upload(sptr.data(), sptr.size(), s);
+        m.e_src.upload(src.data(), src.size(), s);
+        m.e_diag.upload(diag.data(), diag.size(), s);
+        init_erow(c->L, s, m, rowof);
+        // instance ranges of the boundary primitives (vertex ids are instance-major,
+        // so each instance's primitives are contiguous in the surface order)
+        const int B = (int)vbeg.size() - 1;
+        auto inst_of = [&](int v) { return (int)(std::upper_bound(vbeg.begin(), vbeg.end(), (long)v) - vbeg.begin()) - 1; };
+        auto ranges = [&](int n, auto first_vertex, std::vector<long>& beg) -> bool {
+            beg.assign(B + 1, 0);
+            int prev = 0;
+            for (int k = 0; k < n; ++k) {
+                const int in = inst_of(first_vertex(k));
+                if (in < prev) return false;
+                for (int q = prev + 1; q <= in; ++q) beg[q] = k;
+                prev = in;
+            }
+            for (int q = prev + 1; q <= B; ++q) beg[q] = n;
+            return true;
+        };
+        std::vector<long> nbeg, ebeg, fbeg;
+        if (!ranges(m.nb, [&](int k) { return bn[k]; }, nbeg) ||
+            !ranges(m.ne, [&](int k) { return std::min(be[k][0], be[k][1]); }, ebeg) ||
+            !ranges(m.nf, [&](int k) { return std::min(bt[k][0], std::min(bt[k][1], bt[k][2])); }, fbeg))
+            return IDP_E_INVALID;
+        build_inst_layout(m, vbeg, tbeg, nbeg, ebeg, fbeg, s);
+        c->cw.h_cbeg.clear();
+        // state
+        c->x.upload(X_rest, 3 * (size_t)nV, s);
+        std::vector<double> zero(3 * (size_t)nV, 0.0);
+        c->v.upload(zero.data(), zero.size(), s);
+        c->cw.ncs = 0;
+        c->ncs_prev = 0;
+        IDP_HIP(hipStreamSynchronize(s));
+        c->have_mesh = true;
+        c->snap_valid = false;
+        return IDP_OK;
 }
 
 }  // namespace
@@ -300,6 +920,7 @@ int idp_ctx_create(int device, idp_ctx** out)
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return IDP_E_NO_GPU;
     if (device < 0 || device >= n) return IDP_E_INVALID;
     idp_ctx* c = new idp_ctx();
+    c->eng = new BatchEngine();
     c->device = device;
     try {
         IDP_HIP(hipSetDevice(device));
@@ -309,6 +930,7 @@ int idp_ctx_create(int device, idp_ctx** out)
         c->cw.prof = &c->prof;
         c->L.prof = &c->prof;
     } catch (const HipError& e) {
+        delete c->eng;
         delete c;
         return e.code;
     }
@@ -324,6 +946,7 @@ int idp_ctx_destroy(idp_ctx* c)
         (void)hipStreamSynchronize(c->s);
         (void)hipStreamDestroy(c->s);
     }
+    delete c->eng;
     delete c;
     return IDP_OK;
 }
@@ -376,6 +999,8 @@ int idp_profile_read(idp_ctx* c, idp_profile* out)
             fl += (cnt[2 * k] - cnt[2 * k + 1]) * barrier_hess_flops(k, false) + cnt[2 * k + 1] * barrier_hess_flops(k, true);
         }
         out->work[IDP_K_BARRIER_HESS] = fl;
+        out->ccd_steps = c->prof.ccd_steps;
+        out->ccd_max_steps = c->prof.ccd_max_steps;
         return IDP_OK;
     });
 }
@@ -458,84 +1083,7 @@ int idp_set_mesh(idp_ctx* c, int nV, const double* X_rest, int nT, const int* te
                  const double* vol, const double* lambda, const double* mu, const double* mass)
 {
     return guarded(c, [&]() -> int {
-        if (nV <= 0 || nT <= 0 || !X_rest || !tets || !IB || !vol || !lambda || !mu || !mass) return IDP_E_INVALID;
-        for (long i = 0; i < 4L * nT; ++i)
-            if (tets[i] < 0 || tets[i] >= nV) return IDP_E_INVALID;
-        IDP_HIP(hipSetDevice(c->device));
-        auto s = c->s;
-        MeshDev& m = c->m;
-        m.nV = nV;
-        m.nT = nT;
-        m.X0.upload(X_rest, 3 * (size_t)nV, s);
-        m.mass.upload(mass, nV, s);
-        m.tets.upload(tets, 4 * (size_t)nT, s);
-        m.IB.upload(IB, 9 * (size_t)nT, s);
-        m.vol.upload(vol, nT, s);
-        m.lam.upload(lambda, nT, s);
-        m.mu.upload(mu, nT, s);
-        std::vector<unsigned char> dbc(nV, 0);
-        m.dbc.upload(dbc.data(), nV, s);
-        c->ndbc = 0;
-        // surface
-        std::vector<int> bn;
-        std::vector<std::array<int, 2>> be;
-        std::vector<std::array<int, 3>> bt;
-        surface_host(nV, X_rest, nT, tets, bn, be, bt);
-        m.nb = (int)bn.size();
-        m.ne = (int)be.size();
-        m.nf = (int)bt.size();
-        m.bn.upload(bn.data(), bn.size(), s);
-        m.be.upload((const int*)be.data(), 2 * be.size(), s);
-        m.bt.upload((const int*)bt.data(), 3 * bt.size(), s);
-        std::vector<int> v2b(nV, -1);
-        for (int i = 0; i < m.nb; ++i) v2b[bn[i]] = i;
-        m.v2b.upload(v2b.data(), nV, s);
-        // static elastic pattern: bucket records by row, sort each row by (col, src)
-        std::vector<int> rcount(nV + 1, 0);
-        for (int t = 0; t < nT; ++t)
-            for (int a = 0; a < 4; ++a) rcount[tets[4 * t + a] + 1] += 4;
-        for (int v = 0; v < nV; ++v) rcount[v + 1] += rcount[v];
-        std::vector<std::pair<int, int>> rec(16 * (size_t)nT);  // (col, src)
-        std::vector<int> fill(rcount.begin(), rcount.end() - 1);
-        for (int t = 0; t < nT; ++t)
-            for (int a = 0; a < 4; ++a)
-                for (int b = 0; b < 4; ++b)
-                    rec[fill[tets[4 * t + a]]++] = {tets[4 * t + b], t * 16 + a * 4 + b};
-        std::vector<int> rowptr(nV + 1, 0), col, sptr, src, diag(nV, -1), rowof;
-        col.reserve(27 * (size_t)nV);
-        src.reserve(16 * (size_t)nT);
-        sptr.push_back(0);
-        for (int v = 0; v < nV; ++v) {
-            std::sort(rec.begin() + rcount[v], rec.begin() + rcount[v + 1]);
-            for (int k = rcount[v]; k < rcount[v + 1];) {
-                int j = k;
-                while (j < rcount[v + 1] && rec[j].first == rec[k].first) src.push_back(rec[j++].second);
-                if (rec[k].first == v) diag[v] = (int)col.size();
-                col.push_back(rec[k].first);
-                rowof.push_back(v);
-                sptr.push_back((int)src.size());
-                k = j;
-            }
-            rowptr[v + 1] = (int)col.size();
-            if (diag[v] < 0) return IDP_E_INVALID;  // vertex not in any tet
-        }
-        m.nnzb_e = (long)col.size();
-        m.e_rowptr.upload(rowptr.data(), rowptr.size(), s);
-        m.e_col.upload(col.data(), col.size(), s);
-        m.e_src_ptr.upload(sptr.data(), sptr.size(), s);
-        m.e_src.upload(src.data(), src.size(), s);
-        m.e_diag.upload(diag.data(), diag.size(), s);
-        init_erow(c->L, s, m, rowof);
-        // state
-        c->x.upload(X_rest, 3 * (size_t)nV, s);
-        std::vector<double> zero(3 * (size_t)nV, 0.0);
-        c->v.upload(zero.data(), zero.size(), s);
-        c->cw.ncs = 0;
-        c->ncs_prev = 0;
-        IDP_HIP(hipStreamSynchronize(s));
-        c->have_mesh = true;
-        c->snap_valid = false;
-        return IDP_OK;
+        return setup_mesh(c, nV, X_rest, nT, tets, IB, vol, lambda, mu, mass, {0, (long)nV}, {0, (long)nT});
     });
 }
 
@@ -647,18 +1195,20 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
         IDP_HIP(hipMemcpyAsync(xn, x, n3 * 8, hipMemcpyDeviceToDevice, s));
         dev_axpy(s, n3, x, h, c->v.p, xt);
         double kappa = kappa3[0];
+        set_bparams1(c, dHat2, kappa, thickness);
         auto t0 = now_ms();
         auto cset = [&](const double* X) {
             double ta = now_ms();
             const int tk = c->prof.begin(IDP_K_CONSTRAINT_SET);
-            contact_constraint_set(c->cw, s, m, X, dHat2, thickness);
+            contact_constraint_set(c->cw, s, m, X, &dHat2, thickness);
+            update_cseg(c);
             c->prof.end(IDP_K_CONSTRAINT_SET, tk);
             S.constraint_sets++;
             S.ms_constraint_set += now_ms() - ta;
         };
         if (coll) cset(x);
         t0 = now_ms();
-        EnergyOut e0 = energy(c, x, coll, h, dHat2, kappa, thickness);
+        EnergyOut e0 = energy(c, x, coll, h);
         S.ms_energy += now_ms() - t0;
         if (!e0.valid) return IDP_E_INVALID_START;
         if (!e0.dist_ok) return IDP_E_NONPOS_DIST;
@@ -674,12 +1224,12 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
             t0 = now_ms();
             if (coll) contact_structure(c->L, s, m.nV, c->cw.ncs, (const int*)c->cw.cs.p);
             else c->L.ncb = 0;
-            gradient(c, x, h, dHat2, kappa, thickness, coll, g, rhs);
+            gradient(c, x, h, coll, g, rhs);
             IDP_HIP(hipStreamSynchronize(s));
             S.ms_gradient += now_ms() - t0;
             if (!useGD) {
                 t0 = now_ms();
-                hessian(c, x, h, dHat2, kappa, thickness, coll);
+                hessian(c, x, h, coll);
                 IDP_HIP(hipStreamSynchronize(s));
                 S.ms_hessian += now_ms() - t0;
                 t0 = now_ms();
@@ -707,7 +1257,7 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
                 if (++guard > 4000) return IDP_E_INVALID;
                 dev_axpy(s, n3, xprev, alpha, sol, x);
                 t0 = now_ms();
-                EnergyOut eo = energy(c, x, false, h, dHat2, kappa, thickness);
+                EnergyOut eo = energy(c, x, false, h);
                 S.ms_energy += now_ms() - t0;
                 valid = eo.valid;
                 if (!valid) alpha /= 2.0;
@@ -727,7 +1277,7 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
                 if (++guard > 4000) return IDP_E_INVALID;  // alpha underflow: x no longer moves
                 dev_axpy(s, n3, xprev, alpha, sol, x);
                 t0 = now_ms();
-                EnergyOut eo = energy(c, x, false, h, dHat2, kappa, thickness);
+                EnergyOut eo = energy(c, x, false, h);
                 S.ms_energy += now_ms() - t0;
                 valid = eo.valid;
                 E = eo.E;
@@ -746,7 +1296,7 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
                         }
                     }
                     t0 = now_ms();
-                    eo = energy(c, x, coll, h, dHat2, kappa, thickness);
+                    eo = energy(c, x, coll, h);
                     S.ms_energy += now_ms() - t0;
                     valid = eo.valid;
                     E = eo.E;
@@ -776,7 +1326,8 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
                 if (upd && kappa < kappa3[1]) {
                     kappa *= 2;
                     kappa3[0] = kappa;
-                    EnergyOut eo = energy(c, x, coll, h, dHat2, kappa, thickness);
+                    set_bparams1(c, dHat2, kappa, thickness);
+                    EnergyOut eo = energy(c, x, coll, h);
                     Eprev = eo.E;
                 }
             }
@@ -828,7 +1379,8 @@ int idp_constraint_set(idp_ctx* c, const double* X, double dHat2, double thickne
     return guarded(c, [&]() -> int {
         if (!c->have_mesh) return IDP_E_INVALID;
         const double* xd = stage_x(c, X);
-        long k = contact_constraint_set(c->cw, c->s, c->m, xd, dHat2, thickness);
+        long k = contact_constraint_set(c->cw, c->s, c->m, xd, &dHat2, thickness);
+        update_cseg(c);
         IDP_HIP(hipStreamSynchronize(c->s));
         if (n) *n = k;
         return IDP_OK;
@@ -852,13 +1404,13 @@ int idp_barrier_energy(idp_ctx* c, const double* X, long n, const int* cs, doubl
         if (!c->have_mesh) return IDP_E_INVALID;
         const double* xd = stage_x(c, X);
         set_constraints_from_host(c, n, cs);
+        set_bparams1(c, dHat2, kappa, thickness);
         int* fl = c->flags.ensure(4, c->s);
         IDP_HIP(hipMemsetAsync(fl, 0, sizeof(int), c->s));
-        int nb = barrier_energy_blocks(n);
+        const int nb = c->cseg.nblk;
         double* pb = c->part_b.ensure(nb + 1, c->s);
-        double dh = dHat2 + 2 * std::sqrt(dHat2) * thickness;
-        if (n) launch_barrier_energy(c->s, n, (const int*)c->cw.cs.p, xd, c->m.X0.p, dh, kappa, thickness * thickness,
-                                     pb, fl);
+        if (n) launch_barrier_energy(c->s, nb, c->cseg.lo.p, c->cseg.hi.p, (const int*)c->cw.cs.p, xd, c->m.X0.p,
+                                     bparams(c), pb, fl);
         int hf = 0;
         IDP_HIP(hipMemcpyAsync(&hf, fl, sizeof(int), hipMemcpyDeviceToHost, c->s));
         IDP_HIP(hipStreamSynchronize(c->s));
@@ -877,11 +1429,11 @@ int idp_barrier_gradient(idp_ctx* c, const double* X, long n, const int* cs, dou
         const double* xd = stage_x(c, X);
         set_constraints_from_host(c, n, cs);
         contact_structure(c->L, s, c->m.nV, n, (const int*)c->cw.cs.p);
+        set_bparams1(c, dHat2, kappa, thickness);
         int* fl = c->flags.ensure(4, s);
         IDP_HIP(hipMemsetAsync(fl, 0, sizeof(int), s));
-        double dh = dHat2 + 2 * std::sqrt(dHat2) * thickness;
         double* b = c->gb.ensure(12 * (size_t)std::max(n, 1L), s);
-        if (n) launch_barrier_grad(s, n, (const int*)c->cw.cs.p, xd, c->m.X0.p, dh, kappa, thickness * thickness, b, fl);
+        if (n) launch_barrier_grad(s, n, (const int*)c->cw.cs.p, xd, c->m.X0.p, bparams(c), b, fl);
         double* gd = c->g.ensure(3 * (size_t)c->m.nV, s);
         contact_gradient_gather(c->L, s, c->m, xd, nullptr, nullptr, n ? b : nullptr, gd, nullptr);
         int hf = 0;
@@ -900,15 +1452,13 @@ int idp_barrier_hessian_local(idp_ctx* c, const double* X, long n, const int* cs
         auto s = c->s;
         const double* xd = stage_x(c, X);
         set_constraints_from_host(c, n, cs);
+        set_bparams1(c, dHat2, kappa, thickness);
         int* fl = c->flags.ensure(4, s);
         IDP_HIP(hipMemsetAsync(fl, 0, sizeof(int), s));
-        double dh = dHat2 + 2 * std::sqrt(dHat2) * thickness;
         double* b = c->Hb.ensure(144 * (size_t)std::max(n, 1L), s);
         DBuf<int> dv;
         int* dvid = dv.ensure(4 * (size_t)std::max(n, 1L), s);
-        if (n)
-            launch_barrier_hess(s, n, (const int*)c->cw.cs.p, xd, c->m.X0.p, dh, kappa, thickness * thickness, project,
-                                0, b, dvid, fl);
+        if (n) launch_barrier_hess(s, n, (const int*)c->cw.cs.p, xd, c->m.X0.p, bparams(c), project, 0, b, dvid, fl);
         int hf = 0;
         IDP_HIP(hipMemcpyAsync(&hf, fl, sizeof(int), hipMemcpyDeviceToHost, s));
         if (n) {
@@ -962,9 +1512,9 @@ int idp_elastic_energy(idp_ctx* c, const double* X, double w, double* E, int* va
         const double* xd = stage_x(c, X);
         int* fl = c->flags.ensure(4, s);
         IDP_HIP(hipMemsetAsync(fl, 0, sizeof(int), s));
-        int nb = tet_energy_blocks(c->m.nT);
+        const int nb = c->m.inst.tblk.nblk;
         double* pe = c->part_e.ensure(nb + 1, s);
-        launch_tet_energy(s, c->m, xd, w, pe, fl);
+        launch_tet_energy(s, c->m, c->m.inst.tblk, xd, w, pe, fl);
         int hf = 0;
         IDP_HIP(hipMemcpyAsync(&hf, fl, sizeof(int), hipMemcpyDeviceToHost, s));
         IDP_HIP(hipStreamSynchronize(s));
@@ -1016,13 +1566,15 @@ int idp_newton_direction(idp_ctx* c, const double* X, const double* xtilde, doub
         size_t n3 = 3 * (size_t)m.nV;
         const double* xd = stage_x(c, X);
         c->xt.upload(xtilde, n3, s);
-        contact_constraint_set(c->cw, s, m, xd, dHat2, thickness);
+        contact_constraint_set(c->cw, s, m, xd, &dHat2, thickness);
+        update_cseg(c);
         contact_structure(c->L, s, m.nV, c->cw.ncs, (const int*)c->cw.cs.p);
         double* gd = c->g.ensure(n3, s);
         double* rhs = c->rhs.ensure(n3, s);
         double* sd = c->sol.ensure(n3, s);
-        gradient(c, xd, h, dHat2, kappa, thickness, true, gd, rhs);
-        hessian(c, xd, h, dHat2, kappa, thickness, true);
+        set_bparams1(c, dHat2, kappa, thickness);
+        gradient(c, xd, h, true, gd, rhs);
+        hessian(c, xd, h, true);
         double rtol = (opts && opts->pcg_rtol > 0) ? opts->pcg_rtol : 1e-8;
         int maxit = (opts && opts->pcg_maxit > 0) ? opts->pcg_maxit : 10000;
         double rel = 0;
@@ -1048,13 +1600,19 @@ int idp_get_bsr(idp_ctx* c, long* nnzb, int* rowptr, int* col, double* val, long
         std::vector<double> ev(9 * m.nnzb_e), cv(9 * L.ncb);
         m.e_rowptr.download(erp.data(), m.nV + 1, s);
         m.e_col.download(ecol.data(), m.nnzb_e, s);
-        L.evals.download(ev.data(), 9 * m.nnzb_e, s);
+        std::vector<double> evp(9 * m.nnzb_e), cvp(9 * L.ncb);
+        L.evals.download(evp.data(), 9 * m.nnzb_e, s);
         if (L.ncb) {
             L.crp.download(crp.data(), m.nV + 1, s);
             L.ccol.download(ccol.data(), L.ncb, s);
-            L.cvals.download(cv.data(), 9 * L.ncb, s);
+            L.cvals.download(cvp.data(), 9 * L.ncb, s);
         }
         IDP_HIP(hipStreamSynchronize(s));
+        // device values are 9 component planes; regroup per block
+        for (long k = 0; k < m.nnzb_e; ++k)
+            for (int e = 0; e < 9; ++e) ev[9 * k + e] = evp[e * m.nnzb_e + k];
+        for (long k = 0; k < L.ncb; ++k)
+            for (int e = 0; e < 9; ++e) cv[9 * k + e] = cvp[e * L.ncb + k];
         std::vector<int> orp(1, 0), oc;
         std::vector<double> ov;
         for (int r = 0; r < m.nV; ++r) {
@@ -1084,6 +1642,146 @@ int idp_get_bsr(idp_ctx* c, long* nnzb, int* rowptr, int* col, double* val, long
         std::memcpy(rowptr, orp.data(), orp.size() * sizeof(int));
         std::memcpy(col, oc.data(), oc.size() * sizeof(int));
         std::memcpy(val, ov.data(), ov.size() * sizeof(double));
+        return IDP_OK;
+    });
+}
+
+
+// ---------------------------------------------------------------------------
+// batched independent solves
+// ---------------------------------------------------------------------------
+int idp_set_mesh_batch(idp_ctx* c, int B, const int* nV, const int* nT, const double* X_rest, const int* tets,
+                       const double* IB, const double* vol, const double* lambda, const double* mu,
+                       const double* mass)
+{
+    return guarded(c, [&]() -> int {
+        if (B <= 0 || !nV || !nT || !X_rest || !tets) return IDP_E_INVALID;
+        std::vector<long> vbeg(B + 1, 0), tbeg(B + 1, 0);
+        for (int i = 0; i < B; ++i) {
+            if (nV[i] <= 0 || nT[i] <= 0) return IDP_E_INVALID;
+            vbeg[i + 1] = vbeg[i] + nV[i];
+            tbeg[i + 1] = tbeg[i] + nT[i];
+            // the merge keys of PP/PE duplicates hold 3 local vertex ids in 56 bits
+            if (B > 1 && (double)nV[i] * nV[i] * (nV[i] + 1.0) >= 7.2e16) return IDP_E_INVALID;
+        }
+        if (vbeg[B] > INT32_MAX / 4 || tbeg[B] > INT32_MAX / 16) return IDP_E_INVALID;
+        std::vector<int> gt(4 * (size_t)tbeg[B]);
+        for (int i = 0; i < B; ++i)
+            for (long t = tbeg[i]; t < tbeg[i + 1]; ++t)
+                for (int a = 0; a < 4; ++a) {
+                    const int v = tets[4 * t + a];
+                    if (v < 0 || v >= nV[i]) return IDP_E_INVALID;
+                    gt[4 * t + a] = (int)(v + vbeg[i]);
+                }
+        return setup_mesh(c, (int)vbeg[B], X_rest, (int)tbeg[B], gt.data(), IB, vol, lambda, mu, mass, vbeg, tbeg);
+    });
+}
+
+int idp_batch_size(idp_ctx* c, int* B)
+{
+    return guarded(c, [&]() -> int {
+        if (!B) return IDP_E_INVALID;
+        *B = c->have_mesh ? c->m.inst.B : 0;
+        return IDP_OK;
+    });
+}
+
+int idp_advance_batch(idp_ctx* c, double h, const double* newton_tol, int with_collision, const double* dHat2,
+                      double* kappa3, double thickness, const idp_solver_opts* opts, int* newton_iters,
+                      idp_step_stats* stats)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh || !kappa3 || !newton_tol || !dHat2 || !newton_iters || !(h > 0)) return IDP_E_INVALID;
+        IDP_HIP(hipSetDevice(c->device));
+        const int B = c->m.inst.B;
+        BatchEngine& E = *c->eng;
+        int rc = E.configure(c, h, with_collision != 0, thickness, opts);
+        if (rc) return rc;
+        std::vector<int> all(B);
+        for (int i = 0; i < B; ++i) all[i] = i;
+        rc = E.start(c, all, newton_tol, dHat2, kappa3);
+        if (rc) return rc;
+        while (E.n_active > 0) {
+            rc = E.iterate(c);
+            if (rc) return rc;
+        }
+        for (int i = 0; i < B; ++i) {
+            newton_iters[i] = E.PN[i];
+            kappa3[3 * i] = E.kappa[i];
+            if (stats) stats[i] = E.S[i];
+        }
+        return IDP_OK;
+    });
+}
+
+int idp_serve_configure(idp_ctx* c, double h, int with_collision, double thickness, const idp_solver_opts* opts)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh || !(h > 0)) return IDP_E_INVALID;
+        return c->eng->configure(c, h, with_collision != 0, thickness, opts);
+    });
+}
+
+int idp_serve_start(idp_ctx* c, int n, const int* slots, const double* X, const double* v, const double* newton_tol,
+                    const double* dHat2, const double* kappa3)
+{
+    return guarded(c, [&]() -> int {
+        BatchEngine& E = *c->eng;
+        if (!c->have_mesh || !E.configured || n <= 0 || !slots || !X || !v) return IDP_E_INVALID;
+        const InstDev& I = c->m.inst;
+        std::vector<int> sl(slots, slots + n);
+        std::vector<double> tol(I.B), d2(I.B), k3(3 * I.B);
+        long off = 0;
+        for (int k = 0; k < n; ++k) {
+            const int i = sl[k];
+            if (i < 0 || i >= I.B || E.active[i]) return IDP_E_INVALID;
+            const long nv = I.vbeg[i + 1] - I.vbeg[i];
+            IDP_HIP(hipMemcpyAsync(c->x.p + 3 * I.vbeg[i], X + 3 * off, 3 * nv * 8, hipMemcpyHostToDevice, c->s));
+            IDP_HIP(hipMemcpyAsync(c->v.p + 3 * I.vbeg[i], v + 3 * off, 3 * nv * 8, hipMemcpyHostToDevice, c->s));
+            off += nv;
+            tol[i] = newton_tol[k];
+            d2[i] = dHat2[k];
+            for (int q = 0; q < 3; ++q) k3[3 * i + q] = kappa3[3 * k + q];
+        }
+        IDP_HIP(hipStreamSynchronize(c->s));
+        return E.start(c, sl, tol.data(), d2.data(), k3.data(), true);
+    });
+}
+
+int idp_serve_iterate(idp_ctx* c, int* n_active, int* finished)
+{
+    return guarded(c, [&]() -> int {
+        BatchEngine& E = *c->eng;
+        if (!E.configured) return IDP_E_INVALID;
+        const int B = c->m.inst.B;
+        if (finished) std::memset(finished, 0, B * sizeof(int));
+        if (E.n_active > 0) {
+            const int rc = E.iterate(c);
+            if (rc) return rc;
+            if (finished)
+                for (int i = 0; i < B; ++i) finished[i] = E.finished_now[i];
+        }
+        if (n_active) *n_active = E.n_active;
+        return IDP_OK;
+    });
+}
+
+int idp_serve_result(idp_ctx* c, int slot, double* X, double* v, double* kappa3, idp_step_stats* stats)
+{
+    return guarded(c, [&]() -> int {
+        const InstDev& I = c->m.inst;
+        BatchEngine& E = *c->eng;
+        if (!c->have_mesh || slot < 0 || slot >= I.B) return IDP_E_INVALID;
+        const long nv = I.vbeg[slot + 1] - I.vbeg[slot];
+        if (X) IDP_HIP(hipMemcpyAsync(X, c->x.p + 3 * I.vbeg[slot], 3 * nv * 8, hipMemcpyDeviceToHost, c->s));
+        if (v) IDP_HIP(hipMemcpyAsync(v, c->v.p + 3 * I.vbeg[slot], 3 * nv * 8, hipMemcpyDeviceToHost, c->s));
+        IDP_HIP(hipStreamSynchronize(c->s));
+        if (kappa3 && (int)E.kappa.size() == I.B) {
+            kappa3[0] = E.kappa[slot];
+            kappa3[1] = E.kmax[slot];
+            kappa3[2] = 0;
+        }
+        if (stats && (int)E.S.size() == I.B) *stats = E.S[slot];
         return IDP_OK;
     });
 }

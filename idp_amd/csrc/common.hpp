@@ -3,7 +3,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <cstring>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -73,6 +75,11 @@ struct DBuf {
         ensure(n, s);
         if (n) IDP_HIP(hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
     }
+    T* upload_get(const T* h, size_t n, hipStream_t s)
+    {
+        upload(h, n, s);
+        return p;
+    }
     void download(T* h, size_t n, hipStream_t s) const
     {
         if (n) IDP_HIP(hipMemcpyAsync(h, p, n * sizeof(T), hipMemcpyDeviceToHost, s));
@@ -97,6 +104,7 @@ struct Profiler {
     double ms[IDP_K_COUNT] = {0};
     long launches[IDP_K_COUNT] = {0};
     double work[IDP_K_COUNT] = {0};
+    double ccd_steps = 0, ccd_max_steps = 0;  // ACCD iterations: total, longest pair
     ~Profiler()
     {
         for (auto e : pool) (void)hipEventDestroy(e);
@@ -114,6 +122,7 @@ struct Profiler {
     {
         collect();
         for (int i = 0; i < IDP_K_COUNT; ++i) ms[i] = work[i] = 0, launches[i] = 0;
+        ccd_steps = ccd_max_steps = 0;
     }
     // returns a token for end()
     int begin(int fam)

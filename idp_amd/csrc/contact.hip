@@ -27,17 +27,26 @@
 namespace idp {
 
 // ---------------------------------------------------------------------------
-// grid set-up (SPATIAL_HASH.h:60-84 and 455-511), one thread
+// grid set-up (SPATIAL_HASH.h:60-84 and 455-511), one thread per instance:
+// voxel = mean boundary edge length of the instance (its chunk partials added
+// in order), origin and extent from the instance's bounding box
 // ---------------------------------------------------------------------------
-__global__ void k_grid_setup(GridDev* g, const double* __restrict__ mean_parts, int n_mean_parts, long n_mean,
-                             const double* __restrict__ mm /* minx,miny,minz,maxx,maxy,maxz */, double pad)
+__global__ void k_grid_setup(int B, GridDev* __restrict__ grids, const double* __restrict__ mean_parts,
+                             const int* __restrict__ mean_pbeg, const int* __restrict__ ebeg,
+                             const double* __restrict__ mm /* 6 per instance: min xyz, max xyz */, double pad,
+                             const unsigned char* __restrict__ active)
 {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B || (active && !active[i])) return;
+    GridDev* g = grids + i;
     double tot = 0.0;
-    for (int c = 0; c < n_mean_parts; ++c) tot += mean_parts[c];
+    for (int c = mean_pbeg[i]; c < mean_pbeg[i + 1]; ++c) tot += mean_parts[c];
+    const long n_mean = ebeg[i + 1] - ebeg[i];
     double vs = n_mean ? tot / (double)n_mean : 1.0;
     g->voxel = vs;
-    double lo[3] = {mm[0] - pad, mm[1] - pad, mm[2] - pad};
-    double hi[3] = {mm[3] + pad, mm[4] + pad, mm[5] + pad};
+    const double* m6 = mm + 6 * i;
+    double lo[3] = {m6[0] - pad, m6[1] - pad, m6[2] - pad};
+    double hi[3] = {m6[3] + pad, m6[4] + pad, m6[5] + pad};
     double range[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
     double inv = 1.0 / vs;
     long vcl[3];
@@ -89,49 +98,56 @@ __device__ inline double chunk_sum_lds(double v, double* sh, int nvalid)
     return s;
 }
 
-// chunked partial sums of boundary edge lengths (one block of 256 per chunk)
-__global__ void __launch_bounds__(256) k_edge_len_parts(int ne, const int2* __restrict__ E,
-                                                        const double* __restrict__ X, double* __restrict__ parts)
+// chunked partial sums of boundary edge lengths (one block per chunk of 256
+// edges of one instance)
+__global__ void __launch_bounds__(256) k_edge_len_parts(const int* __restrict__ lo, const int* __restrict__ hi,
+                                                        const int2* __restrict__ E, const double* __restrict__ X,
+                                                        double* __restrict__ parts)
 {
     __shared__ double sh[256];
-    const long b = (long)blockIdx.x * 256, i = b + threadIdx.x;
+    const int b = lo[blockIdx.x], e = hi[blockIdx.x], i = b + threadIdx.x;
     double len = 0.0;
-    if (i < ne) {
+    if (i < e) {
         int2 ed = E[i];
         len = sqrt(sqn(ld3(X, ed.x) - ld3(X, ed.y)));
     }
-    const double s = chunk_sum_lds(len, sh, (int)min(256L, ne - b));
+    const double s = chunk_sum_lds(len, sh, e - b);
     if (threadIdx.x == 0) parts[blockIdx.x] = s;
 }
 
-// chunked partial sums of |p| over boundary nodes (x, y, z in order)
-__global__ void __launch_bounds__(256) k_psize_parts(int nb, const int* __restrict__ bn,
-                                                     const double* __restrict__ p, double* __restrict__ parts)
+// chunked partial sums of |p| over boundary nodes (x, y, z in order); the
+// chunks index the flattened (node, component) sequence of each instance
+__global__ void __launch_bounds__(256) k_psize_parts(const int* __restrict__ lo, const int* __restrict__ hi,
+                                                     const int* __restrict__ bn, const double* __restrict__ p,
+                                                     double* __restrict__ parts)
 {
     __shared__ double sh[256];
-    const long n = 3L * nb, b = (long)blockIdx.x * 256, i = b + threadIdx.x;
-    const double a = i < n ? fabs(p[3 * bn[i / 3] + (i % 3)]) : 0.0;
-    const double s = chunk_sum_lds(a, sh, (int)min(256L, n - b));
+    const int b = lo[blockIdx.x], e = hi[blockIdx.x], i = b + threadIdx.x;
+    const double a = i < e ? fabs(p[3 * bn[i / 3] + (i % 3)]) : 0.0;
+    const double s = chunk_sum_lds(a, sh, e - b);
     if (threadIdx.x == 0) parts[blockIdx.x] = s;
 }
 
-// min / max of positions (and of x + alpha p when p is given): per-block
-// partials, exact (order independent)
-__global__ void __launch_bounds__(RB) k_minmax_parts(long n, const int* __restrict__ idx,
+// min / max of positions (and of x + alpha_i p when p is given) over the
+// elements [lo, hi) of each block (vertex ids, or indices into idx):
+// per-block partials, exact (order independent)
+__global__ void __launch_bounds__(RB) k_minmax_parts(const int* __restrict__ lo, const int* __restrict__ hi,
+                                                     const int* __restrict__ binst, const int* __restrict__ idx,
                                                      const double* __restrict__ X, const double* __restrict__ p,
                                                      const double* __restrict__ alpha_p, double* __restrict__ parts)
 {
     __shared__ double sh[6][RB];
-    long i = (long)blockIdx.x * RB + threadIdx.x;
+    const int e = lo[blockIdx.x] + threadIdx.x;
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    if (i < n) {
-        long v = idx ? idx[i] : i;
+    if (e < hi[blockIdx.x]) {
+        long v = idx ? idx[e] : e;
+        const double alpha = p ? alpha_p[binst[blockIdx.x]] : 0.0;
         for (int d = 0; d < 3; ++d) {
             double a = X[3 * v + d];
             mn[d] = fmin(mn[d], a);
             mx[d] = fmax(mx[d], a);
             if (p) {
-                double b = X[3 * v + d] + *alpha_p * p[3 * v + d];
+                double b = X[3 * v + d] + alpha * p[3 * v + d];
                 mn[d] = fmin(mn[d], b);
                 mx[d] = fmax(mx[d], b);
             }
@@ -154,37 +170,42 @@ __global__ void __launch_bounds__(RB) k_minmax_parts(long n, const int* __restri
         for (int d = 0; d < 6; ++d) parts[6 * blockIdx.x + d] = sh[d][0];
 }
 
-__global__ void __launch_bounds__(RB) k_minmax_final(int np, const double* __restrict__ parts,
-                                                     double* __restrict__ out)
+// per instance: min / max over its block partials
+__global__ void k_minmax_final(int B, const int* __restrict__ pbeg, const double* __restrict__ parts,
+                               double* __restrict__ out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B) return;
     double r[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-    for (int b = 0; b < np; ++b)
+    for (int b = pbeg[i]; b < pbeg[i + 1]; ++b)
         for (int d = 0; d < 3; ++d) {
             r[d] = fmin(r[d], parts[6 * b + d]);
             r[3 + d] = fmax(r[3 + d], parts[6 * b + 3 + d]);
         }
-    for (int d = 0; d < 6; ++d) out[d] = r[d];
+    for (int d = 0; d < 6; ++d) out[6 * i + d] = r[d];
 }
 
 // ---------------------------------------------------------------------------
 // boxes and cell records
 // ---------------------------------------------------------------------------
-// vertex boxes: static -> point voxel; swept -> voxel box of
-// [min(x, x + a p), max(..)] -/+ th/2 with a read from device memory
-__global__ void k_vertex_boxes(int nb, const int* __restrict__ bn, const double* __restrict__ X,
-                               const double* __restrict__ p, const double* __restrict__ alpha_p, double halfth,
-                               const GridDev* __restrict__ gp, int* __restrict__ vbox /* 6 per boundary node */)
+// vertex boxes in the grid of the vertex's instance: static -> point voxel;
+// swept -> voxel box of [min(x, x + a p), max(..)] -/+ th/2, a = alpha of the
+// instance (device memory)
+__global__ void k_vertex_boxes(int nb, const int* __restrict__ bn, const int* __restrict__ vinst,
+                               const double* __restrict__ X, const double* __restrict__ p,
+                               const double* __restrict__ alpha_p, double halfth, const GridDev* __restrict__ grids,
+                               int* __restrict__ vbox /* 6 per boundary node */)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nb) return;
-    const GridDev g = *gp;
     int v = bn[i];
+    const int inst = vinst[v];
+    const GridDev g = grids[inst];
     double a[3], b[3];
     for (int d = 0; d < 3; ++d) {
         double x0 = X[3 * v + d];
         if (p) {
-            double x1 = x0 + *alpha_p * p[3 * v + d];
+            double x1 = x0 + alpha_p[inst] * p[3 * v + d];
             a[d] = fmin(x0, x1) - halfth;
             b[d] = fmax(x0, x1) + halfth;
         } else {
@@ -201,15 +222,18 @@ __global__ void k_vertex_boxes(int nb, const int* __restrict__ bn, const double*
     }
 }
 
-// primitive boxes: union of vertex boxes; counts voxels (clamped to the grid)
+// primitive boxes: union of vertex boxes; counts voxels (clamped to the grid
+// of the primitive's instance); primitives of inactive instances get none
 template <int K>
 __global__ void k_prim_boxes(int n, const int* __restrict__ prim, const int* __restrict__ v2b,
-                             const int* __restrict__ vbox, const GridDev* __restrict__ gp, int* __restrict__ pbox,
-                             unsigned* __restrict__ cnt)
+                             const int* __restrict__ vinst, const unsigned char* __restrict__ active,
+                             const int* __restrict__ vbox, const GridDev* __restrict__ grids,
+                             int* __restrict__ pbox, unsigned* __restrict__ cnt, unsigned* __restrict__ pinst)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const GridDev g = *gp;
+    const int inst = vinst[prim[K * i]];
+    pinst[i] = (unsigned)inst;
     int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
     for (int k = 0; k < K; ++k) {
         int b = v2b[prim[K * i + k]];
@@ -219,6 +243,15 @@ __global__ void k_prim_boxes(int n, const int* __restrict__ prim, const int* __r
             hi[d] = max(hi[d], vbox[6 * b + 3 + d]);
         }
     }
+    if (active && !active[inst]) {
+        for (int d = 0; d < 3; ++d) {
+            pbox[6 * i + d] = lo[d];
+            pbox[6 * i + 3 + d] = hi[d];
+        }
+        cnt[i] = 0;
+        return;
+    }
+    const GridDev g = grids[inst];
     unsigned c = 1;
     for (int d = 0; d < 3; ++d) {
         pbox[6 * i + d] = lo[d];
@@ -229,14 +262,19 @@ __global__ void k_prim_boxes(int n, const int* __restrict__ prim, const int* __r
     cnt[i] = c;
 }
 
-// (2 * voxel + type, primitive) records; primitives [0, nsplit) are type 0
-// (triangles), [nsplit, n) type 1 (edges, numbered from 0)
+// (keybase[inst] + 2 * voxel + type, primitive) records; primitives [0, nsplit)
+// are type 0 (triangles), [nsplit, n) type 1 (edges, numbered from 0)
 __global__ void k_emit_cells(int n, int nsplit, const int* __restrict__ pbox, const unsigned* __restrict__ off,
-                             const GridDev* __restrict__ gp, unsigned* __restrict__ keys, int* __restrict__ vals)
+                             const unsigned* __restrict__ pinst, const unsigned* __restrict__ keybase,
+                             const GridDev* __restrict__ grids, unsigned* __restrict__ keys, int* __restrict__ vals)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const GridDev g = *gp;
+    const unsigned o0 = off[i], o1 = off[i + 1];
+    if (o1 == o0) return;
+    const unsigned inst = pinst[i];
+    const GridDev g = grids[inst];
+    const unsigned base = keybase[inst];
     const unsigned type = i >= nsplit ? 1u : 0u;
     const int id = i - (type ? nsplit : 0);
     int l[3], h[3];
@@ -244,11 +282,11 @@ __global__ void k_emit_cells(int n, int nsplit, const int* __restrict__ pbox, co
         l[d] = max(pbox[6 * i + d], 0);
         h[d] = min(pbox[6 * i + 3 + d], g.cnt[d] - 1);
     }
-    unsigned o = off[i];
+    unsigned o = o0;
     for (int z = l[2]; z <= h[2]; ++z)
         for (int y = l[1]; y <= h[1]; ++y)
             for (int x = l[0]; x <= h[0]; ++x) {
-                keys[o] = 2u * key_of(g, x, y, z) + type;
+                keys[o] = base + 2u * key_of(g, x, y, z) + type;
                 vals[o] = id;
                 ++o;
             }
@@ -308,8 +346,9 @@ __global__ void k_fill_u32(size_t n, unsigned* __restrict__ a, unsigned v)
 // type whose box overlaps the query box is visited exactly once, in the
 // lowest shared voxel (componentwise max of the two box minima, clamped).
 template <int G, class F>
-__device__ inline void walk_lanes(const GridDev& g, const CellTable& T, const int* __restrict__ pbox_t, int type,
-                                  const int* ql, const int* qh, int lane, F&& visit)
+__device__ inline void walk_lanes(const GridDev& g, unsigned keybase, const CellTable& T,
+                                  const int* __restrict__ pbox_t, int type, const int* ql, const int* qh, int lane,
+                                  F&& visit)
 {
     int l[3], n[3];
     for (int d = 0; d < 3; ++d) {
@@ -324,7 +363,7 @@ __device__ inline void walk_lanes(const GridDev& g, const CellTable& T, const in
         const int y = l[1] + (int)(r % n[1]);
         const int z = l[2] + (int)(r / n[1]);
         unsigned s, e;
-        cell_range(T, 2u * key_of(g, x, y, z) + (unsigned)type, s, e);
+        cell_range(T, keybase + 2u * key_of(g, x, y, z) + (unsigned)type, s, e);
         for (unsigned k = s; k < e; ++k) {
             const int f = T.vals[k];
             const int* pb = pbox_t + 6L * f;
@@ -346,22 +385,43 @@ struct CSArgs {
     const int2* be;
     const int3* bt;
     const int* pbox;  // triangles [0, nf), then edges
-    const GridDev* g;
-    int nf;
-    double r, r2;
+    const GridDev* grids;
+    const unsigned* keybase;
+    const int* vinst;
+    const unsigned char* active;  // null: every instance
+    const double* rad;            // per instance: r = sqrt(dHat2) + thickness, then r^2 (2 B values)
+    int B, nf;
 };
 
 // Outputs are appended with atomic counters and put in a fixed order
-// afterwards by sorting on their keys: pass tuples on (vertex, triangle) for
-// PT and (edge, edge) for EE, PP/PE duplicates on the merge key.
+// afterwards by sorting on their keys, built from instance-local indices:
+//   pass:  instance << 40 | part << 39 | local, part 0 = PT (vertex, triangle),
+//          part 1 = EE (edge, edge)
+//   dup:   [instance << 56 |] (s0, s1, s2) merge key in std::map order
+// so every instance's constraints come out in the order of a solo solve.
 struct CSOut {
     unsigned* cnt;  // [0] pass, [1] dup
     unsigned long long* pkey;
     int4* pval;
     unsigned long long* dkey;
     unsigned cap_p, cap_d;
-    unsigned long long nf, ne, V;
+    const int *vbeg, *ebeg, *fbeg;  // instance ranges (B + 1)
+    int inst_shift;                 // 56 with several instances, 0 with one
 };
+
+__device__ inline unsigned long long pt_pass_key(const CSOut& O, int inst, int vI, int f)
+{
+    const unsigned long long nf = O.fbeg[inst + 1] - O.fbeg[inst];
+    return ((unsigned long long)inst << 40) |
+           ((unsigned long long)(vI - O.vbeg[inst]) * nf + (unsigned long long)(f - O.fbeg[inst]));
+}
+
+__device__ inline unsigned long long ee_pass_key(const CSOut& O, int inst, int eI, int eJ)
+{
+    const unsigned long long ne = O.ebeg[inst + 1] - O.ebeg[inst];
+    return ((unsigned long long)inst << 40) | (1ull << 39) |
+           ((unsigned long long)(eI - O.ebeg[inst]) * ne + (unsigned long long)(eJ - O.ebeg[inst]));
+}
 
 __device__ inline void out_pass(const CSOut& O, unsigned long long key, int a, int b, int c, int d)
 {
@@ -373,110 +433,117 @@ __device__ inline void out_pass(const CSOut& O, unsigned long long key, int a, i
 }
 
 // PP / PE duplicate (-p-1, q, e1 or -1, -1): key of the (s0, s1, s2) map entry
-// in std::map order
-__device__ inline void out_dup(const CSOut& O, int a, int b, int c)
+// in std::map order, over instance-local vertex ids
+__device__ inline void out_dup(const CSOut& O, int inst, int a, int b, int c)
 {
     const unsigned i = atomicAdd(O.cnt + 1, 1u);
     if (i < O.cap_d) {
-        const unsigned long long s0 = (unsigned long long)(a + (long long)O.V);  // a in [-V,-1]
-        O.dkey[i] = (s0 * O.V + (unsigned long long)b) * (O.V + 1) + (unsigned long long)(c + 1);
+        const long long vb = O.vbeg[inst];
+        const unsigned long long V = (unsigned long long)(O.vbeg[inst + 1] - vb);
+        const unsigned long long s0 = (unsigned long long)((a + vb) + (long long)V);  // local -p-1 in [-V,-1]
+        const unsigned long long s1 = (unsigned long long)(b - vb);
+        const unsigned long long s2 = c < 0 ? 0ull : (unsigned long long)(c - vb + 1);
+        const unsigned long long key = (s0 * V + s1) * (V + 1) + s2;
+        O.dkey[i] = O.inst_shift ? (((unsigned long long)inst << O.inst_shift) | key) : key;
     }
 }
 
 // Classifies one point-triangle pair (IPC.h:158-268).  Returns the number of
 // tuples written (0/1) into pass (PT) or dup (PP/PE) streams.
-__device__ inline void pt_narrow(const CSArgs& A, int vI, int f, const CSOut& O)
+__device__ inline void pt_narrow(const CSArgs& A, int inst, int vI, int f, const CSOut& O)
 {
+    const double r = A.rad[inst], r2 = A.rad[A.B + inst];
     int3 t = A.bt[f];
     if (vI == t.x || vI == t.y || vI == t.z) return;
     if (A.dbc[vI] && A.dbc[t.x] && A.dbc[t.y] && A.dbc[t.z]) return;
     d3 p = ld3(A.X, vI), t0 = ld3(A.X, t.x), t1 = ld3(A.X, t.y), t2 = ld3(A.X, t.z);
-    if (!pt_cd_broadphase(p, t0, t1, t2, A.r)) return;
+    if (!pt_cd_broadphase(p, t0, t1, t2, r)) return;
     double d;
     switch (pt_type(p, t0, t1, t2)) {
-    case 0: d = d2_pp(p, t0); if (d < A.r2) out_dup(O, -vI - 1, t.x, -1); break;
-    case 1: d = d2_pp(p, t1); if (d < A.r2) out_dup(O, -vI - 1, t.y, -1); break;
-    case 2: d = d2_pp(p, t2); if (d < A.r2) out_dup(O, -vI - 1, t.z, -1); break;
-    case 3: d = d2_pe(p, t0, t1); if (d < A.r2) out_dup(O, -vI - 1, t.x, t.y); break;
-    case 4: d = d2_pe(p, t1, t2); if (d < A.r2) out_dup(O, -vI - 1, t.y, t.z); break;
-    case 5: d = d2_pe(p, t2, t0); if (d < A.r2) out_dup(O, -vI - 1, t.z, t.x); break;
-    default: d = d2_pt(p, t0, t1, t2); if (d < A.r2) out_pass(O, (unsigned long long)vI * O.nf + f, -vI - 1, t.x, t.y, t.z); break;
+    case 0: d = d2_pp(p, t0); if (d < r2) out_dup(O, inst, -vI - 1, t.x, -1); break;
+    case 1: d = d2_pp(p, t1); if (d < r2) out_dup(O, inst, -vI - 1, t.y, -1); break;
+    case 2: d = d2_pp(p, t2); if (d < r2) out_dup(O, inst, -vI - 1, t.z, -1); break;
+    case 3: d = d2_pe(p, t0, t1); if (d < r2) out_dup(O, inst, -vI - 1, t.x, t.y); break;
+    case 4: d = d2_pe(p, t1, t2); if (d < r2) out_dup(O, inst, -vI - 1, t.y, t.z); break;
+    case 5: d = d2_pe(p, t2, t0); if (d < r2) out_dup(O, inst, -vI - 1, t.z, t.x); break;
+    default: d = d2_pt(p, t0, t1, t2); if (d < r2) out_pass(O, pt_pass_key(O, inst, vI, f), -vI - 1, t.x, t.y, t.z); break;
     }
 }
 
 // Classifies one edge-edge pair (IPC.h:371-568)
-__device__ inline void ee_narrow(const CSArgs& A, int eI, int eJ, const CSOut& O)
+__device__ inline void ee_narrow(const CSArgs& A, int inst, int eI, int eJ, const CSOut& O)
 {
+    const double r = A.rad[inst], r2 = A.rad[A.B + inst];
     int2 a = A.be[eI], b = A.be[eJ];
     if (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y || eI > eJ) return;
     if (A.dbc[a.x] && A.dbc[a.y] && A.dbc[b.x] && A.dbc[b.y]) return;
     d3 a0 = ld3(A.X, a.x), a1 = ld3(A.X, a.y), b0 = ld3(A.X, b.x), b1 = ld3(A.X, b.y);
-    if (!ee_cd_broadphase(a0, a1, b0, b1, A.r)) return;
+    if (!ee_cd_broadphase(a0, a1, b0, b1, r)) return;
     const double cr = ee_cross_sqn(a0, a1, b0, b1);
     const double eps = mollifier_eps(ld3(A.X0, a.x), ld3(A.X0, a.y), ld3(A.X0, b.x), ld3(A.X0, b.y));
     const bool m = cr < eps;
-    const unsigned long long ekey = O.V * O.nf + (unsigned long long)eI * O.ne + eJ;
+    const unsigned long long ekey = ee_pass_key(O, inst, eI, eJ);
     double d;
     switch (ee_type(a0, a1, b0, b1)) {
     case 0:
         d = d2_pp(a0, b0);
-        if (d < A.r2) {
+        if (d < r2) {
             if (m) out_pass(O, ekey, a.x, b.x, -a.y - 1, -b.y - 1);
-            else out_dup(O, -a.x - 1, b.x, -1);
+            else out_dup(O, inst, -a.x - 1, b.x, -1);
         }
         break;
     case 1:
         d = d2_pp(a0, b1);
-        if (d < A.r2) {
+        if (d < r2) {
             if (m) out_pass(O, ekey, a.x, b.y, -a.y - 1, -b.x - 1);
-            else out_dup(O, -a.x - 1, b.y, -1);
+            else out_dup(O, inst, -a.x - 1, b.y, -1);
         }
         break;
     case 2:
         d = d2_pe(a0, b0, b1);
-        if (d < A.r2) {
+        if (d < r2) {
             if (m) out_pass(O, ekey, a.x, b.x, b.y, -a.y - 1);
-            else out_dup(O, -a.x - 1, b.x, b.y);
+            else out_dup(O, inst, -a.x - 1, b.x, b.y);
         }
         break;
     case 3:
         d = d2_pp(a1, b0);
-        if (d < A.r2) {
+        if (d < r2) {
             if (m) out_pass(O, ekey, a.y, b.x, -a.x - 1, -b.y - 1);
-            else out_dup(O, -a.y - 1, b.x, -1);
+            else out_dup(O, inst, -a.y - 1, b.x, -1);
         }
         break;
     case 4:
         d = d2_pp(a1, b1);
-        if (d < A.r2) {
+        if (d < r2) {
             if (m) out_pass(O, ekey, a.y, b.y, -a.x - 1, -b.x - 1);
-            else out_dup(O, -a.y - 1, b.y, -1);
+            else out_dup(O, inst, -a.y - 1, b.y, -1);
         }
         break;
     case 5:
         d = d2_pe(a1, b0, b1);
-        if (d < A.r2) {
+        if (d < r2) {
             if (m) out_pass(O, ekey, a.y, b.x, b.y, -a.x - 1);
-            else out_dup(O, -a.y - 1, b.x, b.y);
+            else out_dup(O, inst, -a.y - 1, b.x, b.y);
         }
         break;
     case 6:
         d = d2_pe(b0, a0, a1);
-        if (d < A.r2) {
+        if (d < r2) {
             if (m) out_pass(O, ekey, b.x, a.x, a.y, -b.y - 1);
-            else out_dup(O, -b.x - 1, a.x, a.y);
+            else out_dup(O, inst, -b.x - 1, a.x, a.y);
         }
         break;
     case 7:
         d = d2_pe(b1, a0, a1);
-        if (d < A.r2) {
+        if (d < r2) {
             if (m) out_pass(O, ekey, b.y, a.x, a.y, -b.x - 1);
-            else out_dup(O, -b.y - 1, a.x, a.y);
+            else out_dup(O, inst, -b.y - 1, a.x, a.y);
         }
         break;
     default:
         d = d2_ee(a0, a1, b0, b1);
-        if (d < A.r2) {
+        if (d < r2) {
             if (m) out_pass(O, ekey, a.x, a.y, -b.x - 1, b.y);
             else out_pass(O, ekey, a.x, a.y, b.x, b.y);
         }
@@ -495,22 +562,30 @@ __global__ void __launch_bounds__(256) k_cs_walk(int nb, int ne, CSArgs A, CellT
     const long q = tid / G;
     const int lane = (int)(tid % G);
     if (q >= (long)nb + ne) return;
-    const GridDev& g = *A.g;
     int ql[3], qh[3];
     if (q < nb) {
         const int vI = A.bn[q];
+        const int inst = A.vinst[vI];
+        if (A.active && !A.active[inst]) return;
+        const GridDev& g = A.grids[inst];
+        const double r = A.rad[inst];
         const d3 p = ld3(A.X, vI);
-        locate(g, p.x - A.r, p.y - A.r, p.z - A.r, ql);
-        locate(g, p.x + A.r, p.y + A.r, p.z + A.r, qh);
-        walk_lanes<G>(g, T, A.pbox, 0, ql, qh, lane, [&](int f) { pt_narrow(A, vI, f, O); });
+        locate(g, p.x - r, p.y - r, p.z - r, ql);
+        locate(g, p.x + r, p.y + r, p.z + r, qh);
+        walk_lanes<G>(g, A.keybase[inst], T, A.pbox, 0, ql, qh, lane,
+                      [&](int f) { pt_narrow(A, inst, vI, f, O); });
     } else {
         const int eI = (int)(q - nb);
         const int2 e = A.be[eI];
+        const int inst = A.vinst[e.x];
+        if (A.active && !A.active[inst]) return;
+        const GridDev& g = A.grids[inst];
+        const double r = A.rad[inst];
         const d3 a = ld3(A.X, e.x), b = ld3(A.X, e.y);
-        locate(g, fmin(a.x, b.x) - A.r, fmin(a.y, b.y) - A.r, fmin(a.z, b.z) - A.r, ql);
-        locate(g, fmax(a.x, b.x) + A.r, fmax(a.y, b.y) + A.r, fmax(a.z, b.z) + A.r, qh);
-        walk_lanes<G>(g, T, A.pbox + 6L * A.nf, 1, ql, qh, lane, [&](int f) {
-            if (f > eI) ee_narrow(A, eI, f, O);
+        locate(g, fmin(a.x, b.x) - r, fmin(a.y, b.y) - r, fmin(a.z, b.z) - r, ql);
+        locate(g, fmax(a.x, b.x) + r, fmax(a.y, b.y) + r, fmax(a.z, b.z) + r, qh);
+        walk_lanes<G>(g, A.keybase[inst], T, A.pbox + 6L * A.nf, 1, ql, qh, lane, [&](int f) {
+            if (f > eI) ee_narrow(A, inst, eI, f, O);
         });
     }
 }
@@ -526,7 +601,8 @@ __global__ void k_dup_heads(unsigned n, const unsigned long long* __restrict__ k
 }
 
 __global__ void k_dup_write(unsigned n, const unsigned long long* __restrict__ keys,
-                            const unsigned* __restrict__ headscan, unsigned long long V, int4* __restrict__ out)
+                            const unsigned* __restrict__ headscan, const int* __restrict__ vbeg, int inst_shift,
+                            int4* __restrict__ out)
 {
     unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -534,10 +610,18 @@ __global__ void k_dup_write(unsigned n, const unsigned long long* __restrict__ k
     unsigned j = i + 1;
     while (j < n && keys[j] == keys[i]) ++j;
     unsigned long long k = keys[i];
-    int s2 = (int)(k % (V + 1)) - 1;
+    int inst = 0;
+    if (inst_shift) {
+        inst = (int)(k >> inst_shift);
+        k &= (1ull << inst_shift) - 1;
+    }
+    const long long vb = vbeg[inst];
+    const unsigned long long V = (unsigned long long)(vbeg[inst + 1] - vb);
+    const long long s2l = (long long)(k % (V + 1)) - 1;
     k /= (V + 1);
-    int s1 = (int)(k % V);
-    int s0 = (int)((long long)(k / V) - (long long)V);
+    const long long s1l = (long long)(k % V);
+    const long long s0l = (long long)(k / V) - (long long)V;  // local -p-1
+    const int s0 = (int)(s0l - vb), s1 = (int)(s1l + vb), s2 = s2l < 0 ? -1 : (int)(s2l + vb);
     out[headscan[i]] = make_int4(s0, s1, s2, -(int)(j - i));
 }
 
@@ -553,7 +637,10 @@ struct CCDArgs {
     const int3* bt;
     const int* vbox;  // per boundary node
     const int* pbox;  // triangles [0, nf), then edges
-    const GridDev* g;
+    const GridDev* grids;
+    const unsigned* keybase;
+    const int* vinst;
+    const unsigned char* active;  // null: every instance
     int nf;
     double th;
 };
@@ -570,11 +657,13 @@ __global__ void __launch_bounds__(256) k_ccd_walk(int nb, int ne, CCDArgs A, Cel
     const long q = tid / G;
     const int lane = (int)(tid % G);
     if (q >= (long)nb + ne) return;
-    const GridDev& g = *A.g;
     if (q < nb) {
         const int v = A.bn[q];
+        const int inst = A.vinst[v];
+        if (A.active && !A.active[inst]) return;
+        const GridDev& g = A.grids[inst];
         const d3 p = ld3(A.X, v), dp = ld3(A.P, v);
-        walk_lanes<G>(g, T, A.pbox, 0, A.vbox + 6 * q, A.vbox + 6 * q + 3, lane, [&](int f) {
+        walk_lanes<G>(g, A.keybase[inst], T, A.pbox, 0, A.vbox + 6 * q, A.vbox + 6 * q + 3, lane, [&](int f) {
             int3 t = A.bt[f];
             if (v == t.x || v == t.y || v == t.z) return;
             if (A.dbc[v] && A.dbc[t.x] && A.dbc[t.y] && A.dbc[t.z]) return;
@@ -587,10 +676,13 @@ __global__ void __launch_bounds__(256) k_ccd_walk(int nb, int ne, CCDArgs A, Cel
     } else {
         const int eI = (int)(q - nb);
         const int2 a = A.be[eI];
+        const int inst = A.vinst[a.x];
+        if (A.active && !A.active[inst]) return;
+        const GridDev& g = A.grids[inst];
         const d3 a0 = ld3(A.X, a.x), a1 = ld3(A.X, a.y), da0 = ld3(A.P, a.x), da1 = ld3(A.P, a.y);
         const int* pe = A.pbox + 6L * A.nf;
         const int* eb = pe + 6L * eI;
-        walk_lanes<G>(g, T, pe, 1, eb, eb + 3, lane, [&](int f) {
+        walk_lanes<G>(g, A.keybase[inst], T, pe, 1, eb, eb + 3, lane, [&](int f) {
             if (f <= eI) return;
             int2 b = A.be[f];
             if (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y) return;
@@ -604,7 +696,7 @@ __global__ void __launch_bounds__(256) k_ccd_walk(int nb, int ne, CCDArgs A, Cel
     }
 }
 
-// ACCD over the candidates as a work queue.  Each lane carries one pair
+// ACCD over the candidates as a work queue (amin: running minimum per instance).  Each lane carries one pair
 // through the additive CCD iteration of CCD.h:280-395 (same arithmetic as
 // accd_pt / accd_ee, step by step); a lane whose pair ends (hit, passed the
 // bound, or zero motion) takes the next pair, one atomic per wavefront for
@@ -616,7 +708,7 @@ __global__ void __launch_bounds__(256) k_ccd_walk(int nb, int ne, CCDArgs A, Cel
 struct AccdLane {
     d3 x[4], dx[4];
     double maxDisp, d2, dc, f, gap, toc, tmax;
-    int guard;
+    int guard, inst;
     bool ee;
 };
 
@@ -637,12 +729,14 @@ __device__ inline void accd_fix_ee(AccdLane& L, double th)
 }
 
 // returns false when the pair has no motion (no hit)
-__device__ inline bool accd_init(AccdLane& L, int2 c, const CCDArgs& A, double eta, double tmax)
+__device__ inline bool accd_init(AccdLane& L, int2 c, const CCDArgs& A, double eta,
+                                 const unsigned long long* amin)
 {
     const double th = A.th;
     if (c.x >= 0) {
         const int3 t = A.bt[c.y];
         L.ee = false;
+        L.inst = A.vinst[c.x];
         L.x[0] = ld3(A.X, c.x); L.x[1] = ld3(A.X, t.x); L.x[2] = ld3(A.X, t.y); L.x[3] = ld3(A.X, t.z);
         d3 dp = ld3(A.P, c.x), dt0 = ld3(A.P, t.x), dt1 = ld3(A.P, t.y), dt2 = ld3(A.P, t.z);
         const d3 mov = 0.25 * (((dt0 + dt1) + dt2) + dp);
@@ -653,6 +747,7 @@ __device__ inline bool accd_init(AccdLane& L, int2 c, const CCDArgs& A, double e
     } else {
         const int2 a = A.be[-c.x - 1], b = A.be[c.y];
         L.ee = true;
+        L.inst = A.vinst[a.x];
         L.x[0] = ld3(A.X, a.x); L.x[1] = ld3(A.X, a.y); L.x[2] = ld3(A.X, b.x); L.x[3] = ld3(A.X, b.y);
         d3 da0 = ld3(A.P, a.x), da1 = ld3(A.P, a.y), db0 = ld3(A.P, b.x), db1 = ld3(A.P, b.y);
         const d3 mov = 0.25 * (((da0 + da1) + db0) + db1);
@@ -667,7 +762,7 @@ __device__ inline bool accd_init(AccdLane& L, int2 c, const CCDArgs& A, double e
     L.dc = sqrt(L.d2);
     L.gap = eta * L.f / (L.dc + th);
     L.toc = 0;
-    L.tmax = tmax;
+    L.tmax = __longlong_as_double((long long)*((volatile const unsigned long long*)(amin + L.inst)));
     L.guard = 0;
     return true;
 }
@@ -691,7 +786,8 @@ __device__ inline int accd_step(AccdLane& L, double eta, double th)
 
 __global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, unsigned cap,
                                               const int2* __restrict__ cand, CCDArgs A, double eta,
-                                              unsigned long long* __restrict__ amin, unsigned* __restrict__ next)
+                                              unsigned long long* __restrict__ amin, unsigned* __restrict__ next,
+                                              unsigned long long* __restrict__ stepstat /* total, max */)
 {
     const unsigned npt = min(cnt[0], cap), nee = min(cnt[1], cap - npt);
     const unsigned n = npt + nee;
@@ -712,8 +808,7 @@ __global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, 
                     finished = true;
                 } else {
                     const int2 c = j < npt ? cand[j] : cand[cap - 1 - (j - npt)];
-                    const double tmax = __longlong_as_double((long long)*((volatile unsigned long long*)amin));
-                    active = accd_init(L, c, A, eta, tmax);
+                    active = accd_init(L, c, A, eta, amin);
                 }
             }
         }
@@ -723,42 +818,94 @@ __global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, 
         }
         if (active) {
             const int r = accd_step(L, eta, A.th);
-            if (r == 1) atomicMin(amin, (unsigned long long)__double_as_longlong(L.toc));
-            if (r != 0) active = false;
-            else if ((++steps & 15) == 0)
-                L.tmax = fmin(L.tmax, __longlong_as_double((long long)*((volatile unsigned long long*)amin)));
+            if (r == 1) atomicMin(amin + L.inst, (unsigned long long)__double_as_longlong(L.toc));
+            ++steps;
+            if (r != 0) {
+                active = false;
+                atomicMax(stepstat + 1, (unsigned long long)L.guard + 1);
+            } else if ((steps & 15) == 0) {
+                L.tmax = fmin(L.tmax,
+                              __longlong_as_double((long long)*((volatile unsigned long long*)(amin + L.inst))));
+            }
         }
     }
+    // total steps of this lane (all its pairs)
+    const unsigned long long tot = (unsigned long long)steps;
+    if (tot) atomicAdd(stepstat, tot);
 }
 
-// span cap (SPATIAL_HASH.h:466-482), one thread; also seeds the ACCD bound
-__global__ void k_ccd_cap(const double* __restrict__ eparts, int nep, long ne, const double* __restrict__ pparts,
-                          int npp, long nb, double alpha_in, double* __restrict__ alpha_out,
-                          unsigned long long* __restrict__ amin, unsigned* __restrict__ cnt)
+// span cap (SPATIAL_HASH.h:466-482), one thread per instance; also seeds the
+// instance's ACCD bound
+__global__ void k_ccd_cap(int B, const double* __restrict__ eparts, const int* __restrict__ epbeg,
+                          const int* __restrict__ ebeg, const double* __restrict__ pparts,
+                          const int* __restrict__ ppbeg, const int* __restrict__ nbeg,
+                          const double* __restrict__ alpha_in, double* __restrict__ alpha_out,
+                          unsigned long long* __restrict__ amin, const unsigned char* __restrict__ active)
 {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B || (active && !active[i])) return;
     double vs = 0.0, ps = 0.0;
-    for (int c = 0; c < nep; ++c) vs += eparts[c];
+    for (int c = epbeg[i]; c < epbeg[i + 1]; ++c) vs += eparts[c];
+    const long ne = ebeg[i + 1] - ebeg[i], nb = nbeg[i + 1] - nbeg[i];
     vs = ne ? vs / (double)ne : 1.0;
-    for (int c = 0; c < npp; ++c) ps += pparts[c];
+    for (int c = ppbeg[i]; c < ppbeg[i + 1]; ++c) ps += pparts[c];
     ps /= (double)(nb * 3);
-    double a = alpha_in;
+    double a = alpha_in[i];
     const double span = a * ps / vs;
     if (span > 1) a /= span;
-    *alpha_out = a;
-    *amin = (unsigned long long)__double_as_longlong(a);
-    cnt[0] = cnt[1] = cnt[2] = 0u;
+    alpha_out[i] = a;
+    amin[i] = (unsigned long long)__double_as_longlong(a);
 }
 
-__global__ void k_reset_ccd(const double* __restrict__ alpha, unsigned long long* __restrict__ amin,
-                            unsigned* __restrict__ cnt)
+__global__ void k_reset_amin(int B, const double* __restrict__ alpha, unsigned long long* __restrict__ amin)
 {
-    *amin = (unsigned long long)__double_as_longlong(*alpha);
-    cnt[0] = cnt[1] = cnt[2] = 0u;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < B) amin[i] = (unsigned long long)__double_as_longlong(alpha[i]);
+}
+
+__global__ void k_zero_u64(unsigned long long* __restrict__ a, int n)
+{
+    if ((int)threadIdx.x < n) a[threadIdx.x] = 0ull;
 }
 
 __global__ void k_zero_u32(unsigned* __restrict__ a, int n)
 {
     if ((int)threadIdx.x < n) a[threadIdx.x] = 0u;
+}
+
+// per-instance counts of sorted pass tuples (instance in the key's high bits)
+// and of merged tuples (instance of their first vertex)
+__global__ void k_count_pass(unsigned n, const unsigned long long* __restrict__ keys, unsigned* __restrict__ cnt)
+{
+    unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(cnt + (unsigned)(keys[i] >> 40), 1u);
+}
+__global__ void k_count_merged(unsigned n, const int4* __restrict__ t, const int* __restrict__ vinst,
+                               unsigned* __restrict__ cnt)
+{
+    unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(cnt + vinst[-t[i].x - 1], 1u);
+}
+
+// place sorted pass tuples and merged tuples at their instance's range:
+// instance i gets [pass_i | merged_i] at cbeg[i]
+__global__ void k_place_pass(unsigned n, const unsigned long long* __restrict__ keys, const int4* __restrict__ t,
+                             const long* __restrict__ cbeg, const long* __restrict__ pstart,
+                             int4* __restrict__ out)
+{
+    unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned inst = (unsigned)(keys[i] >> 40);
+    out[cbeg[inst] + (i - pstart[inst])] = t[i];
+}
+__global__ void k_place_merged(unsigned n, const int4* __restrict__ t, const int* __restrict__ vinst,
+                               const long* __restrict__ cbeg, const long* __restrict__ mstart,
+                               const unsigned* __restrict__ pcnt, int4* __restrict__ out)
+{
+    unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int inst = vinst[-t[i].x - 1];
+    out[cbeg[inst] + pcnt[inst] + (i - mstart[inst])] = t[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -815,36 +962,50 @@ int bits_for(unsigned long long maxv)
 int walk_blocks(long nq) { return grid_for(nq * WALK_G, 256); }
 
 // Boxes of the triangles and edges from the vertex boxes, their sorted
-// (2 * voxel + type, primitive) records and the cell table.  One host
-// synchronisation (record count and grid size).
-CellTable build_cells(ContactWork& w, hipStream_t s, const MeshDev& m, const GridDev* g, const int* vbox)
+// (keybase[instance] + 2 * voxel + type, primitive) records and the cell
+// table.  One host synchronisation (record count and the instance grids).
+CellTable build_cells(ContactWork& w, hipStream_t s, const MeshDev& m, const int* vbox,
+                      const unsigned char* active)
 {
+    const InstDev& I = m.inst;
+    const int B = I.B;
     const int n = m.nf + m.ne;
     int* pbox = w.pbox.ensure(6 * (size_t)std::max(n, 1), s);
     unsigned* cnt = w.u0.ensure(n + 1, s);
     unsigned* off = w.u1.ensure(n + 1, s);
-    IDP_LAUNCH(k_prim_boxes<3>, grid_for(m.nf, 256), 256, 0, s, m.nf, m.bt.p, m.v2b.p, vbox, g, pbox, cnt);
-    IDP_LAUNCH(k_prim_boxes<2>, grid_for(m.ne, 256), 256, 0, s, m.ne, m.be.p, m.v2b.p, vbox, g,
-               pbox + 6 * (size_t)m.nf, cnt + m.nf);
+    unsigned* pinst = w.pinst.ensure(n + 1, s);
+    IDP_LAUNCH(k_prim_boxes<3>, grid_for(m.nf, 256), 256, 0, s, m.nf, m.bt.p, m.v2b.p, I.vinst.p, active, vbox,
+               w.grids.p, pbox, cnt, pinst);
+    IDP_LAUNCH(k_prim_boxes<2>, grid_for(m.ne, 256), 256, 0, s, m.ne, m.be.p, m.v2b.p, I.vinst.p, active, vbox,
+               w.grids.p, pbox + 6 * (size_t)m.nf, cnt + m.nf, pinst + m.nf);
     IDP_LAUNCH(k_zero_u32, 1, 64, 0, s, cnt + n, 1);
     excl_scan(s, w.tmp, cnt, off, n + 1);
-    struct {
-        unsigned total;
-        GridDev g;
-    } h;
-    IDP_HIP(hipMemcpyAsync(&h.total, off + n, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-    IDP_HIP(hipMemcpyAsync(&h.g, g, sizeof(GridDev), hipMemcpyDeviceToHost, s));
+    unsigned total = 0;
+    std::vector<GridDev> hg(B);
+    IDP_HIP(hipMemcpyAsync(&total, off + n, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    IDP_HIP(hipMemcpyAsync(hg.data(), w.grids.p, B * sizeof(GridDev), hipMemcpyDeviceToHost, s));
     IDP_HIP(hipStreamSynchronize(s));
     CellTable T{};
-    T.n = h.total;
-    if (h.total == 0) return T;
-    unsigned* k0 = w.k0.ensure(h.total, s);
-    int* v0 = w.v0.ensure(h.total, s);
-    IDP_LAUNCH(k_emit_cells, grid_for(n, 256), 256, 0, s, n, m.nf, pbox, off, g, k0, v0);
-    unsigned* k1 = w.k1.ensure(h.total, s);
-    int* v1 = w.v1.ensure(h.total, s);
-    const unsigned long long nkeys = 2ull * h.g.cnt[0] * h.g.cnt[1] * h.g.cnt[2];
-    sort_pairs(s, w.tmp, k0, k1, v0, v1, h.total, bits_for(nkeys));
+    T.n = total;
+    // key bases: instance i owns keys [base_i, base_i + 2 * voxels_i)
+    std::vector<unsigned> base(B + 1, 0);
+    unsigned long long acc = 0;
+    for (int i = 0; i < B; ++i) {
+        base[i] = (unsigned)acc;
+        acc += 2ull * hg[i].cnt[0] * hg[i].cnt[1] * hg[i].cnt[2];
+        if (acc >= 0xFFFFFFFFull) throw HipError(IDP_E_CAPACITY, "voxel key space exceeds 32 bits");
+    }
+    base[B] = (unsigned)acc;
+    w.keybase.upload(base.data(), B + 1, s);
+    if (total == 0) return T;
+    unsigned* k0 = w.k0.ensure(total, s);
+    int* v0 = w.v0.ensure(total, s);
+    IDP_LAUNCH(k_emit_cells, grid_for(n, 256), 256, 0, s, n, m.nf, pbox, off, pinst, w.keybase.p, w.grids.p, k0,
+               v0);
+    unsigned* k1 = w.k1.ensure(total, s);
+    int* v1 = w.v1.ensure(total, s);
+    const unsigned long long nkeys = acc;
+    sort_pairs(s, w.tmp, k0, k1, v0, v1, total, bits_for(nkeys));
     T.keys = k1;
     T.vals = v1;
     if (nkeys <= DENSE_MAX) {
@@ -854,7 +1015,7 @@ CellTable build_cells(ContactWork& w, hipStream_t s, const MeshDev& m, const Gri
             IDP_LAUNCH(k_fill_u32, grid_for((long)w.cstart.cap, 256), 256, 0, s, w.cstart.cap, w.cstart.p, 0u);
             IDP_LAUNCH(k_fill_u32, grid_for((long)w.cend.cap, 256), 256, 0, s, w.cend.cap, w.cend.p, 0u);
         }
-        IDP_LAUNCH(k_cell_bounds, grid_for(h.total, 256), 256, 0, s, h.total, k1, w.cstart.p, w.cend.p, 0);
+        IDP_LAUNCH(k_cell_bounds, grid_for(total, 256), 256, 0, s, total, k1, w.cstart.p, w.cend.p, 0);
         T.start = w.cstart.p;
         T.end = w.cend.p;
     }
@@ -867,38 +1028,56 @@ void clear_cells(ContactWork& w, hipStream_t s, const CellTable& T)
         IDP_LAUNCH(k_cell_bounds, grid_for(T.n, 256), 256, 0, s, T.n, T.keys, w.cstart.p, w.cend.p, 1);
 }
 
-void grid_from(ContactWork& w, hipStream_t s, const MeshDev& m, const double* X, const int* idx, long nidx,
-               const double* p, const double* alpha_p, double pad, const double* ep, int nep)
+// per-instance grids from the chunked mean edge length (ep, echunk table)
+// and the bounding box of the elements of seg (idx: element -> vertex)
+void grid_from(ContactWork& w, hipStream_t s, const MeshDev& m, const double* X, const SegDev& seg, const int* idx,
+               const double* p, const double* alpha_p, double pad, const double* ep, const unsigned char* active)
 {
-    int nmm = grid_for(nidx, RB);
-    double* mm = w.mmparts.ensure(6 * (size_t)nmm + 6, s);
-    IDP_LAUNCH(k_minmax_parts, nmm, RB, 0, s, nidx, idx, X, p, alpha_p, mm);
-    IDP_LAUNCH(k_minmax_final, 1, RB, 0, s, nmm, mm, mm + 6 * (size_t)nmm);
-    GridDev* g = w.grid.ensure(1, s);
-    IDP_LAUNCH(k_grid_setup, 1, 1, 0, s, g, ep, nep, (long)m.ne, mm + 6 * (size_t)nmm, pad);
+    const InstDev& I = m.inst;
+    const int B = I.B;
+    double* mm = w.mmparts.ensure(6 * (size_t)seg.nblk + 6 * (size_t)B + 6, s);
+    double* mmf = mm + 6 * (size_t)seg.nblk;
+    IDP_LAUNCH(k_minmax_parts, seg.nblk, RB, 0, s, seg.lo.p, seg.hi.p, seg.inst.p, idx, X, p, alpha_p, mm);
+    IDP_LAUNCH(k_minmax_final, grid_for(B, 64), 64, 0, s, B, seg.pbeg.p, mm, mmf);
+    w.grids.ensure(B, s);
+    IDP_LAUNCH(k_grid_setup, grid_for(B, 64), 64, 0, s, B, w.grids.p, ep, I.echunk.pbeg.p, I.d_ebeg.p, mmf, pad,
+               active);
 }
 
 }  // namespace
 
-// Constraint set at X (device); result in w.cs (count returned).  Order:
-// PT pass tuples by (vertex, triangle), EE pass tuples by (edge, edge), then
-// the merged PP/PE entries in map order.
-long contact_constraint_set(ContactWork& w, hipStream_t s, const MeshDev& m, const double* X, double dHat2,
-                            double thickness)
+// Constraint set at X (device) for the instances of the active mask (device,
+// one byte per instance; null = all); the other instances keep theirs.
+// Result in w.cs, instance i's constraints at [w.h_cbeg[i], w.h_cbeg[i+1]),
+// each in the order PT pass tuples by (vertex, triangle), EE pass tuples by
+// (edge, edge), then the merged PP/PE entries in map order.
+long contact_constraint_set(ContactWork& w, hipStream_t s, const MeshDev& m, const double* X, const double* dHat2,
+                            double thickness, const unsigned char* active)
 {
-    const double r = sqrt(dHat2) + thickness;  // IPC.h:51-52
-    const double r2 = r * r;
-    w.ncs = 0;
-    if (m.nb == 0) return 0;
-    const int nep = grid_for(m.ne, 256);
-    double* ep = w.dparts.ensure(nep + 1, s);
-    IDP_LAUNCH(k_edge_len_parts, nep, 256, 0, s, m.ne, (const int2*)m.be.p, X, ep);
-    grid_from(w, s, m, X, nullptr, m.nV, nullptr, nullptr, 0.0, ep, nep);
-    const GridDev* g = w.grid.p;
+    const InstDev& I = m.inst;
+    const int B = I.B;
+    std::vector<double> rad(2 * B);
+    for (int i = 0; i < B; ++i) {
+        const double r = sqrt(dHat2[i]) + thickness;  // IPC.h:51-52
+        rad[i] = r;
+        rad[B + i] = r * r;
+    }
+    const std::vector<long> old_cbeg = w.h_cbeg.size() == (size_t)B + 1 ? w.h_cbeg : std::vector<long>(B + 1, 0);
+    const long old_ncs = w.ncs;
+    // rad is uploaded below, after the early exit
+    if (m.nb == 0) {
+        w.ncs = 0;
+        w.h_cbeg.assign(B + 1, 0);
+        w.cbeg.upload(w.h_cbeg.data(), B + 1, s);
+        return 0;
+    }
+    double* ep = w.dparts.ensure(I.echunk.nblk + 1, s);
+    IDP_LAUNCH(k_edge_len_parts, I.echunk.nblk, 256, 0, s, I.echunk.lo.p, I.echunk.hi.p, (const int2*)m.be.p, X, ep);
+    grid_from(w, s, m, X, I.vblk, nullptr, nullptr, nullptr, 0.0, ep, active);
     int* vbox = w.vbox.ensure(6 * (size_t)m.nb, s);
-    IDP_LAUNCH(k_vertex_boxes, grid_for(m.nb, 256), 256, 0, s, m.nb, m.bn.p, X, (const double*)nullptr,
-               (const double*)nullptr, 0.0, g, vbox);
-    CellTable T = build_cells(w, s, m, g, vbox);
+    IDP_LAUNCH(k_vertex_boxes, grid_for(m.nb, 256), 256, 0, s, m.nb, m.bn.p, I.vinst.p, X, (const double*)nullptr,
+               (const double*)nullptr, 0.0, w.grids.p, vbox);
+    CellTable T = build_cells(w, s, m, vbox, active);
 
     CSArgs A;
     A.X = X;
@@ -908,14 +1087,18 @@ long contact_constraint_set(ContactWork& w, hipStream_t s, const MeshDev& m, con
     A.be = (const int2*)m.be.p;
     A.bt = (const int3*)m.bt.p;
     A.pbox = w.pbox.p;
-    A.g = g;
+    A.grids = w.grids.p;
+    A.keybase = w.keybase.p;
+    A.vinst = I.vinst.p;
+    A.active = active;
+    A.rad = w.rad.upload_get(rad.data(), rad.size(), s);
+    A.B = B;
     A.nf = m.nf;
-    A.r = r;
-    A.r2 = r2;
     CSOut O;
-    O.nf = (unsigned long long)std::max(m.nf, 1);
-    O.ne = (unsigned long long)std::max(m.ne, 1);
-    O.V = (unsigned long long)m.nV;
+    O.vbeg = I.d_vbeg.p;
+    O.ebeg = I.d_ebeg.p;
+    O.fbeg = I.d_fbeg.p;
+    O.inst_shift = B > 1 ? 56 : 0;
     unsigned* cnt = w.cnt.ensure(4, s);
     if (w.cap_pass == 0) w.cap_pass = 4u * (m.nb + m.ne) + 1024;
     if (w.cap_dup == 0) w.cap_dup = 4u * (m.nb + m.ne) + 1024;
@@ -940,19 +1123,17 @@ long contact_constraint_set(ContactWork& w, hipStream_t s, const MeshDev& m, con
         clear_cells(w, s, T);
     }
     const unsigned npass = hc[0], nd = hc[1];
-    const unsigned long long V = O.V;
     // ---- order the pass tuples
     if (npass) {
         unsigned long long* pk1 = w.pk1.ensure(npass, s);
         int4* pv1 = w.dup.ensure(npass, s);
-        const unsigned long long maxkey = V * O.nf + O.ne * O.ne;
-        sort_pairs(s, w.tmp, w.pk0.p, pk1, w.pass.p, pv1, npass, bits_for(maxkey));
+        sort_pairs(s, w.tmp, w.pk0.p, pk1, w.pass.p, pv1, npass, bits_for(((unsigned long long)B << 40) | (1ull << 40)));
     }
     // ---- merge duplicates (IPC.h:599-654)
     unsigned nmerged = 0;
     if (nd) {
         unsigned long long* k1 = w.dk1.ensure(nd, s);
-        sort_keys(s, w.tmp, w.dk0.p, k1, nd, bits_for(V * V * (V + 1)));
+        sort_keys(s, w.tmp, w.dk0.p, k1, nd, 64);
         unsigned* head = w.u0.ensure(nd + 1, s);
         unsigned* hs = w.u1.ensure(nd + 1, s);
         IDP_LAUNCH(k_dup_heads, grid_for(nd, 256), 256, 0, s, nd, k1, head);
@@ -960,41 +1141,102 @@ long contact_constraint_set(ContactWork& w, hipStream_t s, const MeshDev& m, con
         excl_scan(s, w.tmp, head, hs, nd + 1);
         nmerged = read_u(s, hs + nd);
         w.merged.ensure(nmerged + 1, s);
-        IDP_LAUNCH(k_dup_write, grid_for(nd, 256), 256, 0, s, nd, k1, hs, V, w.merged.p);
+        IDP_LAUNCH(k_dup_write, grid_for(nd, 256), 256, 0, s, nd, k1, hs, I.d_vbeg.p, O.inst_shift, w.merged.p);
     }
-    long total = (long)npass + nmerged;
-    int4* cs = w.cs.ensure(total + 1, s);
-    if (npass) IDP_HIP(hipMemcpyAsync(cs, w.dup.p, sizeof(int4) * npass, hipMemcpyDeviceToDevice, s));
+    if (B == 1 && !active) {
+        const long total = (long)npass + nmerged;
+        int4* cs = w.cs.ensure(total + 1, s);
+        if (npass) IDP_HIP(hipMemcpyAsync(cs, w.dup.p, sizeof(int4) * npass, hipMemcpyDeviceToDevice, s));
+        if (nmerged)
+            IDP_HIP(hipMemcpyAsync(cs + npass, w.merged.p, sizeof(int4) * nmerged, hipMemcpyDeviceToDevice, s));
+        w.ncs = total;
+        w.h_cbeg = {0, total};
+        w.cbeg.upload(w.h_cbeg.data(), 2, s);
+        return total;
+    }
+    // ---- several instances: [pass_i | merged_i] per instance; inactive
+    // instances keep their previous constraints
+    std::vector<unsigned char> act(B, 1);
+    if (active) {
+        IDP_HIP(hipMemcpyAsync(act.data(), active, B, hipMemcpyDeviceToHost, s));
+    }
+    unsigned* ic = w.icnt.ensure(2 * (size_t)B, s);
+    IDP_HIP(hipMemsetAsync(ic, 0, 2 * B * sizeof(unsigned), s));
+    if (npass) IDP_LAUNCH(k_count_pass, grid_for(npass, 256), 256, 0, s, npass, w.pk1.p, ic);
+    if (nmerged) IDP_LAUNCH(k_count_merged, grid_for(nmerged, 256), 256, 0, s, nmerged, w.merged.p, I.vinst.p, ic + B);
+    std::vector<unsigned> hcnt(2 * B);
+    IDP_HIP(hipMemcpyAsync(hcnt.data(), ic, 2 * B * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    IDP_HIP(hipStreamSynchronize(s));
+    std::vector<long> cbeg(B + 1, 0), pstart(B, 0), mstart(B, 0);
+    long ps = 0, ms = 0;
+    for (int i = 0; i < B; ++i) {
+        pstart[i] = ps;
+        mstart[i] = ms;
+        ps += hcnt[i];
+        ms += hcnt[B + i];
+        const long n_i = act[i] ? (long)hcnt[i] + hcnt[B + i] : old_cbeg[i + 1] - old_cbeg[i];
+        cbeg[i + 1] = cbeg[i] + n_i;
+    }
+    const long total = cbeg[B];
+    // previous constraints of inactive instances
+    bool keep = false;
+    for (int i = 0; i < B; ++i) keep |= !act[i] && old_cbeg[i + 1] > old_cbeg[i];
+    if (keep) {
+        w.cs_old.ensure(old_ncs + 1, s);
+        IDP_HIP(hipMemcpyAsync(w.cs_old.p, w.cs.p, sizeof(int4) * old_ncs, hipMemcpyDeviceToDevice, s));
+    }
+    int4* cs = w.cs.ensure(total + 1, s, keep);
+    w.cbeg.upload(cbeg.data(), B + 1, s);
+    long* dps = (long*)w.dparts2.ensure(2 * (size_t)B + 2, s);
+    IDP_HIP(hipMemcpyAsync(dps, pstart.data(), B * sizeof(long), hipMemcpyHostToDevice, s));
+    IDP_HIP(hipMemcpyAsync(dps + B, mstart.data(), B * sizeof(long), hipMemcpyHostToDevice, s));
+    if (keep)
+        for (int i = 0; i < B; ++i)
+            if (!act[i] && old_cbeg[i + 1] > old_cbeg[i])
+                IDP_HIP(hipMemcpyAsync(cs + cbeg[i], w.cs_old.p + old_cbeg[i],
+                                       sizeof(int4) * (old_cbeg[i + 1] - old_cbeg[i]), hipMemcpyDeviceToDevice, s));
+    if (npass) IDP_LAUNCH(k_place_pass, grid_for(npass, 256), 256, 0, s, npass, w.pk1.p, w.dup.p, w.cbeg.p, dps, cs);
     if (nmerged)
-        IDP_HIP(hipMemcpyAsync(cs + npass, w.merged.p, sizeof(int4) * nmerged, hipMemcpyDeviceToDevice, s));
+        IDP_LAUNCH(k_place_merged, grid_for(nmerged, 256), 256, 0, s, nmerged, w.merged.p, I.vinst.p, w.cbeg.p,
+                   dps + B, ic, cs);
+    IDP_HIP(hipStreamSynchronize(s));  // host vectors above are read by async copies
     w.ncs = total;
+    w.h_cbeg = cbeg;
     return total;
 }
 
-// ACCD step size along P from X; returns alpha (host)
-double contact_ccd(ContactWork& w, hipStream_t s, const MeshDev& m, const double* X, const double* P,
-                   double thickness, double alpha_in, long* ncand_out)
+// ACCD step sizes along P from X, per instance: alpha_io (host, B) holds the
+// upper bounds on entry and the step sizes on return; instances outside the
+// active mask (device, null = all) are left untouched.
+void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const double* X, const double* P,
+                       double thickness, double* alpha_io, const unsigned char* active, long* ncand_out)
 {
+    const InstDev& I = m.inst;
+    const int B = I.B;
     if (ncand_out) *ncand_out = 0;
-    if (m.nb == 0) return alpha_in;
-    // span cap (mean edge length and mean |p|), kept on the device
-    const int nep = grid_for(m.ne, 256);
-    const int npp = grid_for(3L * m.nb, 256);
-    double* ep = w.dparts.ensure(nep + npp + 2, s);
-    double* pp = ep + nep + 1;
-    IDP_LAUNCH(k_edge_len_parts, nep, 256, 0, s, m.ne, (const int2*)m.be.p, X, ep);
-    IDP_LAUNCH(k_psize_parts, npp, 256, 0, s, m.nb, m.bn.p, P, pp);
-    double* dalpha = w.scal.ensure(4, s);
-    unsigned long long* amin = (unsigned long long*)w.scal.p + 1;
+    if (m.nb == 0) return;
+    // span cap (mean edge length and mean |p| per instance), kept on the device
+    double* ep = w.dparts.ensure(I.echunk.nblk + I.pchunk.nblk + 2, s);
+    double* pp = ep + I.echunk.nblk + 1;
+    IDP_LAUNCH(k_edge_len_parts, I.echunk.nblk, 256, 0, s, I.echunk.lo.p, I.echunk.hi.p, (const int2*)m.be.p, X, ep);
+    IDP_LAUNCH(k_psize_parts, I.pchunk.nblk, 256, 0, s, I.pchunk.lo.p, I.pchunk.hi.p, m.bn.p, P, pp);
+    double* ain = (double*)w.scal2.ensure(2 * (size_t)B + 4, s);
+    IDP_HIP(hipMemcpyAsync(ain, alpha_io, B * sizeof(double), hipMemcpyHostToDevice, s));
+    double* dalpha = w.ialpha.ensure(B, s);
+    unsigned long long* amin = w.iamin.ensure(B, s);
     unsigned* cnt = w.cnt.ensure(4, s);
-    IDP_LAUNCH(k_ccd_cap, 1, 1, 0, s, ep, nep, (long)m.ne, pp, npp, (long)m.nb, alpha_in, dalpha, amin, cnt);
+    unsigned long long* stepstat = (unsigned long long*)w.scal.ensure(4, s) + 2;
+    IDP_LAUNCH(k_ccd_cap, grid_for(B, 64), 64, 0, s, B, ep, I.echunk.pbeg.p, I.d_ebeg.p, pp, I.pchunk.pbeg.p,
+               I.d_nbeg.p, ain, dalpha, amin, active);
+    IDP_LAUNCH(k_zero_u32, 1, 64, 0, s, cnt, 3);
+    IDP_LAUNCH(k_zero_u64, 1, 64, 0, s, stepstat, 2);
 
     // swept grid over boundary nodes at [x, x + alpha p] -/+ th/2 (SPATIAL_HASH.h:484-511)
-    grid_from(w, s, m, X, m.bn.p, m.nb, P, dalpha, thickness / 2, ep, nep);
-    const GridDev* g = w.grid.p;
+    grid_from(w, s, m, X, I.nblk, m.bn.p, P, dalpha, thickness / 2, ep, active);
     int* vbox = w.vbox.ensure(6 * (size_t)m.nb, s);
-    IDP_LAUNCH(k_vertex_boxes, grid_for(m.nb, 256), 256, 0, s, m.nb, m.bn.p, X, P, dalpha, thickness / 2, g, vbox);
-    CellTable T = build_cells(w, s, m, g, vbox);
+    IDP_LAUNCH(k_vertex_boxes, grid_for(m.nb, 256), 256, 0, s, m.nb, m.bn.p, I.vinst.p, X, P, dalpha,
+               thickness / 2, w.grids.p, vbox);
+    CellTable T = build_cells(w, s, m, vbox, active);
 
     CCDArgs A;
     A.X = X;
@@ -1005,39 +1247,62 @@ double contact_ccd(ContactWork& w, hipStream_t s, const MeshDev& m, const double
     A.bt = (const int3*)m.bt.p;
     A.vbox = vbox;
     A.pbox = w.pbox.p;
-    A.g = g;
+    A.grids = w.grids.p;
+    A.keybase = w.keybase.p;
+    A.vinst = I.vinst.p;
+    A.active = active;
     A.nf = m.nf;
     A.th = thickness;
     if (w.cap_cand == 0) w.cap_cand = 16u * (m.nb + m.ne) + 4096;
-    struct {
-        unsigned long long abits;
-        unsigned n[2];
-    } h{0, {0, 0}};
+    std::vector<unsigned long long> ha(B);
+    unsigned long long hsteps[2] = {0, 0};
+    unsigned hn[2] = {0, 0};
     for (int attempt = 0; attempt < 2; ++attempt) {
         int2* cand = w.cand.ensure(w.cap_cand, s);
         const unsigned cap = (unsigned)w.cand.cap;
-        if (attempt) IDP_LAUNCH(k_reset_ccd, 1, 1, 0, s, dalpha, amin, cnt);
+        if (attempt) {
+            IDP_LAUNCH(k_reset_amin, grid_for(B, 64), 64, 0, s, B, dalpha, amin);
+            IDP_LAUNCH(k_zero_u32, 1, 64, 0, s, cnt, 3);
+            IDP_LAUNCH(k_zero_u64, 1, 64, 0, s, stepstat, 2);
+        }
         if (T.n)
             IDP_LAUNCH(k_ccd_walk<WALK_G>, walk_blocks((long)m.nb + m.ne), 256, 0, s, m.nb, m.ne, A, T, cnt, cand,
                        cap);
         const int tok = w.prof ? w.prof->begin(IDP_K_CCD) : -1;
         // a work queue needs no more lanes than the chip holds at once
-        IDP_LAUNCH(k_accd, 2048, 256, 0, s, cnt, cap, cand, A, 0.1, amin, cnt + 2);
+        IDP_LAUNCH(k_accd, 2048, 256, 0, s, cnt, cap, cand, A, 0.1, amin, cnt + 2, stepstat);
         if (w.prof) w.prof->end(IDP_K_CCD, tok);
-        IDP_HIP(hipMemcpyAsync(&h.abits, amin, sizeof(h.abits), hipMemcpyDeviceToHost, s));
-        IDP_HIP(hipMemcpyAsync(h.n, cnt, sizeof(h.n), hipMemcpyDeviceToHost, s));
+        IDP_HIP(hipMemcpyAsync(ha.data(), amin, B * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+        IDP_HIP(hipMemcpyAsync(hsteps, stepstat, sizeof(hsteps), hipMemcpyDeviceToHost, s));
+        IDP_HIP(hipMemcpyAsync(hn, cnt, sizeof(hn), hipMemcpyDeviceToHost, s));
         IDP_HIP(hipStreamSynchronize(s));
-        const size_t total = (size_t)h.n[0] + h.n[1];
+        const size_t total = (size_t)hn[0] + hn[1];
         if (total <= cap) break;
         w.cap_cand = std::max<size_t>(w.cap_cand, total + total / 4 + 4096);
         if (attempt == 1) throw HipError(IDP_E_CAPACITY, "CCD candidate overflow");
     }
     clear_cells(w, s, T);
-    if (w.prof) w.prof->add_work(IDP_K_CCD, (double)h.n[0] + h.n[1]);
-    if (ncand_out) *ncand_out = (long)h.n[0] + h.n[1];
-    double alpha;
-    std::memcpy(&alpha, &h.abits, sizeof(alpha));
-    return alpha;
+    if (w.prof) {
+        w.prof->add_work(IDP_K_CCD, (double)hn[0] + hn[1]);
+        w.prof->ccd_steps += (double)hsteps[0];
+        w.prof->ccd_max_steps = std::max(w.prof->ccd_max_steps, (double)hsteps[1]);
+    }
+    if (ncand_out) *ncand_out = (long)hn[0] + hn[1];
+    std::vector<unsigned char> act(B, 1);
+    if (active) {
+        IDP_HIP(hipMemcpyAsync(act.data(), active, B, hipMemcpyDeviceToHost, s));
+        IDP_HIP(hipStreamSynchronize(s));
+    }
+    for (int i = 0; i < B; ++i)
+        if (act[i]) std::memcpy(&alpha_io[i], &ha[i], sizeof(double));
+}
+
+double contact_ccd(ContactWork& w, hipStream_t s, const MeshDev& m, const double* X, const double* P,
+                   double thickness, double alpha_in, long* ncand_out)
+{
+    double a = alpha_in;
+    contact_ccd_batch(w, s, m, X, P, thickness, &a, nullptr, ncand_out);
+    return a;
 }
 
 }  // namespace idp

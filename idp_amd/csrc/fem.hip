@@ -174,21 +174,22 @@ __device__ inline void make_pd3(double* A)
 }  // namespace
 
 // psi per tet (w * vol * psi), reduced per block; invalid flag if det F <= 0
-__global__ void __launch_bounds__(RB) k_tet_energy(int nT, const int4* __restrict__ tets,
+__global__ void __launch_bounds__(RB) k_tet_energy(const int* __restrict__ lo, const int* __restrict__ hi,
+                                                   const int* __restrict__ binst, const int4* __restrict__ tets,
                                                    const double* __restrict__ IB, const double* __restrict__ vol,
                                                    const double* __restrict__ lam, const double* __restrict__ mu,
                                                    const double* __restrict__ X, double w,
                                                    double* __restrict__ part, int* __restrict__ invalid)
 {
     __shared__ double sh[RB];
-    int t = blockIdx.x * RB + threadIdx.x;
+    const int t = lo[blockIdx.x] + threadIdx.x;
     double e = 0.0;
-    if (t < nT) {
+    if (t < hi[blockIdx.x]) {
         double F[9];
         def_grad(X, tets[t], IB + 9 * (size_t)t, F);
         double J = det3(F);
         if (J <= 0) {
-            atomicOr(invalid, 1);
+            atomicOr(invalid + binst[blockIdx.x], 1);
         } else {
 #pragma clang fp contract(fast)
             double s2 = 0.0;
@@ -348,14 +349,14 @@ __global__ void __launch_bounds__(128) k_tet_hess(int nT, const int4* __restrict
         }
 }
 
-__global__ void __launch_bounds__(RB) k_inertia_energy(int nV, const double* __restrict__ X,
-                                                       const double* __restrict__ Xt,
+__global__ void __launch_bounds__(RB) k_inertia_energy(const int* __restrict__ lo, const int* __restrict__ hi,
+                                                       const double* __restrict__ X, const double* __restrict__ Xt,
                                                        const double* __restrict__ mass, double* __restrict__ part)
 {
     __shared__ double sh[RB];
-    int v = blockIdx.x * RB + threadIdx.x;
+    const int v = lo[blockIdx.x] + threadIdx.x;
     double e = 0.0;
-    if (v < nV) {
+    if (v < hi[blockIdx.x]) {
         double m = mass[v];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -368,13 +369,11 @@ __global__ void __launch_bounds__(RB) k_inertia_energy(int nV, const double* __r
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-int tet_energy_blocks(int nT) { return grid_for(nT, RB); }
-int inertia_energy_blocks(int nV) { return grid_for(nV, RB); }
-
-void launch_tet_energy(hipStream_t s, const MeshDev& m, const double* X, double w, double* part, int* invalid)
+void launch_tet_energy(hipStream_t s, const MeshDev& m, const SegDev& seg, const double* X, double w, double* part,
+                       int* invalid)
 {
-    IDP_LAUNCH(k_tet_energy, grid_for(m.nT, RB), RB, 0, s, m.nT, (const int4*)m.tets.p, m.IB.p, m.vol.p, m.lam.p,
-               m.mu.p, X, w, part, invalid);
+    IDP_LAUNCH(k_tet_energy, seg.nblk, RB, 0, s, seg.lo.p, seg.hi.p, seg.inst.p, (const int4*)m.tets.p, m.IB.p,
+               m.vol.p, m.lam.p, m.mu.p, X, w, part, invalid);
 }
 void launch_tet_grad(hipStream_t s, const MeshDev& m, const double* X, double w, double* gl)
 {
@@ -391,10 +390,10 @@ void launch_tet_hess_dense(hipStream_t s, const MeshDev& m, const double* X, dou
     IDP_LAUNCH(k_tet_hess, grid_for(m.nT, 128), 128, 0, s, m.nT, (const int4*)m.tets.p, m.IB.p, m.vol.p, m.lam.p,
                m.mu.p, X, w, project, 1, Hl);
 }
-void launch_inertia_energy(hipStream_t s, int nV, const double* X, const double* Xt, const double* mass,
+void launch_inertia_energy(hipStream_t s, const SegDev& seg, const double* X, const double* Xt, const double* mass,
                            double* part)
 {
-    IDP_LAUNCH(k_inertia_energy, grid_for(nV, RB), RB, 0, s, nV, X, Xt, mass, part);
+    IDP_LAUNCH(k_inertia_energy, seg.nblk, RB, 0, s, seg.lo.p, seg.hi.p, X, Xt, mass, part);
 }
 
 }  // namespace idp

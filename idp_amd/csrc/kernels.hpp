@@ -14,6 +14,57 @@ struct GridDev {
     int pad_;
 };
 
+// Blocks of a segmented reduction: the elements of instance i are
+// [beg[i], beg[i+1]); each instance is cut into blocks of bs elements starting
+// at its first element, so a block never straddles two instances and an
+// instance's partials are [pbeg[i], pbeg[i+1]).  With one instance this is
+// the plain grid of ceil(n / bs) blocks.
+struct SegDev {
+    int nblk = 0, bs = 0;
+    DBuf<int> lo, hi, inst, pbeg;
+    std::vector<int> h_pbeg;
+    void build(const std::vector<long>& beg, int block, hipStream_t s)
+    {
+        bs = block;
+        const int B = (int)beg.size() - 1;
+        std::vector<int> l, h, in;
+        h_pbeg.assign(B + 1, 0);
+        for (int i = 0; i < B; ++i) {
+            h_pbeg[i] = (int)l.size();
+            for (long a = beg[i]; a < beg[i + 1]; a += block) {
+                l.push_back((int)a);
+                h.push_back((int)std::min<long>(a + block, beg[i + 1]));
+                in.push_back(i);
+            }
+        }
+        h_pbeg[B] = (int)l.size();
+        nblk = (int)l.size();
+        lo.upload(l.data(), l.size(), s);
+        hi.upload(h.data(), h.size(), s);
+        inst.upload(in.data(), in.size(), s);
+        pbeg.upload(h_pbeg.data(), h_pbeg.size(), s);
+    }
+};
+
+// Instance layout of a mesh holding B independent meshes, each a contiguous
+// range of vertices, tets and boundary primitives (a plain mesh is B = 1).
+// Contact detection, CCD step sizes, energies and the PCG are computed per
+// instance; instances never interact.
+struct InstDev {
+    int B = 0;
+    std::vector<long> vbeg, tbeg, nbeg, ebeg, fbeg;  // host ranges, B + 1 each
+    DBuf<int> vinst;                                 // vertex -> instance
+    DBuf<int> d_vbeg, d_nbeg, d_ebeg, d_fbeg;        // device copies
+    SegDev echunk;   // boundary edges, chunks of 256 (mean edge length)
+    SegDev pchunk;   // 3 x boundary nodes, chunks of 256 (mean |p|)
+    SegDev vblk;     // vertices, blocks of RB (bounding box, inertia energy)
+    SegDev nblk;     // boundary nodes, blocks of RB (swept bounding box)
+    SegDev tblk;     // tets, blocks of RB (elastic energy)
+    SegDev pcg_rows; // vertices, 32-row blocks (PCG SpMV, 8 lanes per row)
+    SegDev pcg_upd;  // vertices, 64-row blocks (PCG update)
+    SegDev dblk;     // 3 x vertices, blocks of RB (per-instance dot products)
+};
+
 // Mesh resident on the device.  Vertex arrays are 3-vectors interleaved
 // (x0,y0,z0,x1,...): a vertex's coordinates share one 32-byte sector for the
 // gathers of the contact and element kernels, and the Newton vectors match
@@ -35,11 +86,12 @@ struct MeshDev {
     DBuf<int> e_src_ptr, e_src;      // per block: contributing (tet*16 + a*4 + b)
     DBuf<int> e_diag;                // block index of (v,v)
     long nnzb_e = 0;
+    InstDev inst;                    // instance layout (one instance for a plain mesh)
 };
 
 struct ContactWork {
     DBuf<unsigned char> tmp;
-    DBuf<double> dparts, dparts2, mmparts, scal;
+    DBuf<double> dparts, dparts2, mmparts, scal, scal2;
     DBuf<GridDev> grid;
     DBuf<int> vbox, pbox;
     DBuf<unsigned> u0, u1, cP, cD, oP, oD, k0, k1;
@@ -48,38 +100,63 @@ struct ContactWork {
     DBuf<unsigned long long> dk0, dk1, pk0, pk1;
     DBuf<unsigned> cstart, cend, cnt;  // dense cell table, atomic counters
     DBuf<int2> cand;
+    // per-instance data of the batched path
+    DBuf<GridDev> grids;
+    DBuf<unsigned> keybase, pinst;
+    DBuf<double> ialpha;                   // CCD: capped alpha per instance
+    DBuf<double> rad;                      // per-instance contact radius and its square
+    DBuf<unsigned long long> iamin;        // CCD: running minimum per instance
+    DBuf<unsigned> icnt;                   // per-instance counters
+    DBuf<long> cbeg;                       // constraint range per instance
+    std::vector<long> h_cbeg;
+    DBuf<int4> cs_old;
     size_t cap_pass = 0, cap_dup = 0, cap_cand = 0;
     long ncs = 0;
     Profiler* prof = nullptr;
 };
 
 // barrier.hip
+// Barrier parameters per instance (device arrays of B entries); a constraint's
+// instance is that of its first vertex (vinst null: instance 0).
+struct BarrierParams {
+    const double* dh;     // dHat2 + 2 sqrt(dHat2) thickness
+    const double* kappa;
+    const double* th2;    // thickness^2
+    const int* vinst;
+    __device__ int inst_of(int v) const { return vinst ? vinst[v] : 0; }
+};
 void barrier_init_constants();
-void launch_barrier_energy(hipStream_t s, long n, const int* cs, const double* X, const double* X0, double dh,
-                           double kappa, double th2, double* part, int* err);
-int barrier_energy_blocks(long n);
-void launch_min_dist2(hipStream_t s, long n, const int* cs, const double* X, double* dist2,
+void launch_barrier_energy(hipStream_t s, int nblk, const int* lo, const int* hi, const int* cs, const double* X,
+                           const double* X0, const BarrierParams& bp, double* part, int* err);
+void launch_min_dist2(hipStream_t s, long n, const int* cs, const double* X, const int* vinst, double* dist2,
                       unsigned long long* dmin);
-void launch_barrier_grad(hipStream_t s, long n, const int* cs, const double* X, const double* X0, double dh,
-                         double kappa, double th2, double* gout, int* err);
-void launch_barrier_hess(hipStream_t s, long n, const int* cs, const double* X, const double* X0, double dh,
-                         double kappa, double th2, int project_spd, int mode, double* out, int* vid, int* err,
+void launch_barrier_grad(hipStream_t s, long n, const int* cs, const double* X, const double* X0,
+                         const BarrierParams& bp, double* gout, int* err);
+void launch_barrier_hess(hipStream_t s, long n, const int* cs, const double* X, const double* X0,
+                         const BarrierParams& bp, int project_spd, int mode, double* out, int* vid, int* err,
                          int* counts = nullptr);
 void launch_constraint_vids(hipStream_t s, long n, const int* cs, int* vid, int* nv);
 
 // contact.hip
-long contact_constraint_set(ContactWork& w, hipStream_t s, const MeshDev& m, const double* X, double dHat2,
-                            double thickness);
+// dHat2: host array, one per instance
+long contact_constraint_set(ContactWork& w, hipStream_t s, const MeshDev& m, const double* X, const double* dHat2,
+                            double thickness, const unsigned char* active = nullptr);
 double contact_ccd(ContactWork& w, hipStream_t s, const MeshDev& m, const double* X, const double* P,
                    double thickness, double alpha_in, long* ncand);
+void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const double* X, const double* P,
+                       double thickness, double* alpha_io, const unsigned char* active, long* ncand);
 
-// fem.hip
-int tet_energy_blocks(int nT);
-void launch_tet_energy(hipStream_t s, const MeshDev& m, const double* X, double w, double* part, int* invalid);
+// linalg.hip: instance layout (ranges are B + 1 offsets; vinst: vertex -> instance)
+void build_inst_layout(MeshDev& m, const std::vector<long>& vbeg, const std::vector<long>& tbeg,
+                       const std::vector<long>& nbeg, const std::vector<long>& ebeg, const std::vector<long>& fbeg,
+                       hipStream_t s);
+
+// fem.hip (energies over block tables: lo/hi element ranges, binst block -> instance)
+void launch_tet_energy(hipStream_t s, const MeshDev& m, const SegDev& seg, const double* X, double w, double* part,
+                       int* invalid);
 void launch_tet_grad(hipStream_t s, const MeshDev& m, const double* X, double w, double* gl);
 void launch_tet_hess(hipStream_t s, const MeshDev& m, const double* X, double w, int project, double* Hl);
-void launch_inertia_energy(hipStream_t s, int nV, const double* X, const double* Xt, const double* mass,
+void launch_inertia_energy(hipStream_t s, const SegDev& seg, const double* X, const double* Xt, const double* mass,
                            double* part);
-int inertia_energy_blocks(int nV);
 
 }  // namespace idp

@@ -44,9 +44,8 @@ __global__ void k_elastic_values(long nnzb, const int* __restrict__ rowof, const
         acc[4] += m;
         acc[8] += m;
     }
-    double* o = val + 9 * (size_t)k;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) o[i] = acc[i];
+    for (int i = 0; i < 9; ++i) val[i * (size_t)nnzb + k] = acc[i];  // component planes
 }
 
 // g_v = elastic (sum over incident tets) + M (x - x~) + barrier (sum over the
@@ -181,7 +180,7 @@ __global__ void k_contact_values(unsigned nb, const unsigned* __restrict__ seg, 
         for (int i = 0; i < 9; ++i) acc[i] += h[i];
     }
 #pragma unroll
-    for (int i = 0; i < 9; ++i) cval[9 * (size_t)b + i] = acc[i];
+    for (int i = 0; i < 9; ++i) cval[i * (size_t)nb + b] = acc[i];  // component planes
 }
 
 // DBC projection (CSR_MATRIX.h:130-141): blocks touching a DBC node -> delta_ij
@@ -194,34 +193,34 @@ __global__ void k_project_dbc(long nnzb, const int* __restrict__ row, const int*
     if (!(dbc[r] || dbc[c])) return;
     double d = (r == c && keep_diag_identity) ? 1.0 : 0.0;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) val[9 * (size_t)k + i] = (i % 4 == 0) ? d : 0.0;
+    for (int i = 0; i < 9; ++i) val[i * (size_t)nnzb + k] = (i % 4 == 0) ? d : 0.0;
 }
 
 // ---------------------------------------------------------------------------
 // PCG kernels
 // ---------------------------------------------------------------------------
 // block-Jacobi: inverse of E_vv + C_vv
-__global__ void k_block_jacobi(int nV, const int* __restrict__ ediag, const double* __restrict__ evals,
-                               const int* __restrict__ cdiag, const double* __restrict__ cvals,
-                               double* __restrict__ Pinv, int* __restrict__ err)
+__global__ void k_block_jacobi(int nV, const int* __restrict__ ediag, const double* __restrict__ evals, long eld,
+                               const int* __restrict__ cdiag, const double* __restrict__ cvals, long cld,
+                               const int* __restrict__ vinst, double* __restrict__ Pinv, int* __restrict__ err)
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nV) return;
     double a[9];
-    const double* e = evals + 9 * (size_t)ediag[v];
+    const long ed = ediag[v];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) a[i] = e[i];
+    for (int i = 0; i < 9; ++i) a[i] = evals[i * eld + ed];
     if (cdiag) {
         int c = cdiag[v];
         if (c >= 0)
 #pragma unroll
-            for (int i = 0; i < 9; ++i) a[i] += cvals[9 * (size_t)c + i];
+            for (int i = 0; i < 9; ++i) a[i] += cvals[i * cld + c];
     }
     double c0 = a[4] * a[8] - a[5] * a[7], c1 = a[5] * a[6] - a[3] * a[8], c2 = a[3] * a[7] - a[4] * a[6];
     double det = a[0] * c0 + a[1] * c1 + a[2] * c2;
     double* o = Pinv + 9 * (size_t)v;
     if (!(det > 0)) {
-        atomicOr(err, 1);
+        atomicOr(err + vinst[v], 1);
         det = 1.0;
     }
     double id = 1.0 / det;
@@ -256,10 +255,11 @@ struct PcgArgs {
     int nV;
     const int* erp;
     const int* ecol;
-    const double* eval;
-    const int* crp;  // null when there are no contact blocks
+    const double* eval;  // 9 component planes of eld blocks
+    const int* crp;      // null when there are no contact blocks
     const int* ccol;
-    const double* cval;
+    const double* cval;  // 9 component planes of cld blocks
+    long eld, cld;
     const double* b;
     const double* Pinv;
     double* x;
@@ -271,42 +271,86 @@ struct PcgArgs {
     double* part_rz;  // 2 x nbB
     double* part_rr;  // 2 x nbB
     int nbA, nbB;
-    int* flag;        // 0 running, 1 stopped (converged or maxit), 2 breakdown
-    double* scal;     // [0] |b|^2, [1] final |r|^2, [2] iterations
+    // instance tables: SpMV blocks of 32 rows (A*) and update blocks of 64
+    // rows (B*), each aligned to its instance's first row
+    const int *alo, *ahi, *ainst, *apbeg;
+    const int *blo, *bhi, *binst, *bpbeg;
+    int* flag;        // per instance: 0 running, 1 stopped (converged or maxit), 2 breakdown, 3 not solving
+    double* scal;     // per instance, 3 each: |b|^2, final |r|^2, iterations
     double rtol2;
     int maxit;
 };
 
-constexpr int PCG_SPMV_BLOCK = 1024;
+constexpr int PCG_SPMV_BLOCK = 256;  // 4 wavefronts: enough blocks to cover every CU
+constexpr int PCG_UPD_BLOCK = 64;    // one wavefront: reductions need no barrier
 
-template <int G>
-__device__ inline void group_row_spmv(int v, int lane, const int* __restrict__ rp, const int* __restrict__ col,
-                                      const double* __restrict__ val, const double* __restrict__ z,
-                                      const double* __restrict__ po, double beta, double& y0, double& y1, double& y2)
+// Sum over a wavefront by a fixed xor butterfly (every lane gets the total).
+__device__ inline double wave_sum(double v)
 {
-    const int k1 = rp[v + 1];
-    for (int k = rp[v] + lane; k < k1; k += G) {
-        const double* bk = val + 9 * (size_t)k;
-        const size_t c = 3 * (size_t)col[k];
-        double x0 = z[c], x1 = z[c + 1], x2 = z[c + 2];
-        if (po) {
-            x0 += beta * po[c];
-            x1 += beta * po[c + 1];
-            x2 += beta * po[c + 2];
-        }
-        y0 += bk[0] * x0 + bk[1] * x1 + bk[2] * x2;
-        y1 += bk[3] * x0 + bk[4] * x1 + bk[5] * x2;
-        y2 += bk[6] * x0 + bk[7] * x1 + bk[8] * x2;
-    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
 
-// x = 0, r = b, z = P b, partials of r.z and r.r (parity 0)
-__global__ void __launch_bounds__(RB) k_pcg_init(PcgArgs A)
+// Sum of n partials by one wavefront in a fixed order.  The loads of a chunk
+// of 16 per lane are issued together (independent), then added in order, so
+// the latency is paid once per 1024 partials rather than once per 64.
+__device__ inline double wave_sum_partials(const double* __restrict__ part, int n)
 {
-    __shared__ double sh[RB];
-    const int v = blockIdx.x * RB + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    double acc = 0.0;
+    for (int base = 0; base < n; base += 1024) {
+        double v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int i = base + lane + 64 * k;
+            v[k] = i < n ? part[i] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) acc += v[k];
+    }
+    return wave_sum(acc);
+}
+
+// y += A_row z and w += A_row p_old over this lane's blocks of row v (lanes
+// of a group take blocks lane, lane + G, ...; two blocks per step so their
+// loads are in flight together).  val holds the 9 block components as planes
+// of ld doubles, so the lanes of a group read consecutive addresses.
+template <int G>
+__device__ inline void group_row_spmv2(int v, int lane, const int* __restrict__ rp, const int* __restrict__ col,
+                                       const double* __restrict__ val, long ld, const double* __restrict__ z,
+                                       const double* __restrict__ po, double* y, double* w)
+{
+    const int k1 = rp[v + 1];
+    int k = rp[v] + lane;
+    auto block = [&](int kk) {
+        const double bk[9] = {val[kk], val[ld + kk], val[2 * ld + kk], val[3 * ld + kk], val[4 * ld + kk],
+                              val[5 * ld + kk], val[6 * ld + kk], val[7 * ld + kk], val[8 * ld + kk]};
+        const size_t c = 3 * (size_t)col[kk];
+        const double x0 = z[c], x1 = z[c + 1], x2 = z[c + 2];
+        y[0] += bk[0] * x0 + bk[1] * x1 + bk[2] * x2;
+        y[1] += bk[3] * x0 + bk[4] * x1 + bk[5] * x2;
+        y[2] += bk[6] * x0 + bk[7] * x1 + bk[8] * x2;
+        if (po) {
+            const double p0 = po[c], p1 = po[c + 1], p2 = po[c + 2];
+            w[0] += bk[0] * p0 + bk[1] * p1 + bk[2] * p2;
+            w[1] += bk[3] * p0 + bk[4] * p1 + bk[5] * p2;
+            w[2] += bk[6] * p0 + bk[7] * p1 + bk[8] * p2;
+        }
+    };
+    for (; k + G < k1; k += 2 * G) {
+        block(k);
+        block(k + G);
+    }
+    if (k < k1) block(k);
+}
+
+// x = 0, r = b, z = P b, partials of r.z and r.r (parity 0); one wavefront per block
+__global__ void __launch_bounds__(PCG_UPD_BLOCK) k_pcg_init(PcgArgs A)
+{
+    const int v = A.blo[blockIdx.x] + threadIdx.x;
     double drz = 0.0, drr = 0.0;
-    if (v < A.nV) {
+    if (v < A.bhi[blockIdx.x]) {
         const double rv[3] = {A.b[3 * v], A.b[3 * v + 1], A.b[3 * v + 2]};
         const double* P = A.Pinv + 9 * (size_t)v;
 #pragma unroll
@@ -319,99 +363,134 @@ __global__ void __launch_bounds__(RB) k_pcg_init(PcgArgs A)
             drr += rv[i] * rv[i];
         }
     }
-    const double a = block_sum<RB>(drz, sh), c = block_sum<RB>(drr, sh);
+    drz = wave_sum(drz);
+    drr = wave_sum(drr);
     if (threadIdx.x == 0) {
-        A.part_rz[blockIdx.x] = a;
-        A.part_rr[blockIdx.x] = c;
-        if (blockIdx.x == 0) A.flag[0] = 0;
+        A.part_rz[blockIdx.x] = drz;
+        A.part_rr[blockIdx.x] = drr;
     }
 }
 
+// q = A p_it with p_it = z + beta p_(it-1), computed as A z + beta A p_(it-1)
+// so that the matrix stream does not wait for beta (the reduction of the
+// previous partials runs in wavefront 0 while the others load)
 template <int G>
 __global__ void __launch_bounds__(PCG_SPMV_BLOCK) k_pcg_spmv(PcgArgs A, int it)
 {
-    __shared__ double sh[PCG_SPMV_BLOCK];
+    __shared__ double sc[4];
+    __shared__ double wsum[PCG_SPMV_BLOCK / 64];
     __shared__ int sflag;
-    if (threadIdx.x == 0) sflag = *(volatile int*)A.flag;
+    const int par = it & 1;
+    const int inst = A.ainst[blockIdx.x];
+    const bool skip = *(volatile int*)(A.flag + inst) != 0;  // one load per wavefront
+    const double* po = it > 0 ? A.pbuf + (size_t)(par ^ 1) * 3 * A.nV : nullptr;
+    const int lane = threadIdx.x & (G - 1);
+    const int v = A.alo[blockIdx.x] + threadIdx.x / G;
+    const bool row = v < A.ahi[blockIdx.x] && !skip;
+    double y[3] = {0.0, 0.0, 0.0}, w[3] = {0.0, 0.0, 0.0};
+    if (threadIdx.x < 64) {
+        // wavefront 0: reduce the instance's partials of the previous update and decide
+        const int f = *(volatile int*)(A.flag + inst);
+        const int pb = A.bpbeg[inst], np = A.bpbeg[inst + 1] - pb;
+        const double rz = wave_sum_partials(A.part_rz + par * A.nbB + pb, np);
+        const double rr = wave_sum_partials(A.part_rr + par * A.nbB + pb, np);
+        const double rzo = it > 0 ? wave_sum_partials(A.part_rz + (par ^ 1) * A.nbB + pb, np) : 1.0;
+        if (threadIdx.x == 0) {
+            double* sci = A.scal + 3 * inst;
+            const double bb = it == 0 ? rr : sci[0];
+            int stop = f != 0 ? 2 : 0;
+            if (!stop && (!(bb > 0.0) || (it > 0 && rr <= A.rtol2 * bb) || it >= A.maxit)) {
+                stop = 1;
+                if (blockIdx.x == A.apbeg[inst]) {
+                    A.flag[inst] = 1;
+                    sci[0] = bb;
+                    sci[1] = rr;
+                    sci[2] = (double)it;
+                }
+            }
+            if (!stop && it == 0 && blockIdx.x == A.apbeg[inst]) sci[0] = bb;
+            sflag = stop;
+            sc[0] = it > 0 ? rz / rzo : 0.0;  // beta
+        }
+    }
+    if (row) {
+        group_row_spmv2<G>(v, lane, A.erp, A.ecol, A.eval, A.eld, A.z, po, y, w);
+        if (A.crp) group_row_spmv2<G>(v, lane, A.crp, A.ccol, A.cval, A.cld, A.z, po, y, w);
+    }
     __syncthreads();
     if (sflag != 0) return;  // block-uniform
-    const int par = it & 1;
-    const double rz = sum_partials_block_n<PCG_SPMV_BLOCK>(A.part_rz + par * A.nbB, A.nbB, sh);
-    const double rr = sum_partials_block_n<PCG_SPMV_BLOCK>(A.part_rr + par * A.nbB, A.nbB, sh);
-    const double bb = it == 0 ? rr : A.scal[0];
-    if (!(bb > 0.0) || (it > 0 && rr <= A.rtol2 * bb) || it >= A.maxit) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            A.flag[0] = 1;
-            A.scal[0] = bb;
-            A.scal[1] = rr;
-            A.scal[2] = (double)it;
-        }
-        return;
-    }
-    if (it == 0 && blockIdx.x == 0 && threadIdx.x == 0) A.scal[0] = bb;
-    double beta = 0.0;
-    const double* po = nullptr;
-    if (it > 0) {
-        beta = rz / sum_partials_block_n<PCG_SPMV_BLOCK>(A.part_rz + (par ^ 1) * A.nbB, A.nbB, sh);
-        po = A.pbuf + (size_t)(par ^ 1) * 3 * A.nV;
-    }
-    double* pn = A.pbuf + (size_t)par * 3 * A.nV;
-    const int lane = threadIdx.x & (G - 1);
-    const int v = (blockIdx.x * PCG_SPMV_BLOCK + threadIdx.x) / G;
+    const double beta = sc[0];
     double dpq = 0.0;
-    if (v < A.nV) {
-        double y0 = 0.0, y1 = 0.0, y2 = 0.0;
-        group_row_spmv<G>(v, lane, A.erp, A.ecol, A.eval, A.z, po, beta, y0, y1, y2);
-        if (A.crp) group_row_spmv<G>(v, lane, A.crp, A.ccol, A.cval, A.z, po, beta, y0, y1, y2);
+    if (row) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) y[d] += beta * w[d];
 #pragma unroll
         for (int o = G / 2; o > 0; o >>= 1) {
-            y0 += __shfl_xor(y0, o, G);
-            y1 += __shfl_xor(y1, o, G);
-            y2 += __shfl_xor(y2, o, G);
+            y[0] += __shfl_xor(y[0], o, G);
+            y[1] += __shfl_xor(y[1], o, G);
+            y[2] += __shfl_xor(y[2], o, G);
         }
         if (lane < 3) {
-            const double yl = lane == 0 ? y0 : (lane == 1 ? y1 : y2);
+            const double yl = lane == 0 ? y[0] : (lane == 1 ? y[1] : y[2]);
             const size_t i = 3 * (size_t)v + lane;
             const double pv = po ? A.z[i] + beta * po[i] : A.z[i];
-            pn[i] = pv;
+            A.pbuf[(size_t)par * 3 * A.nV + i] = pv;
             A.q[i] = yl;
             dpq = pv * yl;
         }
     }
-    const double a = block_sum<PCG_SPMV_BLOCK>(dpq, sh);
-    if (threadIdx.x == 0) A.part_pq[blockIdx.x] = a;
+    dpq = wave_sum(dpq);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = dpq;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0;
+#pragma unroll
+        for (int w2 = 0; w2 < PCG_SPMV_BLOCK / 64; ++w2) a += wsum[w2];
+        A.part_pq[blockIdx.x] = a;
+    }
 }
 
-__global__ void __launch_bounds__(RB) k_pcg_update(PcgArgs A, int it)
+// one wavefront per block: no barriers
+__global__ void __launch_bounds__(PCG_UPD_BLOCK) k_pcg_update(PcgArgs A, int it)
 {
-    __shared__ double sh[RB];
-    __shared__ int sflag;
-    if (threadIdx.x == 0) sflag = *(volatile int*)A.flag;
-    __syncthreads();
-    if (sflag != 0) return;  // block-uniform
+    const int inst = A.binst[blockIdx.x];
+    if (*(volatile int*)(A.flag + inst) != 0) return;  // one load: uniform over the wavefront
     const int par = it & 1;
-    const double pq = sum_partials_block(A.part_pq, A.nbA, sh);
-    const double rz = sum_partials_block(A.part_rz + par * A.nbB, A.nbB, sh);
+    const double* p = A.pbuf + (size_t)par * 3 * A.nV;
+    const int v = A.blo[blockIdx.x] + threadIdx.x;
+    const bool own = v < A.bhi[blockIdx.x];
+    // this vertex's vectors are loaded before alpha is known (independent loads)
+    double xv[3] = {0, 0, 0}, pv[3] = {0, 0, 0}, rv[3] = {0, 0, 0}, qv[3] = {0, 0, 0}, P[9];
+    if (own) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            xv[i] = A.x[3 * v + i];
+            pv[i] = p[3 * v + i];
+            rv[i] = A.r[3 * v + i];
+            qv[i] = A.q[3 * v + i];
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) P[i] = A.Pinv[9 * (size_t)v + i];
+    }
+    const double pq = wave_sum_partials(A.part_pq + A.apbeg[inst], A.apbeg[inst + 1] - A.apbeg[inst]);
+    const int pb = A.bpbeg[inst];
+    const double rz = wave_sum_partials(A.part_rz + par * A.nbB + pb, A.bpbeg[inst + 1] - pb);
     if (!(pq > 0.0)) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            A.flag[0] = 2;
-            A.scal[2] = (double)it;
+        if (blockIdx.x == pb && threadIdx.x == 0) {
+            A.flag[inst] = 2;
+            A.scal[3 * inst + 2] = (double)it;
         }
         return;
     }
     const double alpha = rz / pq;
-    const double* p = A.pbuf + (size_t)par * 3 * A.nV;
-    const int v = blockIdx.x * RB + threadIdx.x;
     double drz = 0.0, drr = 0.0;
-    if (v < A.nV) {
-        double rv[3];
+    if (own) {
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            A.x[3 * v + i] += alpha * p[3 * v + i];
-            rv[i] = A.r[3 * v + i] - alpha * A.q[3 * v + i];
+            A.x[3 * v + i] = xv[i] + alpha * pv[i];
+            rv[i] = rv[i] - alpha * qv[i];
             A.r[3 * v + i] = rv[i];
         }
-        const double* P = A.Pinv + 9 * (size_t)v;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             const double zi = P[3 * i] * rv[0] + P[3 * i + 1] * rv[1] + P[3 * i + 2] * rv[2];
@@ -420,10 +499,11 @@ __global__ void __launch_bounds__(RB) k_pcg_update(PcgArgs A, int it)
             drr += rv[i] * rv[i];
         }
     }
-    const double a = block_sum<RB>(drz, sh), c = block_sum<RB>(drr, sh);
+    drz = wave_sum(drz);
+    drr = wave_sum(drr);
     if (threadIdx.x == 0) {
-        A.part_rz[(par ^ 1) * A.nbB + blockIdx.x] = a;
-        A.part_rr[(par ^ 1) * A.nbB + blockIdx.x] = c;
+        A.part_rz[(par ^ 1) * A.nbB + blockIdx.x] = drz;
+        A.part_rr[(par ^ 1) * A.nbB + blockIdx.x] = drr;
     }
 }
 
@@ -571,26 +651,79 @@ void assemble(LinWork& L, hipStream_t s, const MeshDev& m, const double* Hl, con
         L.prof->end(IDP_K_ASSEMBLY, tok, 80.0 * ncontrib + 76.0 * ((double)m.nnzb_e + L.ncb) + 4.0 * (m.nV + 1));
 }
 
+void build_inst_layout(MeshDev& m, const std::vector<long>& vbeg, const std::vector<long>& tbeg,
+                       const std::vector<long>& nbeg, const std::vector<long>& ebeg, const std::vector<long>& fbeg,
+                       hipStream_t s)
+{
+    InstDev& I = m.inst;
+    I.B = (int)vbeg.size() - 1;
+    I.vbeg = vbeg;
+    I.tbeg = tbeg;
+    I.nbeg = nbeg;
+    I.ebeg = ebeg;
+    I.fbeg = fbeg;
+    std::vector<int> vi(std::max<long>(vbeg.back(), 1), 0);
+    for (int i = 0; i < I.B; ++i)
+        for (long v = vbeg[i]; v < vbeg[i + 1]; ++v) vi[v] = i;
+    I.vinst.upload(vi.data(), vi.size(), s);
+    auto up = [&](DBuf<int>& d, const std::vector<long>& h) {
+        std::vector<int> t(h.begin(), h.end());
+        d.upload(t.data(), t.size(), s);
+    };
+    up(I.d_vbeg, vbeg);
+    up(I.d_nbeg, nbeg);
+    up(I.d_ebeg, ebeg);
+    up(I.d_fbeg, fbeg);
+    std::vector<long> p3(nbeg.size());
+    for (size_t i = 0; i < nbeg.size(); ++i) p3[i] = 3 * nbeg[i];
+    I.echunk.build(ebeg, 256, s);
+    I.pchunk.build(p3, 256, s);
+    I.vblk.build(vbeg, RB, s);
+    I.nblk.build(nbeg, RB, s);
+    I.tblk.build(tbeg, RB, s);
+    I.pcg_rows.build(vbeg, 32, s);
+    I.pcg_upd.build(vbeg, 64, s);
+    std::vector<long> v3(vbeg.size());
+    for (size_t i = 0; i < vbeg.size(); ++i) v3[i] = 3 * vbeg[i];
+    I.dblk.build(v3, RB, s);
+    IDP_HIP(hipStreamSynchronize(s));
+}
+
 void init_erow(LinWork& L, hipStream_t s, const MeshDev& m, const std::vector<int>& rowof)
 {
     L.erow.upload(rowof.data(), rowof.size(), s);
 }
 
-int pcg_solve_persistent(LinWork& L, hipStream_t s, const MeshDev& m, const double* b, double* x, double rtol,
-                         int maxit, double* relres)
+__global__ void k_pcg_start(int B, const unsigned char* __restrict__ solve, int* __restrict__ flag,
+                            int* __restrict__ err)
 {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B) return;
+    flag[i] = (solve && !solve[i]) ? 3 : 0;
+    err[i] = 0;
+}
+
+// Block-Jacobi PCG per instance on the assembled system (E + C); b, x: 3V.
+// solve (device, B bytes; null = all) selects the instances to solve.
+// Per instance on return: iters[i] >= 0, or -1 (singular diagonal block) /
+// -2 (breakdown, p.q <= 0); relres[i] = |r|/|b|.
+void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* b, double* x, double rtol,
+                     int maxit, const unsigned char* solve, int* iters, double* relres)
+{
+    const InstDev& I = m.inst;
+    const int B = I.B;
     const int nV = m.nV;
     const double nnzb = (double)m.nnzb_e + (double)L.ncb;
-    // lanes per block row from the mean row length (a tet-mesh row holds ~15 blocks)
-    const double avg = nnzb / std::max(nV, 1);
-    const int G = avg <= 8.0 ? 4 : (avg <= 24.0 ? 8 : 16);
+    constexpr int G = 8;  // lanes per block row (32-row SpMV blocks)
     double* Pinv = L.pinv.ensure(9 * (size_t)nV, s);
-    int* err = L.err.ensure(4, s);
-    IDP_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
+    int* err = L.err.ensure(2 * (size_t)B + 2, s);
+    int* flag = err + B;
+    IDP_LAUNCH(k_pcg_start, grid_for(B, 64), 64, 0, s, B, solve, flag, err);
     const bool haveC = L.ncb > 0;
-    const int tok = L.prof ? L.prof->begin(IDP_K_PCG) : -1;
-    IDP_LAUNCH(k_block_jacobi, grid_for(nV, 256), 256, 0, s, nV, m.e_diag.p, L.evals.p,
-               haveC ? L.cdiag.p : (const int*)nullptr, L.cvals.p, Pinv, err);
+    Profiler* P = L.prof;
+    const int tok = P ? P->begin(IDP_K_PCG) : -1;
+    IDP_LAUNCH(k_block_jacobi, grid_for(nV, 256), 256, 0, s, nV, m.e_diag.p, L.evals.p, m.nnzb_e,
+               haveC ? L.cdiag.p : (const int*)nullptr, L.cvals.p, L.ncb, I.vinst.p, Pinv, err);
     PcgArgs A;
     A.nV = nV;
     A.erp = m.e_rowptr.p;
@@ -599,6 +732,8 @@ int pcg_solve_persistent(LinWork& L, hipStream_t s, const MeshDev& m, const doub
     A.crp = haveC ? L.crp.p : nullptr;
     A.ccol = L.ccol.p;
     A.cval = L.cvals.p;
+    A.eld = m.nnzb_e;
+    A.cld = L.ncb;
     A.b = b;
     A.Pinv = Pinv;
     A.x = x;
@@ -606,26 +741,31 @@ int pcg_solve_persistent(LinWork& L, hipStream_t s, const MeshDev& m, const doub
     A.z = L.z.ensure(3 * (size_t)nV, s);
     A.pbuf = L.p.ensure(6 * (size_t)nV, s);
     A.q = L.q.ensure(3 * (size_t)nV, s);
-    A.nbA = std::max(1, grid_for((long)nV * G, PCG_SPMV_BLOCK));
-    A.nbB = std::max(1, grid_for(nV, RB));
-    A.part_pq = L.part0.ensure(A.nbA, s);
-    A.part_rz = L.part1.ensure(2 * (size_t)A.nbB, s);
-    A.part_rr = L.part2.ensure(2 * (size_t)A.nbB, s);
-    A.flag = err + 1;
-    A.scal = L.scal.ensure(8, s);
+    A.nbA = I.pcg_rows.nblk;
+    A.nbB = I.pcg_upd.nblk;
+    A.alo = I.pcg_rows.lo.p;
+    A.ahi = I.pcg_rows.hi.p;
+    A.ainst = I.pcg_rows.inst.p;
+    A.apbeg = I.pcg_rows.pbeg.p;
+    A.blo = I.pcg_upd.lo.p;
+    A.bhi = I.pcg_upd.hi.p;
+    A.binst = I.pcg_upd.inst.p;
+    A.bpbeg = I.pcg_upd.pbeg.p;
+    A.part_pq = L.part0.ensure(A.nbA + 1, s);
+    A.part_rz = L.part1.ensure(2 * (size_t)A.nbB + 2, s);
+    A.part_rr = L.part2.ensure(2 * (size_t)A.nbB + 2, s);
+    A.flag = flag;
+    A.scal = L.scal.ensure(3 * (size_t)B + 3, s);
     A.rtol2 = rtol * rtol;
     A.maxit = maxit;
-    IDP_LAUNCH(k_pcg_init, A.nbB, RB, 0, s, A);
-    struct {
-        int err, flag;
-    } hf{0, 0};
-    double hs[3] = {0, 0, 0};
+    IDP_LAUNCH(k_pcg_init, A.nbB, PCG_UPD_BLOCK, 0, s, A);
     // algorithmic bytes of one SpMV launch (blocks + columns, two row-pointer
     // arrays, z and p_old read, p_new and q written) and of one update launch
     // (x, p, r, q, P^-1 read; x, r, z written)
     const double spmv_bytes = 76.0 * nnzb + 8.0 * (nV + 1) + 96.0 * nV;
     const double upd_bytes = 240.0 * nV;
-    Profiler* P = L.prof;
+    std::vector<int> hf(2 * B);
+    std::vector<double> hs(3 * B);
     int it = 0, chunk = 8;
     for (;;) {
         for (int j = 0; j < chunk && it <= maxit; ++j, ++it) {
@@ -633,38 +773,57 @@ int pcg_solve_persistent(LinWork& L, hipStream_t s, const MeshDev& m, const doub
             // every launch would itself widen the launch gaps)
             const bool sample = P && (it & 7) == 3;
             int t = sample ? P->begin(IDP_K_PCG_SPMV) : -1;
-            if (G == 4) IDP_LAUNCH(k_pcg_spmv<4>, A.nbA, PCG_SPMV_BLOCK, 0, s, A, it);
-            else if (G == 8) IDP_LAUNCH(k_pcg_spmv<8>, A.nbA, PCG_SPMV_BLOCK, 0, s, A, it);
-            else IDP_LAUNCH(k_pcg_spmv<16>, A.nbA, PCG_SPMV_BLOCK, 0, s, A, it);
+            IDP_LAUNCH(k_pcg_spmv<G>, A.nbA, PCG_SPMV_BLOCK, 0, s, A, it);
             if (sample) P->end(IDP_K_PCG_SPMV, t, spmv_bytes, it);
             t = sample ? P->begin(IDP_K_PCG_UPDATE) : -1;
-            IDP_LAUNCH(k_pcg_update, A.nbB, RB, 0, s, A, it);
+            IDP_LAUNCH(k_pcg_update, A.nbB, PCG_UPD_BLOCK, 0, s, A, it);
             if (sample) P->end(IDP_K_PCG_UPDATE, t, upd_bytes, it);
         }
-        IDP_HIP(hipMemcpyAsync(&hf, err, sizeof(hf), hipMemcpyDeviceToHost, s));
-        IDP_HIP(hipMemcpyAsync(hs, A.scal, sizeof(hs), hipMemcpyDeviceToHost, s));
+        IDP_HIP(hipMemcpyAsync(hf.data(), err, 2 * B * sizeof(int), hipMemcpyDeviceToHost, s));
         IDP_HIP(hipStreamSynchronize(s));
-        if (hf.err || hf.flag || it > maxit) break;
+        bool all = true;
+        for (int i = 0; i < B; ++i) all &= hf[B + i] != 0;
+        if (all || it > maxit) break;
         chunk = std::min(chunk * 2, 256);
     }
     if (P) P->end(IDP_K_PCG, tok);
-    if (hf.err) return -1;
-    if (hf.flag == 0) throw HipError(IDP_E_HIP, "PCG did not stop");
-    const int iters = (int)hs[2];
-    if (P) {
-        // launches at or after the stopping iteration did no SpMV/update work
-        P->discard_from(IDP_K_PCG_SPMV, iters);
-        P->discard_from(IDP_K_PCG_UPDATE, iters);
-        P->checkpoint();
+    IDP_HIP(hipMemcpyAsync(hs.data(), A.scal, 3 * B * sizeof(double), hipMemcpyDeviceToHost, s));
+    IDP_HIP(hipStreamSynchronize(s));
+    int maxdone = 0;
+    double done_iters = 0;
+    for (int i = 0; i < B; ++i) {
+        const int f = hf[B + i];
+        if (f == 0) throw HipError(IDP_E_HIP, "PCG did not stop");
+        iters[i] = 0;
+        relres[i] = 0;
+        if (f == 3) continue;
+        const int k = (int)hs[3 * i + 2];
+        maxdone = std::max(maxdone, k);
+        if (hf[i]) iters[i] = -1;
+        else if (f == 2) iters[i] = -2;
+        else {
+            iters[i] = k;
+            relres[i] = hs[3 * i] > 0 ? std::sqrt(hs[3 * i + 1] / hs[3 * i]) : 0.0;
+            done_iters += k;
+        }
     }
-    if (hf.flag == 2) return -2;
-    *relres = hs[0] > 0 ? std::sqrt(hs[1] / hs[0]) : 0.0;
-    // algorithmic bytes per PCG iteration (each array touched once):
-    //   SpMV: blocks + columns 76 nnzb, two row pointers 8 (V+1),
-    //         read z, p_old, write p_new, q: 96 V
-    //   update: read x, p_new, r, q, P^-1 (96 V + 72 V), write x, r, z (72 V)
-    if (P) P->add_work(IDP_K_PCG, iters * (spmv_bytes + upd_bytes));
-    return iters;
+    if (P) {
+        // launches at or after the last stopping iteration did no work
+        P->discard_from(IDP_K_PCG_SPMV, maxdone);
+        P->discard_from(IDP_K_PCG_UPDATE, maxdone);
+        P->checkpoint();
+        P->add_work(IDP_K_PCG, B == 1 ? done_iters * (spmv_bytes + upd_bytes) : 0.0);
+    }
+}
+
+int pcg_solve_persistent(LinWork& L, hipStream_t s, const MeshDev& m, const double* b, double* x, double rtol,
+                         int maxit, double* relres)
+{
+    int it = 0;
+    double rel = 0;
+    pcg_solve_batch(L, s, m, b, x, rtol, maxit, nullptr, &it, &rel);
+    *relres = rel;
+    return it;
 }
 
 double dev_dot(LinWork& L, hipStream_t s, long n, const double* a, const double* b)

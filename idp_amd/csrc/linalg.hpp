@@ -27,6 +27,8 @@ void contact_gradient_gather(LinWork& L, hipStream_t s, const MeshDev& m, const 
                              const double* gl, const double* gb, double* g, double* rhs);
 void assemble(LinWork& L, hipStream_t s, const MeshDev& m, const double* Hl, const double* Hb, bool any_dbc);
 void init_erow(LinWork& L, hipStream_t s, const MeshDev& m, const std::vector<int>& rowof);
+void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* b, double* x, double rtol,
+                     int maxit, const unsigned char* solve, int* iters, double* relres);
 int pcg_solve_persistent(LinWork& L, hipStream_t s, const MeshDev& m, const double* b, double* x, double rtol,
                          int maxit, double* relres);
 double dev_dot(LinWork& L, hipStream_t s, long n, const double* a, const double* b);

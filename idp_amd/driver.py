@@ -41,3 +41,42 @@ class TetAnimFix:
 
     def state(self):
         return self.ctx.get_state()
+
+
+class BatchAnimFix:
+    """Several independent animation-fix scenes advanced together in one
+    context: every stage runs over all instances at once, while each instance
+    keeps its own Newton, line-search and kappa sequence."""
+
+    def __init__(self, scenes_, device=0, pcg_rtol=1e-8, kappa_mult=1.0):
+        self.scenes = list(scenes_)
+        meshes, dh2, kap, vel = [], [], [], []
+        for sc in self.scenes:
+            X = np.ascontiguousarray(sc.X, dtype=np.float64)
+            tets = np.ascontiguousarray(sc.tets, dtype=np.int32)
+            vol, IB, lam, mu, mass = capi.fem_init(X, tets, sc.density, sc.E, sc.nu)
+            meshes.append((X, tets, IB, vol, lam, mu, mass))
+            d2, k = capi.initialize_oipc(sc.dhat, mass, kappa_mult)
+            dh2.append(d2)
+            kap.append(k)
+            vel.append(sc.velocity)
+        self.ctx = capi.Context(device)
+        self.ctx.set_mesh_batch(meshes)
+        self.dHat2 = np.array(dh2)
+        self.kappa = np.ascontiguousarray(np.stack(kap))
+        self.h = self.scenes[0].h
+        if any(sc.h != self.h for sc in self.scenes):
+            raise ValueError("one time step for the whole batch")
+        self.tol = np.array([sc.newton_tol for sc in self.scenes])
+        self.pcg_rtol = pcg_rtol
+        self.offsets = np.concatenate([[0], np.cumsum([sc.X.shape[0] for sc in self.scenes])])
+        self.ctx.set_state(np.concatenate([sc.X for sc in self.scenes]), np.concatenate(vel))
+
+    def advance_one_time_step(self, max_newton=0):
+        return self.ctx.advance_batch(self.h, self.tol, self.dHat2, self.kappa, 0.0, True, self.pcg_rtol,
+                                      max_newton=max_newton)
+
+    def state(self, i):
+        X, v = self.ctx.get_state()
+        a, b = self.offsets[i], self.offsets[i + 1]
+        return X[a:b], v[a:b]

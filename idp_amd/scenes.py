@@ -164,6 +164,14 @@ def config2():
     return bar_twist_fold()
 
 
+def config3(count=256, seed=3):
+    """count cubes of n^3 cells (n uniform in 4..16, seeded), jitter 0.03*spacing,
+    targets from 4 seeded sinusoids (amplitude 0.3-0.6, wavelength 0.5-1.0 of the side)."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(4, 17, size=count)
+    return [cube_sinusoid(int(n), seed=int(seed * 100003 + i)) for i, n in enumerate(ns)]
+
+
 def config5():
     """160x48x48 cells, jitter 0.02*spacing, crumple of 12 sinusoidal modes."""
     X, tets = kuhn_block(160, 48, 48, 0.05)

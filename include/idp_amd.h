@@ -79,6 +79,8 @@ typedef struct idp_profile {
     double work[IDP_K_COUNT];
     long barrier_pairs[8];   /* Hessian evaluations per constraint kind */
     long barrier_clamped[8]; /* ... of which needed the PSD clamp */
+    double ccd_steps;        /* additive-CCD iterations over all pairs */
+    double ccd_max_steps;    /* iterations of the longest pair */
 } idp_profile;
 
 /* ---- context ---------------------------------------------------------- */
@@ -166,6 +168,42 @@ int idp_newton_direction(idp_ctx* ctx, const double* X, const double* xtilde, do
                          double* relres, int* iters);
 /* BSR of the last assembled system: sizes, then arrays (rowptr nV+1, col nnzb, val 9*nnzb) */
 int idp_get_bsr(idp_ctx* ctx, long* nnzb, int* rowptr, int* col, double* val, long cap);
+
+/* ---- batched independent solves ---------------------------------------- */
+/* No counterpart in the reference, which advances one mesh per call
+ * (Advance_One_Step_IE_Discrete_Shell, IMPLICIT_EULER.h:152-891).  B meshes
+ * ("instances") are held in one context: arrays are concatenated over
+ * instances (nV[i] vertices, nT[i] tets each; tets in instance-local vertex
+ * ids).  Every stage runs over all instances at once, and each instance
+ * follows its own Newton / line-search / kappa sequence, with every reduction
+ * taken over the instance alone, so an instance's result is the same whether
+ * it is solved alone or with others.  idp_set_state / idp_get_state /
+ * idp_set_dbc address the concatenated vertex arrays. */
+int idp_set_mesh_batch(idp_ctx* ctx, int B, const int* nV, const int* nT, const double* X_rest, const int* tets,
+                       const double* IB, const double* vol, const double* lambda, const double* mu,
+                       const double* mass);
+int idp_batch_size(idp_ctx* ctx, int* B);
+/* One implicit-Euler step of every instance: newton_tol[B], dHat2[B],
+ * kappa3[3 B] (in/out), newton_iters[B] (out), stats[B] (out, may be null). */
+int idp_advance_batch(idp_ctx* ctx, double h, const double* newton_tol, int with_collision, const double* dHat2,
+                      double* kappa3, double thickness, const idp_solver_opts* opts, int* newton_iters,
+                      idp_step_stats* stats);
+
+/* ---- continuous batching (serving) -------------------------------------- */
+/* No counterpart in the reference.  The instances of a batch mesh are slots;
+ * a request is started in a free slot while other slots are mid-solve, and
+ * every idp_serve_iterate call advances all busy slots by one Newton
+ * iteration.  configure: time step and solver options for all slots. */
+int idp_serve_configure(idp_ctx* ctx, double h, int with_collision, double thickness, const idp_solver_opts* opts);
+/* start n requests in the given free slots: X and v concatenated over the
+ * listed slots (3 nV[slot] each), newton_tol[n], dHat2[n], kappa3[3 n] */
+int idp_serve_start(idp_ctx* ctx, int n, const int* slots, const double* X, const double* v,
+                    const double* newton_tol, const double* dHat2, const double* kappa3);
+/* one Newton iteration of every busy slot; finished[B] (may be null) flags
+ * the slots whose solve ended in this call; *n_active = busy slots left */
+int idp_serve_iterate(idp_ctx* ctx, int* n_active, int* finished);
+/* result of a slot (positions, velocities, kappa3, stats; any may be null) */
+int idp_serve_result(idp_ctx* ctx, int slot, double* X, double* v, double* kappa3, idp_step_stats* stats);
 
 #ifdef __cplusplus
 }
