@@ -129,7 +129,7 @@ def bench_batch(a):
         b.ctx.restore()
         b.kappa[:] = k0
         t0 = time.perf_counter()
-        its, st = b.advance_one_time_step(max_newton=max_newton)
+        its, st = b.advance_one_time_step(max_newton=max_newton, verbose=10)
         return its, st, time.perf_counter() - t0
 
     solve(2)  # buffer priming
@@ -154,8 +154,9 @@ def bench_batch(a):
         d.ctx.restore()
         d.kappa[:] = k0[i]
         t0 = time.perf_counter()
-        d.advance_one_time_step(max_newton=a.max_newton)
+        it, _ = d.advance_one_time_step(max_newton=a.max_newton)
         tp += time.perf_counter() - t0
+        log(f"[bench] plain instance {i}: {it} Newton iterations, {tp:.2f} s so far")
     plain = len(idx) / tp
     log(f"[bench] plain path: {len(idx)} instances in {tp:.2f} s")
     lat_ms = np.array(lat) * 1e3
@@ -361,7 +362,7 @@ def main():
         d.kappa[:] = k0
         t0 = time.perf_counter()
         # returns after the context's stream has drained (stats and kappa are copied back)
-        it, st = d.advance_one_time_step(max_newton=max_newton)
+        it, st = d.advance_one_time_step(max_newton=max_newton, verbose=50)
         return it, st, time.perf_counter() - t0
 
     # buffer priming (device allocations grow on first use): a 2-iteration solve

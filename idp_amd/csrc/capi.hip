@@ -436,6 +436,7 @@ struct BatchEngine {
         rtol = (opts && opts->pcg_rtol > 0) ? opts->pcg_rtol : 1e-8;
         maxit = (opts && opts->pcg_maxit > 0) ? opts->pcg_maxit : 10000;
         max_newton = opts ? opts->max_newton : 0;
+        c->L.precond = opts ? opts->precond : 0;
         kappa.assign(B, 0);
         kmax.assign(B, 0);
         tol.assign(B, 0);
@@ -1180,6 +1181,7 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
         const int maxit = (opts && opts->pcg_maxit > 0) ? opts->pcg_maxit : 10000;
         const int max_newton = opts ? opts->max_newton : 0;
         const int verbose = opts ? opts->verbose : 0;
+        c->L.precond = opts ? opts->precond : 0;
         auto s = c->s;
         MeshDev& m = c->m;
         const long n3 = 3L * m.nV;
@@ -1577,6 +1579,7 @@ int idp_newton_direction(idp_ctx* c, const double* X, const double* xtilde, doub
         hessian(c, xd, h, true);
         double rtol = (opts && opts->pcg_rtol > 0) ? opts->pcg_rtol : 1e-8;
         int maxit = (opts && opts->pcg_maxit > 0) ? opts->pcg_maxit : 10000;
+        c->L.precond = opts ? opts->precond : 0;
         double rel = 0;
         int it = pcg_solve_persistent(c->L, s, m, rhs, sd, rtol, maxit, &rel);
         if (g) c->g.download(g, n3, s);
@@ -1701,9 +1704,16 @@ int idp_advance_batch(idp_ctx* c, double h, const double* newton_tol, int with_c
         for (int i = 0; i < B; ++i) all[i] = i;
         rc = E.start(c, all, newton_tol, dHat2, kappa3);
         if (rc) return rc;
-        while (E.n_active > 0) {
+        const int verbose = opts ? opts->verbose : 0;
+        const double t0 = now_ms();
+        for (int k = 1; E.n_active > 0; ++k) {
             rc = E.iterate(c);
             if (rc) return rc;
+            if (verbose > 0 && k % verbose == 0) {
+                std::fprintf(stderr, "[idp] batch Newton iteration %d: %d of %d instances active, %.2f s\n", k,
+                             E.n_active, B, (now_ms() - t0) * 1e-3);
+                std::fflush(stderr);
+            }
         }
         for (int i = 0; i < B; ++i) {
             newton_iters[i] = E.PN[i];

@@ -37,8 +37,14 @@ __global__ void k_grid_setup(int B, GridDev* __restrict__ grids, const double* _
                              const unsigned char* __restrict__ active)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= B || (active && !active[i])) return;
+    if (i >= B) return;
     GridDev* g = grids + i;
+    if (active && !active[i]) {  // an instance left out: a one-voxel grid (no records, no key space)
+        g->lbc[0] = g->lbc[1] = g->lbc[2] = 0.0;
+        g->inv = g->voxel = 1.0;
+        g->cnt[0] = g->cnt[1] = g->cnt[2] = 1;
+        return;
+    }
     double tot = 0.0;
     for (int c = mean_pbeg[i]; c < mean_pbeg[i + 1]; ++c) tot += mean_parts[c];
     const long n_mean = ebeg[i + 1] - ebeg[i];

@@ -236,6 +236,270 @@ __global__ void k_block_jacobi(int nV, const int* __restrict__ ediag, const doub
 }
 
 // ---------------------------------------------------------------------------
+// Contact-cluster block-Jacobi preconditioner.  Stiff barrier terms couple the
+// vertices of a contact pair far more strongly than mass or elasticity do;
+// a 3x3 block-Jacobi leaves those couplings to the iteration.  Vertices are
+// grouped by two rounds of mutual heavy-edge matching on the contact blocks
+// (clusters of at most 4 vertices, never across instances), and each
+// cluster's (3k)^2 diagonal block of E + C is inverted exactly.  Vertex v
+// stores the 3 x 3k rows of its cluster inverse: z_v = Z_v [r_m0; ...; r_mk-1].
+// ---------------------------------------------------------------------------
+constexpr int CL_MAX = 4;
+
+__device__ inline double cblock_norm2(const double* __restrict__ val, long ld, int k)
+{
+    double w = 0.0;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) w += val[e * ld + k] * val[e * ld + k];
+    return w;
+}
+
+// strongest contact neighbour of every vertex (ties: smaller id); -1 if none
+__global__ void k_cl_best(int nV, const int* __restrict__ crp, const int* __restrict__ ccol,
+                          const double* __restrict__ cval, long cld, int* __restrict__ best)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nV) return;
+    int b = -1;
+    double bw = 0.0;
+    if (crp)
+        for (int k = crp[v]; k < crp[v + 1]; ++k) {
+            const int u = ccol[k];
+            if (u == v) continue;
+            const double w = cblock_norm2(cval, cld, k);
+            if (w > bw || (w == bw && b >= 0 && u < b)) {
+                bw = w;
+                b = u;
+            }
+        }
+    best[v] = b;
+}
+
+__global__ void k_cl_pair(int nV, const int* __restrict__ best, int* __restrict__ root, int* __restrict__ size)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nV) return;
+    const int b = best[v];
+    const bool pair = b >= 0 && best[b] == v;
+    root[v] = pair ? min(v, b) : v;
+    size[v] = pair ? 2 : 1;  // meaningful at roots
+}
+
+// round 2: strongest neighbouring cluster of every cluster (max block norm
+// over member-to-member contact blocks), packed (float weight, -root) so that
+// a 64-bit max picks the heaviest, then the lowest root
+__global__ void k_cl_best2(int nV, const int* __restrict__ crp, const int* __restrict__ ccol,
+                           const double* __restrict__ cval, long cld, const int* __restrict__ root,
+                           const int* __restrict__ size, unsigned long long* __restrict__ pack)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nV || !crp) return;
+    const int r = root[v];
+    unsigned long long bestp = 0;
+    for (int k = crp[v]; k < crp[v + 1]; ++k) {
+        const int ru = root[ccol[k]];
+        if (ru == r || size[r] + size[ru] > CL_MAX) continue;
+        const float wf = (float)cblock_norm2(cval, cld, k);
+        const unsigned long long p =
+            ((unsigned long long)__float_as_uint(wf) << 32) | (unsigned long long)(0xFFFFFFFFu - (unsigned)ru);
+        if (p > bestp) bestp = p;
+    }
+    if (bestp) atomicMax(pack + r, bestp);
+}
+
+__global__ void k_cl_merge(int nV, int* __restrict__ root, const unsigned long long* __restrict__ pack,
+                           int* __restrict__ size, int* __restrict__ root2)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nV) return;
+    const int r = root[v];
+    int nr = r;
+    const unsigned long long p = pack[r];
+    if (p) {
+        const int b = (int)(0xFFFFFFFFu - (unsigned)(p & 0xFFFFFFFFull));
+        const unsigned long long q = pack[b];
+        if (q && (int)(0xFFFFFFFFu - (unsigned)(q & 0xFFFFFFFFull)) == r) nr = min(r, b);
+    }
+    root2[v] = nr;
+}
+
+__global__ void k_cl_members(int nV, const int* __restrict__ root, int* __restrict__ cnt, int* __restrict__ mem)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nV) return;
+    const int r = root[v];
+    const int slot = atomicAdd(cnt + r, 1);
+    if (slot < CL_MAX) mem[CL_MAX * r + slot] = v;
+}
+
+// members of each cluster in ascending order; per vertex: cluster size and
+// the size of its Z rows (9 k doubles)
+__global__ void k_cl_sort(int nV, const int* __restrict__ root, const int* __restrict__ cnt, int* __restrict__ mem,
+                          int* __restrict__ zlen)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nV) return;
+    const int r = root[v];
+    const int k = min(cnt[r], CL_MAX);
+    if (v == r) {
+        int* m = mem + CL_MAX * r;
+        for (int i = 1; i < k; ++i)
+            for (int j = i; j > 0 && m[j - 1] > m[j]; --j) {
+                const int t = m[j - 1];
+                m[j - 1] = m[j];
+                m[j] = t;
+            }
+    }
+    zlen[v] = 9 * k;
+}
+
+// locate block (a, b) in a BSR row; returns -1 when absent
+__device__ inline int bsr_find(const int* __restrict__ rp, const int* __restrict__ col, int a, int b)
+{
+    for (int k = rp[a]; k < rp[a + 1]; ++k)
+        if (col[k] == b) return k;
+    return -1;
+}
+
+// one thread per cluster root: gather the (3k)^2 block of E + C, Cholesky,
+// inverse, and scatter the member rows to Z.  Singletons keep the 3x3 inverse
+// of the block-Jacobi preconditioner (same arithmetic).
+__global__ void k_cl_build(int nV, const int* __restrict__ root, const int* __restrict__ cnt,
+                           const int* __restrict__ mem, const int* __restrict__ ediag,
+                           const int* __restrict__ erp, const int* __restrict__ ecol, const double* __restrict__ evals,
+                           long eld, const int* __restrict__ cdiag, const int* __restrict__ crp,
+                           const int* __restrict__ ccol, const double* __restrict__ cvals, long cld,
+                           const int* __restrict__ vinst, const int* __restrict__ zoff, double* __restrict__ Z,
+                           double* __restrict__ work, int* __restrict__ err)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nV || root[r] != r) return;
+    const int k = min(cnt[r], CL_MAX);
+    const int* m = mem + CL_MAX * r;
+    if (k == 1) {
+        double a[9];
+        const long ed = ediag[r];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) a[i] = evals[i * eld + ed];
+        if (cdiag) {
+            const int c = cdiag[r];
+            if (c >= 0)
+#pragma unroll
+                for (int i = 0; i < 9; ++i) a[i] += cvals[i * cld + c];
+        }
+        double c0 = a[4] * a[8] - a[5] * a[7], c1 = a[5] * a[6] - a[3] * a[8], c2 = a[3] * a[7] - a[4] * a[6];
+        double det = a[0] * c0 + a[1] * c1 + a[2] * c2;
+        double* o = Z + zoff[r];
+        if (!(det > 0)) {
+            atomicOr(err + vinst[r], 1);
+            det = 1.0;
+        }
+        double id = 1.0 / det;
+        o[0] = c0 * id;
+        o[3] = c1 * id;
+        o[6] = c2 * id;
+        o[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+        o[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+        o[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+        o[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+        o[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+        o[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+        return;
+    }
+    const int n = 3 * k;
+    double* A = work + (size_t)2 * 144 * r;  // n x n, row-major
+    double* Ainv = A + 144;
+    for (int a = 0; a < k; ++a)
+        for (int b = 0; b < k; ++b) {
+            double blk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            const int ke = bsr_find(erp, ecol, m[a], m[b]);
+            if (ke >= 0)
+                for (int i = 0; i < 9; ++i) blk[i] += evals[i * eld + ke];
+            if (crp) {
+                const int kc = bsr_find(crp, ccol, m[a], m[b]);
+                if (kc >= 0)
+                    for (int i = 0; i < 9; ++i) blk[i] += cvals[i * cld + kc];
+            }
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) A[(3 * a + i) * n + 3 * b + j] = blk[3 * i + j];
+        }
+    // symmetrise (the assembled blocks are symmetric up to rounding) and factor A = L L^T in place
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < i; ++j) {
+            const double sym = 0.5 * (A[i * n + j] + A[j * n + i]);
+            A[i * n + j] = sym;
+            A[j * n + i] = sym;
+        }
+    bool ok = true;
+    for (int j = 0; j < n && ok; ++j) {
+        double d = A[j * n + j];
+        for (int q = 0; q < j; ++q) d -= A[j * n + q] * A[j * n + q];
+        if (!(d > 0)) {
+            ok = false;
+            break;
+        }
+        const double ljj = sqrt(d);
+        A[j * n + j] = ljj;
+        for (int i = j + 1; i < n; ++i) {
+            double t = A[i * n + j];
+            for (int q = 0; q < j; ++q) t -= A[i * n + q] * A[j * n + q];
+            A[i * n + j] = t / ljj;
+        }
+    }
+    if (!ok) {
+        atomicOr(err + vinst[r], 1);
+        for (int i = 0; i < n * n; ++i) Ainv[i] = (i % (n + 1)) == 0 ? 1.0 : 0.0;
+    } else {
+        // inverse column by column: solve L y = e_c, L^T x = y
+        for (int c = 0; c < n; ++c) {
+            double y[12];
+            for (int i = 0; i < n; ++i) {
+                double t = (i == c) ? 1.0 : 0.0;
+                for (int q = 0; q < i; ++q) t -= A[i * n + q] * y[q];
+                y[i] = t / A[i * n + i];
+            }
+            for (int i = n - 1; i >= 0; --i) {
+                double t = y[i];
+                for (int q = i + 1; q < n; ++q) t -= A[q * n + i] * Ainv[q * n + c];
+                Ainv[i * n + c] = t / A[i * n + i];
+            }
+        }
+    }
+    for (int a = 0; a < k; ++a) {
+        double* o = Z + zoff[m[a]];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < n; ++j) o[i * n + j] = Ainv[(3 * a + i) * n + j];
+    }
+}
+
+// z_v = Z_v r_members (members' r given by rfun)
+template <class RF>
+__device__ inline void cl_apply(int v, const int* __restrict__ root, const int* __restrict__ cnt,
+                                const int* __restrict__ mem, const int* __restrict__ zoff,
+                                const double* __restrict__ Z, RF&& rfun, double* zv)
+{
+    const int r = root[v];
+    const int k = min(cnt[r], CL_MAX);
+    const double* o = Z + zoff[v];
+    const int n = 3 * k;
+    zv[0] = zv[1] = zv[2] = 0.0;
+    if (k == 1) {
+        double rv[3];
+        rfun(v, rv);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) zv[i] = o[3 * i] * rv[0] + o[3 * i + 1] * rv[1] + o[3 * i + 2] * rv[2];
+        return;
+    }
+    for (int b = 0; b < k; ++b) {
+        double rb[3];
+        rfun(mem[CL_MAX * r + b], rb);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            zv[i] += o[i * n + 3 * b] * rb[0] + o[i * n + 3 * b + 1] * rb[1] + o[i * n + 3 * b + 2] * rb[2];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Block-Jacobi PCG as two kernels per iteration, enqueued in chunks with no
 // host synchronisation inside a chunk.  A grid-wide barrier inside one
 // persistent kernel costs a device-scope release/acquire per block (the
@@ -261,9 +525,10 @@ struct PcgArgs {
     const double* cval;  // 9 component planes of cld blocks
     long eld, cld;
     const double* b;
-    const double* Pinv;
+    const double* Z;      // preconditioner rows (per vertex 3 x 3k, at zoff[v])
+    const int *croot, *ccnt, *cmem, *zoff;  // vertex -> cluster root, sizes, members
     double* x;
-    double* r;
+    double* r;        // 2 x 3V: the update of iteration it reads half it&1, writes the other
     double* z;
     double* pbuf;     // 2 x 3V
     double* q;
@@ -352,14 +617,20 @@ __global__ void __launch_bounds__(PCG_UPD_BLOCK) k_pcg_init(PcgArgs A)
     double drz = 0.0, drr = 0.0;
     if (v < A.bhi[blockIdx.x]) {
         const double rv[3] = {A.b[3 * v], A.b[3 * v + 1], A.b[3 * v + 2]};
-        const double* P = A.Pinv + 9 * (size_t)v;
+        double zv[3];
+        cl_apply(v, A.croot, A.ccnt, A.cmem, A.zoff, A.Z,
+                 [&](int u, double* o) {
+                     o[0] = A.b[3 * u];
+                     o[1] = A.b[3 * u + 1];
+                     o[2] = A.b[3 * u + 2];
+                 },
+                 zv);
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             A.x[3 * v + i] = 0.0;
             A.r[3 * v + i] = rv[i];
-            const double zi = P[3 * i] * rv[0] + P[3 * i + 1] * rv[1] + P[3 * i + 2] * rv[2];
-            A.z[3 * v + i] = zi;
-            drz += rv[i] * zi;
+            A.z[3 * v + i] = zv[i];
+            drz += rv[i] * zv[i];
             drr += rv[i] * rv[i];
         }
     }
@@ -457,20 +728,20 @@ __global__ void __launch_bounds__(PCG_UPD_BLOCK) k_pcg_update(PcgArgs A, int it)
     if (*(volatile int*)(A.flag + inst) != 0) return;  // one load: uniform over the wavefront
     const int par = it & 1;
     const double* p = A.pbuf + (size_t)par * 3 * A.nV;
+    const double* r_in = A.r + (size_t)par * 3 * A.nV;  // not written by this kernel: members read it safely
+    double* r_out = A.r + (size_t)(par ^ 1) * 3 * A.nV;
     const int v = A.blo[blockIdx.x] + threadIdx.x;
     const bool own = v < A.bhi[blockIdx.x];
     // this vertex's vectors are loaded before alpha is known (independent loads)
-    double xv[3] = {0, 0, 0}, pv[3] = {0, 0, 0}, rv[3] = {0, 0, 0}, qv[3] = {0, 0, 0}, P[9];
+    double xv[3] = {0, 0, 0}, pv[3] = {0, 0, 0}, rv[3] = {0, 0, 0}, qv[3] = {0, 0, 0};
     if (own) {
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             xv[i] = A.x[3 * v + i];
             pv[i] = p[3 * v + i];
-            rv[i] = A.r[3 * v + i];
+            rv[i] = r_in[3 * v + i];
             qv[i] = A.q[3 * v + i];
         }
-#pragma unroll
-        for (int i = 0; i < 9; ++i) P[i] = A.Pinv[9 * (size_t)v + i];
     }
     const double pq = wave_sum_partials(A.part_pq + A.apbeg[inst], A.apbeg[inst + 1] - A.apbeg[inst]);
     const int pb = A.bpbeg[inst];
@@ -489,13 +760,26 @@ __global__ void __launch_bounds__(PCG_UPD_BLOCK) k_pcg_update(PcgArgs A, int it)
         for (int i = 0; i < 3; ++i) {
             A.x[3 * v + i] = xv[i] + alpha * pv[i];
             rv[i] = rv[i] - alpha * qv[i];
-            A.r[3 * v + i] = rv[i];
+            r_out[3 * v + i] = rv[i];
         }
+        // z = M r with the members' updated residuals (the own one as above)
+        double zv[3];
+        cl_apply(v, A.croot, A.ccnt, A.cmem, A.zoff, A.Z,
+                 [&](int u, double* o) {
+                     if (u == v) {
+                         o[0] = rv[0];
+                         o[1] = rv[1];
+                         o[2] = rv[2];
+                     } else {
+#pragma unroll
+                         for (int i = 0; i < 3; ++i) o[i] = r_in[3 * u + i] - alpha * A.q[3 * u + i];
+                     }
+                 },
+                 zv);
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            const double zi = P[3 * i] * rv[0] + P[3 * i + 1] * rv[1] + P[3 * i + 2] * rv[2];
-            A.z[3 * v + i] = zi;
-            drz += rv[i] * zi;
+            A.z[3 * v + i] = zv[i];
+            drz += rv[i] * zv[i];
             drr += rv[i] * rv[i];
         }
     }
@@ -703,7 +987,8 @@ __global__ void k_pcg_start(int B, const unsigned char* __restrict__ solve, int*
     err[i] = 0;
 }
 
-// Block-Jacobi PCG per instance on the assembled system (E + C); b, x: 3V.
+// Preconditioned CG per instance on the assembled system (E + C); b, x: 3V.
+// L.precond picks contact-cluster (0) or 3x3 (1) block Jacobi.
 // solve (device, B bytes; null = all) selects the instances to solve.
 // Per instance on return: iters[i] >= 0, or -1 (singular diagonal block) /
 // -2 (breakdown, p.q <= 0); relres[i] = |r|/|b|.
@@ -715,15 +1000,44 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
     const int nV = m.nV;
     const double nnzb = (double)m.nnzb_e + (double)L.ncb;
     constexpr int G = 8;  // lanes per block row (32-row SpMV blocks)
-    double* Pinv = L.pinv.ensure(9 * (size_t)nV, s);
     int* err = L.err.ensure(2 * (size_t)B + 2, s);
     int* flag = err + B;
     IDP_LAUNCH(k_pcg_start, grid_for(B, 64), 64, 0, s, B, solve, flag, err);
     const bool haveC = L.ncb > 0;
     Profiler* P = L.prof;
     const int tok = P ? P->begin(IDP_K_PCG) : -1;
-    IDP_LAUNCH(k_block_jacobi, grid_for(nV, 256), 256, 0, s, nV, m.e_diag.p, L.evals.p, m.nnzb_e,
-               haveC ? L.cdiag.p : (const int*)nullptr, L.cvals.p, L.ncb, I.vinst.p, Pinv, err);
+    // contact-cluster block-Jacobi preconditioner
+    const int vg = grid_for(nV, 256);
+    int* best = L.cl_best.ensure(nV, s);
+    int* root = L.cl_root.ensure(nV, s);
+    int* size = L.cl_size.ensure(nV, s);
+    int* root2 = L.cl_root2.ensure(nV, s);
+    int* cnt = L.cl_cnt.ensure(nV, s);
+    int* mem = L.cl_mem.ensure(CL_MAX * (size_t)nV, s);
+    int* zlen = L.cl_zlen.ensure(nV + 1, s);
+    int* zoff = L.cl_zoff.ensure(nV + 1, s);
+    unsigned long long* pack = L.cl_pack.ensure(nV, s);
+    double* Z = L.pinv.ensure(9 * CL_MAX * (size_t)nV, s);
+    const int* crp = haveC ? L.crp.p : nullptr;
+    const int* crp_match = L.precond == 0 ? crp : nullptr;  // no contact rows: singletons (3x3 block Jacobi)
+    IDP_LAUNCH(k_cl_best, vg, 256, 0, s, nV, crp_match, L.ccol.p, L.cvals.p, L.ncb, best);
+    IDP_LAUNCH(k_cl_pair, vg, 256, 0, s, nV, best, root, size);
+    IDP_HIP(hipMemsetAsync(pack, 0, nV * sizeof(unsigned long long), s));
+    IDP_LAUNCH(k_cl_best2, vg, 256, 0, s, nV, crp_match, L.ccol.p, L.cvals.p, L.ncb, root, size, pack);
+    IDP_LAUNCH(k_cl_merge, vg, 256, 0, s, nV, root, pack, size, root2);
+    IDP_HIP(hipMemsetAsync(cnt, 0, nV * sizeof(int), s));
+    IDP_LAUNCH(k_cl_members, vg, 256, 0, s, nV, root2, cnt, mem);
+    IDP_LAUNCH(k_cl_sort, vg, 256, 0, s, nV, root2, cnt, mem, zlen);
+    {
+        size_t bytes = 0;
+        IDP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, zlen, zoff, nV, s));
+        L.tmp.ensure(bytes + 16, s);
+        IDP_HIP(hipcub::DeviceScan::ExclusiveSum(L.tmp.p, bytes, zlen, zoff, nV, s));
+    }
+    double* work = L.cl_work.ensure(2 * 144 * (size_t)nV, s);
+    IDP_LAUNCH(k_cl_build, vg, 256, 0, s, nV, root2, cnt, mem, m.e_diag.p, m.e_rowptr.p, m.e_col.p, L.evals.p,
+               m.nnzb_e, haveC ? L.cdiag.p : (const int*)nullptr, crp, L.ccol.p, L.cvals.p, L.ncb, I.vinst.p, zoff,
+               Z, work, err);
     PcgArgs A;
     A.nV = nV;
     A.erp = m.e_rowptr.p;
@@ -735,9 +1049,13 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
     A.eld = m.nnzb_e;
     A.cld = L.ncb;
     A.b = b;
-    A.Pinv = Pinv;
+    A.Z = Z;
+    A.croot = root2;
+    A.ccnt = cnt;
+    A.cmem = mem;
+    A.zoff = zoff;
     A.x = x;
-    A.r = L.r.ensure(3 * (size_t)nV, s);
+    A.r = L.r.ensure(6 * (size_t)nV, s);
     A.z = L.z.ensure(3 * (size_t)nV, s);
     A.pbuf = L.p.ensure(6 * (size_t)nV, s);
     A.q = L.q.ensure(3 * (size_t)nV, s);

@@ -13,12 +13,17 @@ struct LinWork {
     DBuf<unsigned> u0, u1, seg, rv0, rv1;
     DBuf<unsigned long long> key0, key1;
     long ncb = 0, nrec = 0, ncs = 0;
+    int precond = 0;  // 0: contact-cluster block Jacobi, 1: 3x3 block Jacobi
     // values
     DBuf<int> erow;  // row of every elastic block
     DBuf<double> evals, cvals;
     // PCG
     DBuf<double> pinv, r, z, p, p1, q, part0, part1, part2, part3, scal, scal2;
     DBuf<int> err;
+    // contact-cluster preconditioner
+    DBuf<int> cl_best, cl_root, cl_size, cl_root2, cl_cnt, cl_mem, cl_zlen, cl_zoff;
+    DBuf<unsigned long long> cl_pack;
+    DBuf<double> cl_work;
     Profiler* prof = nullptr;
 };
 

@@ -41,6 +41,7 @@ typedef struct idp_solver_opts {
     int pcg_maxit;    /* default 10000 */
     int max_newton;   /* 0 = until the reference stopping rule holds */
     int verbose;      /* > 0: print a progress line to stderr every `verbose` Newton iterations */
+    int precond;      /* 0: contact-cluster block Jacobi (default), 1: 3x3 block Jacobi */
 } idp_solver_opts;
 
 typedef struct idp_step_stats {
