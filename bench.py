@@ -117,7 +117,7 @@ def copy_calibration(mib):
 def bench_batch(a):
     """256 independent cube solves batched in one context vs the plain
     one-mesh-at-a-time path (timed on an evenly spaced sample of instances)."""
-    from idp_amd import driver, scenes
+    from idp_amd import capi, driver, scenes
 
     scs = scenes.config3()
     log(f"[bench] cubes256: {sum(x.X.shape[0] for x in scs)} vertices, {sum(x.tets.shape[0] for x in scs)} tets")
@@ -136,13 +136,25 @@ def bench_batch(a):
     for w in range(a.warmup):
         solve(a.max_newton)
     lat, newton = [], []
+    b.ctx.profile(True)
     t_start = time.perf_counter()
     for k in range(a.steps):
         its, st, t = solve(a.max_newton)
         lat.append(t)
         newton.append(int(its.max()))
-        log(f"[bench] batch step {k}: {len(scs)} solves in {t:.2f} s, Newton max {its.max()} mean {its.mean():.1f}")
+        log(f"[bench] batch step {k}: {len(scs)} solves in {t:.2f} s, Newton max {its.max()} mean {its.mean():.1f}, "
+            f"PCG max {max(x['pcg_iters'] for x in st)}")
     total = time.perf_counter() - t_start
+    prof = b.ctx.profile_read()
+    b.ctx.profile(False)
+    kern = {k: {"ms": round(prof[k]["ms"], 3), "launches": prof[k]["launches"]} for k in capi.K_FAMILIES}
+    for k in ("pcg_spmv", "pcg_update", "assembly"):
+        if prof[k]["ms"] > 0:
+            kern[k]["gbs"] = prof[k]["work"] / (prof[k]["ms"] * 1e-3) / 1e9
+    if prof["barrier_hess"]["ms"] > 0:
+        kern["barrier_hess"]["tflops"] = prof["barrier_hess"]["work"] / (prof["barrier_hess"]["ms"] * 1e-3) / 1e12
+    kern["ccd"]["steps"] = prof["ccd_steps"]
+    kern["ccd"]["max_steps"] = prof["ccd_max_steps"]
     batched = a.steps * len(scs) / total
     # plain path: the same instances one at a time
     idx = list(range(0, len(scs), max(1, len(scs) // a.plain_sample)))[: a.plain_sample]
@@ -178,6 +190,7 @@ def bench_batch(a):
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
         "plain_path": {"solves_per_s": plain, "sample": f"{len(idx)} evenly spaced instances, one at a time"},
         "batched_vs_plain": batched / plain,
+        "kernels": kern,
     }
     print(json.dumps(out), flush=True)
 

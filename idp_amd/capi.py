@@ -47,6 +47,8 @@ EXPORTED = [
 # PCG preconditioner for every solve of this process: 0 contact-cluster block
 # Jacobi, 1 plain 3x3 block Jacobi (IDP_PRECOND in the environment)
 PRECOND = int(os.environ.get("IDP_PRECOND", "0"))
+# 1: every PCG solve starts from zero instead of the previous direction (IDP_PCG_COLD)
+PCG_COLD = int(os.environ.get("IDP_PCG_COLD", "0"))
 
 K_FAMILIES = ["pcg", "barrier_hess", "tet_hess", "assembly", "constraint_set", "ccd", "energy", "gradient",
               "pcg_spmv", "pcg_update"]
@@ -60,7 +62,7 @@ class IdpError(RuntimeError):
 
 class SolverOpts(C.Structure):
     _fields_ = [("pcg_rtol", C.c_double), ("pcg_maxit", C.c_int), ("max_newton", C.c_int), ("verbose", C.c_int),
-                ("precond", C.c_int)]
+                ("precond", C.c_int), ("pcg_cold_start", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -284,7 +286,7 @@ class Context:
             raise ValueError("kappa3 must be a contiguous float64 (B,3) array")
         its = np.zeros(B, np.int32)
         st = (StepStats * B)()
-        opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, verbose, PRECOND)
+        opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, verbose, PRECOND, PCG_COLD)
         self._check(lib().idp_advance_batch(self._h, h, tol.ctypes.data_as(dp), int(with_collision),
                                             d2.ctypes.data_as(dp), kappa3.ctypes.data_as(dp), thickness,
                                             C.byref(opts), its.ctypes.data_as(ip), st))
@@ -292,7 +294,7 @@ class Context:
 
     # ---- continuous batching over the slots of a batch mesh ----------------
     def serve_configure(self, h, with_collision=True, thickness=0.0, pcg_rtol=1e-8, pcg_maxit=10000, max_newton=0):
-        opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, 0, PRECOND)
+        opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, 0, PRECOND, PCG_COLD)
         self._check(lib().idp_serve_configure(self._h, h, int(with_collision), thickness, C.byref(opts)))
 
     def serve_start(self, slots, X, v, tol, dHat2, kappa3):
@@ -362,7 +364,7 @@ class Context:
         kappa_arr = np.asarray(kappa, dtype=np.float64)
         if not kappa_arr.flags["C_CONTIGUOUS"]:
             raise ValueError("kappa must be contiguous float64")
-        opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, verbose, PRECOND)
+        opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, verbose, PRECOND, PCG_COLD)
         st = StepStats()
         it = self._check(lib().idp_advance_one_step(self._h, h, tol, int(with_collision), dHat2,
                                                     kappa_arr.ctypes.data_as(dp), thickness, C.byref(opts),
@@ -445,7 +447,7 @@ class Context:
         sol, sp = _out_d(X.size)
         rel = C.c_double()
         it = C.c_int()
-        opts = SolverOpts(pcg_rtol, pcg_maxit, 0, 0, PRECOND)
+        opts = SolverOpts(pcg_rtol, pcg_maxit, 0, 0, PRECOND, PCG_COLD)
         self._check(lib().idp_newton_direction(self._h, Xp, xtp, h, dHat2, kappa, thickness, C.byref(opts), gp, sp,
                                                C.byref(rel), C.byref(it)))
         return g, sol, rel.value, it.value

@@ -36,6 +36,7 @@ struct idp_ctx {
     DBuf<double> gl, gb, Hl, Hb;  // per-tet / per-constraint local terms
     DBuf<double> part_e, part_i, part_b;
     DBuf<int> flags;
+    DBuf<unsigned char> one;  // a device byte holding 1 (warm-start mask of a one-mesh solve)
     DBuf<unsigned long long> dmin;
     DBuf<double> dist2, dist2_prev;
     DBuf<int4> cs_prev;
@@ -422,7 +423,8 @@ struct BatchEngine {
     std::vector<int> PN, ndbc;
     std::vector<idp_step_stats> S;
     std::vector<unsigned char> mask;
-    DBuf<unsigned char> dmask, dmask2;
+    DBuf<unsigned char> dmask, dmask2, dmask3;
+    bool cold = false;  // every PCG solve from zero
     DBuf<double> dalpha;
     std::vector<EnergyOut> eo;
 
@@ -437,6 +439,7 @@ struct BatchEngine {
         maxit = (opts && opts->pcg_maxit > 0) ? opts->pcg_maxit : 10000;
         max_newton = opts ? opts->max_newton : 0;
         c->L.precond = opts ? opts->precond : 0;
+        cold = opts && opts->pcg_cold_start;
         kappa.assign(B, 0);
         kmax.assign(B, 0);
         tol.assign(B, 0);
@@ -585,9 +588,13 @@ struct BatchEngine {
         if (anyN) {
             hessian(c, x, h, coll);
             std::vector<unsigned char> sm(B);
-            for (int i = 0; i < B; ++i) sm[i] = active[i] && !useGD[i];
+            std::vector<unsigned char> wm(B);
+            for (int i = 0; i < B; ++i) {
+                sm[i] = active[i] && !useGD[i];
+                wm[i] = sm[i] && PN[i] > 0 && !cold;  // sol holds this step's previous direction
+            }
             pcg_solve_batch(c->L, s, m, rhs, sol, rtol, maxit, dmask2.upload_get(sm.data(), B, s), its.data(),
-                            rel.data());
+                            rel.data(), dmask3.upload_get(wm.data(), B, s));
             for (int i = 0; i < B; ++i) {
                 if (!sm[i]) continue;
                 if (its[i] < 0) useGD[i] = 1;  // DIRECT_SOLVER.h:42-44 -> gradient descent
@@ -1182,6 +1189,7 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
         const int max_newton = opts ? opts->max_newton : 0;
         const int verbose = opts ? opts->verbose : 0;
         c->L.precond = opts ? opts->precond : 0;
+        const bool cold = opts && opts->pcg_cold_start;
         auto s = c->s;
         MeshDev& m = c->m;
         const long n3 = 3L * m.nV;
@@ -1236,7 +1244,10 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
                 S.ms_hessian += now_ms() - t0;
                 t0 = now_ms();
                 double rel = 0;
-                int it = pcg_solve_persistent(c->L, s, m, rhs, sol, rtol, maxit, &rel);
+                // from the second Newton iteration on, sol holds this step's previous direction
+                const unsigned char on = 1;
+                const unsigned char* warm = PNIter > 0 && !cold ? c->one.upload_get(&on, 1, s) : nullptr;
+                int it = pcg_solve_persistent(c->L, s, m, rhs, sol, rtol, maxit, &rel, warm);
                 S.ms_solve += now_ms() - t0;
                 if (it < 0) {
                     useGD = true;  // DIRECT_SOLVER.h:42-44 -> gradient descent (IMPLICIT_EULER.h:531-551)

@@ -713,10 +713,16 @@ __global__ void __launch_bounds__(256) k_ccd_walk(int nb, int ne, CCDArgs A, Cel
 // whatever bound it started with, so the result is the minimum over pairs.
 struct AccdLane {
     d3 x[4], dx[4];
-    double maxDisp, d2, dc, f, gap, toc, tmax;
+    double maxDisp, d2, dc, f, gap, toc, tmax, d0, tmax0;
     int guard, inst;
+    int2 c;
     bool ee;
 };
+
+// pairs that ran at least this many ACCD steps are logged (diagnostics)
+constexpr int ACCD_LOG_STEPS = 1 << 16;
+constexpr int ACCD_LOG_MAX = 8;
+constexpr int ACCD_LOG_W = 10;  // doubles per entry
 
 __device__ inline double accd_dist(const AccdLane& L)
 {
@@ -770,6 +776,9 @@ __device__ inline bool accd_init(AccdLane& L, int2 c, const CCDArgs& A, double e
     L.toc = 0;
     L.tmax = __longlong_as_double((long long)*((volatile const unsigned long long*)(amin + L.inst)));
     L.guard = 0;
+    L.d0 = L.dc;
+    L.tmax0 = L.tmax;
+    L.c = c;
     return true;
 }
 
@@ -793,7 +802,8 @@ __device__ inline int accd_step(AccdLane& L, double eta, double th)
 __global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, unsigned cap,
                                               const int2* __restrict__ cand, CCDArgs A, double eta,
                                               unsigned long long* __restrict__ amin, unsigned* __restrict__ next,
-                                              unsigned long long* __restrict__ stepstat /* total, max */)
+                                              unsigned long long* __restrict__ stepstat /* total, max */,
+                                              unsigned* __restrict__ nlog, double* __restrict__ lg)
 {
     const unsigned npt = min(cnt[0], cap), nee = min(cnt[1], cap - npt);
     const unsigned n = npt + nee;
@@ -829,6 +839,22 @@ __global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, 
             if (r != 0) {
                 active = false;
                 atomicMax(stepstat + 1, (unsigned long long)L.guard + 1);
+                if (L.guard + 1 >= ACCD_LOG_STEPS) {
+                    const unsigned k = atomicAdd(nlog, 1u);
+                    if (k < ACCD_LOG_MAX) {
+                        double* e = lg + ACCD_LOG_W * k;
+                        e[0] = L.c.x;
+                        e[1] = L.c.y;
+                        e[2] = L.inst;
+                        e[3] = L.d0;
+                        e[4] = L.maxDisp;
+                        e[5] = L.tmax0;
+                        e[6] = L.toc;
+                        e[7] = L.guard + 1;
+                        e[8] = r;
+                        e[9] = L.dc;
+                    }
+                }
             } else if ((steps & 15) == 0) {
                 L.tmax = fmin(L.tmax,
                               __longlong_as_double((long long)*((volatile unsigned long long*)(amin + L.inst))));
@@ -1234,8 +1260,9 @@ void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const do
     unsigned long long* stepstat = (unsigned long long*)w.scal.ensure(4, s) + 2;
     IDP_LAUNCH(k_ccd_cap, grid_for(B, 64), 64, 0, s, B, ep, I.echunk.pbeg.p, I.d_ebeg.p, pp, I.pchunk.pbeg.p,
                I.d_nbeg.p, ain, dalpha, amin, active);
-    IDP_LAUNCH(k_zero_u32, 1, 64, 0, s, cnt, 3);
+    IDP_LAUNCH(k_zero_u32, 1, 64, 0, s, cnt, 4);
     IDP_LAUNCH(k_zero_u64, 1, 64, 0, s, stepstat, 2);
+    double* lg = w.accd_log.ensure(ACCD_LOG_W * ACCD_LOG_MAX, s);
 
     // swept grid over boundary nodes at [x, x + alpha p] -/+ th/2 (SPATIAL_HASH.h:484-511)
     grid_from(w, s, m, X, I.nblk, m.bn.p, P, dalpha, thickness / 2, ep, active);
@@ -1262,13 +1289,13 @@ void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const do
     if (w.cap_cand == 0) w.cap_cand = 16u * (m.nb + m.ne) + 4096;
     std::vector<unsigned long long> ha(B);
     unsigned long long hsteps[2] = {0, 0};
-    unsigned hn[2] = {0, 0};
+    unsigned hn[4] = {0, 0, 0, 0};
     for (int attempt = 0; attempt < 2; ++attempt) {
         int2* cand = w.cand.ensure(w.cap_cand, s);
         const unsigned cap = (unsigned)w.cand.cap;
         if (attempt) {
             IDP_LAUNCH(k_reset_amin, grid_for(B, 64), 64, 0, s, B, dalpha, amin);
-            IDP_LAUNCH(k_zero_u32, 1, 64, 0, s, cnt, 3);
+            IDP_LAUNCH(k_zero_u32, 1, 64, 0, s, cnt, 4);
             IDP_LAUNCH(k_zero_u64, 1, 64, 0, s, stepstat, 2);
         }
         if (T.n)
@@ -1276,7 +1303,7 @@ void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const do
                        cap);
         const int tok = w.prof ? w.prof->begin(IDP_K_CCD) : -1;
         // a work queue needs no more lanes than the chip holds at once
-        IDP_LAUNCH(k_accd, 2048, 256, 0, s, cnt, cap, cand, A, 0.1, amin, cnt + 2, stepstat);
+        IDP_LAUNCH(k_accd, 2048, 256, 0, s, cnt, cap, cand, A, 0.1, amin, cnt + 2, stepstat, cnt + 3, lg);
         if (w.prof) w.prof->end(IDP_K_CCD, tok);
         IDP_HIP(hipMemcpyAsync(ha.data(), amin, B * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
         IDP_HIP(hipMemcpyAsync(hsteps, stepstat, sizeof(hsteps), hipMemcpyDeviceToHost, s));
@@ -1288,6 +1315,22 @@ void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const do
         if (attempt == 1) throw HipError(IDP_E_CAPACITY, "CCD candidate overflow");
     }
     clear_cells(w, s, T);
+    static const bool log_long = std::getenv("IDP_ACCD_LOG") != nullptr;
+    if (log_long && hn[3]) {
+        // pairs whose ACCD iteration ran long: candidate, instance, initial
+        // distance, motion bound, bound on entry, toc, steps, result, distance
+        const int k = (int)std::min<unsigned>(hn[3], ACCD_LOG_MAX);
+        std::vector<double> e(ACCD_LOG_W * k);
+        IDP_HIP(hipMemcpyAsync(e.data(), lg, e.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        IDP_HIP(hipStreamSynchronize(s));
+        for (int j = 0; j < k; ++j) {
+            const double* q = e.data() + ACCD_LOG_W * j;
+            std::fprintf(stderr,
+                         "[idp] long ACCD pair (%d,%d) inst %d d0 %.3e maxDisp %.3e tmax %.3e toc %.3e steps %.0f "
+                         "result %.0f d %.3e\n",
+                         (int)q[0], (int)q[1], (int)q[2], q[3], q[4], q[5], q[6], q[7], q[8], q[9]);
+        }
+    }
     if (w.prof) {
         w.prof->add_work(IDP_K_CCD, (double)hn[0] + hn[1]);
         w.prof->ccd_steps += (double)hsteps[0];

@@ -106,6 +106,7 @@ struct ContactWork {
     DBuf<double> ialpha;                   // CCD: capped alpha per instance
     DBuf<double> rad;                      // per-instance contact radius and its square
     DBuf<unsigned long long> iamin;        // CCD: running minimum per instance
+    DBuf<double> accd_log;                 // CCD: pairs whose ACCD iteration ran long
     DBuf<unsigned> icnt;                   // per-instance counters
     DBuf<long> cbeg;                       // constraint range per instance
     std::vector<long> h_cbeg;

@@ -533,6 +533,8 @@ struct PcgArgs {
     double* pbuf;     // 2 x 3V
     double* q;
     double* part_pq;  // nbA
+    double* part_bx;  // nbA: b.x of the warm-start pass
+    const unsigned char* warm;  // per instance: start from gamma x (x holds the previous direction); null = none
     double* part_rz;  // 2 x nbB
     double* part_rr;  // 2 x nbB
     int nbA, nbB;
@@ -613,38 +615,104 @@ __device__ inline void group_row_spmv2(int v, int lane, const int* __restrict__ 
 // x = 0, r = b, z = P b, partials of r.z and r.r (parity 0); one wavefront per block
 __global__ void __launch_bounds__(PCG_UPD_BLOCK) k_pcg_init(PcgArgs A)
 {
+    const int inst = A.binst[blockIdx.x];
+    if (A.flag[inst] != 0) return;  // not solving: x is left as it is
     const int v = A.blo[blockIdx.x] + threadIdx.x;
-    double drz = 0.0, drr = 0.0;
+    // warm start: x0 = gamma x_prev with gamma = b.x / x.Ax (the A-norm-closest
+    // multiple of the previous direction), r0 = b - gamma A x_prev
+    double gamma = 0.0;
+    if (A.warm && A.warm[inst]) {
+        const int ab = A.apbeg[inst], na = A.apbeg[inst + 1] - ab;
+        const double xq = wave_sum_partials(A.part_pq + ab, na);
+        const double bx = wave_sum_partials(A.part_bx + ab, na);
+        gamma = xq > 0.0 ? bx / xq : 0.0;
+    }
+    const bool w = gamma != 0.0;
+    auto res = [&](int u, double* o) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) o[i] = w ? A.b[3 * u + i] - gamma * A.q[3 * u + i] : A.b[3 * u + i];
+    };
+    double drz = 0.0, drr = 0.0, dbb = 0.0;
     if (v < A.bhi[blockIdx.x]) {
-        const double rv[3] = {A.b[3 * v], A.b[3 * v + 1], A.b[3 * v + 2]};
-        double zv[3];
-        cl_apply(v, A.croot, A.ccnt, A.cmem, A.zoff, A.Z,
-                 [&](int u, double* o) {
-                     o[0] = A.b[3 * u];
-                     o[1] = A.b[3 * u + 1];
-                     o[2] = A.b[3 * u + 2];
-                 },
-                 zv);
+        double rv[3], zv[3];
+        res(v, rv);
+        cl_apply(v, A.croot, A.ccnt, A.cmem, A.zoff, A.Z, res, zv);
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            A.x[3 * v + i] = 0.0;
+            const double bi = A.b[3 * v + i];
+            A.x[3 * v + i] = w ? gamma * A.x[3 * v + i] : 0.0;
             A.r[3 * v + i] = rv[i];
             A.z[3 * v + i] = zv[i];
             drz += rv[i] * zv[i];
             drr += rv[i] * rv[i];
+            dbb += bi * bi;
         }
     }
     drz = wave_sum(drz);
     drr = wave_sum(drr);
+    dbb = wave_sum(dbb);
     if (threadIdx.x == 0) {
         A.part_rz[blockIdx.x] = drz;
         A.part_rr[blockIdx.x] = drr;
+        A.part_rr[A.nbB + blockIdx.x] = dbb;  // |b|^2, read by the first SpMV (that half is free then)
     }
 }
 
 // q = A p_it with p_it = z + beta p_(it-1), computed as A z + beta A p_(it-1)
 // so that the matrix stream does not wait for beta (the reduction of the
 // previous partials runs in wavefront 0 while the others load)
+// warm start: q = A x for the instances that start from their previous
+// direction, with per-block partials of x.q and b.x
+template <int G>
+__global__ void __launch_bounds__(PCG_SPMV_BLOCK) k_pcg_warm(PcgArgs A)
+{
+    __shared__ double wsum[2][PCG_SPMV_BLOCK / 64];
+    const int inst = A.ainst[blockIdx.x];
+    const bool on = A.warm[inst] && A.flag[inst] == 0;  // uniform over the block
+    const int lane = threadIdx.x & (G - 1);
+    const int v = A.alo[blockIdx.x] + threadIdx.x / G;
+    const bool row = on && v < A.ahi[blockIdx.x];
+    double y[3] = {0.0, 0.0, 0.0}, w[3] = {0.0, 0.0, 0.0};
+    if (row) {
+        group_row_spmv2<G>(v, lane, A.erp, A.ecol, A.eval, A.eld, A.x, nullptr, y, w);
+        if (A.crp) group_row_spmv2<G>(v, lane, A.crp, A.ccol, A.cval, A.cld, A.x, nullptr, y, w);
+    }
+    double dxq = 0.0, dbx = 0.0;
+    if (row) {
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) {
+            y[0] += __shfl_xor(y[0], o, G);
+            y[1] += __shfl_xor(y[1], o, G);
+            y[2] += __shfl_xor(y[2], o, G);
+        }
+        if (lane < 3) {
+            const double yl = lane == 0 ? y[0] : (lane == 1 ? y[1] : y[2]);
+            const size_t i = 3 * (size_t)v + lane;
+            const double xi = A.x[i];
+            A.q[i] = yl;
+            dxq = xi * yl;
+            dbx = A.b[i] * xi;
+        }
+    }
+    dxq = wave_sum(dxq);
+    dbx = wave_sum(dbx);
+    if ((threadIdx.x & 63) == 0) {
+        wsum[0][threadIdx.x >> 6] = dxq;
+        wsum[1][threadIdx.x >> 6] = dbx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, c = 0.0;
+#pragma unroll
+        for (int w2 = 0; w2 < PCG_SPMV_BLOCK / 64; ++w2) {
+            a += wsum[0][w2];
+            c += wsum[1][w2];
+        }
+        A.part_pq[blockIdx.x] = a;
+        A.part_bx[blockIdx.x] = c;
+    }
+}
+
 template <int G>
 __global__ void __launch_bounds__(PCG_SPMV_BLOCK) k_pcg_spmv(PcgArgs A, int it)
 {
@@ -666,9 +734,10 @@ __global__ void __launch_bounds__(PCG_SPMV_BLOCK) k_pcg_spmv(PcgArgs A, int it)
         const double rz = wave_sum_partials(A.part_rz + par * A.nbB + pb, np);
         const double rr = wave_sum_partials(A.part_rr + par * A.nbB + pb, np);
         const double rzo = it > 0 ? wave_sum_partials(A.part_rz + (par ^ 1) * A.nbB + pb, np) : 1.0;
+        const double bbp = it == 0 ? wave_sum_partials(A.part_rr + A.nbB + pb, np) : 0.0;
         if (threadIdx.x == 0) {
             double* sci = A.scal + 3 * inst;
-            const double bb = it == 0 ? rr : sci[0];
+            const double bb = it == 0 ? bbp : sci[0];
             int stop = f != 0 ? 2 : 0;
             if (!stop && (!(bb > 0.0) || (it > 0 && rr <= A.rtol2 * bb) || it >= A.maxit)) {
                 stop = 1;
@@ -993,7 +1062,7 @@ __global__ void k_pcg_start(int B, const unsigned char* __restrict__ solve, int*
 // Per instance on return: iters[i] >= 0, or -1 (singular diagonal block) /
 // -2 (breakdown, p.q <= 0); relres[i] = |r|/|b|.
 void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* b, double* x, double rtol,
-                     int maxit, const unsigned char* solve, int* iters, double* relres)
+                     int maxit, const unsigned char* solve, int* iters, double* relres, const unsigned char* warm)
 {
     const InstDev& I = m.inst;
     const int B = I.B;
@@ -1070,12 +1139,15 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
     A.binst = I.pcg_upd.inst.p;
     A.bpbeg = I.pcg_upd.pbeg.p;
     A.part_pq = L.part0.ensure(A.nbA + 1, s);
+    A.part_bx = L.part_bx.ensure(A.nbA + 1, s);
+    A.warm = warm;
     A.part_rz = L.part1.ensure(2 * (size_t)A.nbB + 2, s);
     A.part_rr = L.part2.ensure(2 * (size_t)A.nbB + 2, s);
     A.flag = flag;
     A.scal = L.scal.ensure(3 * (size_t)B + 3, s);
     A.rtol2 = rtol * rtol;
     A.maxit = maxit;
+    if (warm) IDP_LAUNCH(k_pcg_warm<G>, A.nbA, PCG_SPMV_BLOCK, 0, s, A);
     IDP_LAUNCH(k_pcg_init, A.nbB, PCG_UPD_BLOCK, 0, s, A);
     // algorithmic bytes of one SpMV launch (blocks + columns, two row-pointer
     // arrays, z and p_old read, p_new and q written) and of one update launch
@@ -1135,11 +1207,11 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
 }
 
 int pcg_solve_persistent(LinWork& L, hipStream_t s, const MeshDev& m, const double* b, double* x, double rtol,
-                         int maxit, double* relres)
+                         int maxit, double* relres, const unsigned char* warm)
 {
     int it = 0;
     double rel = 0;
-    pcg_solve_batch(L, s, m, b, x, rtol, maxit, nullptr, &it, &rel);
+    pcg_solve_batch(L, s, m, b, x, rtol, maxit, nullptr, &it, &rel, warm);
     *relres = rel;
     return it;
 }

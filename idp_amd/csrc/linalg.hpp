@@ -18,7 +18,7 @@ struct LinWork {
     DBuf<int> erow;  // row of every elastic block
     DBuf<double> evals, cvals;
     // PCG
-    DBuf<double> pinv, r, z, p, p1, q, part0, part1, part2, part3, scal, scal2;
+    DBuf<double> pinv, r, z, p, p1, q, part0, part1, part2, part3, part_bx, scal, scal2;
     DBuf<int> err;
     // contact-cluster preconditioner
     DBuf<int> cl_best, cl_root, cl_size, cl_root2, cl_cnt, cl_mem, cl_zlen, cl_zoff;
@@ -32,10 +32,13 @@ void contact_gradient_gather(LinWork& L, hipStream_t s, const MeshDev& m, const 
                              const double* gl, const double* gb, double* g, double* rhs);
 void assemble(LinWork& L, hipStream_t s, const MeshDev& m, const double* Hl, const double* Hb, bool any_dbc);
 void init_erow(LinWork& L, hipStream_t s, const MeshDev& m, const std::vector<int>& rowof);
+// warm (device, B bytes; null = none): instances that start from a multiple
+// of what x holds on entry (their previous direction) instead of zero
 void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* b, double* x, double rtol,
-                     int maxit, const unsigned char* solve, int* iters, double* relres);
+                     int maxit, const unsigned char* solve, int* iters, double* relres,
+                     const unsigned char* warm = nullptr);
 int pcg_solve_persistent(LinWork& L, hipStream_t s, const MeshDev& m, const double* b, double* x, double rtol,
-                         int maxit, double* relres);
+                         int maxit, double* relres, const unsigned char* warm = nullptr);
 double dev_dot(LinWork& L, hipStream_t s, long n, const double* a, const double* b);
 void dev_velocity(hipStream_t s, long n, const double* x, const double* xn, double h, double* v);
 void dev_axpy(hipStream_t s, long n, const double* a, double sc, const double* b, double* x);

@@ -42,6 +42,8 @@ typedef struct idp_solver_opts {
     int max_newton;   /* 0 = until the reference stopping rule holds */
     int verbose;      /* > 0: print a progress line to stderr every `verbose` Newton iterations */
     int precond;      /* 0: contact-cluster block Jacobi (default), 1: 3x3 block Jacobi */
+    int pcg_cold_start; /* 1: every PCG solve starts from zero; 0 (default): from the second Newton
+                           iteration of a step on, from the best multiple of the previous direction */
 } idp_solver_opts;
 
 typedef struct idp_step_stats {
