@@ -195,10 +195,11 @@ def bench_batch(a):
     print(json.dumps(out), flush=True)
 
 
-def serve_sequences(frames):
+def serve_sequences(frames, total=200):
     """Animation-fix sequences: bars of 8x4x4 (6), 32x8x8 (4) and 64x16x16 (2)
-    cells bent from 0 to 270 degrees over `frames` frames (bend radius 0.15 of
-    the bar length).  Returns [(rest scene, [target of frame 1..frames])]."""
+    cells whose target bends from 0 to 270 degrees over a `total`-frame
+    sequence (bend radius 0.15 of the bar length); the first `frames` frames
+    of each are requested.  Returns [(rest scene, [target of frame 1..frames])]."""
     from idp_amd import scenes
 
     out = []
@@ -206,7 +207,7 @@ def serve_sequences(frames):
         for _ in range(count):
             L, W = nx * 0.05, ny * 0.05
             mid = np.array([L / 2, W / 2, 0.0])  # the bend map is centred on the bar's middle
-            tg = [scenes.bar_bend(nx, ny, nz, 0.05, 0.15 * L, 270.0 * k / frames).target + mid
+            tg = [scenes.bar_bend(nx, ny, nz, 0.05, 0.15 * L, 270.0 * k / total).target + mid
                   for k in range(1, frames + 1)]
             out.append((scenes.bar_bend(nx, ny, nz, 0.05, 0.15 * L, 0.0), tg))
     return out
@@ -235,9 +236,9 @@ def bench_serve(a):
     k_init = b.kappa.copy()
     h = b.h
 
-    def run(rate, seed):
+    def run(rate, seed, frames=F):
         rng = np.random.default_rng(seed)
-        order = [(s, k) for k in range(F) for s in range(S)]
+        order = [(s, k) for k in range(frames) for s in range(S)]
         arrive = np.cumsum(rng.exponential(1.0 / rate, len(order)))
         arr = {o: t for o, t in zip(order, arrive)}
         state = [rest[s].X.copy() for s in range(S)]
@@ -245,10 +246,13 @@ def bench_serve(a):
         nxt = [0] * S          # next frame per sequence
         busy = [False] * S
         lat, t0 = [], time.perf_counter()
-        done = 0
-        while done < S * F:
+        done, last = 0, 0.0
+        while done < S * frames:
             now = time.perf_counter() - t0
-            ready = [s for s in range(S) if not busy[s] and nxt[s] < F and arr[(s, nxt[s])] <= now]
+            if now - last > 30.0:
+                log(f"[bench] serve rate {rate}: {done} of {S * frames} frames after {now:.0f} s")
+                last = now
+            ready = [s for s in range(S) if not busy[s] and nxt[s] < frames and arr[(s, nxt[s])] <= now]
             if ready:
                 X = [state[s] for s in ready]
                 V = [(seqs[s][1][nxt[s]] - state[s]) / h for s in ready]
@@ -267,13 +271,14 @@ def bench_serve(a):
                     busy[s] = False
                     done += 1
             else:
-                waits = [arr[(s, nxt[s])] for s in range(S) if nxt[s] < F]
+                waits = [arr[(s, nxt[s])] for s in range(S) if nxt[s] < frames]
                 time.sleep(max(0.0, min(waits) - (time.perf_counter() - t0)))
         span = time.perf_counter() - t0
-        return S * F / span, np.array(lat) * 1e3
+        return S * frames / span, np.array(lat) * 1e3
 
-    # warm-up pass (device buffers reach their working sizes)
-    run(1e6, 0)
+    # warm-up pass (device buffers reach their working sizes): the first
+    # frame of every sequence, from the rest state again afterwards
+    run(1e6, 0, frames=1)
     results = {}
     for rate in [float(r) for r in a.rates.split(",")]:
         thr, lat = run(rate, 1)
@@ -338,7 +343,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--workload", default="bar_twist_fold",
                     choices=["bar_twist_fold", "bar_bend", "cubes256", "serve"])
-    ap.add_argument("--frames", type=int, default=10, help="serve: frames per sequence (0 to 270 degrees)")
+    ap.add_argument("--frames", type=int, default=20,
+                    help="serve: frames requested per sequence (the first of a 200-frame bend to 270 degrees)")
     ap.add_argument("--rates", default="5,20,100,200", help="serve: Poisson arrival rates (req/s)")
     ap.add_argument("--plain-sample", type=int, default=16,
                     help="cubes256: instances solved one at a time for the plain-path comparison")

@@ -49,6 +49,9 @@ EXPORTED = [
 PRECOND = int(os.environ.get("IDP_PRECOND", "0"))
 # 1: every PCG solve starts from zero instead of the previous direction (IDP_PCG_COLD)
 PCG_COLD = int(os.environ.get("IDP_PCG_COLD", "0"))
+# batched engine: ACCD steps per pair and Newton iteration before a hand-over
+# (0 default, < 0 never; IDP_ACCD_BUDGET)
+ACCD_BUDGET = int(os.environ.get("IDP_ACCD_BUDGET", "0"))
 
 K_FAMILIES = ["pcg", "barrier_hess", "tet_hess", "assembly", "constraint_set", "ccd", "energy", "gradient",
               "pcg_spmv", "pcg_update"]
@@ -62,7 +65,7 @@ class IdpError(RuntimeError):
 
 class SolverOpts(C.Structure):
     _fields_ = [("pcg_rtol", C.c_double), ("pcg_maxit", C.c_int), ("max_newton", C.c_int), ("verbose", C.c_int),
-                ("precond", C.c_int), ("pcg_cold_start", C.c_int)]
+                ("precond", C.c_int), ("pcg_cold_start", C.c_int), ("accd_budget", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -275,9 +278,11 @@ class Context:
                                              tp, ibp, vp, lp, mp, msp))
 
     def advance_batch(self, h, tol, dHat2, kappa3, thickness=0.0, with_collision=True, pcg_rtol=1e-8,
-                      pcg_maxit=10000, max_newton=0, verbose=0):
+                      pcg_maxit=10000, max_newton=0, verbose=0, accd_budget=None):
         """One step of every instance; kappa3 (B,3) is updated in place.
-        verbose > 0 prints a progress line to stderr every `verbose` Newton iterations.
+        verbose > 0 prints a progress line to stderr every `verbose` Newton iterations;
+        accd_budget: ACCD steps per pair and Newton iteration before a hand-over
+        (None: the process default, 0: the library default, < 0: never).
         Returns (newton iterations (B,), list of per-instance stats dicts)."""
         B = len(self.batch_nV)
         tol = np.ascontiguousarray(tol, dtype=np.float64)
@@ -286,15 +291,18 @@ class Context:
             raise ValueError("kappa3 must be a contiguous float64 (B,3) array")
         its = np.zeros(B, np.int32)
         st = (StepStats * B)()
-        opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, verbose, PRECOND, PCG_COLD)
+        opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, verbose, PRECOND, PCG_COLD,
+                          ACCD_BUDGET if accd_budget is None else accd_budget)
         self._check(lib().idp_advance_batch(self._h, h, tol.ctypes.data_as(dp), int(with_collision),
                                             d2.ctypes.data_as(dp), kappa3.ctypes.data_as(dp), thickness,
                                             C.byref(opts), its.ctypes.data_as(ip), st))
         return its, [x.as_dict() for x in st]
 
     # ---- continuous batching over the slots of a batch mesh ----------------
-    def serve_configure(self, h, with_collision=True, thickness=0.0, pcg_rtol=1e-8, pcg_maxit=10000, max_newton=0):
-        opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, 0, PRECOND, PCG_COLD)
+    def serve_configure(self, h, with_collision=True, thickness=0.0, pcg_rtol=1e-8, pcg_maxit=10000, max_newton=0,
+                        accd_budget=None):
+        opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, 0, PRECOND, PCG_COLD,
+                          ACCD_BUDGET if accd_budget is None else accd_budget)
         self._check(lib().idp_serve_configure(self._h, h, int(with_collision), thickness, C.byref(opts)))
 
     def serve_start(self, slots, X, v, tol, dHat2, kappa3):
@@ -364,7 +372,7 @@ class Context:
         kappa_arr = np.asarray(kappa, dtype=np.float64)
         if not kappa_arr.flags["C_CONTIGUOUS"]:
             raise ValueError("kappa must be contiguous float64")
-        opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, verbose, PRECOND, PCG_COLD)
+        opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, verbose, PRECOND, PCG_COLD, ACCD_BUDGET)
         st = StepStats()
         it = self._check(lib().idp_advance_one_step(self._h, h, tol, int(with_collision), dHat2,
                                                     kappa_arr.ctypes.data_as(dp), thickness, C.byref(opts),
@@ -447,7 +455,7 @@ class Context:
         sol, sp = _out_d(X.size)
         rel = C.c_double()
         it = C.c_int()
-        opts = SolverOpts(pcg_rtol, pcg_maxit, 0, 0, PRECOND, PCG_COLD)
+        opts = SolverOpts(pcg_rtol, pcg_maxit, 0, 0, PRECOND, PCG_COLD, ACCD_BUDGET)
         self._check(lib().idp_newton_direction(self._h, Xp, xtp, h, dHat2, kappa, thickness, C.byref(opts), gp, sp,
                                                C.byref(rel), C.byref(it)))
         return g, sol, rel.value, it.value

@@ -410,6 +410,10 @@ __global__ void k_kappa_test(long n, const int4* __restrict__ cs, const int* __r
 // iterate() advances every active instance by one Newton iteration, so
 // instances can join and leave between iterations (continuous batching).
 // ---------------------------------------------------------------------------
+// ACCD steps a pair may take in one engine iteration before it is handed over
+// to the next (about a millisecond of one lane's time)
+constexpr int ACCD_STEP_BUDGET = 1 << 14;
+
 struct BatchEngine {
     bool configured = false;
     double h = 0, thickness = 0;
@@ -420,11 +424,13 @@ struct BatchEngine {
     std::vector<double> kappa, kmax, tol, dHat2, th, Eprev, alpha, feasible, E;
     std::vector<std::deque<double>> res;
     std::vector<char> useGD, active, finished_now;
+    std::vector<char> pend;  // a CCD pair of this instance was handed over to the next call
     std::vector<int> PN, ndbc;
     std::vector<idp_step_stats> S;
     std::vector<unsigned char> mask;
     DBuf<unsigned char> dmask, dmask2, dmask3;
     bool cold = false;  // every PCG solve from zero
+    int accd_budget = ACCD_STEP_BUDGET;  // 0: no hand-over
     DBuf<double> dalpha;
     std::vector<EnergyOut> eo;
 
@@ -440,6 +446,7 @@ struct BatchEngine {
         max_newton = opts ? opts->max_newton : 0;
         c->L.precond = opts ? opts->precond : 0;
         cold = opts && opts->pcg_cold_start;
+        accd_budget = !opts || opts->accd_budget == 0 ? ACCD_STEP_BUDGET : (opts->accd_budget < 0 ? 0 : opts->accd_budget);
         kappa.assign(B, 0);
         kmax.assign(B, 0);
         tol.assign(B, 0);
@@ -451,6 +458,7 @@ struct BatchEngine {
         E.assign(B, 0);
         res.assign(B, {});
         useGD.assign(B, 0);
+        pend.assign(B, 0);
         active.assign(B, 0);
         finished_now.assign(B, 0);
         PN.assign(B, 0);
@@ -504,6 +512,7 @@ struct BatchEngine {
             kmax[i] = kappa3[3 * i + 1];
             res[i].clear();
             useGD[i] = 0;
+            pend[i] = 0;
             PN[i] = 0;
             std::memset(&S[i], 0, sizeof(idp_step_stats));
         }
@@ -561,7 +570,10 @@ struct BatchEngine {
         return c->cw.h_cbeg.size() == (size_t)B + 1 && c->cw.h_cbeg[i + 1] > c->cw.h_cbeg[i];
     }
 
-    // one Newton iteration of every active instance
+    // One Newton iteration of every active instance.  An instance whose CCD
+    // handed a pair over (ACCD step budget) waits: it continues that pair in
+    // the next call and takes its line search once the step size is final,
+    // while the other instances go on.  Handing over changes no arithmetic.
     int iterate(idp_ctx* c)
     {
         auto s = c->s;
@@ -574,15 +586,24 @@ struct BatchEngine {
         double* rhs = c->rhs.p;
         double* g = c->g.p;
         std::fill(finished_now.begin(), finished_now.end(), 0);
-        for (int i = 0; i < B; ++i) mask[i] = active[i];
+        // fresh: instances that begin a Newton iteration now
+        std::vector<char> fresh(B, 0);
+        int nfresh = 0;
+        bool anyres = false;
+        for (int i = 0; i < B; ++i) {
+            fresh[i] = active[i] && !pend[i];
+            nfresh += fresh[i];
+            anyres |= active[i] && pend[i];
+        }
+        for (int i = 0; i < B; ++i) mask[i] = fresh[i];
         const unsigned char* dact = up_mask();
         // ---- gradient and Hessian at x with the current constraint sets
         double t0 = now_ms();
         if (coll) contact_structure(c->L, s, m.nV, c->cw.ncs, (const int*)c->cw.cs.p);
         else c->L.ncb = 0;
-        gradient(c, x, h, coll, g, rhs);
+        if (nfresh) gradient(c, x, h, coll, g, rhs);
         bool anyN = false;
-        for (int i = 0; i < B; ++i) anyN |= active[i] && !useGD[i];
+        for (int i = 0; i < B; ++i) anyN |= fresh[i] && !useGD[i];
         std::vector<int> its(B, 0);
         std::vector<double> rel(B, 0.0);
         if (anyN) {
@@ -590,7 +611,7 @@ struct BatchEngine {
             std::vector<unsigned char> sm(B);
             std::vector<unsigned char> wm(B);
             for (int i = 0; i < B; ++i) {
-                sm[i] = active[i] && !useGD[i];
+                sm[i] = fresh[i] && !useGD[i];
                 wm[i] = sm[i] && PN[i] > 0 && !cold;  // sol holds this step's previous direction
             }
             pcg_solve_batch(c->L, s, m, rhs, sol, rtol, maxit, dmask2.upload_get(sm.data(), B, s), its.data(),
@@ -604,20 +625,21 @@ struct BatchEngine {
                 }
             }
         }
-        for (int i = 0; i < B; ++i) mask[i] = active[i] && useGD[i];
+        for (int i = 0; i < B; ++i) mask[i] = fresh[i] && useGD[i];
         IDP_LAUNCH(k_copy_inst, vgrid, 256, 0, s, m.nV, I.vinst.p, up_mask(), rhs, sol);
         const double tsolve = now_ms() - t0;
         for (int i = 0; i < B; ++i) {
-            if (!active[i]) continue;
+            if (!fresh[i]) continue;
             if (useGD[i]) S[i].gd_iters++;
             S[i].ms_solve += tsolve;
         }
         // ---- Line_Search, part 1: halve until no tet inverts
-        for (int i = 0; i < B; ++i) mask[i] = active[i];
+        for (int i = 0; i < B; ++i) mask[i] = fresh[i];
         dact = up_mask();
         IDP_LAUNCH(k_copy_inst, vgrid, 256, 0, s, m.nV, I.vinst.p, dact, x, xprev);
-        for (int i = 0; i < B; ++i) alpha[i] = 1.0;
-        std::vector<unsigned char> srch(active.begin(), active.end());
+        for (int i = 0; i < B; ++i)
+            if (fresh[i]) alpha[i] = 1.0;
+        std::vector<unsigned char> srch(fresh.begin(), fresh.end());
         for (int guard = 0;; ++guard) {
             if (guard > 4000) return IDP_E_INVALID;
             bool any = false;
@@ -633,30 +655,37 @@ struct BatchEngine {
                     else alpha[i] /= 2.0;
                 }
         }
-        for (int i = 0; i < B; ++i) mask[i] = active[i];
+        for (int i = 0; i < B; ++i) mask[i] = fresh[i];
         dact = up_mask();
         IDP_LAUNCH(k_copy_inst, vgrid, 256, 0, s, m.nV, I.vinst.p, dact, xprev, x);
+        // go: instances whose step size is final in this call
+        std::vector<char> go(fresh);
         if (coll) {
             t0 = now_ms();
             long nc = 0;
             std::vector<double> a2(alpha);
-            contact_ccd_batch(c->cw, s, m, x, sol, thickness, a2.data(), dact, &nc);
+            std::vector<unsigned char> pnow(B, 0);
+            // with nothing else in flight a handed-over pair just runs to its end
+            const int budget = nfresh > 0 ? accd_budget : 0;
+            contact_ccd_batch(c->cw, s, m, x, sol, thickness, a2.data(), dact, &nc, budget, anyres, pnow.data());
             const double tc = now_ms() - t0;
             for (int i = 0; i < B; ++i)
                 if (active[i]) {
-                    alpha[i] = a2[i];
+                    go[i] = !pnow[i];
+                    if (go[i]) alpha[i] = a2[i];
+                    pend[i] = pnow[i];
                     S[i].ms_ccd += tc;
                     if (n_active == 1) S[i].ccd_candidates += nc;
                 }
         }
         for (int i = 0; i < B; ++i)
-            if (active[i]) {
+            if (go[i]) {
                 feasible[i] = alpha[i];
                 if (PN[i] == 0) S[i].first_ccd_alpha = alpha[i];
             }
         // ---- Line_Search, part 2: backtracking on energy with the
         // minimum-distance safeguard (IMPLICIT_EULER.h:102-146)
-        srch.assign(active.begin(), active.end());
+        srch.assign(go.begin(), go.end());
         for (int guard = 0;; ++guard) {
             bool any = false;
             for (int i = 0; i < B; ++i) any |= srch[i] != 0;
@@ -706,7 +735,7 @@ struct BatchEngine {
             }
         }
         for (int i = 0; i < B; ++i)
-            if (active[i]) {
+            if (go[i]) {
                 Eprev[i] = E[i];
                 S[i].last_alpha = alpha[i] * 2;
             }
@@ -726,7 +755,7 @@ struct BatchEngine {
             IDP_HIP(hipStreamSynchronize(s));
             bool anyk = false;
             for (int i = 0; i < B; ++i)
-                if (active[i] && PN[i] > 0 && hk[i] && kappa[i] < kmax[i]) {
+                if (go[i] && PN[i] > 0 && hk[i] && kappa[i] < kmax[i]) {
                     kappa[i] *= 2;
                     anyk = true;
                 } else {
@@ -745,7 +774,7 @@ struct BatchEngine {
             std::vector<double> md;
             min_dist(c, d2, md);
             for (int i = 0; i < B; ++i)
-                if (active[i] && has_constraints(c, i)) S[i].min_dist2 = md[i] - thickness * thickness;
+                if (go[i] && has_constraints(c, i)) S[i].min_dist2 = md[i] - thickness * thickness;
             c->cs_prev.ensure(n + 1, s);
             if (n) IDP_HIP(hipMemcpyAsync(c->cs_prev.p, c->cw.cs.p, n * sizeof(int4), hipMemcpyDeviceToDevice, s));
             c->ncs_prev = n;
@@ -761,7 +790,7 @@ struct BatchEngine {
         std::fill(mask.begin(), mask.end(), 0);
         bool anyf = false;
         for (int i = 0; i < B; ++i) {
-            if (!active[i]) continue;
+            if (!go[i]) continue;
             const long nv = I.vbeg[i + 1] - I.vbeg[i];
             double L2 = std::sqrt(ss[i] / (nv - ndbc[i])) / h;
             ++PN[i];
@@ -937,6 +966,11 @@ int idp_ctx_create(int device, idp_ctx** out)
         c->prof.s = c->s;
         c->cw.prof = &c->prof;
         c->L.prof = &c->prof;
+        // SpMV launch shape override (tuning runs): an index into the shape table
+        if (const char* e = std::getenv("IDP_SPMV_SHAPE")) {
+            const int k = std::atoi(e);
+            if (k >= 0 && k < 8) c->L.spmv_shape = k;
+        }
     } catch (const HipError& e) {
         delete c->eng;
         delete c;

@@ -803,14 +803,17 @@ __global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, 
                                               const int2* __restrict__ cand, CCDArgs A, double eta,
                                               unsigned long long* __restrict__ amin, unsigned* __restrict__ next,
                                               unsigned long long* __restrict__ stepstat /* total, max */,
-                                              unsigned* __restrict__ nlog, double* __restrict__ lg)
+                                              unsigned* __restrict__ nlog, double* __restrict__ lg, int budget,
+                                              const AccdLane* __restrict__ res_in, unsigned nres,
+                                              AccdLane* __restrict__ res_out, unsigned* __restrict__ nres_out,
+                                              unsigned res_cap, unsigned char* __restrict__ pend)
 {
     const unsigned npt = min(cnt[0], cap), nee = min(cnt[1], cap - npt);
-    const unsigned n = npt + nee;
+    const unsigned n = nres + npt + nee;  // resumed pairs first, then PT, then EE candidates
     const int lane = threadIdx.x & 63;
     AccdLane L;
     bool active = false, finished = false;
-    int steps = 0;
+    int steps = 0, slice = 0;
     for (;;) {
         const unsigned long long idle = __ballot(!active && !finished);
         if (idle) {
@@ -820,10 +823,19 @@ __global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, 
             base = __shfl(base, leader);
             if (!active && !finished) {
                 const unsigned j = base + (unsigned)__popcll(idle & ((1ull << lane) - 1));
+                slice = 0;
                 if (j >= n) {
                     finished = true;
+                } else if (j < nres) {
+                    // a pair stopped by the step budget of an earlier call: its state is
+                    // continued exactly; only its bound may have come down since
+                    L = res_in[j];
+                    L.tmax = fmin(L.tmax, __longlong_as_double(
+                                              (long long)*((volatile unsigned long long*)(amin + L.inst))));
+                    active = true;
                 } else {
-                    const int2 c = j < npt ? cand[j] : cand[cap - 1 - (j - npt)];
+                    const unsigned q = j - nres;
+                    const int2 c = q < npt ? cand[q] : cand[cap - 1 - (q - npt)];
                     active = accd_init(L, c, A, eta, amin);
                 }
             }
@@ -854,6 +866,16 @@ __global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, 
                         e[8] = r;
                         e[9] = L.dc;
                     }
+                }
+            } else if (budget > 0 && ++slice >= budget) {
+                // step budget of this call spent: hand the pair's state to the next call
+                const unsigned k = atomicAdd(nres_out, 1u);
+                if (k < res_cap) {
+                    res_out[k] = L;
+                    pend[L.inst] = 1;
+                    active = false;
+                } else {
+                    slice = INT_MIN;  // no room: run this pair to its end
                 }
             } else if ((steps & 15) == 0) {
                 L.tmax = fmin(L.tmax,
@@ -889,15 +911,22 @@ __global__ void k_ccd_cap(int B, const double* __restrict__ eparts, const int* _
     amin[i] = (unsigned long long)__double_as_longlong(a);
 }
 
-__global__ void k_reset_amin(int B, const double* __restrict__ alpha, unsigned long long* __restrict__ amin)
+__global__ void k_reset_amin(int B, const double* __restrict__ alpha, unsigned long long* __restrict__ amin,
+                             const unsigned char* __restrict__ active)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < B) amin[i] = (unsigned long long)__double_as_longlong(alpha[i]);
+    if (i < B && (!active || active[i])) amin[i] = (unsigned long long)__double_as_longlong(alpha[i]);
 }
 
 __global__ void k_zero_u64(unsigned long long* __restrict__ a, int n)
 {
     if ((int)threadIdx.x < n) a[threadIdx.x] = 0ull;
+}
+
+__global__ void k_zero_u8(unsigned char* __restrict__ a, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0;
 }
 
 __global__ void k_zero_u32(unsigned* __restrict__ a, int n)
@@ -1131,7 +1160,7 @@ long contact_constraint_set(ContactWork& w, hipStream_t s, const MeshDev& m, con
     O.ebeg = I.d_ebeg.p;
     O.fbeg = I.d_fbeg.p;
     O.inst_shift = B > 1 ? 56 : 0;
-    unsigned* cnt = w.cnt.ensure(4, s);
+    unsigned* cnt = w.cnt.ensure(6, s);
     if (w.cap_pass == 0) w.cap_pass = 4u * (m.nb + m.ne) + 1024;
     if (w.cap_dup == 0) w.cap_dup = 4u * (m.nb + m.ne) + 1024;
     unsigned hc[2] = {0, 0};
@@ -1241,12 +1270,20 @@ long contact_constraint_set(ContactWork& w, hipStream_t s, const MeshDev& m, con
 // upper bounds on entry and the step sizes on return; instances outside the
 // active mask (device, null = all) are left untouched.
 void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const double* X, const double* P,
-                       double thickness, double* alpha_io, const unsigned char* active, long* ncand_out)
+                       double thickness, double* alpha_io, const unsigned char* active, long* ncand_out,
+                       int budget, bool resume, unsigned char* pending)
 {
     const InstDev& I = m.inst;
     const int B = I.B;
     if (ncand_out) *ncand_out = 0;
+    if (pending) std::fill(pending, pending + B, (unsigned char)0);
     if (m.nb == 0) return;
+    // pairs handed over by the previous call (step budget), and room for this call's
+    constexpr unsigned RES_CAP = 4096;
+    const unsigned nres = resume ? w.n_res : 0u;
+    const AccdLane* res_in = (const AccdLane*)w.accd_res[w.res_cur].ensure(RES_CAP * sizeof(AccdLane), s);
+    AccdLane* res_out = (AccdLane*)w.accd_res[w.res_cur ^ 1].ensure(RES_CAP * sizeof(AccdLane), s);
+    unsigned char* pend = w.pend.ensure(B + 1, s);
     // span cap (mean edge length and mean |p| per instance), kept on the device
     double* ep = w.dparts.ensure(I.echunk.nblk + I.pchunk.nblk + 2, s);
     double* pp = ep + I.echunk.nblk + 1;
@@ -1256,12 +1293,13 @@ void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const do
     IDP_HIP(hipMemcpyAsync(ain, alpha_io, B * sizeof(double), hipMemcpyHostToDevice, s));
     double* dalpha = w.ialpha.ensure(B, s);
     unsigned long long* amin = w.iamin.ensure(B, s);
-    unsigned* cnt = w.cnt.ensure(4, s);
+    unsigned* cnt = w.cnt.ensure(6, s);
     unsigned long long* stepstat = (unsigned long long*)w.scal.ensure(4, s) + 2;
     IDP_LAUNCH(k_ccd_cap, grid_for(B, 64), 64, 0, s, B, ep, I.echunk.pbeg.p, I.d_ebeg.p, pp, I.pchunk.pbeg.p,
                I.d_nbeg.p, ain, dalpha, amin, active);
-    IDP_LAUNCH(k_zero_u32, 1, 64, 0, s, cnt, 4);
+    IDP_LAUNCH(k_zero_u32, 1, 64, 0, s, cnt, 6);
     IDP_LAUNCH(k_zero_u64, 1, 64, 0, s, stepstat, 2);
+    IDP_LAUNCH(k_zero_u8, grid_for(B, 256), 256, 0, s, pend, B);
     double* lg = w.accd_log.ensure(ACCD_LOG_W * ACCD_LOG_MAX, s);
 
     // swept grid over boundary nodes at [x, x + alpha p] -/+ th/2 (SPATIAL_HASH.h:484-511)
@@ -1289,21 +1327,23 @@ void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const do
     if (w.cap_cand == 0) w.cap_cand = 16u * (m.nb + m.ne) + 4096;
     std::vector<unsigned long long> ha(B);
     unsigned long long hsteps[2] = {0, 0};
-    unsigned hn[4] = {0, 0, 0, 0};
+    unsigned hn[6] = {0, 0, 0, 0, 0, 0};
     for (int attempt = 0; attempt < 2; ++attempt) {
         int2* cand = w.cand.ensure(w.cap_cand, s);
         const unsigned cap = (unsigned)w.cand.cap;
         if (attempt) {
-            IDP_LAUNCH(k_reset_amin, grid_for(B, 64), 64, 0, s, B, dalpha, amin);
-            IDP_LAUNCH(k_zero_u32, 1, 64, 0, s, cnt, 4);
+            IDP_LAUNCH(k_reset_amin, grid_for(B, 64), 64, 0, s, B, dalpha, amin, active);
+            IDP_LAUNCH(k_zero_u32, 1, 64, 0, s, cnt, 6);
             IDP_LAUNCH(k_zero_u64, 1, 64, 0, s, stepstat, 2);
+            IDP_LAUNCH(k_zero_u8, grid_for(B, 256), 256, 0, s, pend, B);
         }
         if (T.n)
             IDP_LAUNCH(k_ccd_walk<WALK_G>, walk_blocks((long)m.nb + m.ne), 256, 0, s, m.nb, m.ne, A, T, cnt, cand,
                        cap);
         const int tok = w.prof ? w.prof->begin(IDP_K_CCD) : -1;
         // a work queue needs no more lanes than the chip holds at once
-        IDP_LAUNCH(k_accd, 2048, 256, 0, s, cnt, cap, cand, A, 0.1, amin, cnt + 2, stepstat, cnt + 3, lg);
+        IDP_LAUNCH(k_accd, 2048, 256, 0, s, cnt, cap, cand, A, 0.1, amin, cnt + 2, stepstat, cnt + 3, lg, budget,
+                   res_in, nres, res_out, cnt + 4, RES_CAP, pend);
         if (w.prof) w.prof->end(IDP_K_CCD, tok);
         IDP_HIP(hipMemcpyAsync(ha.data(), amin, B * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
         IDP_HIP(hipMemcpyAsync(hsteps, stepstat, sizeof(hsteps), hipMemcpyDeviceToHost, s));
@@ -1342,15 +1382,27 @@ void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const do
         IDP_HIP(hipMemcpyAsync(act.data(), active, B, hipMemcpyDeviceToHost, s));
         IDP_HIP(hipStreamSynchronize(s));
     }
-    for (int i = 0; i < B; ++i)
-        if (act[i]) std::memcpy(&alpha_io[i], &ha[i], sizeof(double));
+    std::vector<unsigned char> hp(B, 0);
+    if (budget > 0) {
+        IDP_HIP(hipMemcpyAsync(hp.data(), pend, B, hipMemcpyDeviceToHost, s));
+        IDP_HIP(hipStreamSynchronize(s));
+    }
+    w.n_res = budget > 0 ? std::min(hn[4], RES_CAP) : 0u;
+    w.res_cur ^= 1;
+    // an instance's step is final unless one of its pairs was handed over;
+    // resumed instances are those whose pairs came in (all of the last call's)
+    for (int i = 0; i < B; ++i) {
+        if (pending) pending[i] = hp[i];
+        if ((act[i] || (nres && w.res_inst[i])) && !hp[i]) std::memcpy(&alpha_io[i], &ha[i], sizeof(double));
+    }
+    w.res_inst.assign(hp.begin(), hp.end());
 }
 
 double contact_ccd(ContactWork& w, hipStream_t s, const MeshDev& m, const double* X, const double* P,
                    double thickness, double alpha_in, long* ncand_out)
 {
     double a = alpha_in;
-    contact_ccd_batch(w, s, m, X, P, thickness, &a, nullptr, ncand_out);
+    contact_ccd_batch(w, s, m, X, P, thickness, &a, nullptr, ncand_out, 0, false, nullptr);
     return a;
 }
 

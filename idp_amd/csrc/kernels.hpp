@@ -22,7 +22,7 @@ struct GridDev {
 struct SegDev {
     int nblk = 0, bs = 0;
     DBuf<int> lo, hi, inst, pbeg;
-    std::vector<int> h_pbeg;
+    std::vector<int> h_pbeg, h_lo, h_hi, h_inst;
     void build(const std::vector<long>& beg, int block, hipStream_t s)
     {
         bs = block;
@@ -43,6 +43,9 @@ struct SegDev {
         hi.upload(h.data(), h.size(), s);
         inst.upload(in.data(), in.size(), s);
         pbeg.upload(h_pbeg.data(), h_pbeg.size(), s);
+        h_lo = std::move(l);
+        h_hi = std::move(h);
+        h_inst = std::move(in);
     }
 };
 
@@ -60,7 +63,7 @@ struct InstDev {
     SegDev vblk;     // vertices, blocks of RB (bounding box, inertia energy)
     SegDev nblk;     // boundary nodes, blocks of RB (swept bounding box)
     SegDev tblk;     // tets, blocks of RB (elastic energy)
-    SegDev pcg_rows; // vertices, 32-row blocks (PCG SpMV, 8 lanes per row)
+    SegDev pcg_rows[4];  // vertices, blocks of 256 / G rows for the SpMV with G = 4, 8, 16, 32 lanes per row
     SegDev pcg_upd;  // vertices, 64-row blocks (PCG update)
     SegDev dblk;     // 3 x vertices, blocks of RB (per-instance dot products)
 };
@@ -107,6 +110,11 @@ struct ContactWork {
     DBuf<double> rad;                      // per-instance contact radius and its square
     DBuf<unsigned long long> iamin;        // CCD: running minimum per instance
     DBuf<double> accd_log;                 // CCD: pairs whose ACCD iteration ran long
+    DBuf<unsigned char> accd_res[2];       // CCD: pair states handed from one call to the next (ping-pong)
+    unsigned n_res = 0;                    // records in accd_res[res_cur]
+    int res_cur = 0;
+    std::vector<unsigned char> res_inst;   // instances owning those records
+    DBuf<unsigned char> pend;              // CCD: per instance, a pair was handed over
     DBuf<unsigned> icnt;                   // per-instance counters
     DBuf<long> cbeg;                       // constraint range per instance
     std::vector<long> h_cbeg;
@@ -144,8 +152,15 @@ long contact_constraint_set(ContactWork& w, hipStream_t s, const MeshDev& m, con
                             double thickness, const unsigned char* active = nullptr);
 double contact_ccd(ContactWork& w, hipStream_t s, const MeshDev& m, const double* X, const double* P,
                    double thickness, double alpha_in, long* ncand);
+// CCD step sizes of the active instances (device mask; null = all).
+// budget > 0 bounds the ACCD steps a pair may take in this call: a pair that
+// runs out is handed over, state and all, to the next call made with resume
+// set, and its instance is flagged in pending (host, B) with no step size
+// yet.  Handing over changes no arithmetic: the result equals an unbounded
+// call's.
 void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const double* X, const double* P,
-                       double thickness, double* alpha_io, const unsigned char* active, long* ncand);
+                       double thickness, double* alpha_io, const unsigned char* active, long* ncand,
+                       int budget = 0, bool resume = false, unsigned char* pending = nullptr);
 
 // linalg.hip: instance layout (ranges are B + 1 offsets; vinst: vertex -> instance)
 void build_inst_layout(MeshDev& m, const std::vector<long>& vbeg, const std::vector<long>& tbeg,

@@ -242,9 +242,12 @@ __global__ void k_block_jacobi(int nV, const int* __restrict__ ediag, const doub
 // grouped by two rounds of mutual heavy-edge matching on the contact blocks
 // (clusters of at most 4 vertices, never across instances), and each
 // cluster's (3k)^2 diagonal block of E + C is inverted exactly.  Vertex v
-// stores the 3 x 3k rows of its cluster inverse: z_v = Z_v [r_m0; ...; r_mk-1].
+// stores the 3 x 3k rows of its cluster inverse as k 3x3 blocks at a fixed
+// stride, z_v = sum_b Z_vb r_mb, and the member list (ascending, -1 padded)
+// as one int4, so the PCG reads both without a dependent load.
 // ---------------------------------------------------------------------------
 constexpr int CL_MAX = 4;
+constexpr int CL_ZS = 9 * CL_MAX;  // doubles of preconditioner rows per vertex
 
 __device__ inline double cblock_norm2(const double* __restrict__ val, long ld, int k)
 {
@@ -332,10 +335,8 @@ __global__ void k_cl_members(int nV, const int* __restrict__ root, int* __restri
     if (slot < CL_MAX) mem[CL_MAX * r + slot] = v;
 }
 
-// members of each cluster in ascending order; per vertex: cluster size and
-// the size of its Z rows (9 k doubles)
-__global__ void k_cl_sort(int nV, const int* __restrict__ root, const int* __restrict__ cnt, int* __restrict__ mem,
-                          int* __restrict__ zlen)
+// members of each cluster in ascending order
+__global__ void k_cl_sort(int nV, const int* __restrict__ root, const int* __restrict__ cnt, int* __restrict__ mem)
 {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nV) return;
@@ -350,7 +351,6 @@ __global__ void k_cl_sort(int nV, const int* __restrict__ root, const int* __res
                 m[j] = t;
             }
     }
-    zlen[v] = 9 * k;
 }
 
 // locate block (a, b) in a BSR row; returns -1 when absent
@@ -369,13 +369,15 @@ __global__ void k_cl_build(int nV, const int* __restrict__ root, const int* __re
                            const int* __restrict__ erp, const int* __restrict__ ecol, const double* __restrict__ evals,
                            long eld, const int* __restrict__ cdiag, const int* __restrict__ crp,
                            const int* __restrict__ ccol, const double* __restrict__ cvals, long cld,
-                           const int* __restrict__ vinst, const int* __restrict__ zoff, double* __restrict__ Z,
+                           const int* __restrict__ vinst, double* __restrict__ Z, int4* __restrict__ cm4,
                            double* __restrict__ work, int* __restrict__ err)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nV || root[r] != r) return;
     const int k = min(cnt[r], CL_MAX);
     const int* m = mem + CL_MAX * r;
+    const int4 list = make_int4(m[0], k > 1 ? m[1] : -1, k > 2 ? m[2] : -1, k > 3 ? m[3] : -1);
+    for (int a = 0; a < k; ++a) cm4[m[a]] = list;
     if (k == 1) {
         double a[9];
         const long ed = ediag[r];
@@ -389,7 +391,7 @@ __global__ void k_cl_build(int nV, const int* __restrict__ root, const int* __re
         }
         double c0 = a[4] * a[8] - a[5] * a[7], c1 = a[5] * a[6] - a[3] * a[8], c2 = a[3] * a[7] - a[4] * a[6];
         double det = a[0] * c0 + a[1] * c1 + a[2] * c2;
-        double* o = Z + zoff[r];
+        double* o = Z + (size_t)CL_ZS * r;
         if (!(det > 0)) {
             atomicOr(err + vinst[r], 1);
             det = 1.0;
@@ -465,37 +467,34 @@ __global__ void k_cl_build(int nV, const int* __restrict__ root, const int* __re
             }
         }
     }
+    // member a's rows, as k 3x3 blocks (one per member b, row-major)
     for (int a = 0; a < k; ++a) {
-        double* o = Z + zoff[m[a]];
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < n; ++j) o[i * n + j] = Ainv[(3 * a + i) * n + j];
+        double* o = Z + (size_t)CL_ZS * m[a];
+        for (int b = 0; b < k; ++b)
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) o[9 * b + 3 * i + j] = Ainv[(3 * a + i) * n + 3 * b + j];
     }
 }
 
 // z_v = Z_v r_members (members' r given by rfun)
 template <class RF>
-__device__ inline void cl_apply(int v, const int* __restrict__ root, const int* __restrict__ cnt,
-                                const int* __restrict__ mem, const int* __restrict__ zoff,
-                                const double* __restrict__ Z, RF&& rfun, double* zv)
+__device__ inline void cl_apply(int4 cm, const double* __restrict__ o, const double* own, RF&& rfun, double* zv)
 {
-    const int r = root[v];
-    const int k = min(cnt[r], CL_MAX);
-    const double* o = Z + zoff[v];
-    const int n = 3 * k;
-    zv[0] = zv[1] = zv[2] = 0.0;
-    if (k == 1) {
-        double rv[3];
-        rfun(v, rv);
+    if (cm.y < 0) {  // singleton: the 3x3 inverse of the diagonal block
 #pragma unroll
-        for (int i = 0; i < 3; ++i) zv[i] = o[3 * i] * rv[0] + o[3 * i + 1] * rv[1] + o[3 * i + 2] * rv[2];
+        for (int i = 0; i < 3; ++i) zv[i] = o[3 * i] * own[0] + o[3 * i + 1] * own[1] + o[3 * i + 2] * own[2];
         return;
     }
-    for (int b = 0; b < k; ++b) {
+    const int mb[CL_MAX] = {cm.x, cm.y, cm.z, cm.w};
+    zv[0] = zv[1] = zv[2] = 0.0;
+#pragma unroll
+    for (int b = 0; b < CL_MAX; ++b) {
+        if (mb[b] < 0) break;
         double rb[3];
-        rfun(mem[CL_MAX * r + b], rb);
+        rfun(mb[b], rb);
 #pragma unroll
         for (int i = 0; i < 3; ++i)
-            zv[i] += o[i * n + 3 * b] * rb[0] + o[i * n + 3 * b + 1] * rb[1] + o[i * n + 3 * b + 2] * rb[2];
+            zv[i] += o[9 * b + 3 * i] * rb[0] + o[9 * b + 3 * i + 1] * rb[1] + o[9 * b + 3 * i + 2] * rb[2];
     }
 }
 
@@ -508,7 +507,8 @@ __device__ inline void cl_apply(int v, const int* __restrict__ root, const int* 
 //  * k_pcg_spmv(it): every block re-reduces the previous update's partials
 //    (r.z, r.r) in a fixed order, decides convergence identically, forms
 //    p_it = z + beta p_(it-1) on the fly and computes q = A p_it with a group
-//    of G lanes per block row (xor-butterfly row reduction), plus partial p.q.
+//    of PCG_G lanes per block row (xor-butterfly row reduction), plus
+//    partial p.q.
 //  * k_pcg_update(it): alpha = r.z / p.q; x += alpha p; r -= alpha q; z = P r;
 //    partials of the new r.z and r.r.
 // p and the r.z / r.r partials ping-pong on the iteration parity.  When the
@@ -525,8 +525,8 @@ struct PcgArgs {
     const double* cval;  // 9 component planes of cld blocks
     long eld, cld;
     const double* b;
-    const double* Z;      // preconditioner rows (per vertex 3 x 3k, at zoff[v])
-    const int *croot, *ccnt, *cmem, *zoff;  // vertex -> cluster root, sizes, members
+    const double* Z;      // preconditioner rows: CL_ZS doubles per vertex (k 3x3 blocks)
+    const int4* cm4;      // per vertex: its cluster's members, ascending, -1 padded
     double* x;
     double* r;        // 2 x 3V: the update of iteration it reads half it&1, writes the other
     double* z;
@@ -538,18 +538,31 @@ struct PcgArgs {
     double* part_rz;  // 2 x nbB
     double* part_rr;  // 2 x nbB
     int nbA, nbB;
-    // instance tables: SpMV blocks of 32 rows (A*) and update blocks of 64
-    // rows (B*), each aligned to its instance's first row
-    const int *alo, *ahi, *ainst, *apbeg;
-    const int *blo, *bhi, *binst, *bpbeg;
+    // launch tables: one int4 {first row, end row, instance, block id} per
+    // launched block, SpMV blocks of PCG_ROWS rows (atab) and update blocks of
+    // 64 rows (btab), each aligned to its instance's first row.  Only the
+    // blocks of instances still iterating are launched; partials are indexed
+    // by the block id, so an instance's partial range [apbeg, bpbeg) is the
+    // same whatever else is launched.
+    const int4 *atab, *btab;
+    const int *apbeg, *bpbeg;
     int* flag;        // per instance: 0 running, 1 stopped (converged or maxit), 2 breakdown, 3 not solving
     double* scal;     // per instance, 3 each: |b|^2, final |r|^2, iterations
     double rtol2;
     int maxit;
 };
 
-constexpr int PCG_SPMV_BLOCK = 256;  // 4 wavefronts: enough blocks to cover every CU
-constexpr int PCG_UPD_BLOCK = 64;    // one wavefront: reductions need no barrier
+constexpr int PCG_ROW_THREADS = 256;         // row threads of a SpMV block (4 wavefronts), 256 / G rows
+constexpr int PCG_UPD_BLOCK = 64;            // one wavefront: reductions need no barrier
+// SpMV launch shapes: G lanes per block row, scalars in wavefront 0 (before
+// its rows) or in a fifth wavefront of their own
+struct SpmvShape {
+    int g;
+    bool sw;
+};
+constexpr SpmvShape SPMV_SHAPES[] = {{4, false}, {8, false}, {16, false}, {32, false},
+                                     {4, true},  {8, true},  {16, true},  {32, true}};
+inline int spmv_rows(int g) { return PCG_ROW_THREADS / g; }
 
 // Sum over a wavefront by a fixed xor butterfly (every lane gets the total).
 __device__ inline double wave_sum(double v)
@@ -560,40 +573,45 @@ __device__ inline double wave_sum(double v)
 }
 
 // Sum of n partials by one wavefront in a fixed order.  The loads of a chunk
-// of 16 per lane are issued together (independent), then added in order, so
-// the latency is paid once per 1024 partials rather than once per 64.
+// of NL per lane are issued together (independent), then added in order, so
+// the latency is paid once per 64 NL partials rather than once per 64.
+template <int NL = 16>
 __device__ inline double wave_sum_partials(const double* __restrict__ part, int n)
 {
     const int lane = threadIdx.x & 63;
     double acc = 0.0;
-    for (int base = 0; base < n; base += 1024) {
-        double v[16];
+    for (int base = 0; base < n; base += 64 * NL) {
+        double v[NL];
 #pragma unroll
-        for (int k = 0; k < 16; ++k) {
+        for (int k = 0; k < NL; ++k) {
             const int i = base + lane + 64 * k;
             v[k] = i < n ? part[i] : 0.0;
         }
 #pragma unroll
-        for (int k = 0; k < 16; ++k) acc += v[k];
+        for (int k = 0; k < NL; ++k) acc += v[k];
     }
     return wave_sum(acc);
 }
 
-// y += A_row z and w += A_row p_old over this lane's blocks of row v (lanes
-// of a group take blocks lane, lane + G, ...; two blocks per step so their
-// loads are in flight together).  val holds the 9 block components as planes
-// of ld doubles, so the lanes of a group read consecutive addresses.
+// y += A_row z (and w += A_row po when po is given) over this lane's blocks of
+// row v: the row's elastic blocks followed by its contact blocks, lanes of
+// a group taking slots lane, lane + G, ...  Rows of at most G blocks
+// (most of them) take one slot per lane, so every load of a lane is issued at
+// once.  Values are 9 component planes of ld doubles: the lanes of a group
+// read consecutive addresses.
 template <int G>
-__device__ inline void group_row_spmv2(int v, int lane, const int* __restrict__ rp, const int* __restrict__ col,
-                                       const double* __restrict__ val, long ld, const double* __restrict__ z,
-                                       const double* __restrict__ po, double* y, double* w)
+__device__ inline void row_spmv(int lane, int e0, int ne, int c0, int nc, const PcgArgs& A,
+                                const double* __restrict__ z, const double* __restrict__ po, double* y, double* w)
 {
-    const int k1 = rp[v + 1];
-    int k = rp[v] + lane;
-    auto block = [&](int kk) {
-        const double bk[9] = {val[kk], val[ld + kk], val[2 * ld + kk], val[3 * ld + kk], val[4 * ld + kk],
-                              val[5 * ld + kk], val[6 * ld + kk], val[7 * ld + kk], val[8 * ld + kk]};
-        const size_t c = 3 * (size_t)col[kk];
+    for (int j = lane; j < ne + nc; j += G) {
+        const bool el = j < ne;
+        const double* __restrict__ val = el ? A.eval : A.cval;
+        const long ld = el ? A.eld : A.cld;
+        const int k = el ? e0 + j : c0 + (j - ne);
+        const int col = el ? A.ecol[k] : A.ccol[k];
+        const double bk[9] = {val[k], val[ld + k], val[2 * ld + k], val[3 * ld + k], val[4 * ld + k],
+                              val[5 * ld + k], val[6 * ld + k], val[7 * ld + k], val[8 * ld + k]};
+        const size_t c = 3 * (size_t)col;
         const double x0 = z[c], x1 = z[c + 1], x2 = z[c + 2];
         y[0] += bk[0] * x0 + bk[1] * x1 + bk[2] * x2;
         y[1] += bk[3] * x0 + bk[4] * x1 + bk[5] * x2;
@@ -604,27 +622,57 @@ __device__ inline void group_row_spmv2(int v, int lane, const int* __restrict__ 
             w[1] += bk[3] * p0 + bk[4] * p1 + bk[5] * p2;
             w[2] += bk[6] * p0 + bk[7] * p1 + bk[8] * p2;
         }
-    };
-    for (; k + G < k1; k += 2 * G) {
-        block(k);
-        block(k + G);
     }
-    if (k < k1) block(k);
 }
 
-// x = 0, r = b, z = P b, partials of r.z and r.r (parity 0); one wavefront per block
+// row extents of row v in both block sets
+__device__ inline void row_extents(const PcgArgs& A, int v, int& e0, int& ne, int& c0, int& nc)
+{
+    e0 = A.erp[v];
+    ne = A.erp[v + 1] - e0;
+    c0 = A.crp ? A.crp[v] : 0;
+    nc = A.crp ? A.crp[v + 1] - c0 : 0;
+}
+
+template <int G>
+__device__ inline double row_butterfly(double* y, int lane)
+{
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        y[0] += __shfl_xor(y[0], o, G);
+        y[1] += __shfl_xor(y[1], o, G);
+        y[2] += __shfl_xor(y[2], o, G);
+    }
+    return lane == 0 ? y[0] : (lane == 1 ? y[1] : y[2]);
+}
+
+// sum of one double per thread of the row threads -> one partial per block
+__device__ inline double block_row_sum(double v, double* wsum)
+{
+    v = wave_sum(v);
+    if (threadIdx.x < PCG_ROW_THREADS && (threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double a = 0.0;
+#pragma unroll
+    for (int w2 = 0; w2 < PCG_ROW_THREADS / 64; ++w2) a += wsum[w2];
+    return a;
+}
+
+// x0 = gamma x_prev (gamma = b.x / x.Ax, the A-norm-closest multiple of the
+// previous direction) for warm instances, else 0; r0 = b - gamma A x_prev;
+// z0 = P r0; partials of r.z, r.r and b.b (the last in the second half of
+// the r.r partials, read by the first SpMV)
 __global__ void __launch_bounds__(PCG_UPD_BLOCK) k_pcg_init(PcgArgs A)
 {
-    const int inst = A.binst[blockIdx.x];
+    const int4 t = A.btab[blockIdx.x];
+    const int inst = t.z, bid = t.w;
     if (A.flag[inst] != 0) return;  // not solving: x is left as it is
-    const int v = A.blo[blockIdx.x] + threadIdx.x;
-    // warm start: x0 = gamma x_prev with gamma = b.x / x.Ax (the A-norm-closest
-    // multiple of the previous direction), r0 = b - gamma A x_prev
+    const int v = t.x + threadIdx.x;
     double gamma = 0.0;
     if (A.warm && A.warm[inst]) {
         const int ab = A.apbeg[inst], na = A.apbeg[inst + 1] - ab;
-        const double xq = wave_sum_partials(A.part_pq + ab, na);
-        const double bx = wave_sum_partials(A.part_bx + ab, na);
+        const double xq = wave_sum_partials<32>(A.part_pq + ab, na);
+        const double bx = wave_sum_partials<32>(A.part_bx + ab, na);
         gamma = xq > 0.0 ? bx / xq : 0.0;
     }
     const bool w = gamma != 0.0;
@@ -633,10 +681,10 @@ __global__ void __launch_bounds__(PCG_UPD_BLOCK) k_pcg_init(PcgArgs A)
         for (int i = 0; i < 3; ++i) o[i] = w ? A.b[3 * u + i] - gamma * A.q[3 * u + i] : A.b[3 * u + i];
     };
     double drz = 0.0, drr = 0.0, dbb = 0.0;
-    if (v < A.bhi[blockIdx.x]) {
+    if (v < t.y) {
         double rv[3], zv[3];
         res(v, rv);
-        cl_apply(v, A.croot, A.ccnt, A.cmem, A.zoff, A.Z, res, zv);
+        cl_apply(A.cm4[v], A.Z + (size_t)CL_ZS * v, rv, res, zv);
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             const double bi = A.b[3 * v + i];
@@ -652,41 +700,32 @@ __global__ void __launch_bounds__(PCG_UPD_BLOCK) k_pcg_init(PcgArgs A)
     drr = wave_sum(drr);
     dbb = wave_sum(dbb);
     if (threadIdx.x == 0) {
-        A.part_rz[blockIdx.x] = drz;
-        A.part_rr[blockIdx.x] = drr;
-        A.part_rr[A.nbB + blockIdx.x] = dbb;  // |b|^2, read by the first SpMV (that half is free then)
+        A.part_rz[bid] = drz;
+        A.part_rr[bid] = drr;
+        A.part_rr[A.nbB + bid] = dbb;
     }
 }
 
-// q = A p_it with p_it = z + beta p_(it-1), computed as A z + beta A p_(it-1)
-// so that the matrix stream does not wait for beta (the reduction of the
-// previous partials runs in wavefront 0 while the others load)
 // warm start: q = A x for the instances that start from their previous
 // direction, with per-block partials of x.q and b.x
 template <int G>
-__global__ void __launch_bounds__(PCG_SPMV_BLOCK) k_pcg_warm(PcgArgs A)
+__global__ void __launch_bounds__(PCG_ROW_THREADS) k_pcg_warm(PcgArgs A)
 {
-    __shared__ double wsum[2][PCG_SPMV_BLOCK / 64];
-    const int inst = A.ainst[blockIdx.x];
+    __shared__ double wsum[2][PCG_ROW_THREADS / 64];
+    const int4 t = A.atab[blockIdx.x];
+    const int inst = t.z;
     const bool on = A.warm[inst] && A.flag[inst] == 0;  // uniform over the block
     const int lane = threadIdx.x & (G - 1);
-    const int v = A.alo[blockIdx.x] + threadIdx.x / G;
-    const bool row = on && v < A.ahi[blockIdx.x];
+    const int v = t.x + threadIdx.x / G;
+    const bool row = on && threadIdx.x < PCG_ROW_THREADS && v < t.y;
     double y[3] = {0.0, 0.0, 0.0}, w[3] = {0.0, 0.0, 0.0};
-    if (row) {
-        group_row_spmv2<G>(v, lane, A.erp, A.ecol, A.eval, A.eld, A.x, nullptr, y, w);
-        if (A.crp) group_row_spmv2<G>(v, lane, A.crp, A.ccol, A.cval, A.cld, A.x, nullptr, y, w);
-    }
     double dxq = 0.0, dbx = 0.0;
     if (row) {
-#pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) {
-            y[0] += __shfl_xor(y[0], o, G);
-            y[1] += __shfl_xor(y[1], o, G);
-            y[2] += __shfl_xor(y[2], o, G);
-        }
+        int e0, ne, c0, nc;
+        row_extents(A, v, e0, ne, c0, nc);
+        row_spmv<G>(lane, e0, ne, c0, nc, A, A.x, nullptr, y, w);
+        const double yl = row_butterfly<G>(y, lane);
         if (lane < 3) {
-            const double yl = lane == 0 ? y[0] : (lane == 1 ? y[1] : y[2]);
             const size_t i = 3 * (size_t)v + lane;
             const double xi = A.x[i];
             A.q[i] = yl;
@@ -696,7 +735,7 @@ __global__ void __launch_bounds__(PCG_SPMV_BLOCK) k_pcg_warm(PcgArgs A)
     }
     dxq = wave_sum(dxq);
     dbx = wave_sum(dbx);
-    if ((threadIdx.x & 63) == 0) {
+    if (threadIdx.x < PCG_ROW_THREADS && (threadIdx.x & 63) == 0) {
         wsum[0][threadIdx.x >> 6] = dxq;
         wsum[1][threadIdx.x >> 6] = dbx;
     }
@@ -704,119 +743,130 @@ __global__ void __launch_bounds__(PCG_SPMV_BLOCK) k_pcg_warm(PcgArgs A)
     if (threadIdx.x == 0) {
         double a = 0.0, c = 0.0;
 #pragma unroll
-        for (int w2 = 0; w2 < PCG_SPMV_BLOCK / 64; ++w2) {
+        for (int w2 = 0; w2 < PCG_ROW_THREADS / 64; ++w2) {
             a += wsum[0][w2];
             c += wsum[1][w2];
         }
-        A.part_pq[blockIdx.x] = a;
-        A.part_bx[blockIdx.x] = c;
+        A.part_pq[t.w] = a;
+        A.part_bx[t.w] = c;
     }
 }
 
-template <int G>
-__global__ void __launch_bounds__(PCG_SPMV_BLOCK) k_pcg_spmv(PcgArgs A, int it)
+// SpMV step of iteration it.  The scalar wavefront re-reduces the instance's
+// partials of the previous update and decides convergence while the row
+// wavefronts stream their blocks; p_it = z + beta p_(it-1) is never read:
+// the rows accumulate A z and A p_(it-1) separately and combine them once
+// beta is known, q = A z + beta A p_(it-1).
+template <int G, bool SW>
+__global__ void __launch_bounds__(PCG_ROW_THREADS + (SW ? 64 : 0)) k_pcg_spmv(PcgArgs A, int it)
 {
-    __shared__ double sc[4];
-    __shared__ double wsum[PCG_SPMV_BLOCK / 64];
+    __shared__ double sbeta;
+    __shared__ double wsum[PCG_ROW_THREADS / 64];
     __shared__ int sflag;
     const int par = it & 1;
-    const int inst = A.ainst[blockIdx.x];
-    const bool skip = *(volatile int*)(A.flag + inst) != 0;  // one load per wavefront
+    const int4 t = A.atab[blockIdx.x];
+    const int inst = t.z, bid = t.w;
     const double* po = it > 0 ? A.pbuf + (size_t)(par ^ 1) * 3 * A.nV : nullptr;
-    const int lane = threadIdx.x & (G - 1);
-    const int v = A.alo[blockIdx.x] + threadIdx.x / G;
-    const bool row = v < A.ahi[blockIdx.x] && !skip;
     double y[3] = {0.0, 0.0, 0.0}, w[3] = {0.0, 0.0, 0.0};
-    if (threadIdx.x < 64) {
-        // wavefront 0: reduce the instance's partials of the previous update and decide
+    double zo = 0.0, poo = 0.0;
+    const int lane = threadIdx.x & (G - 1);
+    const int v = t.x + threadIdx.x / G;
+    bool row = false;
+    auto scalars = [&]() {
         const int f = *(volatile int*)(A.flag + inst);
         const int pb = A.bpbeg[inst], np = A.bpbeg[inst + 1] - pb;
         const double rz = wave_sum_partials(A.part_rz + par * A.nbB + pb, np);
         const double rr = wave_sum_partials(A.part_rr + par * A.nbB + pb, np);
         const double rzo = it > 0 ? wave_sum_partials(A.part_rz + (par ^ 1) * A.nbB + pb, np) : 1.0;
         const double bbp = it == 0 ? wave_sum_partials(A.part_rr + A.nbB + pb, np) : 0.0;
-        if (threadIdx.x == 0) {
+        if ((threadIdx.x & 63) == 0) {
             double* sci = A.scal + 3 * inst;
             const double bb = it == 0 ? bbp : sci[0];
             int stop = f != 0 ? 2 : 0;
             if (!stop && (!(bb > 0.0) || (it > 0 && rr <= A.rtol2 * bb) || it >= A.maxit)) {
                 stop = 1;
-                if (blockIdx.x == A.apbeg[inst]) {
+                if (bid == A.apbeg[inst]) {
                     A.flag[inst] = 1;
                     sci[0] = bb;
                     sci[1] = rr;
                     sci[2] = (double)it;
                 }
             }
-            if (!stop && it == 0 && blockIdx.x == A.apbeg[inst]) sci[0] = bb;
+            if (!stop && it == 0 && bid == A.apbeg[inst]) sci[0] = bb;
             sflag = stop;
-            sc[0] = it > 0 ? rz / rzo : 0.0;  // beta
+            sbeta = it > 0 ? rz / rzo : 0.0;
         }
+    };
+    const bool scalar_wave = SW ? threadIdx.x >= PCG_ROW_THREADS : threadIdx.x < 64;
+    if (!SW && scalar_wave) scalars();
+    if (threadIdx.x < PCG_ROW_THREADS && v < t.y) {
+        // the flag is loaded with the row extents; a stopped instance skips its blocks
+        const int f = *(volatile int*)(A.flag + inst);
+        int e0, ne, c0, nc;
+        row_extents(A, v, e0, ne, c0, nc);
+        if (lane < 3) {
+            zo = A.z[3 * v + lane];
+            poo = po ? po[3 * v + lane] : 0.0;
+        }
+        row = f == 0;
+        if (row) row_spmv<G>(lane, e0, ne, c0, nc, A, A.z, po, y, w);
     }
-    if (row) {
-        group_row_spmv2<G>(v, lane, A.erp, A.ecol, A.eval, A.eld, A.z, po, y, w);
-        if (A.crp) group_row_spmv2<G>(v, lane, A.crp, A.ccol, A.cval, A.cld, A.z, po, y, w);
-    }
+    if (SW && scalar_wave) scalars();
     __syncthreads();
     if (sflag != 0) return;  // block-uniform
-    const double beta = sc[0];
+    const double beta = sbeta;
     double dpq = 0.0;
     if (row) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) y[d] += beta * w[d];
-#pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) {
-            y[0] += __shfl_xor(y[0], o, G);
-            y[1] += __shfl_xor(y[1], o, G);
-            y[2] += __shfl_xor(y[2], o, G);
-        }
+        const double yl = row_butterfly<G>(y, lane);
         if (lane < 3) {
-            const double yl = lane == 0 ? y[0] : (lane == 1 ? y[1] : y[2]);
             const size_t i = 3 * (size_t)v + lane;
-            const double pv = po ? A.z[i] + beta * po[i] : A.z[i];
+            const double pv = po ? zo + beta * poo : zo;
             A.pbuf[(size_t)par * 3 * A.nV + i] = pv;
             A.q[i] = yl;
             dpq = pv * yl;
         }
     }
-    dpq = wave_sum(dpq);
-    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = dpq;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double a = 0.0;
-#pragma unroll
-        for (int w2 = 0; w2 < PCG_SPMV_BLOCK / 64; ++w2) a += wsum[w2];
-        A.part_pq[blockIdx.x] = a;
-    }
+    const double a = block_row_sum(dpq, wsum);
+    if (threadIdx.x == 0) A.part_pq[bid] = a;
 }
 
-// one wavefront per block: no barriers
+// Update step of iteration it: alpha = r.z / p.q; x += alpha p; r -= alpha q;
+// z = P r (cluster members' residuals recomputed from the unwritten half of
+// r); partials of the new r.z and r.r.  Every load that does not depend on
+// alpha, the cluster's member list and the preconditioner rows included, is
+// issued before the partials are reduced.
 __global__ void __launch_bounds__(PCG_UPD_BLOCK) k_pcg_update(PcgArgs A, int it)
 {
-    const int inst = A.binst[blockIdx.x];
-    if (*(volatile int*)(A.flag + inst) != 0) return;  // one load: uniform over the wavefront
+    const int4 t = A.btab[blockIdx.x];
+    const int inst = t.z, bid = t.w;
     const int par = it & 1;
     const double* p = A.pbuf + (size_t)par * 3 * A.nV;
     const double* r_in = A.r + (size_t)par * 3 * A.nV;  // not written by this kernel: members read it safely
     double* r_out = A.r + (size_t)(par ^ 1) * 3 * A.nV;
-    const int v = A.blo[blockIdx.x] + threadIdx.x;
-    const bool own = v < A.bhi[blockIdx.x];
-    // this vertex's vectors are loaded before alpha is known (independent loads)
-    double xv[3] = {0, 0, 0}, pv[3] = {0, 0, 0}, rv[3] = {0, 0, 0}, qv[3] = {0, 0, 0};
-    if (own) {
+    const int v = t.x + threadIdx.x;
+    const bool own = v < t.y;
+    const int vc = own ? v : t.x;
+    const int f = *(volatile int*)(A.flag + inst);
+    double xv[3], pv[3], rv[3], qv[3], o0[9];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            xv[i] = A.x[3 * v + i];
-            pv[i] = p[3 * v + i];
-            rv[i] = r_in[3 * v + i];
-            qv[i] = A.q[3 * v + i];
-        }
+    for (int i = 0; i < 3; ++i) {
+        xv[i] = A.x[3 * vc + i];
+        pv[i] = p[3 * vc + i];
+        rv[i] = r_in[3 * vc + i];
+        qv[i] = A.q[3 * vc + i];
     }
-    const double pq = wave_sum_partials(A.part_pq + A.apbeg[inst], A.apbeg[inst + 1] - A.apbeg[inst]);
+    const int4 cm = A.cm4[vc];
+    const double* o = A.Z + (size_t)CL_ZS * vc;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) o0[i] = o[i];
+    if (f != 0) return;  // uniform over the wavefront
+    const double pq = wave_sum_partials<32>(A.part_pq + A.apbeg[inst], A.apbeg[inst + 1] - A.apbeg[inst]);
     const int pb = A.bpbeg[inst];
     const double rz = wave_sum_partials(A.part_rz + par * A.nbB + pb, A.bpbeg[inst + 1] - pb);
     if (!(pq > 0.0)) {
-        if (blockIdx.x == pb && threadIdx.x == 0) {
+        if (bid == pb && threadIdx.x == 0) {
             A.flag[inst] = 2;
             A.scal[3 * inst + 2] = (double)it;
         }
@@ -831,20 +881,25 @@ __global__ void __launch_bounds__(PCG_UPD_BLOCK) k_pcg_update(PcgArgs A, int it)
             rv[i] = rv[i] - alpha * qv[i];
             r_out[3 * v + i] = rv[i];
         }
-        // z = M r with the members' updated residuals (the own one as above)
         double zv[3];
-        cl_apply(v, A.croot, A.ccnt, A.cmem, A.zoff, A.Z,
-                 [&](int u, double* o) {
-                     if (u == v) {
-                         o[0] = rv[0];
-                         o[1] = rv[1];
-                         o[2] = rv[2];
-                     } else {
+        if (cm.y < 0) {
 #pragma unroll
-                         for (int i = 0; i < 3; ++i) o[i] = r_in[3 * u + i] - alpha * A.q[3 * u + i];
-                     }
-                 },
-                 zv);
+            for (int i = 0; i < 3; ++i) zv[i] = o0[3 * i] * rv[0] + o0[3 * i + 1] * rv[1] + o0[3 * i + 2] * rv[2];
+        } else {
+            // z = M r with the members' updated residuals (the own one as above)
+            cl_apply(cm, o, rv,
+                     [&](int u, double* out) {
+                         if (u == v) {
+                             out[0] = rv[0];
+                             out[1] = rv[1];
+                             out[2] = rv[2];
+                         } else {
+#pragma unroll
+                             for (int i = 0; i < 3; ++i) out[i] = r_in[3 * u + i] - alpha * A.q[3 * u + i];
+                         }
+                     },
+                     zv);
+        }
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             A.z[3 * v + i] = zv[i];
@@ -855,8 +910,8 @@ __global__ void __launch_bounds__(PCG_UPD_BLOCK) k_pcg_update(PcgArgs A, int it)
     drz = wave_sum(drz);
     drr = wave_sum(drr);
     if (threadIdx.x == 0) {
-        A.part_rz[(par ^ 1) * A.nbB + blockIdx.x] = drz;
-        A.part_rr[(par ^ 1) * A.nbB + blockIdx.x] = drr;
+        A.part_rz[(par ^ 1) * A.nbB + bid] = drz;
+        A.part_rr[(par ^ 1) * A.nbB + bid] = drr;
     }
 }
 
@@ -1034,7 +1089,7 @@ void build_inst_layout(MeshDev& m, const std::vector<long>& vbeg, const std::vec
     I.vblk.build(vbeg, RB, s);
     I.nblk.build(nbeg, RB, s);
     I.tblk.build(tbeg, RB, s);
-    I.pcg_rows.build(vbeg, 32, s);
+    for (int gi = 0; gi < 4; ++gi) I.pcg_rows[gi].build(vbeg, spmv_rows(4 << gi), s);
     I.pcg_upd.build(vbeg, 64, s);
     std::vector<long> v3(vbeg.size());
     for (size_t i = 0; i < vbeg.size(); ++i) v3[i] = 3 * vbeg[i];
@@ -1068,7 +1123,6 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
     const int B = I.B;
     const int nV = m.nV;
     const double nnzb = (double)m.nnzb_e + (double)L.ncb;
-    constexpr int G = 8;  // lanes per block row (32-row SpMV blocks)
     int* err = L.err.ensure(2 * (size_t)B + 2, s);
     int* flag = err + B;
     IDP_LAUNCH(k_pcg_start, grid_for(B, 64), 64, 0, s, B, solve, flag, err);
@@ -1083,10 +1137,9 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
     int* root2 = L.cl_root2.ensure(nV, s);
     int* cnt = L.cl_cnt.ensure(nV, s);
     int* mem = L.cl_mem.ensure(CL_MAX * (size_t)nV, s);
-    int* zlen = L.cl_zlen.ensure(nV + 1, s);
-    int* zoff = L.cl_zoff.ensure(nV + 1, s);
     unsigned long long* pack = L.cl_pack.ensure(nV, s);
-    double* Z = L.pinv.ensure(9 * CL_MAX * (size_t)nV, s);
+    double* Z = L.pinv.ensure(CL_ZS * (size_t)nV, s);
+    int4* cm4 = (int4*)L.cl_cm4.ensure(4 * (size_t)nV, s);
     const int* crp = haveC ? L.crp.p : nullptr;
     const int* crp_match = L.precond == 0 ? crp : nullptr;  // no contact rows: singletons (3x3 block Jacobi)
     IDP_LAUNCH(k_cl_best, vg, 256, 0, s, nV, crp_match, L.ccol.p, L.cvals.p, L.ncb, best);
@@ -1096,17 +1149,11 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
     IDP_LAUNCH(k_cl_merge, vg, 256, 0, s, nV, root, pack, size, root2);
     IDP_HIP(hipMemsetAsync(cnt, 0, nV * sizeof(int), s));
     IDP_LAUNCH(k_cl_members, vg, 256, 0, s, nV, root2, cnt, mem);
-    IDP_LAUNCH(k_cl_sort, vg, 256, 0, s, nV, root2, cnt, mem, zlen);
-    {
-        size_t bytes = 0;
-        IDP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, zlen, zoff, nV, s));
-        L.tmp.ensure(bytes + 16, s);
-        IDP_HIP(hipcub::DeviceScan::ExclusiveSum(L.tmp.p, bytes, zlen, zoff, nV, s));
-    }
+    IDP_LAUNCH(k_cl_sort, vg, 256, 0, s, nV, root2, cnt, mem);
     double* work = L.cl_work.ensure(2 * 144 * (size_t)nV, s);
     IDP_LAUNCH(k_cl_build, vg, 256, 0, s, nV, root2, cnt, mem, m.e_diag.p, m.e_rowptr.p, m.e_col.p, L.evals.p,
-               m.nnzb_e, haveC ? L.cdiag.p : (const int*)nullptr, crp, L.ccol.p, L.cvals.p, L.ncb, I.vinst.p, zoff,
-               Z, work, err);
+               m.nnzb_e, haveC ? L.cdiag.p : (const int*)nullptr, crp, L.ccol.p, L.cvals.p, L.ncb, I.vinst.p, Z,
+               cm4, work, err);
     PcgArgs A;
     A.nV = nV;
     A.erp = m.e_rowptr.p;
@@ -1119,25 +1166,46 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
     A.cld = L.ncb;
     A.b = b;
     A.Z = Z;
-    A.croot = root2;
-    A.ccnt = cnt;
-    A.cmem = mem;
-    A.zoff = zoff;
+    A.cm4 = cm4;
     A.x = x;
     A.r = L.r.ensure(6 * (size_t)nV, s);
     A.z = L.z.ensure(3 * (size_t)nV, s);
     A.pbuf = L.p.ensure(6 * (size_t)nV, s);
     A.q = L.q.ensure(3 * (size_t)nV, s);
-    A.nbA = I.pcg_rows.nblk;
+    const SpmvShape shape = SPMV_SHAPES[L.spmv_shape];
+    const int gi = shape.g == 4 ? 0 : shape.g == 8 ? 1 : shape.g == 16 ? 2 : 3;
+    const SegDev& rows = I.pcg_rows[gi];
+    A.nbA = rows.nblk;
     A.nbB = I.pcg_upd.nblk;
-    A.alo = I.pcg_rows.lo.p;
-    A.ahi = I.pcg_rows.hi.p;
-    A.ainst = I.pcg_rows.inst.p;
-    A.apbeg = I.pcg_rows.pbeg.p;
-    A.blo = I.pcg_upd.lo.p;
-    A.bhi = I.pcg_upd.hi.p;
-    A.binst = I.pcg_upd.inst.p;
+    A.apbeg = rows.pbeg.p;
     A.bpbeg = I.pcg_upd.pbeg.p;
+    // launch tables over the instances that solve (re-made as instances stop)
+    std::vector<int> hf(2 * B);
+    IDP_HIP(hipMemcpyAsync(hf.data(), err, 2 * B * sizeof(int), hipMemcpyDeviceToHost, s));
+    IDP_HIP(hipStreamSynchronize(s));
+    std::vector<char> run(B);
+    for (int i = 0; i < B; ++i) run[i] = hf[B + i] == 0;
+    int nA = 0, nB = 0;
+    double frac = 1.0;  // launched share of the rows (exact for one instance, row-proportional otherwise)
+    auto tables = [&]() {
+        auto fill = [&](const SegDev& sd, std::vector<int4>& out) {
+            out.clear();
+            for (int i = 0; i < B; ++i)
+                if (run[i])
+                    for (int k = sd.h_pbeg[i]; k < sd.h_pbeg[i + 1]; ++k)
+                        out.push_back(make_int4(sd.h_lo[k], sd.h_hi[k], i, k));
+        };
+        fill(rows, L.h_atab);
+        fill(I.pcg_upd, L.h_btab);
+        nA = (int)L.h_atab.size();
+        nB = (int)L.h_btab.size();
+        long rows = 0;
+        for (const int4& t : L.h_btab) rows += t.y - t.x;
+        frac = nV ? (double)rows / nV : 0.0;
+        A.atab = L.atab.upload_get(L.h_atab.data(), L.h_atab.size(), s);
+        A.btab = L.btab.upload_get(L.h_btab.data(), L.h_btab.size(), s);
+    };
+    tables();
     A.part_pq = L.part0.ensure(A.nbA + 1, s);
     A.part_bx = L.part_bx.ensure(A.nbA + 1, s);
     A.warm = warm;
@@ -1147,33 +1215,57 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
     A.scal = L.scal.ensure(3 * (size_t)B + 3, s);
     A.rtol2 = rtol * rtol;
     A.maxit = maxit;
-    if (warm) IDP_LAUNCH(k_pcg_warm<G>, A.nbA, PCG_SPMV_BLOCK, 0, s, A);
-    IDP_LAUNCH(k_pcg_init, A.nbB, PCG_UPD_BLOCK, 0, s, A);
+    if (warm && nA) {
+        if (shape.g == 4) IDP_LAUNCH(k_pcg_warm<4>, nA, PCG_ROW_THREADS, 0, s, A);
+        else if (shape.g == 8) IDP_LAUNCH(k_pcg_warm<8>, nA, PCG_ROW_THREADS, 0, s, A);
+        else if (shape.g == 16) IDP_LAUNCH(k_pcg_warm<16>, nA, PCG_ROW_THREADS, 0, s, A);
+        else IDP_LAUNCH(k_pcg_warm<32>, nA, PCG_ROW_THREADS, 0, s, A);
+    }
+    auto spmv = [&](int it) {
+        const int nt = PCG_ROW_THREADS + (shape.sw ? 64 : 0);
+        switch (L.spmv_shape) {
+        case 0: IDP_LAUNCH((k_pcg_spmv<4, false>), nA, nt, 0, s, A, it); break;
+        case 1: IDP_LAUNCH((k_pcg_spmv<8, false>), nA, nt, 0, s, A, it); break;
+        case 2: IDP_LAUNCH((k_pcg_spmv<16, false>), nA, nt, 0, s, A, it); break;
+        case 3: IDP_LAUNCH((k_pcg_spmv<32, false>), nA, nt, 0, s, A, it); break;
+        case 4: IDP_LAUNCH((k_pcg_spmv<4, true>), nA, nt, 0, s, A, it); break;
+        case 5: IDP_LAUNCH((k_pcg_spmv<8, true>), nA, nt, 0, s, A, it); break;
+        case 6: IDP_LAUNCH((k_pcg_spmv<16, true>), nA, nt, 0, s, A, it); break;
+        default: IDP_LAUNCH((k_pcg_spmv<32, true>), nA, nt, 0, s, A, it); break;
+        }
+    };
+    if (nB) IDP_LAUNCH(k_pcg_init, nB, PCG_UPD_BLOCK, 0, s, A);
     // algorithmic bytes of one SpMV launch (blocks + columns, two row-pointer
     // arrays, z and p_old read, p_new and q written) and of one update launch
     // (x, p, r, q, P^-1 read; x, r, z written)
     const double spmv_bytes = 76.0 * nnzb + 8.0 * (nV + 1) + 96.0 * nV;
     const double upd_bytes = 240.0 * nV;
-    std::vector<int> hf(2 * B);
     std::vector<double> hs(3 * B);
     int it = 0, chunk = 8;
-    for (;;) {
+    for (; nA > 0;) {
         for (int j = 0; j < chunk && it <= maxit; ++j, ++it) {
             // per-kernel events on one iteration in 8 (an event pair between
             // every launch would itself widen the launch gaps)
             const bool sample = P && (it & 7) == 3;
             int t = sample ? P->begin(IDP_K_PCG_SPMV) : -1;
-            IDP_LAUNCH(k_pcg_spmv<G>, A.nbA, PCG_SPMV_BLOCK, 0, s, A, it);
-            if (sample) P->end(IDP_K_PCG_SPMV, t, spmv_bytes, it);
+            spmv(it);
+            if (sample) P->end(IDP_K_PCG_SPMV, t, spmv_bytes * frac, it);
             t = sample ? P->begin(IDP_K_PCG_UPDATE) : -1;
-            IDP_LAUNCH(k_pcg_update, A.nbB, PCG_UPD_BLOCK, 0, s, A, it);
-            if (sample) P->end(IDP_K_PCG_UPDATE, t, upd_bytes, it);
+            IDP_LAUNCH(k_pcg_update, nB, PCG_UPD_BLOCK, 0, s, A, it);
+            if (sample) P->end(IDP_K_PCG_UPDATE, t, upd_bytes * frac, it);
         }
         IDP_HIP(hipMemcpyAsync(hf.data(), err, 2 * B * sizeof(int), hipMemcpyDeviceToHost, s));
         IDP_HIP(hipStreamSynchronize(s));
-        bool all = true;
-        for (int i = 0; i < B; ++i) all &= hf[B + i] != 0;
+        bool all = true, fewer = false;
+        for (int i = 0; i < B; ++i) {
+            all &= hf[B + i] != 0;
+            if (run[i] && hf[B + i] != 0) {
+                run[i] = 0;
+                fewer = true;
+            }
+        }
         if (all || it > maxit) break;
+        if (fewer) tables();  // the stream is idle here: the tables may be rewritten
         chunk = std::min(chunk * 2, 256);
     }
     if (P) P->end(IDP_K_PCG, tok);

@@ -14,6 +14,9 @@ struct LinWork {
     DBuf<unsigned long long> key0, key1;
     long ncb = 0, nrec = 0, ncs = 0;
     int precond = 0;  // 0: contact-cluster block Jacobi, 1: 3x3 block Jacobi
+    int spmv_shape = 5;  // PCG SpMV launch shape: 8 lanes per row, scalars in a wavefront of their own
+    DBuf<int4> atab, btab;  // PCG launch tables
+    std::vector<int4> h_atab, h_btab;
     // values
     DBuf<int> erow;  // row of every elastic block
     DBuf<double> evals, cvals;
@@ -21,7 +24,7 @@ struct LinWork {
     DBuf<double> pinv, r, z, p, p1, q, part0, part1, part2, part3, part_bx, scal, scal2;
     DBuf<int> err;
     // contact-cluster preconditioner
-    DBuf<int> cl_best, cl_root, cl_size, cl_root2, cl_cnt, cl_mem, cl_zlen, cl_zoff;
+    DBuf<int> cl_best, cl_root, cl_size, cl_root2, cl_cnt, cl_mem, cl_cm4;
     DBuf<unsigned long long> cl_pack;
     DBuf<double> cl_work;
     Profiler* prof = nullptr;

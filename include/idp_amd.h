@@ -44,6 +44,9 @@ typedef struct idp_solver_opts {
     int precond;      /* 0: contact-cluster block Jacobi (default), 1: 3x3 block Jacobi */
     int pcg_cold_start; /* 1: every PCG solve starts from zero; 0 (default): from the second Newton
                            iteration of a step on, from the best multiple of the previous direction */
+    int accd_budget;    /* batched engine: ACCD steps a pair may take per Newton iteration before it is
+                           handed over to the next one (its instance waits, the others go on); 0: the
+                           default (16384), < 0: never.  Results do not depend on it. */
 } idp_solver_opts;
 
 typedef struct idp_step_stats {
@@ -203,7 +206,9 @@ int idp_serve_configure(idp_ctx* ctx, double h, int with_collision, double thick
 int idp_serve_start(idp_ctx* ctx, int n, const int* slots, const double* X, const double* v,
                     const double* newton_tol, const double* dHat2, const double* kappa3);
 /* one Newton iteration of every busy slot; finished[B] (may be null) flags
- * the slots whose solve ended in this call; *n_active = busy slots left */
+ * the slots whose solve ended in this call; *n_active = busy slots left.  A
+ * slot whose CCD pair ran past the step budget (idp_solver_opts.accd_budget)
+ * continues that pair in the next call before its line search. */
 int idp_serve_iterate(idp_ctx* ctx, int* n_active, int* finished);
 /* result of a slot (positions, velocities, kappa3, stats; any may be null) */
 int idp_serve_result(idp_ctx* ctx, int slot, double* X, double* v, double* kappa3, idp_step_stats* stats);

@@ -67,3 +67,22 @@ def test_batch_matches_oracle():
         err = np.abs(X - Xo).max() / sc.diag
         say(f"[batch] {sc.name}: gpu newton {its[i]} oracle {it} err/diag {err:.2e}")
         assert err < 1e-6
+
+
+def test_batch_accd_handover_changes_nothing():
+    """A tiny ACCD step budget hands long pairs from one Newton iteration to
+    the next (the instance waits while the others go on); the results must be
+    those of a batch that never hands over, bit for bit."""
+    mix = _mix()
+    say("[batch] ACCD hand-over: budget 4 vs never")
+    a = driver.BatchAnimFix(mix)
+    ia, _ = a.advance_one_time_step(accd_budget=4)
+    b = driver.BatchAnimFix(mix)
+    ib, _ = b.advance_one_time_step(accd_budget=-1)
+    say(f"[batch] newton {list(ia)} vs {list(ib)}")
+    assert list(ia) == list(ib)
+    for i in range(len(mix)):
+        Xa, va = a.state(i)
+        Xb, vb = b.state(i)
+        assert np.array_equal(Xa, Xb) and np.array_equal(va, vb)
+        assert a.kappa[i, 0] == b.kappa[i, 0]
