@@ -171,6 +171,21 @@ def bench_batch(a):
         log(f"[bench] plain instance {i}: {it} Newton iterations, {tp:.2f} s so far")
     plain = len(idx) / tp
     log(f"[bench] plain path: {len(idx)} instances in {tp:.2f} s")
+    # the batch cannot end before its slowest instance: time that one alone too
+    slow = int(np.argmax(its))
+    d = driver.TetAnimFix(scs[slow], pcg_rtol=a.pcg_rtol)
+    d.ctx.snapshot()
+    d.advance_one_time_step(max_newton=2)
+    d.ctx.restore()
+    d.kappa[:] = k0[slow]
+    t0 = time.perf_counter()
+    it_slow, st_slow = d.advance_one_time_step(max_newton=a.max_newton, verbose=200)
+    t_slow = time.perf_counter() - t0
+    log(f"[bench] slowest instance {slow} ({scs[slow].name}) alone: {it_slow} Newton iterations, "
+        f"{st_slow['pcg_iters']} PCG iterations, {t_slow:.2f} s")
+    top = np.argsort(-np.asarray(its))[:5]
+    per_inst = [{"instance": int(i), "scene": scs[i].name, "vertices": int(scs[i].X.shape[0]),
+                 "newton": int(its[i]), "pcg": int(st[i]["pcg_iters"])} for i in top]
     lat_ms = np.array(lat) * 1e3
     out = {
         "metric": "solves/sec + p50/p99 latency at fixed accuracy; barrier pair-evals/sec vs plain",
@@ -190,6 +205,9 @@ def bench_batch(a):
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
         "plain_path": {"solves_per_s": plain, "sample": f"{len(idx)} evenly spaced instances, one at a time"},
         "batched_vs_plain": batched / plain,
+        "slowest_alone": {"instance": slow, "newton": int(it_slow), "pcg": int(st_slow["pcg_iters"]),
+                          "seconds": t_slow},
+        "slowest_in_batch": per_inst,
         "kernels": kern,
     }
     print(json.dumps(out), flush=True)
@@ -339,7 +357,7 @@ def bench_serve(a):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--workload", default="bar_twist_fold",
                     choices=["bar_twist_fold", "bar_bend", "cubes256", "serve"])

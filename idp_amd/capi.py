@@ -52,6 +52,8 @@ PCG_COLD = int(os.environ.get("IDP_PCG_COLD", "0"))
 # batched engine: ACCD steps per pair and Newton iteration before a hand-over
 # (0 default, < 0 never; IDP_ACCD_BUDGET)
 ACCD_BUDGET = int(os.environ.get("IDP_ACCD_BUDGET", "0"))
+# batched engine: PCG hand-over (0 adaptive default, < 0 never; IDP_PCG_HANDOVER)
+PCG_HANDOVER = int(os.environ.get("IDP_PCG_HANDOVER", "0"))
 
 K_FAMILIES = ["pcg", "barrier_hess", "tet_hess", "assembly", "constraint_set", "ccd", "energy", "gradient",
               "pcg_spmv", "pcg_update"]
@@ -65,7 +67,8 @@ class IdpError(RuntimeError):
 
 class SolverOpts(C.Structure):
     _fields_ = [("pcg_rtol", C.c_double), ("pcg_maxit", C.c_int), ("max_newton", C.c_int), ("verbose", C.c_int),
-                ("precond", C.c_int), ("pcg_cold_start", C.c_int), ("accd_budget", C.c_int)]
+                ("precond", C.c_int), ("pcg_cold_start", C.c_int), ("accd_budget", C.c_int),
+                ("pcg_handover", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -278,11 +281,12 @@ class Context:
                                              tp, ibp, vp, lp, mp, msp))
 
     def advance_batch(self, h, tol, dHat2, kappa3, thickness=0.0, with_collision=True, pcg_rtol=1e-8,
-                      pcg_maxit=10000, max_newton=0, verbose=0, accd_budget=None):
+                      pcg_maxit=10000, max_newton=0, verbose=0, accd_budget=None, pcg_handover=None):
         """One step of every instance; kappa3 (B,3) is updated in place.
         verbose > 0 prints a progress line to stderr every `verbose` Newton iterations;
         accd_budget: ACCD steps per pair and Newton iteration before a hand-over
-        (None: the process default, 0: the library default, < 0: never).
+        (None: the process default, 0: the library default, < 0: never);
+        pcg_handover: None the process default, 0 adaptive, < 0 never.
         Returns (newton iterations (B,), list of per-instance stats dicts)."""
         B = len(self.batch_nV)
         tol = np.ascontiguousarray(tol, dtype=np.float64)
@@ -292,7 +296,8 @@ class Context:
         its = np.zeros(B, np.int32)
         st = (StepStats * B)()
         opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, verbose, PRECOND, PCG_COLD,
-                          ACCD_BUDGET if accd_budget is None else accd_budget)
+                          ACCD_BUDGET if accd_budget is None else accd_budget,
+                          PCG_HANDOVER if pcg_handover is None else pcg_handover)
         self._check(lib().idp_advance_batch(self._h, h, tol.ctypes.data_as(dp), int(with_collision),
                                             d2.ctypes.data_as(dp), kappa3.ctypes.data_as(dp), thickness,
                                             C.byref(opts), its.ctypes.data_as(ip), st))
@@ -300,9 +305,10 @@ class Context:
 
     # ---- continuous batching over the slots of a batch mesh ----------------
     def serve_configure(self, h, with_collision=True, thickness=0.0, pcg_rtol=1e-8, pcg_maxit=10000, max_newton=0,
-                        accd_budget=None):
+                        accd_budget=None, pcg_handover=None):
         opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, 0, PRECOND, PCG_COLD,
-                          ACCD_BUDGET if accd_budget is None else accd_budget)
+                          ACCD_BUDGET if accd_budget is None else accd_budget,
+                          PCG_HANDOVER if pcg_handover is None else pcg_handover)
         self._check(lib().idp_serve_configure(self._h, h, int(with_collision), thickness, C.byref(opts)))
 
     def serve_start(self, slots, X, v, tol, dHat2, kappa3):
@@ -372,7 +378,7 @@ class Context:
         kappa_arr = np.asarray(kappa, dtype=np.float64)
         if not kappa_arr.flags["C_CONTIGUOUS"]:
             raise ValueError("kappa must be contiguous float64")
-        opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, verbose, PRECOND, PCG_COLD, ACCD_BUDGET)
+        opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, verbose, PRECOND, PCG_COLD, ACCD_BUDGET, PCG_HANDOVER)
         st = StepStats()
         it = self._check(lib().idp_advance_one_step(self._h, h, tol, int(with_collision), dHat2,
                                                     kappa_arr.ctypes.data_as(dp), thickness, C.byref(opts),
@@ -455,7 +461,7 @@ class Context:
         sol, sp = _out_d(X.size)
         rel = C.c_double()
         it = C.c_int()
-        opts = SolverOpts(pcg_rtol, pcg_maxit, 0, 0, PRECOND, PCG_COLD, ACCD_BUDGET)
+        opts = SolverOpts(pcg_rtol, pcg_maxit, 0, 0, PRECOND, PCG_COLD, ACCD_BUDGET, PCG_HANDOVER)
         self._check(lib().idp_newton_direction(self._h, Xp, xtp, h, dHat2, kappa, thickness, C.byref(opts), gp, sp,
                                                C.byref(rel), C.byref(it)))
         return g, sol, rel.value, it.value

@@ -424,13 +424,15 @@ struct BatchEngine {
     std::vector<double> kappa, kmax, tol, dHat2, th, Eprev, alpha, feasible, E;
     std::vector<std::deque<double>> res;
     std::vector<char> useGD, active, finished_now;
-    std::vector<char> pend;  // a CCD pair of this instance was handed over to the next call
+    std::vector<char> pend;   // a CCD pair of this instance was handed over to the next call
+    std::vector<char> ppend;  // its PCG solve was handed over to the next call
     std::vector<int> PN, ndbc;
     std::vector<idp_step_stats> S;
     std::vector<unsigned char> mask;
     DBuf<unsigned char> dmask, dmask2, dmask3;
     bool cold = false;  // every PCG solve from zero
     int accd_budget = ACCD_STEP_BUDGET;  // 0: no hand-over
+    bool pcg_handover = true;
     DBuf<double> dalpha;
     std::vector<EnergyOut> eo;
 
@@ -447,6 +449,7 @@ struct BatchEngine {
         c->L.precond = opts ? opts->precond : 0;
         cold = opts && opts->pcg_cold_start;
         accd_budget = !opts || opts->accd_budget == 0 ? ACCD_STEP_BUDGET : (opts->accd_budget < 0 ? 0 : opts->accd_budget);
+        pcg_handover = !opts || opts->pcg_handover >= 0;
         kappa.assign(B, 0);
         kmax.assign(B, 0);
         tol.assign(B, 0);
@@ -459,6 +462,7 @@ struct BatchEngine {
         res.assign(B, {});
         useGD.assign(B, 0);
         pend.assign(B, 0);
+        ppend.assign(B, 0);
         active.assign(B, 0);
         finished_now.assign(B, 0);
         PN.assign(B, 0);
@@ -513,6 +517,7 @@ struct BatchEngine {
             res[i].clear();
             useGD[i] = 0;
             pend[i] = 0;
+            ppend[i] = 0;
             PN[i] = 0;
             std::memset(&S[i], 0, sizeof(idp_step_stats));
         }
@@ -587,13 +592,17 @@ struct BatchEngine {
         double* g = c->g.p;
         std::fill(finished_now.begin(), finished_now.end(), 0);
         // fresh: instances that begin a Newton iteration now
-        std::vector<char> fresh(B, 0);
+        // (instances with a handed-over PCG solve or CCD pair continue that first)
+        std::vector<char> fresh(B, 0), was_ppend(B, 0), was_pend(B, 0);
         int nfresh = 0;
-        bool anyres = false;
+        bool anyres = false, anycont = false;
         for (int i = 0; i < B; ++i) {
-            fresh[i] = active[i] && !pend[i];
+            fresh[i] = active[i] && !pend[i] && !ppend[i];
+            was_ppend[i] = active[i] && ppend[i];
+            was_pend[i] = active[i] && pend[i];
             nfresh += fresh[i];
-            anyres |= active[i] && pend[i];
+            anyres |= was_pend[i] != 0;
+            anycont |= was_ppend[i] != 0;
         }
         for (int i = 0; i < B; ++i) mask[i] = fresh[i];
         const unsigned char* dact = up_mask();
@@ -602,7 +611,7 @@ struct BatchEngine {
         if (coll) contact_structure(c->L, s, m.nV, c->cw.ncs, (const int*)c->cw.cs.p);
         else c->L.ncb = 0;
         if (nfresh) gradient(c, x, h, coll, g, rhs);
-        bool anyN = false;
+        bool anyN = anycont;
         for (int i = 0; i < B; ++i) anyN |= fresh[i] && !useGD[i];
         std::vector<int> its(B, 0);
         std::vector<double> rel(B, 0.0);
@@ -611,13 +620,15 @@ struct BatchEngine {
             std::vector<unsigned char> sm(B);
             std::vector<unsigned char> wm(B);
             for (int i = 0; i < B; ++i) {
-                sm[i] = fresh[i] && !useGD[i];
-                wm[i] = sm[i] && PN[i] > 0 && !cold;  // sol holds this step's previous direction
+                sm[i] = fresh[i] && !useGD[i] ? 1 : (was_ppend[i] ? 2 : 0);
+                wm[i] = sm[i] == 1 && PN[i] > 0 && !cold;  // sol holds this step's previous direction
             }
-            pcg_solve_batch(c->L, s, m, rhs, sol, rtol, maxit, dmask2.upload_get(sm.data(), B, s), its.data(),
-                            rel.data(), dmask3.upload_get(wm.data(), B, s));
+            pcg_solve_batch(c->L, s, m, rhs, sol, rtol, maxit, sm.data(), its.data(), rel.data(),
+                            dmask3.upload_get(wm.data(), B, s), pcg_handover);
             for (int i = 0; i < B; ++i) {
                 if (!sm[i]) continue;
+                ppend[i] = its[i] == PCG_HANDED_OVER;
+                if (ppend[i]) continue;
                 if (its[i] < 0) useGD[i] = 1;  // DIRECT_SOLVER.h:42-44 -> gradient descent
                 else {
                     S[i].pcg_iters += its[i];
@@ -625,21 +636,27 @@ struct BatchEngine {
                 }
             }
         }
-        for (int i = 0; i < B; ++i) mask[i] = fresh[i] && useGD[i];
+        // ls: instances that go on to the line search in this call
+        std::vector<char> ls(B, 0);
+        int nls = 0;
+        for (int i = 0; i < B; ++i) {
+            ls[i] = (fresh[i] || was_ppend[i]) && !ppend[i];
+            nls += ls[i];
+        }
+        for (int i = 0; i < B; ++i) mask[i] = ls[i] && useGD[i];
         IDP_LAUNCH(k_copy_inst, vgrid, 256, 0, s, m.nV, I.vinst.p, up_mask(), rhs, sol);
         const double tsolve = now_ms() - t0;
         for (int i = 0; i < B; ++i) {
-            if (!fresh[i]) continue;
-            if (useGD[i]) S[i].gd_iters++;
-            S[i].ms_solve += tsolve;
+            if (fresh[i] || was_ppend[i]) S[i].ms_solve += tsolve;
+            if (ls[i] && useGD[i]) S[i].gd_iters++;
         }
         // ---- Line_Search, part 1: halve until no tet inverts
-        for (int i = 0; i < B; ++i) mask[i] = fresh[i];
+        for (int i = 0; i < B; ++i) mask[i] = ls[i];
         dact = up_mask();
         IDP_LAUNCH(k_copy_inst, vgrid, 256, 0, s, m.nV, I.vinst.p, dact, x, xprev);
         for (int i = 0; i < B; ++i)
-            if (fresh[i]) alpha[i] = 1.0;
-        std::vector<unsigned char> srch(fresh.begin(), fresh.end());
+            if (ls[i]) alpha[i] = 1.0;
+        std::vector<unsigned char> srch(ls.begin(), ls.end());
         for (int guard = 0;; ++guard) {
             if (guard > 4000) return IDP_E_INVALID;
             bool any = false;
@@ -655,22 +672,22 @@ struct BatchEngine {
                     else alpha[i] /= 2.0;
                 }
         }
-        for (int i = 0; i < B; ++i) mask[i] = fresh[i];
+        for (int i = 0; i < B; ++i) mask[i] = ls[i];
         dact = up_mask();
         IDP_LAUNCH(k_copy_inst, vgrid, 256, 0, s, m.nV, I.vinst.p, dact, xprev, x);
         // go: instances whose step size is final in this call
-        std::vector<char> go(fresh);
+        std::vector<char> go(ls);
         if (coll) {
             t0 = now_ms();
             long nc = 0;
             std::vector<double> a2(alpha);
             std::vector<unsigned char> pnow(B, 0);
             // with nothing else in flight a handed-over pair just runs to its end
-            const int budget = nfresh > 0 ? accd_budget : 0;
+            const int budget = nls > 0 ? accd_budget : 0;
             contact_ccd_batch(c->cw, s, m, x, sol, thickness, a2.data(), dact, &nc, budget, anyres, pnow.data());
             const double tc = now_ms() - t0;
             for (int i = 0; i < B; ++i)
-                if (active[i]) {
+                if (ls[i] || was_pend[i]) {  // took part in this CCD
                     go[i] = !pnow[i];
                     if (go[i]) alpha[i] = a2[i];
                     pend[i] = pnow[i];

@@ -545,7 +545,9 @@ struct PcgArgs {
     // by the block id, so an instance's partial range [apbeg, bpbeg) is the
     // same whatever else is launched.
     const int4 *atab, *btab;
+    const int *aoff, *boff;  // per launched block: iterations its instance did in earlier calls
     const int *apbeg, *bpbeg;
+    const unsigned char* solve;  // per instance: 0 not solving, 1 new solve, 2 continued from an earlier call
     int* flag;        // per instance: 0 running, 1 stopped (converged or maxit), 2 breakdown, 3 not solving
     double* scal;     // per instance, 3 each: |b|^2, final |r|^2, iterations
     double rtol2;
@@ -666,7 +668,7 @@ __global__ void __launch_bounds__(PCG_UPD_BLOCK) k_pcg_init(PcgArgs A)
 {
     const int4 t = A.btab[blockIdx.x];
     const int inst = t.z, bid = t.w;
-    if (A.flag[inst] != 0) return;  // not solving: x is left as it is
+    if (A.flag[inst] != 0 || (A.solve && A.solve[inst] != 1)) return;  // not solving, or continued
     const int v = t.x + threadIdx.x;
     double gamma = 0.0;
     if (A.warm && A.warm[inst]) {
@@ -758,11 +760,12 @@ __global__ void __launch_bounds__(PCG_ROW_THREADS) k_pcg_warm(PcgArgs A)
 // the rows accumulate A z and A p_(it-1) separately and combine them once
 // beta is known, q = A z + beta A p_(it-1).
 template <int G, bool SW>
-__global__ void __launch_bounds__(PCG_ROW_THREADS + (SW ? 64 : 0)) k_pcg_spmv(PcgArgs A, int it)
+__global__ void __launch_bounds__(PCG_ROW_THREADS + (SW ? 64 : 0)) k_pcg_spmv(PcgArgs A, int it_call)
 {
     __shared__ double sbeta;
     __shared__ double wsum[PCG_ROW_THREADS / 64];
     __shared__ int sflag;
+    const int it = it_call + A.aoff[blockIdx.x];  // the instance's own iteration
     const int par = it & 1;
     const int4 t = A.atab[blockIdx.x];
     const int inst = t.z, bid = t.w;
@@ -837,8 +840,9 @@ __global__ void __launch_bounds__(PCG_ROW_THREADS + (SW ? 64 : 0)) k_pcg_spmv(Pc
 // r); partials of the new r.z and r.r.  Every load that does not depend on
 // alpha, the cluster's member list and the preconditioner rows included, is
 // issued before the partials are reduced.
-__global__ void __launch_bounds__(PCG_UPD_BLOCK) k_pcg_update(PcgArgs A, int it)
+__global__ void __launch_bounds__(PCG_UPD_BLOCK) k_pcg_update(PcgArgs A, int it_call)
 {
+    const int it = it_call + A.boff[blockIdx.x];  // the instance's own iteration
     const int4 t = A.btab[blockIdx.x];
     const int inst = t.z, bid = t.w;
     const int par = it & 1;
@@ -1107,7 +1111,7 @@ __global__ void k_pcg_start(int B, const unsigned char* __restrict__ solve, int*
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= B) return;
-    flag[i] = (solve && !solve[i]) ? 3 : 0;
+    flag[i] = (solve && !solve[i]) ? 3 : 0;  // new and continued solves iterate
     err[i] = 0;
 }
 
@@ -1117,11 +1121,19 @@ __global__ void k_pcg_start(int B, const unsigned char* __restrict__ solve, int*
 // Per instance on return: iters[i] >= 0, or -1 (singular diagonal block) /
 // -2 (breakdown, p.q <= 0); relres[i] = |r|/|b|.
 void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* b, double* x, double rtol,
-                     int maxit, const unsigned char* solve, int* iters, double* relres, const unsigned char* warm)
+                     int maxit, const unsigned char* solve_h, int* iters, double* relres, const unsigned char* warm,
+                     bool handover)
 {
     const InstDev& I = m.inst;
     const int B = I.B;
     const int nV = m.nV;
+    // per instance: 0 not solving, 1 new solve, 2 continued (its state is on the device)
+    std::vector<unsigned char> sv(B, 1);
+    if (solve_h) sv.assign(solve_h, solve_h + B);
+    L.h_it0.resize(B, 0);
+    for (int i = 0; i < B; ++i)
+        if (sv[i] != 2) L.h_it0[i] = 0;
+    const unsigned char* solve = L.solve_d.upload_get(sv.data(), B, s);
     const double nnzb = (double)m.nnzb_e + (double)L.ncb;
     int* err = L.err.ensure(2 * (size_t)B + 2, s);
     int* flag = err + B;
@@ -1188,15 +1200,18 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
     int nA = 0, nB = 0;
     double frac = 1.0;  // launched share of the rows (exact for one instance, row-proportional otherwise)
     auto tables = [&]() {
-        auto fill = [&](const SegDev& sd, std::vector<int4>& out) {
+        auto fill = [&](const SegDev& sd, std::vector<int4>& out, std::vector<int>& off) {
             out.clear();
+            off.clear();
             for (int i = 0; i < B; ++i)
                 if (run[i])
-                    for (int k = sd.h_pbeg[i]; k < sd.h_pbeg[i + 1]; ++k)
+                    for (int k = sd.h_pbeg[i]; k < sd.h_pbeg[i + 1]; ++k) {
                         out.push_back(make_int4(sd.h_lo[k], sd.h_hi[k], i, k));
+                        off.push_back(L.h_it0[i]);
+                    }
         };
-        fill(rows, L.h_atab);
-        fill(I.pcg_upd, L.h_btab);
+        fill(rows, L.h_atab, L.h_aoff);
+        fill(I.pcg_upd, L.h_btab, L.h_boff);
         nA = (int)L.h_atab.size();
         nB = (int)L.h_btab.size();
         long rows = 0;
@@ -1204,7 +1219,10 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
         frac = nV ? (double)rows / nV : 0.0;
         A.atab = L.atab.upload_get(L.h_atab.data(), L.h_atab.size(), s);
         A.btab = L.btab.upload_get(L.h_btab.data(), L.h_btab.size(), s);
+        A.aoff = L.aoff.upload_get(L.h_aoff.data(), L.h_aoff.size(), s);
+        A.boff = L.boff.upload_get(L.h_boff.data(), L.h_boff.size(), s);
     };
+    A.solve = solve;
     tables();
     A.part_pq = L.part0.ensure(A.nbA + 1, s);
     A.part_bx = L.part_bx.ensure(A.nbA + 1, s);
@@ -1241,9 +1259,15 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
     const double spmv_bytes = 76.0 * nnzb + 8.0 * (nV + 1) + 96.0 * nV;
     const double upd_bytes = 240.0 * nV;
     std::vector<double> hs(3 * B);
+    // Hand-over (batched engine): once half of the new and continued solves
+    // have stopped, the others get as many iterations again and then keep
+    // their state for the next call.  One solve alone is never handed over.
+    int nstart = 0;
+    for (int i = 0; i < B; ++i) nstart += run[i];
+    int it_half = -1, it_end = INT_MAX;
     int it = 0, chunk = 8;
-    for (; nA > 0;) {
-        for (int j = 0; j < chunk && it <= maxit; ++j, ++it) {
+    for (; nA > 0 && it < it_end;) {
+        for (int j = 0; j < chunk && it <= maxit && it < it_end; ++j, ++it) {
             // per-kernel events on one iteration in 8 (an event pair between
             // every launch would itself widen the launch gaps)
             const bool sample = P && (it & 7) == 3;
@@ -1257,14 +1281,21 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
         IDP_HIP(hipMemcpyAsync(hf.data(), err, 2 * B * sizeof(int), hipMemcpyDeviceToHost, s));
         IDP_HIP(hipStreamSynchronize(s));
         bool all = true, fewer = false;
+        int nrun = 0;
         for (int i = 0; i < B; ++i) {
             all &= hf[B + i] != 0;
             if (run[i] && hf[B + i] != 0) {
                 run[i] = 0;
                 fewer = true;
             }
+            nrun += run[i];
         }
         if (all || it > maxit) break;
+        if (handover && nstart > 1 && it_half < 0 && 2 * nrun <= nstart) {
+            it_half = it;
+            it_end = 2 * it;
+        }
+        if (it >= it_end) break;
         if (fewer) tables();  // the stream is idle here: the tables may be rewritten
         chunk = std::min(chunk * 2, 256);
     }
@@ -1275,12 +1306,18 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
     double done_iters = 0;
     for (int i = 0; i < B; ++i) {
         const int f = hf[B + i];
-        if (f == 0) throw HipError(IDP_E_HIP, "PCG did not stop");
         iters[i] = 0;
         relres[i] = 0;
+        if (f == 0) {
+            if (it < it_end) throw HipError(IDP_E_HIP, "PCG did not stop");
+            iters[i] = PCG_HANDED_OVER;  // continues in the next call
+            L.h_it0[i] += it;
+            maxdone = std::max(maxdone, it);
+            continue;
+        }
         if (f == 3) continue;
         const int k = (int)hs[3 * i + 2];
-        maxdone = std::max(maxdone, k);
+        maxdone = std::max(maxdone, k - L.h_it0[i]);
         if (hf[i]) iters[i] = -1;
         else if (f == 2) iters[i] = -2;
         else {
@@ -1303,7 +1340,7 @@ int pcg_solve_persistent(LinWork& L, hipStream_t s, const MeshDev& m, const doub
 {
     int it = 0;
     double rel = 0;
-    pcg_solve_batch(L, s, m, b, x, rtol, maxit, nullptr, &it, &rel, warm);
+    pcg_solve_batch(L, s, m, b, x, rtol, maxit, nullptr, &it, &rel, warm, false);
     *relres = rel;
     return it;
 }

@@ -17,6 +17,9 @@ struct LinWork {
     int spmv_shape = 5;  // PCG SpMV launch shape: 8 lanes per row, scalars in a wavefront of their own
     DBuf<int4> atab, btab;  // PCG launch tables
     std::vector<int4> h_atab, h_btab;
+    DBuf<int> aoff, boff;   // per launched block: its instance's iterations in earlier calls
+    std::vector<int> h_aoff, h_boff, h_it0;
+    DBuf<unsigned char> solve_d;
     // values
     DBuf<int> erow;  // row of every elastic block
     DBuf<double> evals, cvals;
@@ -35,11 +38,16 @@ void contact_gradient_gather(LinWork& L, hipStream_t s, const MeshDev& m, const 
                              const double* gl, const double* gb, double* g, double* rhs);
 void assemble(LinWork& L, hipStream_t s, const MeshDev& m, const double* Hl, const double* Hb, bool any_dbc);
 void init_erow(LinWork& L, hipStream_t s, const MeshDev& m, const std::vector<int>& rowof);
-// warm (device, B bytes; null = none): instances that start from a multiple
-// of what x holds on entry (their previous direction) instead of zero
+// solve_h (host, B; null = all new): 0 not solving, 1 new solve, 2 continue
+// a solve handed over by the previous call.  warm (device, B bytes; null =
+// none): new solves that start from a multiple of what x holds on entry
+// (their previous direction) instead of zero.  handover: once half of the
+// solves have stopped, the rest get as many iterations again and are then
+// handed over (iters = PCG_HANDED_OVER); a hand-over changes no arithmetic.
+constexpr int PCG_HANDED_OVER = -3;
 void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* b, double* x, double rtol,
-                     int maxit, const unsigned char* solve, int* iters, double* relres,
-                     const unsigned char* warm = nullptr);
+                     int maxit, const unsigned char* solve_h, int* iters, double* relres,
+                     const unsigned char* warm = nullptr, bool handover = false);
 int pcg_solve_persistent(LinWork& L, hipStream_t s, const MeshDev& m, const double* b, double* x, double rtol,
                          int maxit, double* relres, const unsigned char* warm = nullptr);
 double dev_dot(LinWork& L, hipStream_t s, long n, const double* a, const double* b);

@@ -72,9 +72,10 @@ class BatchAnimFix:
         self.offsets = np.concatenate([[0], np.cumsum([sc.X.shape[0] for sc in self.scenes])])
         self.ctx.set_state(np.concatenate([sc.X for sc in self.scenes]), np.concatenate(vel))
 
-    def advance_one_time_step(self, max_newton=0, verbose=0, accd_budget=None):
+    def advance_one_time_step(self, max_newton=0, verbose=0, accd_budget=None, pcg_handover=None):
         return self.ctx.advance_batch(self.h, self.tol, self.dHat2, self.kappa, 0.0, True, self.pcg_rtol,
-                                      max_newton=max_newton, verbose=verbose, accd_budget=accd_budget)
+                                      max_newton=max_newton, verbose=verbose, accd_budget=accd_budget,
+                                      pcg_handover=pcg_handover)
 
     def state(self, i):
         X, v = self.ctx.get_state()

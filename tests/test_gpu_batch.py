@@ -69,16 +69,18 @@ def test_batch_matches_oracle():
         assert err < 1e-6
 
 
-def test_batch_accd_handover_changes_nothing():
+def test_batch_handover_changes_nothing():
     """A tiny ACCD step budget hands long pairs from one Newton iteration to
-    the next (the instance waits while the others go on); the results must be
-    those of a batch that never hands over, bit for bit."""
+    the next, and PCG solves still running when half of the instances' solves
+    have stopped are continued in the next one (the instance waits while the
+    others go on); the results must be those of a batch that never hands
+    over, bit for bit."""
     mix = _mix()
-    say("[batch] ACCD hand-over: budget 4 vs never")
+    say("[batch] hand-over: ACCD budget 4 and PCG hand-over vs never")
     a = driver.BatchAnimFix(mix)
-    ia, _ = a.advance_one_time_step(accd_budget=4)
+    ia, _ = a.advance_one_time_step(accd_budget=4, pcg_handover=0)
     b = driver.BatchAnimFix(mix)
-    ib, _ = b.advance_one_time_step(accd_budget=-1)
+    ib, _ = b.advance_one_time_step(accd_budget=-1, pcg_handover=-1)
     say(f"[batch] newton {list(ia)} vs {list(ib)}")
     assert list(ia) == list(ib)
     for i in range(len(mix)):
