@@ -370,7 +370,7 @@ def main():
     ap.add_argument("--max-newton", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--traffic", default=os.path.join(HERE, "profiles", "traffic_bar_twist_fold.txt"))
+    ap.add_argument("--traffic", default=os.path.join(HERE, "profiles", "k_pcg_spmv_traffic_bar_twist_fold.txt"))
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration)")
     a = ap.parse_args()

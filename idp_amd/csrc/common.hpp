@@ -2,6 +2,7 @@
 // HIP error handling, growable device buffers, deterministic reductions.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cstdint>
@@ -32,6 +33,16 @@ struct HipError : std::runtime_error {
     do {                                                                                                \
         if ((grid) > 0) {                                                                               \
             hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), shmem, stream, __VA_ARGS__);           \
+            IDP_HIP(hipGetLastError());                                                                 \
+        }                                                                                               \
+    } while (0)
+
+// launch with a start/stop event pair recorded by the dispatch itself (kernel
+// time only, as a profiler measures it); null events: a plain launch
+#define IDP_LAUNCH_TIMED(kernel, grid, block, shmem, stream, ev0, ev1, ...)                             \
+    do {                                                                                                \
+        if ((grid) > 0) {                                                                               \
+            hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), shmem, stream, ev0, ev1, 0, __VA_ARGS__); \
             IDP_HIP(hipGetLastError());                                                                 \
         }                                                                                               \
     } while (0)
@@ -138,6 +149,22 @@ struct Profiler {
         int b = ev();
         IDP_HIP(hipEventRecord(pool[b], s));
         open.push_back({fam, tok, b, tag, w});
+    }
+    // an event pair for a launch that records them itself (IDP_LAUNCH_TIMED):
+    // returns a token for end_pair(), or -1 (and null events) when off
+    int pair(hipEvent_t& e0, hipEvent_t& e1)
+    {
+        e0 = e1 = nullptr;
+        if (!on) return -1;
+        const int a = ev(), b = ev();
+        e0 = pool[a];
+        e1 = pool[b];
+        return a;
+    }
+    void end_pair(int fam, int tok, double w = 0.0, int tag = 0)
+    {
+        if (!on || tok < 0) return;
+        open.push_back({fam, tok, tok + 1, tag, w});
     }
     void discard_from(int fam, int tag)
     {

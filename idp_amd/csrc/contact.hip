@@ -714,6 +714,7 @@ __global__ void __launch_bounds__(256) k_ccd_walk(int nb, int ne, CCDArgs A, Cel
 struct AccdLane {
     d3 x[4], dx[4];
     double maxDisp, d2, dc, f, gap, toc, tmax, d0, tmax0;
+    double lb;  // the next step's lower bound, computed as soon as dc is known
     int guard, inst;
     int2 c;
     bool ee;
@@ -773,6 +774,7 @@ __device__ inline bool accd_init(AccdLane& L, int2 c, const CCDArgs& A, double e
     accd_fix_ee(L, th);
     L.dc = sqrt(L.d2);
     L.gap = eta * L.f / (L.dc + th);
+    L.lb = (1 - eta) * L.f / ((L.dc + th) * L.maxDisp);
     L.toc = 0;
     L.tmax = __longlong_as_double((long long)*((volatile const unsigned long long*)(amin + L.inst)));
     L.guard = 0;
@@ -782,17 +784,22 @@ __device__ inline bool accd_init(AccdLane& L, int2 c, const CCDArgs& A, double e
     return true;
 }
 
-// one additive step; returns 0 running, 1 hit (toc), 2 no hit
+// one additive step; returns 0 running, 1 hit (toc), 2 no hit.  The next
+// step's bound is computed before the gap test is resolved (the same
+// expression the loop would evaluate after it), so the two divisions overlap
+// instead of following each other across the branch.
 __device__ inline int accd_step(AccdLane& L, double eta, double th)
 {
-    const double lb = (1 - eta) * L.f / ((L.dc + th) * L.maxDisp);
+    const double lb = L.lb;
 #pragma unroll
     for (int k = 0; k < 4; ++k) L.x[k] = L.x[k] + lb * L.dx[k];
     L.d2 = accd_dist(L);
     L.f = L.d2 - th * th;
     accd_fix_ee(L, th);
     L.dc = sqrt(L.d2);
-    if (L.toc != 0.0 && (L.f / (L.dc + th) < L.gap)) return 1;
+    const double den = L.dc + th;
+    L.lb = (1 - eta) * L.f / (den * L.maxDisp);
+    if (L.toc != 0.0 && (L.f / den < L.gap)) return 1;
     L.toc += lb;
     if (L.toc > L.tmax) return 2;
     if (++L.guard >= (1 << 20)) return 1;

@@ -645,7 +645,12 @@ __device__ inline double row_butterfly(double* y, int lane)
         y[1] += __shfl_xor(y[1], o, G);
         y[2] += __shfl_xor(y[2], o, G);
     }
-    return lane == 0 ? y[0] : (lane == 1 ? y[1] : y[2]);
+    // select by value (an index into y would put the array in scratch memory)
+    const double y0 = y[0], y1 = y[1], y2 = y[2];
+    double r = y2;
+    r = lane == 1 ? y1 : r;
+    r = lane == 0 ? y0 : r;
+    return r;
 }
 
 // sum of one double per thread of the row threads -> one partial per block
@@ -1239,17 +1244,17 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
         else if (shape.g == 16) IDP_LAUNCH(k_pcg_warm<16>, nA, PCG_ROW_THREADS, 0, s, A);
         else IDP_LAUNCH(k_pcg_warm<32>, nA, PCG_ROW_THREADS, 0, s, A);
     }
-    auto spmv = [&](int it) {
+    auto spmv = [&](int it, hipEvent_t e0, hipEvent_t e1) {
         const int nt = PCG_ROW_THREADS + (shape.sw ? 64 : 0);
         switch (L.spmv_shape) {
-        case 0: IDP_LAUNCH((k_pcg_spmv<4, false>), nA, nt, 0, s, A, it); break;
-        case 1: IDP_LAUNCH((k_pcg_spmv<8, false>), nA, nt, 0, s, A, it); break;
-        case 2: IDP_LAUNCH((k_pcg_spmv<16, false>), nA, nt, 0, s, A, it); break;
-        case 3: IDP_LAUNCH((k_pcg_spmv<32, false>), nA, nt, 0, s, A, it); break;
-        case 4: IDP_LAUNCH((k_pcg_spmv<4, true>), nA, nt, 0, s, A, it); break;
-        case 5: IDP_LAUNCH((k_pcg_spmv<8, true>), nA, nt, 0, s, A, it); break;
-        case 6: IDP_LAUNCH((k_pcg_spmv<16, true>), nA, nt, 0, s, A, it); break;
-        default: IDP_LAUNCH((k_pcg_spmv<32, true>), nA, nt, 0, s, A, it); break;
+        case 0: IDP_LAUNCH_TIMED((k_pcg_spmv<4, false>), nA, nt, 0, s, e0, e1, A, it); break;
+        case 1: IDP_LAUNCH_TIMED((k_pcg_spmv<8, false>), nA, nt, 0, s, e0, e1, A, it); break;
+        case 2: IDP_LAUNCH_TIMED((k_pcg_spmv<16, false>), nA, nt, 0, s, e0, e1, A, it); break;
+        case 3: IDP_LAUNCH_TIMED((k_pcg_spmv<32, false>), nA, nt, 0, s, e0, e1, A, it); break;
+        case 4: IDP_LAUNCH_TIMED((k_pcg_spmv<4, true>), nA, nt, 0, s, e0, e1, A, it); break;
+        case 5: IDP_LAUNCH_TIMED((k_pcg_spmv<8, true>), nA, nt, 0, s, e0, e1, A, it); break;
+        case 6: IDP_LAUNCH_TIMED((k_pcg_spmv<16, true>), nA, nt, 0, s, e0, e1, A, it); break;
+        default: IDP_LAUNCH_TIMED((k_pcg_spmv<32, true>), nA, nt, 0, s, e0, e1, A, it); break;
         }
     };
     if (nB) IDP_LAUNCH(k_pcg_init, nB, PCG_UPD_BLOCK, 0, s, A);
@@ -1268,15 +1273,16 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
     int it = 0, chunk = 8;
     for (; nA > 0 && it < it_end;) {
         for (int j = 0; j < chunk && it <= maxit && it < it_end; ++j, ++it) {
-            // per-kernel events on one iteration in 8 (an event pair between
-            // every launch would itself widen the launch gaps)
+            // kernel times (events recorded by the dispatch itself) on one
+            // iteration in 8
             const bool sample = P && (it & 7) == 3;
-            int t = sample ? P->begin(IDP_K_PCG_SPMV) : -1;
-            spmv(it);
-            if (sample) P->end(IDP_K_PCG_SPMV, t, spmv_bytes * frac, it);
-            t = sample ? P->begin(IDP_K_PCG_UPDATE) : -1;
-            IDP_LAUNCH(k_pcg_update, nB, PCG_UPD_BLOCK, 0, s, A, it);
-            if (sample) P->end(IDP_K_PCG_UPDATE, t, upd_bytes * frac, it);
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            int t = sample ? P->pair(e0, e1) : -1;
+            spmv(it, e0, e1);
+            if (sample) P->end_pair(IDP_K_PCG_SPMV, t, spmv_bytes * frac, it);
+            t = sample ? P->pair(e0, e1) : -1;
+            IDP_LAUNCH_TIMED(k_pcg_update, nB, PCG_UPD_BLOCK, 0, s, e0, e1, A, it);
+            if (sample) P->end_pair(IDP_K_PCG_UPDATE, t, upd_bytes * frac, it);
         }
         IDP_HIP(hipMemcpyAsync(hf.data(), err, 2 * B * sizeof(int), hipMemcpyDeviceToHost, s));
         IDP_HIP(hipStreamSynchronize(s));
