@@ -363,7 +363,7 @@ def main():
                     choices=["bar_twist_fold", "bar_bend", "cubes256", "serve"])
     ap.add_argument("--frames", type=int, default=20,
                     help="serve: frames requested per sequence (the first of a 200-frame bend to 270 degrees)")
-    ap.add_argument("--rates", default="5,20,100,200", help="serve: Poisson arrival rates (req/s)")
+    ap.add_argument("--rates", default="5,20,100,200,1000,5000", help="serve: Poisson arrival rates (req/s)")
     ap.add_argument("--plain-sample", type=int, default=16,
                     help="cubes256: instances solved one at a time for the plain-path comparison")
     ap.add_argument("--pcg-rtol", type=float, default=1e-8)

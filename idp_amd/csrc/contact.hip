@@ -655,29 +655,63 @@ struct CCDArgs {
 // PT as (vertex, triangle) from the front of the buffer (counter cnt[0]), EE
 // as (-eI-1, eJ) from the back (counter cnt[1]).  The step size is the minimum
 // over candidates, so the order does not matter.
-template <int G>
-__global__ void __launch_bounds__(256) k_ccd_walk(int nb, int ne, CCDArgs A, CellTable T,
-                                                  unsigned* __restrict__ cnt, int2* __restrict__ cand, unsigned cap)
+// Append for the lanes of a wavefront that call this together: one atomic
+// per wavefront (the lanes with pred take consecutive slots in lane order);
+// returns the slot, or UINT_MAX for a lane without pred.
+__device__ inline unsigned wave_append(unsigned* ctr, bool pred)
 {
-    const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long q = tid / G;
-    const int lane = (int)(tid % G);
-    if (q >= (long)nb + ne) return;
+    const unsigned long long m = __ballot(pred);
+    if (m == 0) return UINT_MAX;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(ctr, (unsigned)__popcll(m));
+    base = __shfl(base, leader);
+    return pred ? base + (unsigned)__popcll(m & ((1ull << lane) - 1)) : UINT_MAX;
+}
+
+constexpr long WALK_BIG_VOX = 512;  // CCD queries over more voxels are walked by a workgroup
+
+// voxels a query box covers (0 when it lies outside the grid)
+__device__ inline long walk_nvox(const GridDev& g, const int* ql, const int* qh)
+{
+    long n = 1;
+    for (int d = 0; d < 3; ++d) {
+        const int k = min(qh[d], g.cnt[d] - 1) - max(ql[d], 0) + 1;
+        if (k <= 0) return 0;
+        n *= k;
+    }
+    return n;
+}
+
+// Swept-box candidates of the CCD (SPATIAL_HASH.h:484-690): G lanes per
+// query.  BIG: the first pass (BIG = false) hands every query whose box
+// covers more than WALK_BIG_VOX voxels to a list that the second pass walks
+// with a whole workgroup per query, so one fast-moving primitive does not
+// hold the launch up on eight lanes.
+template <int G, bool BIG>
+__device__ inline void ccd_walk_query(long q, int lane, int nb, const CCDArgs& A, const CellTable& T,
+                                      unsigned* __restrict__ cnt, int2* __restrict__ cand, unsigned cap,
+                                      unsigned* __restrict__ nbig, unsigned* __restrict__ big)
+{
     if (q < nb) {
         const int v = A.bn[q];
         const int inst = A.vinst[v];
         if (A.active && !A.active[inst]) return;
         const GridDev& g = A.grids[inst];
+        if (!BIG && walk_nvox(g, A.vbox + 6 * q, A.vbox + 6 * q + 3) > WALK_BIG_VOX) {
+            if (lane == 0) big[atomicAdd(nbig, 1u)] = (unsigned)q;
+            return;
+        }
         const d3 p = ld3(A.X, v), dp = ld3(A.P, v);
         walk_lanes<G>(g, A.keybase[inst], T, A.pbox, 0, A.vbox + 6 * q, A.vbox + 6 * q + 3, lane, [&](int f) {
-            int3 t = A.bt[f];
-            if (v == t.x || v == t.y || v == t.z) return;
-            if (A.dbc[v] && A.dbc[t.x] && A.dbc[t.y] && A.dbc[t.z]) return;
-            if (!pt_ccd_broadphase(p, ld3(A.X, t.x), ld3(A.X, t.y), ld3(A.X, t.z), dp, ld3(A.P, t.x),
-                                   ld3(A.P, t.y), ld3(A.P, t.z), A.th))
-                return;
-            const unsigned i = atomicAdd(cnt, 1u);
-            if (i < cap) cand[i] = make_int2(v, f);  // PT from the front
+            const int3 t = A.bt[f];
+            const bool ok = !(v == t.x || v == t.y || v == t.z) &&
+                            !(A.dbc[v] && A.dbc[t.x] && A.dbc[t.y] && A.dbc[t.z]) &&
+                            pt_ccd_broadphase(p, ld3(A.X, t.x), ld3(A.X, t.y), ld3(A.X, t.z), dp, ld3(A.P, t.x),
+                                              ld3(A.P, t.y), ld3(A.P, t.z), A.th);
+            const unsigned i = wave_append(cnt, ok);
+            if (ok && i < cap) cand[i] = make_int2(v, f);  // PT from the front
         });
     } else {
         const int eI = (int)(q - nb);
@@ -685,20 +719,43 @@ __global__ void __launch_bounds__(256) k_ccd_walk(int nb, int ne, CCDArgs A, Cel
         const int inst = A.vinst[a.x];
         if (A.active && !A.active[inst]) return;
         const GridDev& g = A.grids[inst];
-        const d3 a0 = ld3(A.X, a.x), a1 = ld3(A.X, a.y), da0 = ld3(A.P, a.x), da1 = ld3(A.P, a.y);
         const int* pe = A.pbox + 6L * A.nf;
         const int* eb = pe + 6L * eI;
+        if (!BIG && walk_nvox(g, eb, eb + 3) > WALK_BIG_VOX) {
+            if (lane == 0) big[atomicAdd(nbig, 1u)] = (unsigned)q;
+            return;
+        }
+        const d3 a0 = ld3(A.X, a.x), a1 = ld3(A.X, a.y), da0 = ld3(A.P, a.x), da1 = ld3(A.P, a.y);
         walk_lanes<G>(g, A.keybase[inst], T, pe, 1, eb, eb + 3, lane, [&](int f) {
-            if (f <= eI) return;
-            int2 b = A.be[f];
-            if (a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y) return;
-            if (A.dbc[a.x] && A.dbc[a.y] && A.dbc[b.x] && A.dbc[b.y]) return;
-            if (!ee_ccd_broadphase(a0, a1, ld3(A.X, b.x), ld3(A.X, b.y), da0, da1, ld3(A.P, b.x), ld3(A.P, b.y),
-                                   A.th))
-                return;
-            const unsigned i = atomicAdd(cnt + 1, 1u);
-            if (i < cap) cand[cap - 1 - i] = make_int2(-eI - 1, f);  // EE from the back
+            bool ok = f > eI;
+            if (ok) {
+                const int2 b = A.be[f];
+                ok = !(a.x == b.x || a.x == b.y || a.y == b.x || a.y == b.y) &&
+                     !(A.dbc[a.x] && A.dbc[a.y] && A.dbc[b.x] && A.dbc[b.y]) &&
+                     ee_ccd_broadphase(a0, a1, ld3(A.X, b.x), ld3(A.X, b.y), da0, da1, ld3(A.P, b.x),
+                                       ld3(A.P, b.y), A.th);
+            }
+            const unsigned i = wave_append(cnt + 1, ok);
+            if (ok && i < cap) cand[cap - 1 - i] = make_int2(-eI - 1, f);  // EE from the back
         });
+    }
+}
+
+template <int G, bool BIG>
+__global__ void __launch_bounds__(256) k_ccd_walk(int nb, int ne, CCDArgs A, CellTable T,
+                                                  unsigned* __restrict__ cnt, int2* __restrict__ cand, unsigned cap,
+                                                  unsigned* __restrict__ nbig, unsigned* __restrict__ big)
+{
+    if (BIG) {
+        // a workgroup per listed query (the list is final: the first pass has ended)
+        const unsigned n = *nbig;
+        for (unsigned k = blockIdx.x; k < n; k += gridDim.x)
+            ccd_walk_query<G, true>((long)big[k], (int)threadIdx.x, nb, A, T, cnt, cand, cap, nbig, big);
+    } else {
+        const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+        const long q = tid / G;
+        if (q >= (long)nb + ne) return;
+        ccd_walk_query<G, false>(q, (int)(tid % G), nb, A, T, cnt, cand, cap, nbig, big);
     }
 }
 
@@ -715,6 +772,7 @@ struct AccdLane {
     d3 x[4], dx[4];
     double maxDisp, d2, dc, f, gap, toc, tmax, d0, tmax0;
     double lb;  // the next step's lower bound, computed as soon as dc is known
+    int ty;     // distance type of the last evaluation (speculated for the next)
     int guard, inst;
     int2 c;
     bool ee;
@@ -729,6 +787,86 @@ __device__ inline double accd_dist(const AccdLane& L)
 {
     return L.ee ? d2_ee_unclassified(L.x[0], L.x[1], L.x[2], L.x[3])
                 : d2_pt_unclassified(L.x[0], L.x[1], L.x[2], L.x[3]);
+}
+
+// Unclassified distance with the classification and the distance of the
+// last step's type evaluated side by side (one basic block each, so their
+// instructions interleave); a long pair's type rarely changes from one step
+// to the next.  When the classification differs, the distance of the new
+// type is computed.  The value is always that of the classified type's
+// formula, as in accd_dist.
+template <int T>
+__device__ inline double ee_d2_of(d3 a0, d3 a1, d3 b0, d3 b1)
+{
+    switch (T) {
+    case 0: return d2_pp(a0, b0);
+    case 1: return d2_pp(a0, b1);
+    case 2: return d2_pe(a0, b0, b1);
+    case 3: return d2_pp(a1, b0);
+    case 4: return d2_pp(a1, b1);
+    case 5: return d2_pe(a1, b0, b1);
+    case 6: return d2_pe(b0, a0, a1);
+    case 7: return d2_pe(b1, a0, a1);
+    default: return d2_ee(a0, a1, b0, b1);
+    }
+}
+template <int T>
+__device__ inline double pt_d2_of(d3 p, d3 t0, d3 t1, d3 t2)
+{
+    switch (T) {
+    case 0: return d2_pp(p, t0);
+    case 1: return d2_pp(p, t1);
+    case 2: return d2_pp(p, t2);
+    case 3: return d2_pe(p, t0, t1);
+    case 4: return d2_pe(p, t1, t2);
+    case 5: return d2_pe(p, t2, t0);
+    default: return d2_pt(p, t0, t1, t2);
+    }
+}
+__device__ inline double accd_dist_spec(AccdLane& L)
+{
+    const d3 a = L.x[0], b = L.x[1], c = L.x[2], d = L.x[3];
+    double d2 = 0.0;
+    int t = L.ty;
+    if (L.ee) {
+        switch (L.ty) {
+#define IDP_EE_CASE(K)                 \
+    case K:                            \
+        d2 = ee_d2_of<K>(a, b, c, d);  \
+        t = ee_type(a, b, c, d);       \
+        break;
+            IDP_EE_CASE(0) IDP_EE_CASE(1) IDP_EE_CASE(2) IDP_EE_CASE(3) IDP_EE_CASE(4) IDP_EE_CASE(5)
+            IDP_EE_CASE(6) IDP_EE_CASE(7)
+        default:
+            d2 = ee_d2_of<8>(a, b, c, d);
+            t = ee_type(a, b, c, d);
+            break;
+#undef IDP_EE_CASE
+        }
+        if (t != L.ty) {
+            d2 = d2_ee_unclassified(a, b, c, d);
+            L.ty = t;
+        }
+    } else {
+        switch (L.ty) {
+#define IDP_PT_CASE(K)                 \
+    case K:                            \
+        d2 = pt_d2_of<K>(a, b, c, d);  \
+        t = pt_type(a, b, c, d);       \
+        break;
+            IDP_PT_CASE(0) IDP_PT_CASE(1) IDP_PT_CASE(2) IDP_PT_CASE(3) IDP_PT_CASE(4) IDP_PT_CASE(5)
+        default:
+            d2 = pt_d2_of<6>(a, b, c, d);
+            t = pt_type(a, b, c, d);
+            break;
+#undef IDP_PT_CASE
+        }
+        if (t != L.ty) {
+            d2 = d2_pt_unclassified(a, b, c, d);
+            L.ty = t;
+        }
+    }
+    return d2;
 }
 
 // EE: fall back to the closest endpoint pair when the line distance is inside
@@ -769,6 +907,7 @@ __device__ inline bool accd_init(AccdLane& L, int2 c, const CCDArgs& A, double e
         L.maxDisp = sqrt(fmax(sqn(da0), sqn(da1))) + sqrt(fmax(sqn(db0), sqn(db1)));
     }
     if (L.maxDisp == 0) return false;
+    L.ty = L.ee ? ee_type(L.x[0], L.x[1], L.x[2], L.x[3]) : pt_type(L.x[0], L.x[1], L.x[2], L.x[3]);
     L.d2 = accd_dist(L);
     L.f = L.d2 - th * th;
     accd_fix_ee(L, th);
@@ -793,7 +932,7 @@ __device__ inline int accd_step(AccdLane& L, double eta, double th)
     const double lb = L.lb;
 #pragma unroll
     for (int k = 0; k < 4; ++k) L.x[k] = L.x[k] + lb * L.dx[k];
-    L.d2 = accd_dist(L);
+    L.d2 = accd_dist_spec(L);
     L.f = L.d2 - th * th;
     accd_fix_ee(L, th);
     L.dc = sqrt(L.d2);
@@ -1344,9 +1483,16 @@ void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const do
             IDP_LAUNCH(k_zero_u64, 1, 64, 0, s, stepstat, 2);
             IDP_LAUNCH(k_zero_u8, grid_for(B, 256), 256, 0, s, pend, B);
         }
-        if (T.n)
-            IDP_LAUNCH(k_ccd_walk<WALK_G>, walk_blocks((long)m.nb + m.ne), 256, 0, s, m.nb, m.ne, A, T, cnt, cand,
-                       cap);
+        if (T.n) {
+            unsigned* nbig = cnt + 5;
+            unsigned* big = w.walk_big.ensure((size_t)m.nb + m.ne + 1, s);
+            IDP_LAUNCH(k_zero_u32, 1, 64, 0, s, nbig, 1);
+            IDP_LAUNCH((k_ccd_walk<WALK_G, false>), walk_blocks((long)m.nb + m.ne), 256, 0, s, m.nb, m.ne, A, T,
+                       cnt, cand, cap, nbig, big);
+            // large queries: one workgroup each
+            IDP_LAUNCH((k_ccd_walk<256, true>), (int)std::min<long>((long)m.nb + m.ne, 2048), 256, 0, s, m.nb,
+                       m.ne, A, T, cnt, cand, cap, nbig, big);
+        }
         const int tok = w.prof ? w.prof->begin(IDP_K_CCD) : -1;
         // a work queue needs no more lanes than the chip holds at once
         IDP_LAUNCH(k_accd, 2048, 256, 0, s, cnt, cap, cand, A, 0.1, amin, cnt + 2, stepstat, cnt + 3, lg, budget,

@@ -115,6 +115,7 @@ struct ContactWork {
     int res_cur = 0;
     std::vector<unsigned char> res_inst;   // instances owning those records
     DBuf<unsigned char> pend;              // CCD: per instance, a pair was handed over
+    DBuf<unsigned> walk_big;               // CCD: queries whose swept box is walked by a workgroup
     DBuf<unsigned> icnt;                   // per-instance counters
     DBuf<long> cbeg;                       // constraint range per instance
     std::vector<long> h_cbeg;

@@ -546,6 +546,7 @@ struct PcgArgs {
     // same whatever else is launched.
     const int4 *atab, *btab;
     const int *aoff, *boff;  // per launched block: iterations its instance did in earlier calls
+    int ident;               // one instance, all blocks launched: entries follow from blockIdx (no table load)
     const int *apbeg, *bpbeg;
     const unsigned char* solve;  // per instance: 0 not solving, 1 new solve, 2 continued from an earlier call
     int* flag;        // per instance: 0 running, 1 stopped (converged or maxit), 2 breakdown, 3 not solving
@@ -565,6 +566,22 @@ struct SpmvShape {
 constexpr SpmvShape SPMV_SHAPES[] = {{4, false}, {8, false}, {16, false}, {32, false},
                                      {4, true},  {8, true},  {16, true},  {32, true}};
 inline int spmv_rows(int g) { return PCG_ROW_THREADS / g; }
+
+// Launch-table entry of this block: {first row, end row, instance, block id}.
+// A one-instance launch derives it from blockIdx, saving a dependent load.
+template <int ROWS>
+__device__ inline int4 launch_entry(const PcgArgs& A, const int4* __restrict__ tab)
+{
+    if (A.ident) {
+        const int lo = blockIdx.x * ROWS;
+        return make_int4(lo, min(lo + ROWS, A.nV), 0, (int)blockIdx.x);
+    }
+    return tab[blockIdx.x];
+}
+__device__ inline int launch_off(const PcgArgs& A, const int* __restrict__ off)
+{
+    return A.ident ? A.ident - 1 : off[blockIdx.x];
+}
 
 // Sum over a wavefront by a fixed xor butterfly (every lane gets the total).
 __device__ inline double wave_sum(double v)
@@ -593,6 +610,61 @@ __device__ inline double wave_sum_partials(const double* __restrict__ part, int 
         for (int k = 0; k < NL; ++k) acc += v[k];
     }
     return wave_sum(acc);
+}
+
+// Three sums of n partials each, every load issued before any add (one
+// memory round trip for up to 512 partials per array); each sum is formed in
+// the order wave_sum_partials uses, so the results are the same.
+__device__ inline void wave_sum_partials3(const double* __restrict__ p0, const double* __restrict__ p1,
+                                          const double* __restrict__ p2, int n, double& s0, double& s1,
+                                          double& s2)
+{
+    const int lane = threadIdx.x & 63;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int base = 0; base < n; base += 64 * 8) {
+        double v0[8], v1[8], v2[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int i = base + lane + 64 * k;
+            const bool in = i < n;
+            v0[k] = in ? p0[i] : 0.0;
+            v1[k] = in ? p1[i] : 0.0;
+            v2[k] = in ? p2[i] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            a0 += v0[k];
+            a1 += v1[k];
+            a2 += v2[k];
+        }
+    }
+    s0 = wave_sum(a0);
+    s1 = wave_sum(a1);
+    s2 = wave_sum(a2);
+}
+
+// Two sums of partials (na and nb of them) with the first NA and NB loads
+// per lane of both issued before any add; the per-lane order of the adds is
+// that of wave_sum_partials, so the sums are the same.
+template <int NA, int NB>
+__device__ inline void wave_sum_partials2(const double* __restrict__ pa, int na, const double* __restrict__ pb,
+                                          int nb, double& sa, double& sb)
+{
+    const int lane = threadIdx.x & 63;
+    double va[NA], vb[NB];
+#pragma unroll
+    for (int k = 0; k < NA; ++k) va[k] = lane + 64 * k < na ? pa[lane + 64 * k] : 0.0;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) vb[k] = lane + 64 * k < nb ? pb[lane + 64 * k] : 0.0;
+    double a = 0.0, b = 0.0;
+#pragma unroll
+    for (int k = 0; k < NA; ++k) a += va[k];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) b += vb[k];
+    for (int i = lane + 64 * NA; i < na; i += 64) a += pa[i];
+    for (int i = lane + 64 * NB; i < nb; i += 64) b += pb[i];
+    sa = wave_sum(a);
+    sb = wave_sum(b);
 }
 
 // y += A_row z (and w += A_row po when po is given) over this lane's blocks of
@@ -671,7 +743,7 @@ __device__ inline double block_row_sum(double v, double* wsum)
 // the r.r partials, read by the first SpMV)
 __global__ void __launch_bounds__(PCG_UPD_BLOCK) k_pcg_init(PcgArgs A)
 {
-    const int4 t = A.btab[blockIdx.x];
+    const int4 t = launch_entry<PCG_UPD_BLOCK>(A, A.btab);
     const int inst = t.z, bid = t.w;
     if (A.flag[inst] != 0 || (A.solve && A.solve[inst] != 1)) return;  // not solving, or continued
     const int v = t.x + threadIdx.x;
@@ -719,7 +791,7 @@ template <int G>
 __global__ void __launch_bounds__(PCG_ROW_THREADS) k_pcg_warm(PcgArgs A)
 {
     __shared__ double wsum[2][PCG_ROW_THREADS / 64];
-    const int4 t = A.atab[blockIdx.x];
+    const int4 t = launch_entry<PCG_ROW_THREADS / G>(A, A.atab);
     const int inst = t.z;
     const bool on = A.warm[inst] && A.flag[inst] == 0;  // uniform over the block
     const int lane = threadIdx.x & (G - 1);
@@ -770,9 +842,9 @@ __global__ void __launch_bounds__(PCG_ROW_THREADS + (SW ? 64 : 0)) k_pcg_spmv(Pc
     __shared__ double sbeta;
     __shared__ double wsum[PCG_ROW_THREADS / 64];
     __shared__ int sflag;
-    const int it = it_call + A.aoff[blockIdx.x];  // the instance's own iteration
+    const int it = it_call + launch_off(A, A.aoff);  // the instance's own iteration
     const int par = it & 1;
-    const int4 t = A.atab[blockIdx.x];
+    const int4 t = launch_entry<PCG_ROW_THREADS / G>(A, A.atab);
     const int inst = t.z, bid = t.w;
     const double* po = it > 0 ? A.pbuf + (size_t)(par ^ 1) * 3 * A.nV : nullptr;
     double y[3] = {0.0, 0.0, 0.0}, w[3] = {0.0, 0.0, 0.0};
@@ -783,10 +855,12 @@ __global__ void __launch_bounds__(PCG_ROW_THREADS + (SW ? 64 : 0)) k_pcg_spmv(Pc
     auto scalars = [&]() {
         const int f = *(volatile int*)(A.flag + inst);
         const int pb = A.bpbeg[inst], np = A.bpbeg[inst + 1] - pb;
-        const double rz = wave_sum_partials(A.part_rz + par * A.nbB + pb, np);
-        const double rr = wave_sum_partials(A.part_rr + par * A.nbB + pb, np);
-        const double rzo = it > 0 ? wave_sum_partials(A.part_rz + (par ^ 1) * A.nbB + pb, np) : 1.0;
-        const double bbp = it == 0 ? wave_sum_partials(A.part_rr + A.nbB + pb, np) : 0.0;
+        // r.z and r.r of this iteration, and r.z of the previous one (|b|^2 at it = 0)
+        double rz, rr, third;
+        wave_sum_partials3(A.part_rz + par * A.nbB + pb, A.part_rr + par * A.nbB + pb,
+                           it > 0 ? A.part_rz + (par ^ 1) * A.nbB + pb : A.part_rr + A.nbB + pb, np, rz, rr, third);
+        const double rzo = it > 0 ? third : 1.0;
+        const double bbp = it == 0 ? third : 0.0;
         if ((threadIdx.x & 63) == 0) {
             double* sci = A.scal + 3 * inst;
             const double bb = it == 0 ? bbp : sci[0];
@@ -847,8 +921,8 @@ __global__ void __launch_bounds__(PCG_ROW_THREADS + (SW ? 64 : 0)) k_pcg_spmv(Pc
 // issued before the partials are reduced.
 __global__ void __launch_bounds__(PCG_UPD_BLOCK) k_pcg_update(PcgArgs A, int it_call)
 {
-    const int it = it_call + A.boff[blockIdx.x];  // the instance's own iteration
-    const int4 t = A.btab[blockIdx.x];
+    const int it = it_call + launch_off(A, A.boff);  // the instance's own iteration
+    const int4 t = launch_entry<PCG_UPD_BLOCK>(A, A.btab);
     const int inst = t.z, bid = t.w;
     const int par = it & 1;
     const double* p = A.pbuf + (size_t)par * 3 * A.nV;
@@ -871,9 +945,10 @@ __global__ void __launch_bounds__(PCG_UPD_BLOCK) k_pcg_update(PcgArgs A, int it_
 #pragma unroll
     for (int i = 0; i < 9; ++i) o0[i] = o[i];
     if (f != 0) return;  // uniform over the wavefront
-    const double pq = wave_sum_partials<32>(A.part_pq + A.apbeg[inst], A.apbeg[inst + 1] - A.apbeg[inst]);
     const int pb = A.bpbeg[inst];
-    const double rz = wave_sum_partials(A.part_rz + par * A.nbB + pb, A.bpbeg[inst + 1] - pb);
+    double pq, rz;
+    wave_sum_partials2<16, 8>(A.part_pq + A.apbeg[inst], A.apbeg[inst + 1] - A.apbeg[inst],
+                              A.part_rz + par * A.nbB + pb, A.bpbeg[inst + 1] - pb, pq, rz);
     if (!(pq > 0.0)) {
         if (bid == pb && threadIdx.x == 0) {
             A.flag[inst] = 2;
@@ -1226,6 +1301,8 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
         A.btab = L.btab.upload_get(L.h_btab.data(), L.h_btab.size(), s);
         A.aoff = L.aoff.upload_get(L.h_aoff.data(), L.h_aoff.size(), s);
         A.boff = L.boff.upload_get(L.h_boff.data(), L.h_boff.size(), s);
+        // iteration offset + 1 when one instance spans the launch (0: use the tables)
+        A.ident = (B == 1 && nA == A.nbA && nB == A.nbB) ? L.h_it0[0] + 1 : 0;
     };
     A.solve = solve;
     tables();

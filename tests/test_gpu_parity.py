@@ -127,6 +127,24 @@ def test_ccd_bit_exact(case):
         assert 0 < ag <= 1.0
 
 
+def test_ccd_bit_exact_concentrated_motion(case):
+    """A few fast vertices and a still rest: the mean motion (span cap) stays
+    small, so the movers' swept boxes cover hundreds of voxels and go through
+    the workgroup-per-query pass of the broad phase."""
+    sc = case.sc
+    X = sc.X.ravel()
+    d = (sc.target - sc.X).copy()
+    movers = sc.X[:, 0] <= np.quantile(sc.X[:, 0], 0.05)
+    d[~movers] = 0.0
+    for scale in (1.0, 4.0):
+        p = scale * d.ravel()
+        say(f"[{sc.name}] CCD concentrated motion x{scale}")
+        ag, _ = case.ctx.ccd(X, p, 1.0)
+        ao, _ = case.m.ccd(X, p, 1.0)
+        assert ag == ao
+        assert 0 < ag <= 1.0
+
+
 def test_newton_direction_vs_direct_solve(case):
     import scipy.sparse as sps
     import scipy.sparse.linalg as spla
