@@ -765,7 +765,7 @@ __global__ void __launch_bounds__(256) k_ccd_walk(int nb, int ne, CCDArgs A, Cel
 // bound, or zero motion) takes the next pair, one atomic per wavefront for
 // all idle lanes, so long and short pairs do not serialise a wavefront.
 // PT candidates are queued before EE ones.  The bound is the running
-// minimum, refreshed every 16 steps: a pair that cannot go below it cannot
+// minimum, refreshed every 64 steps (with a value loaded 64 steps earlier): a pair that cannot go below it cannot
 // change the minimum, and a pair that does reaches the same time of impact
 // whatever bound it started with, so the result is the minimum over pairs.
 struct AccdLane {
@@ -960,6 +960,7 @@ __global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, 
     AccdLane L;
     bool active = false, finished = false;
     int steps = 0, slice = 0;
+    double amin_pf = 0.0;  // running minimum of this lane's instance, loaded ahead of use
     for (;;) {
         const unsigned long long idle = __ballot(!active && !finished);
         if (idle) {
@@ -978,11 +979,13 @@ __global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, 
                     L = res_in[j];
                     L.tmax = fmin(L.tmax, __longlong_as_double(
                                               (long long)*((volatile unsigned long long*)(amin + L.inst))));
+                    amin_pf = L.tmax;
                     active = true;
                 } else {
                     const unsigned q = j - nres;
                     const int2 c = q < npt ? cand[q] : cand[cap - 1 - (q - npt)];
                     active = accd_init(L, c, A, eta, amin);
+                    amin_pf = L.tmax;
                 }
             }
         }
@@ -1023,9 +1026,10 @@ __global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, 
                 } else {
                     slice = INT_MIN;  // no room: run this pair to its end
                 }
-            } else if ((steps & 15) == 0) {
-                L.tmax = fmin(L.tmax,
-                              __longlong_as_double((long long)*((volatile unsigned long long*)(amin + L.inst))));
+            } else if ((steps & 63) == 0) {
+                // the bound loaded 64 steps ago (its latency hidden behind them); reload
+                L.tmax = fmin(L.tmax, amin_pf);
+                amin_pf = __longlong_as_double((long long)*((volatile unsigned long long*)(amin + L.inst)));
             }
         }
     }
@@ -1494,10 +1498,29 @@ void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const do
                        m.ne, A, T, cnt, cand, cap, nbig, big);
         }
         const int tok = w.prof ? w.prof->begin(IDP_K_CCD) : -1;
+        static const bool log_time = std::getenv("IDP_ACCD_LOG") != nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (log_time) {
+            IDP_HIP(hipEventCreate(&e0));
+            IDP_HIP(hipEventCreate(&e1));
+        }
         // a work queue needs no more lanes than the chip holds at once
-        IDP_LAUNCH(k_accd, 2048, 256, 0, s, cnt, cap, cand, A, 0.1, amin, cnt + 2, stepstat, cnt + 3, lg, budget,
-                   res_in, nres, res_out, cnt + 4, RES_CAP, pend);
+        IDP_LAUNCH_TIMED(k_accd, 2048, 256, 0, s, e0, e1, cnt, cap, cand, A, 0.1, amin, cnt + 2, stepstat, cnt + 3,
+                         lg, budget, res_in, nres, res_out, cnt + 4, RES_CAP, pend);
         if (w.prof) w.prof->end(IDP_K_CCD, tok);
+        if (log_time) {
+            // kernel time against the longest pair's steps in this call
+            unsigned long long st[2];
+            float ms = 0.f;
+            IDP_HIP(hipMemcpyAsync(st, stepstat, sizeof(st), hipMemcpyDeviceToHost, s));
+            IDP_HIP(hipStreamSynchronize(s));
+            IDP_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (st[1] >= 4096)
+                std::fprintf(stderr, "[idp] accd call: %.3f ms, longest pair %llu steps (%.1f ns per step), %llu steps\n",
+                             ms, st[1], 1e6 * ms / (double)st[1], st[0]);
+            IDP_HIP(hipEventDestroy(e0));
+            IDP_HIP(hipEventDestroy(e1));
+        }
         IDP_HIP(hipMemcpyAsync(ha.data(), amin, B * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
         IDP_HIP(hipMemcpyAsync(hsteps, stepstat, sizeof(hsteps), hipMemcpyDeviceToHost, s));
         IDP_HIP(hipMemcpyAsync(hn, cnt, sizeof(hn), hipMemcpyDeviceToHost, s));
