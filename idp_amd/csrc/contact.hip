@@ -765,18 +765,20 @@ __global__ void __launch_bounds__(256) k_ccd_walk(int nb, int ne, CCDArgs A, Cel
 // bound, or zero motion) takes the next pair, one atomic per wavefront for
 // all idle lanes, so long and short pairs do not serialise a wavefront.
 // PT candidates are queued before EE ones.  The bound is the running
-// minimum, refreshed every 64 steps (with a value loaded 64 steps earlier): a pair that cannot go below it cannot
+// minimum, refreshed after every run of steps (with a value loaded a run earlier): a pair that cannot go below it cannot
 // change the minimum, and a pair that does reaches the same time of impact
 // whatever bound it started with, so the result is the minimum over pairs.
 struct AccdLane {
     d3 x[4], dx[4];
     double maxDisp, d2, dc, f, gap, toc, tmax, d0, tmax0;
     double lb;  // the next step's lower bound, computed as soon as dc is known
-    int ty;     // distance type of the last evaluation (speculated for the next)
     int guard, inst;
     int2 c;
     bool ee;
 };
+
+// steps a lane takes between visits to the work queue
+constexpr int ACCD_RUN = 32;
 
 // pairs that ran at least this many ACCD steps are logged (diagnostics)
 constexpr int ACCD_LOG_STEPS = 1 << 16;
@@ -787,86 +789,6 @@ __device__ inline double accd_dist(const AccdLane& L)
 {
     return L.ee ? d2_ee_unclassified(L.x[0], L.x[1], L.x[2], L.x[3])
                 : d2_pt_unclassified(L.x[0], L.x[1], L.x[2], L.x[3]);
-}
-
-// Unclassified distance with the classification and the distance of the
-// last step's type evaluated side by side (one basic block each, so their
-// instructions interleave); a long pair's type rarely changes from one step
-// to the next.  When the classification differs, the distance of the new
-// type is computed.  The value is always that of the classified type's
-// formula, as in accd_dist.
-template <int T>
-__device__ inline double ee_d2_of(d3 a0, d3 a1, d3 b0, d3 b1)
-{
-    switch (T) {
-    case 0: return d2_pp(a0, b0);
-    case 1: return d2_pp(a0, b1);
-    case 2: return d2_pe(a0, b0, b1);
-    case 3: return d2_pp(a1, b0);
-    case 4: return d2_pp(a1, b1);
-    case 5: return d2_pe(a1, b0, b1);
-    case 6: return d2_pe(b0, a0, a1);
-    case 7: return d2_pe(b1, a0, a1);
-    default: return d2_ee(a0, a1, b0, b1);
-    }
-}
-template <int T>
-__device__ inline double pt_d2_of(d3 p, d3 t0, d3 t1, d3 t2)
-{
-    switch (T) {
-    case 0: return d2_pp(p, t0);
-    case 1: return d2_pp(p, t1);
-    case 2: return d2_pp(p, t2);
-    case 3: return d2_pe(p, t0, t1);
-    case 4: return d2_pe(p, t1, t2);
-    case 5: return d2_pe(p, t2, t0);
-    default: return d2_pt(p, t0, t1, t2);
-    }
-}
-__device__ inline double accd_dist_spec(AccdLane& L)
-{
-    const d3 a = L.x[0], b = L.x[1], c = L.x[2], d = L.x[3];
-    double d2 = 0.0;
-    int t = L.ty;
-    if (L.ee) {
-        switch (L.ty) {
-#define IDP_EE_CASE(K)                 \
-    case K:                            \
-        d2 = ee_d2_of<K>(a, b, c, d);  \
-        t = ee_type(a, b, c, d);       \
-        break;
-            IDP_EE_CASE(0) IDP_EE_CASE(1) IDP_EE_CASE(2) IDP_EE_CASE(3) IDP_EE_CASE(4) IDP_EE_CASE(5)
-            IDP_EE_CASE(6) IDP_EE_CASE(7)
-        default:
-            d2 = ee_d2_of<8>(a, b, c, d);
-            t = ee_type(a, b, c, d);
-            break;
-#undef IDP_EE_CASE
-        }
-        if (t != L.ty) {
-            d2 = d2_ee_unclassified(a, b, c, d);
-            L.ty = t;
-        }
-    } else {
-        switch (L.ty) {
-#define IDP_PT_CASE(K)                 \
-    case K:                            \
-        d2 = pt_d2_of<K>(a, b, c, d);  \
-        t = pt_type(a, b, c, d);       \
-        break;
-            IDP_PT_CASE(0) IDP_PT_CASE(1) IDP_PT_CASE(2) IDP_PT_CASE(3) IDP_PT_CASE(4) IDP_PT_CASE(5)
-        default:
-            d2 = pt_d2_of<6>(a, b, c, d);
-            t = pt_type(a, b, c, d);
-            break;
-#undef IDP_PT_CASE
-        }
-        if (t != L.ty) {
-            d2 = d2_pt_unclassified(a, b, c, d);
-            L.ty = t;
-        }
-    }
-    return d2;
 }
 
 // EE: fall back to the closest endpoint pair when the line distance is inside
@@ -907,7 +829,6 @@ __device__ inline bool accd_init(AccdLane& L, int2 c, const CCDArgs& A, double e
         L.maxDisp = sqrt(fmax(sqn(da0), sqn(da1))) + sqrt(fmax(sqn(db0), sqn(db1)));
     }
     if (L.maxDisp == 0) return false;
-    L.ty = L.ee ? ee_type(L.x[0], L.x[1], L.x[2], L.x[3]) : pt_type(L.x[0], L.x[1], L.x[2], L.x[3]);
     L.d2 = accd_dist(L);
     L.f = L.d2 - th * th;
     accd_fix_ee(L, th);
@@ -932,7 +853,7 @@ __device__ inline int accd_step(AccdLane& L, double eta, double th)
     const double lb = L.lb;
 #pragma unroll
     for (int k = 0; k < 4; ++k) L.x[k] = L.x[k] + lb * L.dx[k];
-    L.d2 = accd_dist_spec(L);
+    L.d2 = accd_dist(L);
     L.f = L.d2 - th * th;
     accd_fix_ee(L, th);
     L.dc = sqrt(L.d2);
@@ -994,9 +915,17 @@ __global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, 
             continue;
         }
         if (active) {
-            const int r = accd_step(L, eta, A.th);
+            // a run of steps between visits to the queue (a long pair's lane does
+            // little besides stepping); it stops at the pair's end or the budget
+            const int room = budget > 0 ? min(ACCD_RUN, budget - slice) : ACCD_RUN;
+            int r = 0, k = 0;
+            do {
+                r = accd_step(L, eta, A.th);
+                ++k;
+            } while (r == 0 && k < room);
+            steps += k;
+            slice += k;
             if (r == 1) atomicMin(amin + L.inst, (unsigned long long)__double_as_longlong(L.toc));
-            ++steps;
             if (r != 0) {
                 active = false;
                 atomicMax(stepstat + 1, (unsigned long long)L.guard + 1);
@@ -1016,7 +945,7 @@ __global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, 
                         e[9] = L.dc;
                     }
                 }
-            } else if (budget > 0 && ++slice >= budget) {
+            } else if (budget > 0 && slice >= budget) {
                 // step budget of this call spent: hand the pair's state to the next call
                 const unsigned k = atomicAdd(nres_out, 1u);
                 if (k < res_cap) {
@@ -1024,10 +953,10 @@ __global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, 
                     pend[L.inst] = 1;
                     active = false;
                 } else {
-                    slice = INT_MIN;  // no room: run this pair to its end
+                    slice = -(1 << 30);  // no room: run this pair to its end (past the guard)
                 }
-            } else if ((steps & 63) == 0) {
-                // the bound loaded 64 steps ago (its latency hidden behind them); reload
+            } else {
+                // the bound loaded one run ago (its latency hidden behind the run); reload
                 L.tmax = fmin(L.tmax, amin_pf);
                 amin_pf = __longlong_as_double((long long)*((volatile unsigned long long*)(amin + L.inst)));
             }
