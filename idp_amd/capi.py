@@ -36,7 +36,7 @@ _STATUS = {
 
 EXPORTED = [
     "idp_ctx_create", "idp_ctx_destroy", "idp_last_error", "idp_device_count", "idp_fem_init", "idp_surface",
-    "idp_initialize_oipc", "idp_set_mesh", "idp_set_dbc", "idp_set_state", "idp_get_state", "idp_snapshot_state",
+    "idp_initialize_oipc", "idp_set_mesh", "idp_set_dbc", "idp_set_dbc_targets", "idp_set_state", "idp_get_state", "idp_snapshot_state",
     "idp_restore_state", "idp_surface_sizes", "idp_advance_one_step", "idp_constraint_set", "idp_get_constraints",
     "idp_barrier_energy", "idp_barrier_gradient", "idp_barrier_hessian_local", "idp_min_dist2", "idp_ccd_stepsize",
     "idp_elastic_energy", "idp_elastic_gradient", "idp_elastic_hessian_local", "idp_newton_direction", "idp_get_bsr",
@@ -79,7 +79,7 @@ class StepStats(C.Structure):
         ("last_relres", C.c_double), ("min_dist2", C.c_double), ("E", C.c_double), ("kappa", C.c_double),
         ("ms_total", C.c_double), ("ms_constraint_set", C.c_double), ("ms_ccd", C.c_double),
         ("ms_energy", C.c_double), ("ms_gradient", C.c_double), ("ms_hessian", C.c_double),
-        ("ms_solve", C.c_double),
+        ("ms_solve", C.c_double), ("dbc_alpha", C.c_double), ("dbc_stiff", C.c_double),
     ]
 
     def as_dict(self):
@@ -124,6 +124,7 @@ def lib():
         L.idp_initialize_oipc.argtypes = [C.c_double, C.c_int, dp, C.c_double, dp, dp]
         L.idp_set_mesh.argtypes = [vp, C.c_int, dp, C.c_int, ip, dp, dp, dp, dp, dp]
         L.idp_set_dbc.argtypes = [vp, C.c_int, ip]
+        L.idp_set_dbc_targets.argtypes = [vp, C.c_int, ip, dp]
         L.idp_set_state.argtypes = [vp, dp, dp]
         L.idp_get_state.argtypes = [vp, dp, dp]
         L.idp_snapshot_state.argtypes = [vp]
@@ -341,6 +342,14 @@ class Context:
     def set_dbc(self, idx):
         a, ap = _i(idx)
         self._check(lib().idp_set_dbc(self._h, a.size, ap))
+
+    def set_dbc_targets(self, idx, targets):
+        """Moving Dirichlet nodes: idx (n,) driven to targets (n, 3) in every step."""
+        a, ap = _i(idx)
+        t, tp = _d(np.asarray(targets, dtype=np.float64).reshape(-1))
+        if t.size != 3 * a.size:
+            raise ValueError("targets must be (n, 3)")
+        self._check(lib().idp_set_dbc_targets(self._h, a.size, ap, tp))
 
     def surface_sizes(self):
         nb, ne, nf = C.c_int(), C.c_int(), C.c_int()

@@ -31,6 +31,12 @@ struct idp_ctx {
     LinWork L;
     bool have_mesh = false;
     int ndbc = 0;
+    // moving DBC targets (idp_set_dbc_targets): entries in the caller's order
+    std::vector<int> dbc_list;
+    std::vector<double> dbc_tgt;
+    DBuf<int> dbc_idx_d;
+    DBuf<double> dbc_tgt_d, dbc_x_d;
+    DBuf<unsigned char> dbc_fixed_d;
     // state and work vectors (3 nV)
     DBuf<double> x, v, xt, xn, xprev, sol, rhs, g, tmpv, snap_x, snap_v, hx, hp;
     DBuf<double> gl, gb, Hl, Hb;  // per-tet / per-constraint local terms
@@ -340,6 +346,63 @@ const double* stage_x(idp_ctx* c, const double* X)
 // per-instance vector helpers of the batched Newton loop (mask: instances
 // that take part; the arithmetic per element is that of the plain loop)
 // ---------------------------------------------------------------------------
+// ---- moving Dirichlet nodes (Shell/DIRICHLET.h) --------------------------
+// x of the DBC entries, in entry order
+__global__ void k_dbc_gather(int n, const int* __restrict__ idx, const double* __restrict__ X, double* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int v = idx[i];
+    out[3 * i] = X[3 * v];
+    out[3 * i + 1] = X[3 * v + 1];
+    out[3 * i + 2] = X[3 * v + 2];
+}
+
+// g += k m (x - x_D) for the DBC entries that move (DIRICHLET.h:50-70; the
+// rows of the nodes that stay are zero), rhs = -g
+__global__ void k_dbc_grad(int n, const int* __restrict__ idx, const double* __restrict__ tgt,
+                           const double* __restrict__ mass, const unsigned char* __restrict__ fixed,
+                           const double* __restrict__ X, double k, double* __restrict__ g, double* __restrict__ rhs)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int v = idx[i];
+    if (fixed[v]) return;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double gv = g[3 * v + d] + k * mass[v] * (X[3 * v + d] - tgt[3 * i + d]);
+        g[3 * v + d] = gv;
+        rhs[3 * v + d] = -gv;
+    }
+}
+
+// x_D of every DBC entry (the whole move fits, IMPLICIT_EULER.h:385-396)
+__global__ void k_dbc_set(int n, const int* __restrict__ idx, const double* __restrict__ tgt, double* __restrict__ X)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int v = idx[i];
+    X[3 * v] = tgt[3 * i];
+    X[3 * v + 1] = tgt[3 * i + 1];
+    X[3 * v + 2] = tgt[3 * i + 2];
+}
+
+// Hessian diagonal += k m for the DBC entries that move (DIRICHLET.h:72-93)
+__global__ void k_dbc_hess(int n, const int* __restrict__ idx, const double* __restrict__ mass,
+                           const unsigned char* __restrict__ fixed, const int* __restrict__ ediag, long eld, double k,
+                           double* __restrict__ ev)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int v = idx[i];
+    if (fixed[v]) return;
+    const long b = ediag[v];
+    const double a = k * mass[v];
+    ev[0 * eld + b] += a;
+    ev[4 * eld + b] += a;
+    ev[8 * eld + b] += a;
+}
+
 __global__ void k_axpy_inst(int nV, const int* __restrict__ vinst, const unsigned char* __restrict__ mask,
                             const double* __restrict__ a, const double* __restrict__ alpha,
                             const double* __restrict__ b, double* __restrict__ x)
@@ -1158,6 +1221,30 @@ int idp_set_dbc(idp_ctx* c, int n, const int* idx)
         c->ndbc = 0;
         for (auto b : d) c->ndbc += b;
         c->m.dbc.upload(d.data(), d.size(), c->s);
+        c->dbc_list.assign(idx, idx + n);
+        c->dbc_tgt.clear();  // the nodes stay where each step finds them
+        IDP_HIP(hipStreamSynchronize(c->s));
+        return IDP_OK;
+    });
+}
+
+int idp_set_dbc_targets(idp_ctx* c, int n, const int* idx, const double* targets)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh || n < 0 || (n && (!idx || !targets))) return IDP_E_INVALID;
+        std::vector<unsigned char> d(c->m.nV, 0);
+        for (int i = 0; i < n; ++i) {
+            if (idx[i] < 0 || idx[i] >= c->m.nV || d[idx[i]]) return IDP_E_INVALID;  // no repeated node
+            d[idx[i]] = 1;
+        }
+        c->ndbc = n;
+        c->m.dbc.upload(d.data(), d.size(), c->s);
+        c->dbc_list.assign(idx, idx + n);
+        c->dbc_tgt.assign(targets, targets + 3 * (size_t)n);
+        if (n) {
+            c->dbc_idx_d.upload(idx, n, c->s);
+            c->dbc_tgt_d.upload(targets, 3 * (size_t)n, c->s);
+        }
         IDP_HIP(hipStreamSynchronize(c->s));
         return IDP_OK;
     });
@@ -1257,6 +1344,73 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
         dev_axpy(s, n3, x, h, c->v.p, xt);
         double kappa = kappa3[0];
         set_bparams1(c, dHat2, kappa, thickness);
+
+        // ---- Dirichlet nodes toward their targets (IMPLICIT_EULER.h:304-401): the
+        // CCD-limited fraction of the move; all of it when that is 1, otherwise
+        // none, and a penalty (augmented Lagrangian, Shell/DIRICHLET.h) pulls them
+        const int nd = (int)(c->dbc_tgt.size() / 3);  // moving-DBC entries (0: the DBC nodes stay)
+        double DBCStiff = 0.0, DBCPenaltyXn = 0.0, DBCAlpha = 1.0;
+        m.proj = nullptr;
+        std::vector<double> hxd(3 * (size_t)nd);
+        auto dbc_x = [&](const double* X) {  // x of the DBC entries on the host, entry order
+            double* o = c->dbc_x_d.ensure(3 * (size_t)nd, s);
+            IDP_LAUNCH(k_dbc_gather, grid_for(nd, 256), 256, 0, s, nd, c->dbc_idx_d.p, X, o);
+            IDP_HIP(hipMemcpyAsync(hxd.data(), o, 3 * (size_t)nd * 8, hipMemcpyDeviceToHost, s));
+            IDP_HIP(hipStreamSynchronize(s));
+        };
+        auto dbc_dist2 = [&](const double* X) {  // DIRICHLET.h:6-23
+            dbc_x(X);
+            double sum = 0.0;
+            for (int i = 0; i < nd; ++i) {
+                double pen = 0.0;
+                for (int d = 0; d < 3; ++d)
+                    pen += (c->dbc_tgt[3 * i + d] - hxd[3 * i + d]) * (c->dbc_tgt[3 * i + d] - hxd[3 * i + d]);
+                sum += pen;
+            }
+            return sum;
+        };
+        std::vector<double> hmass;
+        auto dbc_energy = [&](const double* X, double& E) {  // DIRICHLET.h:25-48
+            dbc_x(X);
+            double sum = 0.0;
+            for (int i = 0; i < nd; ++i) {
+                double pen = 0.0;
+                for (int d = 0; d < 3; ++d)
+                    pen += (c->dbc_tgt[3 * i + d] - hxd[3 * i + d]) * (c->dbc_tgt[3 * i + d] - hxd[3 * i + d]);
+                pen *= hmass[i];
+                sum += pen;
+            }
+            E += 0.5 * DBCStiff * sum;
+        };
+        if (nd) {
+            hmass.resize(nd);
+            {
+                std::vector<double> mall(m.nV);
+                m.mass.download(mall.data(), m.nV, s);
+                for (int i = 0; i < nd; ++i) hmass[i] = mall[c->dbc_list[i]];
+            }
+            dbc_x(x);
+            std::vector<double> hp(n3, 0.0);
+            std::vector<unsigned char> fixed(m.nV, 0);
+            for (int i = 0; i < nd; ++i) {
+                const int v = c->dbc_list[i];
+                for (int d = 0; d < 3; ++d) hp[3 * v + d] = c->dbc_tgt[3 * i + d] - hxd[3 * i + d];
+                if (!(hp[3 * v] || hp[3 * v + 1] || hp[3 * v + 2])) fixed[v] = 1;
+            }
+            if (coll) {
+                long nc = 0;
+                double* P = c->hp.upload_get(hp.data(), n3, s);
+                DBCAlpha = contact_ccd(c->cw, s, m, x, P, thickness, 1.0, &nc);
+            }
+            if (DBCAlpha == 1.0) {
+                IDP_LAUNCH(k_dbc_set, grid_for(nd, 256), 256, 0, s, nd, c->dbc_idx_d.p, c->dbc_tgt_d.p, x);
+            } else {
+                DBCStiff = 1e6;
+                DBCPenaltyXn = dbc_dist2(xn);
+                m.proj = c->dbc_fixed_d.upload_get(fixed.data(), m.nV, s);
+            }
+        }
+        S.dbc_alpha = DBCAlpha;
         auto t0 = now_ms();
         auto cset = [&](const double* X) {
             double ta = now_ms();
@@ -1274,6 +1428,7 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
         if (!e0.valid) return IDP_E_INVALID_START;
         if (!e0.dist_ok) return IDP_E_NONPOS_DIST;
         double Eprev = e0.E;
+        if (DBCStiff) dbc_energy(x, Eprev);
         std::deque<double> res;
         bool useGD = false;
         int PNIter = 0;
@@ -1286,11 +1441,17 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
             if (coll) contact_structure(c->L, s, m.nV, c->cw.ncs, (const int*)c->cw.cs.p);
             else c->L.ncb = 0;
             gradient(c, x, h, coll, g, rhs);
+            if (DBCStiff)
+                IDP_LAUNCH(k_dbc_grad, grid_for(nd, 256), 256, 0, s, nd, c->dbc_idx_d.p, c->dbc_tgt_d.p, m.mass.p,
+                           m.proj, x, DBCStiff, g, rhs);
             IDP_HIP(hipStreamSynchronize(s));
             S.ms_gradient += now_ms() - t0;
             if (!useGD) {
                 t0 = now_ms();
                 hessian(c, x, h, coll);
+                if (DBCStiff)
+                    IDP_LAUNCH(k_dbc_hess, grid_for(nd, 256), 256, 0, s, nd, c->dbc_idx_d.p, m.mass.p, m.proj,
+                               m.e_diag.p, m.nnzb_e, DBCStiff, c->L.evals.p);
                 IDP_HIP(hipStreamSynchronize(s));
                 S.ms_hessian += now_ms() - t0;
                 t0 = now_ms();
@@ -1365,6 +1526,7 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
                     valid = eo.valid;
                     E = eo.E;
                     if (!eo.dist_ok) return IDP_E_NONPOS_DIST;
+                    if (valid && DBCStiff) dbc_energy(x, E);
                 }
                 alpha /= 2.0;
                 S.line_search_trials++;
@@ -1393,6 +1555,7 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
                     set_bparams1(c, dHat2, kappa, thickness);
                     EnergyOut eo = energy(c, x, coll, h);
                     Eprev = eo.E;
+                    if (DBCStiff) dbc_energy(x, Eprev);
                 }
             }
             if (coll) {
@@ -1413,6 +1576,24 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
             L2 = *std::max_element(res.begin(), res.end());
             if (useGD) L2 = newton_tol * 10;
             useGD = (alpha * 2 < 1e-8 && feasibleAlpha > 1e-8);
+            // moving DBC: raise the penalty near convergence until the nodes are
+            // within 1% of their move, then drop it (IMPLICIT_EULER.h:701-755)
+            if (DBCStiff) {
+                const double progress = 1 - std::sqrt(dbc_dist2(x) / DBCPenaltyXn);
+                if (progress < 0.99) {
+                    if (L2 < newton_tol * 10 && DBCStiff < 1e8) {
+                        DBCStiff *= 2;
+                        Eprev = energy(c, x, coll, h).E;
+                        dbc_energy(x, Eprev);
+                    }
+                    L2 = newton_tol * 10;
+                } else {
+                    DBCStiff = 0.0;
+                    m.proj = nullptr;
+                    Eprev = energy(c, x, coll, h).E;
+                }
+            }
+            S.dbc_stiff = DBCStiff;
             S.newton_iters = PNIter;
             if (verbose > 0 && PNIter % verbose == 0)
                 std::fprintf(stderr, "[idp] newton %d res %.3e tol %.3e alpha %.3e feasible %.3e ncs %ld E %.6e kappa %.3e t %.1f ms\n",
@@ -1423,6 +1604,7 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
             if (max_newton > 0 && PNIter >= max_newton) break;
         } while (res.size() < 3 || L2 > newton_tol);
 
+        m.proj = nullptr;
         // v = (x - xn) / h
         dev_velocity(s, n3, x, xn, h, c->v.p);
         IDP_HIP(hipStreamSynchronize(s));

@@ -80,6 +80,11 @@ struct MeshDev {
     DBuf<double> IB;      // 9 per tet, row-major
     DBuf<double> vol, lam, mu;
     DBuf<unsigned char> dbc;
+    // rows/columns projected out of the gradient and the Hessian: the DBC
+    // nodes, or only the ones that do not move while a moving-DBC penalty is
+    // on (null: dbc)
+    const unsigned char* proj = nullptr;
+    const unsigned char* proj_mask() const { return proj ? proj : dbc.p; }
     DBuf<int> bn;         // boundary nodes
     DBuf<int> be;         // boundary edges, 2 per edge
     DBuf<int> bt;         // boundary triangles, 3 per tri

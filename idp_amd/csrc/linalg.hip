@@ -1118,7 +1118,7 @@ void contact_gradient_gather(LinWork& L, hipStream_t s, const MeshDev& m, const 
 {
     bool haveC = L.ncb > 0 && gb;
     IDP_LAUNCH(k_gradient_gather, grid_for(m.nV, 256), 256, 0, s, m.nV, m.e_diag.p, m.e_src_ptr.p, m.e_src.p, gl, X,
-               Xt, m.mass.p, haveC ? L.cdiag.p : (const int*)nullptr, L.seg.p, L.rv1.p, gb, m.dbc.p, g, rhs);
+               Xt, m.mass.p, haveC ? L.cdiag.p : (const int*)nullptr, L.seg.p, L.rv1.p, gb, m.proj_mask(), g, rhs);
 }
 
 void assemble(LinWork& L, hipStream_t s, const MeshDev& m, const double* Hl, const double* Hb, bool any_dbc)
@@ -1134,9 +1134,10 @@ void assemble(LinWork& L, hipStream_t s, const MeshDev& m, const double* Hl, con
         IDP_LAUNCH(k_contact_values, grid_for(L.ncb, 256), 256, 0, s, (unsigned)L.ncb, L.seg.p, L.rv1.p, Hb, cv);
     }
     if (any_dbc) {
-        IDP_LAUNCH(k_project_dbc, grid_for(m.nnzb_e, 256), 256, 0, s, m.nnzb_e, L.erow.p, m.e_col.p, m.dbc.p, 1, ev);
+        IDP_LAUNCH(k_project_dbc, grid_for(m.nnzb_e, 256), 256, 0, s, m.nnzb_e, L.erow.p, m.e_col.p, m.proj_mask(), 1,
+                   ev);
         if (L.ncb)
-            IDP_LAUNCH(k_project_dbc, grid_for(L.ncb, 256), 256, 0, s, (long)L.ncb, L.crow.p, L.ccol.p, m.dbc.p, 0,
+            IDP_LAUNCH(k_project_dbc, grid_for(L.ncb, 256), 256, 0, s, (long)L.ncb, L.crow.p, L.ccol.p, m.proj_mask(), 0,
                        L.cvals.p);
     }
     if (L.prof)

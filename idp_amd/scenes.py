@@ -185,3 +185,21 @@ def config5():
 def small_bend(nx=8, ny=2, nz=2, spacing=0.05, angle_deg=250.0, radius=0.06):
     """Tiny bar used by the smoke check and fast parity tests."""
     return bar_bend(nx, ny, nz, spacing, radius, angle_deg, name=f"bar{nx}x{ny}x{nz}_bend{int(angle_deg)}")
+
+def two_blocks_push(n=3, spacing=0.05, gap=0.2, push=0.5):
+    """Two n^3 Kuhn blocks side by side along x, gap (in cells) apart.  Moving
+    Dirichlet nodes: block A's face toward block B is pushed toward B by
+    `push` cells (past the gap when push > gap, so the CCD stops the move and
+    the penalty takes over); block B's far face is held.  Returns (scene, dbc indices, dbc
+    targets (n, 3)).  Every other vertex starts at rest with target = rest."""
+    Xa, Ta = kuhn_block(n, n, n, spacing)
+    Xb = Xa + np.array([(n + gap) * spacing, 0.0, 0.0])
+    X = np.vstack([Xa, Xb])
+    T = np.vstack([Ta, Ta + Xa.shape[0]]).astype(np.int32)
+    face_a = np.nonzero(np.isclose(X[:, 0], n * spacing))[0]  # A's face toward B
+    face_b = np.nonzero(np.isclose(X[:, 0], X[:, 0].max()))[0]
+    idx = np.concatenate([face_a, face_b]).astype(np.int32)
+    tgt = X[idx].copy()
+    tgt[: face_a.size, 0] += push * spacing
+    sc = Scene(f"two_blocks{n}_push", X.copy(), T, X.copy())
+    return sc, idx, tgt

@@ -65,6 +65,9 @@ typedef struct idp_step_stats {
     double kappa;
     /* stage wall time in ms (stream-synchronous timings) */
     double ms_total, ms_constraint_set, ms_ccd, ms_energy, ms_gradient, ms_hessian, ms_solve;
+    /* moving DBC: the CCD-limited fraction of the move taken at the start of
+       the step (1: all of it), and the penalty stiffness at the end (0: off) */
+    double dbc_alpha, dbc_stiff;
 } idp_step_stats;
 
 /* Kernel-family profile: device time from HIP events recorded on the
@@ -127,6 +130,12 @@ int idp_set_mesh(idp_ctx* ctx, int nV, const double* X_rest, int nT, const int* 
                  const double* vol, const double* lambda, const double* mu, const double* mass);
 /* DBC vertex list (VECTOR_STORAGE DBC of the reference, fixed targets = current x). */
 int idp_set_dbc(idp_ctx* ctx, int n, const int* idx);
+/* Moving Dirichlet nodes: node idx[i] is driven to targets[3 i .. 3 i + 2]
+ * in every step (IMPLICIT_EULER.h:304-401, 438-474, 701-755: the CCD-limited
+ * move, then an augmented-Lagrangian penalty when the move is cut short).
+ * Replaces the set of idp_set_dbc; nodes may not repeat.  The reference keeps
+ * these as the rows (node, x, y, z) of its DBC storage. */
+int idp_set_dbc_targets(idp_ctx* ctx, int n, const int* idx, const double* targets);
 int idp_set_state(idp_ctx* ctx, const double* X, const double* v);
 int idp_get_state(idp_ctx* ctx, double* X, double* v);
 /* device-to-device save/restore of (x, v, kappa) for repeated solves */

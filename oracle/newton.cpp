@@ -170,6 +170,12 @@ struct Ctx {
     const double *IB, *vol, *lam, *mu;
     const int *bnode, *bedge, *btri;
     std::vector<unsigned char> dbc;
+    // moving DBC (Shell/IMPLICIT_EULER.h:304-401, Shell/DIRICHLET.h): DBC entries in order,
+    // their targets, the nodes with zero displacement, and the penalty stiffness
+    std::vector<int> dbc_idx;
+    std::vector<double> dbc_tgt;
+    std::vector<unsigned char> dbc_fixed;
+    double dbc_stiff = 0;
     double h, dHat2, thickness;
     double* kappa;
     bool withCollision;
@@ -201,6 +207,33 @@ void compute_cs(Ctx& c, const double* X)
     }
     c.t_cs += now() - t0;
     if (c.st) c.st->n_constraint_sets++;
+}
+
+// Shell/DIRICHLET.h:6-23
+double dbc_dist2(const Ctx& c, const double* X)
+{
+    std::vector<double> pen(c.dbc_idx.size(), 0.0);
+    for (size_t i = 0; i < c.dbc_idx.size(); ++i) {
+        const int v = c.dbc_idx[i];
+        for (int d = 0; d < 3; ++d) pen[i] += (c.dbc_tgt[3 * i + d] - X[3 * v + d]) * (c.dbc_tgt[3 * i + d] - X[3 * v + d]);
+    }
+    double s = 0.0;
+    for (double p : pen) s += p;
+    return s;
+}
+
+// Shell/DIRICHLET.h:25-48: E += 1/2 k sum m |x_D - x|^2
+void dbc_energy(const Ctx& c, const double* X, double& E)
+{
+    std::vector<double> pen(c.dbc_idx.size(), 0.0);
+    for (size_t i = 0; i < c.dbc_idx.size(); ++i) {
+        const int v = c.dbc_idx[i];
+        for (int d = 0; d < 3; ++d) pen[i] += (c.dbc_tgt[3 * i + d] - X[3 * v + d]) * (c.dbc_tgt[3 * i + d] - X[3 * v + d]);
+        pen[i] *= c.mass[v];
+    }
+    double s = 0.0;
+    for (double p : pen) s += p;
+    E += 0.5 * c.dbc_stiff * s;
 }
 
 // Compute_IncPotential (tet + inertia + barrier); returns validity
@@ -259,11 +292,22 @@ void assemble_sys(Ctx& c, const double* X, Sys& S)
         k.v[0] = k.v[4] = k.v[8] = c.mass[v];
         blks.push_back(k);
     }
+    if (c.dbc_stiff) {  // Shell/DIRICHLET.h:72-93
+        for (int v : c.dbc_idx) {
+            Blk k;
+            k.key = (long)v * nV + v;
+            std::memset(k.v, 0, sizeof(k.v));
+            k.v[0] = k.v[4] = k.v[8] = c.dbc_stiff * c.mass[v];
+            blks.push_back(k);
+        }
+    }
     build_sys(blks, nV, S);
-    // Project_DBC: rows/cols of DBC nodes -> identity
+    // Project_DBC: rows/cols of DBC nodes -> identity; under the penalty only
+    // the nodes that do not move (INC_POTENTIAL.h:387-394)
+    const std::vector<unsigned char>& pm = c.dbc_stiff ? c.dbc_fixed : c.dbc;
     for (int r = 0; r < nV; ++r)
         for (int k = S.rowptr[r]; k < S.rowptr[r + 1]; ++k)
-            if (c.dbc[r] || c.dbc[S.col[k]])
+            if (pm[r] || pm[S.col[k]])
                 for (int i = 0; i < 3; ++i)
                     for (int j = 0; j < 3; ++j) S.val[9 * (size_t)k + 3 * i + j] = (r == S.col[k] && i == j);
 }
@@ -274,8 +318,9 @@ extern "C" int orc_advance_one_step(int nV, double* X, const double* X0, double*
                                     int nT, const int* tets, const double* IB, const double* vol,
                                     const double* lam, const double* mu, int nb, const int* bnode, int ne,
                                     const int* bedge, int nf, const int* btri, int nDBC, const int* dbc_idx,
-                                    double h, double NewtonTol, int withCollision, double dHat2, double* kappa,
-                                    double thickness, double lin_rtol, int max_newton, orc_stats* st)
+                                    const double* dbc_target, double h, double NewtonTol, int withCollision,
+                                    double dHat2, double* kappa, double thickness, double lin_rtol, int max_newton,
+                                    orc_stats* st)
 {
     Ctx c;
     c.nV = nV; c.nT = nT; c.nb = nb; c.ne = ne; c.nf = nf;
@@ -292,9 +337,41 @@ extern "C" int orc_advance_one_step(int nV, double* X, const double* X0, double*
     // IMPLICIT_EULER.h:196-217 with zero body force: x~ = x + h v
     for (size_t i = 0; i < n3; ++i) c.Xtilde[i] = X[i] + (h * vel[i] + h * h * 0.0);
 
+    // Dirichlet nodes toward their targets (IMPLICIT_EULER.h:304-401): the
+    // CCD-limited fraction of the move; all of it when that is 1, otherwise
+    // none, and the penalty (augmented Lagrangian) pulls them in the loop
+    c.dbc_idx.assign(dbc_idx, dbc_idx + nDBC);
+    c.dbc_tgt.resize(3 * (size_t)nDBC);
+    c.dbc_fixed.assign(nV, 0);
+    std::vector<double> disp(n3, 0.0);
+    for (int i = 0; i < nDBC; ++i) {
+        const int v = dbc_idx[i];
+        for (int d = 0; d < 3; ++d) {
+            c.dbc_tgt[3 * i + d] = dbc_target ? dbc_target[3 * i + d] : X[3 * v + d];
+            disp[3 * v + d] = c.dbc_tgt[3 * i + d] - X[3 * v + d];
+        }
+        if (!(disp[3 * v] || disp[3 * v + 1] || disp[3 * v + 2])) c.dbc_fixed[v] = 1;
+    }
+    double DBCAlpha = 1.0;
+    if (c.withCollision) {
+        std::vector<unsigned char> dbcm = c.dbc;
+        orc_ccd_stepsize(nV, X, disp.data(), dbcm.data(), nb, bnode, ne, bedge, nf, btri, thickness, 1.0, &DBCAlpha,
+                         nullptr);
+    }
+    double DBCPenaltyXn = 0.0;
+    if (DBCAlpha == 1.0) {
+        for (int i = 0; i < nDBC; ++i)
+            for (int d = 0; d < 3; ++d) X[3 * dbc_idx[i] + d] = c.dbc_tgt[3 * i + d];
+    } else {
+        c.dbc_stiff = 1e6;
+        DBCPenaltyXn = dbc_dist2(c, Xn.data());
+    }
+    if (st) st->dbc_alpha = DBCAlpha;
+
     if (c.withCollision) compute_cs(c, X);
     double Eprev;
     if (!energy(c, X, c.withCollision, Eprev)) return ORC_E_INVALID_START;
+    if (c.dbc_stiff) dbc_energy(c, X, Eprev);
     std::deque<double> resRecord;
     std::vector<double> g(n3), rhs(n3), sol(n3), Xprev(n3), dist2, dist2_prev;
     std::vector<int> cs_prev;
@@ -311,8 +388,17 @@ extern "C" int orc_advance_one_step(int nV, double* X, const double* X0, double*
             for (int d = 0; d < 3; ++d) g[3 * v + d] += mass[v] * (X[3 * v + d] - c.Xtilde[3 * v + d]);
         if (c.withCollision && c.ncs)
             orc_barrier_gradient(X, X0, c.ncs, c.cs.data(), dHat2, kappa[0], thickness, nV, g.data());
-        for (int v = 0; v < nV; ++v)
-            if (c.dbc[v]) g[3 * v] = g[3 * v + 1] = g[3 * v + 2] = 0;
+        if (c.dbc_stiff) {  // Shell/DIRICHLET.h:50-70, then the fixed nodes' rows (IMPLICIT_EULER.h:467-474)
+            for (int i = 0; i < nDBC; ++i) {
+                const int v = dbc_idx[i];
+                for (int d = 0; d < 3; ++d) g[3 * v + d] += c.dbc_stiff * mass[v] * (X[3 * v + d] - c.dbc_tgt[3 * i + d]);
+            }
+            for (int v = 0; v < nV; ++v)
+                if (c.dbc_fixed[v]) g[3 * v] = g[3 * v + 1] = g[3 * v + 2] = 0;
+        } else {
+            for (int v = 0; v < nV; ++v)
+                if (c.dbc[v]) g[3 * v] = g[3 * v + 1] = g[3 * v + 2] = 0;
+        }
         for (size_t i = 0; i < n3; ++i) rhs[i] = -g[i];
         c.t_grad += now() - t0;
 
@@ -373,6 +459,7 @@ extern "C" int orc_advance_one_step(int nV, double* X, const double* X0, double*
                     }
                 }
                 valid = energy(c, X, c.withCollision, E);
+                if (valid && c.dbc_stiff) dbc_energy(c, X, E);
             }
             alpha /= 2.0;
             if (st) st->line_search_trials++;
@@ -394,6 +481,7 @@ extern "C" int orc_advance_one_step(int nV, double* X, const double* X0, double*
             if (upd && kappa[0] < kappa[1]) {
                 kappa[0] *= 2;
                 energy(c, X, c.withCollision, Eprev);
+                if (c.dbc_stiff) dbc_energy(c, X, Eprev);
             }
         }
         dist2.assign(c.ncs, 0.0);
@@ -417,11 +505,30 @@ extern "C" int orc_advance_one_step(int nV, double* X, const double* X0, double*
         if (useGD) L2Norm = NewtonTol * 10;
         if (alpha * 2 < 1e-8 && feasibleAlpha > 1e-8) useGD = true;
         else useGD = false;
+        // moving DBC: raise the penalty near convergence until the nodes are
+        // within 1% of their move, then drop it (IMPLICIT_EULER.h:701-755)
+        if (c.dbc_stiff) {
+            const double progress = 1 - std::sqrt(dbc_dist2(c, X) / DBCPenaltyXn);
+            if (progress < 0.99) {
+                if (L2Norm < NewtonTol * 10) {
+                    if (c.dbc_stiff < 1e8) {
+                        c.dbc_stiff *= 2;
+                        energy(c, X, c.withCollision, Eprev);
+                        dbc_energy(c, X, Eprev);
+                    }
+                }
+                L2Norm = NewtonTol * 10;
+            } else {
+                c.dbc_stiff = 0;
+                energy(c, X, c.withCollision, Eprev);
+            }
+        }
         if (st) {
             st->n_newton = PNIter;
             st->n_constraints = c.ncs;
             st->last_res = L2Norm;
             st->E = Eprev;
+            st->dbc_stiff = c.dbc_stiff;
         }
         if (max_newton > 0 && PNIter >= max_newton) break;
     } while (resRecord.size() < 3 || L2Norm > NewtonTol);

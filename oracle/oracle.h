@@ -17,6 +17,7 @@ typedef struct orc_stats {
     long n_constraints;
     double last_alpha, first_ccd_alpha, last_res, last_relres, min_dist2, E;
     double t_constraint_set, t_ccd, t_solve, t_hessian, t_gradient, t_energy;
+    double dbc_alpha, dbc_stiff; /* moving DBC: CCD-limited move at the start, penalty stiffness at the end */
 } orc_stats;
 
 int orc_surface(int nV, const double* X, int nT, const int* tets, int* nb, int* ne, int* nf, int* bnode,
@@ -51,9 +52,9 @@ long orc_newton_system(int nV, const double* X, const double* Xtilde, const doub
 int orc_advance_one_step(int nV, double* X, const double* X0, double* vel, const double* mass, int nT,
                          const int* tets, const double* IB, const double* vol, const double* lam,
                          const double* mu, int nb, const int* bnode, int ne, const int* bedge, int nf,
-                         const int* btri, int nDBC, const int* dbc_idx, double h, double NewtonTol,
-                         int withCollision, double dHat2, double* kappa, double thickness, double lin_rtol,
-                         int max_newton, orc_stats* st);
+                         const int* btri, int nDBC, const int* dbc_idx, const double* dbc_target, double h,
+                         double NewtonTol, int withCollision, double dHat2, double* kappa, double thickness,
+                         double lin_rtol, int max_newton, orc_stats* st);
 
 #ifdef __cplusplus
 }

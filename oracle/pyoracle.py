@@ -39,6 +39,8 @@ class Stats(C.Structure):
         ("t_hessian", C.c_double),
         ("t_gradient", C.c_double),
         ("t_energy", C.c_double),
+        ("dbc_alpha", C.c_double),
+        ("dbc_stiff", C.c_double),
     ]
 
     def as_dict(self):
@@ -77,7 +79,7 @@ def lib():
         L.orc_elastic_hessian_local.argtypes = [dp, C.c_int, ip, dp, dp, dp, dp, C.c_double, C.c_int, dp]
         L.orc_elastic_hessian_local.restype = None
         L.orc_advance_one_step.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int, ip,
-                                           C.c_int, ip, C.c_int, ip, C.c_int, ip, C.c_double, C.c_double,
+                                           C.c_int, ip, C.c_int, ip, C.c_int, ip, C.c_void_p, C.c_double, C.c_double,
                                            C.c_int, C.c_double, dp, C.c_double, C.c_double, C.c_int,
                                            C.POINTER(Stats)]
         L.orc_newton_system.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int, ip, C.c_int,
@@ -214,15 +216,18 @@ class Mesh:
             cap = -n
 
     def advance(self, X, vel, h, tol, dHat2, kappa, thickness=0.0, lin_rtol=1e-12, max_newton=0,
-                with_collision=True, dbc_idx=None):
+                with_collision=True, dbc_idx=None, dbc_target=None):
+        """One step; dbc_target (n, 3) moves the DBC nodes dbc_idx (None: they stay)."""
         X = _f(X).copy()
         vel = _f(vel).copy()
         kap = np.ascontiguousarray(kappa, dtype=np.float64).copy()
         dbc_idx = np.zeros(0, np.int32) if dbc_idx is None else _i(dbc_idx)
+        tgt = None if dbc_target is None else _f(dbc_target)
         st = Stats()
         it = lib().orc_advance_one_step(self.nV, X, self.X0, vel, self.mass, self.nT, self.tets, self.IB, self.vol,
                                         self.lam, self.mu, self.bnode.size, self.bnode, self.bedge.shape[0],
                                         _i(self.bedge), self.btri.shape[0], _i(self.btri), dbc_idx.size, dbc_idx,
+                                        None if tgt is None else tgt.ctypes.data_as(C.c_void_p),
                                         h, tol, int(with_collision), dHat2, kap, thickness, lin_rtol, max_newton,
                                         C.byref(st))
         return it, X.reshape(-1, 3), vel.reshape(-1, 3), kap, st.as_dict()

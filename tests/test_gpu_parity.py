@@ -251,3 +251,53 @@ def test_advance_with_dbc_matches_oracle():
     assert it == it_o
     assert np.array_equal(X[fixed], sc.X[fixed])
     assert np.abs(X - Xo).max() / sc.diag < 1e-6
+
+
+@pytest.mark.parametrize("push,gap", [(0.1, 0.5), (0.5, 0.2)])
+def test_advance_with_moving_dbc_matches_oracle(push, gap):
+    """Moving DBC (IMPLICIT_EULER.h:304-401, 701-755): one block's face is
+    driven toward a second block.  With push < gap the whole move fits and the
+    nodes are set to their targets; with push > gap the CCD cuts the move
+    short, the augmented-Lagrangian penalty pulls the nodes in, its stiffness
+    doubles near convergence and is dropped once they are within 1% of the
+    move.  Same CCD fraction (bit for bit), Newton count and end state as the
+    oracle."""
+    sc, idx, tgt = scenes.two_blocks_push(push=push, gap=gap)
+    m = pyoracle.Mesh(sc.X, sc.tets, sc.density, sc.E, sc.nu)
+    dHat2, kappa = pyoracle.initialize_oipc(sc.dhat, m.mass)
+    it_o, Xo, _, _, st_o = m.advance(sc.X, sc.velocity, sc.h, sc.newton_tol, dHat2, kappa, dbc_idx=idx,
+                                     dbc_target=tgt)
+    say(f"[mdbc] push {push} gap {gap}: oracle newton {it_o}, dbc alpha {st_o['dbc_alpha']}")
+    d = driver.TetAnimFix(sc, pcg_rtol=1e-12)
+    d.ctx.set_dbc_targets(idx, tgt)
+    it, st = d.advance_one_time_step()
+    X, _ = d.state()
+    say(f"[mdbc] gpu newton {it}, dbc alpha {st['dbc_alpha']}, stiffness {st['dbc_stiff']}")
+    assert st["dbc_alpha"] == st_o["dbc_alpha"]
+    assert st["dbc_stiff"] == st_o["dbc_stiff"] == 0.0
+    if push < gap:
+        assert it == it_o
+        assert np.abs(X - Xo).max() / sc.diag < 1e-6
+        assert np.array_equal(X[idx], tgt)
+    else:
+        # The penalty path ends on discrete decisions (stiffness doubling,
+        # the 99% switch-off, kappa doubling), so its end state is set only to
+        # about the Newton tolerance: the oracle's own end state moves by
+        # 6e-4 of the diagonal between linear tolerances 1e-8 and 1e-12 here.
+        err = np.abs(X - Xo).max() / sc.diag
+        say(f"[mdbc] end state vs oracle: {err:.2e} of the diagonal")
+        # the penalty's energy, gradient and Hessian pinned before any switch:
+        # the first three Newton iterations agree to 1e-6 of the diagonal
+        _, X3o, _, _, _ = m.advance(sc.X, sc.velocity, sc.h, sc.newton_tol, dHat2, kappa, dbc_idx=idx,
+                                    dbc_target=tgt, max_newton=3)
+        d3 = driver.TetAnimFix(sc, pcg_rtol=1e-12)
+        d3.ctx.set_dbc_targets(idx, tgt)
+        d3.advance_one_time_step(max_newton=3)
+        X3, _ = d3.state()
+        err3 = np.abs(X3 - X3o).max() / sc.diag
+        say(f"[mdbc] after 3 Newton iterations: {err3:.2e} of the diagonal")
+        assert err3 < 1e-6
+        assert abs(it - it_o) <= 6
+        assert err < 2e-3
+        move = np.abs(tgt - sc.X[idx]).max()
+        assert np.abs(X[idx] - tgt).max() <= 0.01 * move

@@ -133,3 +133,32 @@ def test_neohookean(case):
         K = m.elastic_hessian_local(Xd, w, project=project)[0].reshape(12, 12)
         Ke = w * vol * D.T @ ref @ D
         assert np.abs(K - Ke).max() <= 1e-9 * np.abs(Ke).max()
+
+
+def test_oracle_moving_dbc_paths():
+    """Moving DBC in the oracle: a move that fits sets the nodes to their
+    targets; one that the CCD cuts short goes through the penalty, which ends
+    off with the nodes within 1% of the move; targets equal to the current
+    positions give the static-DBC step (the oracle's threaded sums vary in the
+    last bits from run to run, so to 1e-14 of the diagonal)."""
+    from idp_amd import scenes
+
+    for push, gap in ((0.1, 0.5), (0.5, 0.2)):
+        sc, idx, tgt = scenes.two_blocks_push(push=push, gap=gap)
+        m = pyoracle.Mesh(sc.X, sc.tets, sc.density, sc.E, sc.nu)
+        dHat2, kappa = pyoracle.initialize_oipc(sc.dhat, m.mass)
+        it, X, _, _, st = m.advance(sc.X, sc.velocity, sc.h, sc.newton_tol, dHat2, kappa, dbc_idx=idx,
+                                    dbc_target=tgt)
+        move = np.abs(tgt - sc.X[idx]).max()
+        if push < gap:
+            assert st["dbc_alpha"] == 1.0
+            assert np.array_equal(X[idx], tgt)
+        else:
+            assert 0.0 < st["dbc_alpha"] < 1.0
+            assert st["dbc_stiff"] == 0.0
+            assert np.abs(X[idx] - tgt).max() <= 0.01 * move
+        # targets at the start positions: the static path
+        it_s, Xs, _, _, _ = m.advance(sc.X, sc.velocity, sc.h, sc.newton_tol, dHat2, kappa, dbc_idx=idx)
+        it_t, Xt, _, _, _ = m.advance(sc.X, sc.velocity, sc.h, sc.newton_tol, dHat2, kappa, dbc_idx=idx,
+                                      dbc_target=sc.X[idx])
+        assert it_s == it_t and np.abs(Xs - Xt).max() <= 1e-14 * sc.diag
