@@ -498,6 +498,20 @@ struct BatchEngine {
     bool pcg_handover = true;
     DBuf<double> dalpha;
     std::vector<EnergyOut> eo;
+    // stage wall times (IDP_STAGE_LOG=1: the stream is drained at each stage
+    // boundary, so each stage is charged its own GPU work)
+    static constexpr int NSTAGE = 7;
+    bool stage_log = false;
+    double stage_ms[NSTAGE] = {};
+    double stage_t = 0;
+    void stage(int k)
+    {
+        if (!stage_log) return;
+        IDP_HIP(hipStreamSynchronize(stream));
+        const double t = now_ms();
+        if (k >= 0) stage_ms[k] += t - stage_t;
+        stage_t = t;
+    }
 
     int configure(idp_ctx* c, double h_, bool coll_, double th_, const idp_solver_opts* opts)
     {
@@ -513,6 +527,9 @@ struct BatchEngine {
         cold = opts && opts->pcg_cold_start;
         accd_budget = !opts || opts->accd_budget == 0 ? ACCD_STEP_BUDGET : (opts->accd_budget < 0 ? 0 : opts->accd_budget);
         pcg_handover = !opts || opts->pcg_handover >= 0;
+        const char* sl = std::getenv("IDP_STAGE_LOG");
+        stage_log = sl && std::atoi(sl) > 0;
+        std::fill(stage_ms, stage_ms + NSTAGE, 0.0);
         kappa.assign(B, 0);
         kmax.assign(B, 0);
         tol.assign(B, 0);
@@ -670,6 +687,7 @@ struct BatchEngine {
         for (int i = 0; i < B; ++i) mask[i] = fresh[i];
         const unsigned char* dact = up_mask();
         // ---- gradient and Hessian at x with the current constraint sets
+        stage(-1);
         double t0 = now_ms();
         if (coll) contact_structure(c->L, s, m.nV, c->cw.ncs, (const int*)c->cw.cs.p);
         else c->L.ncb = 0;
@@ -680,6 +698,7 @@ struct BatchEngine {
         std::vector<double> rel(B, 0.0);
         if (anyN) {
             hessian(c, x, h, coll);
+            stage(0);
             std::vector<unsigned char> sm(B);
             std::vector<unsigned char> wm(B);
             for (int i = 0; i < B; ++i) {
@@ -699,6 +718,7 @@ struct BatchEngine {
                 }
             }
         }
+        stage(1);
         // ls: instances that go on to the line search in this call
         std::vector<char> ls(B, 0);
         int nls = 0;
@@ -738,6 +758,7 @@ struct BatchEngine {
         for (int i = 0; i < B; ++i) mask[i] = ls[i];
         dact = up_mask();
         IDP_LAUNCH(k_copy_inst, vgrid, 256, 0, s, m.nV, I.vinst.p, dact, xprev, x);
+        stage(2);
         // go: instances whose step size is final in this call
         std::vector<char> go(ls);
         if (coll) {
@@ -758,6 +779,7 @@ struct BatchEngine {
                     if (n_active == 1) S[i].ccd_candidates += nc;
                 }
         }
+        stage(3);
         for (int i = 0; i < B; ++i)
             if (go[i]) {
                 feasible[i] = alpha[i];
@@ -819,6 +841,7 @@ struct BatchEngine {
                 Eprev[i] = E[i];
                 S[i].last_alpha = alpha[i] * 2;
             }
+        stage(4);
         // ---- kappa update (IMPLICIT_EULER.h:566-595), instances past their
         // first iteration (the previous constraint set is theirs)
         if (coll && c->ncs_prev) {
@@ -899,6 +922,7 @@ struct BatchEngine {
             IDP_LAUNCH(k_velocity_inst, vgrid, 256, 0, s, m.nV, I.vinst.p, up_mask(), x, c->xn.p, h, c->v.p);
             IDP_HIP(hipStreamSynchronize(s));
         }
+        stage(5);
         c->prof.checkpoint();
         return IDP_OK;
     }
@@ -1956,6 +1980,12 @@ int idp_advance_batch(idp_ctx* c, double h, const double* newton_tol, int with_c
             if (verbose > 0 && k % verbose == 0) {
                 std::fprintf(stderr, "[idp] batch Newton iteration %d: %d of %d instances active, %.2f s\n", k,
                              E.n_active, B, (now_ms() - t0) * 1e-3);
+                if (E.stage_log)
+                    std::fprintf(stderr,
+                                 "[idp]   stage s: grad+hess %.2f pcg %.2f ls1 %.2f ccd %.2f ls2 %.2f "
+                                 "kappa+stop %.2f\n",
+                                 E.stage_ms[0] * 1e-3, E.stage_ms[1] * 1e-3, E.stage_ms[2] * 1e-3,
+                                 E.stage_ms[3] * 1e-3, E.stage_ms[4] * 1e-3, E.stage_ms[5] * 1e-3);
                 std::fflush(stderr);
             }
         }
