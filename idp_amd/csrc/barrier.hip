@@ -768,6 +768,7 @@ __global__ void __launch_bounds__(256) k_barrier_grad(long n, const int4* __rest
     if (i >= n) return;
     Decoded k = decode(cs[i]);
     const int inst = bp.inst_of(k.v[0]);
+    if (bp.skip(inst)) return;
     IDP_KIND_SWITCH(k.kind, grad_body, k, X, X0, bp.dh[inst], bp.kappa[inst], bp.th2[inst], gout + 12 * i,
                     err + inst)
 }
@@ -787,6 +788,7 @@ __global__ void __launch_bounds__(WS) k_barrier_hess(long n, const int4* __restr
     if (i >= n) return;
     Decoded k = decode(cs[i]);
     const int inst = bp.inst_of(k.v[0]);
+    if (bp.skip(inst)) return;
     IDP_KIND_SWITCH(k.kind, hess_body, k, X, X0, bp.dh[inst], bp.kappa[inst], bp.th2[inst], project_spd, mode,
                     out + 144 * i, err + inst, counts,
                     wsh + threadIdx.x)

@@ -280,14 +280,16 @@ double min_dist2_dev(idp_ctx* c, const double* X, long n, const int* cs, double*
     return d;
 }
 
-// gradient at X with the current constraint set (structure must be built)
-void gradient(idp_ctx* c, const double* X, double h, bool with_barrier, double* g, double* rhs)
+// gradient at X with the current constraint set (structure must be built);
+// act (device, one byte per instance; null = all): the instances it is formed for
+void gradient(idp_ctx* c, const double* X, double h, bool with_barrier, double* g, double* rhs,
+              const unsigned char* act = nullptr)
 {
     auto s = c->s;
     auto& m = c->m;
     double* gl = c->gl.ensure(12 * (size_t)m.nT, s);
     const int tok = c->prof.begin(IDP_K_GRADIENT);
-    launch_tet_grad(s, m, X, h * h, gl);
+    launch_tet_grad(s, m, X, h * h, gl, act);
     const double* gb = nullptr;
     long ncs = c->cw.ncs;
     if (with_barrier && ncs) {
@@ -295,21 +297,25 @@ void gradient(idp_ctx* c, const double* X, double h, bool with_barrier, double* 
         int* fl = c->flags.ensure(4 * (size_t)B + 4, s) + 2 * B;
         IDP_HIP(hipMemsetAsync(fl, 0, B * sizeof(int), s));
         double* b = c->gb.ensure(12 * (size_t)ncs, s);
-        launch_barrier_grad(s, ncs, (const int*)c->cw.cs.p, X, m.X0.p, bparams(c), b, fl);
+        BarrierParams bp = bparams(c);
+        bp.act = act;
+        launch_barrier_grad(s, ncs, (const int*)c->cw.cs.p, X, m.X0.p, bp, b, fl);
         gb = b;
     }
-    contact_gradient_gather(c->L, s, m, X, c->xt.p, gl, gb, g, rhs);
+    contact_gradient_gather(c->L, s, m, X, c->xt.p, gl, gb, g, rhs, act);
     c->prof.end(IDP_K_GRADIENT, tok);
 }
 
-void hessian(idp_ctx* c, const double* X, double h, bool with_barrier)
+// Hessian and assembly at X; act as for gradient(), nT_act: tets of those instances
+void hessian(idp_ctx* c, const double* X, double h, bool with_barrier, const unsigned char* act = nullptr,
+             long nT_act = -1)
 {
     auto s = c->s;
     auto& m = c->m;
     double* Hl = c->Hl.ensure(144 * (size_t)m.nT, s);
     int tok = c->prof.begin(IDP_K_TET_HESS);
-    launch_tet_hess(s, m, X, h * h, 1, Hl);
-    c->prof.end(IDP_K_TET_HESS, tok, 6500.0 * m.nT);
+    launch_tet_hess(s, m, X, h * h, 1, Hl, act);
+    c->prof.end(IDP_K_TET_HESS, tok, 6500.0 * (nT_act < 0 ? m.nT : nT_act));
     const double* Hb = nullptr;
     long ncs = c->cw.ncs;
     if (with_barrier && ncs) {
@@ -317,12 +323,14 @@ void hessian(idp_ctx* c, const double* X, double h, bool with_barrier)
         int* fl = c->flags.ensure(4 * (size_t)B + 4, s) + 3 * B;
         double* b = c->Hb.ensure(144 * (size_t)ncs, s);
         tok = c->prof.begin(IDP_K_BARRIER_HESS);
-        launch_barrier_hess(s, ncs, (const int*)c->cw.cs.p, X, m.X0.p, bparams(c), 1, 1, b, nullptr, fl,
+        BarrierParams bp = bparams(c);
+        bp.act = act;
+        launch_barrier_hess(s, ncs, (const int*)c->cw.cs.p, X, m.X0.p, bp, 1, 1, b, nullptr, fl,
                             c->prof.on ? c->bcounts.p : nullptr);
         c->prof.end(IDP_K_BARRIER_HESS, tok);
         Hb = b;
     }
-    assemble(c->L, s, m, Hl, Hb, c->ndbc > 0);
+    assemble(c->L, s, m, Hl, Hb, c->ndbc > 0, act);
 }
 
 void set_constraints_from_host(idp_ctx* c, long n, const int* cs)
@@ -684,27 +692,31 @@ struct BatchEngine {
             anyres |= was_pend[i] != 0;
             anycont |= was_ppend[i] != 0;
         }
-        for (int i = 0; i < B; ++i) mask[i] = fresh[i];
+        // ---- gradient and Hessian at x with the current constraint sets, formed
+        // for the instances that iterate now (a continued PCG solve's system is
+        // formed again: its constraint and contact-block positions may move)
+        for (int i = 0; i < B; ++i) mask[i] = fresh[i] || was_ppend[i];
         const unsigned char* dact = up_mask();
-        // ---- gradient and Hessian at x with the current constraint sets
         stage(-1);
         double t0 = now_ms();
         if (coll) contact_structure(c->L, s, m.nV, c->cw.ncs, (const int*)c->cw.cs.p);
         else c->L.ncb = 0;
-        if (nfresh) gradient(c, x, h, coll, g, rhs);
+        if (nfresh) gradient(c, x, h, coll, g, rhs, dact);
         bool anyN = anycont;
         for (int i = 0; i < B; ++i) anyN |= fresh[i] && !useGD[i];
         std::vector<int> its(B, 0);
         std::vector<double> rel(B, 0.0);
         if (anyN) {
-            hessian(c, x, h, coll);
-            stage(0);
             std::vector<unsigned char> sm(B);
             std::vector<unsigned char> wm(B);
+            long nT_act = 0;
             for (int i = 0; i < B; ++i) {
                 sm[i] = fresh[i] && !useGD[i] ? 1 : (was_ppend[i] ? 2 : 0);
                 wm[i] = sm[i] == 1 && PN[i] > 0 && !cold;  // sol holds this step's previous direction
+                if (sm[i]) nT_act += I.tbeg[i + 1] - I.tbeg[i];
             }
+            hessian(c, x, h, coll, dmask2.upload_get(sm.data(), B, s), nT_act);
+            stage(0);
             pcg_solve_batch(c->L, s, m, rhs, sol, rtol, maxit, sm.data(), its.data(), rel.data(),
                             dmask3.upload_get(wm.data(), B, s), pcg_handover);
             for (int i = 0; i < B; ++i) {
@@ -1075,6 +1087,7 @@ int idp_ctx_create(int device, idp_ctx** out)
             const int k = std::atoi(e);
             if (k >= 0 && k < 8) c->L.spmv_shape = k;
         }
+        if (const char* e = std::getenv("IDP_SPMV_PIPE")) c->L.spmv_pipe2 = std::atoi(e) > 0;
     } catch (const HipError& e) {
         delete c->eng;
         delete c;

@@ -208,14 +208,17 @@ __global__ void __launch_bounds__(RB) k_tet_energy(const int* __restrict__ lo, c
 __global__ void __launch_bounds__(256) k_tet_grad(int nT, const int4* __restrict__ tets,
                                                   const double* __restrict__ IB, const double* __restrict__ vol,
                                                   const double* __restrict__ lam, const double* __restrict__ mu,
-                                                  const double* __restrict__ X, double w, double* __restrict__ gl)
+                                                  const double* __restrict__ X, double w, const int* __restrict__ vinst,
+                                                  const unsigned char* __restrict__ act, double* __restrict__ gl)
 {
 #pragma clang fp contract(fast)
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nT) return;
+    const int4 tt = tets[t];
+    if (act && !act[vinst[tt.x]]) return;  // instance not iterating
     double F[9], C[9], P[9];
     const double* ib = IB + 9 * (size_t)t;
-    def_grad(X, tets[t], ib, F);
+    def_grad(X, tt, ib, F);
     double J = det3(F);
     cof3(F, C);
     double lj = log(J), iJ = 1.0 / J, wv = w * vol[t], m = mu[t], l = lam[t];
@@ -243,14 +246,17 @@ __global__ void __launch_bounds__(128) k_tet_hess(int nT, const int4* __restrict
                                                   const double* __restrict__ IB, const double* __restrict__ vol,
                                                   const double* __restrict__ lam, const double* __restrict__ mu,
                                                   const double* __restrict__ X, double w, int project, int dense,
+                                                  const int* __restrict__ vinst, const unsigned char* __restrict__ act,
                                                   double* __restrict__ Hl)
 {
 #pragma clang fp contract(fast)
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nT) return;
+    const int4 tt = tets[t];
+    if (act && !act[vinst[tt.x]]) return;  // instance not iterating
     double F[9], U[9], s[3], V[9];
     const double* ib = IB + 9 * (size_t)t;
-    def_grad(X, tets[t], ib, F);
+    def_grad(X, tt, ib, F);
     svd3(F, U, s, V);
     const double m = mu[t], l = lam[t], wv = w * vol[t];
     // NEOHOOKEAN.h:154-189: d psi/d sigma, d2 psi/d sigma2, B-block coefficients
@@ -375,20 +381,21 @@ void launch_tet_energy(hipStream_t s, const MeshDev& m, const SegDev& seg, const
     IDP_LAUNCH(k_tet_energy, seg.nblk, RB, 0, s, seg.lo.p, seg.hi.p, seg.inst.p, (const int4*)m.tets.p, m.IB.p,
                m.vol.p, m.lam.p, m.mu.p, X, w, part, invalid);
 }
-void launch_tet_grad(hipStream_t s, const MeshDev& m, const double* X, double w, double* gl)
+void launch_tet_grad(hipStream_t s, const MeshDev& m, const double* X, double w, double* gl, const unsigned char* act)
 {
     IDP_LAUNCH(k_tet_grad, grid_for(m.nT, 256), 256, 0, s, m.nT, (const int4*)m.tets.p, m.IB.p, m.vol.p, m.lam.p,
-               m.mu.p, X, w, gl);
+               m.mu.p, X, w, m.inst.vinst.p, act, gl);
 }
-void launch_tet_hess(hipStream_t s, const MeshDev& m, const double* X, double w, int project, double* Hl)
+void launch_tet_hess(hipStream_t s, const MeshDev& m, const double* X, double w, int project, double* Hl,
+                     const unsigned char* act)
 {
     IDP_LAUNCH(k_tet_hess, grid_for(m.nT, 128), 128, 0, s, m.nT, (const int4*)m.tets.p, m.IB.p, m.vol.p, m.lam.p,
-               m.mu.p, X, w, project, 0, Hl);
+               m.mu.p, X, w, project, 0, m.inst.vinst.p, act, Hl);
 }
 void launch_tet_hess_dense(hipStream_t s, const MeshDev& m, const double* X, double w, int project, double* Hl)
 {
     IDP_LAUNCH(k_tet_hess, grid_for(m.nT, 128), 128, 0, s, m.nT, (const int4*)m.tets.p, m.IB.p, m.vol.p, m.lam.p,
-               m.mu.p, X, w, project, 1, Hl);
+               m.mu.p, X, w, project, 1, m.inst.vinst.p, (const unsigned char*)nullptr, Hl);
 }
 void launch_inertia_energy(hipStream_t s, const SegDev& seg, const double* X, const double* Xt, const double* mass,
                            double* part)

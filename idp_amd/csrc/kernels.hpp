@@ -138,7 +138,9 @@ struct BarrierParams {
     const double* kappa;
     const double* th2;    // thickness^2
     const int* vinst;
+    const unsigned char* act;  // instances whose constraints are evaluated (null = all); gradient and Hessian only
     __device__ int inst_of(int v) const { return vinst ? vinst[v] : 0; }
+    __device__ bool skip(int inst) const { return act && !act[inst]; }
 };
 void barrier_init_constants();
 void launch_barrier_energy(hipStream_t s, int nblk, const int* lo, const int* hi, const int* cs, const double* X,
@@ -176,8 +178,11 @@ void build_inst_layout(MeshDev& m, const std::vector<long>& vbeg, const std::vec
 // fem.hip (energies over block tables: lo/hi element ranges, binst block -> instance)
 void launch_tet_energy(hipStream_t s, const MeshDev& m, const SegDev& seg, const double* X, double w, double* part,
                        int* invalid);
-void launch_tet_grad(hipStream_t s, const MeshDev& m, const double* X, double w, double* gl);
-void launch_tet_hess(hipStream_t s, const MeshDev& m, const double* X, double w, int project, double* Hl);
+// act (device, one byte per instance; null = all): instances whose elements are evaluated
+void launch_tet_grad(hipStream_t s, const MeshDev& m, const double* X, double w, double* gl,
+                     const unsigned char* act = nullptr);
+void launch_tet_hess(hipStream_t s, const MeshDev& m, const double* X, double w, int project, double* Hl,
+                     const unsigned char* act = nullptr);
 void launch_inertia_energy(hipStream_t s, const SegDev& seg, const double* X, const double* Xt, const double* mass,
                            double* part);
 

@@ -25,10 +25,12 @@ namespace idp {
 __global__ void k_elastic_values(long nnzb, const int* __restrict__ rowof, const int* __restrict__ col,
                                  const int* __restrict__ sptr, const int* __restrict__ src,
                                  const double* __restrict__ Hl, const double* __restrict__ mass,
+                                 const int* __restrict__ vinst, const unsigned char* __restrict__ act,
                                  double* __restrict__ val)
 {
     long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nnzb) return;
+    if (act && !act[vinst[rowof[k]]]) return;  // instance not iterating
     double acc[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) acc[i] = 0.0;
@@ -56,10 +58,12 @@ __global__ void k_gradient_gather(int nV, const int* __restrict__ ediag, const i
                                   const double* __restrict__ mass, const int* __restrict__ cdiag,
                                   const unsigned* __restrict__ cseg, const unsigned* __restrict__ cval,
                                   const double* __restrict__ gb, const unsigned char* __restrict__ dbc,
+                                  const int* __restrict__ vinst, const unsigned char* __restrict__ act,
                                   double* __restrict__ g, double* __restrict__ rhs)
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nV) return;
+    if (act && !act[vinst[v]]) return;  // instance not iterating
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     if (gl) {
         int k = ediag[v];
@@ -167,10 +171,13 @@ __global__ void k_rowptr_from_rows(unsigned nb, const int* __restrict__ row, int
 }
 
 __global__ void k_contact_values(unsigned nb, const unsigned* __restrict__ seg, const unsigned* __restrict__ vals,
-                                 const double* __restrict__ Hb, double* __restrict__ cval)
+                                 const double* __restrict__ Hb, const int* __restrict__ crow,
+                                 const int* __restrict__ vinst, const unsigned char* __restrict__ act,
+                                 double* __restrict__ cval)
 {
     unsigned b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
+    if (act && !act[vinst[crow[b]]]) return;  // instance not iterating
     double acc[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) acc[i] = 0.0;
@@ -257,12 +264,20 @@ __device__ inline double cblock_norm2(const double* __restrict__ val, long ld, i
     return w;
 }
 
+// vertices of the instances that solve (act null: all); every cluster lies
+// in one instance, so a skipped vertex is never read by a kept one
+struct ClMask {
+    const int* vinst;
+    const unsigned char* act;
+    __device__ bool skip(int v) const { return act && !act[vinst[v]]; }
+};
+
 // strongest contact neighbour of every vertex (ties: smaller id); -1 if none
 __global__ void k_cl_best(int nV, const int* __restrict__ crp, const int* __restrict__ ccol,
-                          const double* __restrict__ cval, long cld, int* __restrict__ best)
+                          const double* __restrict__ cval, long cld, ClMask cm, int* __restrict__ best)
 {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nV) return;
+    if (v >= nV || cm.skip(v)) return;
     int b = -1;
     double bw = 0.0;
     if (crp)
@@ -278,10 +293,11 @@ __global__ void k_cl_best(int nV, const int* __restrict__ crp, const int* __rest
     best[v] = b;
 }
 
-__global__ void k_cl_pair(int nV, const int* __restrict__ best, int* __restrict__ root, int* __restrict__ size)
+__global__ void k_cl_pair(int nV, const int* __restrict__ best, ClMask cm, int* __restrict__ root,
+                          int* __restrict__ size)
 {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nV) return;
+    if (v >= nV || cm.skip(v)) return;
     const int b = best[v];
     const bool pair = b >= 0 && best[b] == v;
     root[v] = pair ? min(v, b) : v;
@@ -293,10 +309,10 @@ __global__ void k_cl_pair(int nV, const int* __restrict__ best, int* __restrict_
 // a 64-bit max picks the heaviest, then the lowest root
 __global__ void k_cl_best2(int nV, const int* __restrict__ crp, const int* __restrict__ ccol,
                            const double* __restrict__ cval, long cld, const int* __restrict__ root,
-                           const int* __restrict__ size, unsigned long long* __restrict__ pack)
+                           const int* __restrict__ size, ClMask cm, unsigned long long* __restrict__ pack)
 {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nV || !crp) return;
+    if (v >= nV || !crp || cm.skip(v)) return;
     const int r = root[v];
     unsigned long long bestp = 0;
     for (int k = crp[v]; k < crp[v + 1]; ++k) {
@@ -311,10 +327,10 @@ __global__ void k_cl_best2(int nV, const int* __restrict__ crp, const int* __res
 }
 
 __global__ void k_cl_merge(int nV, int* __restrict__ root, const unsigned long long* __restrict__ pack,
-                           int* __restrict__ size, int* __restrict__ root2)
+                           int* __restrict__ size, ClMask cm, int* __restrict__ root2)
 {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nV) return;
+    if (v >= nV || cm.skip(v)) return;
     const int r = root[v];
     int nr = r;
     const unsigned long long p = pack[r];
@@ -326,20 +342,22 @@ __global__ void k_cl_merge(int nV, int* __restrict__ root, const unsigned long l
     root2[v] = nr;
 }
 
-__global__ void k_cl_members(int nV, const int* __restrict__ root, int* __restrict__ cnt, int* __restrict__ mem)
+__global__ void k_cl_members(int nV, const int* __restrict__ root, ClMask cm, int* __restrict__ cnt,
+                             int* __restrict__ mem)
 {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nV) return;
+    if (v >= nV || cm.skip(v)) return;
     const int r = root[v];
     const int slot = atomicAdd(cnt + r, 1);
     if (slot < CL_MAX) mem[CL_MAX * r + slot] = v;
 }
 
 // members of each cluster in ascending order
-__global__ void k_cl_sort(int nV, const int* __restrict__ root, const int* __restrict__ cnt, int* __restrict__ mem)
+__global__ void k_cl_sort(int nV, const int* __restrict__ root, const int* __restrict__ cnt, ClMask cm,
+                          int* __restrict__ mem)
 {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nV) return;
+    if (v >= nV || cm.skip(v)) return;
     const int r = root[v];
     const int k = min(cnt[r], CL_MAX);
     if (v == r) {
@@ -369,11 +387,11 @@ __global__ void k_cl_build(int nV, const int* __restrict__ root, const int* __re
                            const int* __restrict__ erp, const int* __restrict__ ecol, const double* __restrict__ evals,
                            long eld, const int* __restrict__ cdiag, const int* __restrict__ crp,
                            const int* __restrict__ ccol, const double* __restrict__ cvals, long cld,
-                           const int* __restrict__ vinst, double* __restrict__ Z, int4* __restrict__ cm4,
-                           double* __restrict__ work, int* __restrict__ err)
+                           const int* __restrict__ vinst, ClMask cmask, double* __restrict__ Z,
+                           int4* __restrict__ cm4, double* __restrict__ work, int* __restrict__ err)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nV || root[r] != r) return;
+    if (r >= nV || cmask.skip(r) || root[r] != r) return;
     const int k = min(cnt[r], CL_MAX);
     const int* m = mem + CL_MAX * r;
     const int4 list = make_int4(m[0], k > 1 ? m[1] : -1, k > 2 ? m[2] : -1, k > 3 ? m[3] : -1);
@@ -391,7 +409,7 @@ __global__ void k_cl_build(int nV, const int* __restrict__ root, const int* __re
         }
         double c0 = a[4] * a[8] - a[5] * a[7], c1 = a[5] * a[6] - a[3] * a[8], c2 = a[3] * a[7] - a[4] * a[6];
         double det = a[0] * c0 + a[1] * c1 + a[2] * c2;
-        double* o = Z + (size_t)CL_ZS * r;
+        double* o = Z + 9 * (size_t)r;  // plane 0
         if (!(det > 0)) {
             atomicOr(err + vinst[r], 1);
             det = 1.0;
@@ -467,19 +485,24 @@ __global__ void k_cl_build(int nV, const int* __restrict__ root, const int* __re
             }
         }
     }
-    // member a's rows, as k 3x3 blocks (one per member b, row-major)
-    for (int a = 0; a < k; ++a) {
-        double* o = Z + (size_t)CL_ZS * m[a];
-        for (int b = 0; b < k; ++b)
+    // member a's rows, as k 3x3 blocks: block (a, b) in plane b at row m[a]
+    for (int a = 0; a < k; ++a)
+        for (int b = 0; b < k; ++b) {
+            double* o = Z + 9 * ((size_t)b * nV + m[a]);
             for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j) o[9 * b + 3 * i + j] = Ainv[(3 * a + i) * n + 3 * b + j];
-    }
+                for (int j = 0; j < 3; ++j) o[3 * i + j] = Ainv[(3 * a + i) * n + 3 * b + j];
+        }
 }
 
-// z_v = Z_v r_members (members' r given by rfun)
+// z_v = Z_v r_members (members' r given by rfun).  Z holds CL_MAX planes of
+// 9 nV doubles: the 3x3 block of vertex v for its cluster's b-th member is
+// at plane b, row v, so singletons (most vertices) read plane 0 only and a
+// wavefront's reads of a plane are contiguous.
 template <class RF>
-__device__ inline void cl_apply(int4 cm, const double* __restrict__ o, const double* own, RF&& rfun, double* zv)
+__device__ inline void cl_apply(int4 cm, const double* __restrict__ Z, int nV, int v, const double* own, RF&& rfun,
+                                double* zv)
 {
+    const double* o = Z + 9 * (size_t)v;
     if (cm.y < 0) {  // singleton: the 3x3 inverse of the diagonal block
 #pragma unroll
         for (int i = 0; i < 3; ++i) zv[i] = o[3 * i] * own[0] + o[3 * i + 1] * own[1] + o[3 * i + 2] * own[2];
@@ -492,9 +515,9 @@ __device__ inline void cl_apply(int4 cm, const double* __restrict__ o, const dou
         if (mb[b] < 0) break;
         double rb[3];
         rfun(mb[b], rb);
+        const double* ob = o + 9 * (size_t)b * nV;
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
-            zv[i] += o[9 * b + 3 * i] * rb[0] + o[9 * b + 3 * i + 1] * rb[1] + o[9 * b + 3 * i + 2] * rb[2];
+        for (int i = 0; i < 3; ++i) zv[i] += ob[3 * i] * rb[0] + ob[3 * i + 1] * rb[1] + ob[3 * i + 2] * rb[2];
     }
 }
 
@@ -525,7 +548,7 @@ struct PcgArgs {
     const double* cval;  // 9 component planes of cld blocks
     long eld, cld;
     const double* b;
-    const double* Z;      // preconditioner rows: CL_ZS doubles per vertex (k 3x3 blocks)
+    const double* Z;      // preconditioner rows: CL_MAX planes of 9 nV doubles (cl_apply)
     const int4* cm4;      // per vertex: its cluster's members, ascending, -1 padded
     double* x;
     double* r;        // 2 x 3V: the update of iteration it reads half it&1, writes the other
@@ -547,6 +570,7 @@ struct PcgArgs {
     const int4 *atab, *btab;
     const int *aoff, *boff;  // per launched block: iterations its instance did in earlier calls
     int ident;               // one instance, all blocks launched: entries follow from blockIdx (no table load)
+    int nbl_upd;             // update blocks launched (PCG_UPD_WAVES per workgroup)
     const int *apbeg, *bpbeg;
     const unsigned char* solve;  // per instance: 0 not solving, 1 new solve, 2 continued from an earlier call
     int* flag;        // per instance: 0 running, 1 stopped (converged or maxit), 2 breakdown, 3 not solving
@@ -556,7 +580,8 @@ struct PcgArgs {
 };
 
 constexpr int PCG_ROW_THREADS = 256;         // row threads of a SpMV block (4 wavefronts), 256 / G rows
-constexpr int PCG_UPD_BLOCK = 64;            // one wavefront: reductions need no barrier
+constexpr int PCG_UPD_BLOCK = 64;            // rows of an update block: one wavefront, reductions need no barrier
+constexpr int PCG_UPD_WAVES = 4;             // update blocks per workgroup (independent wavefronts)
 // SpMV launch shapes: G lanes per block row, scalars in wavefront 0 (before
 // its rows) or in a fifth wavefront of their own
 struct SpmvShape {
@@ -570,17 +595,26 @@ inline int spmv_rows(int g) { return PCG_ROW_THREADS / g; }
 // Launch-table entry of this block: {first row, end row, instance, block id}.
 // A one-instance launch derives it from blockIdx, saving a dependent load.
 template <int ROWS>
-__device__ inline int4 launch_entry(const PcgArgs& A, const int4* __restrict__ tab)
+__device__ inline int4 launch_entry(const PcgArgs& A, const int4* __restrict__ tab, int blk)
 {
     if (A.ident) {
-        const int lo = blockIdx.x * ROWS;
-        return make_int4(lo, min(lo + ROWS, A.nV), 0, (int)blockIdx.x);
+        const int lo = blk * ROWS;
+        return make_int4(lo, min(lo + ROWS, A.nV), 0, blk);
     }
-    return tab[blockIdx.x];
+    return tab[blk];
+}
+template <int ROWS>
+__device__ inline int4 launch_entry(const PcgArgs& A, const int4* __restrict__ tab)
+{
+    return launch_entry<ROWS>(A, tab, (int)blockIdx.x);
+}
+__device__ inline int launch_off(const PcgArgs& A, const int* __restrict__ off, int blk)
+{
+    return A.ident ? A.ident - 1 : off[blk];
 }
 __device__ inline int launch_off(const PcgArgs& A, const int* __restrict__ off)
 {
-    return A.ident ? A.ident - 1 : off[blockIdx.x];
+    return launch_off(A, off, (int)blockIdx.x);
 }
 
 // Sum over a wavefront by a fixed xor butterfly (every lane gets the total).
@@ -699,6 +733,61 @@ __device__ inline void row_spmv(int lane, int e0, int ne, int c0, int nc, const 
     }
 }
 
+// row_spmv with the slots of a lane taken two at a time: the blocks, columns
+// and gathered vectors of slots j and j + G are loaded together, then added
+// in slot order (j first), so the sums are those of row_spmv while a row of
+// up to 2G blocks costs two dependent round trips instead of four.  The
+// second slot's loads use the first slot's (valid) index when it is absent.
+template <int G>
+__device__ inline void row_spmv2(int lane, int e0, int ne, int c0, int nc, const PcgArgs& A,
+                                 const double* __restrict__ z, const double* __restrict__ po, double* y, double* w)
+{
+    const int n = ne + nc;
+    for (int j = lane; j < n; j += 2 * G) {
+        const int j2 = j + G;
+        const bool has2 = j2 < n;
+        const int ja = j, jb = has2 ? j2 : j;
+        const bool ela = ja < ne, elb = jb < ne;
+        const double* __restrict__ va = ela ? A.eval : A.cval;
+        const double* __restrict__ vb = elb ? A.eval : A.cval;
+        const long lda = ela ? A.eld : A.cld, ldb = elb ? A.eld : A.cld;
+        const int ka = ela ? e0 + ja : c0 + (ja - ne);
+        const int kb = elb ? e0 + jb : c0 + (jb - ne);
+        const int cola = ela ? A.ecol[ka] : A.ccol[ka];
+        const int colb = elb ? A.ecol[kb] : A.ccol[kb];
+        const double ba[9] = {va[ka], va[lda + ka], va[2 * lda + ka], va[3 * lda + ka], va[4 * lda + ka],
+                              va[5 * lda + ka], va[6 * lda + ka], va[7 * lda + ka], va[8 * lda + ka]};
+        const double bb[9] = {vb[kb], vb[ldb + kb], vb[2 * ldb + kb], vb[3 * ldb + kb], vb[4 * ldb + kb],
+                              vb[5 * ldb + kb], vb[6 * ldb + kb], vb[7 * ldb + kb], vb[8 * ldb + kb]};
+        const size_t ca = 3 * (size_t)cola, cb = 3 * (size_t)colb;
+        const double xa0 = z[ca], xa1 = z[ca + 1], xa2 = z[ca + 2];
+        const double xb0 = z[cb], xb1 = z[cb + 1], xb2 = z[cb + 2];
+        double pa0 = 0.0, pa1 = 0.0, pa2 = 0.0, pb0 = 0.0, pb1 = 0.0, pb2 = 0.0;
+        if (po) {
+            pa0 = po[ca]; pa1 = po[ca + 1]; pa2 = po[ca + 2];
+            pb0 = po[cb]; pb1 = po[cb + 1]; pb2 = po[cb + 2];
+        }
+        y[0] += ba[0] * xa0 + ba[1] * xa1 + ba[2] * xa2;
+        y[1] += ba[3] * xa0 + ba[4] * xa1 + ba[5] * xa2;
+        y[2] += ba[6] * xa0 + ba[7] * xa1 + ba[8] * xa2;
+        if (po) {
+            w[0] += ba[0] * pa0 + ba[1] * pa1 + ba[2] * pa2;
+            w[1] += ba[3] * pa0 + ba[4] * pa1 + ba[5] * pa2;
+            w[2] += ba[6] * pa0 + ba[7] * pa1 + ba[8] * pa2;
+        }
+        if (has2) {
+            y[0] += bb[0] * xb0 + bb[1] * xb1 + bb[2] * xb2;
+            y[1] += bb[3] * xb0 + bb[4] * xb1 + bb[5] * xb2;
+            y[2] += bb[6] * xb0 + bb[7] * xb1 + bb[8] * xb2;
+            if (po) {
+                w[0] += bb[0] * pb0 + bb[1] * pb1 + bb[2] * pb2;
+                w[1] += bb[3] * pb0 + bb[4] * pb1 + bb[5] * pb2;
+                w[2] += bb[6] * pb0 + bb[7] * pb1 + bb[8] * pb2;
+            }
+        }
+    }
+}
+
 // row extents of row v in both block sets
 __device__ inline void row_extents(const PcgArgs& A, int v, int& e0, int& ne, int& c0, int& nc)
 {
@@ -763,7 +852,7 @@ __global__ void __launch_bounds__(PCG_UPD_BLOCK) k_pcg_init(PcgArgs A)
     if (v < t.y) {
         double rv[3], zv[3];
         res(v, rv);
-        cl_apply(A.cm4[v], A.Z + (size_t)CL_ZS * v, rv, res, zv);
+        cl_apply(A.cm4[v], A.Z, A.nV, v, rv, res, zv);
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             const double bi = A.b[3 * v + i];
@@ -836,7 +925,7 @@ __global__ void __launch_bounds__(PCG_ROW_THREADS) k_pcg_warm(PcgArgs A)
 // wavefronts stream their blocks; p_it = z + beta p_(it-1) is never read:
 // the rows accumulate A z and A p_(it-1) separately and combine them once
 // beta is known, q = A z + beta A p_(it-1).
-template <int G, bool SW>
+template <int G, bool SW, bool P2 = false>
 __global__ void __launch_bounds__(PCG_ROW_THREADS + (SW ? 64 : 0)) k_pcg_spmv(PcgArgs A, int it_call)
 {
     __shared__ double sbeta;
@@ -891,7 +980,10 @@ __global__ void __launch_bounds__(PCG_ROW_THREADS + (SW ? 64 : 0)) k_pcg_spmv(Pc
             poo = po ? po[3 * v + lane] : 0.0;
         }
         row = f == 0;
-        if (row) row_spmv<G>(lane, e0, ne, c0, nc, A, A.z, po, y, w);
+        if (row) {
+            if (P2) row_spmv2<G>(lane, e0, ne, c0, nc, A, A.z, po, y, w);
+            else row_spmv<G>(lane, e0, ne, c0, nc, A, A.z, po, y, w);
+        }
     }
     if (SW && scalar_wave) scalars();
     __syncthreads();
@@ -919,16 +1011,21 @@ __global__ void __launch_bounds__(PCG_ROW_THREADS + (SW ? 64 : 0)) k_pcg_spmv(Pc
 // r); partials of the new r.z and r.r.  Every load that does not depend on
 // alpha, the cluster's member list and the preconditioner rows included, is
 // issued before the partials are reduced.
-__global__ void __launch_bounds__(PCG_UPD_BLOCK) k_pcg_update(PcgArgs A, int it_call)
+__global__ void __launch_bounds__(PCG_UPD_BLOCK * PCG_UPD_WAVES) k_pcg_update(PcgArgs A, int it_call)
 {
-    const int it = it_call + launch_off(A, A.boff);  // the instance's own iteration
-    const int4 t = launch_entry<PCG_UPD_BLOCK>(A, A.btab);
+    // each wavefront is one update block of 64 rows, independent of the
+    // others in its workgroup
+    const int blk = blockIdx.x * PCG_UPD_WAVES + (int)(threadIdx.x >> 6);
+    if (blk >= A.nbl_upd) return;  // uniform over the wavefront
+    const int lane = threadIdx.x & 63;
+    const int it = it_call + launch_off(A, A.boff, blk);  // the instance's own iteration
+    const int4 t = launch_entry<PCG_UPD_BLOCK>(A, A.btab, blk);
     const int inst = t.z, bid = t.w;
     const int par = it & 1;
     const double* p = A.pbuf + (size_t)par * 3 * A.nV;
     const double* r_in = A.r + (size_t)par * 3 * A.nV;  // not written by this kernel: members read it safely
     double* r_out = A.r + (size_t)(par ^ 1) * 3 * A.nV;
-    const int v = t.x + threadIdx.x;
+    const int v = t.x + lane;
     const bool own = v < t.y;
     const int vc = own ? v : t.x;
     const int f = *(volatile int*)(A.flag + inst);
@@ -941,16 +1038,18 @@ __global__ void __launch_bounds__(PCG_UPD_BLOCK) k_pcg_update(PcgArgs A, int it_
         qv[i] = A.q[3 * vc + i];
     }
     const int4 cm = A.cm4[vc];
-    const double* o = A.Z + (size_t)CL_ZS * vc;
+    const double* o = A.Z + 9 * (size_t)vc;  // plane 0
 #pragma unroll
     for (int i = 0; i < 9; ++i) o0[i] = o[i];
-    if (f != 0) return;  // uniform over the wavefront
+    // the partials are read in the same round trip as the flag and the rows
+    // (a stopped instance's sums are formed and dropped)
     const int pb = A.bpbeg[inst];
     double pq, rz;
-    wave_sum_partials2<16, 8>(A.part_pq + A.apbeg[inst], A.apbeg[inst + 1] - A.apbeg[inst],
+    wave_sum_partials2<8, 4>(A.part_pq + A.apbeg[inst], A.apbeg[inst + 1] - A.apbeg[inst],
                               A.part_rz + par * A.nbB + pb, A.bpbeg[inst + 1] - pb, pq, rz);
+    if (f != 0) return;  // uniform over the wavefront
     if (!(pq > 0.0)) {
-        if (bid == pb && threadIdx.x == 0) {
+        if (bid == pb && lane == 0) {
             A.flag[inst] = 2;
             A.scal[3 * inst + 2] = (double)it;
         }
@@ -971,7 +1070,7 @@ __global__ void __launch_bounds__(PCG_UPD_BLOCK) k_pcg_update(PcgArgs A, int it_
             for (int i = 0; i < 3; ++i) zv[i] = o0[3 * i] * rv[0] + o0[3 * i + 1] * rv[1] + o0[3 * i + 2] * rv[2];
         } else {
             // z = M r with the members' updated residuals (the own one as above)
-            cl_apply(cm, o, rv,
+            cl_apply(cm, A.Z, A.nV, v, rv,
                      [&](int u, double* out) {
                          if (u == v) {
                              out[0] = rv[0];
@@ -993,7 +1092,7 @@ __global__ void __launch_bounds__(PCG_UPD_BLOCK) k_pcg_update(PcgArgs A, int it_
     }
     drz = wave_sum(drz);
     drr = wave_sum(drr);
-    if (threadIdx.x == 0) {
+    if (lane == 0) {
         A.part_rz[(par ^ 1) * A.nbB + bid] = drz;
         A.part_rr[(par ^ 1) * A.nbB + bid] = drr;
     }
@@ -1114,24 +1213,27 @@ void contact_structure(LinWork& L, hipStream_t s, int nV, long ncs, const int* c
 }
 
 void contact_gradient_gather(LinWork& L, hipStream_t s, const MeshDev& m, const double* X, const double* Xt,
-                             const double* gl, const double* gb, double* g, double* rhs)
+                             const double* gl, const double* gb, double* g, double* rhs, const unsigned char* act)
 {
     bool haveC = L.ncb > 0 && gb;
     IDP_LAUNCH(k_gradient_gather, grid_for(m.nV, 256), 256, 0, s, m.nV, m.e_diag.p, m.e_src_ptr.p, m.e_src.p, gl, X,
-               Xt, m.mass.p, haveC ? L.cdiag.p : (const int*)nullptr, L.seg.p, L.rv1.p, gb, m.proj_mask(), g, rhs);
+               Xt, m.mass.p, haveC ? L.cdiag.p : (const int*)nullptr, L.seg.p, L.rv1.p, gb, m.proj_mask(),
+               m.inst.vinst.p, act, g, rhs);
 }
 
-void assemble(LinWork& L, hipStream_t s, const MeshDev& m, const double* Hl, const double* Hb, bool any_dbc)
+void assemble(LinWork& L, hipStream_t s, const MeshDev& m, const double* Hl, const double* Hb, bool any_dbc,
+              const unsigned char* act)
 {
     // algorithmic bytes: 80 per contribution (9 doubles + key), 76 per block, rowptr
     const double ncontrib = 16.0 * m.nT + (double)L.nrec + m.nV;
     const int tok = L.prof ? L.prof->begin(IDP_K_ASSEMBLY) : -1;
     double* ev = L.evals.ensure(9 * (size_t)m.nnzb_e, s);
     IDP_LAUNCH(k_elastic_values, grid_for(m.nnzb_e, 256), 256, 0, s, m.nnzb_e, L.erow.p, m.e_col.p, m.e_src_ptr.p,
-               m.e_src.p, Hl, m.mass.p, ev);
+               m.e_src.p, Hl, m.mass.p, m.inst.vinst.p, act, ev);
     if (L.ncb) {
         double* cv = L.cvals.ensure(9 * (size_t)L.ncb, s);
-        IDP_LAUNCH(k_contact_values, grid_for(L.ncb, 256), 256, 0, s, (unsigned)L.ncb, L.seg.p, L.rv1.p, Hb, cv);
+        IDP_LAUNCH(k_contact_values, grid_for(L.ncb, 256), 256, 0, s, (unsigned)L.ncb, L.seg.p, L.rv1.p, Hb, L.crow.p,
+                   m.inst.vinst.p, act, cv);
     }
     if (any_dbc) {
         IDP_LAUNCH(k_project_dbc, grid_for(m.nnzb_e, 256), 256, 0, s, m.nnzb_e, L.erow.p, m.e_col.p, m.proj_mask(), 1,
@@ -1235,17 +1337,19 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
     int4* cm4 = (int4*)L.cl_cm4.ensure(4 * (size_t)nV, s);
     const int* crp = haveC ? L.crp.p : nullptr;
     const int* crp_match = L.precond == 0 ? crp : nullptr;  // no contact rows: singletons (3x3 block Jacobi)
-    IDP_LAUNCH(k_cl_best, vg, 256, 0, s, nV, crp_match, L.ccol.p, L.cvals.p, L.ncb, best);
-    IDP_LAUNCH(k_cl_pair, vg, 256, 0, s, nV, best, root, size);
+    // only the instances that solve (new or continued) build their clusters
+    const ClMask cmk{I.vinst.p, solve};
+    IDP_LAUNCH(k_cl_best, vg, 256, 0, s, nV, crp_match, L.ccol.p, L.cvals.p, L.ncb, cmk, best);
+    IDP_LAUNCH(k_cl_pair, vg, 256, 0, s, nV, best, cmk, root, size);
     IDP_HIP(hipMemsetAsync(pack, 0, nV * sizeof(unsigned long long), s));
-    IDP_LAUNCH(k_cl_best2, vg, 256, 0, s, nV, crp_match, L.ccol.p, L.cvals.p, L.ncb, root, size, pack);
-    IDP_LAUNCH(k_cl_merge, vg, 256, 0, s, nV, root, pack, size, root2);
+    IDP_LAUNCH(k_cl_best2, vg, 256, 0, s, nV, crp_match, L.ccol.p, L.cvals.p, L.ncb, root, size, cmk, pack);
+    IDP_LAUNCH(k_cl_merge, vg, 256, 0, s, nV, root, pack, size, cmk, root2);
     IDP_HIP(hipMemsetAsync(cnt, 0, nV * sizeof(int), s));
-    IDP_LAUNCH(k_cl_members, vg, 256, 0, s, nV, root2, cnt, mem);
-    IDP_LAUNCH(k_cl_sort, vg, 256, 0, s, nV, root2, cnt, mem);
+    IDP_LAUNCH(k_cl_members, vg, 256, 0, s, nV, root2, cmk, cnt, mem);
+    IDP_LAUNCH(k_cl_sort, vg, 256, 0, s, nV, root2, cnt, cmk, mem);
     double* work = L.cl_work.ensure(2 * 144 * (size_t)nV, s);
     IDP_LAUNCH(k_cl_build, vg, 256, 0, s, nV, root2, cnt, mem, m.e_diag.p, m.e_rowptr.p, m.e_col.p, L.evals.p,
-               m.nnzb_e, haveC ? L.cdiag.p : (const int*)nullptr, crp, L.ccol.p, L.cvals.p, L.ncb, I.vinst.p, Z,
+               m.nnzb_e, haveC ? L.cdiag.p : (const int*)nullptr, crp, L.ccol.p, L.cvals.p, L.ncb, I.vinst.p, cmk, Z,
                cm4, work, err);
     PcgArgs A;
     A.nV = nV;
@@ -1295,6 +1399,7 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
         fill(I.pcg_upd, L.h_btab, L.h_boff);
         nA = (int)L.h_atab.size();
         nB = (int)L.h_btab.size();
+        A.nbl_upd = nB;
         long rows = 0;
         for (const int4& t : L.h_btab) rows += t.y - t.x;
         frac = nV ? (double)rows / nV : 0.0;
@@ -1330,7 +1435,10 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
         case 2: IDP_LAUNCH_TIMED((k_pcg_spmv<16, false>), nA, nt, 0, s, e0, e1, A, it); break;
         case 3: IDP_LAUNCH_TIMED((k_pcg_spmv<32, false>), nA, nt, 0, s, e0, e1, A, it); break;
         case 4: IDP_LAUNCH_TIMED((k_pcg_spmv<4, true>), nA, nt, 0, s, e0, e1, A, it); break;
-        case 5: IDP_LAUNCH_TIMED((k_pcg_spmv<8, true>), nA, nt, 0, s, e0, e1, A, it); break;
+        case 5:
+            if (L.spmv_pipe2) IDP_LAUNCH_TIMED((k_pcg_spmv<8, true, true>), nA, nt, 0, s, e0, e1, A, it);
+            else IDP_LAUNCH_TIMED((k_pcg_spmv<8, true>), nA, nt, 0, s, e0, e1, A, it);
+            break;
         case 6: IDP_LAUNCH_TIMED((k_pcg_spmv<16, true>), nA, nt, 0, s, e0, e1, A, it); break;
         default: IDP_LAUNCH_TIMED((k_pcg_spmv<32, true>), nA, nt, 0, s, e0, e1, A, it); break;
         }
@@ -1359,7 +1467,8 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
             spmv(it, e0, e1);
             if (sample) P->end_pair(IDP_K_PCG_SPMV, t, spmv_bytes * frac, it);
             t = sample ? P->pair(e0, e1) : -1;
-            IDP_LAUNCH_TIMED(k_pcg_update, nB, PCG_UPD_BLOCK, 0, s, e0, e1, A, it);
+            IDP_LAUNCH_TIMED(k_pcg_update, (nB + PCG_UPD_WAVES - 1) / PCG_UPD_WAVES, PCG_UPD_BLOCK * PCG_UPD_WAVES, 0,
+                             s, e0, e1, A, it);
             if (sample) P->end_pair(IDP_K_PCG_UPDATE, t, upd_bytes * frac, it);
         }
         IDP_HIP(hipMemcpyAsync(hf.data(), err, 2 * B * sizeof(int), hipMemcpyDeviceToHost, s));

@@ -14,6 +14,7 @@ struct LinWork {
     DBuf<unsigned long long> key0, key1;
     long ncb = 0, nrec = 0, ncs = 0;
     int precond = 0;  // 0: contact-cluster block Jacobi, 1: 3x3 block Jacobi
+    bool spmv_pipe2 = false;  // SpMV row walk two slots per lane at a time
     int spmv_shape = 5;  // PCG SpMV launch shape: 8 lanes per row, scalars in a wavefront of their own
     DBuf<int4> atab, btab;  // PCG launch tables
     std::vector<int4> h_atab, h_btab;
@@ -34,9 +35,12 @@ struct LinWork {
 };
 
 void contact_structure(LinWork& L, hipStream_t s, int nV, long ncs, const int* cs);
+// act (device, one byte per instance; null = all): the instances whose rows are formed
 void contact_gradient_gather(LinWork& L, hipStream_t s, const MeshDev& m, const double* X, const double* Xt,
-                             const double* gl, const double* gb, double* g, double* rhs);
-void assemble(LinWork& L, hipStream_t s, const MeshDev& m, const double* Hl, const double* Hb, bool any_dbc);
+                             const double* gl, const double* gb, double* g, double* rhs,
+                             const unsigned char* act = nullptr);
+void assemble(LinWork& L, hipStream_t s, const MeshDev& m, const double* Hl, const double* Hb, bool any_dbc,
+              const unsigned char* act = nullptr);
 void init_erow(LinWork& L, hipStream_t s, const MeshDev& m, const std::vector<int>& rowof);
 // solve_h (host, B; null = all new): 0 not solving, 1 new solve, 2 continue
 // a solve handed over by the previous call.  warm (device, B bytes; null =
