@@ -1087,7 +1087,6 @@ int idp_ctx_create(int device, idp_ctx** out)
             const int k = std::atoi(e);
             if (k >= 0 && k < 8) c->L.spmv_shape = k;
         }
-        if (const char* e = std::getenv("IDP_SPMV_PIPE")) c->L.spmv_pipe2 = std::atoi(e) > 0;
     } catch (const HipError& e) {
         delete c->eng;
         delete c;

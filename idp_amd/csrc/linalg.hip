@@ -733,61 +733,6 @@ __device__ inline void row_spmv(int lane, int e0, int ne, int c0, int nc, const 
     }
 }
 
-// row_spmv with the slots of a lane taken two at a time: the blocks, columns
-// and gathered vectors of slots j and j + G are loaded together, then added
-// in slot order (j first), so the sums are those of row_spmv while a row of
-// up to 2G blocks costs two dependent round trips instead of four.  The
-// second slot's loads use the first slot's (valid) index when it is absent.
-template <int G>
-__device__ inline void row_spmv2(int lane, int e0, int ne, int c0, int nc, const PcgArgs& A,
-                                 const double* __restrict__ z, const double* __restrict__ po, double* y, double* w)
-{
-    const int n = ne + nc;
-    for (int j = lane; j < n; j += 2 * G) {
-        const int j2 = j + G;
-        const bool has2 = j2 < n;
-        const int ja = j, jb = has2 ? j2 : j;
-        const bool ela = ja < ne, elb = jb < ne;
-        const double* __restrict__ va = ela ? A.eval : A.cval;
-        const double* __restrict__ vb = elb ? A.eval : A.cval;
-        const long lda = ela ? A.eld : A.cld, ldb = elb ? A.eld : A.cld;
-        const int ka = ela ? e0 + ja : c0 + (ja - ne);
-        const int kb = elb ? e0 + jb : c0 + (jb - ne);
-        const int cola = ela ? A.ecol[ka] : A.ccol[ka];
-        const int colb = elb ? A.ecol[kb] : A.ccol[kb];
-        const double ba[9] = {va[ka], va[lda + ka], va[2 * lda + ka], va[3 * lda + ka], va[4 * lda + ka],
-                              va[5 * lda + ka], va[6 * lda + ka], va[7 * lda + ka], va[8 * lda + ka]};
-        const double bb[9] = {vb[kb], vb[ldb + kb], vb[2 * ldb + kb], vb[3 * ldb + kb], vb[4 * ldb + kb],
-                              vb[5 * ldb + kb], vb[6 * ldb + kb], vb[7 * ldb + kb], vb[8 * ldb + kb]};
-        const size_t ca = 3 * (size_t)cola, cb = 3 * (size_t)colb;
-        const double xa0 = z[ca], xa1 = z[ca + 1], xa2 = z[ca + 2];
-        const double xb0 = z[cb], xb1 = z[cb + 1], xb2 = z[cb + 2];
-        double pa0 = 0.0, pa1 = 0.0, pa2 = 0.0, pb0 = 0.0, pb1 = 0.0, pb2 = 0.0;
-        if (po) {
-            pa0 = po[ca]; pa1 = po[ca + 1]; pa2 = po[ca + 2];
-            pb0 = po[cb]; pb1 = po[cb + 1]; pb2 = po[cb + 2];
-        }
-        y[0] += ba[0] * xa0 + ba[1] * xa1 + ba[2] * xa2;
-        y[1] += ba[3] * xa0 + ba[4] * xa1 + ba[5] * xa2;
-        y[2] += ba[6] * xa0 + ba[7] * xa1 + ba[8] * xa2;
-        if (po) {
-            w[0] += ba[0] * pa0 + ba[1] * pa1 + ba[2] * pa2;
-            w[1] += ba[3] * pa0 + ba[4] * pa1 + ba[5] * pa2;
-            w[2] += ba[6] * pa0 + ba[7] * pa1 + ba[8] * pa2;
-        }
-        if (has2) {
-            y[0] += bb[0] * xb0 + bb[1] * xb1 + bb[2] * xb2;
-            y[1] += bb[3] * xb0 + bb[4] * xb1 + bb[5] * xb2;
-            y[2] += bb[6] * xb0 + bb[7] * xb1 + bb[8] * xb2;
-            if (po) {
-                w[0] += bb[0] * pb0 + bb[1] * pb1 + bb[2] * pb2;
-                w[1] += bb[3] * pb0 + bb[4] * pb1 + bb[5] * pb2;
-                w[2] += bb[6] * pb0 + bb[7] * pb1 + bb[8] * pb2;
-            }
-        }
-    }
-}
-
 // row extents of row v in both block sets
 __device__ inline void row_extents(const PcgArgs& A, int v, int& e0, int& ne, int& c0, int& nc)
 {
@@ -925,7 +870,7 @@ __global__ void __launch_bounds__(PCG_ROW_THREADS) k_pcg_warm(PcgArgs A)
 // wavefronts stream their blocks; p_it = z + beta p_(it-1) is never read:
 // the rows accumulate A z and A p_(it-1) separately and combine them once
 // beta is known, q = A z + beta A p_(it-1).
-template <int G, bool SW, bool P2 = false>
+template <int G, bool SW>
 __global__ void __launch_bounds__(PCG_ROW_THREADS + (SW ? 64 : 0)) k_pcg_spmv(PcgArgs A, int it_call)
 {
     __shared__ double sbeta;
@@ -980,10 +925,7 @@ __global__ void __launch_bounds__(PCG_ROW_THREADS + (SW ? 64 : 0)) k_pcg_spmv(Pc
             poo = po ? po[3 * v + lane] : 0.0;
         }
         row = f == 0;
-        if (row) {
-            if (P2) row_spmv2<G>(lane, e0, ne, c0, nc, A, A.z, po, y, w);
-            else row_spmv<G>(lane, e0, ne, c0, nc, A, A.z, po, y, w);
-        }
+        if (row) row_spmv<G>(lane, e0, ne, c0, nc, A, A.z, po, y, w);
     }
     if (SW && scalar_wave) scalars();
     __syncthreads();
@@ -1435,10 +1377,7 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
         case 2: IDP_LAUNCH_TIMED((k_pcg_spmv<16, false>), nA, nt, 0, s, e0, e1, A, it); break;
         case 3: IDP_LAUNCH_TIMED((k_pcg_spmv<32, false>), nA, nt, 0, s, e0, e1, A, it); break;
         case 4: IDP_LAUNCH_TIMED((k_pcg_spmv<4, true>), nA, nt, 0, s, e0, e1, A, it); break;
-        case 5:
-            if (L.spmv_pipe2) IDP_LAUNCH_TIMED((k_pcg_spmv<8, true, true>), nA, nt, 0, s, e0, e1, A, it);
-            else IDP_LAUNCH_TIMED((k_pcg_spmv<8, true>), nA, nt, 0, s, e0, e1, A, it);
-            break;
+        case 5: IDP_LAUNCH_TIMED((k_pcg_spmv<8, true>), nA, nt, 0, s, e0, e1, A, it); break;
         case 6: IDP_LAUNCH_TIMED((k_pcg_spmv<16, true>), nA, nt, 0, s, e0, e1, A, it); break;
         default: IDP_LAUNCH_TIMED((k_pcg_spmv<32, true>), nA, nt, 0, s, e0, e1, A, it); break;
         }

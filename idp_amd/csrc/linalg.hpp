@@ -14,7 +14,6 @@ struct LinWork {
     DBuf<unsigned long long> key0, key1;
     long ncb = 0, nrec = 0, ncs = 0;
     int precond = 0;  // 0: contact-cluster block Jacobi, 1: 3x3 block Jacobi
-    bool spmv_pipe2 = false;  // SpMV row walk two slots per lane at a time
     int spmv_shape = 5;  // PCG SpMV launch shape: 8 lanes per row, scalars in a wavefront of their own
     DBuf<int4> atab, btab;  // PCG launch tables
     std::vector<int4> h_atab, h_btab;
