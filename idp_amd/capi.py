@@ -41,7 +41,8 @@ EXPORTED = [
     "idp_barrier_energy", "idp_barrier_gradient", "idp_barrier_hessian_local", "idp_min_dist2", "idp_ccd_stepsize",
     "idp_elastic_energy", "idp_elastic_gradient", "idp_elastic_hessian_local", "idp_newton_direction", "idp_get_bsr",
     "idp_profile_enable", "idp_profile_read", "idp_set_mesh_batch", "idp_batch_size", "idp_advance_batch",
-    "idp_serve_configure", "idp_serve_start", "idp_serve_iterate", "idp_serve_result",
+    "idp_serve_configure", "idp_serve_start", "idp_serve_iterate", "idp_serve_result", "idp_set_launch_config",
+    "idp_get_launch_config",
 ]
 
 # PCG preconditioner for every solve of this process: 0 contact-cluster block
@@ -54,6 +55,36 @@ PCG_COLD = int(os.environ.get("IDP_PCG_COLD", "0"))
 ACCD_BUDGET = int(os.environ.get("IDP_ACCD_BUDGET", "0"))
 # batched engine: PCG hand-over (0 adaptive default, < 0 never; IDP_PCG_HANDOVER)
 PCG_HANDOVER = int(os.environ.get("IDP_PCG_HANDOVER", "0"))
+
+# PCG launch configuration written by the autotuner (idp_amd/autotune.py):
+# IDP_TUNING names the file, else tuning.json beside this module; absent or
+# unreadable means the library defaults.  One configuration serves every
+# context of the process, so an instance's result never depends on its mix.
+TUNING_PATH = os.environ.get("IDP_TUNING", os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json"))
+_tuned = None
+
+
+def tuned_config(path=None):
+    """(spmv_shape, upd_waves) from a tuning file, or None."""
+    global _tuned
+    import json
+
+    if path is None and _tuned is not None:
+        return _tuned or None
+    p = path or TUNING_PATH
+    cfg = ()
+    try:
+        with open(p) as f:
+            d = json.load(f)
+        sh, uw = int(d["spmv_shape"]), int(d["upd_waves"])
+        if 0 <= sh <= 7 and 1 <= uw <= 8:
+            cfg = (sh, uw)
+    except (OSError, ValueError, KeyError, TypeError):
+        cfg = ()
+    if path is None:
+        _tuned = cfg
+    return cfg or None
+
 
 K_FAMILIES = ["pcg", "barrier_hess", "tet_hess", "assembly", "constraint_set", "ccd", "energy", "gradient",
               "pcg_spmv", "pcg_update"]
@@ -155,6 +186,8 @@ def lib():
         L.idp_serve_iterate.argtypes = [vp, ip, ip]
         L.idp_serve_result.argtypes = [vp, C.c_int, dp, dp, dp, C.POINTER(StepStats)]
         L.idp_profile_enable.argtypes = [vp, C.c_int]
+        L.idp_set_launch_config.argtypes = [vp, C.c_int, C.c_int]
+        L.idp_get_launch_config.argtypes = [vp, ip, ip]
         L.idp_profile_read.argtypes = [vp, C.POINTER(Profile)]
         _lib = L
     return _lib
@@ -235,6 +268,18 @@ class Context:
         if rc != IDP_OK:
             raise IdpError(rc)
         self.nV = 0
+        cfg = tuned_config()
+        if cfg and "IDP_SPMV_SHAPE" not in os.environ:  # an explicit shape (tuning runs) wins
+            self.set_launch_config(*cfg)
+
+    def set_launch_config(self, spmv_shape, upd_waves):
+        """PCG launch configuration (see idp_set_launch_config)."""
+        self._check(lib().idp_set_launch_config(self._h, int(spmv_shape), int(upd_waves)))
+
+    def launch_config(self):
+        sh, uw = C.c_int(), C.c_int()
+        self._check(lib().idp_get_launch_config(self._h, C.byref(sh), C.byref(uw)))
+        return sh.value, uw.value
 
     def close(self):
         if self._h:

@@ -1111,6 +1111,26 @@ int idp_ctx_destroy(idp_ctx* c)
 
 const char* idp_last_error(const idp_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
+int idp_set_launch_config(idp_ctx* c, int spmv_shape, int upd_waves)
+{
+    return guarded(c, [&]() -> int {
+        if (spmv_shape < 0 || spmv_shape > 7 || upd_waves < 1 || upd_waves > 8) return IDP_E_INVALID;
+        c->L.spmv_shape = spmv_shape;
+        c->L.upd_waves = upd_waves;
+        return IDP_OK;
+    });
+}
+
+int idp_get_launch_config(idp_ctx* c, int* spmv_shape, int* upd_waves)
+{
+    return guarded(c, [&]() -> int {
+        if (!spmv_shape || !upd_waves) return IDP_E_INVALID;
+        *spmv_shape = c->L.spmv_shape;
+        *upd_waves = c->L.upd_waves;
+        return IDP_OK;
+    });
+}
+
 int idp_profile_enable(idp_ctx* c, int on)
 {
     return guarded(c, [&]() -> int {

@@ -581,7 +581,7 @@ struct PcgArgs {
 
 constexpr int PCG_ROW_THREADS = 256;         // row threads of a SpMV block (4 wavefronts), 256 / G rows
 constexpr int PCG_UPD_BLOCK = 64;            // rows of an update block: one wavefront, reductions need no barrier
-constexpr int PCG_UPD_WAVES = 4;             // update blocks per workgroup (independent wavefronts)
+constexpr int PCG_UPD_WAVES_MAX = 8;         // update blocks per workgroup (independent wavefronts), at most
 // SpMV launch shapes: G lanes per block row, scalars in wavefront 0 (before
 // its rows) or in a fifth wavefront of their own
 struct SpmvShape {
@@ -953,11 +953,11 @@ __global__ void __launch_bounds__(PCG_ROW_THREADS + (SW ? 64 : 0)) k_pcg_spmv(Pc
 // r); partials of the new r.z and r.r.  Every load that does not depend on
 // alpha, the cluster's member list and the preconditioner rows included, is
 // issued before the partials are reduced.
-__global__ void __launch_bounds__(PCG_UPD_BLOCK * PCG_UPD_WAVES) k_pcg_update(PcgArgs A, int it_call)
+__global__ void __launch_bounds__(PCG_UPD_BLOCK * PCG_UPD_WAVES_MAX) k_pcg_update(PcgArgs A, int it_call)
 {
     // each wavefront is one update block of 64 rows, independent of the
     // others in its workgroup
-    const int blk = blockIdx.x * PCG_UPD_WAVES + (int)(threadIdx.x >> 6);
+    const int blk = blockIdx.x * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
     if (blk >= A.nbl_upd) return;  // uniform over the wavefront
     const int lane = threadIdx.x & 63;
     const int it = it_call + launch_off(A, A.boff, blk);  // the instance's own iteration
@@ -1312,6 +1312,7 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
     A.pbuf = L.p.ensure(6 * (size_t)nV, s);
     A.q = L.q.ensure(3 * (size_t)nV, s);
     const SpmvShape shape = SPMV_SHAPES[L.spmv_shape];
+    const int uw = std::min(std::max(L.upd_waves, 1), PCG_UPD_WAVES_MAX);
     const int gi = shape.g == 4 ? 0 : shape.g == 8 ? 1 : shape.g == 16 ? 2 : 3;
     const SegDev& rows = I.pcg_rows[gi];
     A.nbA = rows.nblk;
@@ -1406,8 +1407,7 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
             spmv(it, e0, e1);
             if (sample) P->end_pair(IDP_K_PCG_SPMV, t, spmv_bytes * frac, it);
             t = sample ? P->pair(e0, e1) : -1;
-            IDP_LAUNCH_TIMED(k_pcg_update, (nB + PCG_UPD_WAVES - 1) / PCG_UPD_WAVES, PCG_UPD_BLOCK * PCG_UPD_WAVES, 0,
-                             s, e0, e1, A, it);
+            IDP_LAUNCH_TIMED(k_pcg_update, (nB + uw - 1) / uw, PCG_UPD_BLOCK * uw, 0, s, e0, e1, A, it);
             if (sample) P->end_pair(IDP_K_PCG_UPDATE, t, upd_bytes * frac, it);
         }
         IDP_HIP(hipMemcpyAsync(hf.data(), err, 2 * B * sizeof(int), hipMemcpyDeviceToHost, s));

@@ -105,6 +105,15 @@ int idp_device_count(int* n);
  * TIMER_FLAG scopes): on != 0 resets the counters and starts recording. */
 int idp_profile_enable(idp_ctx* ctx, int on);
 int idp_profile_read(idp_ctx* ctx, idp_profile* out);
+/* Launch configuration of the PCG kernels (no counterpart in the reference;
+ * set by the autotuner, idp_amd/autotune.py).  spmv_shape 0..7 indexes
+ * {lanes per block row G} x {scalars in wavefront 0, in a fifth wavefront}
+ * with G = 4, 8, 16, 32 for 0..3 and again for 4..7; upd_waves 1..8 is the
+ * number of 64-row update blocks per workgroup.  upd_waves and the
+ * wavefront placement never change results; G changes the order of each
+ * block row's sum, so one G serves every context of a process. */
+int idp_set_launch_config(idp_ctx* ctx, int spmv_shape, int upd_waves);
+int idp_get_launch_config(idp_ctx* ctx, int* spmv_shape, int* upd_waves);
 
 /* ---- host-side set-up helpers (no device work) ------------------------- */
 /* Compute_Vol_And_Inv_Basis (Library/FEM/DEFORMATION_GRADIENT.h:39-76),
