@@ -45,8 +45,9 @@ EXPORTED = [
     "idp_get_launch_config",
 ]
 
-# PCG preconditioner for every solve of this process: 0 contact-cluster block
-# Jacobi, 1 plain 3x3 block Jacobi (IDP_PRECOND in the environment)
+# PCG preconditioner for every solve of this process: 0 cluster block Jacobi
+# with clusters matched over elastic and contact couplings, 1 plain 3x3 block
+# Jacobi, 2 clusters matched over contact couplings only (IDP_PRECOND)
 PRECOND = int(os.environ.get("IDP_PRECOND", "0"))
 # 1: every PCG solve starts from zero instead of the previous direction (IDP_PCG_COLD)
 PCG_COLD = int(os.environ.get("IDP_PCG_COLD", "0"))

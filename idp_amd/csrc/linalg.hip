@@ -243,15 +243,17 @@ __global__ void k_block_jacobi(int nV, const int* __restrict__ ediag, const doub
 }
 
 // ---------------------------------------------------------------------------
-// Contact-cluster block-Jacobi preconditioner.  Stiff barrier terms couple the
-// vertices of a contact pair far more strongly than mass or elasticity do;
-// a 3x3 block-Jacobi leaves those couplings to the iteration.  Vertices are
-// grouped by two rounds of mutual heavy-edge matching on the contact blocks
-// (clusters of at most 4 vertices, never across instances), and each
-// cluster's (3k)^2 diagonal block of E + C is inverted exactly.  Vertex v
-// stores the 3 x 3k rows of its cluster inverse as k 3x3 blocks at a fixed
-// stride, z_v = sum_b Z_vb r_mb, and the member list (ascending, -1 padded)
-// as one int4, so the PCG reads both without a dependent load.
+// Cluster block-Jacobi preconditioner.  Stiff barrier terms couple the
+// vertices of a contact pair far more strongly than mass or elasticity do,
+// and a 3x3 block-Jacobi leaves those couplings (and the elastic ones) to the
+// iteration.  Vertices are grouped by two rounds of mutual heavy-edge
+// matching over the off-diagonal blocks, elastic and contact (contact only
+// with precond 2), into clusters of at most 4 vertices, never across
+// instances; each cluster's (3k)^2 diagonal block of E + C is inverted
+// exactly.  Vertex v stores the 3 x 3k rows of its cluster inverse as k 3x3
+// blocks (plane b for member b), z_v = sum_b Z_vb r_mb, and the member list
+// (ascending, -1 padded) as one int4, so the PCG reads both without a
+// dependent load.
 // ---------------------------------------------------------------------------
 constexpr int CL_MAX = 4;
 constexpr int CL_ZS = 9 * CL_MAX;  // doubles of preconditioner rows per vertex
@@ -272,24 +274,33 @@ struct ClMask {
     __device__ bool skip(int v) const { return act && !act[vinst[v]]; }
 };
 
+// elastic blocks taken into the matching as well (precond 0); rp null: contact blocks only
+struct ClElastic {
+    const int* rp;
+    const int* col;
+    const double* val;
+    long ld;
+};
+
 // strongest contact neighbour of every vertex (ties: smaller id); -1 if none
 __global__ void k_cl_best(int nV, const int* __restrict__ crp, const int* __restrict__ ccol,
-                          const double* __restrict__ cval, long cld, ClMask cm, int* __restrict__ best)
+                          const double* __restrict__ cval, long cld, ClElastic el, ClMask cm,
+                          int* __restrict__ best)
 {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nV || cm.skip(v)) return;
     int b = -1;
     double bw = 0.0;
-    if (crp)
-        for (int k = crp[v]; k < crp[v + 1]; ++k) {
-            const int u = ccol[k];
-            if (u == v) continue;
-            const double w = cblock_norm2(cval, cld, k);
-            if (w > bw || (w == bw && b >= 0 && u < b)) {
-                bw = w;
-                b = u;
-            }
+    auto consider = [&](int u, double w) {
+        if (u != v && (w > bw || (w == bw && b >= 0 && u < b))) {
+            bw = w;
+            b = u;
         }
+    };
+    if (crp)
+        for (int k = crp[v]; k < crp[v + 1]; ++k) consider(ccol[k], cblock_norm2(cval, cld, k));
+    if (el.rp)
+        for (int k = el.rp[v]; k < el.rp[v + 1]; ++k) consider(el.col[k], cblock_norm2(el.val, el.ld, k));
     best[v] = b;
 }
 
@@ -309,20 +320,25 @@ __global__ void k_cl_pair(int nV, const int* __restrict__ best, ClMask cm, int* 
 // a 64-bit max picks the heaviest, then the lowest root
 __global__ void k_cl_best2(int nV, const int* __restrict__ crp, const int* __restrict__ ccol,
                            const double* __restrict__ cval, long cld, const int* __restrict__ root,
-                           const int* __restrict__ size, ClMask cm, unsigned long long* __restrict__ pack)
+                           const int* __restrict__ size, ClElastic el, ClMask cm,
+                           unsigned long long* __restrict__ pack)
 {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nV || !crp || cm.skip(v)) return;
+    if (v >= nV || (!crp && !el.rp) || cm.skip(v)) return;
     const int r = root[v];
     unsigned long long bestp = 0;
-    for (int k = crp[v]; k < crp[v + 1]; ++k) {
-        const int ru = root[ccol[k]];
-        if (ru == r || size[r] + size[ru] > CL_MAX) continue;
-        const float wf = (float)cblock_norm2(cval, cld, k);
+    auto consider = [&](int u, double w) {
+        const int ru = root[u];
+        if (ru == r || size[r] + size[ru] > CL_MAX) return;
+        const float wf = (float)w;
         const unsigned long long p =
             ((unsigned long long)__float_as_uint(wf) << 32) | (unsigned long long)(0xFFFFFFFFu - (unsigned)ru);
         if (p > bestp) bestp = p;
-    }
+    };
+    if (crp)
+        for (int k = crp[v]; k < crp[v + 1]; ++k) consider(ccol[k], cblock_norm2(cval, cld, k));
+    if (el.rp)
+        for (int k = el.rp[v]; k < el.rp[v + 1]; ++k) consider(el.col[k], cblock_norm2(el.val, el.ld, k));
     if (bestp) atomicMax(pack + r, bestp);
 }
 
@@ -1241,7 +1257,7 @@ __global__ void k_pcg_start(int B, const unsigned char* __restrict__ solve, int*
 }
 
 // Preconditioned CG per instance on the assembled system (E + C); b, x: 3V.
-// L.precond picks contact-cluster (0) or 3x3 (1) block Jacobi.
+// L.precond picks cluster (0: elastic and contact matching, 2: contact only) or 3x3 (1) block Jacobi.
 // solve (device, B bytes; null = all) selects the instances to solve.
 // Per instance on return: iters[i] >= 0, or -1 (singular diagonal block) /
 // -2 (breakdown, p.q <= 0); relres[i] = |r|/|b|.
@@ -1266,7 +1282,7 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
     const bool haveC = L.ncb > 0;
     Profiler* P = L.prof;
     const int tok = P ? P->begin(IDP_K_PCG) : -1;
-    // contact-cluster block-Jacobi preconditioner
+    // cluster block-Jacobi preconditioner
     const int vg = grid_for(nV, 256);
     int* best = L.cl_best.ensure(nV, s);
     int* root = L.cl_root.ensure(nV, s);
@@ -1278,13 +1294,16 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
     double* Z = L.pinv.ensure(CL_ZS * (size_t)nV, s);
     int4* cm4 = (int4*)L.cl_cm4.ensure(4 * (size_t)nV, s);
     const int* crp = haveC ? L.crp.p : nullptr;
-    const int* crp_match = L.precond == 0 ? crp : nullptr;  // no contact rows: singletons (3x3 block Jacobi)
+    const int* crp_match = L.precond != 1 ? crp : nullptr;  // no contact rows: singletons (3x3 block Jacobi)
+    // precond 0: the matching weighs elastic couplings too, so clusters form
+    // everywhere; precond 2: contact couplings only
+    const ClElastic elm{L.precond == 0 ? m.e_rowptr.p : nullptr, m.e_col.p, L.evals.p, m.nnzb_e};
     // only the instances that solve (new or continued) build their clusters
     const ClMask cmk{I.vinst.p, solve};
-    IDP_LAUNCH(k_cl_best, vg, 256, 0, s, nV, crp_match, L.ccol.p, L.cvals.p, L.ncb, cmk, best);
+    IDP_LAUNCH(k_cl_best, vg, 256, 0, s, nV, crp_match, L.ccol.p, L.cvals.p, L.ncb, elm, cmk, best);
     IDP_LAUNCH(k_cl_pair, vg, 256, 0, s, nV, best, cmk, root, size);
     IDP_HIP(hipMemsetAsync(pack, 0, nV * sizeof(unsigned long long), s));
-    IDP_LAUNCH(k_cl_best2, vg, 256, 0, s, nV, crp_match, L.ccol.p, L.cvals.p, L.ncb, root, size, cmk, pack);
+    IDP_LAUNCH(k_cl_best2, vg, 256, 0, s, nV, crp_match, L.ccol.p, L.cvals.p, L.ncb, root, size, elm, cmk, pack);
     IDP_LAUNCH(k_cl_merge, vg, 256, 0, s, nV, root, pack, size, cmk, root2);
     IDP_HIP(hipMemsetAsync(cnt, 0, nV * sizeof(int), s));
     IDP_LAUNCH(k_cl_members, vg, 256, 0, s, nV, root2, cmk, cnt, mem);

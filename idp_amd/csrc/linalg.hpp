@@ -13,7 +13,8 @@ struct LinWork {
     DBuf<unsigned> u0, u1, seg, rv0, rv1;
     DBuf<unsigned long long> key0, key1;
     long ncb = 0, nrec = 0, ncs = 0;
-    int precond = 0;  // 0: contact-cluster block Jacobi, 1: 3x3 block Jacobi
+    int precond = 0;  // 0: cluster block Jacobi over elastic and contact couplings, 1: 3x3 block Jacobi,
+                      // 2: clusters over contact couplings only
     int upd_waves = 4;   // PCG update: 64-row wavefronts per workgroup (1..8; same arithmetic for any value)
     int spmv_shape = 5;  // PCG SpMV launch shape: 8 lanes per row, scalars in a wavefront of their own
     DBuf<int4> atab, btab;  // PCG launch tables

@@ -41,7 +41,9 @@ typedef struct idp_solver_opts {
     int pcg_maxit;    /* default 10000 */
     int max_newton;   /* 0 = until the reference stopping rule holds */
     int verbose;      /* > 0: print a progress line to stderr every `verbose` Newton iterations */
-    int precond;      /* 0: contact-cluster block Jacobi (default), 1: 3x3 block Jacobi */
+    int precond;      /* 0: cluster block Jacobi, clusters matched over elastic and contact
+                         couplings (default); 1: 3x3 block Jacobi; 2: clusters matched over
+                         contact couplings only */
     int pcg_cold_start; /* 1: every PCG solve starts from zero; 0 (default): from the second Newton
                            iteration of a step on, from the best multiple of the previous direction */
     int accd_budget;    /* batched engine: ACCD steps a pair may take per Newton iteration before it is
