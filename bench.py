@@ -479,7 +479,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": sc.name, "vertices": int(sc.X.shape[0]), "tets": int(sc.tets.shape[0]),
                    "pcg_rtol": a.pcg_rtol, "newton_iters": its, "pcg_iters": pcg,
-                   "max_newton": a.max_newton or None},
+                   "max_newton": a.max_newton or None, "precond": capi.PRECOND,
+                   "launch_config": list(d.ctx.launch_config())},
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
         "barrier_hessian_pair_evals_per_s": pairs / total,
         "roofline": roof,
