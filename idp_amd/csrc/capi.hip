@@ -482,8 +482,8 @@ __global__ void k_kappa_test(long n, const int4* __restrict__ cs, const int* __r
 // instances can join and leave between iterations (continuous batching).
 // ---------------------------------------------------------------------------
 // ACCD steps a pair may take in one engine iteration before it is handed over
-// to the next (about a millisecond of one lane's time)
-constexpr int ACCD_STEP_BUDGET = 1 << 14;
+// to the next (about 55 ms of one lane's time at 0.85 us per step)
+constexpr int ACCD_STEP_BUDGET = 1 << 16;
 
 struct BatchEngine {
     bool configured = false;

@@ -48,7 +48,7 @@ typedef struct idp_solver_opts {
                            iteration of a step on, from the best multiple of the previous direction */
     int accd_budget;    /* batched engine: ACCD steps a pair may take per Newton iteration before it is
                            handed over to the next one (its instance waits, the others go on); 0: the
-                           default (16384), < 0: never.  Results do not depend on it. */
+                           default (65536), < 0: never.  Results do not depend on it. */
     int pcg_handover;   /* batched engine: 0 (default): once half of the instances' PCG solves have
                            stopped, the others run as many iterations again and then continue in the
                            next Newton iteration while the rest go on; < 0: never.  Results do not
