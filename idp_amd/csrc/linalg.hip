@@ -586,7 +586,7 @@ struct PcgArgs {
     const int4 *atab, *btab;
     const int *aoff, *boff;  // per launched block: iterations its instance did in earlier calls
     int ident;               // one instance, all blocks launched: entries follow from blockIdx (no table load)
-    int nbl_upd;             // update blocks launched (PCG_UPD_WAVES per workgroup)
+    int nbl_upd;             // update blocks launched (upd_waves per workgroup)
     const int *apbeg, *bpbeg;
     const unsigned char* solve;  // per instance: 0 not solving, 1 new solve, 2 continued from an earlier call
     int* flag;        // per instance: 0 running, 1 stopped (converged or maxit), 2 breakdown, 3 not solving

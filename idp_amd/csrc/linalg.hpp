@@ -28,7 +28,7 @@ struct LinWork {
     // PCG
     DBuf<double> pinv, r, z, p, p1, q, part0, part1, part2, part3, part_bx, scal, scal2;
     DBuf<int> err;
-    // contact-cluster preconditioner
+    // cluster preconditioner
     DBuf<int> cl_best, cl_root, cl_size, cl_root2, cl_cnt, cl_mem, cl_cm4;
     DBuf<unsigned long long> cl_pack;
     DBuf<double> cl_work;
