@@ -42,3 +42,24 @@ def test_tuned_config_parsing(tmp_path):
     p.write_text("not json")
     assert capi.tuned_config(str(p)) is None
     assert capi.tuned_config(str(tmp_path / "absent.json")) is None
+
+
+def test_tune_measures_shapes_then_widths(monkeypatch):
+    """The search times every shape at the default width, then the other
+    widths at the chosen shape, per workload (measurements stubbed)."""
+    calls = []
+
+    def fake_measure(kind, sc, shape, waves, newton):
+        calls.append((shape, waves))
+        spmv = {1: 9.0, 5: 10.0}.get(shape, 20.0)
+        upd = {1: 8.0, 2: 6.0, 4: 7.0, 8: 9.0}[waves]
+        return {"spmv_shape": shape, "upd_waves": waves, "spmv_us": spmv, "update_us": upd, "samples": 1}
+
+    monkeypatch.setattr(autotune, "measure", fake_measure)
+    monkeypatch.setattr(autotune, "_workload", lambda name: ("one", None))
+    (shape, waves), rows = autotune.tune(["a", "b"], newton=1, log=lambda *a: None)
+    assert (shape, waves) == (1, 2)
+    n_shapes = len(autotune.SHAPE_TABLE)
+    assert calls[: 2 * n_shapes] == [(s, autotune.DEFAULT[1]) for s in range(n_shapes)] * 2
+    assert calls[2 * n_shapes:] == [(1, w) for w in autotune.UPD_WAVES if w != autotune.DEFAULT[1]] * 2
+    assert len(rows) == len(calls) and {r["workload"] for r in rows} == {"a", "b"}
