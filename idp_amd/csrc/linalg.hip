@@ -315,7 +315,7 @@ __global__ void k_cl_pair(int nV, const int* __restrict__ best, ClMask cm, int* 
     size[v] = pair ? 2 : 1;  // meaningful at roots
 }
 
-// round 2: strongest neighbouring cluster of every cluster (max block norm
+// second matching pass: strongest neighbouring cluster of every cluster (max block norm
 // over member-to-member contact blocks), packed (float weight, -root) so that
 // a 64-bit max picks the heaviest, then the lowest root
 __global__ void k_cl_best2(int nV, const int* __restrict__ crp, const int* __restrict__ ccol,
