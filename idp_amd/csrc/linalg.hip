@@ -315,9 +315,10 @@ __global__ void k_cl_pair(int nV, const int* __restrict__ best, ClMask cm, int* 
     size[v] = pair ? 2 : 1;  // meaningful at roots
 }
 
-// second matching pass: strongest neighbouring cluster of every cluster (max block norm
-// over member-to-member contact blocks), packed (float weight, -root) so that
-// a 64-bit max picks the heaviest, then the lowest root
+// second matching pass: strongest neighbouring cluster of every cluster (max
+// block norm over member-to-member blocks, contact and, with precond 0,
+// elastic), packed (float weight, -root) so that a 64-bit max picks the
+// heaviest, then the lowest root
 __global__ void k_cl_best2(int nV, const int* __restrict__ crp, const int* __restrict__ ccol,
                            const double* __restrict__ cval, long cld, const int* __restrict__ root,
                            const int* __restrict__ size, ClElastic el, ClMask cm,
