@@ -29,6 +29,7 @@ import time
 
 import numpy as np
 
+T_START = time.monotonic()
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
@@ -59,36 +60,102 @@ def read_traffic(path, kernel):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(sc, gpu_newton, seconds, rtol):
-    """The oracle's Newton loop on this host's cores for a bounded sample.
+def _nproc():
+    """`nproc` of this process (it honours OMP_NUM_THREADS and the CPU affinity)."""
+    import subprocess
 
-    Runs the first M Newton iterations of the same solve (M grown until about
-    `seconds` of CPU time), takes seconds per Newton iteration, and scales to
-    one solve with the GPU solve's Newton-iteration count.  Early iterations
-    carry fewer contacts than late ones, so the estimate favours the CPU.
-    """
-    from oracle import pyoracle
+    try:
+        return max(1, int(subprocess.run(["nproc"], capture_output=True, text=True, check=True).stdout.strip()))
+    except (OSError, ValueError, subprocess.CalledProcessError):
+        return max(1, len(os.sched_getaffinity(0)))
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
-    os.environ["OMP_NUM_THREADS"] = str(threads)
-    m = pyoracle.Mesh(sc.X, sc.tets, sc.density, sc.E, sc.nu)
-    dHat2, kappa = pyoracle.initialize_oipc(sc.dhat, m.mass)
-    M, t = 1, 0.0
-    while True:
-        t0 = time.perf_counter()
-        it, *_ = m.advance(sc.X, sc.velocity, sc.h, sc.newton_tol, dHat2, kappa, lin_rtol=rtol, max_newton=M)
-        t = time.perf_counter() - t0
-        if t >= seconds / 4 or it < M or M >= 64:
-            break
-        M *= 2
+
+def _cpu_model():
+    import subprocess
+
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, check=True).stdout
+        for line in out.splitlines():
+            if line.lower().startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.CalledProcessError):
+        pass
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_run(sc, threads, max_newton, rtol):
+    """One oracle solve (or its first max_newton Newton iterations) on `threads`
+    OpenMP threads, in a child process so that the thread count takes effect;
+    returns (Newton iterations, seconds)."""
+    import subprocess
+
+    code = (
+        "import sys, time, json\n"
+        f"sys.path.insert(0, {HERE!r})\n"
+        "from idp_amd import scenes\n"
+        "from oracle import pyoracle\n"
+        f"sc = scenes.{sc}()\n"
+        "m = pyoracle.Mesh(sc.X, sc.tets, sc.density, sc.E, sc.nu)\n"
+        f"pyoracle.set_accd_max_steps({ACCD_CAP})\n"
+        "d2, k = pyoracle.initialize_oipc(sc.dhat, m.mass)\n"
+        "t0 = time.perf_counter()\n"
+        f"it = m.advance(sc.X, sc.velocity, sc.h, sc.newton_tol, d2, k, lin_rtol={rtol!r}, max_newton={max_newton})[0]\n"
+        "print(json.dumps([it, time.perf_counter() - t0]))\n"
+    )
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, check=True).stdout
+    it, t = json.loads(out.strip().splitlines()[-1])
+    return int(it), float(t)
+
+
+def cpu_baseline(gpu_newton, seconds, rtol, cfg1_gpu_s):
+    """The oracle (the C++/OpenMP restatement of the reference's step) on this
+    host's cores, beside the GPU run:
+      - the bench workload (64x16x16 twist-and-fold bar): the first M Newton
+        iterations (M doubled until the sample takes about `seconds`/2), on
+        `nproc` threads and on 1 thread, scaled to one solve by the GPU
+        solve's Newton count (early iterations carry fewer contacts, so the
+        estimate favours the CPU);
+      - the 20x4x4 bend bar (the reference's CPU-runnable case): complete
+        solves on `nproc` threads and on 1 thread, measured, not scaled."""
+    nproc = _nproc()
+    res = {}
+    for thr in (nproc, 1):
+        M, it, t = 1, 0, 0.0
+        while True:
+            it, t = _oracle_run("config2", thr, M, rtol)
+            if t >= seconds / 2 / 4 or it < M or M >= 64:
+                break
+            M *= 2
+        res[thr] = (it, t)
+        log(f"[bench] cpu baseline: twist-and-fold bar, {thr} thread(s), first {it} Newton iterations in {t:.2f} s")
+    full = {}
+    for thr in (nproc, 1):
+        it, t = _oracle_run("config1", thr, 0, rtol)
+        full[thr] = {"newton": it, "seconds": t}
+        log(f"[bench] cpu baseline: bend bar complete solve, {thr} thread(s): {it} Newton iterations in {t:.2f} s")
+    it, t = res[nproc]
     per_iter = t / max(it, 1)
+    it1, t1 = res[1]
     return {
         "value": 1.0 / (per_iter * gpu_newton),
         "unit": "solves/s",
-        "cores": threads,
+        "cores": nproc,
         "kind": "port",
-        "sample": f"oracle (C++/OpenMP restatement) on the same scene: first {it} Newton iterations in "
-                  f"{t:.1f} s ({per_iter:.3f} s/iteration), scaled by the GPU solve's {gpu_newton} Newton iterations",
+        "cpu_model": _cpu_model(),
+        "sample": (f"oracle (C++/OpenMP restatement of the reference step; block-Jacobi PCG to the same relative "
+                   f"residual in place of CHOLMOD) on the twist-and-fold bar: first {it} Newton iterations in "
+                   f"{t:.2f} s on {nproc} threads ({per_iter:.3f} s/iteration), scaled by the GPU solve's "
+                   f"{gpu_newton} Newton iterations"),
+        "one_thread": {"value": 1.0 / (t1 / max(it1, 1) * gpu_newton), "newton_sampled": it1, "seconds": t1},
+        "bend_bar_full_solve": {"threads": nproc, "cpu": full[nproc], "cpu_1_thread": full[1],
+                                "gpu_seconds": cfg1_gpu_s},
     }
 
 
@@ -357,8 +424,8 @@ def bench_serve(a):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="bar_twist_fold",
                     choices=["bar_twist_fold", "bar_bend", "cubes256", "serve"])
     ap.add_argument("--frames", type=int, default=20,
@@ -368,6 +435,13 @@ def main():
                     help="cubes256: instances solved one at a time for the plain-path comparison")
     ap.add_argument("--pcg-rtol", type=float, default=1e-8)
     ap.add_argument("--max-newton", type=int, default=0)
+    ap.add_argument("--budget-s", type=float, default=480.0,
+                    help="wall-clock budget from process start: no warm-up or timed solve starts unless it "
+                         "and the closing measurements still fit (at least one solve is always timed)")
+    ap.add_argument("--reserve-s", type=float, default=60.0,
+                    help="time kept for the closing measurements (profiled solve, CPU baseline)")
+    ap.add_argument("--profile-newton", type=int, default=60,
+                    help="Newton iterations of the profiled solve that gives the per-family device times")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(HERE, "profiles", "k_pcg_spmv_traffic_bar_twist_fold.txt"))
@@ -388,60 +462,140 @@ def main():
         return bench_batch(a)
     if a.workload == "serve":
         return bench_serve(a)
+    return bench_plain(a)
+
+
+def _roofline(fam, prof, traffic_path):
+    """Roofline position of kernel family `fam` from a profile: algorithmic
+    work of its sampled launches over their summed device time."""
+    f = prof[fam]
+    launch_s = f["ms"] * 1e-3 / max(f["launches"], 1)
+    per_launch = f["work"] / max(f["launches"], 1)
+    if fam in ("barrier_hess", "tet_hess"):
+        # FP64 vector arithmetic (no MFMA instruction is issued): the FP64 vector peak
+        roof = {"kernel": fam, "bound": "fp64-vector", "achieved": per_launch / launch_s / 1e12,
+                "peak": FP64_VEC_PEAK / 1e12, "unit": "TFLOP/s", "traffic": None}
+    else:
+        roof = {"kernel": fam, "bound": "hbm", "achieved": per_launch / launch_s / 1e9, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "traffic": read_traffic(traffic_path, fam)}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["avg_launch_ms"] = launch_s * 1e3
+    roof["algorithmic_per_launch"] = per_launch
+    roof["launches_sampled"] = f["launches"]
+    return roof
+
+
+# ACCD step cap of every solve of the bench (the library default), applied to
+# the oracle's CPU baseline too
+ACCD_CAP = 1 << 20
+
+
+def bench_plain(a):
+    """One step = one complete solve of the workload from the same device
+    state.  Warm-up and timed solves run while the remaining wall-clock
+    budget (--budget-s, counted from process start) still holds one more
+    solve and the closing measurements; at least one solve is timed.  The
+    timed solves record only the sampled PCG kernel times (profile level 2);
+    a profiled partial solve afterwards gives the per-family breakdown."""
+    import signal
+
+    from idp_amd import capi, driver, scenes
+
     sc = workload(a.workload)
-    log(f"[bench] workload {sc.name}: {sc.X.shape[0]} vertices, {sc.tets.shape[0]} tets")
+    log(f"[bench] workload {sc.name}: {sc.X.shape[0]} vertices, {sc.tets.shape[0]} tets, budget {a.budget_s:.0f} s")
     d = driver.TetAnimFix(sc, pcg_rtol=a.pcg_rtol)
     k0 = d.kappa.copy()
     d.ctx.snapshot()
+    state = {"lat": [], "its": [], "pcg": [], "capped": [], "warm": 0, "done": False}
 
     def solve(max_newton):
         d.ctx.restore()
         d.kappa[:] = k0
         t0 = time.perf_counter()
         # returns after the context's stream has drained (stats and kappa are copied back)
-        it, st = d.advance_one_time_step(max_newton=max_newton, verbose=50)
+        it, st = d.advance_one_time_step(max_newton=max_newton, verbose=100)
         return it, st, time.perf_counter() - t0
+
+    def line(partial):
+        lat, its = state["lat"], state["its"]
+        n = len(lat)
+        total = float(sum(lat))
+        lat_ms = np.array(lat) * 1e3 if n else np.array([float("nan")])
+        out = {
+            "metric": "solves/sec + p50/p99 latency at fixed accuracy; barrier pair-evals/sec vs plain",
+            "value": n / total if n else 0.0,
+            "unit": "solves/s",
+            "n_gpus": 1,
+            "steps": n,
+            "warmup": state["warm"],
+            "ms_per_step": total * 1e3 / n if n else None,
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": sc.name, "vertices": int(sc.X.shape[0]), "tets": int(sc.tets.shape[0]),
+                       "pcg_rtol": a.pcg_rtol, "newton_iters": its, "pcg_iters": state["pcg"],
+                       "accd_capped_pairs": state["capped"], "accd_max_steps": ACCD_CAP,
+                       "max_newton": a.max_newton or None, "precond": capi.PRECOND,
+                       "launch_config": list(d.ctx.launch_config()),
+                       "steps_requested": a.steps, "warmup_requested": a.warmup, "budget_s": a.budget_s},
+            "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+        }
+        out.update(state.get("extra", {}))
+        if partial:
+            out["partial"] = partial
+        return out
+
+    def on_term(signum, frame):
+        if not state["done"]:
+            state["done"] = True
+            print(json.dumps(line(f"stopped by signal {signum} before the closing measurements")), flush=True)
+        sys.exit(128 + signum)
+
+    signal.signal(signal.SIGTERM, on_term)
+    elapsed = lambda: time.monotonic() - T_START  # noqa: E731
+    reserve = a.reserve_s  # closing measurements: profiled partial solve, CPU baseline
 
     # buffer priming (device allocations grow on first use): a 2-iteration solve
     solve(2)
+    est = None
     for w in range(a.warmup):
+        if est is not None and elapsed() + 2 * est + reserve > a.budget_s:
+            log(f"[bench] warm-up stopped after {w}: budget")
+            break
         it, st, t = solve(a.max_newton)
-        log(f"[bench] warmup {w}: {it} Newton iterations, {t:.2f} s")
+        est = t if est is None else max(est, t)
+        state["warm"] += 1
+        log(f"[bench] warmup {w}: {it} Newton iterations, {st['pcg_iters']} PCG iterations, {t:.2f} s")
 
-    d.ctx.profile(True)
-    lat, its, pcg, pairs = [], [], [], 0
-    t_start = time.perf_counter()
+    d.ctx.profile(2)  # sampled PCG kernel pairs only: the launch stream runs as it would unprofiled
     for k in range(a.steps):
+        if k > 0 and elapsed() + est + reserve > a.budget_s:
+            log(f"[bench] timed solves stopped after {k}: budget")
+            break
         it, st, t = solve(a.max_newton)
-        lat.append(t)
-        its.append(it)
-        pcg.append(st["pcg_iters"])
+        est = t if est is None else max(est, t)
+        state["lat"].append(t)
+        state["its"].append(it)
+        state["pcg"].append(st["pcg_iters"])
+        state["capped"].append(st["accd_capped"])
         log(f"[bench] step {k}: {it} Newton iterations, {st['pcg_iters']} PCG iterations, "
-            f"{st['n_constraints']} constraints, {t:.2f} s")
-    total = time.perf_counter() - t_start
+            f"{st['n_constraints']} constraints, {st['accd_capped']} capped ACCD pairs, {t:.2f} s")
+    prof_t = d.ctx.profile_read()
+    d.ctx.profile(0)
+    roof = _roofline("pcg_spmv", prof_t, a.traffic)
+    extra = {"roofline": roof}
+    extra["roofline_update_kernel"] = _roofline("pcg_update", prof_t, a.traffic)
+    state["extra"] = extra
+
+    # per-family device times: a profiled solve truncated to a bounded number
+    # of Newton iterations (an event pair around every launch)
+    nprof = a.profile_newton
+    d.ctx.profile(1)
+    it_p, st_p, t_p = solve(nprof)
     prof = d.ctx.profile_read()
-    d.ctx.profile(False)
-
-    # dominant kernel: largest device time among the families with a work model
-    fams = {k: prof[k] for k in ("pcg_spmv", "pcg_update", "barrier_hess", "tet_hess", "assembly")}
-    dom = max(fams, key=lambda k: fams[k]["ms"])
-    f = fams[dom]
-    launch_s = f["ms"] * 1e-3 / max(f["launches"], 1)
-    per_launch = f["work"] / max(f["launches"], 1)
-    if dom == "barrier_hess" or dom == "tet_hess":
-        # FP64 compute bound: MI355X's FP64 vector and FP64 matrix peaks are both 78.6 TFLOP/s
-        roof = {"kernel": dom, "bound": "mfma", "achieved": per_launch / launch_s / 1e12,
-                "peak": FP64_VEC_PEAK / 1e12, "unit": "TFLOP/s"}
-        roof["frac"] = roof["achieved"] / roof["peak"]
-        roof["traffic"] = None
-    else:
-        roof = {"kernel": dom, "bound": "hbm", "achieved": per_launch / launch_s / 1e9, "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s"}
-        roof["frac"] = roof["achieved"] / roof["peak"]
-        roof["traffic"] = read_traffic(a.traffic, dom)
-    roof["avg_launch_ms"] = launch_s * 1e3
-    roof["algorithmic_per_launch"] = per_launch
-
+    d.ctx.profile(0)
     kern = {}
     for k in capi.K_FAMILIES:
         e = prof[k]
@@ -449,6 +603,7 @@ def main():
     bh = prof["barrier_hess"]
     if bh["ms"] > 0:
         kern["barrier_hess"]["tflops"] = bh["work"] / (bh["ms"] * 1e-3) / 1e12
+        kern["barrier_hess"]["frac_fp64_peak"] = bh["work"] / (bh["ms"] * 1e-3) / FP64_VEC_PEAK
     for k in ("pcg", "pcg_spmv", "pcg_update", "assembly"):
         if prof[k]["ms"] > 0:
             kern[k]["gbs"] = prof[k]["work"] / (prof[k]["ms"] * 1e-3) / 1e9
@@ -457,37 +612,29 @@ def main():
     kern["barrier_hess"]["clamped"] = int(sum(prof["barrier_clamped"]))
     kern["ccd"]["steps"] = prof["ccd_steps"]
     kern["ccd"]["max_steps"] = prof["ccd_max_steps"]
+    extra["kernels_profiled_solve"] = {"newton_iters": it_p, "seconds": t_p, "families": kern}
+    extra["barrier_hessian_pair_evals_per_s"] = (pairs / (bh["ms"] * 1e-3)) if bh["ms"] > 0 else None
+    log(f"[bench] profiled solve: {it_p} Newton iterations in {t_p:.2f} s")
 
-    base = None
-    if not a.no_cpu_baseline:
-        log(f"[bench] cpu baseline (oracle, ~{a.cpu_seconds:.0f} s)")
-        base = cpu_baseline(sc, int(np.median(its)), a.cpu_seconds, a.pcg_rtol)
+    # the reference's CPU-runnable case on the GPU, complete solve (for the CPU comparison)
+    d1 = driver.TetAnimFix(scenes.config1(), pcg_rtol=a.pcg_rtol)
+    d1.ctx.snapshot()
+    d1.advance_one_time_step()
+    d1.ctx.restore()
+    d1.kappa[:] = capi.initialize_oipc(d1.scene.dhat, d1.mass)[1]
+    t0 = time.perf_counter()
+    d1.advance_one_time_step()
+    cfg1_gpu_s = time.perf_counter() - t0
+    d1.close()
 
-    lat_ms = np.array(lat) * 1e3
-    out = {
-        "metric": "solves/sec + p50/p99 latency at fixed accuracy; barrier pair-evals/sec vs plain",
-        "value": a.steps / total,
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": total * 1e3 / a.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": sc.name, "vertices": int(sc.X.shape[0]), "tets": int(sc.tets.shape[0]),
-                   "pcg_rtol": a.pcg_rtol, "newton_iters": its, "pcg_iters": pcg,
-                   "max_newton": a.max_newton or None, "precond": capi.PRECOND,
-                   "launch_config": list(d.ctx.launch_config())},
-        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
-        "barrier_hessian_pair_evals_per_s": pairs / total,
-        "roofline": roof,
-        "kernels": kern,
-        "cpu_baseline": base,
-    }
-    print(json.dumps(out), flush=True)
+    if not a.no_cpu_baseline and state["its"]:
+        log(f"[bench] cpu baseline (oracle, about {a.cpu_seconds:.0f} s)")
+        extra["cpu_baseline"] = cpu_baseline(int(np.median(state["its"])), a.cpu_seconds, a.pcg_rtol, cfg1_gpu_s)
+    else:
+        extra["cpu_baseline"] = None
+    state["done"] = True
+    print(json.dumps(line(None)), flush=True)
+    d.close()
 
 
 if __name__ == "__main__":

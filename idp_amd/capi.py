@@ -5,8 +5,11 @@ library has not been built; there is no CPU fallback.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -42,7 +45,7 @@ EXPORTED = [
     "idp_elastic_energy", "idp_elastic_gradient", "idp_elastic_hessian_local", "idp_newton_direction", "idp_get_bsr",
     "idp_profile_enable", "idp_profile_read", "idp_set_mesh_batch", "idp_batch_size", "idp_advance_batch",
     "idp_serve_configure", "idp_serve_start", "idp_serve_iterate", "idp_serve_result", "idp_set_launch_config",
-    "idp_get_launch_config",
+    "idp_get_launch_config", "idp_set_accd_max_steps",
 ]
 
 # PCG preconditioner for every solve of this process: 0 cluster block Jacobi
@@ -97,10 +100,16 @@ class IdpError(RuntimeError):
         self.code = code
 
 
+# ACCD step cap per pair and call (0: the library default, 2^20; < 0: none,
+# as in the reference; IDP_ACCD_MAX_STEPS)
+ACCD_MAX_STEPS = int(os.environ.get("IDP_ACCD_MAX_STEPS", "0"))
+ACCD_MAX_STEPS_DEFAULT = 1 << 20
+
+
 class SolverOpts(C.Structure):
     _fields_ = [("pcg_rtol", C.c_double), ("pcg_maxit", C.c_int), ("max_newton", C.c_int), ("verbose", C.c_int),
                 ("precond", C.c_int), ("pcg_cold_start", C.c_int), ("accd_budget", C.c_int),
-                ("pcg_handover", C.c_int)]
+                ("pcg_handover", C.c_int), ("accd_max_steps", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -112,6 +121,7 @@ class StepStats(C.Structure):
         ("ms_total", C.c_double), ("ms_constraint_set", C.c_double), ("ms_ccd", C.c_double),
         ("ms_energy", C.c_double), ("ms_gradient", C.c_double), ("ms_hessian", C.c_double),
         ("ms_solve", C.c_double), ("dbc_alpha", C.c_double), ("dbc_stiff", C.c_double),
+        ("pcg_unconverged", C.c_int), ("accd_capped", C.c_long),
     ]
 
     def as_dict(self):
@@ -162,8 +172,9 @@ def lib():
         L.idp_snapshot_state.argtypes = [vp]
         L.idp_restore_state.argtypes = [vp]
         L.idp_surface_sizes.argtypes = [vp, ip, ip, ip]
-        L.idp_advance_one_step.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_double, dp, C.c_double,
-                                           C.POINTER(SolverOpts), C.POINTER(StepStats)]
+        L.idp_advance_one_step.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_double, dp, C.c_double, dp,
+                                           C.c_int, C.POINTER(SolverOpts), C.POINTER(StepStats)]
+        L.idp_set_accd_max_steps.argtypes = [vp, C.c_int]
         L.idp_constraint_set.argtypes = [vp, dp, C.c_double, C.c_double, C.POINTER(C.c_long)]
         L.idp_get_constraints.argtypes = [vp, ip, C.c_long]
         L.idp_barrier_energy.argtypes = [vp, dp, C.c_long, ip, C.c_double, C.c_double, C.c_double, dp]
@@ -260,14 +271,30 @@ def initialize_oipc(dhat, mass, stiff_mult=1.0):
     return d2.value, k
 
 
+# Contexts still open when the interpreter exits are destroyed by an atexit
+# hook, which runs before the interpreter tears its modules down and before
+# the HIP runtime's own exit handlers; a context is never destroyed from a
+# finalizer during interpreter shutdown.
+_live = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all():
+    for ctx in list(_live):
+        ctx.close()
+
+
 class Context:
-    """One device context: mesh, state and work buffers on one HIP stream."""
+    """One device context: mesh, state and work buffers on one HIP stream.
+    Close it with close() (or a with-block); contexts left open are closed at
+    interpreter exit."""
 
     def __init__(self, device=0):
         self._h = C.c_void_p()
         rc = lib().idp_ctx_create(device, C.byref(self._h))
         if rc != IDP_OK:
             raise IdpError(rc)
+        _live.add(self)
         self.nV = 0
         cfg = tuned_config()
         if cfg and "IDP_SPMV_SHAPE" not in os.environ:  # an explicit shape (tuning runs) wins
@@ -284,10 +311,21 @@ class Context:
 
     def close(self):
         if self._h:
-            lib().idp_ctx_destroy(self._h)
-            self._h = C.c_void_p()
+            h, self._h = self._h, C.c_void_p()
+            _live.discard(self)
+            lib().idp_ctx_destroy(h)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
+        # during interpreter shutdown the runtime may already be gone: the
+        # atexit hook has closed every context that was still reachable
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -328,7 +366,8 @@ class Context:
                                              tp, ibp, vp, lp, mp, msp))
 
     def advance_batch(self, h, tol, dHat2, kappa3, thickness=0.0, with_collision=True, pcg_rtol=1e-8,
-                      pcg_maxit=10000, max_newton=0, verbose=0, accd_budget=None, pcg_handover=None):
+                      pcg_maxit=10000, max_newton=0, verbose=0, accd_budget=None, pcg_handover=None,
+                      accd_max_steps=None):
         """One step of every instance; kappa3 (B,3) is updated in place.
         verbose > 0 prints a progress line to stderr every `verbose` Newton iterations;
         accd_budget: ACCD steps per pair and Newton iteration before a hand-over
@@ -344,7 +383,8 @@ class Context:
         st = (StepStats * B)()
         opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, verbose, PRECOND, PCG_COLD,
                           ACCD_BUDGET if accd_budget is None else accd_budget,
-                          PCG_HANDOVER if pcg_handover is None else pcg_handover)
+                          PCG_HANDOVER if pcg_handover is None else pcg_handover,
+                          ACCD_MAX_STEPS if accd_max_steps is None else accd_max_steps)
         self._check(lib().idp_advance_batch(self._h, h, tol.ctypes.data_as(dp), int(with_collision),
                                             d2.ctypes.data_as(dp), kappa3.ctypes.data_as(dp), thickness,
                                             C.byref(opts), its.ctypes.data_as(ip), st))
@@ -352,10 +392,11 @@ class Context:
 
     # ---- continuous batching over the slots of a batch mesh ----------------
     def serve_configure(self, h, with_collision=True, thickness=0.0, pcg_rtol=1e-8, pcg_maxit=10000, max_newton=0,
-                        accd_budget=None, pcg_handover=None):
+                        accd_budget=None, pcg_handover=None, accd_max_steps=None):
         opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, 0, PRECOND, PCG_COLD,
                           ACCD_BUDGET if accd_budget is None else accd_budget,
-                          PCG_HANDOVER if pcg_handover is None else pcg_handover)
+                          PCG_HANDOVER if pcg_handover is None else pcg_handover,
+                          ACCD_MAX_STEPS if accd_max_steps is None else accd_max_steps)
         self._check(lib().idp_serve_configure(self._h, h, int(with_collision), thickness, C.byref(opts)))
 
     def serve_start(self, slots, X, v, tol, dHat2, kappa3):
@@ -428,17 +469,27 @@ class Context:
         self._check(lib().idp_restore_state(self._h))
 
     def advance(self, h, tol, dHat2, kappa, thickness=0.0, with_collision=True, pcg_rtol=1e-8, pcg_maxit=10000,
-                max_newton=0, verbose=0):
-        """Advance_One_Step_IE_Discrete_Shell; kappa (3,) is updated in place."""
+                max_newton=0, verbose=0, body_force=None, flow=False, accd_max_steps=None):
+        """Advance_One_Step_IE_Discrete_Shell (flow=False: the Hinge binding,
+        True: the Flow binding); kappa (3,) is updated in place; body_force
+        (nV, 3) or None."""
         kappa_arr = np.asarray(kappa, dtype=np.float64)
-        if not kappa_arr.flags["C_CONTIGUOUS"]:
-            raise ValueError("kappa must be contiguous float64")
-        opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, verbose, PRECOND, PCG_COLD, ACCD_BUDGET, PCG_HANDOVER)
+        if not (kappa_arr.flags["C_CONTIGUOUS"] and kappa_arr.dtype == np.float64 and kappa_arr.size == 3):
+            raise ValueError("kappa must be a contiguous float64 array of 3")
+        bf, bfp = (None, None) if body_force is None else _d(np.asarray(body_force, dtype=np.float64).ravel())
+        if bf is not None and bf.size != 3 * self.nV:
+            raise ValueError("body_force must be (nV, 3)")
+        opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, verbose, PRECOND, PCG_COLD, ACCD_BUDGET, PCG_HANDOVER,
+                          ACCD_MAX_STEPS if accd_max_steps is None else accd_max_steps)
         st = StepStats()
         it = self._check(lib().idp_advance_one_step(self._h, h, tol, int(with_collision), dHat2,
-                                                    kappa_arr.ctypes.data_as(dp), thickness, C.byref(opts),
-                                                    C.byref(st)))
+                                                    kappa_arr.ctypes.data_as(dp), thickness, bfp, int(bool(flow)),
+                                                    C.byref(opts), C.byref(st)))
         return it, st.as_dict()
+
+    def set_accd_max_steps(self, n):
+        """ACCD step cap of the stage calls (0 default, < 0 none)."""
+        self._check(lib().idp_set_accd_max_steps(self._h, int(n)))
 
     # ---- stage entry points -------------------------------------------------
     def constraint_set(self, X, dHat2, thickness=0.0):
@@ -516,7 +567,7 @@ class Context:
         sol, sp = _out_d(X.size)
         rel = C.c_double()
         it = C.c_int()
-        opts = SolverOpts(pcg_rtol, pcg_maxit, 0, 0, PRECOND, PCG_COLD, ACCD_BUDGET, PCG_HANDOVER)
+        opts = SolverOpts(pcg_rtol, pcg_maxit, 0, 0, PRECOND, PCG_COLD, ACCD_BUDGET, PCG_HANDOVER, ACCD_MAX_STEPS)
         self._check(lib().idp_newton_direction(self._h, Xp, xtp, h, dHat2, kappa, thickness, C.byref(opts), gp, sp,
                                                C.byref(rel), C.byref(it)))
         return g, sol, rel.value, it.value

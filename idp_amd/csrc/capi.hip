@@ -55,6 +55,10 @@ struct idp_ctx {
     DBuf<double> bpar;       // per instance: dh[B], kappa[B], th2[B]
     DBuf<double> esum;       // per-instance partial-sum results
     BatchEngine* eng = nullptr;  // batched / served solves
+    // Advance_One_Step_IE_Flow: the volumetric (tet) term is left out of the
+    // energy, gradient and Hessian (INC_POTENTIAL.h:53-73, 191-211, 323-339)
+    bool no_elastic = false;
+    DBuf<double> bforce;  // body force of the current step (3 nV)
 };
 
 namespace {
@@ -62,6 +66,13 @@ namespace {
 double now_ms()
 {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// ACCD step cap of a solve: 0 selects the library default, < 0 none
+int accd_cap_from(const idp_solver_opts* o)
+{
+    if (!o || o->accd_max_steps == 0) return IDP_ACCD_MAX_STEPS_DEFAULT;
+    return o->accd_max_steps < 0 ? 0 : o->accd_max_steps;
 }
 
 template <class F>
@@ -230,7 +241,8 @@ void energies(idp_ctx* c, const double* X, bool with_barrier, double h, std::vec
     const int tok = c->prof.begin(IDP_K_ENERGY);
     double* pe = c->part_e.ensure(I.tblk.nblk + 1, s);
     double* pi = c->part_i.ensure(I.vblk.nblk + 1, s);
-    launch_tet_energy(s, m, I.tblk, X, h * h, pe, fl);
+    if (c->no_elastic) IDP_HIP(hipMemsetAsync(pe, 0, (I.tblk.nblk + 1) * sizeof(double), s));
+    else launch_tet_energy(s, m, I.tblk, X, h * h, pe, fl);
     launch_inertia_energy(s, I.vblk, X, c->xt.p, m.mass.p, pi);
     const bool bar = with_barrier && c->cw.ncs > 0;
     double* es = c->esum.ensure(3 * (size_t)B, s);
@@ -289,7 +301,8 @@ void gradient(idp_ctx* c, const double* X, double h, bool with_barrier, double* 
     auto& m = c->m;
     double* gl = c->gl.ensure(12 * (size_t)m.nT, s);
     const int tok = c->prof.begin(IDP_K_GRADIENT);
-    launch_tet_grad(s, m, X, h * h, gl, act);
+    if (c->no_elastic) IDP_HIP(hipMemsetAsync(gl, 0, 12 * (size_t)m.nT * sizeof(double), s));
+    else launch_tet_grad(s, m, X, h * h, gl, act);
     const double* gb = nullptr;
     long ncs = c->cw.ncs;
     if (with_barrier && ncs) {
@@ -314,7 +327,8 @@ void hessian(idp_ctx* c, const double* X, double h, bool with_barrier, const uns
     auto& m = c->m;
     double* Hl = c->Hl.ensure(144 * (size_t)m.nT, s);
     int tok = c->prof.begin(IDP_K_TET_HESS);
-    launch_tet_hess(s, m, X, h * h, 1, Hl, act);
+    if (c->no_elastic) IDP_HIP(hipMemsetAsync(Hl, 0, 144 * (size_t)m.nT * sizeof(double), s));
+    else launch_tet_hess(s, m, X, h * h, 1, Hl, act);
     c->prof.end(IDP_K_TET_HESS, tok, 6500.0 * (nT_act < 0 ? m.nT : nT_act));
     const double* Hb = nullptr;
     long ncs = c->cw.ncs;
@@ -326,7 +340,7 @@ void hessian(idp_ctx* c, const double* X, double h, bool with_barrier, const uns
         BarrierParams bp = bparams(c);
         bp.act = act;
         launch_barrier_hess(s, ncs, (const int*)c->cw.cs.p, X, m.X0.p, bp, 1, 1, b, nullptr, fl,
-                            c->prof.on ? c->bcounts.p : nullptr);
+                            c->prof.full() ? c->bcounts.p : nullptr);
         c->prof.end(IDP_K_BARRIER_HESS, tok);
         Hb = b;
     }
@@ -356,6 +370,19 @@ const double* stage_x(idp_ctx* c, const double* X)
 // ---------------------------------------------------------------------------
 // ---- moving Dirichlet nodes (Shell/DIRICHLET.h) --------------------------
 // x of the DBC entries, in entry order
+// x~ = x + (h v + h^2 a), a = M^-1 b with the lumped mass; the flow variant
+// zeroes v first (IMPLICIT_EULER.h:196-217)
+__global__ void k_xtilde(long n3, const double* __restrict__ x, double* __restrict__ v,
+                         const double* __restrict__ b, const double* __restrict__ mass, double h, int flow,
+                         double* __restrict__ xt)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    if (flow) v[i] = 0.0;
+    const double a = b ? b[i] / mass[i / 3] : 0.0;
+    xt[i] = x[i] + (h * v[i] + h * h * a);
+}
+
 __global__ void k_dbc_gather(int n, const int* __restrict__ idx, const double* __restrict__ X, double* __restrict__ out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -535,6 +562,7 @@ struct BatchEngine {
         cold = opts && opts->pcg_cold_start;
         accd_budget = !opts || opts->accd_budget == 0 ? ACCD_STEP_BUDGET : (opts->accd_budget < 0 ? 0 : opts->accd_budget);
         pcg_handover = !opts || opts->pcg_handover >= 0;
+        c->cw.accd_max_steps = accd_cap_from(opts);
         const char* sl = std::getenv("IDP_STAGE_LOG");
         stage_log = sl && std::atoi(sl) > 0;
         std::fill(stage_ms, stage_ms + NSTAGE, 0.0);
@@ -727,6 +755,7 @@ struct BatchEngine {
                 else {
                     S[i].pcg_iters += its[i];
                     S[i].last_relres = rel[i];
+                    S[i].pcg_unconverged += c->L.h_maxit[i];
                 }
             }
         }
@@ -784,6 +813,7 @@ struct BatchEngine {
             const double tc = now_ms() - t0;
             for (int i = 0; i < B; ++i)
                 if (ls[i] || was_pend[i]) {  // took part in this CCD
+                    S[i].accd_capped += c->cw.h_capped[i];
                     go[i] = !pnow[i];
                     if (go[i]) alpha[i] = a2[i];
                     pend[i] = pnow[i];
@@ -1134,7 +1164,9 @@ int idp_get_launch_config(idp_ctx* c, int* spmv_shape, int* upd_waves)
 int idp_profile_enable(idp_ctx* c, int on)
 {
     return guarded(c, [&]() -> int {
+        if (on < 0 || on > 2) return IDP_E_INVALID;
         c->prof.reset();
+        c->prof.level = on;
         c->prof.on = on != 0;
         int* b = c->bcounts.ensure(16, c->s);
         IDP_HIP(hipMemsetAsync(b, 0, 16 * sizeof(int), c->s));
@@ -1370,10 +1402,17 @@ int idp_restore_state(idp_ctx* c)
 // the time step
 // ---------------------------------------------------------------------------
 int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_collision, double dHat2, double* kappa3,
-                         double thickness, const idp_solver_opts* opts, idp_step_stats* st)
+                         double thickness, const double* b, int flow, const idp_solver_opts* opts,
+                         idp_step_stats* st)
 {
+    struct ElasticOn {  // the flag never outlives the call, whatever path leaves it
+        idp_ctx* c;
+        ~ElasticOn() { c->no_elastic = false; }
+    } restore_elastic{c};
     return guarded(c, [&]() -> int {
         if (!c->have_mesh || !kappa3 || !(h > 0)) return IDP_E_INVALID;
+        if (c->m.inst.B != 1) return IDP_E_INVALID;  // batched contexts step with idp_advance_batch
+        c->no_elastic = flow != 0;
         IDP_HIP(hipSetDevice(c->device));
         const double t_start = now_ms();
         idp_step_stats S;
@@ -1384,6 +1423,7 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
         const int verbose = opts ? opts->verbose : 0;
         c->L.precond = opts ? opts->precond : 0;
         const bool cold = opts && opts->pcg_cold_start;
+        c->cw.accd_max_steps = accd_cap_from(opts);
         auto s = c->s;
         MeshDev& m = c->m;
         const long n3 = 3L * m.nV;
@@ -1395,9 +1435,13 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
         double* sol = c->sol.ensure(n3, s);
         double* rhs = c->rhs.ensure(n3, s);
         double* g = c->g.ensure(n3, s);
-        // Xn = X; Xtilde = X + h v (zero body force, IMPLICIT_EULER.h:196-217)
+        // Xn = X; Xtilde = X + h v + h^2 M^-1 b, v = 0 first for the flow variant (IMPLICIT_EULER.h:196-217)
         IDP_HIP(hipMemcpyAsync(xn, x, n3 * 8, hipMemcpyDeviceToDevice, s));
-        dev_axpy(s, n3, x, h, c->v.p, xt);
+        if (b || flow)
+            IDP_LAUNCH(k_xtilde, grid_for(n3, 256), 256, 0, s, n3, x, c->v.p, b ? c->bforce.upload_get(b, n3, s) : nullptr,
+                       m.mass.p, h, flow ? 1 : 0, xt);
+        else
+            dev_axpy(s, n3, x, h, c->v.p, xt);
         double kappa = kappa3[0];
         set_bparams1(c, dHat2, kappa, thickness);
 
@@ -1457,6 +1501,10 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
                 long nc = 0;
                 double* P = c->hp.upload_get(hp.data(), n3, s);
                 DBCAlpha = contact_ccd(c->cw, s, m, x, P, thickness, 1.0, &nc);
+                if (DBCAlpha == 0.0) {  // IPC.h:2014-2031 exits on a zero time of impact
+                    c->err = "zero CCD step size for the Dirichlet move";
+                    return IDP_E_CCD_ZERO;
+                }
             }
             if (DBCAlpha == 1.0) {
                 IDP_LAUNCH(k_dbc_set, grid_for(nd, 256), 256, 0, s, nd, c->dbc_idx_d.p, c->dbc_tgt_d.p, x);
@@ -1523,6 +1571,7 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
                 } else {
                     S.pcg_iters += it;
                     S.last_relres = rel;
+                    S.pcg_unconverged += c->L.h_maxit[0];
                 }
             } else {
                 IDP_HIP(hipMemcpyAsync(sol, rhs, n3 * 8, hipMemcpyDeviceToDevice, s));
@@ -1550,6 +1599,12 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
                 alpha = contact_ccd(c->cw, s, m, x, sol, thickness, alpha, &nc);
                 S.ms_ccd += now_ms() - t0;
                 S.ccd_candidates += nc;
+                S.accd_capped += c->cw.last_capped;
+                if (alpha == 0.0) {  // IPC.h:2014-2031 exits on a zero time of impact
+                    c->err = "zero CCD step size";
+                    if (st) *st = S;
+                    return IDP_E_CCD_ZERO;
+                }
             }
             const double feasibleAlpha = alpha;
             if (PNIter == 0) S.first_ccd_alpha = alpha;
@@ -1657,6 +1712,8 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
                              now_ms() - t_start);
             S.last_res = L2;
             S.E = Eprev;
+            // Advance_One_Step_IE_Flow ends once no contact is left (IMPLICIT_EULER.h:850-854)
+            if (flow && (!coll || c->cw.ncs == 0)) break;
             if (max_newton > 0 && PNIter >= max_newton) break;
         } while (res.size() < 3 || L2 > newton_tol);
 
@@ -1802,6 +1859,15 @@ int idp_ccd_stepsize(idp_ctx* c, const double* X, const double* p, double thickn
         long nc = 0;
         *alpha_out = contact_ccd(c->cw, s, c->m, xd, c->hp.p, thickness, alpha_in, &nc);
         if (n_candidates) *n_candidates = nc;
+        if (*alpha_out == 0.0) return IDP_E_CCD_ZERO;  // IPC.h:2014-2031
+        return IDP_OK;
+    });
+}
+
+int idp_set_accd_max_steps(idp_ctx* c, int n)
+{
+    return guarded(c, [&]() -> int {
+        c->cw.accd_max_steps = n == 0 ? IDP_ACCD_MAX_STEPS_DEFAULT : (n < 0 ? 0 : n);
         return IDP_OK;
     });
 }
@@ -2043,14 +2109,24 @@ int idp_serve_start(idp_ctx* c, int n, const int* slots, const double* X, const 
 {
     return guarded(c, [&]() -> int {
         BatchEngine& E = *c->eng;
-        if (!c->have_mesh || !E.configured || n <= 0 || !slots || !X || !v) return IDP_E_INVALID;
+        if (!c->have_mesh || !E.configured || n <= 0 || !slots || !X || !v || !newton_tol || !dHat2 || !kappa3)
+            return IDP_E_INVALID;
         const InstDev& I = c->m.inst;
         std::vector<int> sl(slots, slots + n);
+        // every slot must be free and listed once; nothing is copied before that holds
+        std::vector<char> seen(I.B, 0);
+        for (int k = 0; k < n; ++k) {
+            const int i = sl[k];
+            if (i < 0 || i >= I.B || E.active[i] || seen[i]) {
+                c->err = "idp_serve_start: slot out of range, busy or listed twice";
+                return IDP_E_INVALID;
+            }
+            seen[i] = 1;
+        }
         std::vector<double> tol(I.B), d2(I.B), k3(3 * I.B);
         long off = 0;
         for (int k = 0; k < n; ++k) {
             const int i = sl[k];
-            if (i < 0 || i >= I.B || E.active[i]) return IDP_E_INVALID;
             const long nv = I.vbeg[i + 1] - I.vbeg[i];
             IDP_HIP(hipMemcpyAsync(c->x.p + 3 * I.vbeg[i], X + 3 * off, 3 * nv * 8, hipMemcpyHostToDevice, c->s));
             IDP_HIP(hipMemcpyAsync(c->v.p + 3 * I.vbeg[i], v + 3 * off, 3 * nv * 8, hipMemcpyHostToDevice, c->s));

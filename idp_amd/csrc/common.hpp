@@ -107,7 +107,11 @@ struct Profiler {
         int fam, a, b, tag;
         double work;
     };
-    bool on = false;
+    // 0 off; 1 every family (an event pair around each launch); 2 only the
+    // sampled PCG kernel pairs, so a timed run keeps its launch gaps
+    int level = 0;
+    bool on = false;  // level != 0
+    bool full() const { return level == 1; }
     hipStream_t s = nullptr;
     std::vector<hipEvent_t> pool;
     std::vector<Rec> open;
@@ -138,14 +142,14 @@ struct Profiler {
     // returns a token for end()
     int begin(int fam)
     {
-        if (!on) return -1;
+        if (!full()) return -1;
         int a = ev();
         IDP_HIP(hipEventRecord(pool[a], s));
         return a;
     }
     void end(int fam, int tok, double w = 0.0, int tag = 0)
     {
-        if (!on || tok < 0) return;
+        if (!full() || tok < 0) return;
         int b = ev();
         IDP_HIP(hipEventRecord(pool[b], s));
         open.push_back({fam, tok, b, tag, w});
@@ -175,7 +179,7 @@ struct Profiler {
     }
     void add_work(int fam, double w)
     {
-        if (on) work[fam] += w;
+        if (full()) work[fam] += w;
     }
     // safe point: collect when many records are open
     void checkpoint()

@@ -760,8 +760,8 @@ __global__ void __launch_bounds__(256) k_ccd_walk(int nb, int ne, CCDArgs A, Cel
 }
 
 // ACCD over the candidates as a work queue (amin: running minimum per instance).  Each lane carries one pair
-// through the additive CCD iteration of CCD.h:280-395 (same arithmetic as
-// accd_pt / accd_ee, step by step); a lane whose pair ends (hit, passed the
+// through the additive CCD iteration of CCD.h:280-395 (the reference's
+// arithmetic, step by step); a lane whose pair ends (hit, passed the
 // bound, or zero motion) takes the next pair, one atomic per wavefront for
 // all idle lanes, so long and short pairs do not serialise a wavefront.
 // PT candidates are queued before EE ones.  The bound is the running
@@ -844,11 +844,12 @@ __device__ inline bool accd_init(AccdLane& L, int2 c, const CCDArgs& A, double e
     return true;
 }
 
-// one additive step; returns 0 running, 1 hit (toc), 2 no hit.  The next
-// step's bound is computed before the gap test is resolved (the same
+// one additive step; returns 0 running, 1 hit (toc), 2 no hit, 3 stopped by
+// the step cap (max_steps > 0) and taken as a hit at the toc reached.  The
+// next step's bound is computed before the gap test is resolved (the same
 // expression the loop would evaluate after it), so the two divisions overlap
 // instead of following each other across the branch.
-__device__ inline int accd_step(AccdLane& L, double eta, double th)
+__device__ inline int accd_step(AccdLane& L, double eta, double th, int max_steps)
 {
     const double lb = L.lb;
 #pragma unroll
@@ -862,15 +863,16 @@ __device__ inline int accd_step(AccdLane& L, double eta, double th)
     if (L.toc != 0.0 && (L.f / den < L.gap)) return 1;
     L.toc += lb;
     if (L.toc > L.tmax) return 2;
-    if (++L.guard >= (1 << 20)) return 1;
+    if (++L.guard >= max_steps && max_steps > 0) return 3;
     return 0;
 }
 
 __global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, unsigned cap,
                                               const int2* __restrict__ cand, CCDArgs A, double eta,
                                               unsigned long long* __restrict__ amin, unsigned* __restrict__ next,
-                                              unsigned long long* __restrict__ stepstat /* total, max */,
+                                              unsigned long long* __restrict__ stepstat /* total, max, capped */,
                                               unsigned* __restrict__ nlog, double* __restrict__ lg, int budget,
+                                              int max_steps, unsigned* __restrict__ capcnt,
                                               const AccdLane* __restrict__ res_in, unsigned nres,
                                               AccdLane* __restrict__ res_out, unsigned* __restrict__ nres_out,
                                               unsigned res_cap, unsigned char* __restrict__ pend)
@@ -920,12 +922,16 @@ __global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, 
             const int room = budget > 0 ? min(ACCD_RUN, budget - slice) : ACCD_RUN;
             int r = 0, k = 0;
             do {
-                r = accd_step(L, eta, A.th);
+                r = accd_step(L, eta, A.th, max_steps);
                 ++k;
             } while (r == 0 && k < room);
             steps += k;
             slice += k;
-            if (r == 1) atomicMin(amin + L.inst, (unsigned long long)__double_as_longlong(L.toc));
+            if (r == 1 || r == 3) atomicMin(amin + L.inst, (unsigned long long)__double_as_longlong(L.toc));
+            if (r == 3) {
+                atomicAdd(stepstat + 2, 1ull);
+                atomicAdd(capcnt + L.inst, 1u);
+            }
             if (r != 0) {
                 active = false;
                 atomicMax(stepstat + 1, (unsigned long long)L.guard + 1);
@@ -1373,12 +1379,14 @@ void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const do
     double* dalpha = w.ialpha.ensure(B, s);
     unsigned long long* amin = w.iamin.ensure(B, s);
     unsigned* cnt = w.cnt.ensure(6, s);
-    unsigned long long* stepstat = (unsigned long long*)w.scal.ensure(4, s) + 2;
+    unsigned long long* stepstat = (unsigned long long*)w.scal.ensure(6, s) + 2;  // total, max, capped
     IDP_LAUNCH(k_ccd_cap, grid_for(B, 64), 64, 0, s, B, ep, I.echunk.pbeg.p, I.d_ebeg.p, pp, I.pchunk.pbeg.p,
                I.d_nbeg.p, ain, dalpha, amin, active);
     IDP_LAUNCH(k_zero_u32, 1, 64, 0, s, cnt, 6);
-    IDP_LAUNCH(k_zero_u64, 1, 64, 0, s, stepstat, 2);
+    IDP_LAUNCH(k_zero_u64, 1, 64, 0, s, stepstat, 3);
     IDP_LAUNCH(k_zero_u8, grid_for(B, 256), 256, 0, s, pend, B);
+    unsigned* capcnt = w.icap.ensure(B, s);
+    IDP_LAUNCH(k_fill_u32, grid_for(B, 256), 256, 0, s, (size_t)B, capcnt, 0u);
     double* lg = w.accd_log.ensure(ACCD_LOG_W * ACCD_LOG_MAX, s);
 
     // swept grid over boundary nodes at [x, x + alpha p] -/+ th/2 (SPATIAL_HASH.h:484-511)
@@ -1405,7 +1413,7 @@ void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const do
     A.th = thickness;
     if (w.cap_cand == 0) w.cap_cand = 16u * (m.nb + m.ne) + 4096;
     std::vector<unsigned long long> ha(B);
-    unsigned long long hsteps[2] = {0, 0};
+    unsigned long long hsteps[3] = {0, 0, 0};
     unsigned hn[6] = {0, 0, 0, 0, 0, 0};
     for (int attempt = 0; attempt < 2; ++attempt) {
         int2* cand = w.cand.ensure(w.cap_cand, s);
@@ -1413,8 +1421,9 @@ void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const do
         if (attempt) {
             IDP_LAUNCH(k_reset_amin, grid_for(B, 64), 64, 0, s, B, dalpha, amin, active);
             IDP_LAUNCH(k_zero_u32, 1, 64, 0, s, cnt, 6);
-            IDP_LAUNCH(k_zero_u64, 1, 64, 0, s, stepstat, 2);
+            IDP_LAUNCH(k_zero_u64, 1, 64, 0, s, stepstat, 3);
             IDP_LAUNCH(k_zero_u8, grid_for(B, 256), 256, 0, s, pend, B);
+            IDP_LAUNCH(k_fill_u32, grid_for(B, 256), 256, 0, s, (size_t)B, capcnt, 0u);
         }
         if (T.n) {
             unsigned* nbig = cnt + 5;
@@ -1435,7 +1444,7 @@ void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const do
         }
         // a work queue needs no more lanes than the chip holds at once
         IDP_LAUNCH_TIMED(k_accd, 2048, 256, 0, s, e0, e1, cnt, cap, cand, A, 0.1, amin, cnt + 2, stepstat, cnt + 3,
-                         lg, budget, res_in, nres, res_out, cnt + 4, RES_CAP, pend);
+                         lg, budget, w.accd_max_steps, capcnt, res_in, nres, res_out, cnt + 4, RES_CAP, pend);
         if (w.prof) w.prof->end(IDP_K_CCD, tok);
         if (log_time) {
             // kernel time against the longest pair's steps in this call
@@ -1482,6 +1491,14 @@ void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const do
         w.prof->ccd_max_steps = std::max(w.prof->ccd_max_steps, (double)hsteps[1]);
     }
     if (ncand_out) *ncand_out = (long)hn[0] + hn[1];
+    w.last_capped = (long)hsteps[2];
+    w.h_capped.resize(B);
+    if (hsteps[2]) {
+        IDP_HIP(hipMemcpyAsync(w.h_capped.data(), capcnt, B * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        IDP_HIP(hipStreamSynchronize(s));
+    } else {
+        std::fill(w.h_capped.begin(), w.h_capped.end(), 0u);
+    }
     std::vector<unsigned char> act(B, 1);
     if (active) {
         IDP_HIP(hipMemcpyAsync(act.data(), active, B, hipMemcpyDeviceToHost, s));

@@ -226,80 +226,6 @@ __device__ inline bool ee_ccd_broadphase(d3 a0, d3 a1, d3 b0, d3 b1, d3 da0, d3 
     return true;
 }
 
-// Additive CCD, CCD.h:280-328.  toc: in = current bound, out = time of impact
-// lower bound; returns false when the pair cannot lower the bound.
-__device__ inline bool accd_pt(d3 p, d3 t0, d3 t1, d3 t2, d3 dp, d3 dt0, d3 dt1, d3 dt2, double eta, double th,
-                               double& toc)
-{
-    d3 mov = 0.25 * (((dt0 + dt1) + dt2) + dp);
-    dt0 = dt0 - mov;
-    dt1 = dt1 - mov;
-    dt2 = dt2 - mov;
-    dp = dp - mov;
-    double m2 = fmax(sqn(dt0), fmax(sqn(dt1), sqn(dt2)));
-    double maxDisp = sqrt(sqn(dp)) + sqrt(m2);
-    if (maxDisp == 0) return false;
-    double d2 = d2_pt_unclassified(p, t0, t1, t2);
-    double dc = sqrt(d2);
-    double gap = eta * (d2 - th * th) / (dc + th);
-    double tprev = toc;
-    toc = 0;
-    for (int guard = 0; guard < (1 << 20); ++guard) {
-        double lb = (1 - eta) * (d2 - th * th) / ((dc + th) * maxDisp);
-        p = p + lb * dp;
-        t0 = t0 + lb * dt0;
-        t1 = t1 + lb * dt1;
-        t2 = t2 + lb * dt2;
-        d2 = d2_pt_unclassified(p, t0, t1, t2);
-        dc = sqrt(d2);
-        if (toc != 0.0 && ((d2 - th * th) / (dc + th) < gap)) return true;
-        toc += lb;
-        if (toc > tprev) return false;
-    }
-    return true;
-}
-
-// Additive CCD, CCD.h:331-395
-__device__ inline bool accd_ee(d3 a0, d3 a1, d3 b0, d3 b1, d3 da0, d3 da1, d3 db0, d3 db1, double eta, double th,
-                               double& toc)
-{
-    d3 mov = 0.25 * (((da0 + da1) + db0) + db1);
-    da0 = da0 - mov;
-    da1 = da1 - mov;
-    db0 = db0 - mov;
-    db1 = db1 - mov;
-    double maxDisp = sqrt(fmax(sqn(da0), sqn(da1))) + sqrt(fmax(sqn(db0), sqn(db1)));
-    if (maxDisp == 0) return false;
-    double d2 = d2_ee_unclassified(a0, a1, b0, b1);
-    double f = d2 - th * th;
-    if (f <= 0) {
-        d2 = fmin(fmin(sqn(a0 - b0), sqn(a0 - b1)), fmin(sqn(a1 - b0), sqn(a1 - b1)));
-        f = d2 - th * th;
-    }
-    double dc = sqrt(d2);
-    double gap = eta * f / (dc + th);
-    double tprev = toc;
-    toc = 0;
-    for (int guard = 0; guard < (1 << 20); ++guard) {
-        double lb = (1 - eta) * f / ((dc + th) * maxDisp);
-        a0 = a0 + lb * da0;
-        a1 = a1 + lb * da1;
-        b0 = b0 + lb * db0;
-        b1 = b1 + lb * db1;
-        d2 = d2_ee_unclassified(a0, a1, b0, b1);
-        f = d2 - th * th;
-        if (f <= 0) {
-            d2 = fmin(fmin(sqn(a0 - b0), sqn(a0 - b1)), fmin(sqn(a1 - b0), sqn(a1 - b1)));
-            f = d2 - th * th;
-        }
-        dc = sqrt(d2);
-        if (toc != 0.0 && (f / (dc + th) < gap)) return true;
-        toc += lb;
-        if (toc > tprev) return false;
-    }
-    return true;
-}
-
 // IPC log barrier on squared distance (Library/Math/BARRIER.h:10-62)
 __device__ inline double barrier_b(double d, double dh, double k)
 {

@@ -127,6 +127,12 @@ struct ContactWork {
     DBuf<int4> cs_old;
     size_t cap_pass = 0, cap_dup = 0, cap_cand = 0;
     long ncs = 0;
+    // ACCD step cap per pair and call (<= 0: none, as in CCD.h:308-327); a
+    // capped pair counts as a hit at the time of impact it reached
+    int accd_max_steps = 1 << 20;
+    long last_capped = 0;  // pairs capped in the last CCD call
+    std::vector<unsigned> h_capped;  // ... per instance
+    DBuf<unsigned> icap;
     Profiler* prof = nullptr;
 };
 

@@ -590,7 +590,7 @@ struct PcgArgs {
     int nbl_upd;             // update blocks launched (upd_waves per workgroup)
     const int *apbeg, *bpbeg;
     const unsigned char* solve;  // per instance: 0 not solving, 1 new solve, 2 continued from an earlier call
-    int* flag;        // per instance: 0 running, 1 stopped (converged or maxit), 2 breakdown, 3 not solving
+    int* flag;        // per instance: 0 running, 1 converged, 2 breakdown, 3 not solving, 4 maxit reached
     double* scal;     // per instance, 3 each: |b|^2, final |r|^2, iterations
     double rtol2;
     int maxit;
@@ -916,10 +916,11 @@ __global__ void __launch_bounds__(PCG_ROW_THREADS + (SW ? 64 : 0)) k_pcg_spmv(Pc
             double* sci = A.scal + 3 * inst;
             const double bb = it == 0 ? bbp : sci[0];
             int stop = f != 0 ? 2 : 0;
-            if (!stop && (!(bb > 0.0) || (it > 0 && rr <= A.rtol2 * bb) || it >= A.maxit)) {
+            const bool conv = !(bb > 0.0) || (it > 0 && rr <= A.rtol2 * bb);
+            if (!stop && (conv || it >= A.maxit)) {
                 stop = 1;
                 if (bid == A.apbeg[inst]) {
-                    A.flag[inst] = 1;
+                    A.flag[inst] = conv ? 1 : 4;  // 4: the iteration limit ended the solve
                     sci[0] = bb;
                     sci[1] = rr;
                     sci[2] = (double)it;
@@ -1273,6 +1274,7 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
     std::vector<unsigned char> sv(B, 1);
     if (solve_h) sv.assign(solve_h, solve_h + B);
     L.h_it0.resize(B, 0);
+    L.h_maxit.assign(B, 0);
     for (int i = 0; i < B; ++i)
         if (sv[i] != 2) L.h_it0[i] = 0;
     const unsigned char* solve = L.solve_d.upload_get(sv.data(), B, s);
@@ -1476,6 +1478,7 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
             iters[i] = k;
             relres[i] = hs[3 * i] > 0 ? std::sqrt(hs[3 * i + 1] / hs[3 * i]) : 0.0;
             done_iters += k;
+            if (f == 4) L.h_maxit[i] = 1;
         }
     }
     if (P) {

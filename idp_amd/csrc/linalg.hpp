@@ -21,6 +21,7 @@ struct LinWork {
     std::vector<int4> h_atab, h_btab;
     DBuf<int> aoff, boff;   // per launched block: its instance's iterations in earlier calls
     std::vector<int> h_aoff, h_boff, h_it0;
+    std::vector<char> h_maxit;  // per instance: the last solve stopped at the iteration limit, not converged
     DBuf<unsigned char> solve_d;
     // values
     DBuf<int> erow;  // row of every elastic block

@@ -33,14 +33,18 @@ class TetAnimFix:
         X, _ = self.ctx.get_state()
         self.ctx.set_state(None, (np.asarray(target) - X) / self.h)
 
-    def advance_one_time_step(self, max_newton=0, verbose=0):
+    def advance_one_time_step(self, max_newton=0, verbose=0, body_force=None, flow=False):
+        """Advance_One_Step_IE_Hinge (flow=False) or Advance_One_Step_IE_Flow (flow=True)."""
         it, st = self.ctx.advance(self.h, self.tol, self.dHat2, self.kappa, 0.0, True, self.pcg_rtol,
-                                  max_newton=max_newton, verbose=verbose)
+                                  max_newton=max_newton, verbose=verbose, body_force=body_force, flow=flow)
         self.pn_iters += it
         return it, st
 
     def state(self):
         return self.ctx.get_state()
+
+    def close(self):
+        self.ctx.close()
 
 
 class BatchAnimFix:
@@ -81,3 +85,6 @@ class BatchAnimFix:
         X, v = self.ctx.get_state()
         a, b = self.offsets[i], self.offsets[i + 1]
         return X[a:b], v[a:b]
+
+    def close(self):
+        self.ctx.close()

@@ -203,3 +203,20 @@ def two_blocks_push(n=3, spacing=0.05, gap=0.2, push=0.5):
     tgt[: face_a.size, 0] += push * spacing
     sc = Scene(f"two_blocks{n}_push", X.copy(), T, X.copy())
     return sc, idx, tgt
+
+
+def two_blocks_slide(n=2, spacing=0.05, gap=2e-4, slide=1.0):
+    """Two n^3 Kuhn blocks stacked along y, `gap` cells apart; the search
+    direction moves both blocks down by 0.1 cell (so the swept boxes of the
+    facing surfaces overlap) and slides the upper block by `slide` cells
+    along x.  The facing surfaces stay `gap` apart for the whole motion, so
+    the additive CCD walks each facing pair in steps of about 0.9 gap / slide
+    of the step size.  Returns (scene, direction (V, 3))."""
+    Xa, Ta = kuhn_block(n, n, n, spacing)
+    Xb = Xa + np.array([0.0, (n + gap) * spacing, 0.0])
+    X = np.vstack([Xa, Xb])
+    T = np.vstack([Ta, Ta + Xa.shape[0]]).astype(np.int32)
+    p = np.zeros_like(X)
+    p[:, 1] = -0.1 * spacing
+    p[Xa.shape[0]:, 0] = slide * spacing
+    return Scene(f"two_blocks{n}_slide", X.copy(), T, X.copy()), p

@@ -53,7 +53,14 @@ typedef struct idp_solver_opts {
                            stopped, the others run as many iterations again and then continue in the
                            next Newton iteration while the rest go on; < 0: never.  Results do not
                            depend on it. */
+    int accd_max_steps; /* additive-CCD steps one pair may take in one step-size call before it is
+                           taken as a hit at the time of impact it reached (a smaller, still
+                           conservative step).  0: the default, IDP_ACCD_MAX_STEPS_DEFAULT; < 0: no
+                           cap, as in the reference's loop (CCD.h:308-327, 367-392).  Capped pairs
+                           are counted in idp_step_stats.accd_capped. */
 } idp_solver_opts;
+
+#define IDP_ACCD_MAX_STEPS_DEFAULT (1 << 20)
 
 typedef struct idp_step_stats {
     int newton_iters;
@@ -70,6 +77,13 @@ typedef struct idp_step_stats {
     /* moving DBC: the CCD-limited fraction of the move taken at the start of
        the step (1: all of it), and the penalty stiffness at the end (0: off) */
     double dbc_alpha, dbc_stiff;
+    /* Newton directions taken from a PCG solve that reached pcg_maxit before
+       its residual test held (the reference's direct solve has no such case;
+       the direction is used as it stands) */
+    int pcg_unconverged;
+    /* ACCD pairs stopped by the step cap (accd_max_steps) and taken as hits at
+       the time of impact they had reached */
+    long accd_capped;
 } idp_step_stats;
 
 /* Kernel-family profile: device time from HIP events recorded on the
@@ -104,7 +118,11 @@ int idp_ctx_destroy(idp_ctx* ctx);
 const char* idp_last_error(const idp_ctx* ctx);
 int idp_device_count(int* n);
 /* Profiling (no counterpart in the reference, which only has wall-clock
- * TIMER_FLAG scopes): on != 0 resets the counters and starts recording. */
+ * TIMER_FLAG scopes): on = 1 resets the counters and records every kernel
+ * family (an event pair around each launch, constraint-kind counters in the
+ * barrier Hessian); on = 2 records only the PCG SpMV and update kernels on
+ * one iteration in eight, which leaves the launch stream as it runs
+ * unprofiled; on = 0 stops. */
 int idp_profile_enable(idp_ctx* ctx, int on);
 int idp_profile_read(idp_ctx* ctx, idp_profile* out);
 /* Launch configuration of the PCG kernels (no counterpart in the reference;
@@ -155,15 +173,26 @@ int idp_restore_state(idp_ctx* ctx);
 int idp_surface_sizes(idp_ctx* ctx, int* nb, int* ne, int* nf);
 
 /* ---- the hot path ------------------------------------------------------ */
-/* Advance_One_Step_IE_Discrete_Shell<double,3,false,false> for tet meshes
- * (Library/FEM/Shell/IMPLICIT_EULER.h:152-891, bound as
- * FEM.DiscreteShell.Advance_One_Step_IE_Hinge at DISCRETE_SHELL.h:1110):
+/* Advance_One_Step_IE_Discrete_Shell<double,3,false,false,flow> for tet
+ * meshes (Library/FEM/Shell/IMPLICIT_EULER.h:152-891), bound as
+ * FEM.DiscreteShell.Advance_One_Step_IE_Hinge (flow = 0) and
+ * Advance_One_Step_IE_Flow (flow = 1) at DISCRETE_SHELL.h:1110, 1113:
  * projected Newton with the IPC barrier, ACCD-filtered backtracking line
- * search, kappa doubling and the max-of-last-3 residual stop.  Mutates the
- * context's x, v and kappa3[0] in place (like X, nodeAttr.v and kappaVec).
- * Returns the Newton iteration count (>= 0) or a negative status. */
+ * search, kappa doubling and the max-of-last-3 residual stop.
+ *   b: body force, 3*nV doubles (the reference's `b`), or null for zero; the
+ *      step targets x~ = x + h v + h^2 M^-1 b with the lumped mass
+ *      (IMPLICIT_EULER.h:196-217).
+ *   flow != 0: v is zeroed before x~ is formed, the tet elasticity term is
+ *      left out (INC_POTENTIAL.h:53-73: a tet mesh has no shell flow term),
+ *      and the loop also ends after any iteration that leaves no contact
+ *      (IMPLICIT_EULER.h:850-854).
+ * Mutates the context's x, v and kappa3[0] in place (like X, nodeAttr.v and
+ * kappaVec).  Returns the Newton iteration count (>= 0) or a negative status:
+ * IDP_E_CCD_ZERO when a CCD step size comes out as 0 (the reference exits,
+ * IPC.h:2014-2031), IDP_E_NONPOS_DIST, IDP_E_INVALID_START. */
 int idp_advance_one_step(idp_ctx* ctx, double h, double newton_tol, int with_collision, double dHat2,
-                         double* kappa3, double thickness, const idp_solver_opts* opts, idp_step_stats* stats);
+                         double* kappa3, double thickness, const double* b, int flow, const idp_solver_opts* opts,
+                         idp_step_stats* stats);
 
 /* ---- stage entry points (parity tests and callers with their own loop) --- */
 /* Compute_Constraint_Set (Library/FEM/IPC.h:19-740) at X (host, 3*nV);
@@ -183,9 +212,15 @@ int idp_barrier_hessian_local(idp_ctx* ctx, const double* X, long n, const int* 
 /* Compute_Min_Dist2 (IPC.h:2246-2388) */
 int idp_min_dist2(idp_ctx* ctx, const double* X, long n, const int* cs, double thickness, double* dist2,
                   double* min_dist2);
-/* Compute_Intersection_Free_StepSize (IPC.h:1879-2244) */
+/* Compute_Intersection_Free_StepSize (IPC.h:1879-2244); IDP_E_CCD_ZERO when
+ * the step size comes out as 0 (*alpha_out is still written).  The ACCD step
+ * cap is the context's (idp_set_accd_max_steps, or the last step's
+ * idp_solver_opts.accd_max_steps). */
 int idp_ccd_stepsize(idp_ctx* ctx, const double* X, const double* p, double thickness, double alpha_in,
                      double* alpha_out, long* n_candidates);
+/* ACCD step cap of the context's stage calls: n > 0 steps, 0 the default
+ * (IDP_ACCD_MAX_STEPS_DEFAULT), < 0 none (the reference's loop). */
+int idp_set_accd_max_steps(idp_ctx* ctx, int n);
 /* Neo-Hookean tet terms with weight w = h^2 (INC_POTENTIAL.h:98-109, 228-229, 355-363) */
 int idp_elastic_energy(idp_ctx* ctx, const double* X, double w, double* E, int* valid);
 int idp_elastic_gradient(idp_ctx* ctx, const double* X, double w, double* g);

@@ -525,6 +525,8 @@ extern "C" int orc_min_dist2(const double* X, long n, const int* cs, double thic
 // ---------------------------------------------------------------------------
 namespace {
 
+int g_accd_max_steps = 0;  // orc_set_accd_max_steps; <= 0: no cap
+
 // CCD.h:280-328
 bool accd_pt(V3 p, V3 t0, V3 t1, V3 t2, V3 dp, V3 dt0, V3 dt1, V3 dt2, double eta, double th, double& toc)
 {
@@ -542,6 +544,7 @@ bool accd_pt(V3 p, V3 t0, V3 t1, V3 t2, V3 dp, V3 dt0, V3 dt1, V3 dt2, double et
     double gap = eta * (d2 - th * th) / (dc + th);
     double tprev = toc;
     toc = 0;
+    int steps = 0;
     while (true) {
         double lb = (1 - eta) * (d2 - th * th) / ((dc + th) * maxDisp);
         p = p + lb * dp;
@@ -553,6 +556,7 @@ bool accd_pt(V3 p, V3 t0, V3 t1, V3 t2, V3 dp, V3 dt0, V3 dt1, V3 dt2, double et
         if (toc != 0.0 && ((d2 - th * th) / (dc + th) < gap)) break;
         toc += lb;
         if (toc > tprev) return false;
+        if (++steps >= g_accd_max_steps && g_accd_max_steps > 0) break;
     }
     return true;
 }
@@ -581,6 +585,7 @@ bool accd_ee(V3 a0, V3 a1, V3 b0, V3 b1, V3 da0, V3 da1, V3 db0, V3 db1, double 
     double gap = eta * f / (dc + th);
     double tprev = toc;
     toc = 0;
+    int steps = 0;
     while (true) {
         double lb = (1 - eta) * f / ((dc + th) * maxDisp);
         a0 = a0 + lb * da0;
@@ -597,6 +602,7 @@ bool accd_ee(V3 a0, V3 a1, V3 b0, V3 b1, V3 da0, V3 da1, V3 db0, V3 db1, double 
         if (toc != 0.0 && (f / (dc + th) < gap)) break;
         toc += lb;
         if (toc > tprev) return false;
+        if (++steps >= g_accd_max_steps && g_accd_max_steps > 0) break;
     }
     return true;
 }
@@ -780,6 +786,8 @@ extern "C" int orc_ccd_stepsize(int nV, const double* X, const double* pdir, con
     if (n_cand) *n_cand = ncand;
     return 0;
 }
+
+extern "C" void orc_set_accd_max_steps(int n) { g_accd_max_steps = n; }
 
 // ---------------------------------------------------------------------------
 // Surface primitives (MESHIO.h:450-545 Find_Surface_TriMesh<T,false>, and

@@ -179,6 +179,10 @@ struct Ctx {
     double h, dHat2, thickness;
     double* kappa;
     bool withCollision;
+    // Advance_One_Step_IE_Flow (flow = true): the energy is inertia plus the
+    // barrier, the volumetric term is skipped (INC_POTENTIAL.h:53-73 takes
+    // only the shell flow term, none for a tet mesh)
+    bool flow = false;
     std::vector<double> Xtilde;
     std::vector<int> cs;
     long ncs = 0;
@@ -241,7 +245,7 @@ bool energy(Ctx& c, const double* X, bool withBarrier, double& E)
 {
     double t0 = now();
     double Ee = 0;
-    if (!orc_elastic_energy(X, c.nT, c.tets, c.IB, c.vol, c.lam, c.mu, c.h * c.h, &Ee)) {
+    if (!c.flow && !orc_elastic_energy(X, c.nT, c.tets, c.IB, c.vol, c.lam, c.mu, c.h * c.h, &Ee)) {
         c.t_E += now() - t0;
         return false;
     }
@@ -271,9 +275,11 @@ void assemble_sys(Ctx& c, const double* X, Sys& S)
     const int* tets = c.tets;
     std::vector<Blk> blks;
     blks.reserve(16 * (size_t)nT + nV + 16 * c.ncs);
-    std::vector<double> He(144 * (size_t)nT);
-    orc_elastic_hessian_local(X, nT, tets, c.IB, c.vol, c.lam, c.mu, c.h * c.h, 1, He.data());
-    for (int t = 0; t < nT; ++t) add_local(blks, 4, tets + 4 * t, He.data() + 144 * (size_t)t, nV);
+    if (!c.flow) {
+        std::vector<double> He(144 * (size_t)nT);
+        orc_elastic_hessian_local(X, nT, tets, c.IB, c.vol, c.lam, c.mu, c.h * c.h, 1, He.data());
+        for (int t = 0; t < nT; ++t) add_local(blks, 4, tets + 4 * t, He.data() + 144 * (size_t)t, nV);
+    }
     if (c.withCollision && c.ncs) {
         std::vector<double> Hl(144 * c.ncs);
         std::vector<int> vl(4 * c.ncs);
@@ -320,7 +326,7 @@ extern "C" int orc_advance_one_step(int nV, double* X, const double* X0, double*
                                     const int* bedge, int nf, const int* btri, int nDBC, const int* dbc_idx,
                                     const double* dbc_target, double h, double NewtonTol, int withCollision,
                                     double dHat2, double* kappa, double thickness, double lin_rtol, int max_newton,
-                                    orc_stats* st)
+                                    const double* body_force, int flow, orc_stats* st)
 {
     Ctx c;
     c.nV = nV; c.nT = nT; c.nb = nb; c.ne = ne; c.nf = nf;
@@ -330,12 +336,18 @@ extern "C" int orc_advance_one_step(int nV, double* X, const double* X0, double*
     for (int i = 0; i < nDBC; ++i) c.dbc[dbc_idx[i]] = 1;
     c.h = h; c.dHat2 = dHat2; c.thickness = thickness; c.kappa = kappa; c.withCollision = withCollision != 0;
     c.st = st;
+    c.flow = flow != 0;
     if (st) std::memset(st, 0, sizeof(*st));
     const size_t n3 = 3 * (size_t)nV;
     std::vector<double> Xn(X, X + n3);
     c.Xtilde.resize(n3);
-    // IMPLICIT_EULER.h:196-217 with zero body force: x~ = x + h v
-    for (size_t i = 0; i < n3; ++i) c.Xtilde[i] = X[i] + (h * vel[i] + h * h * 0.0);
+    // IMPLICIT_EULER.h:196-217: a = M^-1 b with the lumped (diagonal) mass,
+    // the flow variant zeroes v, x~ = x + (h v + h^2 a)
+    for (size_t i = 0; i < n3; ++i) {
+        if (c.flow) vel[i] = 0.0;
+        const double a = body_force ? body_force[i] / mass[i / 3] : 0.0;
+        c.Xtilde[i] = X[i] + (h * vel[i] + h * h * a);
+    }
 
     // Dirichlet nodes toward their targets (IMPLICIT_EULER.h:304-401): the
     // CCD-limited fraction of the move; all of it when that is 1, otherwise
@@ -383,7 +395,7 @@ extern "C" int orc_advance_one_step(int nV, double* X, const double* X0, double*
         // gradient (INC_POTENTIAL.h:154-275)
         double t0 = now();
         std::fill(g.begin(), g.end(), 0.0);
-        orc_elastic_gradient(X, nT, tets, IB, vol, lam, mu, h * h, g.data());
+        if (!c.flow) orc_elastic_gradient(X, nT, tets, IB, vol, lam, mu, h * h, g.data());
         for (int v = 0; v < nV; ++v)
             for (int d = 0; d < 3; ++d) g[3 * v + d] += mass[v] * (X[3 * v + d] - c.Xtilde[3 * v + d]);
         if (c.withCollision && c.ncs)
@@ -530,6 +542,8 @@ extern "C" int orc_advance_one_step(int nV, double* X, const double* X0, double*
             st->E = Eprev;
             st->dbc_stiff = c.dbc_stiff;
         }
+        // Advance_One_Step_IE_Flow ends once no contact is left (IMPLICIT_EULER.h:850-854)
+        if (c.flow && (!c.withCollision || c.ncs == 0)) break;
         if (max_newton > 0 && PNIter >= max_newton) break;
     } while (resRecord.size() < 3 || L2Norm > NewtonTol);
 

@@ -54,7 +54,11 @@ int orc_advance_one_step(int nV, double* X, const double* X0, double* vel, const
                          const double* mu, int nb, const int* bnode, int ne, const int* bedge, int nf,
                          const int* btri, int nDBC, const int* dbc_idx, const double* dbc_target, double h,
                          double NewtonTol, int withCollision, double dHat2, double* kappa, double thickness,
-                         double lin_rtol, int max_newton, orc_stats* st);
+                         double lin_rtol, int max_newton, const double* body_force, int flow, orc_stats* st);
+/* ACCD step cap per pair and call for orc_ccd_stepsize and the Newton loop:
+ * <= 0 none (the reference's loop, CCD.h:308-327, 367-392; the default); a
+ * pair that reaches it is taken as a hit at the time of impact it reached. */
+void orc_set_accd_max_steps(int n);
 
 #ifdef __cplusplus
 }

@@ -81,7 +81,9 @@ def lib():
         L.orc_advance_one_step.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int, ip,
                                            C.c_int, ip, C.c_int, ip, C.c_int, ip, C.c_void_p, C.c_double, C.c_double,
                                            C.c_int, C.c_double, dp, C.c_double, C.c_double, C.c_int,
-                                           C.POINTER(Stats)]
+                                           C.c_void_p, C.c_int, C.POINTER(Stats)]
+        L.orc_set_accd_max_steps.argtypes = [C.c_int]
+        L.orc_set_accd_max_steps.restype = None
         L.orc_newton_system.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int, ip, C.c_int,
                                         ip, C.c_int, ip, C.c_int, ip, C.c_double, C.c_double, C.c_double,
                                         C.c_double, ip, ip, dp, dp, C.c_long]
@@ -216,10 +218,12 @@ class Mesh:
             cap = -n
 
     def advance(self, X, vel, h, tol, dHat2, kappa, thickness=0.0, lin_rtol=1e-12, max_newton=0,
-                with_collision=True, dbc_idx=None, dbc_target=None):
-        """One step; dbc_target (n, 3) moves the DBC nodes dbc_idx (None: they stay)."""
+                with_collision=True, dbc_idx=None, dbc_target=None, body_force=None, flow=False):
+        """One step; dbc_target (n, 3) moves the DBC nodes dbc_idx (None: they stay);
+        body_force (nV, 3) or None; flow: the Advance_One_Step_IE_Flow variant."""
         X = _f(X).copy()
         vel = _f(vel).copy()
+        bf = None if body_force is None else _f(body_force).copy()
         kap = np.ascontiguousarray(kappa, dtype=np.float64).copy()
         dbc_idx = np.zeros(0, np.int32) if dbc_idx is None else _i(dbc_idx)
         tgt = None if dbc_target is None else _f(dbc_target)
@@ -229,8 +233,14 @@ class Mesh:
                                         _i(self.bedge), self.btri.shape[0], _i(self.btri), dbc_idx.size, dbc_idx,
                                         None if tgt is None else tgt.ctypes.data_as(C.c_void_p),
                                         h, tol, int(with_collision), dHat2, kap, thickness, lin_rtol, max_newton,
+                                        None if bf is None else bf.ctypes.data_as(C.c_void_p), int(bool(flow)),
                                         C.byref(st))
         return it, X.reshape(-1, 3), vel.reshape(-1, 3), kap, st.as_dict()
+
+
+def set_accd_max_steps(n):
+    """ACCD step cap of every later CCD call of this process (<= 0: none, the default)."""
+    lib().orc_set_accd_max_steps(int(n))
 
 
 def geom_probe(kind, pts):

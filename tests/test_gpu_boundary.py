@@ -1,0 +1,159 @@
+"""The step's boundary on MI355X, through the C ABI, against the oracle:
+body force and the flow variant of idp_advance_one_step (the Hinge and Flow
+bindings, DISCRETE_SHELL.h:1110, 1113; IMPLICIT_EULER.h:196-217, 850-854),
+the ACCD step cap, the zero-step error (IPC.h:2014-2031), and the argument
+checks of the serving entry point.
+
+Bars: same Newton count and end state within 1e-6 of the bounding-box
+diagonal; CCD step sizes bit for bit.
+"""
+import numpy as np
+import pytest
+
+from idp_amd import capi, driver, scenes
+from oracle import pyoracle
+from tests import exact
+from tests.conftest import ORACLE_ACCD_CAP
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _solve_both(sc, body_force, flow, vel=None):
+    vel = sc.velocity if vel is None else vel
+    m = pyoracle.Mesh(sc.X, sc.tets, sc.density, sc.E, sc.nu)
+    d2, k = pyoracle.initialize_oipc(sc.dhat, m.mass)
+    it_o, Xo, vo, ko, st_o = m.advance(sc.X, vel, sc.h, sc.newton_tol, d2, k, body_force=body_force, flow=flow)
+    d = driver.TetAnimFix(sc, pcg_rtol=1e-12)
+    d.ctx.set_state(sc.X, vel)
+    it, st = d.advance_one_time_step(body_force=body_force, flow=flow)
+    X, v = d.state()
+    d.close()
+    return (it_o, Xo, vo, ko, st_o), (it, X, v, d.kappa, st), m
+
+
+def test_gravity_hinge_step_matches_oracle():
+    sc = scenes.small_bend()
+    m0 = pyoracle.Mesh(sc.X, sc.tets, sc.density, sc.E, sc.nu)
+    b = m0.mass[:, None] * np.array([0.0, -9.81, 0.0])[None, :]
+    say("[boundary] hinge step with gravity")
+    (it_o, Xo, vo, _, st_o), (it, X, v, _, st), m = _solve_both(sc, b, False)
+    err = np.abs(X - Xo).max() / sc.diag
+    say(f"[boundary] gravity: gpu newton {it} oracle {it_o}, err/diag {err:.2e}")
+    assert it == it_o
+    assert err < 1e-6
+    assert not exact.surface_self_intersects(X, m.bedge, m.btri)
+    # the force moved the result: compare with the zero-force step
+    d = driver.TetAnimFix(sc, pcg_rtol=1e-12)
+    d.advance_one_time_step()
+    X0, _ = d.state()
+    d.close()
+    assert np.abs(X - X0).max() > 1e-3 * sc.h ** 2 * 9.81
+
+
+def test_flow_step_matches_oracle():
+    """Flow variant pulled into self-contact: the body force makes x~ the
+    bend target (v is zeroed), the tet term is off, contacts hold it back."""
+    sc = scenes.small_bend()
+    m0 = pyoracle.Mesh(sc.X, sc.tets, sc.density, sc.E, sc.nu)
+    b = m0.mass[:, None] * (sc.target - sc.X) / sc.h ** 2
+    say("[boundary] flow step toward the bend target")
+    (it_o, Xo, vo, _, st_o), (it, X, v, _, st), m = _solve_both(sc, b, True)
+    err = np.abs(X - Xo).max() / sc.diag
+    say(f"[boundary] flow: gpu newton {it} oracle {it_o}, constraints {st['n_constraints']}, err/diag {err:.2e}")
+    assert it == it_o
+    assert err < 1e-6
+    assert st["n_constraints"] > 0
+    assert not exact.surface_self_intersects(X, m.bedge, m.btri)
+    assert np.abs(v - (X - sc.X) / sc.h).max() <= 1e-9 * np.abs(v).max()
+
+
+def test_flow_without_contact_is_one_newton_step_to_xtilde():
+    X, T = scenes.kuhn_block(2, 2, 2, 0.05)
+    sc = scenes.Scene("block", X.copy(), T.astype(np.int32), X.copy())
+    m0 = pyoracle.Mesh(sc.X, sc.tets, sc.density, sc.E, sc.nu)
+    acc = np.zeros_like(sc.X)
+    acc[:, 0] = 3.0 * sc.X[:, 1] / sc.h ** 2
+    b = m0.mass[:, None] * acc
+    say("[boundary] flow step without contact")
+    (it_o, Xo, _, _, _), (it, Xg, vg, _, st), _ = _solve_both(sc, b, True, vel=np.full_like(sc.X, 0.7))
+    want = sc.X + sc.h ** 2 * acc
+    assert it == it_o == 1
+    assert st["n_constraints"] == 0
+    assert np.abs(Xg - want).max() <= 1e-12 * np.abs(want).max()
+
+
+def test_accd_step_cap_matches_oracle_bit_for_bit():
+    sc, p = scenes.two_blocks_slide()
+    m = pyoracle.Mesh(sc.X, sc.tets, sc.density, sc.E, sc.nu)
+    vol, IB, lam, mu, mass = capi.fem_init(sc.X, sc.tets, sc.density, sc.E, sc.nu)
+    ctx = capi.Context(0)
+    try:
+        ctx.set_mesh(sc.X, sc.tets, IB, vol, lam, mu, mass)
+        out = {}
+        for cap in (200, 1000, -1):
+            say(f"[boundary] ACCD cap {cap}")
+            ctx.set_accd_max_steps(cap)
+            pyoracle.set_accd_max_steps(max(cap, 0))
+            ag, _ = ctx.ccd(sc.X.ravel(), p.ravel(), 1.0)
+            ao, _ = m.ccd(sc.X, p, 1.0)
+            assert ag == ao
+            out[cap] = ag
+        assert 0 < out[200] < out[1000] < out[-1] <= 1.0
+    finally:
+        pyoracle.set_accd_max_steps(ORACLE_ACCD_CAP)
+        ctx.close()
+
+
+def test_capped_pairs_are_counted_in_the_step_stats():
+    # gap above dhat: no contact at the start, so the first Newton direction
+    # is the slide itself (two rigid translations); each facing pair then
+    # needs about 1 / (0.9 * 0.01) = 111 additive steps
+    sc, p = scenes.two_blocks_slide(gap=0.01)
+    assert 0.01 * 0.05 > sc.dhat
+    sc.target = sc.X + p
+    d = driver.TetAnimFix(sc, pcg_rtol=1e-10)
+    say("[boundary] capped pairs counted")
+    it, st = d.ctx.advance(sc.h, sc.newton_tol, d.dHat2, d.kappa, max_newton=1, accd_max_steps=50)
+    assert st["accd_capped"] > 0
+    d.close()
+    d = driver.TetAnimFix(sc, pcg_rtol=1e-10)
+    it, st = d.ctx.advance(sc.h, sc.newton_tol, d.dHat2, d.kappa, max_newton=1, accd_max_steps=-1)
+    assert st["accd_capped"] == 0
+    d.close()
+
+
+def test_zero_ccd_step_is_reported():
+    sc = scenes.small_bend()
+    vol, IB, lam, mu, mass = capi.fem_init(sc.X, sc.tets, sc.density, sc.E, sc.nu)
+    with capi.Context(0) as ctx:
+        ctx.set_mesh(sc.X, sc.tets, IB, vol, lam, mu, mass)
+        say("[boundary] zero CCD step")
+        with pytest.raises(capi.IdpError) as e:
+            ctx.ccd(sc.X.ravel(), (sc.target - sc.X).ravel(), 0.0)
+        assert e.value.code == capi.IDP_E_CCD_ZERO
+
+
+def test_serve_start_rejects_duplicate_and_busy_slots():
+    scs = [scenes.small_bend(), scenes.small_bend(angle_deg=200.0)]
+    b = driver.BatchAnimFix(scs)
+    ctx = b.ctx
+    ctx.serve_configure(b.h)
+    X = [sc.X for sc in scs]
+    V = [sc.velocity for sc in scs]
+    say("[boundary] serve_start argument checks")
+    with pytest.raises(capi.IdpError):
+        ctx.serve_start([0, 0], [X[0], X[0]], [V[0], V[0]], b.tol[[0, 0]], b.dHat2[[0, 0]], b.kappa[[0, 0]])
+    ctx.serve_start([0], [X[0]], [V[0]], b.tol[[0]], b.dHat2[[0]], b.kappa[[0]])
+    with pytest.raises(capi.IdpError):
+        ctx.serve_start([0], [X[0]], [V[0]], b.tol[[0]], b.dHat2[[0]], b.kappa[[0]])  # busy
+    n = 1
+    for _ in range(2000):
+        n, fin = ctx.serve_iterate()
+        if n == 0:
+            break
+    assert n == 0
+    b.close()
