@@ -561,7 +561,9 @@ def bench_plain(a):
     solve(2)
     est = None
     for w in range(a.warmup):
-        if est is not None and elapsed() + 2 * est + reserve > a.budget_s:
+        # timed solves come first: a further warm-up runs only while up to
+        # three timed solves would still fit after it
+        if est is not None and elapsed() + est * (1 + min(a.steps, 3)) + reserve > a.budget_s:
             log(f"[bench] warm-up stopped after {w}: budget")
             break
         it, st, t = solve(a.max_newton)
