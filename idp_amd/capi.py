@@ -45,7 +45,7 @@ EXPORTED = [
     "idp_elastic_energy", "idp_elastic_gradient", "idp_elastic_hessian_local", "idp_newton_direction", "idp_get_bsr",
     "idp_profile_enable", "idp_profile_read", "idp_set_mesh_batch", "idp_batch_size", "idp_advance_batch",
     "idp_serve_configure", "idp_serve_start", "idp_serve_iterate", "idp_serve_result", "idp_set_launch_config",
-    "idp_get_launch_config", "idp_set_accd_max_steps",
+    "idp_get_launch_config", "idp_set_accd_max_steps", "idp_barrier_energy_noalloc",
 ]
 
 # PCG preconditioner for every solve of this process: 0 cluster block Jacobi
@@ -179,6 +179,7 @@ def lib():
         L.idp_get_constraints.argtypes = [vp, ip, C.c_long]
         L.idp_barrier_energy.argtypes = [vp, dp, C.c_long, ip, C.c_double, C.c_double, C.c_double, dp]
         L.idp_barrier_gradient.argtypes = [vp, dp, C.c_long, ip, C.c_double, C.c_double, C.c_double, dp]
+        L.idp_barrier_energy_noalloc.argtypes = [vp, vp, C.c_long, vp, C.c_double, C.c_double, C.c_double, dp]
         L.idp_barrier_hessian_local.argtypes = [vp, dp, C.c_long, ip, C.c_double, C.c_double, C.c_double, C.c_int, dp,
                                                 ip]
         L.idp_min_dist2.argtypes = [vp, dp, C.c_long, ip, C.c_double, dp, dp]
@@ -505,6 +506,14 @@ class Context:
         cs, csp = _i(cs)
         E = C.c_double()
         self._check(lib().idp_barrier_energy(self._h, Xp, cs.size // 4, csp, dHat2, kappa, thickness, C.byref(E)))
+        return E.value
+
+    def barrier_energy_noalloc(self, X_dev, cs_dev, n, dHat2, kappa, thickness=0.0):
+        """idp_barrier_energy_noalloc: X_dev and cs_dev are device addresses
+        (ints) of 3*nV doubles and 4*n ints."""
+        E = C.c_double()
+        self._check(lib().idp_barrier_energy_noalloc(self._h, C.c_void_p(X_dev), int(n), C.c_void_p(cs_dev), dHat2,
+                                                     kappa, thickness, C.byref(E)))
         return E.value
 
     def barrier_gradient(self, X, cs, dHat2, kappa, thickness=0.0):

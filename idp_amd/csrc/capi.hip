@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -59,6 +60,9 @@ struct idp_ctx {
     // energy, gradient and Hessian (INC_POTENTIAL.h:53-73, 191-211, 323-339)
     bool no_elastic = false;
     DBuf<double> bforce;  // body force of the current step (3 nV)
+    // this device's address of the fixed scratch array of the allocation-free
+    // entry points (resolved once per device when the first context is created)
+    unsigned char* scratch = nullptr;
 };
 
 namespace {
@@ -1086,6 +1090,78 @@ int setup_mesh(idp_ctx* c, int nV, const double* X_rest, int nT, const int* tets
 // ===========================================================================
 // C ABI
 // ===========================================================================
+// ---------------------------------------------------------------------------
+// allocation-free barrier energy: a fixed scratch array per device
+// ---------------------------------------------------------------------------
+// Layout (byte offsets): block partials, the three barrier parameters, the
+// result (E and the d2 <= 0 flag, adjacent so that one copy returns both),
+// then the constraint block bounds.  The block bounds are those of the
+// one-instance constraint table (blocks of RB constraints), so the energy is
+// summed in the same order as by idp_barrier_energy.
+namespace {
+constexpr long NA_MAX_N = IDP_NOALLOC_MAX_CONSTRAINTS;
+constexpr int NA_MAX_BLK = (int)((NA_MAX_N + RB - 1) / RB);
+constexpr size_t NA_PART = 0;
+constexpr size_t NA_PAR = NA_PART + 8 * (size_t)NA_MAX_BLK;
+constexpr size_t NA_OUT = NA_PAR + 8 * 4;
+constexpr size_t NA_LO = NA_OUT + 16;
+constexpr size_t NA_HI = NA_LO + 4 * (size_t)NA_MAX_BLK;
+constexpr size_t NA_BYTES = NA_HI + 4 * (size_t)NA_MAX_BLK;
+struct NaOut {
+    double E;
+    int flag;
+    int pad;
+};
+static_assert(NA_OUT % 8 == 0 && NA_LO % 4 == 0, "scratch layout alignment");
+}  // namespace
+
+// Each GPU the library runs on holds one copy until the process ends.  Host
+// code never passes this name to a kernel or copy: it uses the address that
+// hipGetSymbolAddress returned for the device (idp_ctx::scratch).
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+namespace {
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex na_mu;
+unsigned char* na_addr[NA_MAX_DEVICES] = {};
+
+// this device's scratch address; `device` must be the current device
+int scratch_address(int device, unsigned char** out)
+{
+    if (device < 0 || device >= NA_MAX_DEVICES) return IDP_E_INVALID;
+    std::lock_guard<std::mutex> lk(na_mu);
+    if (!na_addr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return IDP_E_HIP;
+        na_addr[device] = (unsigned char*)p;
+    }
+    *out = na_addr[device];
+    return IDP_OK;
+}
+
+// block bounds of n constraints in blocks of RB, the barrier parameters, and
+// a cleared result
+__global__ void k_noalloc_setup(long n, int nblk, int* __restrict__ lo, int* __restrict__ hi,
+                                double* __restrict__ par, double dh, double kappa, double th2,
+                                NaOut* __restrict__ out)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nblk) {
+        lo[i] = (int)(i * RB);
+        hi[i] = (int)std::min<long>((i + 1) * RB, n);
+    }
+    if (i == 0) {
+        par[0] = dh;
+        par[1] = kappa;
+        par[2] = th2;
+        out->E = 0.0;
+        out->flag = 0;
+        out->pad = 0;
+    }
+}
+}  // namespace
+
 extern "C" {
 
 int idp_device_count(int* n)
@@ -1109,6 +1185,8 @@ int idp_ctx_create(int device, idp_ctx** out)
         IDP_HIP(hipSetDevice(device));
         IDP_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
         barrier_init_constants();
+        if (scratch_address(device, &c->scratch) != IDP_OK)
+            throw HipError(IDP_E_HIP, "hipGetSymbolAddress of the scratch array failed");
         c->prof.s = c->s;
         c->cw.prof = &c->prof;
         c->L.prof = &c->prof;
@@ -1775,6 +1853,44 @@ int idp_barrier_energy(idp_ctx* c, const double* X, long n, const int* cs, doubl
         IDP_HIP(hipStreamSynchronize(c->s));
         if (hf) return IDP_E_NONPOS_DIST;
         *E = n ? dev_sum_partials(c->L, c->s, nb, pb) : 0.0;
+        return IDP_OK;
+    });
+}
+
+int idp_barrier_energy_noalloc(idp_ctx* c, const double* X_dev, long n, const int* cs_dev, double dHat2,
+                               double kappa, double thickness, double* E)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh || c->m.inst.B != 1 || !E || n < 0 || (n > 0 && (!X_dev || !cs_dev)) || !c->scratch)
+            return IDP_E_INVALID;
+        if (n > NA_MAX_N) {
+            c->err = "idp_barrier_energy_noalloc: more constraints than IDP_NOALLOC_MAX_CONSTRAINTS";
+            return IDP_E_CAPACITY;
+        }
+        auto s = c->s;
+        IDP_HIP(hipSetDevice(c->device));
+        unsigned char* base = c->scratch;
+        double* part = (double*)(base + NA_PART);
+        double* par = (double*)(base + NA_PAR);
+        NaOut* out = (NaOut*)(base + NA_OUT);
+        int* lo = (int*)(base + NA_LO);
+        int* hi = (int*)(base + NA_HI);
+        const int nblk = (int)((n + RB - 1) / RB);
+        // the parameters exactly as set_bparams forms them
+        const double dh = dHat2 + 2 * std::sqrt(dHat2) * thickness;
+        const double th2 = thickness * thickness;
+        IDP_LAUNCH(k_noalloc_setup, grid_for(std::max(nblk, 1), 256), 256, 0, s, n, nblk, lo, hi, par, dh, kappa,
+                   th2, out);
+        if (n) {
+            launch_barrier_energy(s, nblk, lo, hi, cs_dev, X_dev, c->m.X0.p,
+                                  BarrierParams{par, par + 1, par + 2, nullptr}, part, &out->flag);
+            launch_sum_partials(s, nblk, part, &out->E);
+        }
+        NaOut h{};
+        IDP_HIP(hipMemcpyAsync(&h, out, sizeof(NaOut), hipMemcpyDeviceToHost, s));
+        IDP_HIP(hipStreamSynchronize(s));
+        if (h.flag) return IDP_E_NONPOS_DIST;
+        *E = h.E;
         return IDP_OK;
     });
 }

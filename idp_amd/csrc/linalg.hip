@@ -1522,6 +1522,11 @@ void dev_velocity(hipStream_t s, long n, const double* x, const double* xn, doub
     IDP_LAUNCH(k_velocity, grid_for(n, 256), 256, 0, s, n, x, xn, h, v);
 }
 
+void launch_sum_partials(hipStream_t s, int npart, const double* part, double* out)
+{
+    IDP_LAUNCH(k_sum_partials, 1, RB, 0, s, npart, part, out);
+}
+
 double dev_sum_partials(LinWork& L, hipStream_t s, int npart, const double* part)
 {
     double* o = L.scal2.ensure(2, s);

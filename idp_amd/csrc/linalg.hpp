@@ -60,6 +60,8 @@ double dev_dot(LinWork& L, hipStream_t s, long n, const double* a, const double*
 void dev_velocity(hipStream_t s, long n, const double* x, const double* xn, double h, double* v);
 void dev_axpy(hipStream_t s, long n, const double* a, double sc, const double* b, double* x);
 double dev_sum_partials(LinWork& L, hipStream_t s, int npart, const double* part);
+// out[0] = sum of part[0, npart) in the fixed order of dev_sum_partials (device result, no copy)
+void launch_sum_partials(hipStream_t s, int npart, const double* part, double* out);
 void launch_tet_hess_dense(hipStream_t s, const MeshDev& m, const double* X, double w, int project, double* Hl);
 
 }  // namespace idp

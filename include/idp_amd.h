@@ -202,6 +202,19 @@ int idp_get_constraints(idp_ctx* ctx, int* cs, long cap);
 /* Compute_Barrier (IPC.h:742-941): E = sum of barrier terms (not +=). */
 int idp_barrier_energy(idp_ctx* ctx, const double* X, long n, const int* cs, double dHat2, double kappa,
                        double thickness, double* E);
+/* Allocation-free variant of idp_barrier_energy.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).
+ * X_dev (3*nV doubles) and cs_dev (4*n ints) are DEVICE pointers; *E is the
+ * same sum, bit for bit, as idp_barrier_energy's on the same input.  The
+ * call makes no device allocation: its partial sums, parameters and block
+ * table live in one fixed scratch array per GPU (about 1 MB, held until the
+ * process ends), so n is limited to IDP_NOALLOC_MAX_CONSTRAINTS
+ * (IDP_E_CAPACITY beyond it).  Calls that overlap on one GPU, from any
+ * context, share that array and race.  Single-mesh contexts only. */
+#define IDP_NOALLOC_MAX_CONSTRAINTS (1L << 24)
+int idp_barrier_energy_noalloc(idp_ctx* ctx, const double* X_dev, long n, const int* cs_dev, double dHat2,
+                               double kappa, double thickness, double* E);
 /* Compute_Barrier_Gradient (IPC.h:943-1256): g (3*nV) is overwritten. */
 int idp_barrier_gradient(idp_ctx* ctx, const double* X, long n, const int* cs, double dHat2, double kappa,
                          double thickness, double* g);
