@@ -48,6 +48,8 @@ def workload(name):
         return scenes.config2()
     if name == "bar_bend":
         return scenes.config1()
+    if name == "bar_crumple":
+        return scenes.config5()
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -280,111 +282,121 @@ def bench_batch(a):
     print(json.dumps(out), flush=True)
 
 
-def serve_sequences(frames, total=200):
-    """Animation-fix sequences: bars of 8x4x4 (6), 32x8x8 (4) and 64x16x16 (2)
-    cells whose target bends from 0 to 270 degrees over a `total`-frame
-    sequence (bend radius 0.15 of the bar length); the first `frames` frames
-    of each are requested.  Returns [(rest scene, [target of frame 1..frames])]."""
+def serve_sequences(frames, total=200, seed=4, count=12):
+    """Animation-fix sequences: `count` bars whose size is drawn (seeded) from
+    {8x4x4, 32x8x8, 64x16x16} cells, each with a target that bends it from 0
+    to 270 degrees over a `total`-frame sequence (bend radius 0.15 of the bar
+    length); the first `frames` frames of each are requested.  Returns
+    (kinds {name: rest scene}, [(kind, [target of frame 1..frames])])."""
     from idp_amd import scenes
 
-    out = []
-    for nx, ny, nz, count in ((8, 4, 4, 6), (32, 8, 8, 4), (64, 16, 16, 2)):
-        for _ in range(count):
+    sizes = ((8, 4, 4), (32, 8, 8), (64, 16, 16))
+    rng = np.random.default_rng(seed)
+    draw = rng.integers(0, len(sizes), count)
+    draw[: len(sizes)] = np.arange(len(sizes))  # every size at least once
+    kinds, seqs, cache = {}, [], {}
+    for d in draw:
+        nx, ny, nz = sizes[d]
+        name = f"bar{nx}x{ny}x{nz}"
+        if name not in cache:
             L, W = nx * 0.05, ny * 0.05
             mid = np.array([L / 2, W / 2, 0.0])  # the bend map is centred on the bar's middle
-            tg = [scenes.bar_bend(nx, ny, nz, 0.05, 0.15 * L, 270.0 * k / total).target + mid
-                  for k in range(1, frames + 1)]
-            out.append((scenes.bar_bend(nx, ny, nz, 0.05, 0.15 * L, 0.0), tg))
-    return out
+            kinds[name] = scenes.bar_bend(nx, ny, nz, 0.05, 0.15 * L, 0.0)
+            cache[name] = [scenes.bar_bend(nx, ny, nz, 0.05, 0.15 * L, 270.0 * k / total).target + mid
+                           for k in range(1, frames + 1)]
+        seqs.append((name, cache[name]))
+    return kinds, seqs
 
 
 def bench_serve(a):
-    """Continuous batching of animation-fix frames arriving as a Poisson stream:
-    one slot per sequence (the sequence's mesh stays on the device and every
-    frame warm-starts from the previous frame's solution); a frame starts as
-    soon as it has arrived and its predecessor has finished, while the other
-    slots are mid-solve.  The plain path serves the same stream one frame at a
+    """Continuous batching of animation-fix frames arriving as a Poisson
+    stream, admitted by a device-memory budget (idp_amd.serve.FrameServer):
+    slots per bar size are planned from the measured per-slot footprint, a
+    frame waits until its predecessor in the sequence has finished and a slot
+    of its size is free, and every frame warm-starts from the previous
+    frame's solution.  The plain path serves the same stream one frame at a
     time (FIFO, service times measured with the one-mesh path)."""
-    from idp_amd import capi, driver
+    from idp_amd import capi, driver, serve
 
-    seqs = serve_sequences(a.frames)
+    kinds, seqs = serve_sequences(a.frames)
     S, F = len(seqs), a.frames
-    rest = [sc for sc, _ in seqs]
-    # admission by HBM budget: every sequence gets a slot when its estimated
-    # device footprint fits in the budget (all of them on one MI355X)
-    per_slot = [2000.0 * sc.X.shape[0] + 3000.0 * sc.tets.shape[0] for sc in rest]  # bytes, generous
-    budget = 64e9
-    assert sum(per_slot) < budget
-    b = driver.BatchAnimFix(rest, pcg_rtol=a.pcg_rtol)
-    ctx = b.ctx
-    ctx.serve_configure(b.h, pcg_rtol=a.pcg_rtol)
-    k_init = b.kappa.copy()
-    h = b.h
+    demand = {k: sum(1 for q, _ in seqs if q == k) for k in kinds}
+    budget = int(a.hbm_budget_gb * 1e9)
+    srv = serve.FrameServer(kinds, budget, demand=demand, pcg_rtol=a.pcg_rtol)
+    log(f"[bench] serve: {S} sequences {demand}, footprints {srv.footprint}, budget {budget} B -> slots "
+        f"{srv.slots_per_kind}, resident {srv.resident_bytes} B")
+    h = srv.batch.h
+    k_init = {k: srv.batch.kappa[srv.slot_kind.index(k)].copy() for k in kinds}
 
     def run(rate, seed, frames=F):
         rng = np.random.default_rng(seed)
         order = [(s, k) for k in range(frames) for s in range(S)]
         arrive = np.cumsum(rng.exponential(1.0 / rate, len(order)))
         arr = {o: t for o, t in zip(order, arrive)}
-        state = [rest[s].X.copy() for s in range(S)]
-        kap = k_init.copy()
-        nxt = [0] * S          # next frame per sequence
+        state = [kinds[seqs[s][0]].X.copy() for s in range(S)]
+        kap = [k_init[seqs[s][0]].copy() for s in range(S)]
+        nxt = [0] * S
         busy = [False] * S
-        lat, t0 = [], time.perf_counter()
+        lat, waits, t0 = [], [], time.perf_counter()
         done, last = 0, 0.0
         while done < S * frames:
             now = time.perf_counter() - t0
             if now - last > 30.0:
                 log(f"[bench] serve rate {rate}: {done} of {S * frames} frames after {now:.0f} s")
                 last = now
-            ready = [s for s in range(S) if not busy[s] and nxt[s] < frames and arr[(s, nxt[s])] <= now]
-            if ready:
-                X = [state[s] for s in ready]
-                V = [(seqs[s][1][nxt[s]] - state[s]) / h for s in ready]
-                ctx.serve_start(ready, X, V, b.tol[ready], b.dHat2[ready], kap[ready])
-                for s in ready:
+            for s in range(S):
+                if not busy[s] and nxt[s] < frames and arr[(s, nxt[s])] <= now:
+                    q = seqs[s][0]
+                    srv.submit((s, nxt[s]), q, state[s], (seqs[s][1][nxt[s]] - state[s]) / h, kap[s])
                     busy[s] = True
-            if any(busy):
-                n, fin = ctx.serve_iterate()
-                tf = time.perf_counter() - t0
-                for s in np.nonzero(fin)[0]:
-                    Xs, _, k3, _ = ctx.serve_result(int(s))
-                    state[s] = Xs
-                    kap[s, 0] = k3[0]
-                    lat.append(tf - arr[(int(s), nxt[s])])
+            if srv.pending():
+                for d in srv.iterate():
+                    s, k = d.tag
+                    state[s] = d.X
+                    kap[s][0] = d.kappa3[0]
+                    lat.append(time.perf_counter() - t0 - arr[(s, k)])
+                    waits.append(d.waited_iters)
                     nxt[s] += 1
                     busy[s] = False
                     done += 1
             else:
-                waits = [arr[(s, nxt[s])] for s in range(S) if nxt[s] < frames]
-                time.sleep(max(0.0, min(waits) - (time.perf_counter() - t0)))
+                pend = [arr[(s, nxt[s])] for s in range(S) if nxt[s] < frames]
+                time.sleep(max(0.0, min(pend) - (time.perf_counter() - t0)))
         span = time.perf_counter() - t0
-        return S * frames / span, np.array(lat) * 1e3
+        return S * frames / span, np.array(lat) * 1e3, np.array(waits)
 
-    # warm-up pass (device buffers reach their working sizes): the first
-    # frame of every sequence, from the rest state again afterwards
+    # warm-up pass (buffers reach their working sizes): the first frame of
+    # every sequence, from the rest state again afterwards
     run(1e6, 0, frames=1)
     results = {}
     for rate in [float(r) for r in a.rates.split(",")]:
-        thr, lat = run(rate, 1)
+        thr, lat, waits = run(rate, 1)
         results[rate] = {"solves_per_s": thr, "p50_ms": float(np.percentile(lat, 50)),
-                         "p99_ms": float(np.percentile(lat, 99))}
+                         "p99_ms": float(np.percentile(lat, 99)), "frames_that_waited_for_a_slot":
+                         int((waits > 0).sum()), "max_wait_newton_iters": int(waits.max())}
         log(f"[bench] serve rate {rate}: {thr:.2f} solves/s, p50 {results[rate]['p50_ms']:.1f} ms, "
-            f"p99 {results[rate]['p99_ms']:.1f} ms")
+            f"p99 {results[rate]['p99_ms']:.1f} ms, {results[rate]['frames_that_waited_for_a_slot']} frames "
+            f"waited for a slot")
     # plain path: one frame at a time, FIFO over the same arrival stream
     svc = {}
+    plain_ctx = {}
     for s in range(S):
-        d = driver.TetAnimFix(rest[s], pcg_rtol=a.pcg_rtol)
-        d.advance_one_time_step(max_newton=1)  # buffer priming
-        d.ctx.set_state(rest[s].X, np.zeros_like(rest[s].X))
-        d.kappa[:] = k_init[s]
-        X = rest[s].X
+        q = seqs[s][0]
+        if q not in plain_ctx:
+            d = driver.TetAnimFix(kinds[q], pcg_rtol=a.pcg_rtol)
+            d.advance_one_time_step(max_newton=1)  # buffer priming
+            plain_ctx[q] = d
+        d = plain_ctx[q]
+        d.kappa[:] = k_init[q]
+        X = kinds[q].X
         for k in range(F):
             d.ctx.set_state(X, (seqs[s][1][k] - X) / h)
             t0 = time.perf_counter()
             d.advance_one_time_step()
             svc[(s, k)] = time.perf_counter() - t0
             X, _ = d.state()
+    for d in plain_ctx.values():
+        d.close()
     plain = {}
     for rate in results:
         rng = np.random.default_rng(1)
@@ -404,20 +416,22 @@ def bench_serve(a):
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": S * F,
-        "warmup": S * F,
+        "warmup": S,
         "ms_per_step": 1e3 / results[top]["solves_per_s"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "serve_bar_frames", "sequences": S, "frames": F, "slots": S,
-                   "rates": list(results), "pcg_rtol": a.pcg_rtol},
+        "config": {"workload": "serve_bar_frames", "sequences": S, "frames": F, "kinds": demand,
+                   "slots": srv.slots_per_kind, "slot_footprint_bytes": srv.footprint, "hbm_budget_bytes": budget,
+                   "resident_bytes": srv.resident_bytes, "rates": list(results), "pcg_rtol": a.pcg_rtol},
         "latency_ms": {"p50": results[top]["p50_ms"], "p99": results[top]["p99_ms"]},
         "served": results,
         "plain_path": plain,
         "served_vs_plain": results[top]["solves_per_s"] / plain[top]["solves_per_s"],
     }
+    srv.close()
     print(json.dumps(out), flush=True)
 
 
@@ -427,10 +441,13 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="bar_twist_fold",
-                    choices=["bar_twist_fold", "bar_bend", "cubes256", "serve"])
-    ap.add_argument("--frames", type=int, default=20,
+                    choices=["bar_twist_fold", "bar_bend", "bar_crumple", "cubes256", "serve"],
+                    help="bar_crumple (160x48x48, 2.2M tets) is a measurement workload: use it with --max-newton")
+    ap.add_argument("--frames", type=int, default=200,
                     help="serve: frames requested per sequence (the first of a 200-frame bend to 270 degrees)")
-    ap.add_argument("--rates", default="5,20,100,200,1000,5000", help="serve: Poisson arrival rates (req/s)")
+    ap.add_argument("--hbm-budget-gb", type=float, default=2.0,
+                    help="serve: device memory the slots may hold (slots per bar size are planned from it)")
+    ap.add_argument("--rates", default="20,200", help="serve: Poisson arrival rates (req/s)")
     ap.add_argument("--plain-sample", type=int, default=16,
                     help="cubes256: instances solved one at a time for the plain-path comparison")
     ap.add_argument("--pcg-rtol", type=float, default=1e-8)
@@ -614,6 +631,17 @@ def bench_plain(a):
     kern["barrier_hess"]["clamped"] = int(sum(prof["barrier_clamped"]))
     kern["ccd"]["steps"] = prof["ccd_steps"]
     kern["ccd"]["max_steps"] = prof["ccd_max_steps"]
+    # what a graph of the PCG iterations could remove at most: the PCG stage's
+    # device time between its kernels (launch gaps, chunk-boundary waits and
+    # the preconditioner build), from the sampled per-launch kernel times
+    sp, up = prof["pcg_spmv"], prof["pcg_update"]
+    if sp["launches"] and up["launches"] and prof["pcg"]["ms"] > 0:
+        per_it = sp["ms"] / sp["launches"] + up["ms"] / up["launches"]
+        kms = per_it * st_p["pcg_iters"]
+        kern["pcg"]["iterations"] = int(st_p["pcg_iters"])
+        kern["pcg"]["kernel_ms_est"] = kms
+        kern["pcg"]["outside_kernels_ms"] = prof["pcg"]["ms"] - kms
+        kern["pcg"]["outside_kernels_frac"] = (prof["pcg"]["ms"] - kms) / prof["pcg"]["ms"]
     extra["kernels_profiled_solve"] = {"newton_iters": it_p, "seconds": t_p, "families": kern}
     extra["barrier_hessian_pair_evals_per_s"] = (pairs / (bh["ms"] * 1e-3)) if bh["ms"] > 0 else None
     log(f"[bench] profiled solve: {it_p} Newton iterations in {t_p:.2f} s")

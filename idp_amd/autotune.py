@@ -1,6 +1,7 @@
-"""Autotuner for the PCG launch configuration on this device.
+"""Autotuner for the launch configuration on this device.
 
-The PCG runs two kernels per iteration.  Two launch parameters are tuned:
+The PCG runs two kernels per iteration.  Two of its launch parameters are
+tuned:
 - the SpMV shape (index into the library's table: G = 4, 8, 16, 32 lanes per
   block row, with the scalar reduction in wavefront 0 for 0..3 or in a fifth
   wavefront for 4..7);
@@ -19,6 +20,12 @@ arithmetic alone and in any batch.
 
 writes the choice and every measurement as JSON; ``capi.Context`` applies it
 to each new context (``IDP_TUNING`` names another file).
+
+The barrier Hessian's projection stage takes one constraint per team of 8,
+16, 32 or 64 lanes; its time per launch is measured for each width at the
+chosen PCG configuration and the fastest (summed over the workloads) is
+written as ``barrier_team``.  No sum depends on the width, so the results do
+not either.
 """
 from __future__ import annotations
 
@@ -32,6 +39,8 @@ from . import capi, driver, scenes
 SHAPE_TABLE = [(4, False), (8, False), (16, False), (32, False), (4, True), (8, True), (16, True), (32, True)]
 UPD_WAVES = [1, 2, 4, 8]
 DEFAULT = (5, 4)
+TEAMS = [8, 16, 32, 64]
+DEFAULT_TEAM = 16
 
 
 def _workload(name):
@@ -62,6 +71,44 @@ def measure(kind, sc, spmv_shape, upd_waves, newton):
         "update_us": 1e3 * up["ms"] / max(up["launches"], 1),
         "samples": int(sp["launches"]),
     }
+
+
+def measure_team(kind, sc, team, newton, config):
+    """Barrier-Hessian kernel time per launch (us, both stages and the kind
+    grouping) over the first `newton` Newton iterations, at `team` lanes per
+    constraint and the PCG configuration `config`."""
+    d = driver.TetAnimFix(sc) if kind == "one" else driver.BatchAnimFix(sc)
+    d.ctx.set_launch_config(*config)
+    d.ctx.set_barrier_team(team)
+    d.ctx.profile(True)
+    d.advance_one_time_step(max_newton=newton)
+    p = d.ctx.profile_read()
+    d.ctx.profile(False)
+    d.close()
+    bh = p["barrier_hess"]
+    return {"barrier_team": team, "barrier_hess_us": 1e3 * bh["ms"] / max(bh["launches"], 1),
+            "launches": int(bh["launches"])}
+
+
+def choose_team(rows):
+    acc = {}
+    for r in rows:
+        acc[r["barrier_team"]] = acc.get(r["barrier_team"], 0.0) + r["barrier_hess_us"]
+    if not acc:
+        return DEFAULT_TEAM
+    return min(acc, key=lambda k: (acc[k], k != DEFAULT_TEAM))
+
+
+def tune_team(workloads, newton, config, log=print):
+    rows = []
+    for name in workloads:
+        kind, sc = _workload(name)
+        for t in TEAMS:
+            r = measure_team(kind, sc, t, newton, config)
+            r["workload"] = name
+            rows.append(r)
+            log(f"[tune] {name} barrier team {t}: {r['barrier_hess_us']:.1f} us per launch")
+    return choose_team(rows), rows
 
 
 def choose(rows):
@@ -116,21 +163,26 @@ def main(argv=None):
     a = ap.parse_args(argv)
     names = [w for w in a.workloads.split(",") if w]
     t0 = time.time()
-    (shape, waves), rows = tune(names, a.newton, log=lambda *x: print(*x, file=sys.stderr, flush=True))
+    lg = lambda *x: print(*x, file=sys.stderr, flush=True)  # noqa: E731
+    (shape, waves), rows = tune(names, a.newton, log=lg)
+    team, team_rows = tune_team(names, a.newton, (shape, waves), log=lg)
     out = {
         "arch": "gfx950",
         "spmv_shape": shape,
         "spmv_lanes_per_row": SHAPE_TABLE[shape][0],
         "spmv_scalar_wavefront": SHAPE_TABLE[shape][1],
         "upd_waves": waves,
+        "barrier_team": team,
         "workloads": names,
         "newton_per_run": a.newton,
         "seconds": round(time.time() - t0, 1),
         "measurements": rows,
+        "barrier_team_measurements": team_rows,
     }
     with open(a.out, "w") as f:
         json.dump(out, f, indent=1)
-    print(json.dumps({k: out[k] for k in ("spmv_shape", "spmv_lanes_per_row", "spmv_scalar_wavefront", "upd_waves")}))
+    print(json.dumps({k: out[k] for k in ("spmv_shape", "spmv_lanes_per_row", "spmv_scalar_wavefront", "upd_waves",
+                                          "barrier_team")}))
 
 
 if __name__ == "__main__":

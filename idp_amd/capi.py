@@ -45,7 +45,8 @@ EXPORTED = [
     "idp_elastic_energy", "idp_elastic_gradient", "idp_elastic_hessian_local", "idp_newton_direction", "idp_get_bsr",
     "idp_profile_enable", "idp_profile_read", "idp_set_mesh_batch", "idp_batch_size", "idp_advance_batch",
     "idp_serve_configure", "idp_serve_start", "idp_serve_iterate", "idp_serve_result", "idp_set_launch_config",
-    "idp_get_launch_config", "idp_set_accd_max_steps", "idp_barrier_energy_noalloc",
+    "idp_get_launch_config", "idp_set_accd_max_steps", "idp_barrier_energy_noalloc", "idp_get_surface", "idp_device_memory", "idp_set_barrier_team",
+    "idp_get_barrier_team", "idp_load_velocity",
 ]
 
 # PCG preconditioner for every solve of this process: 0 cluster block Jacobi
@@ -66,6 +67,18 @@ PCG_HANDOVER = int(os.environ.get("IDP_PCG_HANDOVER", "0"))
 # context of the process, so an instance's result never depends on its mix.
 TUNING_PATH = os.environ.get("IDP_TUNING", os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json"))
 _tuned = None
+
+
+def tuned_barrier_team(path=None):
+    """Barrier Hessian team width from a tuning file, or None."""
+    import json
+
+    try:
+        with open(path or TUNING_PATH) as f:
+            t = int(json.load(f)["barrier_team"])
+        return t if t in (8, 16, 32, 64) else None
+    except (OSError, ValueError, KeyError, TypeError):
+        return None
 
 
 def tuned_config(path=None):
@@ -172,6 +185,11 @@ def lib():
         L.idp_snapshot_state.argtypes = [vp]
         L.idp_restore_state.argtypes = [vp]
         L.idp_surface_sizes.argtypes = [vp, ip, ip, ip]
+        L.idp_get_surface.argtypes = [vp, ip, ip, ip]
+        L.idp_set_barrier_team.argtypes = [vp, C.c_int]
+        L.idp_load_velocity.argtypes = [vp, dp, C.c_double]
+        L.idp_get_barrier_team.argtypes = [vp, ip]
+        L.idp_device_memory.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         L.idp_advance_one_step.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_double, dp, C.c_double, dp,
                                            C.c_int, C.POINTER(SolverOpts), C.POINTER(StepStats)]
         L.idp_set_accd_max_steps.argtypes = [vp, C.c_int]
@@ -230,6 +248,13 @@ def device_count():
     n = C.c_int()
     lib().idp_device_count(C.byref(n))
     return n.value
+
+
+def device_memory():
+    """(bytes now, most bytes at once) held by the library's device buffers in this process."""
+    a, b = C.c_longlong(), C.c_longlong()
+    lib().idp_device_memory(C.byref(a), C.byref(b))
+    return a.value, b.value
 
 
 def fem_init(X, tets, density, E, nu):
@@ -300,6 +325,18 @@ class Context:
         cfg = tuned_config()
         if cfg and "IDP_SPMV_SHAPE" not in os.environ:  # an explicit shape (tuning runs) wins
             self.set_launch_config(*cfg)
+        team = tuned_barrier_team()
+        if team:
+            self.set_barrier_team(team)
+
+    def set_barrier_team(self, lanes):
+        """Lanes per constraint in the barrier Hessian's projection stage (8, 16, 32, 64)."""
+        self._check(lib().idp_set_barrier_team(self._h, int(lanes)))
+
+    def barrier_team(self):
+        t = C.c_int()
+        self._check(lib().idp_get_barrier_team(self._h, C.byref(t)))
+        return t.value
 
     def set_launch_config(self, spmv_shape, upd_waves):
         """PCG launch configuration (see idp_set_launch_config)."""
@@ -444,10 +481,24 @@ class Context:
         self._check(lib().idp_surface_sizes(self._h, C.byref(nb), C.byref(ne), C.byref(nf)))
         return nb.value, ne.value, nf.value
 
+    def get_surface(self):
+        """(boundary nodes, edges (ne, 2), triangles (nf, 3)) extracted by set_mesh."""
+        nb, ne, nf = self.surface_sizes()
+        bn, bnp = _out_i(max(nb, 1))
+        be, bep = _out_i(max(2 * ne, 1))
+        bt, btp = _out_i(max(3 * nf, 1))
+        self._check(lib().idp_get_surface(self._h, bnp, bep, btp))
+        return bn[:nb].copy(), be[: 2 * ne].reshape(-1, 2).copy(), bt[: 3 * nf].reshape(-1, 3).copy()
+
     def set_state(self, X=None, v=None):
         Xa, Xp = _d(X) if X is not None else (None, None)
         va, vp = _d(v) if v is not None else (None, None)
         self._check(lib().idp_set_state(self._h, Xp, vp))
+
+    def load_velocity(self, X_target, h):
+        """v = (X_target - x) / h on the device (Load_Velocity_X0)."""
+        t, tp = _d(X_target)
+        self._check(lib().idp_load_velocity(self._h, tp, h))
 
     def get_state(self):
         X, Xp = _out_d(3 * self.nV)

@@ -497,144 +497,6 @@ __device__ inline const double* bmat()
     return c_bee;
 }
 
-// ---------------------------------------------------------------------------
-// Cyclic Jacobi eigen-decomposition of a symmetric N x N matrix.  A is held in
-// registers as the packed upper triangle; the eigenvector matrix W lives in
-// LDS (element (k, p) of this lane at W[(9 k + p) * WS]).
-// ---------------------------------------------------------------------------
-constexpr int WS = 64;  // lanes per block of the Hessian kernel = LDS stride
-
-template <int N>
-__device__ inline void jacobi(double* A, double* W)
-{
-#pragma clang fp contract(fast)
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-#pragma unroll
-        for (int j = 0; j < N; ++j) W[(i * 9 + j) * WS] = (i == j) ? 1.0 : 0.0;
-#pragma nounroll
-    for (int sweep = 0; sweep < 16; ++sweep) {
-        double off = 0.0, dg = 0.0;
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            dg += A[sx<N>(i, i)] * A[sx<N>(i, i)];
-#pragma unroll
-            for (int j = i + 1; j < N; ++j) off += A[sx<N>(i, j)] * A[sx<N>(i, j)];
-        }
-        if (!(off > 1e-34 * dg)) break;
-#pragma unroll
-        for (int p = 0; p < N - 1; ++p)
-#pragma unroll
-            for (int q = p + 1; q < N; ++q) {
-                const double apq = A[sx<N>(p, q)];
-                if (apq != 0.0) {
-                    const double theta = (A[sx<N>(q, q)] - A[sx<N>(p, p)]) / (2.0 * apq);
-                    const double at = fabs(theta);
-                    const double t = (at > 1e150) ? 0.5 / theta
-                                                  : copysign(1.0, theta) / (at + sqrt(theta * theta + 1.0));
-                    const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
-#pragma unroll
-                    for (int k2 = 0; k2 < N; ++k2) {
-                        if (k2 == p || k2 == q) continue;
-                        const double akp = A[sx<N>(k2, p)], akq = A[sx<N>(k2, q)];
-                        A[sx<N>(k2, p)] = c * akp - s * akq;
-                        A[sx<N>(k2, q)] = s * akp + c * akq;
-                    }
-                    A[sx<N>(p, p)] = A[sx<N>(p, p)] - t * apq;
-                    A[sx<N>(q, q)] = A[sx<N>(q, q)] + t * apq;
-                    A[sx<N>(p, q)] = 0.0;
-#pragma unroll
-                    for (int k2 = 0; k2 < N; ++k2) {
-                        const double wkp = W[(k2 * 9 + p) * WS], wkq = W[(k2 * 9 + q) * WS];
-                        W[(k2 * 9 + p) * WS] = c * wkp - s * wkq;
-                        W[(k2 * 9 + q) * WS] = s * wkp + c * wkq;
-                    }
-                }
-                // keep each rotation's LDS traffic in place (no hoisting of later
-                // rotations' loads, which only raises register pressure)
-                __builtin_amdgcn_sched_barrier(0);
-            }
-    }
-}
-
-// makePD in the reduced space; H (packed sx<3M>) is replaced by its
-// projection.  W: this lane's LDS eigenvector workspace.
-template <int KIND>
-__device__ inline bool project_reduced(double* H, double* W)
-{
-#pragma clang fp contract(fast)
-    constexpr int M = KT<KIND>::M;
-    constexpr int N = 3 * M;
-    const double* S = (M == 3) ? ((KIND == K_PT) ? c_S3pt : c_S3ee) : c_S2;
-    const double* Si = (M == 3) ? ((KIND == K_PT) ? c_Si3pt : c_Si3ee) : c_Si2;
-    double K[N * (N + 1) / 2];
-    // K = C H C, C = S (x) I3  (for M == 1, C = sqrt(2) I); upper triangle
-#pragma unroll
-    for (int a = 0; a < M; ++a)
-#pragma unroll
-        for (int b = a; b < M; ++b)
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    if (a == b && j < i) continue;
-                    double acc = 0.0;
-                    if (M == 1) {
-                        acc = 2.0 * H[sx<N>(i, j)];
-                    } else {
-#pragma unroll
-                        for (int g = 0; g < M; ++g)
-#pragma unroll
-                            for (int h = 0; h < M; ++h)
-                                acc += S[a * M + g] * H[sx<N>(3 * g + i, 3 * h + j)] * S[h * M + b];
-                    }
-                    K[sx<N>(3 * a + i, 3 * b + j)] = acc;
-                }
-    jacobi<N>(K, W);
-    double lmin = K[sx<N>(0, 0)];
-#pragma unroll
-    for (int i = 1; i < N; ++i) lmin = fmin(lmin, K[sx<N>(i, i)]);
-    if (lmin >= 0.0) return false;  // UTILS.h:12-15: nothing to clamp
-    double lam[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) lam[i] = fmax(K[sx<N>(i, i)], 0.0);
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        double wi[N];
-#pragma unroll
-        for (int k2 = 0; k2 < N; ++k2) wi[k2] = W[(i * 9 + k2) * WS] * lam[k2];
-#pragma unroll
-        for (int j = i; j < N; ++j) {
-            double acc = 0.0;
-#pragma unroll
-            for (int k2 = 0; k2 < N; ++k2) acc += wi[k2] * W[(j * 9 + k2) * WS];
-            K[sx<N>(i, j)] = acc;
-        }
-    }
-#pragma unroll
-    for (int a = 0; a < M; ++a)
-#pragma unroll
-        for (int b = a; b < M; ++b)
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    if (a == b && j < i) continue;
-                    double acc = 0.0;
-                    if (M == 1) {
-                        acc = 0.5 * K[sx<N>(i, j)];
-                    } else {
-#pragma unroll
-                        for (int g = 0; g < M; ++g)
-#pragma unroll
-                            for (int h = 0; h < M; ++h)
-                                acc += Si[a * M + g] * K[sx<N>(3 * g + i, 3 * h + j)] * Si[h * M + b];
-                    }
-                    H[sx<N>(3 * a + i, 3 * b + j)] = acc;
-                }
-    return true;
-}
-
 // vertex gradient g_x = B^T g_y (3*NV values)
 template <int KIND>
 __device__ inline void expand_grad(const double* g, double* gx)
@@ -660,37 +522,6 @@ __device__ inline void expand_grad(const double* g, double* gx)
         }
 }
 
-// vertex block (vi, vj) of B^T H B
-template <int KIND>
-__device__ inline void expand_block(const double* H, int vi, int vj, double* blk)
-{
-#pragma clang fp contract(fast)
-    constexpr int M = KT<KIND>::M, NV = KT<KIND>::NV;
-    if (M == 1) {
-        const double s = (vi == vj) ? 1.0 : -1.0;
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) blk[3 * i + j] = s * H[sx<3>(i, j)];
-        return;
-    }
-    const double* B = bmat<KIND>();
-#pragma unroll
-    for (int i = 0; i < 9; ++i) blk[i] = 0.0;
-#pragma unroll
-    for (int a = 0; a < M; ++a) {
-        const double ba = B[a * NV + vi];
-#pragma unroll
-        for (int b = 0; b < M; ++b) {
-            const double s = ba * B[b * NV + vj];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) blk[3 * i + j] += s * H[sx<3 * M>(3 * a + i, 3 * b + j)];
-        }
-    }
-}
-
 template <int KIND>
 __device__ inline void grad_body(const Decoded& k, const double* X, const double* X0, double dh, double kappa,
                                  double th2, double* out, int* err)
@@ -707,44 +538,6 @@ __device__ inline void grad_body(const Decoded& k, const double* X, const double
     constexpr int NV = KT<KIND>::NV;
 #pragma unroll
     for (int j = 0; j < 12; ++j) out[j] = (j < 3 * NV) ? gx[j] : 0.0;
-}
-
-template <int KIND>
-__device__ inline void hess_body(const Decoded& k, const double* X, const double* X0, double dh, double kappa,
-                                 double th2, int project_spd, int mode, double* out, int* err, int* counts,
-                                 double* W)
-{
-    constexpr int NV = KT<KIND>::NV;
-    double g[9], H[45], e;
-    if (!reduced_derivs<KIND, true>(k, X, X0, dh, kappa, th2, g, H, e)) {
-        atomicOr(err, 1);
-        for (int j = 0; j < 144; ++j) out[j] = 0.0;
-        return;
-    }
-    bool clamped = false;
-    if (project_spd) clamped = project_reduced<KIND>(H, W);
-    if (counts) {
-        atomicAdd(counts + 2 * KIND, 1);
-        if (clamped) atomicAdd(counts + 2 * KIND + 1, 1);
-    }
-    double blk[9];
-#pragma nounroll
-    for (int a = 0; a < NV; ++a)
-#pragma nounroll
-        for (int b = 0; b < NV; ++b) {
-            expand_block<KIND>(H, a, b, blk);
-            if (mode == 0) {
-#pragma unroll
-                for (int r = 0; r < 3; ++r)
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) out[(3 * a + r) * (3 * NV) + 3 * b + c] = blk[3 * r + c];
-            } else {
-#pragma unroll
-                for (int r = 0; r < 9; ++r) out[9 * (a * NV + b) + r] = blk[r];
-            }
-        }
-    if (mode == 0)
-        for (int j = 9 * NV * NV; j < 144; ++j) out[j] = 0.0;
 }
 
 #define IDP_KIND_SWITCH(kind, F, ...)                                                                    \
@@ -773,27 +566,389 @@ __global__ void __launch_bounds__(256) k_barrier_grad(long n, const int4* __rest
                     err + inst)
 }
 
-// Hessian: mode 0 writes the dense (3nv)^2 local matrix (144 per constraint,
-// row-major, vertex order of decode); mode 1 writes nv*nv vertex blocks of 9
-// (block (a,b) at 9*(a*nv+b)).
-__global__ void __launch_bounds__(WS) k_barrier_hess(long n, const int4* __restrict__ cs,
-                                                     const double* __restrict__ X,
-                                                     const double* __restrict__ X0, BarrierParams bp,
-                                                     int project_spd, int mode,
-                                                     double* __restrict__ out, int* __restrict__ vid,
-                                                     int* __restrict__ err, int* __restrict__ counts)
+// ---------------------------------------------------------------------------
+// Barrier Hessian with the PSD projection, one constraint per team of TL
+// lanes (HB / TL teams per wavefront-sized workgroup; TL = 8, 16, 32 or 64 is
+// a launch parameter, and no sum depends on it, so neither do the results).  The team's first lane
+// forms the reduced Hessian H_y (N = 3M) and K = C H_y C in the team's LDS;
+// the team then diagonalises K by cyclic Jacobi, each rotation's row and
+// eigenvector updates spread over the lanes (lane k updates row k), and
+// spreads the clamp V max(L, 0) V^T, the map back C^-1 (.) C^-1 and the
+// vertex blocks B^T H_y B over its lanes as well.  The loops index LDS, so
+// the kernel stays small enough for the instruction cache, and a constraint
+// takes TL lanes instead of one, which shortens the serial chain per
+// launch.
+// ---------------------------------------------------------------------------
+constexpr int HB = 64;  // lanes per workgroup: one wavefront
+
+struct TeamLds {
+    double K[2][81];  // the matrix being diagonalised, row-major, stride 9 (double buffer)
+    double V[2][81];  // eigenvectors (columns), stride 9 (double buffer)
+    double H[81];     // H_y, stride 9; projected in place
+    double rc[10], rs[10], rt[10];  // this round's rotation seen from index i: c, signed s, t
+    int rp[10];                      // ... and i's partner (i itself when i does not rotate)
+    double lam[9];
+    int n, kind, ok, clamped;  // ok: 1 formed, 0 d <= 0, 2 nothing to do
+    int done;                  // Jacobi converged
+};
+
+// lanes of one team see each other's LDS writes in program order (the team
+// is inside one wavefront): keep the compiler from moving LDS accesses
+// across this point
+__device__ inline void team_sync()
 {
-    __shared__ double wsh[81 * WS];
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// runtime index of (i, j), i <= j, in a packed upper triangle of order n
+__device__ inline int pk(int i, int j, int n) { return (i * (2 * n - i + 1)) / 2 + (j - i); }
+
+// Stage 1, one constraint per lane: the reduced Hessian H_y (N = 3M) and
+// K = C H_y C (C = S (x) I3; M == 1: sqrt(2) I), both as packed upper
+// triangles of order N (45 doubles each) at F + 90 i; ok[i] = 0 when the
+// distance is not positive (the constraint's blocks are zero).
+template <int KIND>
+__device__ inline void form_body(const Decoded& k, const double* X, const double* X0, double dh, double kappa,
+                                 double th2, double* F, unsigned char* ok, int* err)
+{
+#pragma clang fp contract(fast)
+    constexpr int M = KT<KIND>::M;
+    constexpr int N = 3 * M;
+    double g[9], H[45], e;
+    if (!reduced_derivs<KIND, true>(k, X, X0, dh, kappa, th2, g, H, e)) {
+        atomicOr(err, 1);
+        *ok = 0;
+        return;
+    }
+    const double* S = (M == 3) ? ((KIND == K_PT) ? c_S3pt : c_S3ee) : c_S2;
+#pragma unroll
+    for (int q = 0; q < N * (N + 1) / 2; ++q) F[45 + q] = H[q];
+#pragma unroll
+    for (int a = 0; a < M; ++a)
+#pragma unroll
+        for (int b = a; b < M; ++b)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    if (a == b && j < i) continue;
+                    double acc = 0.0;
+                    if (M == 1) {
+                        acc = 2.0 * H[sx<N>(i, j)];
+                    } else {
+#pragma unroll
+                        for (int gg = 0; gg < M; ++gg)
+#pragma unroll
+                            for (int hh = 0; hh < M; ++hh)
+                                acc += S[a * M + gg] * H[sx<N>(3 * gg + i, 3 * hh + j)] * S[hh * M + b];
+                    }
+                    F[sx<N>(3 * a + i, 3 * b + j)] = acc;
+                }
+    *ok = 1;
+}
+
+__global__ void __launch_bounds__(256) k_barrier_form(long n, const int* __restrict__ perm,
+                                                      const int4* __restrict__ cs, const double* __restrict__ X,
+                                                      const double* __restrict__ X0, BarrierParams bp,
+                                                      double* __restrict__ F, unsigned char* __restrict__ ok,
+                                                      int* __restrict__ err)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const long i = perm ? (long)perm[t] : t;
     Decoded k = decode(cs[i]);
     const int inst = bp.inst_of(k.v[0]);
-    if (bp.skip(inst)) return;
-    IDP_KIND_SWITCH(k.kind, hess_body, k, X, X0, bp.dh[inst], bp.kappa[inst], bp.th2[inst], project_spd, mode,
-                    out + 144 * i, err + inst, counts,
-                    wsh + threadIdx.x)
-    if (vid)
-        for (int j = 0; j < 4; ++j) vid[4 * i + j] = (j < k.nv) ? k.v[j] : -1;
+    if (bp.skip(inst)) {
+        ok[t] = 2;
+        return;
+    }
+    IDP_KIND_SWITCH(k.kind, form_body, k, X, X0, bp.dh[inst], bp.kappa[inst], bp.th2[inst], F + 90 * t, ok + t,
+                    err + inst)
+}
+
+// Partner of index i in round r of the circle schedule over m players
+// (m even): player m-1 stays, the others turn; every pair meets once in
+// m-1 rounds.
+__device__ inline int rr_partner(int i, int r, int m)
+{
+    if (i == m - 1) return r;
+    if (i == r) return m - 1;
+    const int q = (2 * r - i) % (m - 1);
+    return q < 0 ? q + (m - 1) : q;
+}
+
+// Parallel (round-robin) Jacobi on K (n x n): each round applies the
+// rotations of n/2 disjoint index pairs at once, K <- J^T K J and V <- V J,
+// every entry from the previous round's matrices (double buffer); each
+// rotation is the classical one (Golub & Van Loan 8.5), with the rotated
+// pair's diagonal updated as app - t apq, aqq + t apq and apq set to 0.  A
+// lane owns rows of K and V.  Entries are evaluated without contraction and
+// grouped so that (i, j) and (j, i) round alike: K stays exactly symmetric.
+// Returns the buffer holding the result.
+template <int TL>
+__device__ inline int team_jacobi(TeamLds& T, int tl, int n)
+{
+#pragma clang fp contract(off)
+    const int m = n + (n & 1);  // players of the schedule (an odd n gets an idle one)
+    int cur = 0;
+    for (int idx = tl; idx < 81; idx += TL) T.V[0][idx] = (idx / 9 == idx % 9) ? 1.0 : 0.0;
+    team_sync();
+    for (int sweep = 0; sweep < 16; ++sweep) {
+        // convergence test, summed by one lane in a fixed order
+        if (tl == 0) {
+            const double* K = T.K[cur];
+            double off = 0.0, dg = 0.0;
+            for (int i = 0; i < n; ++i) {
+                dg += K[10 * i] * K[10 * i];
+                for (int j = i + 1; j < n; ++j) off += K[9 * i + j] * K[9 * i + j];
+            }
+            T.done = !(off > 1e-34 * dg);
+        }
+        team_sync();
+        if (T.done) break;
+        for (int r = 0; r < m - 1; ++r) {
+            const double* K0 = T.K[cur];
+            const double* V0 = T.V[cur];
+            double* K1 = T.K[cur ^ 1];
+            double* V1 = T.V[cur ^ 1];
+            // this round's rotations, from the lane of each pair's smaller index
+            for (int i = tl; i < n; i += TL) {
+                const int q = rr_partner(i, r, m);
+                if (q >= n) {
+                    T.rp[i] = i;
+                    T.rc[i] = 1.0;
+                    T.rs[i] = 0.0;
+                    T.rt[i] = 0.0;
+                } else if (q > i) {
+                    const double apq = K0[9 * i + q];
+                    double c = 1.0, sn = 0.0, t = 0.0;
+                    if (apq != 0.0) {
+                        const double dd = K0[10 * q] - K0[10 * i];
+                        const double rt = sqrt(dd * dd + 4.0 * apq * apq);
+                        t = copysign(1.0, dd) * (2.0 * apq) / (fabs(dd) + rt);
+                        c = rsqrt(t * t + 1.0);
+                        sn = t * c;
+                    }
+                    T.rp[i] = q;
+                    T.rc[i] = c;
+                    T.rs[i] = -sn;
+                    T.rt[i] = t;
+                    T.rp[q] = i;
+                    T.rc[q] = c;
+                    T.rs[q] = sn;
+                    T.rt[q] = t;
+                }
+            }
+            team_sync();
+            for (int i = tl; i < n; i += TL) {
+                const int pi = T.rp[i];
+                const double ci = T.rc[i], si = T.rs[i];
+                for (int j = 0; j < n; ++j) {
+                    const int pj = T.rp[j];
+                    const double cj = T.rc[j], sj = T.rs[j];
+                    V1[9 * i + j] = cj * V0[9 * i + j] + sj * V0[9 * i + pj];
+                    double v;
+                    if (pi != i && j == pi) {
+                        v = 0.0;  // the rotated pair's off-diagonal entry
+                    } else if (pi != i && j == i) {
+                        const int lo = min(i, pi), hi = max(i, pi);
+                        const double apq = K0[9 * lo + hi];
+                        v = (i == lo) ? K0[10 * i] - T.rt[i] * apq : K0[10 * i] + T.rt[i] * apq;
+                    } else {
+                        const double cc = ci * cj, ss = si * sj;
+                        const double a1 = (ci * sj) * K0[9 * i + pj], a2 = (si * cj) * K0[9 * pi + j];
+                        v = (cc * K0[9 * i + j] + ss * K0[9 * pi + pj]) + (a1 + a2);
+                    }
+                    K1[9 * i + j] = v;
+                }
+            }
+            cur ^= 1;
+            team_sync();
+        }
+    }
+    return cur;
+}
+
+// makePD (UTILS.h:9-27) of H_y through K: returns whether anything was clamped
+template <int TL>
+__device__ inline bool team_project(TeamLds& T, int tl, int n, int M, const double* Si)
+{
+#pragma clang fp contract(fast)
+    const int cur = team_jacobi<TL>(T, tl, n);
+    double* K = T.K[cur];
+    const double* V = T.V[cur];
+    double lmin = K[0];
+    for (int i = 1; i < n; ++i) lmin = fmin(lmin, K[10 * i]);
+    if (lmin >= 0.0) return false;  // UTILS.h:12-15: nothing to clamp
+    if (tl < n) T.lam[tl] = fmax(K[10 * tl], 0.0);
+    team_sync();
+    // K+ = V max(L, 0) V^T (upper triangle, mirrored)
+    for (int idx = tl; idx < n * n; idx += TL) {
+        const int i = idx / n, j = idx % n;
+        if (j < i) continue;
+        double acc = 0.0;
+        for (int k2 = 0; k2 < n; ++k2) acc += V[9 * i + k2] * T.lam[k2] * V[9 * j + k2];
+        K[9 * i + j] = acc;
+        K[9 * j + i] = acc;
+    }
+    team_sync();
+    // H_y = C^-1 K+ C^-1
+    for (int idx = tl; idx < n * n; idx += TL) {
+        const int r = idx / n, cc = idx % n;
+        if (cc < r) continue;
+        const int a = r / 3, i = r % 3, b = cc / 3, j = cc % 3;
+        double acc = 0.0;
+        if (M == 1) {
+            acc = 0.5 * K[9 * i + j];
+        } else {
+            for (int gg = 0; gg < M; ++gg)
+                for (int hh = 0; hh < M; ++hh) acc += Si[a * M + gg] * K[9 * (3 * gg + i) + 3 * hh + j] * Si[hh * M + b];
+        }
+        T.H[9 * r + cc] = acc;
+        T.H[9 * cc + r] = acc;
+    }
+    team_sync();
+    return true;
+}
+
+// Stage 2, one constraint per team: the projection and the vertex blocks.
+// mode 0 writes the dense (3nv)^2 local matrix (144 per constraint,
+// row-major, vertex order of decode); mode 1 writes nv*nv vertex blocks of 9
+// (block (a,b) at 9*(a*nv+b)).  perm (null: identity) lists the constraints
+// grouped by kind (both stages walk it in the same order), so that the lanes
+// of a wavefront take the same branch of the kind switch in stage 1; every
+// result is written at the constraint's own index.
+// counts (null: none): per kind, evaluations and clamped ones, one atomic
+// per wavefront and kind.
+template <int TL>
+__global__ void __launch_bounds__(HB) k_barrier_hess(long n, const int* __restrict__ perm,
+                                                     const int4* __restrict__ cs, const double* __restrict__ F,
+                                                     const unsigned char* __restrict__ ok_in, int project_spd,
+                                                     int mode, double* __restrict__ out, int* __restrict__ vid,
+                                                     int* __restrict__ counts)
+{
+#pragma clang fp contract(fast)
+    constexpr int TEAMS = HB / TL;
+    __shared__ TeamLds lds[TEAMS];
+    const int team = threadIdx.x / TL, tl = threadIdx.x % TL;
+    TeamLds& T = lds[team];
+    const long t = (long)blockIdx.x * TEAMS + team;
+    // the team loads stage 1's packed K and H_y into LDS as full matrices
+    if (tl == 0) {
+        T.ok = 2;
+        T.clamped = 0;
+        if (t < n) {
+            const long i = perm ? (long)perm[t] : t;
+            const Decoded k = decode(cs[i]);
+            T.ok = ok_in[t];
+            T.kind = k.kind;
+            T.n = (k.kind == K_PE) ? 6 : ((k.kind == K_PP) ? 3 : 9);
+            if (vid && T.ok != 2)
+                for (int j = 0; j < 4; ++j) vid[4 * i + j] = (j < k.nv) ? k.v[j] : -1;
+        }
+    }
+    team_sync();
+    if (T.ok == 1) {
+        const int nn = T.n;
+        const double* Fi = F + 90 * t;
+        for (int idx = tl; idx < nn * nn; idx += TL) {
+            const int r = idx / nn, cc = idx % nn;
+            const int q = (r <= cc) ? pk(r, cc, nn) : pk(cc, r, nn);
+            T.K[0][9 * r + cc] = Fi[q];
+            T.H[9 * r + cc] = Fi[45 + q];
+        }
+    }
+    team_sync();
+    const int ok = T.ok;
+    const long ci = (t < n) ? (perm ? (long)perm[t] : t) : 0;
+    double* o = out + 144 * ci;
+    if (ok == 0) {
+        for (int j = tl; j < 144; j += TL) o[j] = 0.0;
+    } else if (ok == 1) {
+        const int kind = T.kind, nn = T.n;
+        const int M = nn / 3;
+        const int NV = (kind == K_PE) ? 3 : ((kind == K_PP) ? 2 : 4);
+        const double* Si = (M == 3) ? ((kind == K_PT) ? c_Si3pt : c_Si3ee) : c_Si2;
+        bool clamped = false;
+        if (project_spd) clamped = team_project<TL>(T, tl, nn, M, Si);
+        if (tl == 0) T.clamped = clamped;
+        // vertex block (vi, vj) of B^T H_y B: one block per lane
+        const double* B = (kind == K_PT) ? c_bpt : ((kind == K_PE) ? c_bpe : c_bee);
+        for (int bk = tl; bk < NV * NV; bk += TL) {
+            const int vi = bk / NV, vj = bk % NV;
+            double blk[9];
+            if (M == 1) {
+                const double sg = (vi == vj) ? 1.0 : -1.0;
+                for (int r = 0; r < 9; ++r) blk[r] = sg * T.H[9 * (r / 3) + r % 3];
+            } else {
+                for (int r = 0; r < 9; ++r) blk[r] = 0.0;
+                for (int a = 0; a < M; ++a) {
+                    const double ba = B[a * NV + vi];
+                    for (int b = 0; b < M; ++b) {
+                        const double sc = ba * B[b * NV + vj];
+                        for (int r = 0; r < 3; ++r)
+                            for (int c2 = 0; c2 < 3; ++c2) blk[3 * r + c2] += sc * T.H[9 * (3 * a + r) + 3 * b + c2];
+                    }
+                }
+            }
+            if (mode == 0) {
+                for (int r = 0; r < 3; ++r)
+                    for (int c2 = 0; c2 < 3; ++c2) o[(3 * vi + r) * (3 * NV) + 3 * vj + c2] = blk[3 * r + c2];
+            } else {
+                for (int r = 0; r < 9; ++r) o[9 * (vi * NV + vj) + r] = blk[r];
+            }
+        }
+        if (mode == 0)
+            for (int j = 9 * NV * NV + tl; j < 144; j += TL) o[j] = 0.0;
+    }
+    if (counts) {
+        const int lane = threadIdx.x & 63;
+        const int kind = (tl == 0 && ok == 1) ? T.kind : -1;
+        const bool cl = tl == 0 && ok == 1 && T.clamped;
+        for (int q = 0; q < 7; ++q) {
+            const unsigned long long m = __ballot(kind == q), mc = __ballot(kind == q && cl);
+            if (m && lane == __ffsll((long long)m) - 1) {
+                atomicAdd(counts + 2 * q, __popcll(m));
+                if (mc) atomicAdd(counts + 2 * q + 1, __popcll(mc));
+            }
+        }
+    }
+}
+
+// Constraint indices grouped by kind: k_kind_count counts the kinds
+// (cnt[0..7)), k_kind_scatter writes each index into its kind's range of perm
+// (cnt[8..15) are the running cursors; both halves start at zero).  The
+// order inside a kind is arbitrary; the Hessian writes by constraint index.
+__global__ void __launch_bounds__(256) k_kind_count(long n, const int4* __restrict__ cs, unsigned* __restrict__ cnt)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int kind = i < n ? decode(cs[i]).kind : -1;
+    const int lane = threadIdx.x & 63;
+    for (int q = 0; q < 7; ++q) {
+        const unsigned long long m = __ballot(kind == q);
+        if (m && lane == __ffsll((long long)m) - 1) atomicAdd(cnt + q, (unsigned)__popcll(m));
+    }
+}
+
+__global__ void __launch_bounds__(256) k_kind_scatter(long n, const int4* __restrict__ cs,
+                                                      unsigned* __restrict__ cnt, int* __restrict__ perm)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int kind = i < n ? decode(cs[i]).kind : -1;
+    const int lane = threadIdx.x & 63;
+    unsigned off = 0;
+    for (int q = 0; q < 7; ++q) off += (q < kind) ? cnt[q] : 0u;
+    for (int q = 0; q < 7; ++q) {
+        const unsigned long long m = __ballot(kind == q);
+        if (!m) continue;
+        const int leader = __ffsll((long long)m) - 1;
+        unsigned base = 0;
+        if (lane == leader) base = atomicAdd(cnt + 8 + q, (unsigned)__popcll(m));
+        base = __shfl(base, leader);
+        if (kind == q) perm[off + base + (unsigned)__popcll(m & ((1ull << lane) - 1))] = (int)i;
+    }
 }
 
 // vertex ids of each constraint in decode order (-1 padded)
@@ -829,10 +984,35 @@ void launch_barrier_grad(hipStream_t s, long n, const int* cs, const double* X, 
 
 void launch_barrier_hess(hipStream_t s, long n, const int* cs, const double* X, const double* X0,
                          const BarrierParams& bp, int project_spd, int mode, double* out, int* vid, int* err,
-                         int* counts)
+                         double* form, unsigned char* form_ok, int* counts, int* perm, unsigned* kcnt, int team)
 {
-    IDP_LAUNCH(k_barrier_hess, grid_for(n, WS), WS, 0, s, n, (const int4*)cs, X, X0, bp, project_spd, mode, out,
-               vid, err, counts);
+    if (n <= 0) return;
+    if (perm && kcnt) {
+        IDP_HIP(hipMemsetAsync(kcnt, 0, 16 * sizeof(unsigned), s));
+        IDP_LAUNCH(k_kind_count, grid_for(n, 256), 256, 0, s, n, (const int4*)cs, kcnt);
+        IDP_LAUNCH(k_kind_scatter, grid_for(n, 256), 256, 0, s, n, (const int4*)cs, kcnt, perm);
+    } else {
+        perm = nullptr;
+    }
+    IDP_LAUNCH(k_barrier_form, grid_for(n, 256), 256, 0, s, n, perm, (const int4*)cs, X, X0, bp, form, form_ok, err);
+    switch (team) {
+    case 8:
+        IDP_LAUNCH(k_barrier_hess<8>, grid_for(n, HB / 8), HB, 0, s, n, perm, (const int4*)cs, form, form_ok,
+                   project_spd, mode, out, vid, counts);
+        break;
+    case 32:
+        IDP_LAUNCH(k_barrier_hess<32>, grid_for(n, HB / 32), HB, 0, s, n, perm, (const int4*)cs, form, form_ok,
+                   project_spd, mode, out, vid, counts);
+        break;
+    case 64:
+        IDP_LAUNCH(k_barrier_hess<64>, grid_for(n, 1), HB, 0, s, n, perm, (const int4*)cs, form, form_ok,
+                   project_spd, mode, out, vid, counts);
+        break;
+    default:
+        IDP_LAUNCH(k_barrier_hess<16>, grid_for(n, HB / 16), HB, 0, s, n, perm, (const int4*)cs, form, form_ok,
+                   project_spd, mode, out, vid, counts);
+        break;
+    }
 }
 
 void launch_constraint_vids(hipStream_t s, long n, const int* cs, int* vid, int* nv)

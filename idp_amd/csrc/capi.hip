@@ -52,6 +52,11 @@ struct idp_ctx {
     bool snap_valid = false;
     Profiler prof;
     DBuf<int> bcounts;
+    DBuf<int> hperm;          // barrier Hessian: constraint indices grouped by kind
+    DBuf<unsigned> hkcnt;     // ... and the per-kind counts / cursors
+    DBuf<double> hform;       // ... packed K and H_y per constraint between its two stages
+    DBuf<unsigned char> hform_ok;
+    int hess_team = 16;       // lanes per constraint in the projection stage
     SegDev cseg;             // constraint blocks per instance (energy)
     DBuf<double> bpar;       // per instance: dh[B], kappa[B], th2[B]
     DBuf<double> esum;       // per-instance partial-sum results
@@ -60,6 +65,10 @@ struct idp_ctx {
     // energy, gradient and Hessian (INC_POTENTIAL.h:53-73, 191-211, 323-339)
     bool no_elastic = false;
     DBuf<double> bforce;  // body force of the current step (3 nV)
+    SurfWork sw;          // device surface extraction
+    double* pinned = nullptr;  // page-locked staging of target positions (3 nV)
+    size_t pinned_n = 0;
+    DBuf<double> target;
     // this device's address of the fixed scratch array of the allocation-free
     // entry points (resolved once per device when the first context is created)
     unsigned char* scratch = nullptr;
@@ -236,6 +245,7 @@ void update_cseg(idp_ctx* c)
 // over the current constraint set (barrier parameters from c->bpar)
 void energies(idp_ctx* c, const double* X, bool with_barrier, double h, std::vector<EnergyOut>& out)
 {
+    IDP_TIMER_FLAG("Compute_IncPotential");
     auto s = c->s;
     auto& m = c->m;
     const InstDev& I = m.inst;
@@ -283,6 +293,7 @@ EnergyOut energy(idp_ctx* c, const double* X, bool with_barrier, double h)
 
 double min_dist2_dev(idp_ctx* c, const double* X, long n, const int* cs, double* dist2)
 {
+    IDP_TIMER_FLAG("Compute_Min_Dist");
     auto s = c->s;
     unsigned long long* dm = c->dmin.ensure(1, s);
     unsigned long long init = 0x7ff0000000000000ull;  // +inf
@@ -301,6 +312,7 @@ double min_dist2_dev(idp_ctx* c, const double* X, long n, const int* cs, double*
 void gradient(idp_ctx* c, const double* X, double h, bool with_barrier, double* g, double* rhs,
               const unsigned char* act = nullptr)
 {
+    IDP_TIMER_FLAG("Compute_IncPotential_Gradient");
     auto s = c->s;
     auto& m = c->m;
     double* gl = c->gl.ensure(12 * (size_t)m.nT, s);
@@ -327,12 +339,17 @@ void gradient(idp_ctx* c, const double* X, double h, bool with_barrier, double* 
 void hessian(idp_ctx* c, const double* X, double h, bool with_barrier, const unsigned char* act = nullptr,
              long nT_act = -1)
 {
+    IDP_TIMER_FLAG("Compute_IncPotential_Hessian");
     auto s = c->s;
     auto& m = c->m;
     double* Hl = c->Hl.ensure(144 * (size_t)m.nT, s);
     int tok = c->prof.begin(IDP_K_TET_HESS);
-    if (c->no_elastic) IDP_HIP(hipMemsetAsync(Hl, 0, 144 * (size_t)m.nT * sizeof(double), s));
-    else launch_tet_hess(s, m, X, h * h, 1, Hl, act);
+    if (c->no_elastic) {
+        IDP_HIP(hipMemsetAsync(Hl, 0, 144 * (size_t)m.nT * sizeof(double), s));
+    } else {
+        IDP_TIMER_FLAG("Compute_Volumetric_Elasticity_Hessian");
+        launch_tet_hess(s, m, X, h * h, 1, Hl, act);
+    }
     c->prof.end(IDP_K_TET_HESS, tok, 6500.0 * (nT_act < 0 ? m.nT : nT_act));
     const double* Hb = nullptr;
     long ncs = c->cw.ncs;
@@ -340,11 +357,14 @@ void hessian(idp_ctx* c, const double* X, double h, bool with_barrier, const uns
         const int B = m.inst.B;
         int* fl = c->flags.ensure(4 * (size_t)B + 4, s) + 3 * B;
         double* b = c->Hb.ensure(144 * (size_t)ncs, s);
+        IDP_TIMER_FLAG("Compute_Barrier_Hessian");
         tok = c->prof.begin(IDP_K_BARRIER_HESS);
         BarrierParams bp = bparams(c);
         bp.act = act;
         launch_barrier_hess(s, ncs, (const int*)c->cw.cs.p, X, m.X0.p, bp, 1, 1, b, nullptr, fl,
-                            c->prof.full() ? c->bcounts.p : nullptr);
+                            c->hform.ensure(90 * (size_t)ncs, s), c->hform_ok.ensure(ncs, s),
+                            c->prof.full() ? c->bcounts.p : nullptr, c->hperm.ensure(ncs, s),
+                            c->hkcnt.ensure(16, s), c->hess_team);
         c->prof.end(IDP_K_BARRIER_HESS, tok);
         Hb = b;
     }
@@ -372,6 +392,14 @@ const double* stage_x(idp_ctx* c, const double* X)
 // per-instance vector helpers of the batched Newton loop (mask: instances
 // that take part; the arithmetic per element is that of the plain loop)
 // ---------------------------------------------------------------------------
+// v = (x_target - x) / h per coordinate (Load_Velocity_X0, MESHIO.h:987-1005)
+__global__ void k_load_velocity(long n3, const double* __restrict__ x, const double* __restrict__ tgt, double h,
+                                double* __restrict__ v)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3) v[i] = (tgt[i] - x[i]) / h;
+}
+
 // ---- moving Dirichlet nodes (Shell/DIRICHLET.h) --------------------------
 // x of the DBC entries, in entry order
 // x~ = x + (h v + h^2 a), a = M^-1 b with the lumped mass; the flow variant
@@ -1004,7 +1032,10 @@ int setup_mesh(idp_ctx* c, int nV, const double* X_rest, int nT, const int* tets
         std::vector<int> bn;
         std::vector<std::array<int, 2>> be;
         std::vector<std::array<int, 3>> bt;
-        surface_host(nV, X_rest, nT, tets, bn, be, bt);
+        // on the device (host restatement only beyond the device keys' widths)
+        IDP_HIP(hipStreamSynchronize(s));
+        if (!surface_device(s, c->sw, nV, m.X0.p, nT, m.tets.p, bn, be, bt))
+            surface_host(nV, X_rest, nT, tets, bn, be, bt);
         m.nb = (int)bn.size();
         m.ne = (int)be.size();
         m.nf = (int)bt.size();
@@ -1212,12 +1243,20 @@ int idp_ctx_destroy(idp_ctx* c)
         (void)hipStreamSynchronize(c->s);
         (void)hipStreamDestroy(c->s);
     }
+    if (c->pinned) (void)hipHostFree(c->pinned);
     delete c->eng;
     delete c;
     return IDP_OK;
 }
 
 const char* idp_last_error(const idp_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int idp_device_memory(long long* in_use, long long* peak)
+{
+    if (in_use) *in_use = dev_mem().in_use.load();
+    if (peak) *peak = dev_mem().peak.load();
+    return IDP_OK;
+}
 
 int idp_set_launch_config(idp_ctx* c, int spmv_shape, int upd_waves)
 {
@@ -1227,6 +1266,20 @@ int idp_set_launch_config(idp_ctx* c, int spmv_shape, int upd_waves)
         c->L.upd_waves = upd_waves;
         return IDP_OK;
     });
+}
+
+int idp_set_barrier_team(idp_ctx* c, int lanes)
+{
+    if (!c || !(lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64)) return IDP_E_INVALID;
+    c->hess_team = lanes;
+    return IDP_OK;
+}
+
+int idp_get_barrier_team(idp_ctx* c, int* lanes)
+{
+    if (!c || !lanes) return IDP_E_INVALID;
+    *lanes = c->hess_team;
+    return IDP_OK;
 }
 
 int idp_get_launch_config(idp_ctx* c, int* spmv_shape, int* upd_waves)
@@ -1425,6 +1478,20 @@ int idp_surface_sizes(idp_ctx* c, int* nb, int* ne, int* nf)
     return IDP_OK;
 }
 
+int idp_get_surface(idp_ctx* c, int* bnode, int* bedge, int* btri)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh) return IDP_E_INVALID;
+        auto s = c->s;
+        const MeshDev& m = c->m;
+        if (bnode && m.nb) IDP_HIP(hipMemcpyAsync(bnode, m.bn.p, m.nb * sizeof(int), hipMemcpyDeviceToHost, s));
+        if (bedge && m.ne) IDP_HIP(hipMemcpyAsync(bedge, m.be.p, 2 * (size_t)m.ne * sizeof(int), hipMemcpyDeviceToHost, s));
+        if (btri && m.nf) IDP_HIP(hipMemcpyAsync(btri, m.bt.p, 3 * (size_t)m.nf * sizeof(int), hipMemcpyDeviceToHost, s));
+        IDP_HIP(hipStreamSynchronize(s));
+        return IDP_OK;
+    });
+}
+
 int idp_set_state(idp_ctx* c, const double* X, const double* v)
 {
     return guarded(c, [&]() -> int {
@@ -1433,6 +1500,31 @@ int idp_set_state(idp_ctx* c, const double* X, const double* v)
         if (X) c->x.upload(X, n, c->s);
         if (v) c->v.upload(v, n, c->s);
         IDP_HIP(hipStreamSynchronize(c->s));
+        return IDP_OK;
+    });
+}
+
+int idp_load_velocity(idp_ctx* c, const double* X_target, double h)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh || !X_target || !(h > 0)) return IDP_E_INVALID;
+        auto s = c->s;
+        const size_t n3 = 3 * (size_t)c->m.nV;
+        IDP_HIP(hipSetDevice(c->device));
+        if (c->pinned_n < n3) {
+            IDP_HIP(hipStreamSynchronize(s));
+            if (c->pinned) IDP_HIP(hipHostFree(c->pinned));
+            c->pinned = nullptr;
+            c->pinned_n = 0;
+            IDP_HIP(hipHostMalloc((void**)&c->pinned, n3 * sizeof(double), hipHostMallocDefault));
+            c->pinned_n = n3;
+        } else {
+            IDP_HIP(hipStreamSynchronize(s));  // the previous target's copy has left the staging buffer
+        }
+        std::memcpy(c->pinned, X_target, n3 * sizeof(double));
+        double* t = c->target.ensure(n3, s);
+        IDP_HIP(hipMemcpyAsync(t, c->pinned, n3 * sizeof(double), hipMemcpyHostToDevice, s));
+        IDP_LAUNCH(k_load_velocity, grid_for((long)n3, 256), 256, 0, s, (long)n3, c->x.p, t, h, c->v.p);
         return IDP_OK;
     });
 }
@@ -1491,6 +1583,7 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
         if (!c->have_mesh || !kappa3 || !(h > 0)) return IDP_E_INVALID;
         if (c->m.inst.B != 1) return IDP_E_INVALID;  // batched contexts step with idp_advance_batch
         c->no_elastic = flow != 0;
+        IDP_TIMER_FLAG("implicitEuler");
         IDP_HIP(hipSetDevice(c->device));
         const double t_start = now_ms();
         idp_step_stats S;
@@ -1595,6 +1688,7 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
         S.dbc_alpha = DBCAlpha;
         auto t0 = now_ms();
         auto cset = [&](const double* X) {
+            IDP_TIMER_FLAG("Compute_Constraint_Set");
             double ta = now_ms();
             const int tk = c->prof.begin(IDP_K_CONSTRAINT_SET);
             contact_constraint_set(c->cw, s, m, X, &dHat2, thickness);
@@ -1641,7 +1735,11 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
                 // from the second Newton iteration on, sol holds this step's previous direction
                 const unsigned char on = 1;
                 const unsigned char* warm = PNIter > 0 && !cold ? c->one.upload_get(&on, 1, s) : nullptr;
-                int it = pcg_solve_persistent(c->L, s, m, rhs, sol, rtol, maxit, &rel, warm);
+                int it;
+                {
+                    IDP_TIMER_FLAG("linearSolve");
+                    it = pcg_solve_persistent(c->L, s, m, rhs, sol, rtol, maxit, &rel, warm);
+                }
                 S.ms_solve += now_ms() - t0;
                 if (it < 0) {
                     useGD = true;  // DIRECT_SOLVER.h:42-44 -> gradient descent (IMPLICIT_EULER.h:531-551)
@@ -1674,7 +1772,10 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
             if (coll) {
                 t0 = now_ms();
                 long nc = 0;
-                alpha = contact_ccd(c->cw, s, m, x, sol, thickness, alpha, &nc);
+                {
+                    IDP_TIMER_FLAG("Compute_Intersection_Free_StepSize");
+                    alpha = contact_ccd(c->cw, s, m, x, sol, thickness, alpha, &nc);
+                }
                 S.ms_ccd += now_ms() - t0;
                 S.ccd_candidates += nc;
                 S.accd_capped += c->cw.last_capped;
@@ -1933,7 +2034,10 @@ int idp_barrier_hessian_local(idp_ctx* c, const double* X, long n, const int* cs
         double* b = c->Hb.ensure(144 * (size_t)std::max(n, 1L), s);
         DBuf<int> dv;
         int* dvid = dv.ensure(4 * (size_t)std::max(n, 1L), s);
-        if (n) launch_barrier_hess(s, n, (const int*)c->cw.cs.p, xd, c->m.X0.p, bparams(c), project, 0, b, dvid, fl);
+        if (n)
+            launch_barrier_hess(s, n, (const int*)c->cw.cs.p, xd, c->m.X0.p, bparams(c), project, 0, b, dvid, fl,
+                                c->hform.ensure(90 * (size_t)n, s), c->hform_ok.ensure(n, s), nullptr,
+                                c->hperm.ensure(n, s), c->hkcnt.ensure(16, s), c->hess_team);
         int hf = 0;
         IDP_HIP(hipMemcpyAsync(&hf, fl, sizeof(int), hipMemcpyDeviceToHost, s));
         if (n) {

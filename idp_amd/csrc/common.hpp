@@ -5,6 +5,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <cstring>
 #include <cstdio>
@@ -12,9 +13,24 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "../../include/idp_amd.h"
 
 namespace idp {
+
+// A roctx range named like the reference's TIMER_FLAG scope it stands for
+// (Library/Utils/PROFILER.h:66-105), so a rocprofv3 --marker-trace shows the
+// step in the reference's own terms; a no-op when no tool is attached.
+struct TimerFlag {
+    explicit TimerFlag(const char* name) { roctxRangePushA(name); }
+    ~TimerFlag() { roctxRangePop(); }
+    TimerFlag(const TimerFlag&) = delete;
+    TimerFlag& operator=(const TimerFlag&) = delete;
+};
+#define IDP_TIMER_CAT2(a, b) a##b
+#define IDP_TIMER_CAT(a, b) IDP_TIMER_CAT2(a, b)
+#define IDP_TIMER_FLAG(name) ::idp::TimerFlag IDP_TIMER_CAT(idp_timer_flag_, __LINE__)(name)
 
 struct HipError : std::runtime_error {
     int code;
@@ -49,6 +65,24 @@ struct HipError : std::runtime_error {
 
 inline int grid_for(long n, int block) { return (int)((n + block - 1) / block); }
 
+// Device bytes held by the library's buffers, process-wide (every context),
+// and the most ever held at once: what a serving loop budgets against.
+struct DevMem {
+    std::atomic<long long> in_use{0}, peak{0};
+    void add(long long b)
+    {
+        const long long now = in_use.fetch_add(b) + b;
+        long long pk = peak.load();
+        while (now > pk && !peak.compare_exchange_weak(pk, now)) {
+        }
+    }
+};
+inline DevMem& dev_mem()
+{
+    static DevMem m;
+    return m;
+}
+
 // Growable device buffer (capacity only grows; contents not preserved on grow
 // unless keep=true).
 template <class T>
@@ -61,7 +95,10 @@ struct DBuf {
     ~DBuf() { release(); }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p) {
+            (void)hipFree(p);
+            dev_mem().add(-(long long)(cap * sizeof(T)));
+        }
         p = nullptr;
         cap = 0;
     }
@@ -72,10 +109,12 @@ struct DBuf {
         nc = std::max(nc, cap + cap / 2);
         T* q = nullptr;
         IDP_HIP(hipMalloc(&q, nc * sizeof(T)));
+        dev_mem().add((long long)(nc * sizeof(T)));
         if (keep && p && cap) IDP_HIP(hipMemcpyAsync(q, p, cap * sizeof(T), hipMemcpyDeviceToDevice, s));
         if (p) {
             if (keep) IDP_HIP(hipStreamSynchronize(s));
             IDP_HIP(hipFree(p));
+            dev_mem().add(-(long long)(cap * sizeof(T)));
         }
         p = q;
         cap = nc;

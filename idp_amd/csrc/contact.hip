@@ -19,10 +19,10 @@
 // integer keys, so the constraint order is deterministic; CCD candidates need
 // no order (the step is a minimum).
 #include "common.hpp"
+#include "prim.hpp"
 #include "geom_dev.hpp"
 #include "kernels.hpp"
 
-#include <hipcub/hipcub.hpp>
 
 namespace idp {
 
@@ -121,17 +121,25 @@ __global__ void __launch_bounds__(256) k_edge_len_parts(const int* __restrict__ 
     if (threadIdx.x == 0) parts[blockIdx.x] = s;
 }
 
-// chunked partial sums of |p| over boundary nodes (x, y, z in order); the
-// chunks index the flattened (node, component) sequence of each instance
-__global__ void __launch_bounds__(256) k_psize_parts(const int* __restrict__ lo, const int* __restrict__ hi,
-                                                     const int* __restrict__ bn, const double* __restrict__ p,
-                                                     double* __restrict__ parts)
+// sum of |p| over an instance's boundary nodes, x, y, z of each node in node
+// order, one add after another: the order of the reference's loop
+// (SPATIAL_HASH.h:466-474), so the span cap rounds as it does there.  One
+// thread per instance; the loads run ahead of the dependent adds.
+__global__ void k_psize_seq(int B, const int* __restrict__ nbeg, const int* __restrict__ bn,
+                            const double* __restrict__ p, double* __restrict__ out,
+                            const unsigned char* __restrict__ active)
 {
-    __shared__ double sh[256];
-    const int b = lo[blockIdx.x], e = hi[blockIdx.x], i = b + threadIdx.x;
-    const double a = i < e ? fabs(p[3 * bn[i / 3] + (i % 3)]) : 0.0;
-    const double s = chunk_sum_lds(a, sh, e - b);
-    if (threadIdx.x == 0) parts[blockIdx.x] = s;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B || (active && !active[i])) return;
+    double ps = 0.0;
+#pragma unroll 8
+    for (int k = nbeg[i]; k < nbeg[i + 1]; ++k) {
+        const int v = bn[k];
+        ps += fabs(p[3 * v]);
+        ps += fabs(p[3 * v + 1]);
+        ps += fabs(p[3 * v + 2]);
+    }
+    out[i] = ps;
 }
 
 // min / max of positions (and of x + alpha_i p when p is given) over the
@@ -976,8 +984,7 @@ __global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, 
 // span cap (SPATIAL_HASH.h:466-482), one thread per instance; also seeds the
 // instance's ACCD bound
 __global__ void k_ccd_cap(int B, const double* __restrict__ eparts, const int* __restrict__ epbeg,
-                          const int* __restrict__ ebeg, const double* __restrict__ pparts,
-                          const int* __restrict__ ppbeg, const int* __restrict__ nbeg,
+                          const int* __restrict__ ebeg, const double* __restrict__ psum, const int* __restrict__ nbeg,
                           const double* __restrict__ alpha_in, double* __restrict__ alpha_out,
                           unsigned long long* __restrict__ amin, const unsigned char* __restrict__ active)
 {
@@ -987,8 +994,7 @@ __global__ void k_ccd_cap(int B, const double* __restrict__ eparts, const int* _
     for (int c = epbeg[i]; c < epbeg[i + 1]; ++c) vs += eparts[c];
     const long ne = ebeg[i + 1] - ebeg[i], nb = nbeg[i + 1] - nbeg[i];
     vs = ne ? vs / (double)ne : 1.0;
-    for (int c = ppbeg[i]; c < ppbeg[i + 1]; ++c) ps += pparts[c];
-    ps /= (double)(nb * 3);
+    ps = psum[i] / (double)(nb * 3);
     double a = alpha_in[i];
     const double span = a * ps / vs;
     if (span > 1) a /= span;
@@ -1061,34 +1067,6 @@ namespace {
 
 constexpr int WALK_G = 8;                        // lanes per query
 constexpr unsigned long long DENSE_MAX = 1ull << 25;  // max record keys of a dense cell table
-
-template <class T>
-void excl_scan(hipStream_t s, DBuf<unsigned char>& tmp, const T* in, T* out, int n)
-{
-    size_t bytes = 0;
-    IDP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, s));
-    tmp.ensure(bytes + 16, s);
-    IDP_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, in, out, n, s));
-}
-
-template <class K, class V>
-void sort_pairs(hipStream_t s, DBuf<unsigned char>& tmp, const K* kin, K* kout, const V* vin, V* vout, long n,
-                int end_bit)
-{
-    size_t bytes = 0;
-    IDP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, (int)n, 0, end_bit, s));
-    tmp.ensure(bytes + 16, s);
-    IDP_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, kin, kout, vin, vout, (int)n, 0, end_bit, s));
-}
-
-template <class K>
-void sort_keys(hipStream_t s, DBuf<unsigned char>& tmp, const K* kin, K* kout, long n, int end_bit)
-{
-    size_t bytes = 0;
-    IDP_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, kin, kout, (int)n, 0, end_bit, s));
-    tmp.ensure(bytes + 16, s);
-    IDP_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, bytes, kin, kout, (int)n, 0, end_bit, s));
-}
 
 unsigned read_u(hipStream_t s, const unsigned* p)
 {
@@ -1370,18 +1348,18 @@ void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const do
     AccdLane* res_out = (AccdLane*)w.accd_res[w.res_cur ^ 1].ensure(RES_CAP * sizeof(AccdLane), s);
     unsigned char* pend = w.pend.ensure(B + 1, s);
     // span cap (mean edge length and mean |p| per instance), kept on the device
-    double* ep = w.dparts.ensure(I.echunk.nblk + I.pchunk.nblk + 2, s);
+    double* ep = w.dparts.ensure(I.echunk.nblk + B + 2, s);
     double* pp = ep + I.echunk.nblk + 1;
     IDP_LAUNCH(k_edge_len_parts, I.echunk.nblk, 256, 0, s, I.echunk.lo.p, I.echunk.hi.p, (const int2*)m.be.p, X, ep);
-    IDP_LAUNCH(k_psize_parts, I.pchunk.nblk, 256, 0, s, I.pchunk.lo.p, I.pchunk.hi.p, m.bn.p, P, pp);
+    IDP_LAUNCH(k_psize_seq, grid_for(B, 64), 64, 0, s, B, I.d_nbeg.p, m.bn.p, P, pp, active);
     double* ain = (double*)w.scal2.ensure(2 * (size_t)B + 4, s);
     IDP_HIP(hipMemcpyAsync(ain, alpha_io, B * sizeof(double), hipMemcpyHostToDevice, s));
     double* dalpha = w.ialpha.ensure(B, s);
     unsigned long long* amin = w.iamin.ensure(B, s);
     unsigned* cnt = w.cnt.ensure(6, s);
     unsigned long long* stepstat = (unsigned long long*)w.scal.ensure(6, s) + 2;  // total, max, capped
-    IDP_LAUNCH(k_ccd_cap, grid_for(B, 64), 64, 0, s, B, ep, I.echunk.pbeg.p, I.d_ebeg.p, pp, I.pchunk.pbeg.p,
-               I.d_nbeg.p, ain, dalpha, amin, active);
+    IDP_LAUNCH(k_ccd_cap, grid_for(B, 64), 64, 0, s, B, ep, I.echunk.pbeg.p, I.d_ebeg.p, pp, I.d_nbeg.p, ain, dalpha,
+               amin, active);
     IDP_LAUNCH(k_zero_u32, 1, 64, 0, s, cnt, 6);
     IDP_LAUNCH(k_zero_u64, 1, 64, 0, s, stepstat, 3);
     IDP_LAUNCH(k_zero_u8, grid_for(B, 256), 256, 0, s, pend, B);

@@ -1,5 +1,7 @@
 // Device data structures and kernel launchers shared by the host code.
 #pragma once
+#include <array>
+#include <vector>
 #include "common.hpp"
 
 namespace idp {
@@ -59,7 +61,6 @@ struct InstDev {
     DBuf<int> vinst;                                 // vertex -> instance
     DBuf<int> d_vbeg, d_nbeg, d_ebeg, d_fbeg;        // device copies
     SegDev echunk;   // boundary edges, chunks of 256 (mean edge length)
-    SegDev pchunk;   // 3 x boundary nodes, chunks of 256 (mean |p|)
     SegDev vblk;     // vertices, blocks of RB (bounding box, inertia energy)
     SegDev nblk;     // boundary nodes, blocks of RB (swept bounding box)
     SegDev tblk;     // tets, blocks of RB (elastic energy)
@@ -96,6 +97,20 @@ struct MeshDev {
     long nnzb_e = 0;
     InstDev inst;                    // instance layout (one instance for a plain mesh)
 };
+
+// device surface extraction work buffers
+struct SurfWork {
+    DBuf<unsigned> cnt;
+    DBuf<unsigned long long> k0, k1, v0, v1;
+    DBuf<unsigned char> tmp, node;
+    DBuf<int> tri, edge, flag, pos, bn;
+};
+// Boundary nodes / edges / triangles of a tet mesh (tets and X on the
+// device) in the order of Find_Surface_TriMesh + Find_Surface_Primitives
+// (MESHIO.h:486-545, 769-835), returned on the host.  false when the mesh is
+// beyond the key widths (more than 2^21 vertices or 1.4M boundary triangles).
+bool surface_device(hipStream_t s, SurfWork& w, int nV, const double* X, int nT, const int* tets,
+                    std::vector<int>& bn, std::vector<std::array<int, 2>>& be, std::vector<std::array<int, 3>>& bt);
 
 struct ContactWork {
     DBuf<unsigned char> tmp;
@@ -155,9 +170,14 @@ void launch_min_dist2(hipStream_t s, long n, const int* cs, const double* X, con
                       unsigned long long* dmin);
 void launch_barrier_grad(hipStream_t s, long n, const int* cs, const double* X, const double* X0,
                          const BarrierParams& bp, double* gout, int* err);
+// perm (n ints) and kcnt (16 unsigned), both device scratch: the launch first groups the
+// constraints by kind (both null: one constraint per lane in index order)
+// form (90 n doubles) and form_ok (n bytes): device scratch of the two stages; team: lanes per
+// constraint of the projection stage (8, 16, 32, 64; results do not depend on it)
 void launch_barrier_hess(hipStream_t s, long n, const int* cs, const double* X, const double* X0,
                          const BarrierParams& bp, int project_spd, int mode, double* out, int* vid, int* err,
-                         int* counts = nullptr);
+                         double* form, unsigned char* form_ok, int* counts = nullptr, int* perm = nullptr,
+                         unsigned* kcnt = nullptr, int team = 16);
 void launch_constraint_vids(hipStream_t s, long n, const int* cs, int* vid, int* nv);
 
 // contact.hip

@@ -12,10 +12,10 @@
 // preconditioner; its dot products are deterministic two-level reductions
 // whose final sum is recomputed by each consumer block.
 #include "common.hpp"
+#include "prim.hpp"
 #include "kernels.hpp"
 #include "linalg.hpp"
 
-#include <hipcub/hipcub.hpp>
 
 namespace idp {
 
@@ -1100,22 +1100,6 @@ __global__ void __launch_bounds__(RB) k_dot_parts(long n, const double* __restri
 // host side
 // ---------------------------------------------------------------------------
 namespace {
-template <class K, class V>
-void sort_pairs(hipStream_t s, DBuf<unsigned char>& tmp, const K* kin, K* kout, const V* vin, V* vout, long n,
-                int end_bit)
-{
-    size_t bytes = 0;
-    IDP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, (int)n, 0, end_bit, s));
-    tmp.ensure(bytes + 16, s);
-    IDP_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, kin, kout, vin, vout, (int)n, 0, end_bit, s));
-}
-void excl_scan_u(hipStream_t s, DBuf<unsigned char>& tmp, const unsigned* in, unsigned* out, int n)
-{
-    size_t bytes = 0;
-    IDP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, s));
-    tmp.ensure(bytes + 16, s);
-    IDP_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, in, out, n, s));
-}
 unsigned read_u(hipStream_t s, const unsigned* p)
 {
     unsigned v;
@@ -1143,7 +1127,7 @@ void contact_structure(LinWork& L, hipStream_t s, int nV, long ncs, const int* c
     unsigned* off = L.u1.ensure(ncs + 1, s);
     IDP_LAUNCH(k_contact_nv2, grid_for(ncs, 256), 256, 0, s, ncs, nv, cnt);
     IDP_HIP(hipMemsetAsync(cnt + ncs, 0, sizeof(unsigned), s));
-    excl_scan_u(s, L.tmp, cnt, off, (int)ncs + 1);
+    excl_scan<unsigned>(s, L.tmp, cnt, off, (int)ncs + 1);
     unsigned nrec = read_u(s, off + ncs);
     unsigned long long V = (unsigned long long)nV;
     unsigned long long* k0 = L.key0.ensure(nrec, s);
@@ -1155,7 +1139,7 @@ void contact_structure(LinWork& L, hipStream_t s, int nV, long ncs, const int* c
     unsigned* head = L.u0.ensure(nrec + 1, s);
     unsigned* hs = L.u1.ensure(nrec + 1, s);
     IDP_LAUNCH(k_run_heads, grid_for(nrec + 1, 256), 256, 0, s, nrec, k1, head);
-    excl_scan_u(s, L.tmp, head, hs, (int)nrec + 1);
+    excl_scan<unsigned>(s, L.tmp, head, hs, (int)nrec + 1);
     unsigned nb = read_u(s, hs + nrec);
     unsigned* seg = L.seg.ensure(nb + 1, s);
     int* crow = L.crow.ensure(nb + 1, s);
@@ -1229,10 +1213,7 @@ void build_inst_layout(MeshDev& m, const std::vector<long>& vbeg, const std::vec
     up(I.d_nbeg, nbeg);
     up(I.d_ebeg, ebeg);
     up(I.d_fbeg, fbeg);
-    std::vector<long> p3(nbeg.size());
-    for (size_t i = 0; i < nbeg.size(); ++i) p3[i] = 3 * nbeg[i];
     I.echunk.build(ebeg, 256, s);
-    I.pchunk.build(p3, 256, s);
     I.vblk.build(vbeg, RB, s);
     I.nblk.build(nbeg, RB, s);
     I.tblk.build(tbeg, RB, s);

@@ -30,8 +30,8 @@ class TetAnimFix:
         self.pn_iters = 0
 
     def set_target(self, target):
-        X, _ = self.ctx.get_state()
-        self.ctx.set_state(None, (np.asarray(target) - X) / self.h)
+        """Load_Velocity_X0: v = (target - x) / h, formed on the device."""
+        self.ctx.load_velocity(target, self.h)
 
     def advance_one_time_step(self, max_newton=0, verbose=0, body_force=None, flow=False):
         """Advance_One_Step_IE_Hinge (flow=False) or Advance_One_Step_IE_Flow (flow=True)."""

@@ -117,6 +117,12 @@ int idp_ctx_create(int device, idp_ctx** out);
 int idp_ctx_destroy(idp_ctx* ctx);
 const char* idp_last_error(const idp_ctx* ctx);
 int idp_device_count(int* n);
+/* Device memory held by the library's buffers in this process, over every
+ * context and GPU (no counterpart in the reference): bytes now and the most
+ * held at once.  Buffers grow to their working sizes on first use and are
+ * kept; a serving loop measures a slot's footprint from the difference
+ * before and after a context's first solve. */
+int idp_device_memory(long long* in_use, long long* peak);
 /* Profiling (no counterpart in the reference, which only has wall-clock
  * TIMER_FLAG scopes): on = 1 resets the counters and records every kernel
  * family (an event pair around each launch, constraint-kind counters in the
@@ -134,6 +140,11 @@ int idp_profile_read(idp_ctx* ctx, idp_profile* out);
  * block row's sum, so one G serves every context of a process. */
 int idp_set_launch_config(idp_ctx* ctx, int spmv_shape, int upd_waves);
 int idp_get_launch_config(idp_ctx* ctx, int* spmv_shape, int* upd_waves);
+/* Lanes per constraint (8, 16, 32 or 64; default 16) in the projection stage
+ * of the barrier Hessian (no counterpart in the reference; set by the
+ * autotuner).  No sum depends on it, so results do not either. */
+int idp_set_barrier_team(idp_ctx* ctx, int lanes);
+int idp_get_barrier_team(idp_ctx* ctx, int* lanes);
 
 /* ---- host-side set-up helpers (no device work) ------------------------- */
 /* Compute_Vol_And_Inv_Basis (Library/FEM/DEFORMATION_GRADIENT.h:39-76),
@@ -167,10 +178,21 @@ int idp_set_dbc(idp_ctx* ctx, int n, const int* idx);
 int idp_set_dbc_targets(idp_ctx* ctx, int n, const int* idx, const double* targets);
 int idp_set_state(idp_ctx* ctx, const double* X, const double* v);
 int idp_get_state(idp_ctx* ctx, double* X, double* v);
+/* Load_Velocity_X0 (Library/Utils/MESHIO.h:987-1005): v = (X_target - x) / h
+ * on the device, from the context's current positions x.  X_target (3 nV,
+ * host) goes through a page-locked staging buffer held by the context; the
+ * call returns once it is copied there (the copy and the kernel run on the
+ * context's stream). */
+int idp_load_velocity(idp_ctx* ctx, const double* X_target, double h);
 /* device-to-device save/restore of (x, v, kappa) for repeated solves */
 int idp_snapshot_state(idp_ctx* ctx);
 int idp_restore_state(idp_ctx* ctx);
 int idp_surface_sizes(idp_ctx* ctx, int* nb, int* ne, int* nf);
+/* The boundary primitives idp_set_mesh extracted on the device, in the order
+ * of Find_Surface_TriMesh<T,false> + Find_Surface_Primitives_And_Compute_Area
+ * (Library/Utils/MESHIO.h:486-545, 769-835): nodes (nb), edges (2 ne),
+ * triangles (3 nf); sizes from idp_surface_sizes; any pointer may be null. */
+int idp_get_surface(idp_ctx* ctx, int* bnode, int* bedge, int* btri);
 
 /* ---- the hot path ------------------------------------------------------ */
 /* Advance_One_Step_IE_Discrete_Shell<double,3,false,false,flow> for tet

@@ -63,3 +63,23 @@ def test_tune_measures_shapes_then_widths(monkeypatch):
     assert calls[: 2 * n_shapes] == [(s, autotune.DEFAULT[1]) for s in range(n_shapes)] * 2
     assert calls[2 * n_shapes:] == [(1, w) for w in autotune.UPD_WAVES if w != autotune.DEFAULT[1]] * 2
     assert len(rows) == len(calls) and {r["workload"] for r in rows} == {"a", "b"}
+
+
+def test_choose_team_sums_workloads_and_prefers_the_default():
+    rows = [{"workload": "a", "barrier_team": 8, "barrier_hess_us": 30.0},
+            {"workload": "a", "barrier_team": 16, "barrier_hess_us": 20.0},
+            {"workload": "b", "barrier_team": 8, "barrier_hess_us": 5.0},
+            {"workload": "b", "barrier_team": 16, "barrier_hess_us": 20.0}]
+    assert autotune.choose_team(rows) == 8
+    assert autotune.choose_team([]) == autotune.DEFAULT_TEAM
+    tie = [{"workload": "a", "barrier_team": t, "barrier_hess_us": 1.0} for t in autotune.TEAMS]
+    assert autotune.choose_team(tie) == autotune.DEFAULT_TEAM
+
+
+def test_tuned_barrier_team_parsing(tmp_path):
+    p = tmp_path / "t.json"
+    p.write_text(json.dumps({"spmv_shape": 1, "upd_waves": 2, "barrier_team": 32}))
+    assert capi.tuned_barrier_team(str(p)) == 32
+    p.write_text(json.dumps({"barrier_team": 12}))
+    assert capi.tuned_barrier_team(str(p)) is None
+    assert capi.tuned_barrier_team(str(tmp_path / "absent.json")) is None
