@@ -594,9 +594,14 @@ struct PcgArgs {
     double* scal;     // per instance, 3 each: |b|^2, final |r|^2, iterations
     double rtol2;
     int maxit;
+    // one instance with many blocks: the sums of the partials are formed once
+    // per kernel boundary by k_pcg_red_spmv / k_pcg_red_upd (r.z, r.r, the
+    // previous r.z or |b|^2; p.q, r.z) instead of by every block (null)
+    double* red;
 };
 
 constexpr int PCG_ROW_THREADS = 256;         // row threads of a SpMV block (4 wavefronts), 256 / G rows
+constexpr int PCG_PREREDUCE_BLOCKS = 1024;   // update blocks of one instance from which partials are pre-reduced
 constexpr int PCG_UPD_BLOCK = 64;            // rows of an update block: one wavefront, reductions need no barrier
 constexpr int PCG_UPD_WAVES_MAX = 8;         // update blocks per workgroup (independent wavefronts), at most
 // SpMV launch shapes: G lanes per block row, scalars in wavefront 0 (before
@@ -908,8 +913,15 @@ __global__ void __launch_bounds__(PCG_ROW_THREADS + (SW ? 64 : 0)) k_pcg_spmv(Pc
         const int pb = A.bpbeg[inst], np = A.bpbeg[inst + 1] - pb;
         // r.z and r.r of this iteration, and r.z of the previous one (|b|^2 at it = 0)
         double rz, rr, third;
-        wave_sum_partials3(A.part_rz + par * A.nbB + pb, A.part_rr + par * A.nbB + pb,
-                           it > 0 ? A.part_rz + (par ^ 1) * A.nbB + pb : A.part_rr + A.nbB + pb, np, rz, rr, third);
+        if (A.red) {
+            rz = A.red[0];
+            rr = A.red[1];
+            third = A.red[2];
+        } else {
+            wave_sum_partials3(A.part_rz + par * A.nbB + pb, A.part_rr + par * A.nbB + pb,
+                               it > 0 ? A.part_rz + (par ^ 1) * A.nbB + pb : A.part_rr + A.nbB + pb, np, rz, rr,
+                               third);
+        }
         const double rzo = it > 0 ? third : 1.0;
         const double bbp = it == 0 ? third : 0.0;
         if ((threadIdx.x & 63) == 0) {
@@ -966,6 +978,72 @@ __global__ void __launch_bounds__(PCG_ROW_THREADS + (SW ? 64 : 0)) k_pcg_spmv(Pc
     if (threadIdx.x == 0) A.part_pq[bid] = a;
 }
 
+// Sums of n partials by one workgroup of PCG_RED threads in a fixed order:
+// thread t adds i = t, t + PCG_RED, ... (loads issued PCG_RED_NL at a time),
+// then the workgroup's sums are added in thread order.
+constexpr int PCG_RED = 256;
+constexpr int PCG_RED_NL = 8;
+
+template <int K>
+__device__ inline void block_sums_partials(const double* const* __restrict__ p, int n, double* out)
+{
+    __shared__ double sh[K][PCG_RED];
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = 0.0;
+    for (int base = 0; base < n; base += PCG_RED * PCG_RED_NL) {
+        double v[K][PCG_RED_NL];
+#pragma unroll
+        for (int j = 0; j < PCG_RED_NL; ++j) {
+            const int i = base + (int)threadIdx.x + PCG_RED * j;
+#pragma unroll
+            for (int k = 0; k < K; ++k) v[k][j] = i < n ? p[k][i] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < PCG_RED_NL; ++j)
+#pragma unroll
+            for (int k = 0; k < K; ++k) acc[k] += v[k][j];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) sh[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    if (threadIdx.x < K) {
+        double a = 0.0;
+        for (int t = 0; t < PCG_RED; ++t) a += sh[threadIdx.x][t];
+        out[threadIdx.x] = a;
+    }
+}
+
+// the sums the SpMV of iteration it needs (one instance spanning the launch)
+__global__ void __launch_bounds__(PCG_RED) k_pcg_red_spmv(PcgArgs A, int it_call)
+{
+    const int it = it_call + A.ident - 1;
+    const int par = it & 1;
+    const int pb = A.bpbeg[0], np = A.bpbeg[1] - pb;
+    const double* p[3] = {A.part_rz + par * A.nbB + pb, A.part_rr + par * A.nbB + pb,
+                          it > 0 ? A.part_rz + (par ^ 1) * A.nbB + pb : A.part_rr + A.nbB + pb};
+    block_sums_partials<3>(p, np, A.red);
+}
+
+// the sums the update of iteration it needs
+__global__ void __launch_bounds__(PCG_RED) k_pcg_red_upd(PcgArgs A, int it_call)
+{
+    const int it = it_call + A.ident - 1;
+    const int par = it & 1;
+    const int pb = A.bpbeg[0];
+    __shared__ double o[2];
+    const double* pa[1] = {A.part_pq + A.apbeg[0]};
+    const double* pr[1] = {A.part_rz + par * A.nbB + pb};
+    block_sums_partials<1>(pa, A.apbeg[1] - A.apbeg[0], o);
+    __syncthreads();
+    block_sums_partials<1>(pr, A.bpbeg[1] - pb, o + 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        A.red[3] = o[0];
+        A.red[4] = o[1];
+    }
+}
+
 // Update step of iteration it: alpha = r.z / p.q; x += alpha p; r -= alpha q;
 // z = P r (cluster members' residuals recomputed from the unwritten half of
 // r); partials of the new r.z and r.r.  Every load that does not depend on
@@ -1005,8 +1083,13 @@ __global__ void __launch_bounds__(PCG_UPD_BLOCK * PCG_UPD_WAVES_MAX) k_pcg_updat
     // (a stopped instance's sums are formed and dropped)
     const int pb = A.bpbeg[inst];
     double pq, rz;
-    wave_sum_partials2<8, 4>(A.part_pq + A.apbeg[inst], A.apbeg[inst + 1] - A.apbeg[inst],
-                              A.part_rz + par * A.nbB + pb, A.bpbeg[inst + 1] - pb, pq, rz);
+    if (A.red) {
+        pq = A.red[3];
+        rz = A.red[4];
+    } else {
+        wave_sum_partials2<8, 4>(A.part_pq + A.apbeg[inst], A.apbeg[inst + 1] - A.apbeg[inst],
+                                  A.part_rz + par * A.nbB + pb, A.bpbeg[inst + 1] - pb, pq, rz);
+    }
     if (f != 0) return;  // uniform over the wavefront
     if (!(pq > 0.0)) {
         if (bid == pb && lane == 0) {
@@ -1367,6 +1450,15 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
     A.scal = L.scal.ensure(3 * (size_t)B + 3, s);
     A.rtol2 = rtol * rtol;
     A.maxit = maxit;
+    // one instance with many update blocks: reduce the partials once per
+    // kernel boundary instead of in every block (IDP_PCG_PREREDUCE = 0 / 1
+    // overrides the size rule)
+    {
+        const char* e = std::getenv("IDP_PCG_PREREDUCE");
+        const int force = e ? std::atoi(e) : -1;
+        const bool big = A.nbB >= PCG_PREREDUCE_BLOCKS;
+        A.red = (B == 1 && A.ident && (force < 0 ? big : force > 0)) ? L.red.ensure(8, s) : nullptr;
+    }
     if (warm && nA) {
         if (shape.g == 4) IDP_LAUNCH(k_pcg_warm<4>, nA, PCG_ROW_THREADS, 0, s, A);
         else if (shape.g == 8) IDP_LAUNCH(k_pcg_warm<8>, nA, PCG_ROW_THREADS, 0, s, A);
@@ -1407,8 +1499,10 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
             const bool sample = P && (it & 7) == 3;
             hipEvent_t e0 = nullptr, e1 = nullptr;
             int t = sample ? P->pair(e0, e1) : -1;
+            if (A.red) IDP_LAUNCH(k_pcg_red_spmv, 1, PCG_RED, 0, s, A, it);
             spmv(it, e0, e1);
             if (sample) P->end_pair(IDP_K_PCG_SPMV, t, spmv_bytes * frac, it);
+            if (A.red) IDP_LAUNCH(k_pcg_red_upd, 1, PCG_RED, 0, s, A, it);
             t = sample ? P->pair(e0, e1) : -1;
             IDP_LAUNCH_TIMED(k_pcg_update, (nB + uw - 1) / uw, PCG_UPD_BLOCK * uw, 0, s, e0, e1, A, it);
             if (sample) P->end_pair(IDP_K_PCG_UPDATE, t, upd_bytes * frac, it);

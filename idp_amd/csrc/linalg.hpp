@@ -27,7 +27,7 @@ struct LinWork {
     DBuf<int> erow;  // row of every elastic block
     DBuf<double> evals, cvals;
     // PCG
-    DBuf<double> pinv, r, z, p, p1, q, part0, part1, part2, part3, part_bx, scal, scal2;
+    DBuf<double> pinv, r, z, p, p1, q, part0, part1, part2, part3, part_bx, scal, scal2, red;
     DBuf<int> err;
     // cluster preconditioner
     DBuf<int> cl_best, cl_root, cl_size, cl_root2, cl_cnt, cl_mem, cl_cm4;

@@ -326,3 +326,31 @@ def test_advance_with_moving_dbc_matches_oracle(push, gap):
         assert err < 2e-3
         move = np.abs(tgt - sc.X[idx]).max()
         assert np.abs(X[idx] - tgt).max() <= 0.01 * move
+
+
+def test_newton_direction_with_prereduced_partials(monkeypatch):
+    """The PCG sums formed once per kernel boundary (the path of meshes with
+    at least 1024 update blocks, forced here on the bend bar) reach the same
+    direction as the sparse direct solve."""
+    import scipy.sparse as sps
+    import scipy.sparse.linalg as spla
+
+    monkeypatch.setenv("IDP_PCG_PREREDUCE", "1")
+    sc = scenes.config1()
+    m = pyoracle.Mesh(sc.X, sc.tets, sc.density, sc.E, sc.nu)
+    dHat2, kappa = pyoracle.initialize_oipc(sc.dhat, m.mass)
+    vol, IB, lam, mu, mass = capi.fem_init(sc.X, sc.tets, sc.density, sc.E, sc.nu)
+    ctx = capi.Context(0)
+    try:
+        ctx.set_mesh(sc.X, sc.tets, IB, vol, lam, mu, mass)
+        X = (sc.X + 0.4 * (sc.target - sc.X)).ravel()
+        say("[bend] newton direction, pre-reduced PCG partials")
+        g, sol, rel, it = ctx.newton_direction(X, sc.target.ravel(), sc.h, dHat2, kappa[0], pcg_rtol=1e-12)
+        assert it > 0 and rel <= 1e-12
+        rp, col, val = ctx.get_bsr()
+        n = rp.size - 1
+        A = sps.bsr_matrix((val, col, rp), shape=(3 * n, 3 * n)).tocsc()
+        d = spla.spsolve(A, -g)
+        assert np.linalg.norm(sol - d) <= 1e-8 * np.linalg.norm(d)
+    finally:
+        ctx.close()
