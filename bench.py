@@ -558,6 +558,8 @@ def bench_plain(a):
                        "launch_config": list(d.ctx.launch_config()),
                        "steps_requested": a.steps, "warmup_requested": a.warmup, "budget_s": a.budget_s},
             "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+            # wall time per stage of each timed solve (host clock around each stage, ms)
+            "stage_ms": state.get("stages", []),
         }
         out.update(state.get("extra", {}))
         if partial:
@@ -599,6 +601,8 @@ def bench_plain(a):
         state["its"].append(it)
         state["pcg"].append(st["pcg_iters"])
         state["capped"].append(st["accd_capped"])
+        state.setdefault("stages", []).append({k2[3:]: round(st[k2], 1) for k2 in (
+            "ms_constraint_set", "ms_ccd", "ms_energy", "ms_gradient", "ms_hessian", "ms_solve", "ms_total")})
         log(f"[bench] step {k}: {it} Newton iterations, {st['pcg_iters']} PCG iterations, "
             f"{st['n_constraints']} constraints, {st['accd_capped']} capped ACCD pairs, {t:.2f} s")
     prof_t = d.ctx.profile_read()

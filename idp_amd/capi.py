@@ -46,7 +46,8 @@ EXPORTED = [
     "idp_profile_enable", "idp_profile_read", "idp_set_mesh_batch", "idp_batch_size", "idp_advance_batch",
     "idp_serve_configure", "idp_serve_start", "idp_serve_iterate", "idp_serve_result", "idp_set_launch_config",
     "idp_get_launch_config", "idp_set_accd_max_steps", "idp_barrier_energy_noalloc", "idp_get_surface", "idp_device_memory", "idp_set_barrier_team",
-    "idp_get_barrier_team", "idp_load_velocity",
+    "idp_get_barrier_team", "idp_load_velocity", "idp_shell_init", "idp_shell_surface", "idp_set_mesh_shell",
+    "idp_normal_flow_force", "idp_set_normal_flow", "idp_newton_direction_flow",
 ]
 
 # PCG preconditioner for every solve of this process: 0 cluster block Jacobi
@@ -220,6 +221,13 @@ def lib():
         L.idp_set_launch_config.argtypes = [vp, C.c_int, C.c_int]
         L.idp_get_launch_config.argtypes = [vp, ip, ip]
         L.idp_profile_read.argtypes = [vp, C.POINTER(Profile)]
+        L.idp_shell_init.argtypes = [C.c_int, dp, C.c_int, ip, C.c_double, C.c_double, C.c_double, C.c_double, ip,
+                                     ip, dp, dp, dp, dp]
+        L.idp_shell_surface.argtypes = [C.c_int, dp, C.c_int, ip, C.c_int, ip, ip, ip, ip, ip, ip, ip]
+        L.idp_set_mesh_shell.argtypes = [vp, C.c_int, dp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int, ip, dp]
+        L.idp_normal_flow_force.argtypes = [vp, dp, C.c_double, dp]
+        L.idp_set_normal_flow.argtypes = [vp, C.c_double]
+        L.idp_newton_direction_flow.argtypes = L.idp_newton_direction.argtypes
         _lib = L
     return _lib
 
@@ -282,6 +290,41 @@ def surface(X, tets):
     bt, btp = _out_i(12 * nT)
     nb, ne, nf = C.c_int(), C.c_int(), C.c_int()
     rc = lib().idp_surface(nV, Xp, nT, tp, C.byref(nb), C.byref(ne), C.byref(nf), bnp, bep, btp)
+    if rc != IDP_OK:
+        raise IdpError(rc)
+    return bn[: nb.value].copy(), be[: 2 * ne.value].reshape(-1, 2).copy(), bt[: 3 * nf.value].reshape(-1, 3).copy()
+
+
+def shell_init(X, tris, density, E, nu, thickness, mass=None):
+    """Initialize_Discrete_Shell (host): (kept triangles, mass, vol, lambda, mu);
+    mass accumulates into a copy of the given one (zeros when None)."""
+    X, Xp = _d(X)
+    tris, tp = _i(tris)
+    nV, nE = X.size // 3, tris.size // 3
+    kept, kp = _out_i(3 * nE)
+    m, mp = _d(np.zeros(nV) if mass is None else np.array(mass, dtype=np.float64))
+    vol, vp = _out_d(nE)
+    lam, lp = _out_d(nE)
+    mu, mup = _out_d(nE)
+    n = C.c_int()
+    rc = lib().idp_shell_init(nV, Xp, nE, tp, density, E, nu, thickness, C.byref(n), kp, mp, vp, lp, mup)
+    if rc != IDP_OK:
+        raise IdpError(rc)
+    k = n.value
+    return kept[: 3 * k].reshape(-1, 3).copy(), m, vol[:k].copy(), lam[:k].copy(), mu[:k].copy()
+
+
+def shell_surface(X, tris, tets=None):
+    """Boundary (nodes, edges, triangles) of tets' surface + shell triangles (host)."""
+    X, Xp = _d(X)
+    tris, trp = _i(tris)
+    tets, tp = _i(np.zeros((0, 4)) if tets is None else tets)
+    nV, nT, nE = X.size // 3, tets.size // 4, tris.size // 3
+    bn, bnp = _out_i(nV)
+    be, bep = _out_i(2 * (3 * nE + 12 * nT))
+    bt, btp = _out_i(3 * (nE + 4 * nT))
+    nb, ne, nf = C.c_int(), C.c_int(), C.c_int()
+    rc = lib().idp_shell_surface(nV, Xp, nT, tp, nE, trp, C.byref(nb), C.byref(ne), C.byref(nf), bnp, bep, btp)
     if rc != IDP_OK:
         raise IdpError(rc)
     return bn[: nb.value].copy(), be[: 2 * ne.value].reshape(-1, 2).copy(), bt[: 3 * nf.value].reshape(-1, 3).copy()
@@ -384,6 +427,37 @@ class Context:
         ms, msp = _d(mass)
         self.nV = X.size // 3
         self._check(lib().idp_set_mesh(self._h, self.nV, Xp, t.size // 4, tp, ibp, vp, lp, mp, msp))
+
+    def set_mesh_shell(self, X_rest, mass, tris, tri_vol, tets=None, IB=None, vol=None, lam=None, mu=None):
+        """A mesh of shell triangles (and optionally tets): idp_set_mesh_shell."""
+        X, Xp = _d(X_rest)
+        m, mp = _d(mass)
+        tr, trp = _i(tris)
+        tv, tvp = _d(tri_vol)
+        nT = 0 if tets is None else np.asarray(tets).size // 4
+        args = [None] * 5
+        if nT:  # the arrays stay referenced by these locals until the call returns
+            te, tep = _i(tets)
+            ib, ibp = _d(IB)
+            vo, vop = _d(vol)
+            la, lap = _d(lam)
+            mu_, mup = _d(mu)
+            args = [tep, ibp, vop, lap, mup]
+        self._check(lib().idp_set_mesh_shell(self._h, X.size // 3, Xp, mp, nT, *args, tr.size // 3, trp, tvp))
+        self.nV = X.size // 3
+        self.nT = nT
+
+    def normal_flow_force(self, magnitude, X=None):
+        """Update_Normal_Flow_Neumann on the device at X (None: current x)."""
+        b, bp = _out_d(3 * self.nV)
+        xp = None
+        if X is not None:
+            X, xp = _d(X)
+        self._check(lib().idp_normal_flow_force(self._h, xp, magnitude, bp))
+        return b.reshape(-1, 3)
+
+    def set_normal_flow(self, magnitude):
+        self._check(lib().idp_set_normal_flow(self._h, magnitude))
 
     def set_mesh_batch(self, meshes):
         """meshes: list of (X_rest (V,3), tets (T,4) local ids, IB, vol, lam, mu, mass)."""
@@ -620,7 +694,10 @@ class Context:
         self._check(lib().idp_elastic_hessian_local(self._h, Xp, w, int(project), Hp))
         return H.reshape(nT, 144)
 
-    def newton_direction(self, X, xtilde, h, dHat2, kappa, thickness=0.0, pcg_rtol=1e-10, pcg_maxit=20000):
+    def newton_direction(self, X, xtilde, h, dHat2, kappa, thickness=0.0, pcg_rtol=1e-10, pcg_maxit=20000,
+                         flow=False):
+        """(g, direction, relative residual, PCG iterations) of the Newton system
+        at X; flow: the flow variant's terms (idp_newton_direction_flow)."""
         X, Xp = _d(X)
         xt, xtp = _d(xtilde)
         g, gp = _out_d(X.size)
@@ -628,8 +705,9 @@ class Context:
         rel = C.c_double()
         it = C.c_int()
         opts = SolverOpts(pcg_rtol, pcg_maxit, 0, 0, PRECOND, PCG_COLD, ACCD_BUDGET, PCG_HANDOVER, ACCD_MAX_STEPS)
-        self._check(lib().idp_newton_direction(self._h, Xp, xtp, h, dHat2, kappa, thickness, C.byref(opts), gp, sp,
-                                               C.byref(rel), C.byref(it)))
+        fn = lib().idp_newton_direction_flow if flow else lib().idp_newton_direction
+        self._check(fn(self._h, Xp, xtp, h, dHat2, kappa, thickness, C.byref(opts), gp, sp, C.byref(rel),
+                       C.byref(it)))
         return g, sol, rel.value, it.value
 
     def get_bsr(self):

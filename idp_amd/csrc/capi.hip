@@ -41,7 +41,7 @@ struct idp_ctx {
     // state and work vectors (3 nV)
     DBuf<double> x, v, xt, xn, xprev, sol, rhs, g, tmpv, snap_x, snap_v, hx, hp;
     DBuf<double> gl, gb, Hl, Hb;  // per-tet / per-constraint local terms
-    DBuf<double> part_e, part_i, part_b;
+    DBuf<double> part_e, part_i, part_b, part_f;
     DBuf<int> flags;
     DBuf<unsigned char> one;  // a device byte holding 1 (warm-start mask of a one-mesh solve)
     DBuf<unsigned long long> dmin;
@@ -62,9 +62,14 @@ struct idp_ctx {
     DBuf<double> esum;       // per-instance partial-sum results
     BatchEngine* eng = nullptr;  // batched / served solves
     // Advance_One_Step_IE_Flow: the volumetric (tet) term is left out of the
-    // energy, gradient and Hessian (INC_POTENTIAL.h:53-73, 191-211, 323-339)
+    // energy, gradient and Hessian, and the shell triangles carry the flow
+    // term (INC_POTENTIAL.h:53-73, 191-211, 323-339)
     bool no_elastic = false;
     DBuf<double> bforce;  // body force of the current step (3 nV)
+    // magnitude of the normal-flow force recomputed from x before each step
+    // that is given no body force (0: none), as the shell driver calls
+    // Update_Normal_Flow_Neumann before each step (FEMDiscreteShellBase.py:269-273)
+    double normal_flow = 0.0;
     SurfWork sw;          // device surface extraction
     double* pinned = nullptr;  // page-locked staging of target positions (3 nV)
     size_t pinned_n = 0;
@@ -117,8 +122,8 @@ inline double det3h(const double* m)
 // lexicographic order of the stored vertex triple; edges take the orientation
 // of their first boundary triangle and are listed lexicographically; nodes
 // are the vertices with nonzero accumulated boundary area, ascending.
-void surface_host(int nV, const double* X, int nT, const int* tets, std::vector<int>& bn,
-                  std::vector<std::array<int, 2>>& be, std::vector<std::array<int, 3>>& bt)
+// This part: the triangles.
+void tet_surface_host(int nT, const int* tets, std::vector<std::array<int, 3>>& bt)
 {
     struct Face {
         int s[3];  // sorted vertex set
@@ -161,6 +166,13 @@ void surface_host(int nV, const double* X, int nT, const int* tets, std::vector<
     }
     std::sort(bt.begin(), bt.end());
     bt.erase(std::unique(bt.begin(), bt.end()), bt.end());
+}
+
+// edges and nodes of a boundary triangle list, taken in its order
+// (MESHIO.h:769-835)
+void surface_prims_host(int nV, const double* X, const std::vector<std::array<int, 3>>& bt, std::vector<int>& bn,
+                        std::vector<std::array<int, 2>>& be)
+{
     // edges: first-seen orientation
     std::unordered_map<long long, std::array<int, 2>> em;
     em.reserve(bt.size() * 2);
@@ -187,6 +199,17 @@ void surface_host(int nV, const double* X, int nT, const int* tets, std::vector<
     bn.clear();
     for (int v = 0; v < nV; ++v)
         if (area[v] != 0.0) bn.push_back(v);
+}
+
+// surface primitives of a tet mesh with shell triangles appended after its
+// surface triangles, as Advance_One_Step_IE_Discrete_Shell lists them
+// (Shell/IMPLICIT_EULER.h:233-241); nE = 0 is a plain tet mesh
+void surface_host(int nV, const double* X, int nT, const int* tets, int nE, const int* tris, std::vector<int>& bn,
+                  std::vector<std::array<int, 2>>& be, std::vector<std::array<int, 3>>& bt)
+{
+    tet_surface_host(nT, tets, bt);
+    for (int e = 0; e < nE; ++e) bt.push_back({tris[3 * e], tris[3 * e + 1], tris[3 * e + 2]});
+    surface_prims_host(nV, X, bt, bn, be);
 }
 
 // ---------------------------------------------------------------------------
@@ -241,6 +264,15 @@ void update_cseg(idp_ctx* c)
     c->cseg.build(c->cw.h_cbeg, RB, c->s);
 }
 
+// Shell triangles carry the flow term only: the membrane and hinge-bending
+// energies of the non-flow variant are not built, so a non-flow evaluation of
+// a mesh with shell triangles is refused rather than run without them.
+void shell_terms_check(const idp_ctx* c)
+{
+    if (c->m.nE > 0 && !c->no_elastic)
+        throw std::invalid_argument("shell triangles: only the flow variant's terms are built (flow = 1)");
+}
+
 // Compute_IncPotential per instance: elastic (w = h^2) + inertia + barrier
 // over the current constraint set (barrier parameters from c->bpar)
 void energies(idp_ctx* c, const double* X, bool with_barrier, double h, std::vector<EnergyOut>& out)
@@ -255,13 +287,20 @@ void energies(idp_ctx* c, const double* X, bool with_barrier, double h, std::vec
     const int tok = c->prof.begin(IDP_K_ENERGY);
     double* pe = c->part_e.ensure(I.tblk.nblk + 1, s);
     double* pi = c->part_i.ensure(I.vblk.nblk + 1, s);
+    shell_terms_check(c);
     if (c->no_elastic) IDP_HIP(hipMemsetAsync(pe, 0, (I.tblk.nblk + 1) * sizeof(double), s));
     else launch_tet_energy(s, m, I.tblk, X, h * h, pe, fl);
     launch_inertia_energy(s, I.vblk, X, c->xt.p, m.mass.p, pi);
     const bool bar = with_barrier && c->cw.ncs > 0;
-    double* es = c->esum.ensure(3 * (size_t)B, s);
+    const bool flow = c->no_elastic && m.nE > 0;
+    double* es = c->esum.ensure(4 * (size_t)B, s);
     IDP_LAUNCH(k_seg_sums, B, RB, 0, s, I.tblk.pbeg.p, pe, es);
     IDP_LAUNCH(k_seg_sums, B, RB, 0, s, I.vblk.pbeg.p, pi, es + B);
+    if (flow) {
+        double* pf = c->part_f.ensure(I.sblk.nblk + 1, s);
+        launch_flow_energy(s, m, X, h, pf);
+        IDP_LAUNCH(k_seg_sums, B, RB, 0, s, I.sblk.pbeg.p, pf, es + 3 * B);
+    }
     if (bar) {
         double* pb = c->part_b.ensure(c->cseg.nblk + 1, s);
         launch_barrier_energy(s, c->cseg.nblk, c->cseg.lo.p, c->cseg.hi.p, (const int*)c->cw.cs.p, X, m.X0.p,
@@ -270,16 +309,17 @@ void energies(idp_ctx* c, const double* X, bool with_barrier, double h, std::vec
     }
     c->prof.end(IDP_K_ENERGY, tok);
     std::vector<int> hf(2 * B);
-    std::vector<double> he(3 * B, 0.0);
+    std::vector<double> he(4 * B, 0.0);
     IDP_HIP(hipMemcpyAsync(hf.data(), fl, 2 * B * sizeof(int), hipMemcpyDeviceToHost, s));
     IDP_HIP(hipMemcpyAsync(he.data(), es, (bar ? 3 : 2) * B * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (flow) IDP_HIP(hipMemcpyAsync(he.data() + 3 * B, es + 3 * B, B * sizeof(double), hipMemcpyDeviceToHost, s));
     IDP_HIP(hipStreamSynchronize(s));
     out.resize(B);
     for (int i = 0; i < B; ++i) {
         EnergyOut& o = out[i];
         o.valid = hf[i] == 0;
         o.dist_ok = hf[B + i] == 0;
-        o.E = o.valid ? he[i] + he[B + i] + (bar ? he[2 * B + i] : 0.0) : 0.0;
+        o.E = o.valid ? (he[i] + he[3 * B + i]) + he[B + i] + (bar ? he[2 * B + i] : 0.0) : 0.0;
     }
 }
 
@@ -315,10 +355,12 @@ void gradient(idp_ctx* c, const double* X, double h, bool with_barrier, double* 
     IDP_TIMER_FLAG("Compute_IncPotential_Gradient");
     auto s = c->s;
     auto& m = c->m;
-    double* gl = c->gl.ensure(12 * (size_t)m.nT, s);
+    shell_terms_check(c);
+    double* gl = c->gl.ensure(12 * ((size_t)m.nT + m.nE), s);
     const int tok = c->prof.begin(IDP_K_GRADIENT);
     if (c->no_elastic) IDP_HIP(hipMemsetAsync(gl, 0, 12 * (size_t)m.nT * sizeof(double), s));
     else launch_tet_grad(s, m, X, h * h, gl, act);
+    if (c->no_elastic) launch_flow_grad(s, m, X, h, gl);  // shell triangles: the flow term
     const double* gb = nullptr;
     long ncs = c->cw.ncs;
     if (with_barrier && ncs) {
@@ -342,10 +384,12 @@ void hessian(idp_ctx* c, const double* X, double h, bool with_barrier, const uns
     IDP_TIMER_FLAG("Compute_IncPotential_Hessian");
     auto s = c->s;
     auto& m = c->m;
-    double* Hl = c->Hl.ensure(144 * (size_t)m.nT, s);
+    shell_terms_check(c);
+    double* Hl = c->Hl.ensure(144 * ((size_t)m.nT + m.nE), s);
     int tok = c->prof.begin(IDP_K_TET_HESS);
     if (c->no_elastic) {
         IDP_HIP(hipMemsetAsync(Hl, 0, 144 * (size_t)m.nT * sizeof(double), s));
+        launch_flow_hess(s, m, h, Hl);  // shell triangles: the flow term
     } else {
         IDP_TIMER_FLAG("Compute_Volumetric_Elasticity_Hessian");
         launch_tet_hess(s, m, X, h * h, 1, Hl, act);
@@ -1008,11 +1052,17 @@ namespace {
 // tbeg; tets in global vertex ids, instance-major).
 int setup_mesh(idp_ctx* c, int nV, const double* X_rest, int nT, const int* tets, const double* IB,
                const double* vol, const double* lambda, const double* mu, const double* mass,
-               const std::vector<long>& vbeg, const std::vector<long>& tbeg)
+               const std::vector<long>& vbeg, const std::vector<long>& tbeg, int nE = 0, const int* tris = nullptr,
+               const double* tvol = nullptr)
 {
-        if (nV <= 0 || nT <= 0 || !X_rest || !tets || !IB || !vol || !lambda || !mu || !mass) return IDP_E_INVALID;
+        if (nV <= 0 || nT < 0 || nE < 0 || nT + (long)nE <= 0 || !X_rest || !mass) return IDP_E_INVALID;
+        if (nT > 0 && (!tets || !IB || !vol || !lambda || !mu)) return IDP_E_INVALID;
+        if (nE > 0 && (!tris || !tvol || vbeg.size() != 2)) return IDP_E_INVALID;  // shells: one instance
+        if ((long)nT + nE > INT32_MAX / 16) return IDP_E_INVALID;
         for (long i = 0; i < 4L * nT; ++i)
             if (tets[i] < 0 || tets[i] >= nV) return IDP_E_INVALID;
+        for (long i = 0; i < 3L * nE; ++i)
+            if (tris[i] < 0 || tris[i] >= nV) return IDP_E_INVALID;
         IDP_HIP(hipSetDevice(c->device));
         auto s = c->s;
         MeshDev& m = c->m;
@@ -1025,6 +1075,20 @@ int setup_mesh(idp_ctx* c, int nV, const double* X_rest, int nT, const int* tets
         m.vol.upload(vol, nT, s);
         m.lam.upload(lambda, nT, s);
         m.mu.upload(mu, nT, s);
+        m.nE = nE;
+        m.tris.upload(tris, 3 * (size_t)nE, s);
+        m.tvol.upload(tvol, nE, s);
+        // vertex -> triangles, ascending (the order of the normal-flow sums)
+        std::vector<int> vtp(nV + 1, 0), vtl(3 * (size_t)nE);
+        for (long i = 0; i < 3L * nE; ++i) ++vtp[tris[i] + 1];
+        for (int v = 0; v < nV; ++v) vtp[v + 1] += vtp[v];
+        {
+            std::vector<int> f(vtp.begin(), vtp.end() - 1);
+            for (int e = 0; e < nE; ++e)
+                for (int k = 0; k < 3; ++k) vtl[f[tris[3 * e + k]]++] = e;
+        }
+        m.vt_ptr.upload(vtp.data(), vtp.size(), s);
+        m.vt.upload(vtl.data(), vtl.size(), s);
         std::vector<unsigned char> dbc(nV, 0);
         m.dbc.upload(dbc.data(), nV, s);
         c->ndbc = 0;
@@ -1034,8 +1098,8 @@ int setup_mesh(idp_ctx* c, int nV, const double* X_rest, int nT, const int* tets
         std::vector<std::array<int, 3>> bt;
         // on the device (host restatement only beyond the device keys' widths)
         IDP_HIP(hipStreamSynchronize(s));
-        if (!surface_device(s, c->sw, nV, m.X0.p, nT, m.tets.p, bn, be, bt))
-            surface_host(nV, X_rest, nT, tets, bn, be, bt);
+        if (nE > 0 || !surface_device(s, c->sw, nV, m.X0.p, nT, m.tets.p, bn, be, bt))
+            surface_host(nV, X_rest, nT, tets, nE, tris, bn, be, bt);
         m.nb = (int)bn.size();
         m.ne = (int)be.size();
         m.nf = (int)bt.size();
@@ -1046,16 +1110,23 @@ int setup_mesh(idp_ctx* c, int nV, const double* X_rest, int nT, const int* tets
         for (int i = 0; i < m.nb; ++i) v2b[bn[i]] = i;
         m.v2b.upload(v2b.data(), nV, s);
         // static elastic pattern: bucket records by row, sort each row by (col, src)
+        // (shell triangle e is element nT + e with its 3 x 3 leading blocks)
         std::vector<int> rcount(nV + 1, 0);
         for (int t = 0; t < nT; ++t)
             for (int a = 0; a < 4; ++a) rcount[tets[4 * t + a] + 1] += 4;
+        for (int e = 0; e < nE; ++e)
+            for (int a = 0; a < 3; ++a) rcount[tris[3 * e + a] + 1] += 3;
         for (int v = 0; v < nV; ++v) rcount[v + 1] += rcount[v];
-        std::vector<std::pair<int, int>> rec(16 * (size_t)nT);  // (col, src)
+        std::vector<std::pair<int, int>> rec(16 * (size_t)nT + 9 * (size_t)nE);  // (col, src)
         std::vector<int> fill(rcount.begin(), rcount.end() - 1);
         for (int t = 0; t < nT; ++t)
             for (int a = 0; a < 4; ++a)
                 for (int b = 0; b < 4; ++b)
                     rec[fill[tets[4 * t + a]]++] = {tets[4 * t + b], t * 16 + a * 4 + b};
+        for (int e = 0; e < nE; ++e)
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b)
+                    rec[fill[tris[3 * e + a]]++] = {tris[3 * e + b], (nT + e) * 16 + a * 4 + b};
         std::vector<int> rowptr(nV + 1, 0), col, sptr, src, diag(nV, -1), rowof;
         col.reserve(27 * (size_t)nV);
         src.reserve(16 * (size_t)nT);
@@ -1072,7 +1143,7 @@ int setup_mesh(idp_ctx* c, int nV, const double* X_rest, int nT, const int* tets
                 k = j;
             }
             rowptr[v + 1] = (int)col.size();
-            if (diag[v] < 0) return IDP_E_INVALID;  // vertex not in any tet
+            if (diag[v] < 0) return IDP_E_INVALID;  // vertex in no element
         }
         m.nnzb_e = (long)col.size();
         m.e_rowptr.upload(rowptr.data(), rowptr.size(), s);
@@ -1103,6 +1174,11 @@ int setup_mesh(idp_ctx* c, int nV, const double* X_rest, int nT, const int* tets
             !ranges(m.nf, [&](int k) { return std::min(bt[k][0], std::min(bt[k][1], bt[k][2])); }, fbeg))
             return IDP_E_INVALID;
         build_inst_layout(m, vbeg, tbeg, nbeg, ebeg, fbeg, s);
+        {
+            std::vector<long> sbeg(B + 1, 0);
+            sbeg[B] = nE;  // nE > 0 only with one instance
+            m.inst.sblk.build(sbeg, RB, s);
+        }
         c->cw.h_cbeg.clear();
         // state
         c->x.upload(X_rest, 3 * (size_t)nV, s);
@@ -1388,7 +1464,7 @@ int idp_surface(int nV, const double* X, int nT, const int* tets, int* nb, int* 
     std::vector<int> bn;
     std::vector<std::array<int, 2>> be;
     std::vector<std::array<int, 3>> bt;
-    surface_host(nV, X, nT, tets, bn, be, bt);
+    surface_host(nV, X, nT, tets, 0, nullptr, bn, be, bt);
     *nb = (int)bn.size();
     *ne = (int)be.size();
     *nf = (int)bt.size();
@@ -1401,6 +1477,73 @@ int idp_surface(int nV, const double* X, int nT, const int* tets, int* nb, int* 
     if (btri)
         for (size_t i = 0; i < bt.size(); ++i)
             for (int k = 0; k < 3; ++k) btri[3 * i + k] = bt[i][k];
+    return IDP_OK;
+}
+
+int idp_shell_init(int nV, const double* X, int nE, const int* tris, double density, double E, double nu,
+                   double thickness, int* nE_kept, int* kept, double* mass, double* vol, double* lambda, double* mu)
+{
+    if (nV <= 0 || nE < 0 || !X || (nE && !tris) || !nE_kept || !kept || !mass || !vol || !lambda || !mu)
+        return IDP_E_INVALID;
+    for (long i = 0; i < 3L * nE; ++i)
+        if (tris[i] < 0 || tris[i] >= nV) return IDP_E_INVALID;
+    auto P = [&](int v, int d) { return X[3 * v + d]; };
+    int n = 0;
+    for (int e = 0; e < nE; ++e) {
+        const int* t = tris + 3 * e;
+        double u[3], w[3];
+        for (int d = 0; d < 3; ++d) {
+            u[d] = P(t[1], d) - P(t[0], d);
+            w[d] = P(t[2], d) - P(t[0], d);
+        }
+        const double a = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        const double b = u[0] * w[0] + u[1] * w[1] + u[2] * w[2];
+        const double dd = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+        if (a * dd - b * b == 0.0) continue;  // degenerate first fundamental form
+        kept[3 * n] = t[0];
+        kept[3 * n + 1] = t[1];
+        kept[3 * n + 2] = t[2];
+        ++n;
+    }
+    const double L = E * nu / (1.0 - nu * nu);
+    const double M = E / (2.0 * (1.0 + nu));
+    for (int e = 0; e < n; ++e) {
+        const int* t = kept + 3 * e;
+        double u[3], w[3];
+        for (int d = 0; d < 3; ++d) {
+            u[d] = P(t[1], d) - P(t[0], d);
+            w[d] = P(t[2], d) - P(t[0], d);
+        }
+        const double c0 = u[1] * w[2] - u[2] * w[1], c1 = u[2] * w[0] - u[0] * w[2], c2 = u[0] * w[1] - u[1] * w[0];
+        const double len = std::sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+        const double portion = len / 2 * thickness * density / 3;
+        for (int k = 0; k < 3; ++k) mass[t[k]] += portion;
+        vol[e] = len / 2 * thickness;
+        lambda[e] = L;
+        mu[e] = M;
+    }
+    *nE_kept = n;
+    return IDP_OK;
+}
+
+int idp_shell_surface(int nV, const double* X, int nT, const int* tets, int nE, const int* tris, int* nb, int* ne,
+                      int* nf, int* bnode, int* bedge, int* btri)
+{
+    if (nV <= 0 || nT < 0 || nE < 0 || !X || (nT && !tets) || (nE && !tris) || !nb || !ne || !nf) return IDP_E_INVALID;
+    for (long i = 0; i < 4L * nT; ++i)
+        if (tets[i] < 0 || tets[i] >= nV) return IDP_E_INVALID;
+    for (long i = 0; i < 3L * nE; ++i)
+        if (tris[i] < 0 || tris[i] >= nV) return IDP_E_INVALID;
+    std::vector<int> bn;
+    std::vector<std::array<int, 2>> be;
+    std::vector<std::array<int, 3>> bt;
+    surface_host(nV, X, nT, tets, nE, tris, bn, be, bt);
+    *nb = (int)bn.size();
+    *ne = (int)be.size();
+    *nf = (int)bt.size();
+    if (bnode) std::memcpy(bnode, bn.data(), bn.size() * sizeof(int));
+    if (bedge) std::memcpy(bedge, be.data(), be.size() * sizeof(be[0]));
+    if (btri) std::memcpy(btri, bt.data(), bt.size() * sizeof(bt[0]));
     return IDP_OK;
 }
 
@@ -1425,6 +1568,40 @@ int idp_set_mesh(idp_ctx* c, int nV, const double* X_rest, int nT, const int* te
 {
     return guarded(c, [&]() -> int {
         return setup_mesh(c, nV, X_rest, nT, tets, IB, vol, lambda, mu, mass, {0, (long)nV}, {0, (long)nT});
+    });
+}
+
+int idp_set_mesh_shell(idp_ctx* c, int nV, const double* X_rest, const double* mass, int nT, const int* tets,
+                       const double* IB, const double* vol, const double* lambda, const double* mu, int nE,
+                       const int* tris, const double* tri_vol)
+{
+    return guarded(c, [&]() -> int {
+        if (nE <= 0) return IDP_E_INVALID;
+        return setup_mesh(c, nV, X_rest, nT, tets, IB, vol, lambda, mu, mass, {0, (long)nV}, {0, (long)nT}, nE,
+                          tris, tri_vol);
+    });
+}
+
+int idp_normal_flow_force(idp_ctx* c, const double* X, double magnitude, double* b)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh || c->m.nE <= 0 || !b) return IDP_E_INVALID;
+        IDP_HIP(hipSetDevice(c->device));
+        const size_t n3 = 3 * (size_t)c->m.nV;
+        double* o = c->bforce.ensure(n3, c->s);
+        launch_normal_flow(c->s, c->m, stage_x(c, X), magnitude, o);
+        c->bforce.download(b, n3, c->s);
+        IDP_HIP(hipStreamSynchronize(c->s));
+        return IDP_OK;
+    });
+}
+
+int idp_set_normal_flow(idp_ctx* c, double magnitude)
+{
+    return guarded(c, [&]() -> int {
+        if (!std::isfinite(magnitude)) return IDP_E_INVALID;
+        c->normal_flow = magnitude;
+        return IDP_OK;
     });
 }
 
@@ -1608,9 +1785,15 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
         double* g = c->g.ensure(n3, s);
         // Xn = X; Xtilde = X + h v + h^2 M^-1 b, v = 0 first for the flow variant (IMPLICIT_EULER.h:196-217)
         IDP_HIP(hipMemcpyAsync(xn, x, n3 * 8, hipMemcpyDeviceToDevice, s));
-        if (b || flow)
-            IDP_LAUNCH(k_xtilde, grid_for(n3, 256), 256, 0, s, n3, x, c->v.p, b ? c->bforce.upload_get(b, n3, s) : nullptr,
-                       m.mass.p, h, flow ? 1 : 0, xt);
+        const double* bdev = nullptr;
+        if (b) {
+            bdev = c->bforce.upload_get(b, n3, s);
+        } else if (c->normal_flow != 0.0 && m.nE > 0) {
+            bdev = c->bforce.ensure(n3, s);
+            launch_normal_flow(s, m, x, c->normal_flow, c->bforce.p);
+        }
+        if (bdev || flow)
+            IDP_LAUNCH(k_xtilde, grid_for(n3, 256), 256, 0, s, n3, x, c->v.p, bdev, m.mass.p, h, flow ? 1 : 0, xt);
         else
             dev_axpy(s, n3, x, h, c->v.p, xt);
         double kappa = kappa3[0];
@@ -2034,10 +2217,14 @@ int idp_barrier_hessian_local(idp_ctx* c, const double* X, long n, const int* cs
         double* b = c->Hb.ensure(144 * (size_t)std::max(n, 1L), s);
         DBuf<int> dv;
         int* dvid = dv.ensure(4 * (size_t)std::max(n, 1L), s);
-        if (n)
+        if (n) {
+            const int tok = c->prof.begin(IDP_K_BARRIER_HESS);  // timed like the solve's launches
             launch_barrier_hess(s, n, (const int*)c->cw.cs.p, xd, c->m.X0.p, bparams(c), project, 0, b, dvid, fl,
-                                c->hform.ensure(90 * (size_t)n, s), c->hform_ok.ensure(n, s), nullptr,
-                                c->hperm.ensure(n, s), c->hkcnt.ensure(16, s), c->hess_team);
+                                c->hform.ensure(90 * (size_t)n, s), c->hform_ok.ensure(n, s),
+                                c->prof.full() ? c->bcounts.p : nullptr, c->hperm.ensure(n, s),
+                                c->hkcnt.ensure(16, s), c->hess_team);
+            c->prof.end(IDP_K_BARRIER_HESS, tok);
+        }
         int hf = 0;
         IDP_HIP(hipMemcpyAsync(&hf, fl, sizeof(int), hipMemcpyDeviceToHost, s));
         if (n) {
@@ -2141,6 +2328,19 @@ int idp_elastic_hessian_local(idp_ctx* c, const double* X, double w, int project
         IDP_HIP(hipStreamSynchronize(s));
         return IDP_OK;
     });
+}
+
+int idp_newton_direction_flow(idp_ctx* c, const double* X, const double* xtilde, double h, double dHat2,
+                              double kappa, double thickness, const idp_solver_opts* opts, double* g, double* sol,
+                              double* relres, int* iters)
+{
+    if (!c) return IDP_E_INVALID;
+    struct ElasticOn {
+        idp_ctx* c;
+        ~ElasticOn() { c->no_elastic = false; }
+    } restore_elastic{c};
+    c->no_elastic = true;
+    return idp_newton_direction(c, X, xtilde, h, dHat2, kappa, thickness, opts, g, sol, relres, iters);
 }
 
 int idp_newton_direction(idp_ctx* c, const double* X, const double* xtilde, double h, double dHat2, double kappa,

@@ -64,6 +64,7 @@ struct InstDev {
     SegDev vblk;     // vertices, blocks of RB (bounding box, inertia energy)
     SegDev nblk;     // boundary nodes, blocks of RB (swept bounding box)
     SegDev tblk;     // tets, blocks of RB (elastic energy)
+    SegDev sblk;     // shell triangles, blocks of RB (flow energy)
     SegDev pcg_rows[4];  // vertices, blocks of 256 / G rows for the SpMV with G = 4, 8, 16, 32 lanes per row
     SegDev pcg_upd;  // vertices, 64-row blocks (PCG update)
     SegDev dblk;     // 3 x vertices, blocks of RB (per-instance dot products)
@@ -81,6 +82,11 @@ struct MeshDev {
     DBuf<double> IB;      // 9 per tet, row-major
     DBuf<double> vol, lam, mu;
     DBuf<unsigned char> dbc;
+    // shell triangles (Elem): elements nT .. nT + nE - 1, vol = area * thickness
+    int nE = 0;
+    DBuf<int> tris;       // 3 per triangle
+    DBuf<double> tvol;
+    DBuf<int> vt_ptr, vt; // vertex -> its triangles, ascending (nV + 1, 3 nE)
     // rows/columns projected out of the gradient and the Hessian: the DBC
     // nodes, or only the ones that do not move while a moving-DBC penalty is
     // on (null: dbc)
@@ -92,7 +98,7 @@ struct MeshDev {
     DBuf<int> v2b;        // vertex -> boundary node index (-1 if interior)
     // static elastic BSR pattern (tet connectivity + diagonal)
     DBuf<int> e_rowptr, e_col;       // nV+1, nnzb_e
-    DBuf<int> e_src_ptr, e_src;      // per block: contributing (tet*16 + a*4 + b)
+    DBuf<int> e_src_ptr, e_src;      // per block: contributing (element*16 + a*4 + b)
     DBuf<int> e_diag;                // block index of (v,v)
     long nnzb_e = 0;
     InstDev inst;                    // instance layout (one instance for a plain mesh)
@@ -209,6 +215,11 @@ void launch_tet_grad(hipStream_t s, const MeshDev& m, const double* X, double w,
                      const unsigned char* act = nullptr);
 void launch_tet_hess(hipStream_t s, const MeshDev& m, const double* X, double w, int project, double* Hl,
                      const unsigned char* act = nullptr);
+// shell.hip: the flow term over the shell triangles and the normal-flow force
+void launch_flow_energy(hipStream_t s, const MeshDev& m, const double* X, double h, double* part);
+void launch_flow_grad(hipStream_t s, const MeshDev& m, const double* X, double h, double* gl);
+void launch_flow_hess(hipStream_t s, const MeshDev& m, double h, double* Hl);
+void launch_normal_flow(hipStream_t s, const MeshDev& m, const double* X, double magnitude, double* b);
 void launch_inertia_energy(hipStream_t s, const SegDev& seg, const double* X, const double* Xt, const double* mass,
                            double* part);
 

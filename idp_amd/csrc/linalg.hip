@@ -1252,7 +1252,7 @@ void assemble(LinWork& L, hipStream_t s, const MeshDev& m, const double* Hl, con
               const unsigned char* act)
 {
     // algorithmic bytes: 80 per contribution (9 doubles + key), 76 per block, rowptr
-    const double ncontrib = 16.0 * m.nT + (double)L.nrec + m.nV;
+    const double ncontrib = 16.0 * m.nT + 9.0 * m.nE + (double)L.nrec + m.nV;
     const int tok = L.prof ? L.prof->begin(IDP_K_ASSEMBLY) : -1;
     double* ev = L.evals.ensure(9 * (size_t)m.nnzb_e, s);
     IDP_LAUNCH(k_elastic_values, grid_for(m.nnzb_e, 256), 256, 0, s, m.nnzb_e, L.erow.p, m.e_col.p, m.e_src_ptr.p,

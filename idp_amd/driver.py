@@ -47,6 +47,41 @@ class TetAnimFix:
         self.ctx.close()
 
 
+class ShellNormalFlow:
+    """Normal-flow shell driver: Initialize_Shell_Hinge_EIPC's set-up
+    (FEMDiscreteShellBase.py:190-203), initialize_OIPC, and per step the
+    normal-flow force (Update_Normal_Flow_Neumann) followed by
+    Advance_One_Step_IE_Flow (FEMDiscreteShellBase.py:268-273, 332-341).  The
+    force is formed on the device from the step's start positions."""
+
+    def __init__(self, scene, device=0, pcg_rtol=1e-8, kappa_mult=1.0):
+        self.scene = scene
+        X = np.ascontiguousarray(scene.X, dtype=np.float64)
+        self.tris, self.mass, self.tvol, _, _ = capi.shell_init(X, scene.tris, scene.density, 0.0, 0.0,
+                                                                  scene.thickness)
+        self.ctx = capi.Context(device)
+        self.ctx.set_mesh_shell(X, self.mass, self.tris, self.tvol)
+        self.ctx.set_normal_flow(scene.magnitude)
+        self.dHat2, self.kappa = capi.initialize_oipc(scene.dhat, self.mass, kappa_mult)
+        self.h = scene.h
+        self.tol = scene.newton_tol
+        self.pcg_rtol = pcg_rtol
+        self.ctx.set_state(X, np.zeros_like(X))
+        self.pn_iters = 0
+
+    def advance_one_time_step(self, max_newton=0, verbose=0):
+        it, st = self.ctx.advance(self.h, self.tol, self.dHat2, self.kappa, 0.0, True, self.pcg_rtol,
+                                  max_newton=max_newton, verbose=verbose, flow=True)
+        self.pn_iters += it
+        return it, st
+
+    def state(self):
+        return self.ctx.get_state()
+
+    def close(self):
+        self.ctx.close()
+
+
 class BatchAnimFix:
     """Several independent animation-fix scenes advanced together in one
     context: every stage runs over all instances at once, while each instance

@@ -220,3 +220,98 @@ def two_blocks_slide(n=2, spacing=0.05, gap=2e-4, slide=1.0):
     p[:, 1] = -0.1 * spacing
     p[Xa.shape[0]:, 0] = slide * spacing
     return Scene(f"two_blocks{n}_slide", X.copy(), T, X.copy()), p
+
+
+# ---------------------------------------------------------------------------
+# Shell scenes for the normal-flow variant (Projects/FEMShell/12-14_normal_flow.py:
+# dt 1 scaled by the smoothing intensity 0.5, normal-flow magnitude 0.005 /
+# 0.5^2, density 1, thickness 1, E = nu = 0, zero gravity, dhat 1e-3,
+# Newton tolerance PNTol 1e-3 of FEMDiscreteShellBase.py:43)
+# ---------------------------------------------------------------------------
+@dataclasses.dataclass
+class ShellScene:
+    name: str
+    X: np.ndarray  # (V,3) float64 start positions
+    tris: np.ndarray  # (F,3) int32, outward oriented
+    h: float = 0.5
+    magnitude: float = 0.02
+    density: float = 1.0
+    thickness: float = 1.0
+    dhat: float = 1e-3
+    newton_tol: float = 1e-3
+
+    @property
+    def diag(self) -> float:
+        return float(np.linalg.norm(self.X.max(0) - self.X.min(0)))
+
+
+def icosphere(level: int, radius: float = 1.0, center=(0.0, 0.0, 0.0)):
+    """Icosahedron subdivided `level` times (each triangle into four at its
+    edge midpoints), vertices pushed onto the sphere; triangles oriented so
+    cross(X2 - X1, X3 - X1) points outward.  10 * 4^level + 2 vertices."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    V = [[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+         [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]]
+    F = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6],
+         [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10],
+         [8, 6, 7], [9, 8, 1]]
+    V = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in V]
+    for _ in range(level):
+        mid = {}
+
+        def m(a, b):
+            k = (min(a, b), max(a, b))
+            if k not in mid:
+                p = V[a] + V[b]
+                V.append(p / np.linalg.norm(p))
+                mid[k] = len(V) - 1
+            return mid[k]
+
+        F2 = []
+        for a, b, c in F:
+            ab, bc, ca = m(a, b), m(b, c), m(c, a)
+            F2 += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        F = F2
+    X = np.array(V) * radius + np.asarray(center, dtype=np.float64)
+    tris = np.array(F, dtype=np.int32)
+    # outward orientation (the icosahedron list above is outward already)
+    c = np.cross(X[tris[:, 1]] - X[tris[:, 0]], X[tris[:, 2]] - X[tris[:, 0]])
+    assert np.all(np.einsum("ij,ij->i", c, X[tris].mean(1) - np.asarray(center)) > 0)
+    return X, tris
+
+
+def sphere_pair(level=2, radius=0.5, gap=0.01, magnitude=0.2, name=None):
+    """Two icospheres whose closest points are `gap` apart along x, with a
+    normal-flow force strong enough to outgrow the flow's shrinking: both
+    inflate until they press on each other."""
+    return sphere_lattice((2, 1, 1), level, radius, gap, magnitude, name=name or f"spheres{level}_gap{gap:g}")
+
+
+def _rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    a, b, c, d = q
+    return np.array([[a * a + b * b - c * c - d * d, 2 * (b * c - a * d), 2 * (b * d + a * c)],
+                     [2 * (b * c + a * d), a * a - b * b + c * c - d * d, 2 * (c * d - a * b)],
+                     [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d]])
+
+
+def sphere_lattice(counts=(4, 4, 4), level=3, radius=0.5, gap=0.01, magnitude=0.2, seed=11, name=None):
+    """counts[0] x counts[1] x counts[2] icospheres on a grid of pitch
+    2 radius + gap, each turned by its own seeded rotation (so no vertex of
+    one sits exactly opposite a vertex or edge of its neighbour); under the
+    normal-flow force each presses on its neighbours (every sphere a separate
+    shell component)."""
+    rng = np.random.default_rng(seed)
+    Xs, Ts, off = [], [], 0
+    for k in range(counts[2]):
+        for j in range(counts[1]):
+            for i in range(counts[0]):
+                c = np.array([i, j, k], dtype=np.float64) * (2 * radius + gap)
+                X, T = icosphere(level, radius)
+                X = X @ _rotation(rng).T + c
+                Xs.append(X)
+                Ts.append(T + off)
+                off += X.shape[0]
+    nm = name or f"spheres{counts[0]}x{counts[1]}x{counts[2]}_l{level}"
+    return ShellScene(nm, np.vstack(Xs), np.vstack(Ts).astype(np.int32), magnitude=magnitude)

@@ -157,6 +157,22 @@ int idp_fem_init(int nV, const double* X, int nT, const int* tets, double densit
  * bedge >= 12*nT. */
 int idp_surface(int nV, const double* X, int nT, const int* tets, int* nb, int* ne, int* nf, int* bnode,
                 int* bedge, int* btri);
+/* Initialize_Discrete_Shell<double,3,false,false> (Library/FEM/Shell/DISCRETE_SHELL.h:219-354),
+ * bound as FEM.DiscreteShell.Initialize_Shell_Hinge: keeps the triangles with
+ * a nonzero first-fundamental-form determinant, in order (kept: 3 nE ints,
+ * nE_kept their count); mass (nV, accumulated into: zero it for a pure
+ * shell) += |cross(X2-X1, X3-X1)| / 2 * thickness * density / 3 per kept
+ * triangle in order; vol = area * thickness, lambda = E nu / (1 - nu^2),
+ * mu = E / (2 (1 + nu)) per kept triangle. */
+int idp_shell_init(int nV, const double* X, int nE, const int* tris, double density, double E, double nu,
+                   double thickness, int* nE_kept, int* kept, double* mass, double* vol, double* lambda, double* mu);
+/* The boundary primitives of a mesh with tets and shell triangles as the
+ * shell step lists them (Library/FEM/Shell/IMPLICIT_EULER.h:233-241): the
+ * tets' surface triangles (MESHIO.h:486-545) followed by the shell triangles
+ * in order, then MESHIO.h:769-835 over that list.  nT may be 0.
+ * bnode >= nV, bedge >= 2 (3 nE + 12 nT), btri >= 3 (nE + 4 nT) ints. */
+int idp_shell_surface(int nV, const double* X, int nT, const int* tets, int nE, const int* tris, int* nb, int* ne,
+                      int* nf, int* bnode, int* bedge, int* btri);
 /* Initialize_EIPC<double,false> as bound to Initialize_OIPC
  * (Library/FEM/Shell/DISCRETE_SHELL.h:555-577, 1105). kappa3 = {kappa, kappa_max, 0}. */
 int idp_initialize_oipc(double dhat, int nV, const double* mass, double stiff_mult, double* kappa3,
@@ -168,6 +184,28 @@ int idp_initialize_oipc(double dhat, int nV, const double* mass, double stiff_mu
  * surface primitives and precomputes the static elastic BSR pattern. */
 int idp_set_mesh(idp_ctx* ctx, int nV, const double* X_rest, int nT, const int* tets, const double* IB,
                  const double* vol, const double* lambda, const double* mu, const double* mass);
+/* A mesh with shell triangles (Elem of the shell step), with or without
+ * tets (nT may be 0; then tets..mu may be null): tri_vol is the vol of the
+ * shell's elasticityAttr (area * thickness, from idp_shell_init).  The
+ * surface primitives are those of idp_shell_surface.  One instance.  Shell
+ * triangles carry the flow variant's -h/2 x^T L x term
+ * (Library/FEM/Shell/INC_POTENTIAL.h:53-73, 191-211, 323-339); their
+ * membrane and bending terms are not built, so only flow = 1 steps and
+ * evaluations are accepted on such a mesh (IDP_E_INVALID otherwise). */
+int idp_set_mesh_shell(idp_ctx* ctx, int nV, const double* X_rest, const double* mass, int nT, const int* tets,
+                       const double* IB, const double* vol, const double* lambda, const double* mu, int nE,
+                       const int* tris, const double* tri_vol);
+/* Update_Normal_Flow_Neumann (Library/FEM/Shell/DISCRETE_SHELL.h:359-395) on
+ * the device: b (3 nV, host) = magnitude * m_v * n_v / |n_v| with n_v the sum
+ * of cross(X2-X1, X3-X1) over the shell triangles of v in order, at X (host,
+ * 3 nV; null: the context's current x).  Equal bit for bit to the
+ * sequential loop. */
+int idp_normal_flow_force(idp_ctx* ctx, const double* X, double magnitude, double* b);
+/* Steps given no body force (b = null) recompute the normal-flow force from
+ * their start positions on the device first, as the shell driver calls
+ * Update_Normal_Flow_Neumann before each step (FEMDiscreteShellBase.py:269-273);
+ * magnitude 0 turns it off (the default). */
+int idp_set_normal_flow(idp_ctx* ctx, double magnitude);
 /* DBC vertex list (VECTOR_STORAGE DBC of the reference, fixed targets = current x). */
 int idp_set_dbc(idp_ctx* ctx, int n, const int* idx);
 /* Moving Dirichlet nodes: node idx[i] is driven to targets[3 i .. 3 i + 2]
@@ -266,6 +304,12 @@ int idp_elastic_hessian_local(idp_ctx* ctx, const double* X, double w, int proje
 int idp_newton_direction(idp_ctx* ctx, const double* X, const double* xtilde, double h, double dHat2,
                          double kappa, double thickness, const idp_solver_opts* opts, double* g, double* sol,
                          double* relres, int* iters);
+/* idp_newton_direction with the flow variant's terms (Advance_One_Step_IE_Flow,
+ * Library/FEM/Shell/INC_POTENTIAL.h:53-73, 191-211, 323-339): no tet
+ * elasticity, the shell triangles' -h/2 x^T L x term, inertia and barrier. */
+int idp_newton_direction_flow(idp_ctx* ctx, const double* X, const double* xtilde, double h, double dHat2,
+                              double kappa, double thickness, const idp_solver_opts* opts, double* g, double* sol,
+                              double* relres, int* iters);
 /* BSR of the last assembled system: sizes, then arrays (rowptr nV+1, col nnzb, val 9*nnzb) */
 int idp_get_bsr(idp_ctx* ctx, long* nnzb, int* rowptr, int* col, double* val, long cap);
 

@@ -183,6 +183,11 @@ struct Ctx {
     // barrier, the volumetric term is skipped (INC_POTENTIAL.h:53-73 takes
     // only the shell flow term, none for a tet mesh)
     bool flow = false;
+    // shell triangles (Elem) and their vol = area * thickness: the flow term
+    // -h/2 x^T L x of INC_POTENTIAL.h:53-73, 191-211, 323-339
+    int nE = 0;
+    const int* tris = nullptr;
+    const double* tvol = nullptr;
     std::vector<double> Xtilde;
     std::vector<int> cs;
     long ncs = 0;
@@ -240,6 +245,21 @@ void dbc_energy(const Ctx& c, const double* X, double& E)
     E += 0.5 * c.dbc_stiff * s;
 }
 
+// Lx of the flow term (INC_POTENTIAL.h:53-66): per triangle in order, for
+// each corner i, Lx_i += vol/6 (x_(i+1) + x_(i+2) - 2 x_i)
+void flow_lx(const Ctx& c, const double* X, std::vector<double>& Lx)
+{
+    Lx.assign(3 * (size_t)c.nV, 0.0);
+    for (int e = 0; e < c.nE; ++e) {
+        const int* t = c.tris + 3 * e;
+        const double w = c.tvol[e] / 6;
+        for (int i = 0; i < 3; ++i) {
+            const int a = t[i], b = t[(i + 1) % 3], d = t[(i + 2) % 3];
+            for (int k = 0; k < 3; ++k) Lx[3 * a + k] += w * ((X[3 * b + k] + X[3 * d + k]) - 2 * X[3 * a + k]);
+        }
+    }
+}
+
 // Compute_IncPotential (tet + inertia + barrier); returns validity
 bool energy(Ctx& c, const double* X, bool withBarrier, double& E)
 {
@@ -248,6 +268,12 @@ bool energy(Ctx& c, const double* X, bool withBarrier, double& E)
     if (!c.flow && !orc_elastic_energy(X, c.nT, c.tets, c.IB, c.vol, c.lam, c.mu, c.h * c.h, &Ee)) {
         c.t_E += now() - t0;
         return false;
+    }
+    if (c.flow && c.nE) {  // INC_POTENTIAL.h:67-72: value += -h/2 x.Lx, vertex by vertex
+        std::vector<double> Lx;
+        flow_lx(c, X, Lx);
+        for (int v = 0; v < c.nV; ++v)
+            for (int d = 0; d < 3; ++d) Ee += -c.h * 0.5 * X[3 * v + d] * Lx[3 * v + d];
     }
     double Ei = 0;
     for (int v = 0; v < c.nV; ++v)
@@ -291,6 +317,23 @@ void assemble_sys(Ctx& c, const double* X, Sys& S)
             add_local(blks, nv, vl.data() + 4 * i, Hl.data() + 144 * i, nV);
         }
     }
+    if (c.flow && c.nE) {  // INC_POTENTIAL.h:323-339: -h vol/6 to both neighbours, 2 h vol/6 on the diagonal
+        for (int e = 0; e < c.nE; ++e) {
+            const int* t = c.tris + 3 * e;
+            for (int i = 0; i < 3; ++i) {
+                const int a = t[i];
+                const int nb3[3] = {t[(i + 1) % 3], t[(i + 2) % 3], a};
+                const double sv[3] = {-c.h * c.tvol[e] / 6, -c.h * c.tvol[e] / 6, 2 * c.h * c.tvol[e] / 6};
+                for (int q = 0; q < 3; ++q) {
+                    Blk k;
+                    k.key = (long)a * nV + nb3[q];
+                    std::memset(k.v, 0, sizeof(k.v));
+                    k.v[0] = k.v[4] = k.v[8] = sv[q];
+                    blks.push_back(k);
+                }
+            }
+        }
+    }
     for (int v = 0; v < nV; ++v) {
         Blk k;
         k.key = (long)v * nV + v;
@@ -320,15 +363,18 @@ void assemble_sys(Ctx& c, const double* X, Sys& S)
 
 }  // namespace
 
-extern "C" int orc_advance_one_step(int nV, double* X, const double* X0, double* vel, const double* mass,
-                                    int nT, const int* tets, const double* IB, const double* vol,
-                                    const double* lam, const double* mu, int nb, const int* bnode, int ne,
-                                    const int* bedge, int nf, const int* btri, int nDBC, const int* dbc_idx,
-                                    const double* dbc_target, double h, double NewtonTol, int withCollision,
-                                    double dHat2, double* kappa, double thickness, double lin_rtol, int max_newton,
-                                    const double* body_force, int flow, orc_stats* st)
+namespace {
+int advance_impl(int nV, double* X, const double* X0, double* vel, const double* mass, int nT, const int* tets,
+                 const double* IB, const double* vol, const double* lam, const double* mu, int nE, const int* tris,
+                 const double* tvol, int nb, const int* bnode, int ne, const int* bedge, int nf, const int* btri,
+                 int nDBC, const int* dbc_idx, const double* dbc_target, double h, double NewtonTol,
+                 int withCollision, double dHat2, double* kappa, double thickness, double lin_rtol, int max_newton,
+                 const double* body_force, int flow, orc_stats* st)
 {
     Ctx c;
+    c.nE = nE;
+    c.tris = tris;
+    c.tvol = tvol;
     c.nV = nV; c.nT = nT; c.nb = nb; c.ne = ne; c.nf = nf;
     c.X0 = X0; c.mass = mass; c.tets = tets; c.IB = IB; c.vol = vol; c.lam = lam; c.mu = mu;
     c.bnode = bnode; c.bedge = bedge; c.btri = btri;
@@ -396,6 +442,11 @@ extern "C" int orc_advance_one_step(int nV, double* X, const double* X0, double*
         double t0 = now();
         std::fill(g.begin(), g.end(), 0.0);
         if (!c.flow) orc_elastic_gradient(X, nT, tets, IB, vol, lam, mu, h * h, g.data());
+        if (c.flow && c.nE) {  // INC_POTENTIAL.h:191-211: g += -h Lx
+            std::vector<double> Lx;
+            flow_lx(c, X, Lx);
+            for (size_t i = 0; i < n3; ++i) g[i] += -h * Lx[i];
+        }
         for (int v = 0; v < nV; ++v)
             for (int d = 0; d < 3; ++d) g[3 * v + d] += mass[v] * (X[3 * v + d] - c.Xtilde[3 * v + d]);
         if (c.withCollision && c.ncs)
@@ -562,16 +613,45 @@ extern "C" int orc_advance_one_step(int nV, double* X, const double* X0, double*
 // The Newton system at X: the projected Hessian (BSR, 3x3 blocks) and the
 // gradient of the incremental potential with inertia target Xtilde, for the
 // constraint set at X.  Returns nnzb, or -nnzb when cap is too small.
-extern "C" long orc_newton_system(int nV, const double* X, const double* Xtilde, const double* X0,
-                                  const double* mass, int nT, const int* tets, const double* IB, const double* vol,
-                                  const double* lam, const double* mu, int nb, const int* bnode, int ne,
-                                  const int* bedge, int nf, const int* btri, int nDBC, const int* dbc_idx, double h,
-                                  double dHat2, double kappa, double thickness, int* rowptr, int* col, double* val,
-                                  double* g, long cap)
+}  // namespace
+
+extern "C" int orc_advance_one_step(int nV, double* X, const double* X0, double* vel, const double* mass,
+                                    int nT, const int* tets, const double* IB, const double* vol,
+                                    const double* lam, const double* mu, int nb, const int* bnode, int ne,
+                                    const int* bedge, int nf, const int* btri, int nDBC, const int* dbc_idx,
+                                    const double* dbc_target, double h, double NewtonTol, int withCollision,
+                                    double dHat2, double* kappa, double thickness, double lin_rtol, int max_newton,
+                                    const double* body_force, int flow, orc_stats* st)
+{
+    return advance_impl(nV, X, X0, vel, mass, nT, tets, IB, vol, lam, mu, 0, nullptr, nullptr, nb, bnode, ne, bedge,
+                        nf, btri, nDBC, dbc_idx, dbc_target, h, NewtonTol, withCollision, dHat2, kappa, thickness,
+                        lin_rtol, max_newton, body_force, flow, st);
+}
+
+extern "C" int orc_advance_shell_flow(int nV, double* X, const double* X0, double* vel, const double* mass, int nE,
+                                      const int* tris, const double* tvol, int nb, const int* bnode, int ne,
+                                      const int* bedge, int nf, const int* btri, int nDBC, const int* dbc_idx,
+                                      double h, double NewtonTol, int withCollision, double dHat2, double* kappa,
+                                      double thickness, double lin_rtol, int max_newton, const double* body_force,
+                                      orc_stats* st)
+{
+    return advance_impl(nV, X, X0, vel, mass, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nE, tris, tvol, nb,
+                        bnode, ne, bedge, nf, btri, nDBC, dbc_idx, nullptr, h, NewtonTol, withCollision, dHat2, kappa,
+                        thickness, lin_rtol, max_newton, body_force, 1, st);
+}
+
+namespace {
+long newton_system_impl(int nV, const double* X, const double* Xtilde, const double* X0, const double* mass, int nT,
+                        const int* tets, const double* IB, const double* vol, const double* lam, const double* mu,
+                        int nE, const int* tris, const double* tvol, bool flow, int nb, const int* bnode, int ne,
+                        const int* bedge, int nf, const int* btri, int nDBC, const int* dbc_idx, double h,
+                        double dHat2, double kappa, double thickness, int* rowptr, int* col, double* val, double* g,
+                        long cap)
 {
     Ctx c;
     c.nV = nV; c.nT = nT; c.nb = nb; c.ne = ne; c.nf = nf;
     c.X0 = X0; c.mass = mass; c.tets = tets; c.IB = IB; c.vol = vol; c.lam = lam; c.mu = mu;
+    c.nE = nE; c.tris = tris; c.tvol = tvol; c.flow = flow;
     c.bnode = bnode; c.bedge = bedge; c.btri = btri;
     c.dbc.assign(nV, 0);
     for (int i = 0; i < nDBC; ++i) c.dbc[dbc_idx[i]] = 1;
@@ -588,11 +668,42 @@ extern "C" long orc_newton_system(int nV, const double* X, const double* Xtilde,
     std::copy(S.col.begin(), S.col.end(), col);
     std::copy(S.val.begin(), S.val.end(), val);
     std::fill(g, g + 3 * (size_t)nV, 0.0);
-    orc_elastic_gradient(X, nT, tets, IB, vol, lam, mu, h * h, g);
+    if (!flow) orc_elastic_gradient(X, nT, tets, IB, vol, lam, mu, h * h, g);
+    if (flow && nE) {
+        std::vector<double> Lx;
+        flow_lx(c, X, Lx);
+        for (size_t i = 0; i < Lx.size(); ++i) g[i] += -h * Lx[i];
+    }
     for (int v = 0; v < nV; ++v)
         for (int d = 0; d < 3; ++d) g[3 * v + d] += mass[v] * (X[3 * v + d] - Xtilde[3 * v + d]);
     if (c.ncs) orc_barrier_gradient(X, X0, c.ncs, c.cs.data(), dHat2, kappa, thickness, nV, g);
     for (int v = 0; v < nV; ++v)
         if (c.dbc[v]) g[3 * v] = g[3 * v + 1] = g[3 * v + 2] = 0;
     return nnzb;
+}
+}  // namespace
+
+extern "C" long orc_newton_system(int nV, const double* X, const double* Xtilde, const double* X0,
+                                  const double* mass, int nT, const int* tets, const double* IB, const double* vol,
+                                  const double* lam, const double* mu, int nb, const int* bnode, int ne,
+                                  const int* bedge, int nf, const int* btri, int nDBC, const int* dbc_idx, double h,
+                                  double dHat2, double kappa, double thickness, int* rowptr, int* col, double* val,
+                                  double* g, long cap)
+{
+    return newton_system_impl(nV, X, Xtilde, X0, mass, nT, tets, IB, vol, lam, mu, 0, nullptr, nullptr, false, nb,
+                              bnode, ne, bedge, nf, btri, nDBC, dbc_idx, h, dHat2, kappa, thickness, rowptr, col, val,
+                              g, cap);
+}
+
+// the flow variant's system on a shell (flow term + inertia + barrier)
+extern "C" long orc_newton_system_shell_flow(int nV, const double* X, const double* Xtilde, const double* X0,
+                                             const double* mass, int nE, const int* tris, const double* tvol, int nb,
+                                             const int* bnode, int ne, const int* bedge, int nf, const int* btri,
+                                             int nDBC, const int* dbc_idx, double h, double dHat2, double kappa,
+                                             double thickness, int* rowptr, int* col, double* val, double* g,
+                                             long cap)
+{
+    return newton_system_impl(nV, X, Xtilde, X0, mass, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nE, tris, tvol,
+                              true, nb, bnode, ne, bedge, nf, btri, nDBC, dbc_idx, h, dHat2, kappa, thickness, rowptr,
+                              col, val, g, cap);
 }

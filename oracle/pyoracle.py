@@ -88,6 +88,21 @@ def lib():
                                         ip, C.c_int, ip, C.c_int, ip, C.c_double, C.c_double, C.c_double,
                                         C.c_double, ip, ip, dp, dp, C.c_long]
         L.orc_newton_system.restype = C.c_long
+        L.orc_shell_init.argtypes = [C.c_int, dp, C.c_int, ip, C.c_double, C.c_double, C.c_double, C.c_double, ip,
+                                     dp, dp, dp, dp]
+        L.orc_shell_surface.argtypes = [C.c_int, dp, C.c_int, ip, C.c_int, ip, C.POINTER(C.c_int),
+                                        C.POINTER(C.c_int), C.POINTER(C.c_int), ip, ip, ip]
+        L.orc_normal_flow_force.argtypes = [C.c_int, dp, C.c_int, ip, dp, C.c_double, dp]
+        L.orc_normal_flow_force.restype = None
+        L.orc_flow_terms.argtypes = [C.c_int, dp, C.c_int, ip, dp, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_flow_terms.restype = None
+        L.orc_advance_shell_flow.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int, ip, dp, C.c_int, ip, C.c_int, ip,
+                                             C.c_int, ip, C.c_int, ip, C.c_double, C.c_double, C.c_int, C.c_double,
+                                             dp, C.c_double, C.c_double, C.c_int, C.c_void_p, C.POINTER(Stats)]
+        L.orc_newton_system_shell_flow.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int, ip, dp, C.c_int, ip, C.c_int,
+                                                   ip, C.c_int, ip, C.c_int, ip, C.c_double, C.c_double, C.c_double,
+                                                   C.c_double, ip, ip, dp, dp, C.c_long]
+        L.orc_newton_system_shell_flow.restype = C.c_long
         _lib = L
     return _lib
 
@@ -265,3 +280,86 @@ def initialize_oipc(dhat, mass, stiff_mult=1.0):
     Hb = barrier_d2b(1.0e-16, dHat2, 1.0)
     k0 = stiff_mult * 1.0e11 * float(np.mean(mass)) * 3 / (4.0e-16 * Hb)
     return dHat2, np.array([k0, 100 * k0, 0.0])
+
+
+class ShellMesh:
+    """A shell (triangles, no tets) as Initialize_Discrete_Shell sets it up."""
+
+    def __init__(self, X, tris, density, thickness, E=0.0, nu=0.0):
+        self.X = _f(X).copy()
+        self.X0 = self.X.copy()
+        tris = _i(tris)
+        self.nV = self.X.size // 3
+        nE = tris.size // 3
+        kept = np.zeros(3 * nE, np.int32)
+        self.mass = np.zeros(self.nV)
+        self.tvol = np.zeros(nE)
+        self.lam = np.zeros(nE)
+        self.mu = np.zeros(nE)
+        n = lib().orc_shell_init(self.nV, self.X, nE, tris, density, E, nu, thickness, kept, self.mass, self.tvol,
+                                 self.lam, self.mu)
+        self.nE = n
+        self.tris = kept[: 3 * n].copy()
+        self.tvol, self.lam, self.mu = self.tvol[:n].copy(), self.lam[:n].copy(), self.mu[:n].copy()
+        bn = np.zeros(self.nV, np.int32)
+        be = np.zeros(6 * n, np.int32)
+        bt = np.zeros(3 * n, np.int32)
+        nb, ne, nf = C.c_int(), C.c_int(), C.c_int()
+        lib().orc_shell_surface(self.nV, self.X, 0, np.zeros(0, np.int32), n, self.tris, C.byref(nb), C.byref(ne),
+                                C.byref(nf), bn, be, bt)
+        self.bnode = bn[: nb.value].copy()
+        self.bedge = be[: 2 * ne.value].reshape(-1, 2).copy()
+        self.btri = bt[: 3 * nf.value].reshape(-1, 3).copy()
+        self.dbc = np.zeros(self.nV, np.uint8)
+
+    def normal_flow_force(self, X, magnitude):
+        b = np.zeros(3 * self.nV)
+        lib().orc_normal_flow_force(self.nV, _f(X), self.nE, self.tris, self.mass, magnitude, b)
+        return b.reshape(-1, 3)
+
+    def flow_terms(self, X, h):
+        """(E, g, H (nE, 3, 3, 3, 3)) of the flow term alone."""
+        E = C.c_double()
+        g = np.zeros(3 * self.nV)
+        H = np.zeros(81 * self.nE)
+        lib().orc_flow_terms(self.nV, _f(X), self.nE, self.tris, self.tvol, h, C.cast(C.byref(E), C.c_void_p),
+                             g.ctypes.data_as(C.c_void_p), H.ctypes.data_as(C.c_void_p))
+        return E.value, g, H.reshape(self.nE, 3, 3, 3, 3)
+
+    def constraint_set(self, X, dHat2, thickness=0.0):
+        return Mesh.constraint_set(self, X, dHat2, thickness)
+
+    def newton_system(self, X, Xtilde, h, dHat2, kappa, thickness=0.0, dbc_idx=None):
+        """(rowptr, col, val (nnzb,3,3), g) of the flow variant's projected system at X."""
+        X = _f(X)
+        Xt = _f(Xtilde)
+        dbc_idx = np.zeros(0, np.int32) if dbc_idx is None else _i(dbc_idx)
+        cap = 64 * self.nV + 1024
+        while True:
+            rp = np.zeros(self.nV + 1, np.int32)
+            col = np.zeros(cap, np.int32)
+            val = np.zeros(9 * cap)
+            g = np.zeros(3 * self.nV)
+            n = lib().orc_newton_system_shell_flow(self.nV, X, Xt, self.X0, self.mass, self.nE, self.tris, self.tvol,
+                                                   self.bnode.size, self.bnode, self.bedge.shape[0], _i(self.bedge),
+                                                   self.btri.shape[0], _i(self.btri), dbc_idx.size, dbc_idx, h,
+                                                   dHat2, kappa, thickness, rp, col, val, g, cap)
+            if n >= 0:
+                return rp, col[:n], val[: 9 * n].reshape(n, 3, 3), g
+            cap = -n
+
+    def advance(self, X, vel, h, tol, dHat2, kappa, thickness=0.0, lin_rtol=1e-12, max_newton=0,
+                with_collision=True, dbc_idx=None, body_force=None):
+        """One Advance_One_Step_IE_Flow step (DBC nodes stay)."""
+        X = _f(X).copy()
+        vel = _f(vel).copy()
+        bf = None if body_force is None else _f(body_force).copy()
+        kap = np.ascontiguousarray(kappa, dtype=np.float64).copy()
+        dbc_idx = np.zeros(0, np.int32) if dbc_idx is None else _i(dbc_idx)
+        st = Stats()
+        it = lib().orc_advance_shell_flow(self.nV, X, self.X0, vel, self.mass, self.nE, self.tris, self.tvol,
+                                          self.bnode.size, self.bnode, self.bedge.shape[0], _i(self.bedge),
+                                          self.btri.shape[0], _i(self.btri), dbc_idx.size, dbc_idx, h, tol,
+                                          int(with_collision), dHat2, kap, thickness, lin_rtol, max_newton,
+                                          None if bf is None else bf.ctypes.data_as(C.c_void_p), C.byref(st))
+        return it, X.reshape(-1, 3), vel.reshape(-1, 3), kap, st.as_dict()

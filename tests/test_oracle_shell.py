@@ -1,0 +1,240 @@
+"""Pins of the oracle's shell pieces against what the mathematics fixes, and
+the C ABI's host-side shell set-up against the oracle:
+
+- Initialize_Discrete_Shell (DISCRETE_SHELL.h:219-354): degenerate
+  triangles are dropped; the lumped masses sum to density * thickness * area
+  and each vertex holds a third of its triangles' masses; vol = area *
+  thickness; the shell Lame parameters of plane stress;
+- boundary primitives of a shell (IMPLICIT_EULER.h:233-241, MESHIO.h:769-835):
+  Euler counts on closed icospheres, every edge once with the orientation of
+  the first triangle that holds it, lexicographic;
+- the flow term (INC_POTENTIAL.h:53-73, 191-211, 323-339): the energy equals
+  the edge form h/12 sum_e vol_e sum |x_i - x_j|^2 (the same quadratic
+  written without the Laplacian), the gradient is its central difference,
+  and the Hessian blocks times x give the gradient (E is quadratic);
+- the normal-flow force (DISCRETE_SHELL.h:359-395): on a regular
+  octahedron every vertex is pushed outward along its axis with length
+  magnitude * m; on a sphere every force points outward with that length;
+- the flow variant's Newton system on a shell in contact is the dense sum of
+  the flow blocks, the barrier blocks and the mass; its gradient is the
+  central difference of the total energy; the flow step leaves after one
+  Newton iteration when nothing is in contact (IMPLICIT_EULER.h:850-854), and
+  its end state in contact meets the stopping rule.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from idp_amd import capi, scenes
+from oracle import pyoracle
+
+
+def _area(X, T):
+    return 0.5 * np.linalg.norm(np.cross(X[T[:, 1]] - X[T[:, 0]], X[T[:, 2]] - X[T[:, 0]]), axis=1)
+
+
+def test_shell_init_mass_vol_and_filter():
+    X, T = scenes.icosphere(2, 0.7)
+    # two degenerate triangles: a repeated vertex and three collinear points
+    Xd = np.vstack([X, [[3.0, 0, 0], [4.0, 0, 0], [5.0, 0, 0]]])
+    n = X.shape[0]
+    Td = np.vstack([T[:5], [[0, 0, 1]], T[5:], [[n, n + 1, n + 2]]]).astype(np.int32)
+    rho, th = 2.5, 0.3
+    m = pyoracle.ShellMesh(Xd, Td, rho, th, E=10.0, nu=0.3)
+    assert m.nE == T.shape[0]
+    assert np.array_equal(m.tris.reshape(-1, 3), T)
+    A = _area(Xd, T)
+    assert np.allclose(m.tvol, A * th, rtol=1e-14)
+    ref = np.zeros(Xd.shape[0])
+    for k in range(3):
+        np.add.at(ref, T[:, k], A * th * rho / 3)
+    assert np.allclose(m.mass, ref, rtol=1e-13, atol=0)
+    assert m.mass.sum() == pytest.approx(rho * th * A.sum(), rel=1e-13)
+    assert np.allclose(m.lam, 10.0 * 0.3 / (1 - 0.09)) and np.allclose(m.mu, 10.0 / 2.6)
+    # the C ABI's host helper is the same computation
+    kept, mass, vol, lam, mu = capi.shell_init(Xd, Td, rho, 10.0, 0.3, th)
+    assert np.array_equal(kept, T)
+    assert np.array_equal(mass, m.mass) and np.array_equal(vol, m.tvol)
+    assert np.array_equal(lam, m.lam) and np.array_equal(mu, m.mu)
+
+
+@pytest.mark.parametrize("level", [0, 1, 2, 3])
+def test_shell_surface_of_closed_spheres(level):
+    X, T = scenes.icosphere(level)
+    m = pyoracle.ShellMesh(X, T, 1.0, 1.0)
+    F = T.shape[0]
+    assert m.btri.shape[0] == F and np.array_equal(m.btri, T)
+    assert m.bedge.shape[0] == 3 * F // 2
+    assert m.bnode.size == X.shape[0] == m.bedge.shape[0] - F + 2  # Euler: V - E + F = 2
+    e = m.bedge
+    assert np.all(np.lexsort((e[:, 1], e[:, 0])) == np.arange(e.shape[0]))
+    seen = {}
+    for t in T:
+        for a, b in ((t[0], t[1]), (t[1], t[2]), (t[2], t[0])):
+            if (b, a) not in seen and (a, b) not in seen:
+                seen[(a, b)] = True
+    assert sorted(seen) == [tuple(x) for x in e]
+    bn, be, bt = capi.shell_surface(X, T)
+    assert np.array_equal(bn, m.bnode) and np.array_equal(be, m.bedge) and np.array_equal(bt, m.btri)
+
+
+def test_shell_surface_with_tets_lists_the_tet_surface_first():
+    sc = scenes.small_bend()
+    Xs, Ts = scenes.icosphere(1, 0.1, (5.0, 0.0, 0.0))
+    X = np.vstack([sc.X, Xs])
+    T = (Ts + sc.X.shape[0]).astype(np.int32)
+    bn, be, bt = capi.shell_surface(X, T, sc.tets)
+    tn, te, tt = pyoracle.surface(sc.X, sc.tets)
+    assert np.array_equal(bt, np.vstack([tt, T]))
+    assert np.array_equal(bn, np.concatenate([tn, np.arange(sc.X.shape[0], X.shape[0])]))
+    assert be.shape[0] == te.shape[0] + 3 * T.shape[0] // 2
+
+
+def _jittered_sphere(seed=0):
+    X, T = scenes.icosphere(2, 0.5)
+    X = X + np.random.default_rng(seed).normal(0, 0.01, X.shape)
+    return X, T
+
+
+def test_flow_energy_is_the_edge_form():
+    X, T = _jittered_sphere()
+    m = pyoracle.ShellMesh(X, T, 1.0, 0.7)
+    h = 0.37
+    Y = X + np.random.default_rng(2).normal(0, 0.05, X.shape)
+    E, _, _ = m.flow_terms(Y, h)
+    t = m.tris.reshape(-1, 3)
+    s = sum(np.sum((Y[t[:, a]] - Y[t[:, b]]) ** 2, axis=1) for a, b in ((0, 1), (1, 2), (2, 0)))
+    ref = h / 12 * float(np.sum(m.tvol * s))
+    assert E == pytest.approx(ref, rel=1e-12)
+    assert E > 0
+
+
+def test_flow_gradient_and_hessian():
+    X, T = _jittered_sphere(1)
+    m = pyoracle.ShellMesh(X, T, 1.0, 0.7)
+    h = 0.5
+    Y = (X + np.random.default_rng(3).normal(0, 0.05, X.shape)).ravel()
+    E, g, H = m.flow_terms(Y, h)
+    rng = np.random.default_rng(4)
+    for _ in range(3):
+        d = rng.standard_normal(Y.size)
+        d /= np.linalg.norm(d)
+        eps = 1e-6
+        fd = (m.flow_terms(Y + eps * d, h)[0] - m.flow_terms(Y - eps * d, h)[0]) / (2 * eps)
+        assert fd == pytest.approx(g @ d, rel=1e-7, abs=1e-10 * np.linalg.norm(g))
+    # assembled Hessian: H x = g, symmetric, positive semidefinite, constant rows sum to zero
+    n = m.nV
+    A = np.zeros((3 * n, 3 * n))
+    t = m.tris.reshape(-1, 3)
+    for e in range(m.nE):
+        for a in range(3):
+            for b in range(3):
+                A[3 * t[e, a]:3 * t[e, a] + 3, 3 * t[e, b]:3 * t[e, b] + 3] += H[e, a, b]
+    assert np.allclose(A @ Y, g, rtol=0, atol=1e-12 * np.abs(g).max())
+    assert np.abs(A - A.T).max() == 0
+    assert np.abs(A @ np.tile([1.0, 0, 0], n)).max() < 1e-13
+    assert np.linalg.eigvalsh(A).min() > -1e-12
+
+
+def test_normal_flow_force_octahedron_and_sphere():
+    X = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], dtype=np.float64)
+    T = []
+    for sx in (0, 1):
+        for sy in (2, 3):
+            for sz in (4, 5):
+                tri = [sx, sy, sz]
+                c = np.cross(X[tri[1]] - X[tri[0]], X[tri[2]] - X[tri[0]])
+                if c @ X[tri].mean(0) < 0:
+                    tri = [sx, sz, sy]
+                T.append(tri)
+    T = np.array(T, np.int32)
+    m = pyoracle.ShellMesh(X, T, 2.0, 0.5)
+    b = m.normal_flow_force(X, 0.3)
+    assert np.allclose(b, X * (0.3 * m.mass)[:, None], rtol=1e-14, atol=1e-15)
+    Xs, Ts = _jittered_sphere(5)
+    ms = pyoracle.ShellMesh(Xs, Ts, 1.0, 1.0)
+    bs = ms.normal_flow_force(Xs, 0.02)
+    assert np.allclose(np.linalg.norm(bs, axis=1), 0.02 * ms.mass, rtol=1e-13)
+    assert np.all(np.einsum("ij,ij->i", bs, Xs) > 0)
+
+
+@pytest.fixture(scope="module")
+def pressed_pair():
+    sc = scenes.sphere_pair()
+    m = pyoracle.ShellMesh(sc.X, sc.tris, sc.density, sc.thickness)
+    dHat2, kappa = pyoracle.initialize_oipc(sc.dhat, m.mass)
+    X = sc.X.copy()
+    v = np.zeros_like(X)
+    its = []
+    for _ in range(3):
+        b = m.normal_flow_force(X, sc.magnitude)
+        it, X, v, kappa, st = m.advance(X, v, sc.h, sc.newton_tol, dHat2, kappa, body_force=b)
+        its.append((it, st["n_constraints"]))
+    return sc, m, dHat2, kappa, X, its
+
+
+def test_flow_step_leaves_without_contact_and_iterates_in_contact(pressed_pair):
+    sc, m, dHat2, kappa, X, its = pressed_pair
+    assert its[0] == (1, 0)  # no contact yet: one Newton iteration
+    assert its[1][0] > 1 and its[1][1] > 0 and its[2][1] > 0
+
+
+def _shell_energy(m, sc, X, Xt, cs, dHat2, kappa):
+    Ef, _, _ = m.flow_terms(X, sc.h)
+    dx = (X - Xt).reshape(-1, 3)
+    return Ef + 0.5 * float(np.sum(m.mass[:, None] * dx * dx)) + pyoracle.Mesh.barrier_energy(m, X, cs, dHat2,
+                                                                                             kappa)
+
+
+def test_flow_newton_system_dense_sum_and_gradient(pressed_pair):
+    sc, m, dHat2, kappa, X, _ = pressed_pair
+    X = X.ravel()
+    b = m.normal_flow_force(X, sc.magnitude)
+    Xt = X + sc.h ** 2 * (b / m.mass[:, None]).ravel()
+    r2 = 400 * dHat2  # a wider contact radius: more constraints of every kind
+    rp, col, val, g = m.newton_system(X, Xt, sc.h, r2, kappa[0])
+    n = m.nV
+    A = np.zeros((3 * n, 3 * n))
+    for r in range(n):
+        for k in range(rp[r], rp[r + 1]):
+            A[3 * r:3 * r + 3, 3 * col[k]:3 * col[k] + 3] += val[k]
+    ref = np.zeros_like(A)
+    _, _, H = m.flow_terms(X, sc.h)
+    t = m.tris.reshape(-1, 3)
+    for e in range(m.nE):
+        for a in range(3):
+            for c in range(3):
+                ref[3 * t[e, a]:3 * t[e, a] + 3, 3 * t[e, c]:3 * t[e, c] + 3] += H[e, a, c]
+    cs = m.constraint_set(X, r2)
+    assert cs.shape[0] > 50
+    Hb, vid = pyoracle.Mesh.barrier_hessian_local(m, X, cs, r2, kappa[0], project=True)
+    for i in range(cs.shape[0]):
+        nv = int((vid[i] >= 0).sum())
+        ids = vid[i][:nv]
+        Hi = Hb[i][:9 * nv * nv].reshape(3 * nv, 3 * nv)
+        for a in range(nv):
+            for c in range(nv):
+                ref[3 * ids[a]:3 * ids[a] + 3, 3 * ids[c]:3 * ids[c] + 3] += Hi[3 * a:3 * a + 3, 3 * c:3 * c + 3]
+    ref += np.diag(np.repeat(m.mass, 3))
+    assert np.abs(A - ref).max() <= 1e-12 * np.abs(ref).max()
+    rng = np.random.default_rng(9)
+    for _ in range(3):
+        d = rng.standard_normal(X.size)
+        d /= np.linalg.norm(d)
+        eps = 1e-9
+        fd = (_shell_energy(m, sc, X + eps * d, Xt, cs, r2, kappa[0]) -
+              _shell_energy(m, sc, X - eps * d, Xt, cs, r2, kappa[0])) / (2 * eps)
+        assert fd == pytest.approx(g @ d, rel=1e-5, abs=1e-6 * np.linalg.norm(g))
+
+
+def test_flow_step_end_state_meets_the_stopping_rule(pressed_pair):
+    sc, m, dHat2, kappa, X, _ = pressed_pair
+    b = m.normal_flow_force(X, sc.magnitude)
+    it, Xo, _, ko, st = m.advance(X, np.zeros_like(X), sc.h, sc.newton_tol, dHat2, kappa, body_force=b)
+    assert it > 1 and st["n_constraints"] > 0
+    Xt = (X + sc.h ** 2 * b / m.mass[:, None]).ravel()
+    rp, col, val, g = m.newton_system(Xo, Xt, sc.h, dHat2, ko[0])
+    A = sp.bsr_matrix((val, col, rp), shape=(3 * m.nV, 3 * m.nV)).tocsc()
+    p = spla.spsolve(A, -g)
+    assert np.sqrt(np.sum(p * p) / m.nV) / sc.h <= sc.newton_tol
