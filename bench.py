@@ -633,6 +633,8 @@ def bench_plain(a):
     pairs = int(sum(prof["barrier_pairs"]))
     kern["barrier_hess"]["pairs"] = pairs
     kern["barrier_hess"]["clamped"] = int(sum(prof["barrier_clamped"]))
+    # evaluations per constraint kind: PT, EE, PE, PP, mollified EE, PE, PP
+    kern["barrier_hess"]["pairs_by_kind"] = [int(x) for x in prof["barrier_pairs"]]
     kern["ccd"]["steps"] = prof["ccd_steps"]
     kern["ccd"]["max_steps"] = prof["ccd_max_steps"]
     # what a graph of the PCG iterations could remove at most: the PCG stage's

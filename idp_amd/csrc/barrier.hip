@@ -12,8 +12,12 @@
 // then H_x = B^T H_y B.  The PSD clamp of H_x is computed exactly in the
 // reduced space: with C = (b b^T)^(1/2) (x) I3, the nonzero spectrum of
 // H_x equals that of K = C H_y C, and makePD(H_x) = B^T C^-1 makePD(K) C^-1 B.
-// K is diagonalised by cyclic Jacobi, one constraint per lane (packed K in
-// registers, eigenvectors in LDS).
+// The Hessian runs in two stages over the constraints grouped by reduced
+// order: k_barrier_form (one per lane) forms H_y and K; then K of order 6
+// (PE) and 3 (PP) is diagonalised by cyclic Jacobi one constraint per lane
+// with K and its eigenvectors in registers (k_barrier_hess_lane), and K of
+// order 9 (PT, EE, mollified) by a round-robin parallel Jacobi on a team of
+// lanes in LDS (k_barrier_hess).
 #include "common.hpp"
 #include "geom_dev.hpp"
 #include "kernels.hpp"
@@ -246,6 +250,17 @@ template <> struct KT<K_PP> { static constexpr int M = 1, NV = 2; static constex
 template <> struct KT<K_MEE> { static constexpr int M = 3, NV = 4; static constexpr bool MOLL = true; };
 template <> struct KT<K_MPE> { static constexpr int M = 3, NV = 4; static constexpr bool MOLL = true; };
 template <> struct KT<K_MPP> { static constexpr int M = 3, NV = 4; static constexpr bool MOLL = true; };
+
+// Range [x, y) of the constraints whose reduced order is N (9: PT, EE and the
+// mollified kinds; 6: PE; 3: PP) in the grouped order, from the kind counts
+__device__ inline int2 group_range(const unsigned* __restrict__ cnt, int N)
+{
+    const int n9 = (int)(cnt[K_PT] + cnt[K_EE] + cnt[K_MEE] + cnt[K_MPE] + cnt[K_MPP]);
+    const int n6 = (int)cnt[K_PE], n3 = (int)cnt[K_PP];
+    if (N == 9) return make_int2(0, n9);
+    if (N == 6) return make_int2(n9, n9 + n6);
+    return make_int2(n9 + n6, n9 + n6 + n3);
+}
 
 // Symmetric matrices are held as packed upper triangles: entry (i, j) of an
 // N x N matrix at sx<N>(i, j).  All loops over entries are fully unrolled, so
@@ -822,19 +837,15 @@ __device__ inline bool team_project(TeamLds& T, int tl, int n, int M, const doub
 // result is written at the constraint's own index.
 // counts (null: none): per kind, evaluations and clamped ones, one atomic
 // per wavefront and kind.
+// one constraint (t < n; t == n: an idle team that still takes part in the
+// wavefront's ballots)
 template <int TL>
-__global__ void __launch_bounds__(HB) k_barrier_hess(long n, const int* __restrict__ perm,
-                                                     const int4* __restrict__ cs, const double* __restrict__ F,
-                                                     const unsigned char* __restrict__ ok_in, int project_spd,
-                                                     int mode, double* __restrict__ out, int* __restrict__ vid,
-                                                     int* __restrict__ counts)
+__device__ inline void team_body(TeamLds& T, int tl, long t, long n, const int* __restrict__ perm,
+                                 const int4* __restrict__ cs, const double* __restrict__ F,
+                                 const unsigned char* __restrict__ ok_in, int project_spd, int mode,
+                                 double* __restrict__ out, int* __restrict__ vid, int* __restrict__ counts)
 {
 #pragma clang fp contract(fast)
-    constexpr int TEAMS = HB / TL;
-    __shared__ TeamLds lds[TEAMS];
-    const int team = threadIdx.x / TL, tl = threadIdx.x % TL;
-    TeamLds& T = lds[team];
-    const long t = (long)blockIdx.x * TEAMS + team;
     // the team loads stage 1's packed K and H_y into LDS as full matrices
     if (tl == 0) {
         T.ok = 2;
@@ -915,6 +926,263 @@ __global__ void __launch_bounds__(HB) k_barrier_hess(long n, const int* __restri
             }
         }
     }
+    team_sync();  // T is reused by the team's next constraint
+}
+
+template <int TL>
+__global__ void __launch_bounds__(HB) k_barrier_hess(long n, const unsigned* __restrict__ kcnt,
+                                                     const int* __restrict__ perm, const int4* __restrict__ cs,
+                                                     const double* __restrict__ F,
+                                                     const unsigned char* __restrict__ ok_in, int project_spd,
+                                                     int mode, double* __restrict__ out, int* __restrict__ vid,
+                                                     int* __restrict__ counts)
+{
+#pragma clang fp contract(fast)
+    constexpr int TEAMS = HB / TL;
+    __shared__ TeamLds lds[TEAMS];
+    const int team = threadIdx.x / TL, tl = threadIdx.x % TL;
+    TeamLds& T = lds[team];
+    // with the kinds grouped (kcnt), this kernel takes the order-9 group;
+    // grid-stride, the same number of passes for every team
+    const long hi = kcnt ? (long)group_range(kcnt, 9).y : n;
+    const long stride = (long)gridDim.x * TEAMS;
+    const long iters = (hi + stride - 1) / stride;
+    for (long it = 0; it < iters; ++it) {
+        const long t = it * stride + (long)blockIdx.x * TEAMS + team;
+        team_body<TL>(T, tl, t < hi ? t : n, n, perm, cs, F, ok_in, project_spd, mode, out, vid, counts);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stage 2, one constraint per lane: cyclic Jacobi on the packed K (order N =
+// 9, 6 or 3) with the eigenvectors, both in registers (every index a
+// compile-time constant), then the clamp, C^-1 K+ C^-1 and the vertex blocks.
+// Each rotation is the classical one (Golub & Van Loan 8.4): t from
+// theta = (aqq - app) / (2 apq), app - t apq, aqq + t apq, apq = 0; sweeps run
+// in row order (p < q) until off(K)^2 <= 1e-34 sum K_ii^2 (at most 16).
+// ---------------------------------------------------------------------------
+// reduced-coordinate maps y = B x per vertex (rows a, columns vertex), the
+// same as the constant tables of barrier_init_constants
+__device__ constexpr double kBpt[12] = {1, -1, 0, 0, 0, -1, 1, 0, 0, -1, 0, 1};
+__device__ constexpr double kBee[12] = {-1, 0, 1, 0, -1, 1, 0, 0, 0, 0, -1, 1};
+__device__ constexpr double kBpe[6] = {1, -1, 0, 0, -1, 1};
+
+template <int N>
+__device__ inline void lane_jacobi(double (&K)[N * (N + 1) / 2], double (&V)[N * N])
+{
+#pragma clang fp contract(fast)
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) V[N * i + j] = (i == j) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 16; ++sweep) {
+        double off = 0.0, dg = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            dg += K[sx<N>(i, i)] * K[sx<N>(i, i)];
+#pragma unroll
+            for (int j = i + 1; j < N; ++j) off += K[sx<N>(i, j)] * K[sx<N>(i, j)];
+        }
+        if (!(off > 1e-34 * dg)) break;
+#pragma unroll
+        for (int p = 0; p < N - 1; ++p)
+#pragma unroll
+            for (int q = p + 1; q < N; ++q) {
+                const double apq = K[sx<N>(p, q)];
+                const double dd = K[sx<N>(q, q)] - K[sx<N>(p, p)];
+                const double rt = sqrt(dd * dd + 4.0 * apq * apq);
+                // apq == 0: the identity (t = 0), also when dd == 0
+                const double t = apq != 0.0 ? copysign(1.0, dd) * (2.0 * apq) / (fabs(dd) + rt) : 0.0;
+                const double c = rsqrt(t * t + 1.0);
+                const double sn = t * c;
+#pragma unroll
+                for (int k = 0; k < N; ++k) {
+                    if (k == p || k == q) continue;
+                    const double kp = K[sx<N>(k, p)], kq = K[sx<N>(k, q)];
+                    K[sx<N>(k, p)] = c * kp - sn * kq;
+                    K[sx<N>(k, q)] = sn * kp + c * kq;
+                }
+                K[sx<N>(p, p)] -= t * apq;
+                K[sx<N>(q, q)] += t * apq;
+                K[sx<N>(p, q)] = 0.0;
+#pragma unroll
+                for (int k = 0; k < N; ++k) {
+                    const double vp = V[N * k + p], vq = V[N * k + q];
+                    V[N * k + p] = c * vp - sn * vq;
+                    V[N * k + q] = sn * vp + c * vq;
+                }
+            }
+    }
+}
+
+// projected H_y of one constraint (packed, order N = 3 M) from the form
+// stage's record Fi (packed K, then packed H_y); returns whether anything
+// was clamped
+template <int N>
+__device__ inline bool lane_project(const double* __restrict__ Fi, const double* Si, int project_spd,
+                                    double (&H)[N * (N + 1) / 2])
+{
+#pragma clang fp contract(fast)
+    constexpr int P = N * (N + 1) / 2, M = N / 3;
+    if (project_spd) {
+        double K[P], V[N * N];
+#pragma unroll
+        for (int q = 0; q < P; ++q) K[q] = Fi[q];
+        lane_jacobi<N>(K, V);
+        double lmin = K[0];
+#pragma unroll
+        for (int i = 1; i < N; ++i) lmin = fmin(lmin, K[sx<N>(i, i)]);
+        if (lmin < 0.0) {  // UTILS.h:12-15: clamp only when an eigenvalue is negative
+            double L[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) L[i] = fmax(K[sx<N>(i, i)], 0.0);
+            double Kp[P];
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+#pragma unroll
+                for (int j = i; j < N; ++j) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int k = 0; k < N; ++k) acc += V[N * i + k] * L[k] * V[N * j + k];
+                    Kp[sx<N>(i, j)] = acc;
+                }
+            // H_y = C^-1 K+ C^-1 with C^-1 = Si (x) I3 (order 3: C = sqrt(2) I)
+#pragma unroll
+            for (int a = 0; a < M; ++a)
+#pragma unroll
+                for (int b = a; b < M; ++b)
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) {
+                            if (a == b && j < i) continue;
+                            double acc = 0.0;
+                            if (M == 1) {
+                                acc = 0.5 * Kp[sx<N>(i, j)];
+                            } else {
+#pragma unroll
+                                for (int gg = 0; gg < M; ++gg)
+#pragma unroll
+                                    for (int hh = 0; hh < M; ++hh)
+                                        acc += Si[a * M + gg] * Kp[sx<N>(3 * gg + i, 3 * hh + j)] * Si[hh * M + b];
+                            }
+                            H[sx<N>(3 * a + i, 3 * b + j)] = acc;
+                        }
+            return true;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < P; ++q) H[q] = Fi[45 + q];
+    return false;
+}
+
+// vertex blocks of B^T H_y B into o (mode 0: dense (3 NV)^2 row-major in 144
+// slots, zero filled; mode 1: block (a, b) at 9 (a NV + b))
+template <int N, bool PT>
+__device__ inline void lane_blocks(const double (&H)[N * (N + 1) / 2], int mode, double* __restrict__ o)
+{
+#pragma clang fp contract(fast)
+    constexpr int M = N / 3, NV = M + 1;
+    constexpr const double* B = (M == 3) ? (PT ? kBpt : kBee) : kBpe;
+#pragma unroll
+    for (int vi = 0; vi < NV; ++vi)
+#pragma unroll
+        for (int vj = 0; vj < NV; ++vj)
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c2 = 0; c2 < 3; ++c2) {
+                    double acc = 0.0;
+                    if (M == 1) {
+                        acc = (vi == vj ? 1.0 : -1.0) * H[sx<N>(r, c2)];
+                    } else {
+#pragma unroll
+                        for (int a = 0; a < M; ++a)
+#pragma unroll
+                            for (int b = 0; b < M; ++b) {
+                                const double sc = B[a * NV + vi] * B[b * NV + vj];
+                                if (sc != 0.0) acc += sc * H[sx<N>(3 * a + r, 3 * b + c2)];
+                            }
+                    }
+                    if (mode == 0) o[(3 * vi + r) * (3 * NV) + 3 * vj + c2] = acc;
+                    else o[9 * (vi * NV + vj) + 3 * r + c2] = acc;
+                }
+    if (mode == 0)
+#pragma unroll
+        for (int j = 9 * NV * NV; j < 144; ++j) o[j] = 0.0;
+}
+
+template <int N>
+__device__ inline bool lane_hess(int kind, const double* __restrict__ Fi, int project_spd, int mode,
+                                 double* __restrict__ o)
+{
+    constexpr int M = N / 3;
+    const double* Si = (M == 3) ? ((kind == K_PT) ? c_Si3pt : c_Si3ee) : c_Si2;
+    double H[N * (N + 1) / 2];
+    const bool clamped = lane_project<N>(Fi, Si, project_spd, H);
+    if (M == 3 && kind == K_PT) lane_blocks<N, true>(H, mode, o);
+    else lane_blocks<N, false>(H, mode, o);
+    return clamped;
+}
+
+// Stage 2 for the constraints of reduced order N (kinds grouped: kcnt gives
+// the group's range), one per lane, grid-stride; an idle lane still takes part
+// in the wavefront's ballots.
+template <int N>
+__device__ inline void lane_body(long t, long n, const int* __restrict__ perm, const int4* __restrict__ cs,
+                                 const double* __restrict__ F, const unsigned char* __restrict__ ok_in,
+                                 int project_spd, int mode, double* __restrict__ out, int* __restrict__ vid,
+                                 int* __restrict__ counts)
+{
+    int kind = -1, ok = 2;
+    bool clamped = false;
+    if (t < n) {
+        const long i = perm[t];
+        const Decoded k = decode(cs[i]);
+        ok = ok_in[t];
+        if (vid && ok != 2)
+            for (int j = 0; j < 4; ++j) vid[4 * i + j] = (j < k.nv) ? k.v[j] : -1;
+        double* o = out + 144 * i;
+        if (ok == 0) {
+            for (int j = 0; j < 144; ++j) o[j] = 0.0;
+        } else if (ok == 1) {
+            kind = k.kind;
+            clamped = lane_hess<N>(kind, F + 90 * t, project_spd, mode, o);
+        }
+    }
+    if (counts) {
+        const int lane = threadIdx.x & 63;
+        for (int q = 0; q < 7; ++q) {
+            const unsigned long long m = __ballot(kind == q), mc = __ballot(kind == q && clamped);
+            if (m && lane == __ffsll((long long)m) - 1) {
+                atomicAdd(counts + 2 * q, __popcll(m));
+                if (mc) atomicAdd(counts + 2 * q + 1, __popcll(mc));
+            }
+        }
+    }
+}
+
+template <int N>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N == 3 ? 4 : 2, 8)))
+k_barrier_hess_lane(long n, const unsigned* __restrict__ kcnt, const int* __restrict__ perm,
+                    const int4* __restrict__ cs, const double* __restrict__ F, const unsigned char* __restrict__ ok_in,
+                    int project_spd, int mode, double* __restrict__ out, int* __restrict__ vid,
+                    int* __restrict__ counts)
+{
+    const int2 r = group_range(kcnt, N);
+    const long stride = (long)gridDim.x * blockDim.x;
+    const long span = r.y - r.x;
+    const long iters = (span + stride - 1) / stride;  // the same for every lane
+    for (long it = 0; it < iters; ++it) {
+        const long t = r.x + it * stride + (long)blockIdx.x * blockDim.x + threadIdx.x;
+        lane_body<N>(t < r.y ? t : n, n, perm, cs, F, ok_in, project_spd, mode, out, vid, counts);
+    }
+}
+
+// position of a kind in the grouped order (reduced order 9, then 6, then 3)
+__device__ inline int kind_rank(int kind)
+{
+    return kind == K_PE ? 5 : (kind == K_PP ? 6 : (kind <= K_EE ? kind : kind - 2));
 }
 
 // Constraint indices grouped by kind: k_kind_count counts the kinds
@@ -938,8 +1206,10 @@ __global__ void __launch_bounds__(256) k_kind_scatter(long n, const int4* __rest
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int kind = i < n ? decode(cs[i]).kind : -1;
     const int lane = threadIdx.x & 63;
+    // groups in the order PT, EE, MEE, MPE, MPP (reduced order 9), PE (6),
+    // PP (3): the lanes of a wavefront then share the projection's size
     unsigned off = 0;
-    for (int q = 0; q < 7; ++q) off += (q < kind) ? cnt[q] : 0u;
+    for (int q = 0; q < 7; ++q) off += (kind >= 0 && kind_rank(q) < kind_rank(kind)) ? cnt[q] : 0u;
     for (int q = 0; q < 7; ++q) {
         const unsigned long long m = __ballot(kind == q);
         if (!m) continue;
@@ -995,21 +1265,32 @@ void launch_barrier_hess(hipStream_t s, long n, const int* cs, const double* X, 
         perm = nullptr;
     }
     IDP_LAUNCH(k_barrier_form, grid_for(n, 256), 256, 0, s, n, perm, (const int4*)cs, X, X0, bp, form, form_ok, err);
+    // grouped: order 9 on teams of `team` lanes, orders 6 and 3 one per lane;
+    // every kernel walks its group's range (read on the device) grid-stride
+    const unsigned* cnt = perm ? kcnt : nullptr;
+    const int gl = (int)std::min<long>(grid_for(n, 256), 2048);
+    if (cnt) {
+        IDP_LAUNCH(k_barrier_hess_lane<6>, gl, 256, 0, s, n, cnt, perm, (const int4*)cs, form, form_ok, project_spd,
+                   mode, out, vid, counts);
+        IDP_LAUNCH(k_barrier_hess_lane<3>, gl, 256, 0, s, n, cnt, perm, (const int4*)cs, form, form_ok, project_spd,
+                   mode, out, vid, counts);
+    }
+    auto team_grid = [&](int lanes) { return (int)std::min<long>(grid_for(n, HB / lanes), 16384); };
     switch (team) {
     case 8:
-        IDP_LAUNCH(k_barrier_hess<8>, grid_for(n, HB / 8), HB, 0, s, n, perm, (const int4*)cs, form, form_ok,
+        IDP_LAUNCH(k_barrier_hess<8>, team_grid(8), HB, 0, s, n, cnt, perm, (const int4*)cs, form, form_ok,
                    project_spd, mode, out, vid, counts);
         break;
     case 32:
-        IDP_LAUNCH(k_barrier_hess<32>, grid_for(n, HB / 32), HB, 0, s, n, perm, (const int4*)cs, form, form_ok,
+        IDP_LAUNCH(k_barrier_hess<32>, team_grid(32), HB, 0, s, n, cnt, perm, (const int4*)cs, form, form_ok,
                    project_spd, mode, out, vid, counts);
         break;
     case 64:
-        IDP_LAUNCH(k_barrier_hess<64>, grid_for(n, 1), HB, 0, s, n, perm, (const int4*)cs, form, form_ok,
+        IDP_LAUNCH(k_barrier_hess<64>, team_grid(64), HB, 0, s, n, cnt, perm, (const int4*)cs, form, form_ok,
                    project_spd, mode, out, vid, counts);
         break;
     default:
-        IDP_LAUNCH(k_barrier_hess<16>, grid_for(n, HB / 16), HB, 0, s, n, perm, (const int4*)cs, form, form_ok,
+        IDP_LAUNCH(k_barrier_hess<16>, team_grid(16), HB, 0, s, n, cnt, perm, (const int4*)cs, form, form_ok,
                    project_spd, mode, out, vid, counts);
         break;
     }
