@@ -39,8 +39,8 @@ from . import capi, driver, scenes
 SHAPE_TABLE = [(4, False), (8, False), (16, False), (32, False), (4, True), (8, True), (16, True), (32, True)]
 UPD_WAVES = [1, 2, 4, 8]
 DEFAULT = (5, 4)
-TEAMS = [8, 16, 32, 64]
-DEFAULT_TEAM = 16
+TEAMS = [3, 8, 16, 32, 64]
+DEFAULT_TEAM = 3
 
 
 def _workload(name):

@@ -77,7 +77,7 @@ def tuned_barrier_team(path=None):
     try:
         with open(path or TUNING_PATH) as f:
             t = int(json.load(f)["barrier_team"])
-        return t if t in (8, 16, 32, 64) else None
+        return t if t in (3, 8, 16, 32, 64) else None
     except (OSError, ValueError, KeyError, TypeError):
         return None
 

@@ -797,7 +797,7 @@ __device__ inline bool team_project(TeamLds& T, int tl, int n, int M, const doub
     double lmin = K[0];
     for (int i = 1; i < n; ++i) lmin = fmin(lmin, K[10 * i]);
     if (lmin >= 0.0) return false;  // UTILS.h:12-15: nothing to clamp
-    if (tl < n) T.lam[tl] = fmax(K[10 * tl], 0.0);
+    for (int i = tl; i < n; i += TL) T.lam[i] = fmax(K[10 * i], 0.0);  // TL may be below n
     team_sync();
     // K+ = V max(L, 0) V^T (upper triangle, mirrored)
     for (int idx = tl; idx < n * n; idx += TL) {
@@ -991,8 +991,10 @@ __device__ inline void lane_jacobi(double (&K)[N * (N + 1) / 2], double (&V)[N *
                 const double apq = K[sx<N>(p, q)];
                 const double dd = K[sx<N>(q, q)] - K[sx<N>(p, p)];
                 const double rt = sqrt(dd * dd + 4.0 * apq * apq);
-                // apq == 0: the identity (t = 0), also when dd == 0
-                const double t = apq != 0.0 ? copysign(1.0, dd) * (2.0 * apq) / (fabs(dd) + rt) : 0.0;
+                // apq == 0: the identity (t = 0), also when dd == 0; no branch, so
+                // the whole sweep is one block and its values stay in registers
+                const double den = fabs(dd) + rt;
+                const double t = copysign(1.0, dd) * (2.0 * apq) / (den != 0.0 ? den : 1.0);
                 const double c = rsqrt(t * t + 1.0);
                 const double sn = t * c;
 #pragma unroll
@@ -1179,46 +1181,296 @@ k_barrier_hess_lane(long n, const unsigned* __restrict__ kcnt, const int* __rest
     }
 }
 
+// ---------------------------------------------------------------------------
+// Order 9 (PT, EE and the mollified kinds): three lanes per constraint, lane
+// r = 0, 1, 2 of a triple owning coordinate component r.  Every lane of the
+// triple runs the same cyclic Jacobi on the whole packed K (identical
+// arithmetic, so the three copies stay equal and the triple stays in step)
+// and rotates only the eigenvector rows (a, r), a = 0..2, of its component.
+// U = C^-1 V mixes rows of one component only, so each lane forms
+// Z_r[v][k] = sum_a B[a][v] U[(a, r)][k] for the 4 vertices; the vertex
+// blocks are sum_k max(L_k, 0) Z_r[vi][k] Z_c[vj][k], with Z_c from lane c of
+// the triple by shuffles.  Lane r writes the entries of row component r.
+// 21 triples per wavefront (lane 63 idles).
+// ---------------------------------------------------------------------------
+constexpr int TRI_PER_WAVE = 21;
+
+__device__ inline void tri_jacobi(double (&K)[45], double (&V)[27], int r)
+{
+#pragma clang fp contract(fast)
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) V[9 * a + k] = (3 * a + r == k) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 16; ++sweep) {
+        double off = 0.0, dg = 0.0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            dg += K[sx<9>(i, i)] * K[sx<9>(i, i)];
+#pragma unroll
+            for (int j = i + 1; j < 9; ++j) off += K[sx<9>(i, j)] * K[sx<9>(i, j)];
+        }
+        if (!(off > 1e-34 * dg)) break;
+#pragma unroll
+        for (int p = 0; p < 8; ++p)
+#pragma unroll
+            for (int q = p + 1; q < 9; ++q) {
+                const double apq = K[sx<9>(p, q)];
+                const double dd = K[sx<9>(q, q)] - K[sx<9>(p, p)];
+                const double rt = sqrt(dd * dd + 4.0 * apq * apq);
+                const double den = fabs(dd) + rt;  // as in lane_jacobi: branch free
+                const double t = copysign(1.0, dd) * (2.0 * apq) / (den != 0.0 ? den : 1.0);
+                const double c = rsqrt(t * t + 1.0);
+                const double sn = t * c;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    if (k == p || k == q) continue;
+                    const double kp = K[sx<9>(k, p)], kq = K[sx<9>(k, q)];
+                    K[sx<9>(k, p)] = c * kp - sn * kq;
+                    K[sx<9>(k, q)] = sn * kp + c * kq;
+                }
+                K[sx<9>(p, p)] -= t * apq;
+                K[sx<9>(q, q)] += t * apq;
+                K[sx<9>(p, q)] = 0.0;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const double vp = V[9 * a + p], vq = V[9 * a + q];
+                    V[9 * a + p] = c * vp - sn * vq;
+                    V[9 * a + q] = sn * vp + c * vq;
+                }
+            }
+    }
+}
+
+// this lane's 48 entries (row component r) of the 4 x 4 vertex blocks
+template <bool PT>
+__device__ inline void tri_blocks(int r, int base, int mode, bool clamped, const double (&L)[9],
+                                  const double (&V)[27], const double* Si, const double* __restrict__ Hy,
+                                  double* __restrict__ o)
+{
+#pragma clang fp contract(fast)
+    constexpr const double* B = PT ? kBpt : kBee;
+    if (!clamped) {  // B^T H_y B from the packed H_y of the record
+#pragma unroll
+        for (int vi = 0; vi < 4; ++vi)
+#pragma unroll
+            for (int vj = 0; vj < 4; ++vj)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+#pragma unroll
+                        for (int b = 0; b < 3; ++b) {
+                            const double sc = B[a * 4 + vi] * B[b * 4 + vj];
+                            if (sc != 0.0) acc += sc * Hy[sx<9>(3 * a + r, 3 * b + c)];
+                        }
+                    if (mode == 0) o[(3 * vi + r) * 12 + 3 * vj + c] = acc;
+                    else o[9 * (vi * 4 + vj) + 3 * r + c] = acc;
+                }
+        return;
+    }
+    double Z[4][9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        double u[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) u[a] = Si[3 * a] * V[k] + Si[3 * a + 1] * V[9 + k] + Si[3 * a + 2] * V[18 + k];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            double acc = 0.0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                if (B[a * 4 + v] != 0.0) acc += B[a * 4 + v] * u[a];
+            Z[v][k] = acc;
+        }
+    }
+#pragma unroll
+    for (int sft = 0; sft < 3; ++sft) {
+        const int c = (r + sft) % 3;
+#pragma unroll
+        for (int vj = 0; vj < 4; ++vj) {
+            double zc[9];  // row vj of Z_c, from lane c of the triple
+#pragma unroll
+            for (int k = 0; k < 9; ++k) zc[k] = __shfl(Z[vj][k], base + c);
+#pragma unroll
+            for (int vi = 0; vi < 4; ++vi) {
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) acc += L[k] * (Z[vi][k] * zc[k]);
+                if (mode == 0) o[(3 * vi + r) * 12 + 3 * vj + c] = acc;
+                else o[9 * (vi * 4 + vj) + 3 * r + c] = acc;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
+k_barrier_hess_tri(long n, const unsigned* __restrict__ kcnt, const int* __restrict__ perm,
+                   const int4* __restrict__ cs, const double* __restrict__ F, const unsigned char* __restrict__ ok_in,
+                   int project_spd, int mode, double* __restrict__ out, int* __restrict__ vid,
+                   int* __restrict__ counts)
+{
+    const int2 rg = group_range(kcnt, 9);
+    const int wl = threadIdx.x & 63, g = wl / 3, r = wl % 3, base = 3 * g;
+    const long waves = (long)gridDim.x * (blockDim.x / 64);
+    const long stride = waves * TRI_PER_WAVE;
+    const long wave_id = (long)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const long span = rg.y - rg.x;
+    const long iters = (span + stride - 1) / stride;
+    for (long it = 0; it < iters; ++it) {
+        const long t = rg.x + it * stride + wave_id * TRI_PER_WAVE + g;
+        int kind = -1, ok = 2;
+        bool clamped = false;
+        if (g < TRI_PER_WAVE && t < rg.y) {
+            const long i = perm[t];
+            const Decoded k = decode(cs[i]);
+            ok = ok_in[t];
+            if (r == 0 && vid && ok != 2)
+                for (int j = 0; j < 4; ++j) vid[4 * i + j] = (j < k.nv) ? k.v[j] : -1;
+            double* o = out + 144 * i;
+            if (ok == 0) {
+                if (r == 0)
+                    for (int j = 0; j < 144; ++j) o[j] = 0.0;
+            } else if (ok == 1) {
+                kind = k.kind;
+                const double* Fi = F + 90 * t;
+                const double* Si = (kind == K_PT) ? c_Si3pt : c_Si3ee;
+                double L[9], V[27];
+                if (project_spd) {
+                    double K[45];
+#pragma unroll
+                    for (int q = 0; q < 45; ++q) K[q] = Fi[q];
+                    tri_jacobi(K, V, r);
+                    double lmin = K[0];
+#pragma unroll
+                    for (int q = 1; q < 9; ++q) lmin = fmin(lmin, K[sx<9>(q, q)]);
+                    clamped = lmin < 0.0;  // UTILS.h:12-15
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) L[q] = fmax(K[sx<9>(q, q)], 0.0);
+                }
+                // unclamped: the packed H_y of the record (Fi + 45) is the result
+                if (kind == K_PT) tri_blocks<true>(r, base, mode, clamped, L, V, Si, Fi + 45, o);
+                else tri_blocks<false>(r, base, mode, clamped, L, V, Si, Fi + 45, o);
+            }
+        }
+        if (counts) {
+            const bool lead = r == 0;
+            for (int q = 0; q < 7; ++q) {
+                const unsigned long long m = __ballot(lead && kind == q), mc = __ballot(lead && kind == q && clamped);
+                if (m && wl == __ffsll((long long)m) - 1) {
+                    atomicAdd(counts + 2 * q, __popcll(m));
+                    if (mc) atomicAdd(counts + 2 * q + 1, __popcll(mc));
+                }
+            }
+        }
+    }
+}
+
 // position of a kind in the grouped order (reduced order 9, then 6, then 3)
 __device__ inline int kind_rank(int kind)
 {
     return kind == K_PE ? 5 : (kind == K_PP ? 6 : (kind <= K_EE ? kind : kind - 2));
 }
 
-// Constraint indices grouped by kind: k_kind_count counts the kinds
-// (cnt[0..7)), k_kind_scatter writes each index into its kind's range of perm
-// (cnt[8..15) are the running cursors; both halves start at zero).  The
-// order inside a kind is arbitrary; the Hessian writes by constraint index.
-__global__ void __launch_bounds__(256) k_kind_count(long n, const int4* __restrict__ cs, unsigned* __restrict__ cnt)
+// Constraint indices grouped by kind (a counting sort, stable, no atomics):
+// k_kind_count writes each block's count per kind (bcnt[8 b + q]);
+// k_kind_scan turns them into each block's offset inside its kind (boff) and
+// the totals (cnt[0..7)); k_kind_scatter places every index at its group's
+// offset (kinds in kind_rank order) + its block's offset + its rank among the
+// block's lanes of that kind.  The Hessian writes by constraint index, so the
+// order only groups the work.
+constexpr int KIND_BLOCK = 256;
+
+__device__ inline void block_kind_counts(int kind, unsigned (&wc)[KIND_BLOCK / 64][8])
 {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int kind = i < n ? decode(cs[i]).kind : -1;
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, w = threadIdx.x / 64;
+#pragma unroll
     for (int q = 0; q < 7; ++q) {
         const unsigned long long m = __ballot(kind == q);
-        if (m && lane == __ffsll((long long)m) - 1) atomicAdd(cnt + q, (unsigned)__popcll(m));
+        if (lane == 0) wc[w][q] = (unsigned)__popcll(m);
     }
 }
 
-__global__ void __launch_bounds__(256) k_kind_scatter(long n, const int4* __restrict__ cs,
-                                                      unsigned* __restrict__ cnt, int* __restrict__ perm)
+__global__ void __launch_bounds__(KIND_BLOCK) k_kind_count(long n, const int4* __restrict__ cs,
+                                                           unsigned* __restrict__ bcnt)
 {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ unsigned wc[KIND_BLOCK / 64][8];
+    const long i = (long)blockIdx.x * KIND_BLOCK + threadIdx.x;
+    block_kind_counts(i < n ? decode(cs[i]).kind : -1, wc);
+    __syncthreads();
+    if (threadIdx.x < 7) {
+        unsigned c = 0;
+        for (int w = 0; w < KIND_BLOCK / 64; ++w) c += wc[w][threadIdx.x];
+        bcnt[8 * (long)blockIdx.x + threadIdx.x] = c;
+    }
+}
+
+// one workgroup: exclusive scan over the blocks for every kind at once;
+// thread t sums a contiguous run of blocks, the 256 run totals are scanned in
+// LDS (one Hillis-Steele pass for the 7 kinds), and each thread then writes
+// its run's offsets
+__global__ void __launch_bounds__(256) k_kind_scan(int nb, const unsigned* __restrict__ bcnt,
+                                                   unsigned* __restrict__ boff, unsigned* __restrict__ cnt)
+{
+    __shared__ unsigned sh[7][256];
+    const int per = (nb + 255) / 256;
+    const int b0 = min(nb, (int)threadIdx.x * per), b1 = min(nb, b0 + per);
+    unsigned run[7];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) run[q] = 0;
+    for (int b = b0; b < b1; ++b)
+#pragma unroll
+        for (int q = 0; q < 7; ++q) run[q] += bcnt[8 * (long)b + q];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) sh[q][threadIdx.x] = run[q];
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {  // inclusive scan of the run totals
+        unsigned a[7];
+#pragma unroll
+        for (int q = 0; q < 7; ++q) a[q] = threadIdx.x >= (unsigned)d ? sh[q][threadIdx.x - d] : 0u;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 7; ++q) sh[q][threadIdx.x] += a[q];
+        __syncthreads();
+    }
+    unsigned acc[7];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) acc[q] = sh[q][threadIdx.x] - run[q];
+    for (int b = b0; b < b1; ++b)
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {
+            boff[8 * (long)b + q] = acc[q];
+            acc[q] += bcnt[8 * (long)b + q];
+        }
+    if (threadIdx.x < 7) cnt[threadIdx.x] = sh[threadIdx.x][255];
+}
+
+__global__ void __launch_bounds__(KIND_BLOCK) k_kind_scatter(long n, const int4* __restrict__ cs,
+                                                             const unsigned* __restrict__ boff,
+                                                             const unsigned* __restrict__ cnt, int* __restrict__ perm)
+{
+    __shared__ unsigned wc[KIND_BLOCK / 64][8];
+    const long i = (long)blockIdx.x * KIND_BLOCK + threadIdx.x;
     const int kind = i < n ? decode(cs[i]).kind : -1;
-    const int lane = threadIdx.x & 63;
+    block_kind_counts(kind, wc);
+    __syncthreads();
+    if (kind < 0) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x / 64;
     // groups in the order PT, EE, MEE, MPE, MPP (reduced order 9), PE (6),
     // PP (3): the lanes of a wavefront then share the projection's size
     unsigned off = 0;
-    for (int q = 0; q < 7; ++q) off += (kind >= 0 && kind_rank(q) < kind_rank(kind)) ? cnt[q] : 0u;
+    for (int q = 0; q < 7; ++q) off += (kind_rank(q) < kind_rank(kind)) ? cnt[q] : 0u;
+    off += boff[8 * (long)blockIdx.x + kind];
+    for (int v = 0; v < w; ++v) off += wc[v][kind];
+    unsigned long long mine = 0;  // lanes of this wavefront with this lane's kind
+#pragma unroll
     for (int q = 0; q < 7; ++q) {
         const unsigned long long m = __ballot(kind == q);
-        if (!m) continue;
-        const int leader = __ffsll((long long)m) - 1;
-        unsigned base = 0;
-        if (lane == leader) base = atomicAdd(cnt + 8 + q, (unsigned)__popcll(m));
-        base = __shfl(base, leader);
-        if (kind == q) perm[off + base + (unsigned)__popcll(m & ((1ull << lane) - 1))] = (int)i;
+        if (kind == q) mine = m;
     }
+    perm[off + (unsigned)__popcll(mine & ((1ull << lane) - 1))] = (int)i;
 }
 
 // vertex ids of each constraint in decode order (-1 padded)
@@ -1258,9 +1510,12 @@ void launch_barrier_hess(hipStream_t s, long n, const int* cs, const double* X, 
 {
     if (n <= 0) return;
     if (perm && kcnt) {
-        IDP_HIP(hipMemsetAsync(kcnt, 0, 16 * sizeof(unsigned), s));
-        IDP_LAUNCH(k_kind_count, grid_for(n, 256), 256, 0, s, n, (const int4*)cs, kcnt);
-        IDP_LAUNCH(k_kind_scatter, grid_for(n, 256), 256, 0, s, n, (const int4*)cs, kcnt, perm);
+        const int nb = grid_for(n, KIND_BLOCK);
+        unsigned* bcnt = kcnt + 16;
+        unsigned* boff = bcnt + 8 * (size_t)nb;
+        IDP_LAUNCH(k_kind_count, nb, KIND_BLOCK, 0, s, n, (const int4*)cs, bcnt);
+        IDP_LAUNCH(k_kind_scan, 1, 256, 0, s, nb, bcnt, boff, kcnt);
+        IDP_LAUNCH(k_kind_scatter, nb, KIND_BLOCK, 0, s, n, (const int4*)cs, boff, kcnt, perm);
     } else {
         perm = nullptr;
     }
@@ -1274,6 +1529,12 @@ void launch_barrier_hess(hipStream_t s, long n, const int* cs, const double* X, 
                    mode, out, vid, counts);
         IDP_LAUNCH(k_barrier_hess_lane<3>, gl, 256, 0, s, n, cnt, perm, (const int4*)cs, form, form_ok, project_spd,
                    mode, out, vid, counts);
+    }
+    if (cnt && team <= 3) {  // order 9 on lane triples
+        const int gt = (int)std::min<long>(grid_for(n, 4 * TRI_PER_WAVE), 4096);
+        IDP_LAUNCH(k_barrier_hess_tri, gt, 256, 0, s, n, cnt, perm, (const int4*)cs, form, form_ok, project_spd,
+                   mode, out, vid, counts);
+        return;
     }
     auto team_grid = [&](int lanes) { return (int)std::min<long>(grid_for(n, HB / lanes), 16384); };
     switch (team) {

@@ -56,7 +56,7 @@ struct idp_ctx {
     DBuf<unsigned> hkcnt;     // ... and the per-kind counts / cursors
     DBuf<double> hform;       // ... packed K and H_y per constraint between its two stages
     DBuf<unsigned char> hform_ok;
-    int hess_team = 16;       // lanes per constraint in the projection stage
+    int hess_team = 3;        // lanes per order-9 constraint in the projection stage
     SegDev cseg;             // constraint blocks per instance (energy)
     DBuf<double> bpar;       // per instance: dh[B], kappa[B], th2[B]
     DBuf<double> esum;       // per-instance partial-sum results
@@ -408,7 +408,7 @@ void hessian(idp_ctx* c, const double* X, double h, bool with_barrier, const uns
         launch_barrier_hess(s, ncs, (const int*)c->cw.cs.p, X, m.X0.p, bp, 1, 1, b, nullptr, fl,
                             c->hform.ensure(90 * (size_t)ncs, s), c->hform_ok.ensure(ncs, s),
                             c->prof.full() ? c->bcounts.p : nullptr, c->hperm.ensure(ncs, s),
-                            c->hkcnt.ensure(16, s), c->hess_team);
+                            c->hkcnt.ensure(barrier_kind_scratch(ncs), s), c->hess_team);
         c->prof.end(IDP_K_BARRIER_HESS, tok);
         Hb = b;
     }
@@ -1346,7 +1346,7 @@ int idp_set_launch_config(idp_ctx* c, int spmv_shape, int upd_waves)
 
 int idp_set_barrier_team(idp_ctx* c, int lanes)
 {
-    if (!c || !(lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64)) return IDP_E_INVALID;
+    if (!c || !(lanes == 3 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64)) return IDP_E_INVALID;
     c->hess_team = lanes;
     return IDP_OK;
 }
@@ -2222,7 +2222,7 @@ int idp_barrier_hessian_local(idp_ctx* c, const double* X, long n, const int* cs
             launch_barrier_hess(s, n, (const int*)c->cw.cs.p, xd, c->m.X0.p, bparams(c), project, 0, b, dvid, fl,
                                 c->hform.ensure(90 * (size_t)n, s), c->hform_ok.ensure(n, s),
                                 c->prof.full() ? c->bcounts.p : nullptr, c->hperm.ensure(n, s),
-                                c->hkcnt.ensure(16, s), c->hess_team);
+                                c->hkcnt.ensure(barrier_kind_scratch(n), s), c->hess_team);
             c->prof.end(IDP_K_BARRIER_HESS, tok);
         }
         int hf = 0;

@@ -176,10 +176,12 @@ void launch_min_dist2(hipStream_t s, long n, const int* cs, const double* X, con
                       unsigned long long* dmin);
 void launch_barrier_grad(hipStream_t s, long n, const int* cs, const double* X, const double* X0,
                          const BarrierParams& bp, double* gout, int* err);
-// perm (n ints) and kcnt (16 unsigned), both device scratch: the launch first groups the
-// constraints by kind (both null: one constraint per lane in index order)
+// perm (n ints) and kcnt (barrier_kind_scratch(n) unsigned), both device scratch: the
+// launch first groups the constraints by kind (both null: one constraint per lane in
+// index order)
 // form (90 n doubles) and form_ok (n bytes): device scratch of the two stages; team: lanes per
 // constraint of the projection stage (8, 16, 32, 64; results do not depend on it)
+inline size_t barrier_kind_scratch(long n) { return 16 + 16 * (size_t)((n + 255) / 256); }
 void launch_barrier_hess(hipStream_t s, long n, const int* cs, const double* X, const double* X0,
                          const BarrierParams& bp, int project_spd, int mode, double* out, int* vid, int* err,
                          double* form, unsigned char* form_ok, int* counts = nullptr, int* perm = nullptr,

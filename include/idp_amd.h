@@ -140,9 +140,11 @@ int idp_profile_read(idp_ctx* ctx, idp_profile* out);
  * block row's sum, so one G serves every context of a process. */
 int idp_set_launch_config(idp_ctx* ctx, int spmv_shape, int upd_waves);
 int idp_get_launch_config(idp_ctx* ctx, int* spmv_shape, int* upd_waves);
-/* Lanes per constraint (8, 16, 32 or 64; default 16) in the projection stage
- * of the barrier Hessian (no counterpart in the reference; set by the
- * autotuner).  No sum depends on it, so results do not either. */
+/* Lanes per order-9 constraint (point-triangle, edge-edge, mollified) in the
+ * projection stage of the barrier Hessian: 3 (a lane triple holding K in
+ * registers, one eigenvector component each) or 8, 16, 32, 64 (a team running
+ * a parallel Jacobi in LDS); default 3.  No counterpart in the reference;
+ * set by the autotuner.  Results agree to rounding across the choices. */
 int idp_set_barrier_team(idp_ctx* ctx, int lanes);
 int idp_get_barrier_team(idp_ctx* ctx, int* lanes);
 

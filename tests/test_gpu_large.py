@@ -6,6 +6,9 @@
 - the constraint set at 90% of that step with a contact radius of 10 dhat
   (about 1M constraints), bit for bit after canonical sort;
 - barrier energy (1e-10 relative) and gradient (1e-10 in norm) on that set;
+- the projected barrier Hessian of every 5th constraint of that set (every
+  kind, ~200k constraints) within 1e-9 of each row's largest entry, with the
+  order-9 kinds on lane triples and on 16-lane teams;
 - the Newton direction there: the gradient against the oracle's elastic +
   inertia + barrier gradient (1e-10), and the PCG solution's residual
   recomputed on the host from the device's own BSR matrix.
@@ -83,6 +86,27 @@ def test_crumple_barrier_energy_and_gradient(crumple):
     say(f"[crumple] barrier E {Eg!r} (oracle {Eo!r}), |dg|/|g| {np.linalg.norm(gg - go) / np.linalg.norm(go):.2e}")
     assert abs(Eg - Eo) <= 1e-10 * abs(Eo)
     assert np.linalg.norm(gg - go) <= 1e-10 * np.linalg.norm(go)
+
+
+def test_crumple_barrier_hessian_sample(crumple):
+    c = crumple
+    cs = c.m.constraint_set(c.X1, c.r2)[::5].copy()
+    k = c.kappa[0]
+    Ho, vo = c.m.barrier_hessian_local(c.X1, cs, c.r2, k, project=True)
+    scale = np.abs(Ho).max(1)
+    team0 = c.ctx.barrier_team()
+    try:
+        for team in (3, 16):
+            c.ctx.set_barrier_team(team)
+            say(f"[crumple] projected barrier Hessian of {cs.shape[0]} constraints, team {team}")
+            Hg, vg = c.ctx.barrier_hessian_local(c.X1, cs, c.r2, k, project=True)
+            assert np.array_equal(vg, vo)
+            err = (np.abs(Hg - Ho).max(1) / np.where(scale > 0, scale, 1.0)).max()
+            say(f"[crumple] team {team}: max row error / row max {err:.2e}")
+            assert err <= 1e-9
+            del Hg
+    finally:
+        c.ctx.set_barrier_team(team0)
 
 
 def test_crumple_newton_direction(crumple):

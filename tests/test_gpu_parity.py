@@ -75,12 +75,20 @@ def test_barrier_energy_gradient_hessian(case):
     gg = case.ctx.barrier_gradient(X, cs, case.dHat2, k)
     go = case.m.barrier_gradient(X, cs, case.dHat2, k)
     assert np.linalg.norm(gg - go) <= 1e-10 * np.linalg.norm(go)
-    for project, tol in ((False, 1e-10), (True, 1e-9)):
-        Hg, vg = case.ctx.barrier_hessian_local(X, cs, case.dHat2, k, project=project)
-        Ho, vo = case.m.barrier_hessian_local(X, cs, case.dHat2, k, project=project)
-        assert np.array_equal(vg, vo)
-        scale = np.abs(Ho).max(1)
-        assert np.all(np.abs(Hg - Ho).max(1) <= tol * scale)
+    team0 = case.ctx.barrier_team()
+    try:
+        # every projection-stage layout of the order-9 kinds: lane triples
+        # (K in registers) and LDS teams of 8, 16 and 64 lanes
+        for team in (3, 8, 16, 64):
+            case.ctx.set_barrier_team(team)
+            for project, tol in ((False, 1e-10), (True, 1e-9)):
+                Hg, vg = case.ctx.barrier_hessian_local(X, cs, case.dHat2, k, project=project)
+                Ho, vo = case.m.barrier_hessian_local(X, cs, case.dHat2, k, project=project)
+                assert np.array_equal(vg, vo)
+                scale = np.abs(Ho).max(1)
+                assert np.all(np.abs(Hg - Ho).max(1) <= tol * scale), (team, project)
+    finally:
+        case.ctx.set_barrier_team(team0)
 
 
 def test_min_dist2(case):
