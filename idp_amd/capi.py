@@ -47,7 +47,8 @@ EXPORTED = [
     "idp_serve_configure", "idp_serve_start", "idp_serve_iterate", "idp_serve_result", "idp_set_launch_config",
     "idp_get_launch_config", "idp_set_accd_max_steps", "idp_barrier_energy_noalloc", "idp_get_surface", "idp_device_memory", "idp_set_barrier_team",
     "idp_get_barrier_team", "idp_load_velocity", "idp_shell_init", "idp_shell_surface", "idp_set_mesh_shell",
-    "idp_normal_flow_force", "idp_set_normal_flow", "idp_newton_direction_flow",
+    "idp_normal_flow_force", "idp_set_normal_flow", "idp_newton_direction_flow", "idp_shell_hinges",
+    "idp_set_shell_elasticity",
 ]
 
 # PCG preconditioner for every solve of this process: 0 cluster block Jacobi
@@ -224,7 +225,9 @@ def lib():
         L.idp_shell_init.argtypes = [C.c_int, dp, C.c_int, ip, C.c_double, C.c_double, C.c_double, C.c_double, ip,
                                      ip, dp, dp, dp, dp]
         L.idp_shell_surface.argtypes = [C.c_int, dp, C.c_int, ip, C.c_int, ip, ip, ip, ip, ip, ip, ip]
-        L.idp_set_mesh_shell.argtypes = [vp, C.c_int, dp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int, ip, dp]
+        L.idp_set_mesh_shell.argtypes = [vp, C.c_int, dp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int, ip, dp, C.c_int, ip]
+        L.idp_set_shell_elasticity.argtypes = [vp, dp, dp, dp, dp, C.c_double]
+        L.idp_shell_hinges.argtypes = [C.c_int, dp, C.c_int, ip, ip, ip, dp, dp]
         L.idp_normal_flow_force.argtypes = [vp, dp, C.c_double, dp]
         L.idp_set_normal_flow.argtypes = [vp, C.c_double]
         L.idp_newton_direction_flow.argtypes = L.idp_newton_direction.argtypes
@@ -330,6 +333,23 @@ def shell_surface(X, tris, tets=None):
     return bn[: nb.value].copy(), be[: 2 * ne.value].reshape(-1, 2).copy(), bt[: 3 * nf.value].reshape(-1, 3).copy()
 
 
+def shell_hinges(X, tris):
+    """(hinges (nH, 4), rest (nH, 3), tri_IB (nE, 3)) of a shell (host)."""
+    X, Xp = _d(X)
+    tris, tp = _i(tris)
+    nV, nE = X.size // 3, tris.size // 3
+    cap = (3 * nE) // 2 + 1
+    hg, hp = _out_i(4 * cap)
+    hr, hrp = _out_d(3 * cap)
+    ib, ibp = _out_d(3 * nE)
+    n = C.c_int()
+    rc = lib().idp_shell_hinges(nV, Xp, nE, tp, C.byref(n), hp, hrp, ibp)
+    if rc != IDP_OK:
+        raise IdpError(rc)
+    k = n.value
+    return hg[: 4 * k].reshape(-1, 4).copy(), hr[: 3 * k].reshape(-1, 3).copy(), ib.reshape(-1, 3)
+
+
 def initialize_oipc(dhat, mass, stiff_mult=1.0):
     mass, mp = _d(mass)
     k, kp = _out_d(3)
@@ -428,12 +448,14 @@ class Context:
         self.nV = X.size // 3
         self._check(lib().idp_set_mesh(self._h, self.nV, Xp, t.size // 4, tp, ibp, vp, lp, mp, msp))
 
-    def set_mesh_shell(self, X_rest, mass, tris, tri_vol, tets=None, IB=None, vol=None, lam=None, mu=None):
-        """A mesh of shell triangles (and optionally tets): idp_set_mesh_shell."""
+    def set_mesh_shell(self, X_rest, mass, tris, tri_vol, tets=None, IB=None, vol=None, lam=None, mu=None,
+                       hinges=None):
+        """A mesh of shell triangles (and optionally tets and hinges): idp_set_mesh_shell."""
         X, Xp = _d(X_rest)
         m, mp = _d(mass)
         tr, trp = _i(tris)
         tv, tvp = _d(tri_vol)
+        hg, hgp = _i(np.zeros((0, 4)) if hinges is None else hinges)
         nT = 0 if tets is None else np.asarray(tets).size // 4
         args = [None] * 5
         if nT:  # the arrays stay referenced by these locals until the call returns
@@ -443,9 +465,17 @@ class Context:
             la, lap = _d(lam)
             mu_, mup = _d(mu)
             args = [tep, ibp, vop, lap, mup]
-        self._check(lib().idp_set_mesh_shell(self._h, X.size // 3, Xp, mp, nT, *args, tr.size // 3, trp, tvp))
+        self._check(lib().idp_set_mesh_shell(self._h, X.size // 3, Xp, mp, nT, *args, tr.size // 3, trp, tvp,
+                                             hg.size // 4, hgp))
         self.nV = X.size // 3
         self.nT = nT
+
+    def set_shell_elasticity(self, tri_IB, tri_lambda, tri_mu, hinge_rest, bend_k):
+        ib, ibp = _d(tri_IB)
+        la, lap = _d(tri_lambda)
+        mu_, mup = _d(tri_mu)
+        hr, hrp = _d(np.zeros(0) if hinge_rest is None else hinge_rest)
+        self._check(lib().idp_set_shell_elasticity(self._h, ibp, lap, mup, hrp, float(bend_k)))
 
     def normal_flow_force(self, magnitude, X=None):
         """Update_Normal_Flow_Neumann on the device at X (None: current x)."""

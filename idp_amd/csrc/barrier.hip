@@ -21,6 +21,7 @@
 #include "common.hpp"
 #include "geom_dev.hpp"
 #include "kernels.hpp"
+#include "proj_dev.hpp"
 
 namespace idp {
 
@@ -86,6 +87,19 @@ void bbt(int r, int c, const double* b, double* o)
 }
 
 }  // namespace
+
+ReducedRoots reduced_roots()
+{
+    ReducedRoots r;
+    const double bpt[12] = {1, -1, 0, 0, 0, -1, 1, 0, 0, -1, 0, 1};
+    const double bpe[6] = {1, -1, 0, 0, -1, 1};
+    double M[9];
+    bbt(3, 4, bpt, M);
+    host_sqrtm(3, M, r.S3, r.Si3);
+    bbt(2, 3, bpe, M);
+    host_sqrtm(2, M, r.S2, r.Si2);
+    return r;
+}
 
 void barrier_init_constants()
 {
@@ -260,15 +274,6 @@ __device__ inline int2 group_range(const unsigned* __restrict__ cnt, int N)
     if (N == 9) return make_int2(0, n9);
     if (N == 6) return make_int2(n9, n9 + n6);
     return make_int2(n9 + n6, n9 + n6 + n3);
-}
-
-// Symmetric matrices are held as packed upper triangles: entry (i, j) of an
-// N x N matrix at sx<N>(i, j).  All loops over entries are fully unrolled, so
-// every index is a compile-time constant and the arrays stay in registers.
-template <int N>
-__device__ constexpr int sx(int i, int j)
-{
-    return (i <= j) ? (i * (2 * N - i + 1)) / 2 + (j - i) : (j * (2 * N - j + 1)) / 2 + (i - j);
 }
 
 // entry (r, c) of [w]x  ([w]x z = w x z)
@@ -961,62 +966,6 @@ __global__ void __launch_bounds__(HB) k_barrier_hess(long n, const unsigned* __r
 // theta = (aqq - app) / (2 apq), app - t apq, aqq + t apq, apq = 0; sweeps run
 // in row order (p < q) until off(K)^2 <= 1e-34 sum K_ii^2 (at most 16).
 // ---------------------------------------------------------------------------
-// reduced-coordinate maps y = B x per vertex (rows a, columns vertex), the
-// same as the constant tables of barrier_init_constants
-__device__ constexpr double kBpt[12] = {1, -1, 0, 0, 0, -1, 1, 0, 0, -1, 0, 1};
-__device__ constexpr double kBee[12] = {-1, 0, 1, 0, -1, 1, 0, 0, 0, 0, -1, 1};
-__device__ constexpr double kBpe[6] = {1, -1, 0, 0, -1, 1};
-
-template <int N>
-__device__ inline void lane_jacobi(double (&K)[N * (N + 1) / 2], double (&V)[N * N])
-{
-#pragma clang fp contract(fast)
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-#pragma unroll
-        for (int j = 0; j < N; ++j) V[N * i + j] = (i == j) ? 1.0 : 0.0;
-    for (int sweep = 0; sweep < 16; ++sweep) {
-        double off = 0.0, dg = 0.0;
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            dg += K[sx<N>(i, i)] * K[sx<N>(i, i)];
-#pragma unroll
-            for (int j = i + 1; j < N; ++j) off += K[sx<N>(i, j)] * K[sx<N>(i, j)];
-        }
-        if (!(off > 1e-34 * dg)) break;
-#pragma unroll
-        for (int p = 0; p < N - 1; ++p)
-#pragma unroll
-            for (int q = p + 1; q < N; ++q) {
-                const double apq = K[sx<N>(p, q)];
-                const double dd = K[sx<N>(q, q)] - K[sx<N>(p, p)];
-                const double rt = sqrt(dd * dd + 4.0 * apq * apq);
-                // apq == 0: the identity (t = 0), also when dd == 0; no branch, so
-                // the whole sweep is one block and its values stay in registers
-                const double den = fabs(dd) + rt;
-                const double t = copysign(1.0, dd) * (2.0 * apq) / (den != 0.0 ? den : 1.0);
-                const double c = rsqrt(t * t + 1.0);
-                const double sn = t * c;
-#pragma unroll
-                for (int k = 0; k < N; ++k) {
-                    if (k == p || k == q) continue;
-                    const double kp = K[sx<N>(k, p)], kq = K[sx<N>(k, q)];
-                    K[sx<N>(k, p)] = c * kp - sn * kq;
-                    K[sx<N>(k, q)] = sn * kp + c * kq;
-                }
-                K[sx<N>(p, p)] -= t * apq;
-                K[sx<N>(q, q)] += t * apq;
-                K[sx<N>(p, q)] = 0.0;
-#pragma unroll
-                for (int k = 0; k < N; ++k) {
-                    const double vp = V[N * k + p], vq = V[N * k + q];
-                    V[N * k + p] = c * vp - sn * vq;
-                    V[N * k + q] = sn * vp + c * vq;
-                }
-            }
-    }
-}
-
 // projected H_y of one constraint (packed, order N = 3 M) from the form
 // stage's record Fi (packed K, then packed H_y); returns whether anything
 // was clamped
@@ -1194,117 +1143,6 @@ k_barrier_hess_lane(long n, const unsigned* __restrict__ kcnt, const int* __rest
 // 21 triples per wavefront (lane 63 idles).
 // ---------------------------------------------------------------------------
 constexpr int TRI_PER_WAVE = 21;
-
-__device__ inline void tri_jacobi(double (&K)[45], double (&V)[27], int r)
-{
-#pragma clang fp contract(fast)
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int k = 0; k < 9; ++k) V[9 * a + k] = (3 * a + r == k) ? 1.0 : 0.0;
-    for (int sweep = 0; sweep < 16; ++sweep) {
-        double off = 0.0, dg = 0.0;
-#pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            dg += K[sx<9>(i, i)] * K[sx<9>(i, i)];
-#pragma unroll
-            for (int j = i + 1; j < 9; ++j) off += K[sx<9>(i, j)] * K[sx<9>(i, j)];
-        }
-        if (!(off > 1e-34 * dg)) break;
-#pragma unroll
-        for (int p = 0; p < 8; ++p)
-#pragma unroll
-            for (int q = p + 1; q < 9; ++q) {
-                const double apq = K[sx<9>(p, q)];
-                const double dd = K[sx<9>(q, q)] - K[sx<9>(p, p)];
-                const double rt = sqrt(dd * dd + 4.0 * apq * apq);
-                const double den = fabs(dd) + rt;  // as in lane_jacobi: branch free
-                const double t = copysign(1.0, dd) * (2.0 * apq) / (den != 0.0 ? den : 1.0);
-                const double c = rsqrt(t * t + 1.0);
-                const double sn = t * c;
-#pragma unroll
-                for (int k = 0; k < 9; ++k) {
-                    if (k == p || k == q) continue;
-                    const double kp = K[sx<9>(k, p)], kq = K[sx<9>(k, q)];
-                    K[sx<9>(k, p)] = c * kp - sn * kq;
-                    K[sx<9>(k, q)] = sn * kp + c * kq;
-                }
-                K[sx<9>(p, p)] -= t * apq;
-                K[sx<9>(q, q)] += t * apq;
-                K[sx<9>(p, q)] = 0.0;
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    const double vp = V[9 * a + p], vq = V[9 * a + q];
-                    V[9 * a + p] = c * vp - sn * vq;
-                    V[9 * a + q] = sn * vp + c * vq;
-                }
-            }
-    }
-}
-
-// this lane's 48 entries (row component r) of the 4 x 4 vertex blocks
-template <bool PT>
-__device__ inline void tri_blocks(int r, int base, int mode, bool clamped, const double (&L)[9],
-                                  const double (&V)[27], const double* Si, const double* __restrict__ Hy,
-                                  double* __restrict__ o)
-{
-#pragma clang fp contract(fast)
-    constexpr const double* B = PT ? kBpt : kBee;
-    if (!clamped) {  // B^T H_y B from the packed H_y of the record
-#pragma unroll
-        for (int vi = 0; vi < 4; ++vi)
-#pragma unroll
-            for (int vj = 0; vj < 4; ++vj)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    double acc = 0.0;
-#pragma unroll
-                    for (int a = 0; a < 3; ++a)
-#pragma unroll
-                        for (int b = 0; b < 3; ++b) {
-                            const double sc = B[a * 4 + vi] * B[b * 4 + vj];
-                            if (sc != 0.0) acc += sc * Hy[sx<9>(3 * a + r, 3 * b + c)];
-                        }
-                    if (mode == 0) o[(3 * vi + r) * 12 + 3 * vj + c] = acc;
-                    else o[9 * (vi * 4 + vj) + 3 * r + c] = acc;
-                }
-        return;
-    }
-    double Z[4][9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        double u[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) u[a] = Si[3 * a] * V[k] + Si[3 * a + 1] * V[9 + k] + Si[3 * a + 2] * V[18 + k];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            double acc = 0.0;
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-                if (B[a * 4 + v] != 0.0) acc += B[a * 4 + v] * u[a];
-            Z[v][k] = acc;
-        }
-    }
-#pragma unroll
-    for (int sft = 0; sft < 3; ++sft) {
-        const int c = (r + sft) % 3;
-#pragma unroll
-        for (int vj = 0; vj < 4; ++vj) {
-            double zc[9];  // row vj of Z_c, from lane c of the triple
-#pragma unroll
-            for (int k = 0; k < 9; ++k) zc[k] = __shfl(Z[vj][k], base + c);
-#pragma unroll
-            for (int vi = 0; vi < 4; ++vi) {
-                double acc = 0.0;
-#pragma unroll
-                for (int k = 0; k < 9; ++k) acc += L[k] * (Z[vi][k] * zc[k]);
-                if (mode == 0) o[(3 * vi + r) * 12 + 3 * vj + c] = acc;
-                else o[9 * (vi * 4 + vj) + 3 * r + c] = acc;
-            }
-            __builtin_amdgcn_sched_barrier(0);
-        }
-    }
-}
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
 k_barrier_hess_tri(long n, const unsigned* __restrict__ kcnt, const int* __restrict__ perm,

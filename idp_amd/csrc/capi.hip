@@ -14,6 +14,8 @@
 #include <string>
 #include <unordered_map>
 #include <vector>
+#include <set>
+#include <map>
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -42,6 +44,8 @@ struct idp_ctx {
     DBuf<double> x, v, xt, xn, xprev, sol, rhs, g, tmpv, snap_x, snap_v, hx, hp;
     DBuf<double> gl, gb, Hl, Hb;  // per-tet / per-constraint local terms
     DBuf<double> part_e, part_i, part_b, part_f;
+    DBuf<double> sform;                 // hinge Hessian records (two stages)
+    DBuf<unsigned char> sform_ok;
     DBuf<int> flags;
     DBuf<unsigned char> one;  // a device byte holding 1 (warm-start mask of a one-mesh solve)
     DBuf<unsigned long long> dmin;
@@ -264,13 +268,14 @@ void update_cseg(idp_ctx* c)
     c->cseg.build(c->cw.h_cbeg, RB, c->s);
 }
 
-// Shell triangles carry the flow term only: the membrane and hinge-bending
-// energies of the non-flow variant are not built, so a non-flow evaluation of
-// a mesh with shell triangles is refused rather than run without them.
+// Shell triangles carry the flow term in the flow variant and the membrane +
+// hinge bending otherwise; a non-flow evaluation of a mesh with shell
+// triangles whose elasticity was never set (idp_set_shell_elasticity) is
+// refused rather than run without those terms.
 void shell_terms_check(const idp_ctx* c)
 {
-    if (c->m.nE > 0 && !c->no_elastic)
-        throw std::invalid_argument("shell triangles: only the flow variant's terms are built (flow = 1)");
+    if (c->m.nE > 0 && !c->no_elastic && !c->m.shell_el)
+        throw std::invalid_argument("shell triangles: the non-flow step needs idp_set_shell_elasticity");
 }
 
 // Compute_IncPotential per instance: elastic (w = h^2) + inertia + barrier
@@ -293,13 +298,21 @@ void energies(idp_ctx* c, const double* X, bool with_barrier, double h, std::vec
     launch_inertia_energy(s, I.vblk, X, c->xt.p, m.mass.p, pi);
     const bool bar = with_barrier && c->cw.ncs > 0;
     const bool flow = c->no_elastic && m.nE > 0;
-    double* es = c->esum.ensure(4 * (size_t)B, s);
+    const bool shell = !c->no_elastic && m.nE > 0;  // membrane + bending
+    double* es = c->esum.ensure(5 * (size_t)B, s);
     IDP_LAUNCH(k_seg_sums, B, RB, 0, s, I.tblk.pbeg.p, pe, es);
     IDP_LAUNCH(k_seg_sums, B, RB, 0, s, I.vblk.pbeg.p, pi, es + B);
     if (flow) {
         double* pf = c->part_f.ensure(I.sblk.nblk + 1, s);
         launch_flow_energy(s, m, X, h, pf);
         IDP_LAUNCH(k_seg_sums, B, RB, 0, s, I.sblk.pbeg.p, pf, es + 3 * B);
+    }
+    if (shell) {
+        double* pm = c->part_f.ensure(I.sblk.nblk + I.hblk.nblk + 2, s);
+        double* ph = pm + I.sblk.nblk + 1;
+        launch_shell_energy(s, m, X, h * h, pm, ph);
+        IDP_LAUNCH(k_seg_sums, B, RB, 0, s, I.sblk.pbeg.p, pm, es + 3 * B);
+        IDP_LAUNCH(k_seg_sums, B, RB, 0, s, I.hblk.pbeg.p, ph, es + 4 * B);
     }
     if (bar) {
         double* pb = c->part_b.ensure(c->cseg.nblk + 1, s);
@@ -309,17 +322,19 @@ void energies(idp_ctx* c, const double* X, bool with_barrier, double h, std::vec
     }
     c->prof.end(IDP_K_ENERGY, tok);
     std::vector<int> hf(2 * B);
-    std::vector<double> he(4 * B, 0.0);
+    std::vector<double> he(5 * B, 0.0);
     IDP_HIP(hipMemcpyAsync(hf.data(), fl, 2 * B * sizeof(int), hipMemcpyDeviceToHost, s));
     IDP_HIP(hipMemcpyAsync(he.data(), es, (bar ? 3 : 2) * B * sizeof(double), hipMemcpyDeviceToHost, s));
-    if (flow) IDP_HIP(hipMemcpyAsync(he.data() + 3 * B, es + 3 * B, B * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (flow || shell)
+        IDP_HIP(hipMemcpyAsync(he.data() + 3 * B, es + 3 * B, (shell ? 2 : 1) * B * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
     IDP_HIP(hipStreamSynchronize(s));
     out.resize(B);
     for (int i = 0; i < B; ++i) {
         EnergyOut& o = out[i];
         o.valid = hf[i] == 0;
         o.dist_ok = hf[B + i] == 0;
-        o.E = o.valid ? (he[i] + he[3 * B + i]) + he[B + i] + (bar ? he[2 * B + i] : 0.0) : 0.0;
+        o.E = o.valid ? ((he[i] + he[3 * B + i]) + he[4 * B + i]) + he[B + i] + (bar ? he[2 * B + i] : 0.0) : 0.0;
     }
 }
 
@@ -356,11 +371,17 @@ void gradient(idp_ctx* c, const double* X, double h, bool with_barrier, double* 
     auto s = c->s;
     auto& m = c->m;
     shell_terms_check(c);
-    double* gl = c->gl.ensure(12 * ((size_t)m.nT + m.nE), s);
+    double* gl = c->gl.ensure(12 * ((size_t)m.nT + m.nE + m.nH), s);
     const int tok = c->prof.begin(IDP_K_GRADIENT);
     if (c->no_elastic) IDP_HIP(hipMemsetAsync(gl, 0, 12 * (size_t)m.nT * sizeof(double), s));
     else launch_tet_grad(s, m, X, h * h, gl, act);
-    if (c->no_elastic) launch_flow_grad(s, m, X, h, gl);  // shell triangles: the flow term
+    if (c->no_elastic) {  // shell triangles: the flow term; hinges: nothing
+        launch_flow_grad(s, m, X, h, gl);
+        if (m.nH)
+            IDP_HIP(hipMemsetAsync(gl + 12 * ((size_t)m.nT + m.nE), 0, 12 * (size_t)m.nH * sizeof(double), s));
+    } else if (m.nE) {
+        launch_shell_grad(s, m, X, h * h, gl);  // membrane and hinge bending
+    }
     const double* gb = nullptr;
     long ncs = c->cw.ncs;
     if (with_barrier && ncs) {
@@ -385,14 +406,19 @@ void hessian(idp_ctx* c, const double* X, double h, bool with_barrier, const uns
     auto s = c->s;
     auto& m = c->m;
     shell_terms_check(c);
-    double* Hl = c->Hl.ensure(144 * ((size_t)m.nT + m.nE), s);
+    double* Hl = c->Hl.ensure(144 * ((size_t)m.nT + m.nE + m.nH), s);
     int tok = c->prof.begin(IDP_K_TET_HESS);
     if (c->no_elastic) {
         IDP_HIP(hipMemsetAsync(Hl, 0, 144 * (size_t)m.nT * sizeof(double), s));
-        launch_flow_hess(s, m, h, Hl);  // shell triangles: the flow term
+        launch_flow_hess(s, m, h, Hl);  // shell triangles: the flow term; hinges: nothing
+        if (m.nH)
+            IDP_HIP(hipMemsetAsync(Hl + 144 * ((size_t)m.nT + m.nE), 0, 144 * (size_t)m.nH * sizeof(double), s));
     } else {
         IDP_TIMER_FLAG("Compute_Volumetric_Elasticity_Hessian");
         launch_tet_hess(s, m, X, h * h, 1, Hl, act);
+        if (m.nE)
+            launch_shell_hess(s, m, X, h * h, 1, c->sform.ensure(90 * (size_t)std::max(m.nH, 1), s),
+                              c->sform_ok.ensure(std::max(m.nH, 1), s), Hl);
     }
     c->prof.end(IDP_K_TET_HESS, tok, 6500.0 * (nT_act < 0 ? m.nT : nT_act));
     const double* Hb = nullptr;
@@ -1053,12 +1079,14 @@ namespace {
 int setup_mesh(idp_ctx* c, int nV, const double* X_rest, int nT, const int* tets, const double* IB,
                const double* vol, const double* lambda, const double* mu, const double* mass,
                const std::vector<long>& vbeg, const std::vector<long>& tbeg, int nE = 0, const int* tris = nullptr,
-               const double* tvol = nullptr)
+               const double* tvol = nullptr, int nH = 0, const int* hinges = nullptr)
 {
         if (nV <= 0 || nT < 0 || nE < 0 || nT + (long)nE <= 0 || !X_rest || !mass) return IDP_E_INVALID;
         if (nT > 0 && (!tets || !IB || !vol || !lambda || !mu)) return IDP_E_INVALID;
         if (nE > 0 && (!tris || !tvol || vbeg.size() != 2)) return IDP_E_INVALID;  // shells: one instance
-        if ((long)nT + nE > INT32_MAX / 16) return IDP_E_INVALID;
+        if ((long)nT + nE + nH > INT32_MAX / 16 || nH < 0 || (nH > 0 && (!hinges || nE <= 0))) return IDP_E_INVALID;
+        for (long i = 0; i < 4L * nH; ++i)
+            if (hinges[i] < 0 || hinges[i] >= nV) return IDP_E_INVALID;
         for (long i = 0; i < 4L * nT; ++i)
             if (tets[i] < 0 || tets[i] >= nV) return IDP_E_INVALID;
         for (long i = 0; i < 3L * nE; ++i)
@@ -1078,6 +1106,9 @@ int setup_mesh(idp_ctx* c, int nV, const double* X_rest, int nT, const int* tets
         m.nE = nE;
         m.tris.upload(tris, 3 * (size_t)nE, s);
         m.tvol.upload(tvol, nE, s);
+        m.nH = nH;
+        m.hinge.upload(hinges, 4 * (size_t)nH, s);
+        m.shell_el = false;  // membrane / bending data come with idp_set_shell_elasticity
         // vertex -> triangles, ascending (the order of the normal-flow sums)
         std::vector<int> vtp(nV + 1, 0), vtl(3 * (size_t)nE);
         for (long i = 0; i < 3L * nE; ++i) ++vtp[tris[i] + 1];
@@ -1116,8 +1147,10 @@ int setup_mesh(idp_ctx* c, int nV, const double* X_rest, int nT, const int* tets
             for (int a = 0; a < 4; ++a) rcount[tets[4 * t + a] + 1] += 4;
         for (int e = 0; e < nE; ++e)
             for (int a = 0; a < 3; ++a) rcount[tris[3 * e + a] + 1] += 3;
+        for (int k = 0; k < nH; ++k)  // hinges: element nT + nE + k, four vertices
+            for (int a = 0; a < 4; ++a) rcount[hinges[4 * k + a] + 1] += 4;
         for (int v = 0; v < nV; ++v) rcount[v + 1] += rcount[v];
-        std::vector<std::pair<int, int>> rec(16 * (size_t)nT + 9 * (size_t)nE);  // (col, src)
+        std::vector<std::pair<int, int>> rec(16 * (size_t)nT + 9 * (size_t)nE + 16 * (size_t)nH);  // (col, src)
         std::vector<int> fill(rcount.begin(), rcount.end() - 1);
         for (int t = 0; t < nT; ++t)
             for (int a = 0; a < 4; ++a)
@@ -1127,6 +1160,10 @@ int setup_mesh(idp_ctx* c, int nV, const double* X_rest, int nT, const int* tets
             for (int a = 0; a < 3; ++a)
                 for (int b = 0; b < 3; ++b)
                     rec[fill[tris[3 * e + a]]++] = {tris[3 * e + b], (nT + e) * 16 + a * 4 + b};
+        for (int k = 0; k < nH; ++k)
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b)
+                    rec[fill[hinges[4 * k + a]]++] = {hinges[4 * k + b], (nT + nE + k) * 16 + a * 4 + b};
         std::vector<int> rowptr(nV + 1, 0), col, sptr, src, diag(nV, -1), rowof;
         col.reserve(27 * (size_t)nV);
         src.reserve(16 * (size_t)nT);
@@ -1175,9 +1212,11 @@ int setup_mesh(idp_ctx* c, int nV, const double* X_rest, int nT, const int* tets
             return IDP_E_INVALID;
         build_inst_layout(m, vbeg, tbeg, nbeg, ebeg, fbeg, s);
         {
-            std::vector<long> sbeg(B + 1, 0);
+            std::vector<long> sbeg(B + 1, 0), hbeg(B + 1, 0);
             sbeg[B] = nE;  // nE > 0 only with one instance
+            hbeg[B] = nH;
             m.inst.sblk.build(sbeg, RB, s);
+            m.inst.hblk.build(hbeg, RB, s);
         }
         c->cw.h_cbeg.clear();
         // state
@@ -1547,6 +1586,110 @@ int idp_shell_surface(int nV, const double* X, int nT, const int* tets, int nE, 
     return IDP_OK;
 }
 
+namespace {
+double host_dihedral(const double* X, int v0, int v1, int v2, int v3)
+{
+    auto sub = [&](int a, int b, double* o) {
+        for (int d = 0; d < 3; ++d) o[d] = X[3 * a + d] - X[3 * b + d];
+    };
+    auto crs = [](const double* a, const double* b, double* o) {
+        o[0] = a[1] * b[2] - a[2] * b[1];
+        o[1] = a[2] * b[0] - a[0] * b[2];
+        o[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    auto dt = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+    double a[3], b[3], n1[3], n2[3], c[3], e[3];
+    sub(v1, v0, a);
+    sub(v2, v0, b);
+    crs(a, b, n1);
+    sub(v2, v3, a);
+    sub(v1, v3, b);
+    crs(a, b, n2);
+    const double cc = dt(n1, n2) / std::sqrt(dt(n1, n1) * dt(n2, n2));
+    double da = std::acos(std::max(-1.0, std::min(1.0, cc)));
+    crs(n2, n1, c);
+    sub(v1, v2, e);
+    if (dt(c, e) < 0) da = -da;
+    return da;
+}
+}  // namespace
+
+int idp_shell_hinges(int nV, const double* X, int nE, const int* tris, int* nH, int* hinges, double* hinge_rest,
+                     double* tri_IB)
+{
+    if (nV <= 0 || nE <= 0 || !X || !tris || !nH || !hinges || !hinge_rest) return IDP_E_INVALID;
+    for (long i = 0; i < 3L * nE; ++i)
+        if (tris[i] < 0 || tris[i] >= nV) return IDP_E_INVALID;
+    // edges of the triangles, first-seen orientation, sorted (MESHIO.h:728-766)
+    std::set<std::array<int, 2>> es;
+    std::map<std::pair<int, int>, int> e2t;
+    for (int e = 0; e < nE; ++e) {
+        const int* t = tris + 3 * e;
+        const int pr[3][2] = {{t[0], t[1]}, {t[1], t[2]}, {t[2], t[0]}};
+        for (auto& q : pr)
+            if (!es.count({q[1], q[0]})) es.insert({q[0], q[1]});
+        e2t[{t[0], t[1]}] = e;
+        e2t[{t[1], t[2]}] = e;
+        e2t[{t[2], t[0]}] = e;
+    }
+    // hinge stencils and rest data (DISCRETE_SHELL.h:169-215)
+    int n = 0;
+    for (auto& ed : es) {
+        const int a = ed[0], b = ed[1];
+        auto f = e2t.find({a, b});
+        auto g = e2t.find({b, a});
+        if (f == e2t.end() || g == e2t.end()) continue;
+        const int* t = tris + 3 * f->second;
+        const int* o = tris + 3 * g->second;
+        int v = -1, w = -1;
+        for (int j = 0; j < 3; ++j) {
+            if (t[j] == b) v = t[(j + 1) % 3];
+            if (o[j] == a && w < 0) w = o[(j + 1) % 3];
+        }
+        int* s = hinges + 4 * n;
+        s[0] = v;
+        s[1] = a;
+        s[2] = b;
+        s[3] = w;
+        double* r = hinge_rest + 3 * n;
+        r[0] = host_dihedral(X, v, a, b, w);
+        double d[3], p[3], q[3], n1[3], n2[3];
+        for (int k = 0; k < 3; ++k) d[k] = X[3 * a + k] - X[3 * b + k];
+        r[1] = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        for (int k = 0; k < 3; ++k) {
+            p[k] = X[3 * a + k] - X[3 * v + k];
+            q[k] = X[3 * b + k] - X[3 * v + k];
+        }
+        n1[0] = p[1] * q[2] - p[2] * q[1];
+        n1[1] = p[2] * q[0] - p[0] * q[2];
+        n1[2] = p[0] * q[1] - p[1] * q[0];
+        for (int k = 0; k < 3; ++k) {
+            p[k] = X[3 * b + k] - X[3 * w + k];
+            q[k] = X[3 * a + k] - X[3 * w + k];
+        }
+        n2[0] = p[1] * q[2] - p[2] * q[1];
+        n2[1] = p[2] * q[0] - p[0] * q[2];
+        n2[2] = p[0] * q[1] - p[1] * q[0];
+        r[2] = (std::sqrt(n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2]) +
+                std::sqrt(n2[0] * n2[0] + n2[1] * n2[1] + n2[2] * n2[2])) / (r[1] * 6);
+        ++n;
+    }
+    *nH = n;
+    if (tri_IB)  // rest first fundamental form (DISCRETE_SHELL.h:146-158)
+        for (int e = 0; e < nE; ++e) {
+            const int* t = tris + 3 * e;
+            double u[3], w[3];
+            for (int k = 0; k < 3; ++k) {
+                u[k] = X[3 * t[1] + k] - X[3 * t[0] + k];
+                w[k] = X[3 * t[2] + k] - X[3 * t[0] + k];
+            }
+            tri_IB[3 * e] = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+            tri_IB[3 * e + 1] = u[0] * w[0] + u[1] * w[1] + u[2] * w[2];
+            tri_IB[3 * e + 2] = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+        }
+    return IDP_OK;
+}
+
 int idp_initialize_oipc(double dhat, int nV, const double* mass, double stiff_mult, double* kappa3, double* dHat2)
 {
     if (nV <= 0 || !mass || !kappa3) return IDP_E_INVALID;
@@ -1573,12 +1716,31 @@ int idp_set_mesh(idp_ctx* c, int nV, const double* X_rest, int nT, const int* te
 
 int idp_set_mesh_shell(idp_ctx* c, int nV, const double* X_rest, const double* mass, int nT, const int* tets,
                        const double* IB, const double* vol, const double* lambda, const double* mu, int nE,
-                       const int* tris, const double* tri_vol)
+                       const int* tris, const double* tri_vol, int nH, const int* hinges)
 {
     return guarded(c, [&]() -> int {
         if (nE <= 0) return IDP_E_INVALID;
         return setup_mesh(c, nV, X_rest, nT, tets, IB, vol, lambda, mu, mass, {0, (long)nV}, {0, (long)nT}, nE,
-                          tris, tri_vol);
+                          tris, tri_vol, nH, hinges);
+    });
+}
+
+int idp_set_shell_elasticity(idp_ctx* c, const double* tri_IB, const double* tri_lambda, const double* tri_mu,
+                             const double* hinge_rest, double bend_k)
+{
+    return guarded(c, [&]() -> int {
+        MeshDev& m = c->m;
+        if (!c->have_mesh || m.nE <= 0 || !tri_IB || !tri_lambda || !tri_mu || (m.nH > 0 && !hinge_rest) ||
+            !std::isfinite(bend_k))
+            return IDP_E_INVALID;
+        m.tIB.upload(tri_IB, 3 * (size_t)m.nE, c->s);
+        m.tlam.upload(tri_lambda, m.nE, c->s);
+        m.tmu.upload(tri_mu, m.nE, c->s);
+        m.hrest.upload(hinge_rest, 3 * (size_t)m.nH, c->s);
+        m.bend_k = bend_k;
+        m.shell_el = true;
+        IDP_HIP(hipStreamSynchronize(c->s));
+        return IDP_OK;
     });
 }
 

@@ -65,6 +65,7 @@ struct InstDev {
     SegDev nblk;     // boundary nodes, blocks of RB (swept bounding box)
     SegDev tblk;     // tets, blocks of RB (elastic energy)
     SegDev sblk;     // shell triangles, blocks of RB (flow energy)
+    SegDev hblk;     // shell hinges, blocks of RB (bending energy)
     SegDev pcg_rows[4];  // vertices, blocks of 256 / G rows for the SpMV with G = 4, 8, 16, 32 lanes per row
     SegDev pcg_upd;  // vertices, 64-row blocks (PCG update)
     SegDev dblk;     // 3 x vertices, blocks of RB (per-instance dot products)
@@ -87,6 +88,14 @@ struct MeshDev {
     DBuf<int> tris;       // 3 per triangle
     DBuf<double> tvol;
     DBuf<int> vt_ptr, vt; // vertex -> its triangles, ascending (nV + 1, 3 nE)
+    // non-flow shell terms: rest first fundamental form (3 per triangle),
+    // Lame parameters, hinges (elements nT + nE + k: 4 vertex ids, rest
+    // (theta, |e|, h)) and the bending stiffness
+    bool shell_el = false;
+    DBuf<double> tIB, tlam, tmu, hrest;
+    int nH = 0;
+    DBuf<int> hinge;
+    double bend_k = 0.0;
     // rows/columns projected out of the gradient and the Hessian: the DBC
     // nodes, or only the ones that do not move while a moving-DBC penalty is
     // on (null: dbc)
@@ -170,6 +179,13 @@ struct BarrierParams {
     __device__ bool skip(int inst) const { return act && !act[inst]; }
 };
 void barrier_init_constants();
+// (B B^T)^(1/2) and its inverse for the order-9 map of the point-triangle
+// coordinates (y = (x0 - x1, x2 - x1, x3 - x1)) and the order-6 map
+// (y = (x1 - x0, x2 - x0) up to vertex order), row-major
+struct ReducedRoots {
+    double S3[9], Si3[9], S2[4], Si2[4];
+};
+ReducedRoots reduced_roots();
 void launch_barrier_energy(hipStream_t s, int nblk, const int* lo, const int* hi, const int* cs, const double* X,
                            const double* X0, const BarrierParams& bp, double* part, int* err);
 void launch_min_dist2(hipStream_t s, long n, const int* cs, const double* X, const int* vinst, double* dist2,
@@ -222,6 +238,12 @@ void launch_flow_energy(hipStream_t s, const MeshDev& m, const double* X, double
 void launch_flow_grad(hipStream_t s, const MeshDev& m, const double* X, double h, double* gl);
 void launch_flow_hess(hipStream_t s, const MeshDev& m, double h, double* Hl);
 void launch_normal_flow(hipStream_t s, const MeshDev& m, const double* X, double magnitude, double* b);
+// membrane and bending (w = h^2): energy partials over sblk / hblk, gradient and
+// projected Hessian into the element slots (F, ok: 90 nH doubles, nH bytes of scratch)
+void launch_shell_energy(hipStream_t s, const MeshDev& m, const double* X, double w, double* pm, double* ph);
+void launch_shell_grad(hipStream_t s, const MeshDev& m, const double* X, double w, double* gl);
+void launch_shell_hess(hipStream_t s, const MeshDev& m, const double* X, double w, int project, double* F,
+                       unsigned char* ok, double* Hl);
 void launch_inertia_energy(hipStream_t s, const SegDev& seg, const double* X, const double* Xt, const double* mass,
                            double* part);
 

@@ -82,6 +82,45 @@ class ShellNormalFlow:
         self.ctx.close()
 
 
+class ShellHinge:
+    """Shell driver of the non-flow step (Advance_One_Step_IE_Hinge): the set-up
+    of Initialize_Shell_Hinge_EIPC (lumped mass, vol, Lame parameters, rest
+    first fundamental forms, hinge stencils and the hinge stiffness
+    E t^3 / (24 (1 - nu^2)) times bend_mult; FEMDiscreteShellBase.py:190-203),
+    gravity as the body force m g, Dirichlet nodes that stay."""
+
+    def __init__(self, X, tris, density, E, nu, thickness, h, newton_tol, dhat, gravity=(0.0, -9.81, 0.0),
+                 bend_mult=1.0, dbc=None, device=0, pcg_rtol=1e-8, kappa_mult=1.0):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        self.tris, self.mass, self.tvol, self.lam, self.mu = capi.shell_init(X, tris, density, E, nu, thickness)
+        self.hinges, self.hrest, self.tIB = capi.shell_hinges(X, self.tris)
+        self.bend_k = bend_mult * E * thickness ** 3 / (24 * (1 - nu * nu))
+        self.ctx = capi.Context(device)
+        self.ctx.set_mesh_shell(X, self.mass, self.tris, self.tvol, hinges=self.hinges)
+        self.ctx.set_shell_elasticity(self.tIB, self.lam, self.mu, self.hrest, self.bend_k)
+        if dbc is not None and len(dbc):
+            self.ctx.set_dbc(np.asarray(dbc, dtype=np.int32))
+        self.body_force = self.mass[:, None] * np.asarray(gravity, dtype=np.float64)[None, :]
+        self.dHat2, self.kappa = capi.initialize_oipc(dhat, self.mass, kappa_mult)
+        self.h = h
+        self.tol = newton_tol
+        self.pcg_rtol = pcg_rtol
+        self.ctx.set_state(X, np.zeros_like(X))
+        self.pn_iters = 0
+
+    def advance_one_time_step(self, max_newton=0, verbose=0):
+        it, st = self.ctx.advance(self.h, self.tol, self.dHat2, self.kappa, 0.0, True, self.pcg_rtol,
+                                  max_newton=max_newton, verbose=verbose, body_force=self.body_force)
+        self.pn_iters += it
+        return it, st
+
+    def state(self):
+        return self.ctx.get_state()
+
+    def close(self):
+        self.ctx.close()
+
+
 class BatchAnimFix:
     """Several independent animation-fix scenes advanced together in one
     context: every stage runs over all instances at once, while each instance

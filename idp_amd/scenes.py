@@ -315,3 +315,30 @@ def sphere_lattice(counts=(4, 4, 4), level=3, radius=0.5, gap=0.01, magnitude=0.
                 off += X.shape[0]
     nm = name or f"spheres{counts[0]}x{counts[1]}x{counts[2]}_l{level}"
     return ShellScene(nm, np.vstack(Xs), np.vstack(Ts).astype(np.int32), magnitude=magnitude)
+
+
+def shell_sheet(n=8, size=1.0, seed=0, amp=0.0):
+    """n x n squares of side size/n in the x-z plane at y = 0, each cut into
+    two triangles with normals along +y, optionally jittered along y."""
+    xs = np.linspace(0.0, size, n + 1)
+    X = np.array([[x, 0.0, z] for z in xs for x in xs])
+    if amp > 0:
+        X[:, 1] += np.random.default_rng(seed).normal(0, amp, X.shape[0])
+    T = []
+    for j in range(n):
+        for i in range(n):
+            a = j * (n + 1) + i
+            b, c, d = a + 1, a + n + 1, a + n + 2
+            T += [[a, c, b], [b, c, d]]
+    return X, np.array(T, np.int32)
+
+
+def draped_sheet(n=8, level=2, radius=0.3, gap=0.003):
+    """A sheet of n x n squares falling on a sphere shell (its nodes are
+    Dirichlet) whose top is `gap` below the sheet's centre.  Returns
+    (X, tris, dbc indices)."""
+    Xc, Tc = shell_sheet(n)
+    Xs, Ts = icosphere(level, radius, (0.5, -(radius + gap), 0.5))
+    X = np.vstack([Xc, Xs])
+    T = np.vstack([Tc, Ts + Xc.shape[0]]).astype(np.int32)
+    return X, T, np.arange(Xc.shape[0], X.shape[0], dtype=np.int32)

@@ -175,6 +175,15 @@ int idp_shell_init(int nV, const double* X, int nE, const int* tris, double dens
  * bnode >= nV, bedge >= 2 (3 nE + 12 nT), btri >= 3 (nE + 4 nT) ints. */
 int idp_shell_surface(int nV, const double* X, int nT, const int* tets, int nE, const int* tris, int* nb, int* ne,
                       int* nf, int* bnode, int* bedge, int* btri);
+/* Hinge stencils and rest data of a shell (Compute_Discrete_Shell_Inv_Basis,
+ * Library/FEM/Shell/DISCRETE_SHELL.h:132-215): for every edge of the
+ * triangles (first-seen orientation, sorted: MESHIO.h:728-766) with a
+ * triangle on both sides, the stencil (v, e0, e1, w) and (rest dihedral
+ * angle, |e|, (|n1| + |n2|) / (6 |e|)); tri_IB (nullable, 3 per triangle): the
+ * rest first fundamental forms.  hinges >= 4 * 3 nE / 2 ints, hinge_rest
+ * >= 3 * 3 nE / 2 doubles. */
+int idp_shell_hinges(int nV, const double* X, int nE, const int* tris, int* nH, int* hinges, double* hinge_rest,
+                     double* tri_IB);
 /* Initialize_EIPC<double,false> as bound to Initialize_OIPC
  * (Library/FEM/Shell/DISCRETE_SHELL.h:555-577, 1105). kappa3 = {kappa, kappa_max, 0}. */
 int idp_initialize_oipc(double dhat, int nV, const double* mass, double stiff_mult, double* kappa3,
@@ -191,12 +200,25 @@ int idp_set_mesh(idp_ctx* ctx, int nV, const double* X_rest, int nT, const int* 
  * shell's elasticityAttr (area * thickness, from idp_shell_init).  The
  * surface primitives are those of idp_shell_surface.  One instance.  Shell
  * triangles carry the flow variant's -h/2 x^T L x term
- * (Library/FEM/Shell/INC_POTENTIAL.h:53-73, 191-211, 323-339); their
- * membrane and bending terms are not built, so only flow = 1 steps and
- * evaluations are accepted on such a mesh (IDP_E_INVALID otherwise). */
+ * (Library/FEM/Shell/INC_POTENTIAL.h:53-73, 191-211, 323-339) and, in
+ * non-flow steps, the membrane and hinge bending of idp_set_shell_elasticity
+ * (IDP_E_INVALID until it is called).  hinges (4 per hinge, from
+ * idp_shell_hinges; nH may be 0) couple the four vertices of each stencil in
+ * the static BSR pattern. */
 int idp_set_mesh_shell(idp_ctx* ctx, int nV, const double* X_rest, const double* mass, int nT, const int* tets,
                        const double* IB, const double* vol, const double* lambda, const double* mu, int nE,
-                       const int* tris, const double* tri_vol);
+                       const int* tris, const double* tri_vol, int nH, const int* hinges);
+/* The non-flow shell terms (Advance_One_Step_IE_Hinge on a shell): the
+ * neo-Hookean membrane (Library/FEM/Shell/MEMBRANE.h:8-312, useNH) per
+ * triangle from its rest first fundamental form tri_IB (3 per triangle:
+ * |E01|^2, E01.E02, |E02|^2) and Lame parameters, and the discrete hinge
+ * bending (Library/FEM/Shell/BENDING.h:9-80, 150-214, 440-497) per hinge of
+ * idp_set_mesh_shell from hinge_rest (3 per hinge: rest angle, |e|, h) with
+ * stiffness bend_k (bendingStiffMult E t^3 / (24 (1 - nu^2)),
+ * DISCRETE_SHELL.h:333-339).  Elements whose vertices are all Dirichlet
+ * nodes contribute nothing, as there. */
+int idp_set_shell_elasticity(idp_ctx* ctx, const double* tri_IB, const double* tri_lambda, const double* tri_mu,
+                             const double* hinge_rest, double bend_k);
 /* Update_Normal_Flow_Neumann (Library/FEM/Shell/DISCRETE_SHELL.h:359-395) on
  * the device: b (3 nV, host) = magnitude * m_v * n_v / |n_v| with n_v the sum
  * of cross(X2-X1, X3-X1) over the shell triangles of v in order, at X (host,

@@ -188,6 +188,12 @@ struct Ctx {
     int nE = 0;
     const int* tris = nullptr;
     const double* tvol = nullptr;
+    // non-flow shell terms (membrane, hinge bending); null IB: none
+    const double *tIB = nullptr, *tlam = nullptr, *tmu = nullptr;
+    int nH = 0;
+    const int* hinge = nullptr;
+    const double* hrest = nullptr;
+    double bend_k = 0.0;
     std::vector<double> Xtilde;
     std::vector<int> cs;
     long ncs = 0;
@@ -245,6 +251,12 @@ void dbc_energy(const Ctx& c, const double* X, double& E)
     E += 0.5 * c.dbc_stiff * s;
 }
 
+extern "C" void orc_shell_elastic_terms(int nV, const double* X, int nE, const int* tris, const double* IB,
+                                        const double* vol, const double* lam, const double* mu, int nH,
+                                        const int* stencil, const double* rest, double k, double w,
+                                        const unsigned char* dbc, int project, double* E, double* g, double* Hm,
+                                        double* Hb);
+
 // Lx of the flow term (INC_POTENTIAL.h:53-66): per triangle in order, for
 // each corner i, Lx_i += vol/6 (x_(i+1) + x_(i+2) - 2 x_i)
 void flow_lx(const Ctx& c, const double* X, std::vector<double>& Lx)
@@ -268,6 +280,12 @@ bool energy(Ctx& c, const double* X, bool withBarrier, double& E)
     if (!c.flow && !orc_elastic_energy(X, c.nT, c.tets, c.IB, c.vol, c.lam, c.mu, c.h * c.h, &Ee)) {
         c.t_E += now() - t0;
         return false;
+    }
+    if (!c.flow && c.nE && c.tIB) {  // membrane and bending, w = h^2 (INC_POTENTIAL.h:80-95)
+        double Es = 0.0;
+        orc_shell_elastic_terms(c.nV, X, c.nE, c.tris, c.tIB, c.tvol, c.tlam, c.tmu, c.nH, c.hinge, c.hrest,
+                                c.bend_k, c.h * c.h, c.dbc.data(), 1, &Es, nullptr, nullptr, nullptr);
+        Ee += Es;
     }
     if (c.flow && c.nE) {  // INC_POTENTIAL.h:67-72: value += -h/2 x.Lx, vertex by vertex
         std::vector<double> Lx;
@@ -317,6 +335,13 @@ void assemble_sys(Ctx& c, const double* X, Sys& S)
             add_local(blks, nv, vl.data() + 4 * i, Hl.data() + 144 * i, nV);
         }
     }
+    if (!c.flow && c.nE && c.tIB) {  // projected membrane (9 x 9) and hinge (12 x 12) blocks
+        std::vector<double> Hm(81 * (size_t)c.nE), Hb(144 * (size_t)c.nH);
+        orc_shell_elastic_terms(nV, X, c.nE, c.tris, c.tIB, c.tvol, c.tlam, c.tmu, c.nH, c.hinge, c.hrest,
+                                c.bend_k, c.h * c.h, c.dbc.data(), 1, nullptr, nullptr, Hm.data(), Hb.data());
+        for (int e = 0; e < c.nE; ++e) add_local(blks, 3, c.tris + 3 * e, Hm.data() + 81 * (size_t)e, nV);
+        for (int k = 0; k < c.nH; ++k) add_local(blks, 4, c.hinge + 4 * k, Hb.data() + 144 * (size_t)k, nV);
+    }
     if (c.flow && c.nE) {  // INC_POTENTIAL.h:323-339: -h vol/6 to both neighbours, 2 h vol/6 on the diagonal
         for (int e = 0; e < c.nE; ++e) {
             const int* t = c.tris + 3 * e;
@@ -364,17 +389,34 @@ void assemble_sys(Ctx& c, const double* X, Sys& S)
 }  // namespace
 
 namespace {
+struct ShellElastic {
+    const double *IB, *lam, *mu;
+    int nH;
+    const int* hinge;
+    const double* rest;
+    double k;
+};
+
 int advance_impl(int nV, double* X, const double* X0, double* vel, const double* mass, int nT, const int* tets,
                  const double* IB, const double* vol, const double* lam, const double* mu, int nE, const int* tris,
                  const double* tvol, int nb, const int* bnode, int ne, const int* bedge, int nf, const int* btri,
                  int nDBC, const int* dbc_idx, const double* dbc_target, double h, double NewtonTol,
                  int withCollision, double dHat2, double* kappa, double thickness, double lin_rtol, int max_newton,
-                 const double* body_force, int flow, orc_stats* st)
+                 const double* body_force, int flow, orc_stats* st, const ShellElastic* se = nullptr)
 {
     Ctx c;
     c.nE = nE;
     c.tris = tris;
     c.tvol = tvol;
+    if (se) {
+        c.tIB = se->IB;
+        c.tlam = se->lam;
+        c.tmu = se->mu;
+        c.nH = se->nH;
+        c.hinge = se->hinge;
+        c.hrest = se->rest;
+        c.bend_k = se->k;
+    }
     c.nV = nV; c.nT = nT; c.nb = nb; c.ne = ne; c.nf = nf;
     c.X0 = X0; c.mass = mass; c.tets = tets; c.IB = IB; c.vol = vol; c.lam = lam; c.mu = mu;
     c.bnode = bnode; c.bedge = bedge; c.btri = btri;
@@ -442,6 +484,9 @@ int advance_impl(int nV, double* X, const double* X0, double* vel, const double*
         double t0 = now();
         std::fill(g.begin(), g.end(), 0.0);
         if (!c.flow) orc_elastic_gradient(X, nT, tets, IB, vol, lam, mu, h * h, g.data());
+        if (!c.flow && c.nE && c.tIB)
+            orc_shell_elastic_terms(nV, X, c.nE, c.tris, c.tIB, c.tvol, c.tlam, c.tmu, c.nH, c.hinge, c.hrest,
+                                    c.bend_k, h * h, c.dbc.data(), 1, nullptr, g.data(), nullptr, nullptr);
         if (c.flow && c.nE) {  // INC_POTENTIAL.h:191-211: g += -h Lx
             std::vector<double> Lx;
             flow_lx(c, X, Lx);
@@ -646,12 +691,16 @@ long newton_system_impl(int nV, const double* X, const double* Xtilde, const dou
                         int nE, const int* tris, const double* tvol, bool flow, int nb, const int* bnode, int ne,
                         const int* bedge, int nf, const int* btri, int nDBC, const int* dbc_idx, double h,
                         double dHat2, double kappa, double thickness, int* rowptr, int* col, double* val, double* g,
-                        long cap)
+                        long cap, const ShellElastic* se = nullptr)
 {
     Ctx c;
     c.nV = nV; c.nT = nT; c.nb = nb; c.ne = ne; c.nf = nf;
     c.X0 = X0; c.mass = mass; c.tets = tets; c.IB = IB; c.vol = vol; c.lam = lam; c.mu = mu;
     c.nE = nE; c.tris = tris; c.tvol = tvol; c.flow = flow;
+    if (se) {
+        c.tIB = se->IB; c.tlam = se->lam; c.tmu = se->mu;
+        c.nH = se->nH; c.hinge = se->hinge; c.hrest = se->rest; c.bend_k = se->k;
+    }
     c.bnode = bnode; c.bedge = bedge; c.btri = btri;
     c.dbc.assign(nV, 0);
     for (int i = 0; i < nDBC; ++i) c.dbc[dbc_idx[i]] = 1;
@@ -669,6 +718,9 @@ long newton_system_impl(int nV, const double* X, const double* Xtilde, const dou
     std::copy(S.val.begin(), S.val.end(), val);
     std::fill(g, g + 3 * (size_t)nV, 0.0);
     if (!flow) orc_elastic_gradient(X, nT, tets, IB, vol, lam, mu, h * h, g);
+    if (!flow && nE && c.tIB)
+        orc_shell_elastic_terms(nV, X, nE, tris, c.tIB, tvol, c.tlam, c.tmu, c.nH, c.hinge, c.hrest, c.bend_k, h * h,
+                                c.dbc.data(), 1, nullptr, g, nullptr, nullptr);
     if (flow && nE) {
         std::vector<double> Lx;
         flow_lx(c, X, Lx);
@@ -683,6 +735,22 @@ long newton_system_impl(int nV, const double* X, const double* Xtilde, const dou
 }
 }  // namespace
 
+// Advance_One_Step_IE_Hinge on a shell: membrane + hinge bending + inertia +
+// barrier, body force b, DBC nodes staying
+extern "C" int orc_advance_shell(int nV, double* X, const double* X0, double* vel, const double* mass, int nE,
+                                 const int* tris, const double* tvol, const double* tIB, const double* tlam,
+                                 const double* tmu, int nH, const int* hinge, const double* hrest, double bend_k,
+                                 int nb, const int* bnode, int ne, const int* bedge, int nf, const int* btri, int nDBC,
+                                 const int* dbc_idx, double h, double NewtonTol, int withCollision, double dHat2,
+                                 double* kappa, double thickness, double lin_rtol, int max_newton,
+                                 const double* body_force, orc_stats* st)
+{
+    ShellElastic se{tIB, tlam, tmu, nH, hinge, hrest, bend_k};
+    return advance_impl(nV, X, X0, vel, mass, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nE, tris, tvol, nb,
+                        bnode, ne, bedge, nf, btri, nDBC, dbc_idx, nullptr, h, NewtonTol, withCollision, dHat2, kappa,
+                        thickness, lin_rtol, max_newton, body_force, 0, st, &se);
+}
+
 extern "C" long orc_newton_system(int nV, const double* X, const double* Xtilde, const double* X0,
                                   const double* mass, int nT, const int* tets, const double* IB, const double* vol,
                                   const double* lam, const double* mu, int nb, const int* bnode, int ne,
@@ -693,6 +761,21 @@ extern "C" long orc_newton_system(int nV, const double* X, const double* Xtilde,
     return newton_system_impl(nV, X, Xtilde, X0, mass, nT, tets, IB, vol, lam, mu, 0, nullptr, nullptr, false, nb,
                               bnode, ne, bedge, nf, btri, nDBC, dbc_idx, h, dHat2, kappa, thickness, rowptr, col, val,
                               g, cap);
+}
+
+// the Hinge variant's system on a shell (membrane + bending + inertia + barrier)
+extern "C" long orc_newton_system_shell(int nV, const double* X, const double* Xtilde, const double* X0,
+                                        const double* mass, int nE, const int* tris, const double* tvol,
+                                        const double* tIB, const double* tlam, const double* tmu, int nH,
+                                        const int* hinge, const double* hrest, double bend_k, int nb,
+                                        const int* bnode, int ne, const int* bedge, int nf, const int* btri, int nDBC,
+                                        const int* dbc_idx, double h, double dHat2, double kappa, double thickness,
+                                        int* rowptr, int* col, double* val, double* g, long cap)
+{
+    ShellElastic se{tIB, tlam, tmu, nH, hinge, hrest, bend_k};
+    return newton_system_impl(nV, X, Xtilde, X0, mass, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nE, tris, tvol,
+                              false, nb, bnode, ne, bedge, nf, btri, nDBC, dbc_idx, h, dHat2, kappa, thickness, rowptr,
+                              col, val, g, cap, &se);
 }
 
 // the flow variant's system on a shell (flow term + inertia + barrier)

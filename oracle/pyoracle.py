@@ -103,6 +103,24 @@ def lib():
                                                    ip, C.c_int, ip, C.c_int, ip, C.c_double, C.c_double, C.c_double,
                                                    C.c_double, ip, ip, dp, dp, C.c_long]
         L.orc_newton_system_shell_flow.restype = C.c_long
+        L.orc_shell_rest_fff.argtypes = [dp, C.c_int, ip, dp]
+        L.orc_shell_rest_fff.restype = None
+        L.orc_shell_hinges.argtypes = [dp, C.c_int, ip, C.c_int, ip, ip, dp]
+        L.orc_shell_elastic_terms.argtypes = [C.c_int, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int, ip, dp, C.c_double,
+                                              C.c_double, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_void_p]
+        L.orc_shell_elastic_terms.restype = None
+        L.orc_dihedral.argtypes = [dp, C.c_void_p]
+        L.orc_dihedral.restype = C.c_double
+        L.orc_advance_shell.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int, ip, dp,
+                                        C.c_double, C.c_int, ip, C.c_int, ip, C.c_int, ip, C.c_int, ip, C.c_double,
+                                        C.c_double, C.c_int, C.c_double, dp, C.c_double, C.c_double, C.c_int,
+                                        C.c_void_p, C.POINTER(Stats)]
+        L.orc_newton_system_shell.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int, ip, dp,
+                                              C.c_double, C.c_int, ip, C.c_int, ip, C.c_int, ip, C.c_int, ip,
+                                              C.c_double, C.c_double, C.c_double, C.c_double, ip, ip, dp, dp,
+                                              C.c_long]
+        L.orc_newton_system_shell.restype = C.c_long
         _lib = L
     return _lib
 
@@ -282,10 +300,24 @@ def initialize_oipc(dhat, mass, stiff_mult=1.0):
     return dHat2, np.array([k0, 100 * k0, 0.0])
 
 
-class ShellMesh:
-    """A shell (triangles, no tets) as Initialize_Discrete_Shell sets it up."""
+def _vp(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
 
-    def __init__(self, X, tris, density, thickness, E=0.0, nu=0.0):
+
+def dihedral(x4, grad=False):
+    """(angle, jet gradient (12,) or None) of the hinge x4 = (x0, x1, x2, x3)."""
+    x = _f(x4)
+    g = np.zeros(12) if grad else None
+    th = lib().orc_dihedral(x, _vp(g))
+    return th, g
+
+
+class ShellMesh:
+    """A shell (triangles, no tets) as Initialize_Discrete_Shell sets it up,
+    with the hinge stencils and rest data of Compute_Discrete_Shell_Inv_Basis;
+    bending stiffness k = bend_mult E t^3 / (24 (1 - nu^2))."""
+
+    def __init__(self, X, tris, density, thickness, E=0.0, nu=0.0, bend_mult=1.0):
         self.X = _f(X).copy()
         self.X0 = self.X.copy()
         tris = _i(tris)
@@ -311,6 +343,68 @@ class ShellMesh:
         self.bedge = be[: 2 * ne.value].reshape(-1, 2).copy()
         self.btri = bt[: 3 * nf.value].reshape(-1, 3).copy()
         self.dbc = np.zeros(self.nV, np.uint8)
+        self.tIB = np.zeros(3 * n)
+        lib().orc_shell_rest_fff(self.X0, n, self.tris, self.tIB)
+        st = np.zeros(4 * max(self.bedge.shape[0], 1), np.int32)
+        rest = np.zeros(3 * max(self.bedge.shape[0], 1))
+        nh = lib().orc_shell_hinges(self.X0, n, self.tris, self.bedge.shape[0], _i(self.bedge), st, rest)
+        self.nH = nh
+        self.hinge = st[: 4 * nh].copy()
+        self.hrest = rest[: 3 * nh].copy()
+        self.bend_k = bend_mult * E * thickness ** 3 / (24 * (1 - nu * nu))
+
+    def set_dbc(self, idx):
+        self.dbc[:] = 0
+        self.dbc[np.asarray(idx, dtype=np.int64)] = 1
+
+    def elastic_terms(self, X, h, project=True, want=("E", "g", "H")):
+        """Membrane + bending: (E, g (3 nV), Hm (nE, 9, 9), Hb (nH, 12, 12))."""
+        E = C.c_double()
+        g = np.zeros(3 * self.nV) if "g" in want else None
+        Hm = np.zeros(81 * self.nE) if "H" in want else None
+        Hb = np.zeros(144 * max(self.nH, 1)) if "H" in want else None
+        lib().orc_shell_elastic_terms(self.nV, _f(X), self.nE, self.tris, self.tIB, self.tvol, self.lam, self.mu,
+                                      self.nH, self.hinge, self.hrest, self.bend_k, h * h, _vp(self.dbc),
+                                      int(project), C.cast(C.byref(E), C.c_void_p), _vp(g), _vp(Hm), _vp(Hb))
+        return (E.value, g, None if Hm is None else Hm.reshape(self.nE, 9, 9),
+                None if Hb is None else Hb[: 144 * self.nH].reshape(self.nH, 12, 12))
+
+    def newton_system_hinge(self, X, Xtilde, h, dHat2, kappa, thickness=0.0, dbc_idx=None):
+        """(rowptr, col, val, g) of the Hinge variant's projected system at X."""
+        X = _f(X)
+        Xt = _f(Xtilde)
+        dbc_idx = np.zeros(0, np.int32) if dbc_idx is None else _i(dbc_idx)
+        cap = 64 * self.nV + 1024
+        while True:
+            rp = np.zeros(self.nV + 1, np.int32)
+            col = np.zeros(cap, np.int32)
+            val = np.zeros(9 * cap)
+            g = np.zeros(3 * self.nV)
+            n = lib().orc_newton_system_shell(self.nV, X, Xt, self.X0, self.mass, self.nE, self.tris, self.tvol,
+                                              self.tIB, self.lam, self.mu, self.nH, self.hinge, self.hrest,
+                                              self.bend_k, self.bnode.size, self.bnode, self.bedge.shape[0],
+                                              _i(self.bedge), self.btri.shape[0], _i(self.btri), dbc_idx.size,
+                                              dbc_idx, h, dHat2, kappa, thickness, rp, col, val, g, cap)
+            if n >= 0:
+                return rp, col[:n], val[: 9 * n].reshape(n, 3, 3), g
+            cap = -n
+
+    def advance_hinge(self, X, vel, h, tol, dHat2, kappa, thickness=0.0, lin_rtol=1e-12, max_newton=0,
+                      with_collision=True, dbc_idx=None, body_force=None):
+        """One Advance_One_Step_IE_Hinge step (DBC nodes stay)."""
+        X = _f(X).copy()
+        vel = _f(vel).copy()
+        bf = None if body_force is None else _f(body_force).copy()
+        kap = np.ascontiguousarray(kappa, dtype=np.float64).copy()
+        dbc_idx = np.zeros(0, np.int32) if dbc_idx is None else _i(dbc_idx)
+        st = Stats()
+        it = lib().orc_advance_shell(self.nV, X, self.X0, vel, self.mass, self.nE, self.tris, self.tvol, self.tIB,
+                                     self.lam, self.mu, self.nH, self.hinge, self.hrest, self.bend_k,
+                                     self.bnode.size, self.bnode, self.bedge.shape[0], _i(self.bedge),
+                                     self.btri.shape[0], _i(self.btri), dbc_idx.size, dbc_idx, h, tol,
+                                     int(with_collision), dHat2, kap, thickness, lin_rtol, max_newton,
+                                     None if bf is None else bf.ctypes.data_as(C.c_void_p), C.byref(st))
+        return it, X.reshape(-1, 3), vel.reshape(-1, 3), kap, st.as_dict()
 
     def normal_flow_force(self, X, magnitude):
         b = np.zeros(3 * self.nV)

@@ -39,7 +39,7 @@ def test_shell_surface_and_normal_flow_force():
         ctx.close()
 
 
-def test_non_flow_step_on_a_shell_is_refused():
+def test_non_flow_step_on_a_shell_without_elasticity_is_refused():
     sc = scenes.sphere_pair()
     ctx = _ctx(sc)
     try:

@@ -238,3 +238,184 @@ def test_flow_step_end_state_meets_the_stopping_rule(pressed_pair):
     A = sp.bsr_matrix((val, col, rp), shape=(3 * m.nV, 3 * m.nV)).tocsc()
     p = spla.spsolve(A, -g)
     assert np.sqrt(np.sum(p * p) / m.nV) / sc.h <= sc.newton_tol
+
+
+# ------------------------------------------------- membrane and hinge bending
+def _sheet(n=6, seed=0, amp=0.02):
+    """n x n grid of squares in the x-z plane, each cut into two triangles
+    (normals +y), lightly jittered out of plane."""
+    xs = np.linspace(0.0, 1.0, n + 1)
+    X = np.array([[x, 0.0, z] for z in xs for x in xs])
+    X[:, 1] += np.random.default_rng(seed).normal(0, amp, X.shape[0])
+    T = []
+    for j in range(n):
+        for i in range(n):
+            a = j * (n + 1) + i
+            b, c, d = a + 1, a + n + 1, a + n + 2
+            T += [[a, c, b], [b, c, d]]
+    return X, np.array(T, np.int32)
+
+
+def _nh_membrane_2d(X0, X, T, vol, lam, mu, w):
+    """Neo-Hookean membrane from the deformation gradient in a 2D rest frame:
+    F = [x1-x0, x2-x0] Dm^-1, C = F^T F, psi = mu/2 (tr C - 2 - 2 ln J) + lam/2 ln^2 J."""
+    E = 0.0
+    for e, t in enumerate(T):
+        a, b = X0[t[1]] - X0[t[0]], X0[t[2]] - X0[t[0]]
+        u = a / np.linalg.norm(a)
+        nrm = np.cross(a, b)
+        v = np.cross(nrm / np.linalg.norm(nrm), u)
+        Dm = np.array([[a @ u, b @ u], [a @ v, b @ v]])
+        Ds = np.stack([X[t[1]] - X[t[0]], X[t[2]] - X[t[0]]], 1)
+        F = Ds @ np.linalg.inv(Dm)
+        Cm = F.T @ F
+        lnJ = 0.5 * np.log(np.linalg.det(Cm))
+        E += w * vol[e] * (mu[e] / 2 * (np.trace(Cm) - 2 - 2 * lnJ) + lam[e] / 2 * lnJ ** 2)
+    return E
+
+
+def _theta_numpy(x0, x1, x2, x3):
+    n1, n2 = np.cross(x1 - x0, x2 - x0), np.cross(x2 - x3, x1 - x3)
+    e = (x2 - x1) / np.linalg.norm(x2 - x1)
+    return np.arctan2(np.cross(n1, n2) @ e, n1 @ n2)
+
+
+def test_membrane_energy_is_the_deformation_gradient_form():
+    X, T = _sheet()
+    m = pyoracle.ShellMesh(X, T, 1.0, 0.01, E=1e4, nu=0.3, bend_mult=0.0)
+    Y = X + np.random.default_rng(1).normal(0, 0.03, X.shape)
+    E, _, _, _ = m.elastic_terms(Y, 0.1, want=("E",))
+    ref = _nh_membrane_2d(X, Y, m.tris.reshape(-1, 3), m.tvol, m.lam, m.mu, 0.01)
+    assert E == pytest.approx(ref, rel=1e-11)
+    assert E > 0
+    # rigid motion leaves it at rest value 0
+    R = np.linalg.qr(np.random.default_rng(2).normal(size=(3, 3)))[0]
+    if np.linalg.det(R) < 0:
+        R[:, 0] *= -1
+    E0, g0, _, _ = m.elastic_terms(X @ R.T + 0.3, 0.1, want=("E", "g"))
+    assert abs(E0) < 1e-14 and np.abs(g0).max() < 1e-12
+
+
+def test_dihedral_angle_closed_forms_and_sign():
+    x1, x2 = np.array([0.0, 0, 0]), np.array([0.0, 0, 1])
+    x0 = np.array([1.0, 0, 0.5])
+    for ang in (0.0, 0.3, -0.7, 1.2, 2.5):
+        # x3 on the far side of the edge, rotated by ang out of the plane of x0
+        x3 = np.array([-np.cos(ang), np.sin(ang), 0.5])
+        th, g = pyoracle.dihedral(np.stack([x0, x1, x2, x3]), grad=True)
+        assert th == pytest.approx(_theta_numpy(x0, x1, x2, x3), abs=1e-12)
+        assert abs(abs(th) - abs(ang)) < 1e-12
+        x = np.stack([x0, x1, x2, x3]).ravel()
+        fd = np.zeros(12)
+        for i in range(12):
+            d = np.zeros(12)
+            d[i] = 1e-7
+            fd[i] = (_theta_numpy(*(x + d).reshape(4, 3)) - _theta_numpy(*(x - d).reshape(4, 3))) / 2e-7
+        assert np.allclose(g, fd, atol=1e-7)
+    # the sign follows (n2 x n1).(x1 - x2)
+    x3 = np.array([-1.0, 0.4, 0.5])
+    th, _ = pyoracle.dihedral(np.stack([x0, x1, x2, x3]))
+    n1, n2 = np.cross(x1 - x0, x2 - x0), np.cross(x2 - x3, x1 - x3)
+    assert np.sign(th) == np.sign(np.cross(n2, n1) @ (x1 - x2))
+
+
+def test_hinge_stencils_and_bending_energy():
+    X, T = _sheet(5, seed=3)
+    m = pyoracle.ShellMesh(X, T, 1.0, 0.02, E=1e4, nu=0.4, bend_mult=2.0)
+    st = m.hinge.reshape(-1, 4)
+    # interior edges of a 5 x 5 grid: 3 n^2 + 2 n - 2 (n + 1) ... counted directly
+    tris = [tuple(t) for t in m.tris.reshape(-1, 3)]
+    interior = {tuple(sorted(e)) for e in m.bedge.tolist()
+                if sum(1 for t in tris if e[0] in t and e[1] in t) == 2}
+    assert st.shape[0] == len(interior)
+    for s in st:
+        assert tuple(sorted((s[1], s[2]))) in interior
+        # v0 lies in a triangle with the directed edge s1 -> s2, s3 in one with s2 -> s1
+        assert any(t == (s[1], s[2], s[0]) or t == (s[2], s[0], s[1]) or t == (s[0], s[1], s[2]) for t in tris)
+        assert any(t == (s[2], s[1], s[3]) or t == (s[1], s[3], s[2]) or t == (s[3], s[2], s[1]) for t in tris)
+    assert m.bend_k == pytest.approx(2.0 * 1e4 * 0.02 ** 3 / (24 * (1 - 0.16)))
+    Y = X + np.random.default_rng(4).normal(0, 0.02, X.shape)
+    h = 0.1
+    mem = pyoracle.ShellMesh(X, T, 1.0, 0.02, E=1e4, nu=0.4, bend_mult=0.0)
+    Eb = m.elastic_terms(Y, h, want=("E",))[0] - mem.elastic_terms(Y, h, want=("E",))[0]
+    r = m.hrest.reshape(-1, 3)
+    ref = sum(h * h * m.bend_k * (_theta_numpy(*Y[s]) - r[i, 0]) ** 2 * r[i, 1] / r[i, 2] for i, s in enumerate(st))
+    assert Eb == pytest.approx(ref, rel=1e-10)
+    for i, s in enumerate(st):  # rest data
+        x0, x1, x2, x3 = X[s]
+        assert r[i, 0] == pytest.approx(_theta_numpy(x0, x1, x2, x3), abs=1e-12)
+        assert r[i, 1] == pytest.approx(np.linalg.norm(x1 - x2))
+        assert r[i, 2] == pytest.approx((np.linalg.norm(np.cross(x1 - x0, x2 - x0)) +
+                                         np.linalg.norm(np.cross(x2 - x3, x1 - x3))) / (6 * r[i, 1]))
+
+
+def test_shell_elastic_gradient_and_hessian():
+    X, T = _sheet(4, seed=5)
+    m = pyoracle.ShellMesh(X, T, 1.0, 0.02, E=1e4, nu=0.3, bend_mult=5.0)
+    h = 0.1
+    Y = (X + np.random.default_rng(6).normal(0, 0.03, X.shape)).ravel()
+    E, g, Hm, Hb = m.elastic_terms(Y, h, project=False)
+    rng = np.random.default_rng(7)
+    for _ in range(3):
+        d = rng.standard_normal(Y.size)
+        d /= np.linalg.norm(d)
+        eps = 1e-6
+        fd = (m.elastic_terms(Y + eps * d, h, want=("E",))[0] - m.elastic_terms(Y - eps * d, h, want=("E",))[0]) / (2 * eps)
+        assert fd == pytest.approx(g @ d, rel=1e-6, abs=1e-9 * np.linalg.norm(g))
+    n = m.nV
+    A = np.zeros((3 * n, 3 * n))
+    for e, t in enumerate(m.tris.reshape(-1, 3)):
+        for a in range(3):
+            for b in range(3):
+                A[3 * t[a]:3 * t[a] + 3, 3 * t[b]:3 * t[b] + 3] += Hm[e, 3 * a:3 * a + 3, 3 * b:3 * b + 3]
+    for k, s in enumerate(m.hinge.reshape(-1, 4)):
+        for a in range(4):
+            for b in range(4):
+                A[3 * s[a]:3 * s[a] + 3, 3 * s[b]:3 * s[b] + 3] += Hb[k, 3 * a:3 * a + 3, 3 * b:3 * b + 3]
+    for _ in range(3):
+        d = rng.standard_normal(Y.size)
+        d /= np.linalg.norm(d)
+        eps = 1e-6
+        fd = (m.elastic_terms(Y + eps * d, h, want=("g",))[1] - m.elastic_terms(Y - eps * d, h, want=("g",))[1]) / (2 * eps)
+        assert np.linalg.norm(fd - A @ d) <= 1e-6 * np.linalg.norm(A @ d)
+    # the projection is the spectral clamp of each local matrix
+    _, _, Pm, Pb = m.elastic_terms(Y, h, project=True)
+    for H, P in list(zip(Hm, Pm)) + list(zip(Hb, Pb)):
+        w, V = np.linalg.eigh(H)
+        ref = (V * np.maximum(w, 0)) @ V.T if w.min() < 0 else H
+        assert np.abs(P - ref).max() <= 1e-9 * max(np.abs(H).max(), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def draped():
+    """A sheet over a sphere: the sphere's nodes are Dirichlet, gravity pulls the sheet down."""
+    Xs, Ts = scenes.icosphere(2, 0.3, (0.5, -0.303, 0.5))
+    Xc, Tc = _sheet(8, seed=8, amp=0.0)
+    X = np.vstack([Xc, Xs])
+    T = np.vstack([Tc, Ts + Xc.shape[0]]).astype(np.int32)
+    m = pyoracle.ShellMesh(X, T, 100.0, 0.003, E=1e4, nu=0.3, bend_mult=1.0)
+    dbc = np.arange(Xc.shape[0], X.shape[0], dtype=np.int32)
+    m.set_dbc(dbc)
+    dHat2, kappa = pyoracle.initialize_oipc(1e-3, m.mass)
+    b = np.zeros_like(X)
+    b[:, 1] = -9.81 * m.mass
+    h = 0.01
+    Xk, v = X.copy(), np.zeros_like(X)
+    its = []
+    for _ in range(4):
+        it, Xk, v, kappa, st = m.advance_hinge(Xk, v, h, 1e-3 * h, dHat2, kappa, dbc_idx=dbc, body_force=b)
+        its.append((it, st["n_constraints"]))
+    return m, X, Xk, v, dbc, dHat2, kappa, b, h, its
+
+
+def test_hinge_step_reaches_contact_and_meets_the_stopping_rule(draped):
+    m, X, Xk, v, dbc, dHat2, kappa, b, h, its = draped
+    assert its[-1][1] > 0  # the sheet reached the sphere
+    assert np.abs(Xk[dbc] - X[dbc]).max() == 0.0
+    it, Xo, _, ko, st = m.advance_hinge(Xk, v, h, 1e-3 * h, dHat2, kappa, dbc_idx=dbc, body_force=b)
+    assert it > 1
+    Xt = (Xk + h * v + h * h * b / m.mass[:, None]).ravel()
+    rp, col, val, g = m.newton_system_hinge(Xo, Xt, h, dHat2, ko[0], dbc_idx=dbc)
+    A = sp.bsr_matrix((val, col, rp), shape=(3 * m.nV, 3 * m.nV)).tocsc()
+    p = spla.spsolve(A, -g)
+    assert np.sqrt(np.sum(p * p) / (m.nV - dbc.size)) / h <= 1e-3 * h
