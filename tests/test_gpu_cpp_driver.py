@@ -1,9 +1,10 @@
 """A compiled C++ program (tests/cpp/step_driver.cpp, no Python in its loop)
 drives the step through the C ABI and idp_amd::StepSession on MI355X, in the
-Hinge and the Flow variant (DISCRETE_SHELL.h:1110, 1113), against the oracle:
-same Newton counts, end state within 1e-6 of the bounding-box diagonal.  The
-program calls set_mesh before each of its two steps; the mesh must go to the
-device once."""
+Hinge and the Flow variant (DISCRETE_SHELL.h:1110, 1113), on tets and on a
+shell with membrane and hinge bending, against the oracle: same Newton
+counts, end state within 1e-6 of the bounding-box diagonal.  The program
+calls set_mesh before each of its two steps; the mesh must go to the device
+once."""
 import os
 import subprocess
 
@@ -23,23 +24,42 @@ def say(*a):
     print(*a, flush=True)
 
 
-def _write_input(path, sc, v, b, flow, steps, dHat2, kappa, dbc=None, tgt=None, zero_v=False):
-    vol, IB, lam, mu, mass = capi.fem_init(sc.X, sc.tets, sc.density, sc.E, sc.nu)
-    nV, nT = sc.X.shape[0], sc.tets.shape[0]
+def _write_input(path, sc, v, b, flow, steps, dHat2, kappa, dbc=None, tgt=None, zero_v=False, shell=None):
+    """shell: None, or (X, tris, tri_vol, tri_lam, tri_mu, tri_IB, hinges,
+    hinge_rest, bend_k, mass) for a shell-only mesh (sc then gives h and the
+    Newton tolerance only)."""
+    if shell is None:
+        X = sc.X
+        vol, IB, lam, mu, mass = capi.fem_init(sc.X, sc.tets, sc.density, sc.E, sc.nu)
+        tets = sc.tets
+        nE = nH = 0
+        bend_k = 0.0
+    else:
+        X, tris, tvol, tlam, tmu, tIB, hinges, hrest, bend_k, mass = shell
+        tets = np.zeros((0, 4), np.int32)
+        IB = vol = lam = mu = np.zeros(0)
+        nE, nH = tris.shape[0], hinges.shape[0]
+    nV, nT = X.shape[0], tets.shape[0]
     dbc = np.zeros(0, np.int32) if dbc is None else np.asarray(dbc, np.int32)
     tgt = np.zeros((0, 3)) if tgt is None else np.asarray(tgt, np.float64)
     with open(path, "wb") as f:
-        np.array([nV, nT, dbc.size, int(flow), int(b is not None), steps, int(zero_v)], np.int32).tofile(f)
-        np.array([sc.h, sc.newton_tol, dHat2, kappa[0], kappa[1], kappa[2], 0.0], np.float64).tofile(f)
-        for a in (sc.X, v, sc.X):
+        np.array([nV, nT, dbc.size, int(flow), int(b is not None), steps, int(zero_v), nE, nH], np.int32).tofile(f)
+        np.array([sc.h, sc.newton_tol, dHat2, kappa[0], kappa[1], kappa[2], 0.0, bend_k], np.float64).tofile(f)
+        for a in (X, v, X):
             np.ascontiguousarray(a, np.float64).tofile(f)
-        np.ascontiguousarray(sc.tets, np.int32).tofile(f)
+        np.ascontiguousarray(tets, np.int32).tofile(f)
         for a in (IB, vol, lam, mu, mass):
             np.ascontiguousarray(a, np.float64).tofile(f)
         if b is not None:
             np.ascontiguousarray(b, np.float64).tofile(f)
         dbc.tofile(f)
         np.ascontiguousarray(tgt, np.float64).tofile(f)
+        if shell is not None:
+            np.ascontiguousarray(tris, np.int32).tofile(f)
+            for a in (tvol, tlam, tmu, tIB):
+                np.ascontiguousarray(a, np.float64).tofile(f)
+            np.ascontiguousarray(hinges, np.int32).tofile(f)
+            np.ascontiguousarray(hrest, np.float64).tofile(f)
     return mass
 
 
@@ -100,3 +120,37 @@ def test_cpp_driver_flow_in_contact(tmp_path):
     assert iters[0] == it_o > 1
     assert uploads == 1
     assert err < 1e-6
+
+
+def test_cpp_driver_shell_hinge_draped(tmp_path):
+    """The Hinge variant on a shell: a sheet under gravity draped over a
+    sphere shell held by Dirichlet rows (node, x, y, z) at its rest
+    positions, membrane and hinge bending through ShellPart, two steps."""
+    from types import SimpleNamespace
+
+    rho, young, nu, thick, h = 100.0, 1e4, 0.3, 0.003, 0.01
+    X, T, dbc = scenes.draped_sheet()
+    tris, mass, tvol, lam, mu = capi.shell_init(X, T, rho, young, nu, thick)
+    hinges, hrest, tIB = capi.shell_hinges(X, tris)
+    bend_k = young * thick ** 3 / (24 * (1 - nu * nu))
+    dHat2, kappa = capi.initialize_oipc(1e-3, mass)
+    b = mass[:, None] * np.array([0.0, -9.81, 0.0])[None, :]
+    sc = SimpleNamespace(h=h, newton_tol=1e-3 * h)
+    v0 = np.zeros_like(X)
+    _write_input(tmp_path / "shell.in", sc, v0, b, False, 2, dHat2, kappa, dbc=dbc, tgt=X[dbc],
+                 shell=(X, tris, tvol, lam, mu, tIB, hinges, hrest, bend_k, mass))
+    iters, uploads, Xg, vg, k3 = _run(tmp_path, "shell", X.shape[0], 2)
+    m = pyoracle.ShellMesh(X, T, rho, thick, E=young, nu=nu, bend_mult=1.0)
+    m.set_dbc(dbc)
+    ko = kappa.copy()
+    Xo, vo, its = X.copy(), v0.copy(), []
+    for _ in range(2):
+        it, Xo, vo, ko, _ = m.advance_hinge(Xo, vo, h, 1e-3 * h, dHat2, ko, dbc_idx=dbc, body_force=b)
+        its.append(it)
+    diag = float(np.linalg.norm(X.max(0) - X.min(0)))
+    err = np.abs(Xg - Xo).max() / diag
+    say(f"[cpp] shell: newton {list(iters)} (oracle {its}), uploads {uploads}, err/diag {err:.2e}")
+    assert list(iters) == its
+    assert uploads == 1
+    assert err < 1e-6
+    assert np.abs(Xg[dbc] - X[dbc]).max() == 0.0
