@@ -48,7 +48,8 @@ EXPORTED = [
     "idp_get_launch_config", "idp_set_accd_max_steps", "idp_barrier_energy_noalloc", "idp_get_surface", "idp_device_memory", "idp_set_barrier_team",
     "idp_get_barrier_team", "idp_load_velocity", "idp_shell_init", "idp_shell_surface", "idp_set_mesh_shell",
     "idp_normal_flow_force", "idp_set_normal_flow", "idp_newton_direction_flow", "idp_shell_hinges",
-    "idp_set_shell_elasticity",
+    "idp_set_shell_elasticity", "idp_set_friction", "idp_friction_basis", "idp_get_friction", "idp_friction_energy",
+    "idp_friction_gradient", "idp_friction_hessian_local",
 ]
 
 # PCG preconditioner for every solve of this process: 0 cluster block Jacobi
@@ -106,7 +107,7 @@ def tuned_config(path=None):
 
 
 K_FAMILIES = ["pcg", "barrier_hess", "tet_hess", "assembly", "constraint_set", "ccd", "energy", "gradient",
-              "pcg_spmv", "pcg_update"]
+              "pcg_spmv", "pcg_update", "friction"]
 
 
 class IdpError(RuntimeError):
@@ -136,7 +137,8 @@ class StepStats(C.Structure):
         ("ms_total", C.c_double), ("ms_constraint_set", C.c_double), ("ms_ccd", C.c_double),
         ("ms_energy", C.c_double), ("ms_gradient", C.c_double), ("ms_hessian", C.c_double),
         ("ms_solve", C.c_double), ("dbc_alpha", C.c_double), ("dbc_stiff", C.c_double),
-        ("pcg_unconverged", C.c_int), ("accd_capped", C.c_long),
+        ("pcg_unconverged", C.c_int), ("accd_capped", C.c_long), ("n_friction", C.c_long),
+        ("friction_updates", C.c_int),
     ]
 
     def as_dict(self):
@@ -230,6 +232,13 @@ def lib():
         L.idp_shell_hinges.argtypes = [C.c_int, dp, C.c_int, ip, ip, ip, dp, dp]
         L.idp_normal_flow_force.argtypes = [vp, dp, C.c_double, dp]
         L.idp_set_normal_flow.argtypes = [vp, C.c_double]
+        L.idp_set_friction.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_int, ip, dp]
+        L.idp_friction_basis.argtypes = [vp, dp, dp, C.c_long, ip, C.c_double, C.c_double, C.c_double,
+                                         C.POINTER(C.c_long)]
+        L.idp_get_friction.argtypes = [vp, ip, dp, dp, dp, C.c_long]
+        L.idp_friction_energy.argtypes = [vp, dp, C.c_double, dp]
+        L.idp_friction_gradient.argtypes = [vp, dp, C.c_double, dp]
+        L.idp_friction_hessian_local.argtypes = [vp, dp, C.c_double, C.c_int, dp]
         L.idp_newton_direction_flow.argtypes = L.idp_newton_direction.argtypes
         _lib = L
     return _lib
@@ -488,6 +497,56 @@ class Context:
 
     def set_normal_flow(self, magnitude):
         self._check(lib().idp_set_normal_flow(self._h, magnitude))
+
+    def set_friction(self, mu, epsv2=1e-6, fric_iters=1, comp_range=None, mu_comp=None):
+        """Lagged friction of this context's steps (mu, epsv2, fricIterAmt,
+        compNodeRange, muComp of the step); mu 0 without components: off."""
+        if comp_range is None:
+            self._check(lib().idp_set_friction(self._h, mu, epsv2, int(fric_iters), 0, None, None))
+            return
+        cr, crp = _i(np.asarray(comp_range))
+        mc, mcp = _d(np.asarray(mu_comp, dtype=np.float64).ravel())
+        if mc.size != cr.size * cr.size:
+            raise ValueError("mu_comp must hold ncomp^2 coefficients")
+        self._check(lib().idp_set_friction(self._h, mu, epsv2, int(fric_iters), cr.size, crp, mcp))
+
+    def friction_basis(self, X, Xn, cs, dHat2, kappa, thickness=0.0):
+        """Compute_Friction_Basis at X on the constraints cs; Xn is kept as the
+        start positions of the friction stage calls.  Returns the count."""
+        X, Xp = _d(X)
+        xn, xnp = _d(Xn)
+        cs, csp = _i(cs)
+        nf = C.c_long()
+        self._check(lib().idp_friction_basis(self._h, Xp, xnp, cs.size // 4, csp, dHat2, kappa, thickness,
+                                             C.byref(nf)))
+        return nf.value
+
+    def get_friction(self, nf):
+        """(fcs (nf, 4), closest points (nf, 2), bases (nf, 6): column 0 then 1, normal forces)."""
+        fcs, fp = _out_i(4 * max(nf, 1))
+        cp, cpp = _out_d(2 * max(nf, 1))
+        b, bp = _out_d(6 * max(nf, 1))
+        N, Np = _out_d(max(nf, 1))
+        self._check(lib().idp_get_friction(self._h, fp, cpp, bp, Np, nf))
+        return fcs[: 4 * nf].reshape(nf, 4), cp[: 2 * nf].reshape(nf, 2), b[: 6 * nf].reshape(nf, 6), N[:nf]
+
+    def friction_energy(self, X, h):
+        X, Xp = _d(X)
+        E, Ep = _out_d(1)
+        self._check(lib().idp_friction_energy(self._h, Xp, h, Ep))
+        return float(E[0])
+
+    def friction_gradient(self, X, h):
+        X, Xp = _d(X)
+        g, gp = _out_d(X.size)
+        self._check(lib().idp_friction_gradient(self._h, Xp, h, gp))
+        return g
+
+    def friction_hessian_local(self, X, h, nf, project=True):
+        X, Xp = _d(X)
+        H, Hp = _out_d(144 * max(nf, 1))
+        self._check(lib().idp_friction_hessian_local(self._h, Xp, h, int(project), Hp))
+        return H[: 144 * nf].reshape(nf, 144)
 
     def set_mesh_batch(self, meshes):
         """meshes: list of (X_rest (V,3), tets (T,4) local ids, IB, vol, lam, mu, mass)."""

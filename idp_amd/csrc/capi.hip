@@ -25,6 +25,28 @@ using namespace idp;
 
 struct BatchEngine;
 
+// Lagged friction of a context (idp_set_friction) and the friction set of its
+// last basis update
+struct FricState {
+    double mu = 0.0, epsv2 = 0.0;
+    int iters = 1, ncomp = 0;
+    DBuf<int> comp_range;
+    DBuf<double> mu_comp;
+    DBuf<int4> fcs;              // friction constraints (int4 entries of the barrier set)
+    DBuf<double> fdat;           // 10 doubles each: closest points, tangent basis, normal force
+    DBuf<double> xn, part;       // x_n of the stage entry points; energy partials
+    DBuf<unsigned> flag, pos;
+    DBuf<int> err;
+    long n = 0;
+    bool have = false;           // a basis exists (stage entry points)
+    // while a step or a stage evaluation uses it: mu (1 with per-component
+    // coefficients), epsv h and x_n
+    bool on = false;
+    double mu_eff = 0.0, epsvh = 0.0;
+    const double* Xn = nullptr;
+    bool enabled() const { return mu > 0 || ncomp > 0; }
+};
+
 struct idp_ctx {
     int device = 0;
     hipStream_t s = nullptr;
@@ -81,6 +103,7 @@ struct idp_ctx {
     // this device's address of the fixed scratch array of the allocation-free
     // entry points (resolved once per device when the first context is created)
     unsigned char* scratch = nullptr;
+    FricState fr;
 };
 
 namespace {
@@ -262,6 +285,38 @@ void set_bparams1(idp_ctx* c, double dHat2, double kappa, double thickness)
 }
 
 // constraint block table from the per-instance constraint ranges
+// Compute_Friction_Basis and, with per-component coefficients,
+// Compute_Friction_Coef (Library/FEM/FRICTION.h:16-169) on the context's
+// current constraint set at X (device)
+void friction_basis(idp_ctx* c, const double* X, double dHat2, double kappa, double thickness)
+{
+    IDP_TIMER_FLAG("Compute_Friction_Basis");
+    auto s = c->s;
+    FricState& f = c->fr;
+    int* err = f.err.ensure(1, s);
+    IDP_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
+    const int tok = c->prof.begin(IDP_K_FRICTION);
+    f.n = fric_basis_update(s, c->L.tmp, c->cw.ncs, (const int*)c->cw.cs.p, X, dHat2 + 2 * std::sqrt(dHat2) * thickness,
+                            thickness * thickness, kappa, f.ncomp, f.comp_range.p, f.mu_comp.p, f.flag, f.pos, f.fcs,
+                            f.fdat, err);
+    // bytes: the constraint entries, flags and positions, the vertices read, the records written
+    c->prof.end(IDP_K_FRICTION, tok, 24.0 * c->cw.ncs + (96.0 + 96.0) * f.n);
+    int he = 0;
+    IDP_HIP(hipMemcpyAsync(&he, err, sizeof(int), hipMemcpyDeviceToHost, s));
+    IDP_HIP(hipStreamSynchronize(s));
+    if (he) throw std::invalid_argument("friction: a contact vertex lies in no component of comp_range");
+    f.have = true;
+}
+
+// the contact BSR structure of the current constraint set, followed by the
+// friction set while friction is on
+void build_contact_structure(idp_ctx* c)
+{
+    const bool fo = c->fr.on && c->fr.n > 0;
+    contact_structure(c->L, c->s, c->m.nV, c->cw.ncs, (const int*)c->cw.cs.p, fo ? c->fr.n : 0,
+                      fo ? (const int*)c->fr.fcs.p : nullptr);
+}
+
 void update_cseg(idp_ctx* c)
 {
     if (c->cw.h_cbeg.size() != (size_t)c->m.inst.B + 1) c->cw.h_cbeg = {0, c->cw.ncs};
@@ -299,7 +354,8 @@ void energies(idp_ctx* c, const double* X, bool with_barrier, double h, std::vec
     const bool bar = with_barrier && c->cw.ncs > 0;
     const bool flow = c->no_elastic && m.nE > 0;
     const bool shell = !c->no_elastic && m.nE > 0;  // membrane + bending
-    double* es = c->esum.ensure(5 * (size_t)B, s);
+    double* es = c->esum.ensure(6 * (size_t)B, s);
+    const bool fric = with_barrier && c->fr.on && c->fr.n > 0;  // single-mesh contexts only
     IDP_LAUNCH(k_seg_sums, B, RB, 0, s, I.tblk.pbeg.p, pe, es);
     IDP_LAUNCH(k_seg_sums, B, RB, 0, s, I.vblk.pbeg.p, pi, es + B);
     if (flow) {
@@ -320,14 +376,21 @@ void energies(idp_ctx* c, const double* X, bool with_barrier, double h, std::vec
                               bparams(c), pb, fl + B);
         IDP_LAUNCH(k_seg_sums, B, RB, 0, s, c->cseg.pbeg.p, pb, es + 2 * B);
     }
+    if (fric) {  // Compute_Friction_Potential (FRICTION.h:171-243)
+        const int nb = grid_for(c->fr.n, RB);
+        double* pf = c->fr.part.ensure(nb + 1, s);
+        launch_fric_energy(s, c->fr.n, c->fr.fcs.p, c->fr.fdat.p, X, c->fr.Xn, c->fr.epsvh, pf);
+        launch_sum_partials(s, nb, pf, es + 5 * B);
+    }
     c->prof.end(IDP_K_ENERGY, tok);
     std::vector<int> hf(2 * B);
-    std::vector<double> he(5 * B, 0.0);
+    std::vector<double> he(6 * B, 0.0);
     IDP_HIP(hipMemcpyAsync(hf.data(), fl, 2 * B * sizeof(int), hipMemcpyDeviceToHost, s));
     IDP_HIP(hipMemcpyAsync(he.data(), es, (bar ? 3 : 2) * B * sizeof(double), hipMemcpyDeviceToHost, s));
     if (flow || shell)
         IDP_HIP(hipMemcpyAsync(he.data() + 3 * B, es + 3 * B, (shell ? 2 : 1) * B * sizeof(double),
                                hipMemcpyDeviceToHost, s));
+    if (fric) IDP_HIP(hipMemcpyAsync(he.data() + 5 * B, es + 5 * B, sizeof(double), hipMemcpyDeviceToHost, s));
     IDP_HIP(hipStreamSynchronize(s));
     out.resize(B);
     for (int i = 0; i < B; ++i) {
@@ -335,6 +398,7 @@ void energies(idp_ctx* c, const double* X, bool with_barrier, double h, std::vec
         o.valid = hf[i] == 0;
         o.dist_ok = hf[B + i] == 0;
         o.E = o.valid ? ((he[i] + he[3 * B + i]) + he[4 * B + i]) + he[B + i] + (bar ? he[2 * B + i] : 0.0) : 0.0;
+        if (o.valid && fric) o.E += c->fr.mu_eff * he[5 * B + i];  // IMPLICIT_EULER.h:137-139
     }
 }
 
@@ -384,14 +448,19 @@ void gradient(idp_ctx* c, const double* X, double h, bool with_barrier, double* 
     }
     const double* gb = nullptr;
     long ncs = c->cw.ncs;
-    if (with_barrier && ncs) {
-        const int B = m.inst.B;
-        int* fl = c->flags.ensure(4 * (size_t)B + 4, s) + 2 * B;
-        IDP_HIP(hipMemsetAsync(fl, 0, B * sizeof(int), s));
-        double* b = c->gb.ensure(12 * (size_t)ncs, s);
-        BarrierParams bp = bparams(c);
-        bp.act = act;
-        launch_barrier_grad(s, ncs, (const int*)c->cw.cs.p, X, m.X0.p, bp, b, fl);
+    const long nfr = (with_barrier && c->fr.on) ? c->fr.n : 0;
+    if (with_barrier && (ncs || nfr)) {
+        double* b = c->gb.ensure(12 * (size_t)(ncs + nfr), s);
+        if (ncs) {
+            const int B = m.inst.B;
+            int* fl = c->flags.ensure(4 * (size_t)B + 4, s) + 2 * B;
+            IDP_HIP(hipMemsetAsync(fl, 0, B * sizeof(int), s));
+            BarrierParams bp = bparams(c);
+            bp.act = act;
+            launch_barrier_grad(s, ncs, (const int*)c->cw.cs.p, X, m.X0.p, bp, b, fl);
+        }
+        if (nfr)  // Compute_Friction_Gradient (FRICTION.h:245-383), after the barrier slots
+            launch_fric_grad(s, nfr, c->fr.fcs.p, c->fr.fdat.p, X, c->fr.Xn, c->fr.epsvh, c->fr.mu_eff, b + 12 * ncs);
         gb = b;
     }
     contact_gradient_gather(c->L, s, m, X, c->xt.p, gl, gb, g, rhs, act);
@@ -423,10 +492,20 @@ void hessian(idp_ctx* c, const double* X, double h, bool with_barrier, const uns
     c->prof.end(IDP_K_TET_HESS, tok, 6500.0 * (nT_act < 0 ? m.nT : nT_act));
     const double* Hb = nullptr;
     long ncs = c->cw.ncs;
+    const long nfr = (with_barrier && c->fr.on) ? c->fr.n : 0;
+    if (with_barrier && nfr) {  // Compute_Friction_Hessian (FRICTION.h:385-660), after the barrier slots
+        double* b = c->Hb.ensure(144 * (size_t)(ncs + nfr), s);
+        tok = c->prof.begin(IDP_K_FRICTION);
+        launch_fric_hess(s, nfr, c->fr.fcs.p, c->fr.fdat.p, X, c->fr.Xn, c->fr.epsvh, c->fr.mu_eff, 1, 1,
+                         b + 144 * ncs);
+        // bytes: entry + record, x and x_n of 4 vertices, 16 blocks (PT / EE bound)
+        c->prof.end(IDP_K_FRICTION, tok, (96.0 + 192.0 + 1152.0) * nfr);
+        Hb = b;
+    }
     if (with_barrier && ncs) {
         const int B = m.inst.B;
         int* fl = c->flags.ensure(4 * (size_t)B + 4, s) + 3 * B;
-        double* b = c->Hb.ensure(144 * (size_t)ncs, s);
+        double* b = c->Hb.ensure(144 * (size_t)(ncs + nfr), s);
         IDP_TIMER_FLAG("Compute_Barrier_Hessian");
         tok = c->prof.begin(IDP_K_BARRIER_HESS);
         BarrierParams bp = bparams(c);
@@ -1758,6 +1837,32 @@ int idp_normal_flow_force(idp_ctx* c, const double* X, double magnitude, double*
     });
 }
 
+int idp_set_friction(idp_ctx* c, double mu, double epsv2, int fric_iters, int ncomp, const int* comp_range,
+                     const double* mu_comp)
+{
+    return guarded(c, [&]() -> int {
+        if (!std::isfinite(mu) || mu < 0 || !std::isfinite(epsv2) || ncomp < 0 ||
+            (ncomp > 0 && (!comp_range || !mu_comp)))
+            return IDP_E_INVALID;
+        if ((mu > 0 || ncomp > 0) && !(epsv2 > 0)) return IDP_E_INVALID;
+        if ((mu > 0 || ncomp > 0) && c->have_mesh && c->m.inst.B != 1) return IDP_E_INVALID;
+        for (int i = 1; i < ncomp; ++i)
+            if (comp_range[i] < comp_range[i - 1]) return IDP_E_INVALID;
+        FricState& f = c->fr;
+        f.mu = mu;
+        f.epsv2 = epsv2;
+        f.iters = fric_iters;
+        f.ncomp = ncomp;
+        if (ncomp > 0) {
+            f.comp_range.upload(comp_range, ncomp, c->s);
+            f.mu_comp.upload(mu_comp, (size_t)ncomp * ncomp, c->s);
+            IDP_HIP(hipStreamSynchronize(c->s));
+        }
+        f.have = false;
+        return IDP_OK;
+    });
+}
+
 int idp_set_normal_flow(idp_ctx* c, double magnitude)
 {
     return guarded(c, [&]() -> int {
@@ -1914,9 +2019,13 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
                          double thickness, const double* b, int flow, const idp_solver_opts* opts,
                          idp_step_stats* st)
 {
-    struct ElasticOn {  // the flag never outlives the call, whatever path leaves it
+    struct ElasticOn {  // the flags never outlive the call, whatever path leaves it
         idp_ctx* c;
-        ~ElasticOn() { c->no_elastic = false; }
+        ~ElasticOn()
+        {
+            c->no_elastic = false;
+            c->fr.on = false;
+        }
     } restore_elastic{c};
     return guarded(c, [&]() -> int {
         if (!c->have_mesh || !kappa3 || !(h > 0)) return IDP_E_INVALID;
@@ -1960,6 +2069,16 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
             dev_axpy(s, n3, x, h, c->v.p, xt);
         double kappa = kappa3[0];
         set_bparams1(c, dHat2, kappa, thickness);
+        // lagged friction (IMPLICIT_EULER.h:413-437): mu becomes 1 with
+        // per-component coefficients; epsv h = sqrt(epsv2 h h); x_n = the start
+        FricState& fr = c->fr;
+        const bool fric = coll && fr.enabled();
+        if (fric) {
+            fr.mu_eff = fr.ncomp > 0 ? 1.0 : fr.mu;
+            fr.epsvh = std::sqrt(fr.epsv2 * h * h);
+            fr.Xn = xn;
+        }
+        int fricIterI = 0;
 
         // ---- Dirichlet nodes toward their targets (IMPLICIT_EULER.h:304-401): the
         // CCD-limited fraction of the move; all of it when that is 1, otherwise
@@ -2043,6 +2162,10 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
             S.ms_constraint_set += now_ms() - ta;
         };
         if (coll) cset(x);
+        if (fric) {
+            friction_basis(c, x, dHat2, kappa, thickness);
+            fr.on = true;
+        }
         t0 = now_ms();
         EnergyOut e0 = energy(c, x, coll, h);
         S.ms_energy += now_ms() - t0;
@@ -2059,7 +2182,7 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
         do {
             // ---- gradient and Hessian at x with the current constraint set
             t0 = now_ms();
-            if (coll) contact_structure(c->L, s, m.nV, c->cw.ncs, (const int*)c->cw.cs.p);
+            if (coll) build_contact_structure(c);
             else c->L.ncb = 0;
             gradient(c, x, h, coll, g, rhs);
             if (DBCStiff)
@@ -2228,6 +2351,44 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
                     Eprev = energy(c, x, coll, h).E;
                 }
             }
+            // ---- lagged friction update at convergence (IMPLICIT_EULER.h:757-848):
+            // a new basis, then one more direction whose size decides the exit
+            if (res.size() == 3 && L2 <= newton_tol) {
+                ++fricIterI;
+                if ((fricIterI < fr.iters || fr.iters <= 0) && fric) {
+                    friction_basis(c, x, dHat2, kappa, thickness);
+                    build_contact_structure(c);
+                    gradient(c, x, h, coll, g, rhs);
+                    if (DBCStiff)
+                        IDP_LAUNCH(k_dbc_grad, grid_for(nd, 256), 256, 0, s, nd, c->dbc_idx_d.p, c->dbc_tgt_d.p,
+                                   m.mass.p, m.proj, x, DBCStiff, g, rhs);
+                    hessian(c, x, h, coll);
+                    if (DBCStiff)
+                        IDP_LAUNCH(k_dbc_hess, grid_for(nd, 256), 256, 0, s, nd, c->dbc_idx_d.p, m.mass.p, m.proj,
+                                   m.e_diag.p, m.nnzb_e, DBCStiff, c->L.evals.p);
+                    double rel = 0;
+                    const unsigned char on = 1;
+                    int it;
+                    {
+                        IDP_TIMER_FLAG("linearSolve");
+                        it = pcg_solve_persistent(c->L, s, m, rhs, sol, rtol, maxit, &rel,
+                                                  cold ? nullptr : c->one.upload_get(&on, 1, s));
+                    }
+                    if (it < 0) {  // the reference exits when this solve fails (IMPLICIT_EULER.h:818-825)
+                        c->err = "linear solve failed after a friction basis update";
+                        if (st) *st = S;
+                        return IDP_E_NOT_SPD;
+                    }
+                    S.pcg_iters += it;
+                    S.pcg_unconverged += c->L.h_maxit[0];
+                    L2 = std::sqrt(dev_dot(c->L, s, n3, sol, sol) / (m.nV - c->ndbc)) / h;
+                    if (L2 > newton_tol) {
+                        Eprev = energy(c, x, coll, h).E;
+                        if (DBCStiff) dbc_energy(x, Eprev);
+                    }
+                    S.friction_updates++;
+                }
+            }
             S.dbc_stiff = DBCStiff;
             S.newton_iters = PNIter;
             if (verbose > 0 && PNIter % verbose == 0)
@@ -2246,6 +2407,7 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
         dev_velocity(s, n3, x, xn, h, c->v.p);
         IDP_HIP(hipStreamSynchronize(s));
         S.n_constraints = c->cw.ncs;
+        S.n_friction = fric ? fr.n : 0;
         S.kappa = kappa;
         c->prof.collect();
         S.ms_total = now_ms() - t_start;
@@ -2398,6 +2560,118 @@ int idp_barrier_hessian_local(idp_ctx* c, const double* X, long n, const int* cs
     });
 }
 
+// friction stage entry points: the set of idp_friction_basis with its x_n,
+// the context's coefficients, epsv h from the given step
+namespace {
+int friction_stage(idp_ctx* c, double h, const double* X, const double** xd)
+{
+    FricState& f = c->fr;
+    if (!c->have_mesh || !f.have || !f.enabled() || !(h > 0)) return IDP_E_INVALID;
+    *xd = stage_x(c, X);
+    f.mu_eff = f.ncomp > 0 ? 1.0 : f.mu;
+    f.epsvh = std::sqrt(f.epsv2 * h * h);
+    f.Xn = f.xn.p;
+    return IDP_OK;
+}
+}  // namespace
+
+int idp_friction_basis(idp_ctx* c, const double* X, const double* Xn, long n, const int* cs, double dHat2,
+                       double kappa, double thickness, long* nf)
+{
+    return guarded(c, [&]() -> int {
+        if (!c->have_mesh || !Xn || n < 0 || (n > 0 && !cs) || c->m.inst.B != 1) return IDP_E_INVALID;
+        if (!c->fr.enabled()) return IDP_E_INVALID;
+        IDP_HIP(hipSetDevice(c->device));
+        const double* xd = stage_x(c, X);
+        set_constraints_from_host(c, n, cs);
+        c->fr.xn.upload(Xn, 3 * (size_t)c->m.nV, c->s);
+        friction_basis(c, xd, dHat2, kappa, thickness);
+        if (nf) *nf = c->fr.n;
+        return IDP_OK;
+    });
+}
+
+int idp_get_friction(idp_ctx* c, int* fcs, double* cp, double* basis, double* nforce, long cap)
+{
+    return guarded(c, [&]() -> int {
+        FricState& f = c->fr;
+        if (!f.have) return IDP_E_INVALID;
+        if (cap < f.n) return IDP_E_CAPACITY;
+        auto s = c->s;
+        std::vector<double> d(10 * (size_t)f.n);
+        if (f.n) {
+            if (fcs) IDP_HIP(hipMemcpyAsync(fcs, f.fcs.p, f.n * sizeof(int4), hipMemcpyDeviceToHost, s));
+            IDP_HIP(hipMemcpyAsync(d.data(), f.fdat.p, d.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        }
+        IDP_HIP(hipStreamSynchronize(s));
+        for (long i = 0; i < f.n; ++i) {
+            if (cp)
+                for (int k = 0; k < 2; ++k) cp[2 * i + k] = d[10 * i + k];
+            if (basis)
+                for (int k = 0; k < 6; ++k) basis[6 * i + k] = d[10 * i + 2 + k];
+            if (nforce) nforce[i] = d[10 * i + 8];
+        }
+        return IDP_OK;
+    });
+}
+
+int idp_friction_energy(idp_ctx* c, const double* X, double h, double* E)
+{
+    return guarded(c, [&]() -> int {
+        const double* xd = nullptr;
+        int rc = friction_stage(c, h, X, &xd);
+        if (rc != IDP_OK || !E) return rc != IDP_OK ? rc : IDP_E_INVALID;
+        FricState& f = c->fr;
+        auto s = c->s;
+        double S = 0.0;
+        if (f.n) {
+            const int nb = grid_for(f.n, RB);
+            double* pf = f.part.ensure(nb + 1, s);
+            launch_fric_energy(s, f.n, f.fcs.p, f.fdat.p, xd, f.Xn, f.epsvh, pf);
+            S = dev_sum_partials(c->L, s, nb, pf);
+        }
+        *E = f.mu_eff * S;
+        return IDP_OK;
+    });
+}
+
+int idp_friction_gradient(idp_ctx* c, const double* X, double h, double* g)
+{
+    return guarded(c, [&]() -> int {
+        const double* xd = nullptr;
+        int rc = friction_stage(c, h, X, &xd);
+        if (rc != IDP_OK || !g) return rc != IDP_OK ? rc : IDP_E_INVALID;
+        FricState& f = c->fr;
+        auto s = c->s;
+        contact_structure(c->L, s, c->m.nV, 0, nullptr, f.n, (const int*)f.fcs.p);
+        double* b = c->gb.ensure(12 * (size_t)std::max(f.n, 1L), s);
+        if (f.n) launch_fric_grad(s, f.n, f.fcs.p, f.fdat.p, xd, f.Xn, f.epsvh, f.mu_eff, b);
+        double* gd = c->g.ensure(3 * (size_t)c->m.nV, s);
+        contact_gradient_gather(c->L, s, c->m, xd, nullptr, nullptr, f.n ? b : nullptr, gd, nullptr);
+        c->g.download(g, 3 * (size_t)c->m.nV, s);
+        IDP_HIP(hipStreamSynchronize(s));
+        return IDP_OK;
+    });
+}
+
+int idp_friction_hessian_local(idp_ctx* c, const double* X, double h, int project, double* H)
+{
+    return guarded(c, [&]() -> int {
+        const double* xd = nullptr;
+        int rc = friction_stage(c, h, X, &xd);
+        if (rc != IDP_OK || !H) return rc != IDP_OK ? rc : IDP_E_INVALID;
+        FricState& f = c->fr;
+        auto s = c->s;
+        if (f.n) {
+            double* b = c->Hb.ensure(144 * (size_t)f.n, s);
+            launch_fric_hess(s, f.n, f.fcs.p, f.fdat.p, xd, f.Xn, f.epsvh, f.mu_eff, project, 0, b);
+            IDP_HIP(hipMemcpyAsync(H, b, 144 * (size_t)f.n * sizeof(double), hipMemcpyDeviceToHost, s));
+        }
+        IDP_HIP(hipStreamSynchronize(s));
+        return IDP_OK;
+    });
+}
+
 int idp_min_dist2(idp_ctx* c, const double* X, long n, const int* cs, double thickness, double* dist2,
                   double* min_dist2)
 {
@@ -2518,7 +2792,15 @@ int idp_newton_direction(idp_ctx* c, const double* X, const double* xtilde, doub
         c->xt.upload(xtilde, n3, s);
         contact_constraint_set(c->cw, s, m, xd, &dHat2, thickness);
         update_cseg(c);
-        contact_structure(c->L, s, m.nV, c->cw.ncs, (const int*)c->cw.cs.p);
+        struct FricOff {
+            idp_ctx* c;
+            ~FricOff() { c->fr.on = false; }
+        } fric_off{c};
+        if (c->fr.have && c->fr.enabled()) {  // the set and x_n of idp_friction_basis
+            const double* unused = nullptr;
+            if (friction_stage(c, h, nullptr, &unused) == IDP_OK) c->fr.on = true;
+        }
+        build_contact_structure(c);
         double* gd = c->g.ensure(n3, s);
         double* rhs = c->rhs.ensure(n3, s);
         double* sd = c->sol.ensure(n3, s);
@@ -2643,6 +2925,10 @@ int idp_advance_batch(idp_ctx* c, double h, const double* newton_tol, int with_c
 {
     return guarded(c, [&]() -> int {
         if (!c->have_mesh || !kappa3 || !newton_tol || !dHat2 || !newton_iters || !(h > 0)) return IDP_E_INVALID;
+        if (c->fr.enabled()) {
+            c->err = "friction is stepped one mesh at a time (idp_advance_one_step)";
+            return IDP_E_INVALID;
+        }
         IDP_HIP(hipSetDevice(c->device));
         const int B = c->m.inst.B;
         BatchEngine& E = *c->eng;
@@ -2682,6 +2968,10 @@ int idp_serve_configure(idp_ctx* c, double h, int with_collision, double thickne
 {
     return guarded(c, [&]() -> int {
         if (!c->have_mesh || !(h > 0)) return IDP_E_INVALID;
+        if (c->fr.enabled()) {
+            c->err = "friction is stepped one mesh at a time (idp_advance_one_step)";
+            return IDP_E_INVALID;
+        }
         return c->eng->configure(c, h, with_collision != 0, thickness, opts);
     });
 }

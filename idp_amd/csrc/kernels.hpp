@@ -247,4 +247,21 @@ void launch_shell_hess(hipStream_t s, const MeshDev& m, const double* X, double 
 void launch_inertia_energy(hipStream_t s, const SegDev& seg, const double* X, const double* Xt, const double* mass,
                            double* part);
 
+// friction.hip: lagged friction.  fric_basis_update compacts the
+// non-mollified constraints of cs into fcs with their 10-double records
+// (closest points 2, tangent basis 6, normal force, pad) and returns their
+// count; err[0] = 1 when a vertex is in no component.  The energy writes
+// per-block partial sums over grid_for(nf, RB) blocks (mu not applied); the
+// gradient 12 doubles and the Hessian 144 doubles per constraint (mode 1:
+// assembly blocks, mode 0: dense row-major).
+long fric_basis_update(hipStream_t s, DBuf<unsigned char>& tmp, long n, const int* cs, const double* X, double dh,
+                       double th2, double kappa, int ncomp, const int* comp_range, const double* mu_comp,
+                       DBuf<unsigned>& flag, DBuf<unsigned>& pos, DBuf<int4>& fcs, DBuf<double>& fdat, int* err);
+void launch_fric_energy(hipStream_t s, long nf, const int4* fcs, const double* fdat, const double* X,
+                        const double* Xn, double epsvh, double* part);
+void launch_fric_grad(hipStream_t s, long nf, const int4* fcs, const double* fdat, const double* X,
+                      const double* Xn, double epsvh, double mu, double* gout);
+void launch_fric_hess(hipStream_t s, long nf, const int4* fcs, const double* fdat, const double* X,
+                      const double* Xn, double epsvh, double mu, int project, int mode, double* out);
+
 }  // namespace idp

@@ -1198,14 +1198,17 @@ int bits_for(unsigned long long maxv)
 }
 }  // namespace
 
-void contact_structure(LinWork& L, hipStream_t s, int nV, long ncs, const int* cs)
+void contact_structure(LinWork& L, hipStream_t s, int nV, long ncs0, const int* cs, long nf, const int* fcs)
 {
     L.ncb = 0;
     L.nrec = 0;
+    const long ncs = ncs0 + nf;
     if (ncs == 0) return;
+    if (ncs >= (1L << 24)) throw std::length_error("contact structure: more than 2^24 stencils");
     int* vid = L.vid.ensure(4 * (size_t)ncs, s);
     int* nv = L.nv.ensure(ncs, s);
-    launch_constraint_vids(s, ncs, cs, vid, nv);
+    if (ncs0) launch_constraint_vids(s, ncs0, cs, vid, nv);
+    if (nf) launch_constraint_vids(s, nf, fcs, vid + 4 * ncs0, nv + ncs0);
     unsigned* cnt = L.u0.ensure(ncs + 1, s);
     unsigned* off = L.u1.ensure(ncs + 1, s);
     IDP_LAUNCH(k_contact_nv2, grid_for(ncs, 256), 256, 0, s, ncs, nv, cnt);

@@ -36,7 +36,11 @@ struct LinWork {
     Profiler* prof = nullptr;
 };
 
-void contact_structure(LinWork& L, hipStream_t s, int nV, long ncs, const int* cs);
+// the contact blocks of the ncs barrier constraints cs followed by nf more
+// stencils in the same int4 encoding (the friction set); local terms of entry
+// i are at gb + 12 i and Hb + 144 i over both lists
+void contact_structure(LinWork& L, hipStream_t s, int nV, long ncs, const int* cs, long nf = 0,
+                       const int* fcs = nullptr);
 // act (device, one byte per instance; null = all): the instances whose rows are formed
 void contact_gradient_gather(LinWork& L, hipStream_t s, const MeshDev& m, const double* X, const double* Xt,
                              const double* gl, const double* gb, double* g, double* rhs,

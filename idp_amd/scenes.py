@@ -222,6 +222,26 @@ def two_blocks_slide(n=2, spacing=0.05, gap=2e-4, slide=1.0):
     return Scene(f"two_blocks{n}_slide", X.copy(), T, X.copy()), p
 
 
+def block_on_slab(n=2, spacing=0.05, gap_dhat=0.5, E=1e5, h=0.01, offset=0.5):
+    """An n^3 Kuhn block resting on a (4n, 1, 2n)-cell slab, `gap_dhat` dhat
+    above it, for friction: the slab's vertices are the Dirichlet nodes
+    (returned as the third value), the block starts at rest, and a caller
+    tilts gravity to make it slide or stick.  offset (cells, in x and z)
+    shifts the block off the slab's grid so that its bottom vertices sit
+    over slab triangles (point-triangle and edge-edge contacts) rather than
+    over slab vertices (point-point contacts).  Returns (scene, slab vertex
+    count, Dirichlet ids)."""
+    Xs, Ts = kuhn_block(4 * n, 1, 2 * n, spacing)
+    Xb, Tb = kuhn_block(n, n, n, spacing)
+    sc0 = Scene("probe", np.vstack([Xs, Xb]), Ts, np.vstack([Xs, Xb]))
+    dhat = sc0.dhat_rel * float(np.linalg.norm(np.array([4 * n, 1 + n, 2 * n]) * spacing))
+    Xb = Xb + np.array([(1.5 * n + offset) * spacing, spacing + gap_dhat * dhat, (0.5 * n + offset) * spacing])
+    X = np.vstack([Xs, Xb])
+    T = np.vstack([Ts, Tb + Xs.shape[0]]).astype(np.int32)
+    sc = Scene(f"block{n}_on_slab", X.copy(), T, X.copy(), h=h, E=E)
+    return sc, Xs.shape[0], np.arange(Xs.shape[0], dtype=np.int32)
+
+
 # ---------------------------------------------------------------------------
 # Shell scenes for the normal-flow variant (Projects/FEMShell/12-14_normal_flow.py:
 # dt 1 scaled by the smoothing intensity 0.5, normal-flow magnitude 0.005 /

@@ -84,6 +84,11 @@ typedef struct idp_step_stats {
     /* ACCD pairs stopped by the step cap (accd_max_steps) and taken as hits at
        the time of impact they had reached */
     long accd_capped;
+    /* lagged friction: constraints in the friction set of the last basis
+       update, and the basis updates after the step's first
+       (IMPLICIT_EULER.h:757-848) */
+    long n_friction;
+    int friction_updates;
 } idp_step_stats;
 
 /* Kernel-family profile: device time from HIP events recorded on the
@@ -100,6 +105,7 @@ enum {
     IDP_K_GRADIENT = 7,     /* tet + barrier gradient and vertex gather */
     IDP_K_PCG_SPMV = 8,     /* PCG SpMV kernel, sampled on one iteration in 8 (bytes) */
     IDP_K_PCG_UPDATE = 9,   /* PCG vector-update kernel, sampled likewise (bytes) */
+    IDP_K_FRICTION = 10,    /* friction basis update and Hessian kernels (bytes) */
     IDP_K_COUNT = 16
 };
 typedef struct idp_profile {
@@ -230,6 +236,23 @@ int idp_normal_flow_force(idp_ctx* ctx, const double* X, double magnitude, doubl
  * Update_Normal_Flow_Neumann before each step (FEMDiscreteShellBase.py:269-273);
  * magnitude 0 turns it off (the default). */
 int idp_set_normal_flow(idp_ctx* ctx, double magnitude);
+/* Lagged IPC friction of this context's steps: the mu, epsv2, fricIterAmt,
+ * compNodeRange and muComp arguments of Advance_One_Step_IE_Discrete_Shell
+ * (Library/FEM/Shell/IMPLICIT_EULER.h:166-169; the friction terms are
+ * Library/FEM/FRICTION.h:16-660).  At the step's start, and again each time
+ * the Newton loop meets its tolerance while fewer than fric_iters bases have
+ * been made (fric_iters <= 0: no limit), the friction set is rebuilt from
+ * the barrier set: the non-mollified constraints with their closest points,
+ * tangent bases and normal forces; the potential mu sum N f0(|T^T dx|) with
+ * epsv h as the static bound then enters the energy, gradient and Hessian.
+ * ncomp > 0: comp_range (ncomp ascending vertex bounds: vertex v is in the
+ * first component c with v < comp_range[c]) and mu_comp (ncomp^2,
+ * mu_comp[c0 + c1 ncomp]) scale each normal force and mu is taken as 1
+ * (Compute_Friction_Coef, FRICTION.h:119-169).  mu 0 and ncomp 0 turn
+ * friction off (the default).  Steps of one mesh (idp_advance_one_step);
+ * idp_advance_batch and serving refuse a context with friction on. */
+int idp_set_friction(idp_ctx* ctx, double mu, double epsv2, int fric_iters, int ncomp, const int* comp_range,
+                     const double* mu_comp);
 /* DBC vertex list (VECTOR_STORAGE DBC of the reference, fixed targets = current x). */
 int idp_set_dbc(idp_ctx* ctx, int n, const int* idx);
 /* Moving Dirichlet nodes: node idx[i] is driven to targets[3 i .. 3 i + 2]
@@ -306,6 +329,24 @@ int idp_barrier_gradient(idp_ctx* ctx, const double* X, long n, const int* cs, d
  * when project != 0 (makePD, Library/Math/UTILS.h:9-27). */
 int idp_barrier_hessian_local(idp_ctx* ctx, const double* X, long n, const int* cs, double dHat2, double kappa,
                               double thickness, int project, double* H, int* vid);
+/* Friction stage entry points (FRICTION.h), on the context's friction
+ * coefficients (idp_set_friction).  idp_friction_basis: Compute_Friction_Basis
+ * (and Compute_Friction_Coef) at X (host, 3 nV; null: the context's x) on the
+ * n barrier constraints cs; Xn (host, 3 nV) is kept as the start positions of
+ * the later calls; *nf = friction constraints.  idp_get_friction: their int4
+ * entries, closest-point parameters (2 each), tangent bases (6 each: column
+ * 0, column 1) and normal forces (cap >= nf).  The energy (mu sum N f0, with
+ * epsv h the static bound: h is the step), the gradient (3 nV, summed per
+ * vertex) and the local Hessians (144 per constraint: the dense (3 nv)^2
+ * row-major matrix, zero padded) are evaluated at X for that set.  After
+ * idp_friction_basis, idp_newton_direction includes the friction terms of
+ * that set with its Xn. */
+int idp_friction_basis(idp_ctx* ctx, const double* X, const double* Xn, long n, const int* cs, double dHat2,
+                       double kappa, double thickness, long* nf);
+int idp_get_friction(idp_ctx* ctx, int* fcs, double* cp, double* basis, double* nforce, long cap);
+int idp_friction_energy(idp_ctx* ctx, const double* X, double h, double* E);
+int idp_friction_gradient(idp_ctx* ctx, const double* X, double h, double* g);
+int idp_friction_hessian_local(idp_ctx* ctx, const double* X, double h, int project, double* H);
 /* Compute_Min_Dist2 (IPC.h:2246-2388) */
 int idp_min_dist2(idp_ctx* ctx, const double* X, long n, const int* cs, double thickness, double* dist2,
                   double* min_dist2);

@@ -4,6 +4,8 @@
 //   Line_Search                         Library/FEM/Shell/IMPLICIT_EULER.h:10-149
 //   Compute_IncPotential{,_Gradient,_Hessian}  Library/FEM/Shell/INC_POTENTIAL.h:15-395
 //   CSR assembly + DBC projection       Library/Math/CSR_MATRIX.h:49-56, 130-141
+//   lagged friction                     Library/FEM/Shell/IMPLICIT_EULER.h:413-437,
+//                                       463-465, 757-848 (friction.cpp)
 // The linear solve restates the direct solve (DIRECT_SOLVER.h:14-89) with a
 // block-Jacobi preconditioned CG run to a tight relative residual (1e-12 by
 // default); the direction therefore agrees with a direct factorization to
@@ -197,6 +199,18 @@ struct Ctx {
     std::vector<double> Xtilde;
     std::vector<int> cs;
     long ncs = 0;
+    // lagged friction (mu > 0 or per-component coefficients): the set, its
+    // closest points, bases and normal forces at the last basis update, and
+    // the step's start positions Xn
+    bool fric = false;
+    double fmu = 0.0, epsvh2 = 0.0;
+    int ncomp = 0;
+    const int* comp_range = nullptr;
+    const double* mu_comp = nullptr;
+    const double* Xn = nullptr;
+    std::vector<int> fcs;
+    std::vector<double> fcp, fbasis, fnf;
+    long nfc = 0;
     orc_stats* st;
     double t_cs = 0, t_ccd = 0, t_solve = 0, t_hess = 0, t_grad = 0, t_E = 0;
 };
@@ -249,6 +263,37 @@ void dbc_energy(const Ctx& c, const double* X, double& E)
     double s = 0.0;
     for (double p : pen) s += p;
     E += 0.5 * c.dbc_stiff * s;
+}
+
+extern "C" long orc_friction_basis(const double* X, long n, const int* cs, double dHat2, double kappa,
+                                   double thickness, int* fcs, double* cp, double* basis, double* nforce);
+extern "C" int orc_friction_coef(long nf, const int* fcs, int ncomp, const int* comp_range, const double* mu_comp,
+                                 double* nforce);
+extern "C" double orc_friction_energy(const double* X, const double* Xn, long nf, const int* fcs, const double* cp,
+                                      const double* basis, const double* nforce, double epsvh2, double mu);
+extern "C" void orc_friction_gradient(const double* X, const double* Xn, long nf, const int* fcs, const double* cp,
+                                      const double* basis, const double* nforce, double epsvh2, double mu, double* g);
+extern "C" void orc_friction_hessian_local(const double* X, const double* Xn, long nf, const int* fcs,
+                                           const double* cp, const double* basis, const double* nforce,
+                                           double epsvh2, double mu, int projectSPD, double* H, int* vid);
+
+// Compute_Friction_Basis (+ Compute_Friction_Coef with per-component
+// coefficients, which sets mu to 1) on the current constraint set at X
+// (IMPLICIT_EULER.h:421-426, 763-768); false when a vertex is in no component
+bool friction_basis(Ctx& c, const double* X)
+{
+    const long n = c.ncs;
+    c.fcs.resize(4 * (size_t)n + 4);
+    c.fcp.resize(2 * (size_t)n + 2);
+    c.fbasis.resize(6 * (size_t)n + 6);
+    c.fnf.resize((size_t)n + 1);
+    c.nfc = orc_friction_basis(X, n, c.cs.data(), c.dHat2, c.kappa[0], c.thickness, c.fcs.data(), c.fcp.data(),
+                               c.fbasis.data(), c.fnf.data());
+    if (c.ncomp > 0) {
+        if (orc_friction_coef(c.nfc, c.fcs.data(), c.ncomp, c.comp_range, c.mu_comp, c.fnf.data())) return false;
+        c.fmu = 1.0;
+    }
+    return true;
 }
 
 extern "C" void orc_shell_elastic_terms(int nV, const double* X, int nE, const int* tris, const double* IB,
@@ -306,6 +351,9 @@ bool energy(Ctx& c, const double* X, bool withBarrier, double& E)
             return false;
         E += Eb;
     }
+    if (withBarrier && c.fric && c.fmu > 0)  // IMPLICIT_EULER.h:137-139: E += mu sum N f0
+        E += orc_friction_energy(X, c.Xn, c.nfc, c.fcs.data(), c.fcp.data(), c.fbasis.data(), c.fnf.data(), c.epsvh2,
+                                 c.fmu);
     c.t_E += now() - t0;
     return true;
 }
@@ -330,6 +378,17 @@ void assemble_sys(Ctx& c, const double* X, Sys& S)
         orc_barrier_hessian_local(X, c.X0, c.ncs, c.cs.data(), c.dHat2, c.kappa[0], c.thickness, 1, Hl.data(),
                                   vl.data());
         for (long i = 0; i < c.ncs; ++i) {
+            int nv = 0;
+            while (nv < 4 && vl[4 * i + nv] >= 0) ++nv;
+            add_local(blks, nv, vl.data() + 4 * i, Hl.data() + 144 * i, nV);
+        }
+    }
+    if (c.withCollision && c.fric && c.fmu > 0 && c.nfc) {  // INC_POTENTIAL.h:375-377
+        std::vector<double> Hl(144 * c.nfc);
+        std::vector<int> vl(4 * c.nfc);
+        orc_friction_hessian_local(X, c.Xn, c.nfc, c.fcs.data(), c.fcp.data(), c.fbasis.data(), c.fnf.data(),
+                                   c.epsvh2, c.fmu, 1, Hl.data(), vl.data());
+        for (long i = 0; i < c.nfc; ++i) {
             int nv = 0;
             while (nv < 4 && vl[4 * i + nv] >= 0) ++nv;
             add_local(blks, nv, vl.data() + 4 * i, Hl.data() + 144 * i, nV);
@@ -397,12 +456,22 @@ struct ShellElastic {
     double k;
 };
 
+// friction arguments of the step: mu, epsv2, fricIterAmt, compNodeRange, muComp
+struct FrictionArgs {
+    double mu, epsv2;
+    int iters;
+    int ncomp;
+    const int* comp_range;
+    const double* mu_comp;
+};
+
 int advance_impl(int nV, double* X, const double* X0, double* vel, const double* mass, int nT, const int* tets,
                  const double* IB, const double* vol, const double* lam, const double* mu, int nE, const int* tris,
                  const double* tvol, int nb, const int* bnode, int ne, const int* bedge, int nf, const int* btri,
                  int nDBC, const int* dbc_idx, const double* dbc_target, double h, double NewtonTol,
                  int withCollision, double dHat2, double* kappa, double thickness, double lin_rtol, int max_newton,
-                 const double* body_force, int flow, orc_stats* st, const ShellElastic* se = nullptr)
+                 const double* body_force, int flow, orc_stats* st, const ShellElastic* se = nullptr,
+                 const FrictionArgs* fa = nullptr)
 {
     Ctx c;
     c.nE = nE;
@@ -428,6 +497,15 @@ int advance_impl(int nV, double* X, const double* X0, double* vel, const double*
     if (st) std::memset(st, 0, sizeof(*st));
     const size_t n3 = 3 * (size_t)nV;
     std::vector<double> Xn(X, X + n3);
+    c.Xn = Xn.data();
+    if (fa && c.withCollision && (fa->mu > 0 || fa->ncomp > 0)) {
+        c.fric = true;
+        c.fmu = fa->mu;
+        c.epsvh2 = fa->epsv2 * h * h;
+        c.ncomp = fa->ncomp;
+        c.comp_range = fa->comp_range;
+        c.mu_comp = fa->mu_comp;
+    }
     c.Xtilde.resize(n3);
     // IMPLICIT_EULER.h:196-217: a = M^-1 b with the lumped (diagonal) mass,
     // the flow variant zeroes v, x~ = x + (h v + h^2 a)
@@ -469,6 +547,7 @@ int advance_impl(int nV, double* X, const double* X0, double* vel, const double*
     if (st) st->dbc_alpha = DBCAlpha;
 
     if (c.withCollision) compute_cs(c, X);
+    if (c.fric && !friction_basis(c, X)) return ORC_E_INVALID_START;
     double Eprev;
     if (!energy(c, X, c.withCollision, Eprev)) return ORC_E_INVALID_START;
     if (c.dbc_stiff) dbc_energy(c, X, Eprev);
@@ -477,11 +556,11 @@ int advance_impl(int nV, double* X, const double* X0, double* vel, const double*
     std::vector<int> cs_prev;
     long ncs_prev = 0;
     bool useGD = false;
-    int PNIter = 0;
+    int PNIter = 0, fricIterI = 0;
     double L2Norm = 0;
-    do {
-        // gradient (INC_POTENTIAL.h:154-275)
-        double t0 = now();
+    // gradient (INC_POTENTIAL.h:154-275), the friction term, DBC rows zeroed
+    // (IMPLICIT_EULER.h:456-491); rhs = -g
+    auto gradient_at = [&]() {
         std::fill(g.begin(), g.end(), 0.0);
         if (!c.flow) orc_elastic_gradient(X, nT, tets, IB, vol, lam, mu, h * h, g.data());
         if (!c.flow && c.nE && c.tIB)
@@ -496,6 +575,9 @@ int advance_impl(int nV, double* X, const double* X0, double* vel, const double*
             for (int d = 0; d < 3; ++d) g[3 * v + d] += mass[v] * (X[3 * v + d] - c.Xtilde[3 * v + d]);
         if (c.withCollision && c.ncs)
             orc_barrier_gradient(X, X0, c.ncs, c.cs.data(), dHat2, kappa[0], thickness, nV, g.data());
+        if (c.fric && c.fmu > 0)
+            orc_friction_gradient(X, c.Xn, c.nfc, c.fcs.data(), c.fcp.data(), c.fbasis.data(), c.fnf.data(),
+                                  c.epsvh2, c.fmu, g.data());
         if (c.dbc_stiff) {  // Shell/DIRICHLET.h:50-70, then the fixed nodes' rows (IMPLICIT_EULER.h:467-474)
             for (int i = 0; i < nDBC; ++i) {
                 const int v = dbc_idx[i];
@@ -508,6 +590,10 @@ int advance_impl(int nV, double* X, const double* X0, double* vel, const double*
                 if (c.dbc[v]) g[3 * v] = g[3 * v + 1] = g[3 * v + 2] = 0;
         }
         for (size_t i = 0; i < n3; ++i) rhs[i] = -g[i];
+    };
+    do {
+        double t0 = now();
+        gradient_at();
         c.t_grad += now() - t0;
 
         if (!useGD) {
@@ -631,9 +717,33 @@ int advance_impl(int nV, double* X, const double* X0, double* vel, const double*
                 energy(c, X, c.withCollision, Eprev);
             }
         }
+        // lagged friction update at convergence (IMPLICIT_EULER.h:757-848): a
+        // new basis, then one more direction whose size decides the exit
+        if (resRecord.size() == 3 && L2Norm <= NewtonTol) {
+            ++fricIterI;
+            if ((fricIterI < (fa ? fa->iters : 1) || (fa && fa->iters <= 0)) && c.fric) {
+                if (!friction_basis(c, X)) return ORC_E_INVALID_START;
+                gradient_at();
+                Sys S;
+                assemble_sys(c, X, S);
+                double relres = 0;
+                int it = pcg(S, rhs.data(), sol.data(), lin_rtol, 100000, &relres);
+                if (it < 0) return ORC_E_DEGENERATE;  // the reference exits when the solve fails
+                if (st) st->linear_iters += it;
+                L2Norm = 0;
+                for (size_t i = 0; i < n3; ++i) L2Norm += sol[i] * sol[i];
+                L2Norm = std::sqrt(L2Norm / (nV - nDBC)) / h;
+                if (L2Norm > NewtonTol) {
+                    energy(c, X, c.withCollision, Eprev);
+                    if (c.dbc_stiff) dbc_energy(c, X, Eprev);
+                }
+                if (st) st->friction_updates++;
+            }
+        }
         if (st) {
             st->n_newton = PNIter;
             st->n_constraints = c.ncs;
+            st->n_friction = c.nfc;
             st->last_res = L2Norm;
             st->E = Eprev;
             st->dbc_stiff = c.dbc_stiff;
@@ -789,4 +899,24 @@ extern "C" long orc_newton_system_shell_flow(int nV, const double* X, const doub
     return newton_system_impl(nV, X, Xtilde, X0, mass, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nE, tris, tvol,
                               true, nb, bnode, ne, bedge, nf, btri, nDBC, dbc_idx, h, dHat2, kappa, thickness, rowptr,
                               col, val, g, cap);
+}
+
+// Advance_One_Step_IE_Hinge / _Flow on a tet mesh with lagged friction: mu,
+// epsv2 (epsv h is the static-friction speed bound), fric_iters
+// (fricIterAmt: basis updates before the step may end; <= 0 until the
+// direction after an update is below the tolerance), and per-component
+// coefficients (ncomp > 0: compNodeRange / muComp, mu then 1)
+extern "C" int orc_advance_friction(int nV, double* X, const double* X0, double* vel, const double* mass, int nT,
+                                    const int* tets, const double* IB, const double* vol, const double* lam,
+                                    const double* mu, int nb, const int* bnode, int ne, const int* bedge, int nf,
+                                    const int* btri, int nDBC, const int* dbc_idx, const double* dbc_target, double h,
+                                    double NewtonTol, int withCollision, double dHat2, double* kappa, double thickness,
+                                    double lin_rtol, int max_newton, const double* body_force, int flow,
+                                    double fmu, double epsv2, int fric_iters, int ncomp, const int* comp_range,
+                                    const double* mu_comp, orc_stats* st)
+{
+    FrictionArgs fa{fmu, epsv2, fric_iters, ncomp, comp_range, mu_comp};
+    return advance_impl(nV, X, X0, vel, mass, nT, tets, IB, vol, lam, mu, 0, nullptr, nullptr, nb, bnode, ne, bedge,
+                        nf, btri, nDBC, dbc_idx, dbc_target, h, NewtonTol, withCollision, dHat2, kappa, thickness,
+                        lin_rtol, max_newton, body_force, flow, st, nullptr, &fa);
 }

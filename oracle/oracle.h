@@ -18,6 +18,8 @@ typedef struct orc_stats {
     double last_alpha, first_ccd_alpha, last_res, last_relres, min_dist2, E;
     double t_constraint_set, t_ccd, t_solve, t_hessian, t_gradient, t_energy;
     double dbc_alpha, dbc_stiff; /* moving DBC: CCD-limited move at the start, penalty stiffness at the end */
+    long n_friction;             /* friction constraints at the last basis update */
+    int friction_updates;        /* lagged basis updates after the first */
 } orc_stats;
 
 int orc_surface(int nV, const double* X, int nT, const int* tets, int* nb, int* ne, int* nf, int* bnode,

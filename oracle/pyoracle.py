@@ -41,6 +41,8 @@ class Stats(C.Structure):
         ("t_energy", C.c_double),
         ("dbc_alpha", C.c_double),
         ("dbc_stiff", C.c_double),
+        ("n_friction", C.c_long),
+        ("friction_updates", C.c_int),
     ]
 
     def as_dict(self):
@@ -121,6 +123,21 @@ def lib():
                                               C.c_double, C.c_double, C.c_double, C.c_double, ip, ip, dp, dp,
                                               C.c_long]
         L.orc_newton_system_shell.restype = C.c_long
+        L.orc_friction_basis.argtypes = [dp, C.c_long, ip, C.c_double, C.c_double, C.c_double, ip, dp, dp, dp]
+        L.orc_friction_basis.restype = C.c_long
+        L.orc_friction_coef.argtypes = [C.c_long, ip, C.c_int, ip, dp, dp]
+        L.orc_friction_energy.argtypes = [dp, dp, C.c_long, ip, dp, dp, dp, C.c_double, C.c_double]
+        L.orc_friction_energy.restype = C.c_double
+        L.orc_friction_gradient.argtypes = [dp, dp, C.c_long, ip, dp, dp, dp, C.c_double, C.c_double, dp]
+        L.orc_friction_gradient.restype = None
+        L.orc_friction_hessian_local.argtypes = [dp, dp, C.c_long, ip, dp, dp, dp, C.c_double, C.c_double, C.c_int,
+                                                 dp, ip]
+        L.orc_friction_hessian_local.restype = None
+        L.orc_advance_friction.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int, ip,
+                                           C.c_int, ip, C.c_int, ip, C.c_int, ip, C.c_void_p, C.c_double, C.c_double,
+                                           C.c_int, C.c_double, dp, C.c_double, C.c_double, C.c_int,
+                                           C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                           C.c_void_p, C.c_void_p, C.POINTER(Stats)]
         _lib = L
     return _lib
 
@@ -250,6 +267,30 @@ class Mesh:
                 return rp, col[:n], val[: 9 * n].reshape(n, 3, 3), g
             cap = -n
 
+    def advance_friction(self, X, vel, h, tol, dHat2, kappa, mu, epsv2, fric_iters=1, comp_range=None,
+                         mu_comp=None, thickness=0.0, lin_rtol=1e-12, max_newton=0, dbc_idx=None, dbc_target=None,
+                         body_force=None, flow=False):
+        """One step with lagged friction (mu, epsv2, fricIterAmt, and optional
+        per-component coefficients compNodeRange / muComp)."""
+        X = _f(X).copy()
+        vel = _f(vel).copy()
+        bf = None if body_force is None else _f(body_force).copy()
+        kap = np.ascontiguousarray(kappa, dtype=np.float64).copy()
+        dbc_idx = np.zeros(0, np.int32) if dbc_idx is None else _i(dbc_idx)
+        tgt = None if dbc_target is None else _f(dbc_target)
+        cr = None if comp_range is None else _i(comp_range)
+        mc = None if mu_comp is None else _f(mu_comp)
+        st = Stats()
+        it = lib().orc_advance_friction(
+            self.nV, X, self.X0, vel, self.mass, self.nT, self.tets, self.IB, self.vol, self.lam, self.mu,
+            self.bnode.size, self.bnode, self.bedge.shape[0], _i(self.bedge), self.btri.shape[0], _i(self.btri),
+            dbc_idx.size, dbc_idx, None if tgt is None else tgt.ctypes.data_as(C.c_void_p), h, tol, 1, dHat2, kap,
+            thickness, lin_rtol, max_newton, None if bf is None else bf.ctypes.data_as(C.c_void_p), int(bool(flow)),
+            mu, epsv2, int(fric_iters), 0 if cr is None else cr.size,
+            None if cr is None else cr.ctypes.data_as(C.c_void_p), None if mc is None else mc.ctypes.data_as(C.c_void_p),
+            C.byref(st))
+        return it, X.reshape(-1, 3), vel.reshape(-1, 3), kap, st.as_dict()
+
     def advance(self, X, vel, h, tol, dHat2, kappa, thickness=0.0, lin_rtol=1e-12, max_newton=0,
                 with_collision=True, dbc_idx=None, dbc_target=None, body_force=None, flow=False):
         """One step; dbc_target (n, 3) moves the DBC nodes dbc_idx (None: they stay);
@@ -269,6 +310,54 @@ class Mesh:
                                         None if bf is None else bf.ctypes.data_as(C.c_void_p), int(bool(flow)),
                                         C.byref(st))
         return it, X.reshape(-1, 3), vel.reshape(-1, 3), kap, st.as_dict()
+
+
+class FrictionSet:
+    """Compute_Friction_Basis at X on the barrier constraint set cs: fcs (n, 4),
+    closest-point parameters cp (n, 2), tangent bases (n, 3, 2) and normal
+    forces (n)."""
+
+    def __init__(self, X, cs, dHat2, kappa, thickness=0.0):
+        cs = _i(cs)
+        n = cs.size // 4
+        self.fcs = np.zeros(4 * n + 4, np.int32)
+        self.cp = np.zeros(2 * n + 2)
+        self.basis = np.zeros(6 * n + 6)
+        self.nforce = np.zeros(n + 1)
+        m = lib().orc_friction_basis(_f(X), n, cs, dHat2, kappa, thickness, self.fcs, self.cp, self.basis,
+                                     self.nforce)
+        self.n = int(m)
+        self.fcs = self.fcs[: 4 * m].copy()
+        self.cp = self.cp[: 2 * m].copy()
+        self.basis = self.basis[: 6 * m].copy()
+        self.nforce = self.nforce[:m].copy()
+
+    def tangent_basis(self):
+        """(n, 3, 2): columns of each constraint's tangent basis."""
+        return self.basis.reshape(-1, 2, 3).transpose(0, 2, 1)
+
+    def coef(self, comp_range, mu_comp):
+        cr = _i(comp_range)
+        rc = lib().orc_friction_coef(self.n, self.fcs, cr.size, cr, _f(mu_comp), self.nforce)
+        if rc:
+            raise ValueError("a friction vertex is in no component")
+
+    def energy(self, X, Xn, epsvh2, mu):
+        return lib().orc_friction_energy(_f(X), _f(Xn), self.n, self.fcs, self.cp, self.basis, self.nforce,
+                                         epsvh2, mu)
+
+    def gradient(self, X, Xn, epsvh2, mu):
+        g = np.zeros(_f(X).size)
+        lib().orc_friction_gradient(_f(X), _f(Xn), self.n, self.fcs, self.cp, self.basis, self.nforce, epsvh2, mu,
+                                    g)
+        return g
+
+    def hessian_local(self, X, Xn, epsvh2, mu, project=True):
+        H = np.zeros(144 * self.n + 1)
+        vid = np.zeros(4 * self.n + 1, np.int32)
+        lib().orc_friction_hessian_local(_f(X), _f(Xn), self.n, self.fcs, self.cp, self.basis, self.nforce, epsvh2,
+                                         mu, int(project), H, vid)
+        return H[: 144 * self.n].reshape(self.n, 144), vid[: 4 * self.n].reshape(self.n, 4)
 
 
 def set_accd_max_steps(n):
