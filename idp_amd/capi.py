@@ -107,7 +107,7 @@ def tuned_config(path=None):
 
 
 K_FAMILIES = ["pcg", "barrier_hess", "tet_hess", "assembly", "constraint_set", "ccd", "energy", "gradient",
-              "pcg_spmv", "pcg_update", "friction"]
+              "pcg_spmv", "pcg_update", "friction", "friction_basis"]
 
 
 class IdpError(RuntimeError):

@@ -295,12 +295,12 @@ void friction_basis(idp_ctx* c, const double* X, double dHat2, double kappa, dou
     FricState& f = c->fr;
     int* err = f.err.ensure(1, s);
     IDP_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
-    const int tok = c->prof.begin(IDP_K_FRICTION);
+    const int tok = c->prof.begin(IDP_K_FRICTION_BASIS);
     f.n = fric_basis_update(s, c->L.tmp, c->cw.ncs, (const int*)c->cw.cs.p, X, dHat2 + 2 * std::sqrt(dHat2) * thickness,
                             thickness * thickness, kappa, f.ncomp, f.comp_range.p, f.mu_comp.p, f.flag, f.pos, f.fcs,
                             f.fdat, err);
     // bytes: the constraint entries, flags and positions, the vertices read, the records written
-    c->prof.end(IDP_K_FRICTION, tok, 24.0 * c->cw.ncs + (96.0 + 96.0) * f.n);
+    c->prof.end(IDP_K_FRICTION_BASIS, tok, 24.0 * c->cw.ncs + (96.0 + 96.0) * f.n);
     int he = 0;
     IDP_HIP(hipMemcpyAsync(&he, err, sizeof(int), hipMemcpyDeviceToHost, s));
     IDP_HIP(hipStreamSynchronize(s));
@@ -2664,7 +2664,9 @@ int idp_friction_hessian_local(idp_ctx* c, const double* X, double h, int projec
         auto s = c->s;
         if (f.n) {
             double* b = c->Hb.ensure(144 * (size_t)f.n, s);
+            const int tok = c->prof.begin(IDP_K_FRICTION);
             launch_fric_hess(s, f.n, f.fcs.p, f.fdat.p, xd, f.Xn, f.epsvh, f.mu_eff, project, 0, b);
+            c->prof.end(IDP_K_FRICTION, tok, (96.0 + 192.0 + 1152.0) * f.n);
             IDP_HIP(hipMemcpyAsync(H, b, 144 * (size_t)f.n * sizeof(double), hipMemcpyDeviceToHost, s));
         }
         IDP_HIP(hipStreamSynchronize(s));

@@ -267,12 +267,20 @@ __global__ void k_fric_grad(long nf, const int4* __restrict__ fcs, const double*
 // Compute_Friction_Hessian (FRICTION.h:385-660) with projectSPD: the blocks
 // c_a c_b B W B^T.  mode 1: 3 x 3 blocks (a, b) at out + 144 i + 9 (a nv + b)
 // (the contact assembly's layout); mode 0: the dense (3 nv)^2 row-major
-// matrix at out + 144 i, zero padded
-__global__ void k_fric_hess(long nf, const int4* __restrict__ fcs, const double* __restrict__ fdat,
-                            const double* __restrict__ X, const double* __restrict__ Xn, double epsvh, double mu,
-                            int project, int mode, double* __restrict__ out)
+// matrix at out + 144 i, zero padded.  A team of FT lanes per constraint:
+// each lane forms W and M = B W B^T from the (shared, cached) record and
+// writes every FT-th double of the constraint's 144, so a wavefront's
+// stores cover whole contiguous runs.
+constexpr int FT = 16;
+
+__global__ void __launch_bounds__(256) k_fric_hess(long nf, const int4* __restrict__ fcs,
+                                                   const double* __restrict__ fdat, const double* __restrict__ X,
+                                                   const double* __restrict__ Xn, double epsvh, double mu,
+                                                   int project, int mode, double* __restrict__ out)
 {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long i = t / FT;
+    const int tl = (int)(t % FT);
     if (i >= nf) return;
     const FricLoc f = fric_load(i, fcs, fdat, X, Xn);
     const double x2 = f.u0 * f.u0 + f.u1 * f.u1, xn = sqrt(x2);
@@ -308,23 +316,40 @@ __global__ void k_fric_hess(long nf, const int4* __restrict__ fcs, const double*
     }
     double c[4];
     tt_coef(f.k.kind, f.cp0, f.cp1, c);
-    const int nv = f.k.nv;
+    const int nv = f.k.nv, n3 = 3 * nv, nval = 9 * nv * nv;
     double* o = out + 144 * i;
-    for (int a = 0; a < nv; ++a)
-        for (int b = 0; b < nv; ++b) {
-            const double cab = c[a] * c[b];
+    for (int e = tl; e < 144; e += FT) {
+        double val = 0.0;
+        if (e < nval) {
+            int a, b, r, cc;
             if (mode == 1) {
-#pragma unroll
-                for (int e = 0; e < 9; ++e) o[9 * (a * nv + b) + e] = cab * M[e];
+                const int bk = e / 9, q = e - 9 * bk;
+                a = bk / nv;
+                b = bk - a * nv;
+                r = q / 3;
+                cc = q - 3 * r;
             } else {
-#pragma unroll
-                for (int r = 0; r < 3; ++r)
-#pragma unroll
-                    for (int cc = 0; cc < 3; ++cc) o[(3 * a + r) * (3 * nv) + 3 * b + cc] = cab * M[3 * r + cc];
+                const int row = e / n3, col = e - row * n3;
+                a = row / 3;
+                r = row - 3 * a;
+                b = col / 3;
+                cc = col - 3 * b;
             }
+            double ca = c[0], cb = c[0];
+#pragma unroll
+            for (int k = 1; k < 4; ++k) {
+                ca = (a == k) ? c[k] : ca;
+                cb = (b == k) ? c[k] : cb;
+            }
+            double m = M[0];
+#pragma unroll
+            for (int k = 1; k < 9; ++k) m = (3 * r + cc == k) ? M[k] : m;
+            val = (ca * cb) * m;
+        } else if (mode == 1) {
+            break;  // the assembly reads only the nv^2 blocks
         }
-    if (mode == 0)
-        for (int e = 9 * nv * nv; e < 144; ++e) o[e] = 0.0;
+        o[e] = val;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -364,7 +389,7 @@ void launch_fric_grad(hipStream_t s, long nf, const int4* fcs, const double* fda
 void launch_fric_hess(hipStream_t s, long nf, const int4* fcs, const double* fdat, const double* X,
                       const double* Xn, double epsvh, double mu, int project, int mode, double* out)
 {
-    IDP_LAUNCH(k_fric_hess, grid_for(nf, 256), 256, 0, s, nf, fcs, fdat, X, Xn, epsvh, mu, project, mode, out);
+    IDP_LAUNCH(k_fric_hess, grid_for(nf * FT, 256), 256, 0, s, nf, fcs, fdat, X, Xn, epsvh, mu, project, mode, out);
 }
 
 }  // namespace idp

@@ -105,7 +105,8 @@ enum {
     IDP_K_GRADIENT = 7,     /* tet + barrier gradient and vertex gather */
     IDP_K_PCG_SPMV = 8,     /* PCG SpMV kernel, sampled on one iteration in 8 (bytes) */
     IDP_K_PCG_UPDATE = 9,   /* PCG vector-update kernel, sampled likewise (bytes) */
-    IDP_K_FRICTION = 10,    /* friction basis update and Hessian kernels (bytes) */
+    IDP_K_FRICTION = 10,    /* friction Hessian kernel (bytes) */
+    IDP_K_FRICTION_BASIS = 11, /* friction basis update: flags, scan, records (bytes) */
     IDP_K_COUNT = 16
 };
 typedef struct idp_profile {
