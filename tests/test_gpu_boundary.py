@@ -1,8 +1,10 @@
 """The step's boundary on MI355X, through the C ABI, against the oracle:
 body force and the flow variant of idp_advance_one_step (the Hinge and Flow
 bindings, DISCRETE_SHELL.h:1110, 1113; IMPLICIT_EULER.h:196-217, 850-854),
-the ACCD step cap, the zero-step error (IPC.h:2014-2031), and the argument
-checks of the serving entry point.
+the ACCD step cap, the zero-step error (IPC.h:2014-2031), the d^2 <= 0 error
+(IPC.h:773-776), the gradient-descent fallback after a failed linear solve
+(DIRECT_SOLVER.h:42-44, IMPLICIT_EULER.h:531-551), and the argument checks of
+the serving entry point.
 
 Bars: same Newton count and end state within 1e-6 of the bounding-box
 diagonal; CCD step sizes bit for bit.
@@ -160,6 +162,69 @@ def test_zero_ccd_step_is_reported():
         with pytest.raises(capi.IdpError) as e:
             ctx.ccd(sc.X.ravel(), (sc.target - sc.X).ravel(), 0.0)
         assert e.value.code == capi.IDP_E_CCD_ZERO
+
+
+def test_nonpositive_distance_is_reported():
+    """Thickness above the gap between two blocks: d^2 - t^2 <= 0 in a barrier
+    term at the start of the step and in the stage gradient."""
+    sc, _ = scenes.two_blocks_slide(gap=2e-4)  # 1e-5 apart
+    vol, IB, lam, mu, mass = capi.fem_init(sc.X, sc.tets, sc.density, sc.E, sc.nu)
+    dHat2, kappa = capi.initialize_oipc(sc.dhat, mass)
+    th = 2e-5
+    with capi.Context(0) as ctx:
+        ctx.set_mesh(sc.X, sc.tets, IB, vol, lam, mu, mass)
+        ctx.set_state(sc.X, np.zeros_like(sc.X))
+        say("[boundary] d^2 <= 0 at the start of a step")
+        with pytest.raises(capi.IdpError) as e:
+            ctx.advance(sc.h, sc.newton_tol, dHat2, kappa, th)
+        assert e.value.code == capi.IDP_E_NONPOS_DIST
+        cs = ctx.constraint_set(sc.X, dHat2, th)
+        assert cs.shape[0] > 0
+        with pytest.raises(capi.IdpError) as e:
+            ctx.barrier_gradient(sc.X, cs, dHat2, kappa[0], th)
+        assert e.value.code == capi.IDP_E_NONPOS_DIST
+
+
+def test_gradient_descent_fallback_matches_oracle():
+    """A small tet far from the bar with no stiffness and a negative mass on
+    its vertices makes the Newton matrix indefinite where the gradient lives:
+    the linear solve fails (the device's preconditioner finds a block that is
+    not positive definite, the oracle's PCG breaks down with p.q < 0) and the
+    step takes the gradient direction instead (DIRECT_SOLVER.h:42-44,
+    IMPLICIT_EULER.h:531-551).  Each such iteration doubles that tet's offset
+    from x~; the bar, at rest at x~, stays put."""
+    sc = scenes.small_bend()
+    nb = sc.X.shape[0]
+    tiny = np.array([[0.0, 0.0, 0.0], [0.01, 0.0, 0.0], [0.0, 0.01, 0.0], [0.0, 0.0, 0.01]]) + 1.0
+    X = np.vstack([sc.X, tiny])
+    T = np.vstack([sc.tets, np.arange(nb, nb + 4)[None, :]]).astype(np.int32)
+    vol, IB, lam, mu, mass = capi.fem_init(X, T, sc.density, sc.E, sc.nu)
+    lam[-1] = mu[-1] = 0.0
+    mass[nb:] = -1.0
+    v = np.zeros_like(X)
+    v[nb:] = [1.0, 0.0, 0.0]
+    m = pyoracle.Mesh(X, T, sc.density, sc.E, sc.nu)
+    m.lam[-1] = m.mu[-1] = 0.0
+    m.mass[nb:] = -1.0
+    dHat2, kappa = capi.initialize_oipc(sc.dhat, mass[:nb])
+    it_o, Xo, _, _, _ = m.advance(X, v, sc.h, sc.newton_tol, dHat2, kappa.copy(), max_newton=3)
+    cold = capi.PCG_COLD
+    capi.PCG_COLD = 1  # every solve from zero, as the oracle's
+    try:
+        with capi.Context(0) as ctx:
+            ctx.set_mesh(X, T, IB, vol, lam, mu, mass)
+            ctx.set_state(X, v)
+            say("[boundary] gradient-descent fallback")
+            it, st = ctx.advance(sc.h, sc.newton_tol, dHat2, kappa.copy(), pcg_rtol=1e-12, max_newton=3)
+            Xg, _ = ctx.get_state()
+    finally:
+        capi.PCG_COLD = cold
+    assert it == it_o == 3
+    assert st["gd_iters"] == 3
+    assert np.abs(Xg - Xo).max() <= 1e-12
+    # x - x~ = -h v doubles each iteration: -h, -2h, -4h, -8h
+    assert np.allclose(Xg[nb:, 0], X[nb:, 0] - 7 * sc.h, rtol=0, atol=1e-12)
+    assert np.abs(Xg[:nb] - X[:nb]).max() <= 1e-15  # the rest shape's rounding-level gradient
 
 
 def test_serve_start_rejects_duplicate_and_busy_slots():
