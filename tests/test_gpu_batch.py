@@ -88,3 +88,28 @@ def test_batch_handover_changes_nothing():
         Xb, vb = b.state(i)
         assert np.array_equal(Xa, Xb) and np.array_equal(va, vb)
         assert a.kappa[i, 0] == b.kappa[i, 0]
+
+
+def test_256_cube_batch_instances_equal_their_solo_solves():
+    """The 256-cube batch of the batched workload (cubes of 4^3..16^3 cells,
+    seeded sinusoid targets), first 20 Newton iterations of every instance:
+    four sampled instances (the smallest, the largest and two between)
+    solved alone give the same iterates bit for bit."""
+    scs = scenes.config3()
+    assert len(scs) == 256
+    say("[batch] 256 cubes, 20 Newton iterations")
+    b = driver.BatchAnimFix(scs)
+    its, _ = b.advance_one_time_step(max_newton=20)
+    assert min(its) >= 1 and max(its) <= 20
+    sizes = np.array([sc.X.shape[0] for sc in scs])
+    pick = [int(np.argmin(sizes)), int(np.argmax(sizes)), 37, 200]
+    for i in pick:
+        say(f"[batch] instance {i} ({scs[i].name}, {sizes[i]} vertices) alone")
+        a = driver.BatchAnimFix([scs[i]])
+        ia, _ = a.advance_one_time_step(max_newton=20)
+        Xa, va = a.state(0)
+        Xm, vm = b.state(i)
+        assert ia[0] == its[i]
+        assert np.array_equal(Xa, Xm) and np.array_equal(va, vm)
+        a.close()
+    b.close()
