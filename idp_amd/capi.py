@@ -63,6 +63,9 @@ PCG_COLD = int(os.environ.get("IDP_PCG_COLD", "0"))
 ACCD_BUDGET = int(os.environ.get("IDP_ACCD_BUDGET", "0"))
 # batched engine: PCG hand-over (0 adaptive default, < 0 never; IDP_PCG_HANDOVER)
 PCG_HANDOVER = int(os.environ.get("IDP_PCG_HANDOVER", "0"))
+# coarse level of the one-mesh PCG preconditioner: free vertices per
+# aggregate (0 library default, < 0 none; IDP_COARSE)
+COARSE_VERTS = int(os.environ.get("IDP_COARSE", "0"))
 
 # PCG launch configuration written by the autotuner (idp_amd/autotune.py):
 # IDP_TUNING names the file, else tuning.json beside this module; absent or
@@ -125,7 +128,7 @@ ACCD_MAX_STEPS_DEFAULT = 1 << 20
 class SolverOpts(C.Structure):
     _fields_ = [("pcg_rtol", C.c_double), ("pcg_maxit", C.c_int), ("max_newton", C.c_int), ("verbose", C.c_int),
                 ("precond", C.c_int), ("pcg_cold_start", C.c_int), ("accd_budget", C.c_int),
-                ("pcg_handover", C.c_int), ("accd_max_steps", C.c_int)]
+                ("pcg_handover", C.c_int), ("accd_max_steps", C.c_int), ("coarse_verts", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -684,7 +687,7 @@ class Context:
         self._check(lib().idp_restore_state(self._h))
 
     def advance(self, h, tol, dHat2, kappa, thickness=0.0, with_collision=True, pcg_rtol=1e-8, pcg_maxit=10000,
-                max_newton=0, verbose=0, body_force=None, flow=False, accd_max_steps=None):
+                max_newton=0, verbose=0, body_force=None, flow=False, accd_max_steps=None, coarse_verts=None):
         """Advance_One_Step_IE_Discrete_Shell (flow=False: the Hinge binding,
         True: the Flow binding); kappa (3,) is updated in place; body_force
         (nV, 3) or None."""
@@ -695,7 +698,8 @@ class Context:
         if bf is not None and bf.size != 3 * self.nV:
             raise ValueError("body_force must be (nV, 3)")
         opts = SolverOpts(pcg_rtol, pcg_maxit, max_newton, verbose, PRECOND, PCG_COLD, ACCD_BUDGET, PCG_HANDOVER,
-                          ACCD_MAX_STEPS if accd_max_steps is None else accd_max_steps)
+                          ACCD_MAX_STEPS if accd_max_steps is None else accd_max_steps,
+                          COARSE_VERTS if coarse_verts is None else coarse_verts)
         st = StepStats()
         it = self._check(lib().idp_advance_one_step(self._h, h, tol, int(with_collision), dHat2,
                                                     kappa_arr.ctypes.data_as(dp), thickness, bfp, int(bool(flow)),
@@ -784,7 +788,7 @@ class Context:
         return H.reshape(nT, 144)
 
     def newton_direction(self, X, xtilde, h, dHat2, kappa, thickness=0.0, pcg_rtol=1e-10, pcg_maxit=20000,
-                         flow=False):
+                         flow=False, coarse_verts=None):
         """(g, direction, relative residual, PCG iterations) of the Newton system
         at X; flow: the flow variant's terms (idp_newton_direction_flow)."""
         X, Xp = _d(X)
@@ -793,7 +797,8 @@ class Context:
         sol, sp = _out_d(X.size)
         rel = C.c_double()
         it = C.c_int()
-        opts = SolverOpts(pcg_rtol, pcg_maxit, 0, 0, PRECOND, PCG_COLD, ACCD_BUDGET, PCG_HANDOVER, ACCD_MAX_STEPS)
+        opts = SolverOpts(pcg_rtol, pcg_maxit, 0, 0, PRECOND, PCG_COLD, ACCD_BUDGET, PCG_HANDOVER, ACCD_MAX_STEPS,
+                          COARSE_VERTS if coarse_verts is None else coarse_verts)
         fn = lib().idp_newton_direction_flow if flow else lib().idp_newton_direction
         self._check(fn(self._h, Xp, xtp, h, dHat2, kappa, thickness, C.byref(opts), gp, sp, C.byref(rel),
                        C.byref(it)))

@@ -114,6 +114,13 @@ double now_ms()
 }
 
 // ACCD step cap of a solve: 0 selects the library default, < 0 none
+// target free vertices per coarse aggregate of a solve (0: no coarse level)
+int coarse_from(const idp_solver_opts* o)
+{
+    const int v = (!o || o->coarse_verts == 0) ? IDP_COARSE_VERTS_DEFAULT : o->coarse_verts;
+    return v > 0 ? v : 0;
+}
+
 int accd_cap_from(const idp_solver_opts* o)
 {
     if (!o || o->accd_max_steps == 0) return IDP_ACCD_MAX_STEPS_DEFAULT;
@@ -1306,6 +1313,7 @@ int setup_mesh(idp_ctx* c, int nV, const double* X_rest, int nT, const int* tets
         c->ncs_prev = 0;
         IDP_HIP(hipStreamSynchronize(s));
         c->have_mesh = true;
+        c->L.coarse.ready = false;
         c->snap_valid = false;
         return IDP_OK;
 }
@@ -1884,6 +1892,7 @@ int idp_set_dbc(idp_ctx* c, int n, const int* idx)
         c->ndbc = 0;
         for (auto b : d) c->ndbc += b;
         c->m.dbc.upload(d.data(), d.size(), c->s);
+        c->L.coarse.ready = false;  // its aggregates leave the Dirichlet nodes out
         c->dbc_list.assign(idx, idx + n);
         c->dbc_tgt.clear();  // the nodes stay where each step finds them
         IDP_HIP(hipStreamSynchronize(c->s));
@@ -1902,6 +1911,7 @@ int idp_set_dbc_targets(idp_ctx* c, int n, const int* idx, const double* targets
         }
         c->ndbc = n;
         c->m.dbc.upload(d.data(), d.size(), c->s);
+        c->L.coarse.ready = false;
         c->dbc_list.assign(idx, idx + n);
         c->dbc_tgt.assign(targets, targets + 3 * (size_t)n);
         if (n) {
@@ -2043,11 +2053,18 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
         c->L.precond = opts ? opts->precond : 0;
         const bool cold = opts && opts->pcg_cold_start;
         c->cw.accd_max_steps = accd_cap_from(opts);
+        c->L.coarse.verts_per_agg = coarse_from(opts);
+        c->L.coarse.ready = false;  // built anew in each step (its Dirichlet set may differ)
         auto s = c->s;
         MeshDev& m = c->m;
         const long n3 = 3L * m.nV;
         const bool coll = with_collision != 0;
         double* x = c->x.p;
+        struct XposOff {  // the coarse level reads the positions of this step's systems only
+            LinWork& L;
+            ~XposOff() { L.xpos = nullptr; }
+        } xpos_off{c->L};
+        c->L.xpos = x;
         double* xn = c->xn.ensure(n3, s);
         double* xt = c->xt.ensure(n3, s);
         double* xprev = c->xprev.ensure(n3, s);
@@ -2812,8 +2829,12 @@ int idp_newton_direction(idp_ctx* c, const double* X, const double* xtilde, doub
         double rtol = (opts && opts->pcg_rtol > 0) ? opts->pcg_rtol : 1e-8;
         int maxit = (opts && opts->pcg_maxit > 0) ? opts->pcg_maxit : 10000;
         c->L.precond = opts ? opts->precond : 0;
+        c->L.coarse.verts_per_agg = coarse_from(opts);
+        c->L.coarse.ready = false;  // built for this system
         double rel = 0;
+        c->L.xpos = xd;
         int it = pcg_solve_persistent(c->L, s, m, rhs, sd, rtol, maxit, &rel);
+        c->L.xpos = nullptr;
         if (g) c->g.download(g, n3, s);
         if (sol) c->sol.download(sol, n3, s);
         IDP_HIP(hipStreamSynchronize(s));

@@ -598,6 +598,10 @@ struct PcgArgs {
     // per kernel boundary by k_pcg_red_spmv / k_pcg_red_upd (r.z, r.r, the
     // previous r.z or |b|^2; p.q, r.z) instead of by every block (null)
     double* red;
+    // coarse level (one instance): partials of r_c . e_c, cnsb per parity,
+    // added to r.z wherever it is formed (null: none)
+    const double* cpart;
+    int cnsb;
 };
 
 constexpr int PCG_ROW_THREADS = 256;         // row threads of a SpMV block (4 wavefronts), 256 / G rows
@@ -922,6 +926,10 @@ __global__ void __launch_bounds__(PCG_ROW_THREADS + (SW ? 64 : 0)) k_pcg_spmv(Pc
                                it > 0 ? A.part_rz + (par ^ 1) * A.nbB + pb : A.part_rr + A.nbB + pb, np, rz, rr,
                                third);
         }
+        if (A.cpart) {  // r . (z1 + P e_c) = r . z1 + r_c . e_c
+            rz += wave_sum_partials<8>(A.cpart + par * A.cnsb, A.cnsb);
+            if (it > 0) third += wave_sum_partials<8>(A.cpart + (par ^ 1) * A.cnsb, A.cnsb);
+        }
         const double rzo = it > 0 ? third : 1.0;
         const double bbp = it == 0 ? third : 0.0;
         if ((threadIdx.x & 63) == 0) {
@@ -1090,6 +1098,7 @@ __global__ void __launch_bounds__(PCG_UPD_BLOCK * PCG_UPD_WAVES_MAX) k_pcg_updat
         wave_sum_partials2<8, 4>(A.part_pq + A.apbeg[inst], A.apbeg[inst + 1] - A.apbeg[inst],
                                   A.part_rz + par * A.nbB + pb, A.bpbeg[inst + 1] - pb, pq, rz);
     }
+    if (A.cpart) rz += wave_sum_partials<8>(A.cpart + par * A.cnsb, A.cnsb);
     if (f != 0) return;  // uniform over the wavefront
     if (!(pq > 0.0)) {
         if (bid == pb && lane == 0) {
@@ -1382,8 +1391,12 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
     IDP_LAUNCH(k_cl_build, vg, 256, 0, s, nV, root2, cnt, mem, m.e_diag.p, m.e_rowptr.p, m.e_col.p, L.evals.p,
                m.nnzb_e, haveC ? L.cdiag.p : (const int*)nullptr, crp, L.ccol.p, L.cvals.p, L.ncb, I.vinst.p, cmk, Z,
                cm4, work, err);
+    // coarse level: one instance with the positions of its system given
+    const bool coarse = B == 1 && L.xpos && L.coarse.verts_per_agg > 0 && sv[0] == 1 && coarse_setup(L, s, m, L.xpos);
     PcgArgs A;
     A.nV = nV;
+    A.cpart = coarse ? L.coarse.part.p : nullptr;
+    A.cnsb = coarse ? L.coarse.nsb : 0;
     A.erp = m.e_rowptr.p;
     A.ecol = m.e_col.p;
     A.eval = L.evals.p;
@@ -1460,7 +1473,7 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
         const char* e = std::getenv("IDP_PCG_PREREDUCE");
         const int force = e ? std::atoi(e) : -1;
         const bool big = A.nbB >= PCG_PREREDUCE_BLOCKS;
-        A.red = (B == 1 && A.ident && (force < 0 ? big : force > 0)) ? L.red.ensure(8, s) : nullptr;
+        A.red = (B == 1 && A.ident && !coarse && (force < 0 ? big : force > 0)) ? L.red.ensure(8, s) : nullptr;
     }
     if (warm && nA) {
         if (shape.g == 4) IDP_LAUNCH(k_pcg_warm<4>, nA, PCG_ROW_THREADS, 0, s, A);
@@ -1482,6 +1495,7 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
         }
     };
     if (nB) IDP_LAUNCH(k_pcg_init, nB, PCG_UPD_BLOCK, 0, s, A);
+    if (coarse) coarse_apply(L, s, nV, A.r, A.z, flag, 0);
     // algorithmic bytes of one SpMV launch (blocks + columns, two row-pointer
     // arrays, z and p_old read, p_new and q written) and of one update launch
     // (x, p, r, q, P^-1 read; x, r, z written)
@@ -1509,6 +1523,7 @@ void pcg_solve_batch(LinWork& L, hipStream_t s, const MeshDev& m, const double* 
             t = sample ? P->pair(e0, e1) : -1;
             IDP_LAUNCH_TIMED(k_pcg_update, (nB + uw - 1) / uw, PCG_UPD_BLOCK * uw, 0, s, e0, e1, A, it);
             if (sample) P->end_pair(IDP_K_PCG_UPDATE, t, upd_bytes * frac, it);
+            if (coarse) coarse_apply(L, s, nV, A.r + (size_t)((it + 1) & 1) * 3 * nV, A.z, flag, (it + 1) & 1);
         }
         IDP_HIP(hipMemcpyAsync(hf.data(), err, 2 * B * sizeof(int), hipMemcpyDeviceToHost, s));
         IDP_HIP(hipStreamSynchronize(s));

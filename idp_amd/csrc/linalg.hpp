@@ -6,6 +6,29 @@
 
 namespace idp {
 
+// coarse level of the one-mesh PCG preconditioner (coarse.hip)
+struct CoarseWork {
+    int verts_per_agg = 0;  // target free vertices per aggregate; 0: no coarse level
+    int rebuild_every = 4;  // Newton systems per build: A_c^-1 and P of an earlier system still give an
+                            // SPD preconditioner, so they are reused in between
+    bool ready = false;     // a level exists (built for this mesh and Dirichlet set)
+    int age = 0, nV = 0;
+    const unsigned char* mask = nullptr;  // the Dirichlet mask it was built with
+    int K = 0, ncd = 0, nsb = 0;
+    DBuf<int> agg, aptr, avert;
+    DBuf<double> P, Ainv, rc, part;
+    DBuf<unsigned long long> key0, key1;
+    DBuf<unsigned> val0, val1, head, hs, seg;
+    DBuf<unsigned char> tmp;
+    DBuf<int> info;
+    std::vector<double> hx;
+    void* handle = nullptr;  // rocblas_handle
+    CoarseWork() = default;
+    CoarseWork(const CoarseWork&) = delete;
+    CoarseWork& operator=(const CoarseWork&) = delete;
+    ~CoarseWork();
+};
+
 struct LinWork {
     DBuf<unsigned char> tmp;
     // contact structure of the current constraint set
@@ -34,7 +57,16 @@ struct LinWork {
     DBuf<unsigned long long> cl_pack;
     DBuf<double> cl_work;
     Profiler* prof = nullptr;
+    CoarseWork coarse;
+    const double* xpos = nullptr;  // positions of the system being solved (coarse aggregation); null: none
 };
+
+// Builds the coarse level for the assembled system at positions X (device);
+// false when it is not used for this system (too few vertices, several
+// instances, A_c not positive definite).
+bool coarse_setup(LinWork& L, hipStream_t s, const MeshDev& m, const double* X);
+// z += P A_c^-1 P^T r and the partials of r_c . e_c at part + parity * nsb
+void coarse_apply(LinWork& L, hipStream_t s, int nV, const double* r, double* z, const int* flag, int parity);
 
 // the contact blocks of the ncs barrier constraints cs followed by nf more
 // stencils in the same int4 encoding (the friction set); local terms of entry

@@ -58,9 +58,15 @@ typedef struct idp_solver_opts {
                            conservative step).  0: the default, IDP_ACCD_MAX_STEPS_DEFAULT; < 0: no
                            cap, as in the reference's loop (CCD.h:308-327, 367-392).  Capped pairs
                            are counted in idp_step_stats.accd_capped. */
+    int coarse_verts;   /* one-mesh steps and Newton directions: a coarse level in the PCG
+                           preconditioner (rigid-body modes of voxel aggregates of about this many
+                           free vertices, added to the cluster block Jacobi); 0: the default,
+                           IDP_COARSE_VERTS_DEFAULT; < 0: none.  Changes the PCG path within its
+                           stated relative residual only. */
 } idp_solver_opts;
 
 #define IDP_ACCD_MAX_STEPS_DEFAULT (1 << 20)
+#define IDP_COARSE_VERTS_DEFAULT 144
 
 typedef struct idp_step_stats {
     int newton_iters;
