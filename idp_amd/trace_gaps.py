@@ -54,6 +54,16 @@ def analyze(path):
         busy += sum(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in run)
         gaps += [int(b["Start_Timestamp"]) - int(a["End_Timestamp"]) for a, b in zip(run, run[1:])]
     g = np.array(gaps, dtype=np.float64) / 1e3 if gaps else np.zeros(1)
+    # PCG launches after a solve's stopping iteration return at once: the
+    # working ones are those above 1.5 x the shortest
+    working = {}
+    for k in ("k_pcg_spmv", "k_pcg_update", "k_coarse_restrict", "k_coarse_solve"):
+        d = np.array([(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in rows
+                      if _base(r["Kernel_Name"]) == k])
+        if d.size:
+            w = d[d >= 1.5 * d.min()]
+            working[k] = {"launches": int(d.size), "working": int(w.size), "avg_us_working": float(w.mean()) if w.size else 0.0,
+                          "avg_us_all": float(d.mean())}
     return {
         "trace_span_ms": (int(rows[-1]["End_Timestamp"]) - int(rows[0]["Start_Timestamp"])) / 1e6,
         "kernel_time_ms": sum(tot.values()) / 1e3,
@@ -63,6 +73,7 @@ def analyze(path):
         "pcg_gap_frac": 1 - busy / span if span else 0.0,
         "gap_us_median": float(np.median(g)), "gap_us_p90": float(np.percentile(g, 90)),
         "gap_us_p99": float(np.percentile(g, 99)), "gap_us_max": float(g.max()), "gaps": len(gaps),
+        "pcg_working_launches": working,
     }
 
 
