@@ -157,6 +157,7 @@ __global__ void k_coarse_sym(int ncd, double* __restrict__ Ac)
 // per PCG iteration
 // ---------------------------------------------------------------------------
 constexpr int COARSE_BLOCK = 256;
+constexpr int COARSE_MAX_AGG = 320;  // aggregates at most (A_c of order 1920, 29 MB)
 
 // fixed-order sum of N values over a workgroup of COARSE_BLOCK threads:
 // xor butterfly per wavefront, then the wavefronts in order (all threads
@@ -335,19 +336,27 @@ bool coarse_setup(LinWork& L, hipStream_t s, const MeshDev& m, const double* X)
             vol *= hi[d] - lo[d];
             ++nd;
         }
-    const double target = (double)nfree / C.verts_per_agg;
-    const double hcell = std::pow(vol / target, 1.0 / std::max(nd, 1));
-    // aggregates: voxel cells in first-seen vertex order
+    const double target = std::min((double)nfree / C.verts_per_agg, (double)COARSE_MAX_AGG);
+    double hcell = std::pow(vol / target, 1.0 / std::max(nd, 1));
+    // aggregates: voxel cells in first-seen vertex order; coarser cells while
+    // there are more than COARSE_MAX_AGG of them (A_c^-1 is read whole by
+    // every PCG iteration)
     std::unordered_map<long long, int> cell;
     std::vector<int> agg(nV, -1);
-    for (int v = 0; v < nV; ++v) {
-        if (dbc[v]) continue;
-        long long k = 0;
-        for (int d = 0; d < 3; ++d) k = k * 2097152 + (long long)std::floor((x[3 * v + d] - lo[d]) / hcell);
-        auto it = cell.emplace(k, (int)cell.size()).first;
-        agg[v] = it->second;
+    for (int pass = 0; pass < 8; ++pass) {
+        cell.clear();
+        for (int v = 0; v < nV; ++v) {
+            if (dbc[v]) continue;
+            long long k = 0;
+            for (int d = 0; d < 3; ++d) k = k * 2097152 + (long long)std::floor((x[3 * v + d] - lo[d]) / hcell);
+            auto it = cell.emplace(k, (int)cell.size()).first;
+            agg[v] = it->second;
+        }
+        if ((int)cell.size() <= COARSE_MAX_AGG) break;
+        hcell *= std::cbrt((double)cell.size() / COARSE_MAX_AGG) * 1.05;
     }
     const int K = (int)cell.size();
+    if (K > COARSE_MAX_AGG || K < 2) return false;
     // per aggregate: vertex lists, centroids, inertia
     std::vector<int> aptr(K + 1, 0), avert(nfree);
     for (int v = 0; v < nV; ++v)

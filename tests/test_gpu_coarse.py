@@ -58,3 +58,22 @@ def test_coarse_level_keeps_dirichlet_rows_zero():
         say(f"[coarse] PCG iterations {it0} -> {it}")
     finally:
         d.close()
+
+
+def test_coarse_level_is_deterministic_and_bounded_on_a_large_mesh():
+    """The 160x48x48 crumple bar (384k vertices): the aggregates are capped in
+    number (the dense A_c^-1 is read by every PCG iteration), and two solves
+    of the same system give the same direction bit for bit."""
+    sc = scenes.config5()
+    d = driver.TetAnimFix(sc)
+    try:
+        X = sc.X
+        Xt = X + 0.5 * sc.h * sc.velocity
+        say("[coarse] 160x48x48 bar: two solves of its first Newton system")
+        g0, s0, rel0, it0 = d.ctx.newton_direction(X, Xt, sc.h, d.dHat2, d.kappa[0], pcg_rtol=1e-8, coarse_verts=144)
+        g1, s1, rel1, it1 = d.ctx.newton_direction(X, Xt, sc.h, d.dHat2, d.kappa[0], pcg_rtol=1e-8, coarse_verts=144)
+        say(f"[coarse] PCG iterations {it0}, {it1}")
+        assert np.array_equal(s0, s1) and it0 == it1
+        assert rel0 <= 1e-8
+    finally:
+        d.close()
