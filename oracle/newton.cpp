@@ -920,3 +920,21 @@ extern "C" int orc_advance_friction(int nV, double* X, const double* X0, double*
                         nf, btri, nDBC, dbc_idx, dbc_target, h, NewtonTol, withCollision, dHat2, kappa, thickness,
                         lin_rtol, max_newton, body_force, flow, st, nullptr, &fa);
 }
+
+// Advance_One_Step_IE_Hinge on a shell with lagged friction (the arguments of
+// orc_advance_shell, then those of orc_advance_friction)
+extern "C" int orc_advance_shell_friction(int nV, double* X, const double* X0, double* vel, const double* mass,
+                                          int nE, const int* tris, const double* tvol, const double* tIB,
+                                          const double* tlam, const double* tmu, int nH, const int* hinge,
+                                          const double* hrest, double bend_k, int nb, const int* bnode, int ne,
+                                          const int* bedge, int nf, const int* btri, int nDBC, const int* dbc_idx,
+                                          double h, double NewtonTol, double dHat2, double* kappa, double thickness,
+                                          double lin_rtol, int max_newton, const double* body_force, double fmu,
+                                          double epsv2, int fric_iters, orc_stats* st)
+{
+    ShellElastic se{tIB, tlam, tmu, nH, hinge, hrest, bend_k};
+    FrictionArgs fa{fmu, epsv2, fric_iters, 0, nullptr, nullptr};
+    return advance_impl(nV, X, X0, vel, mass, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nE, tris, tvol, nb,
+                        bnode, ne, bedge, nf, btri, nDBC, dbc_idx, nullptr, h, NewtonTol, 1, dHat2, kappa, thickness,
+                        lin_rtol, max_newton, body_force, 0, st, &se, &fa);
+}

@@ -123,6 +123,11 @@ def lib():
                                               C.c_double, C.c_double, C.c_double, C.c_double, ip, ip, dp, dp,
                                               C.c_long]
         L.orc_newton_system_shell.restype = C.c_long
+        L.orc_advance_shell_friction.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int, ip, dp, dp, dp, dp, C.c_int, ip,
+                                                 dp, C.c_double, C.c_int, ip, C.c_int, ip, C.c_int, ip, C.c_int, ip,
+                                                 C.c_double, C.c_double, C.c_double, dp, C.c_double, C.c_double,
+                                                 C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_int,
+                                                 C.POINTER(Stats)]
         L.orc_friction_basis.argtypes = [dp, C.c_long, ip, C.c_double, C.c_double, C.c_double, ip, dp, dp, dp]
         L.orc_friction_basis.restype = C.c_long
         L.orc_friction_coef.argtypes = [C.c_long, ip, C.c_int, ip, dp, dp]
@@ -493,6 +498,23 @@ class ShellMesh:
                                      self.btri.shape[0], _i(self.btri), dbc_idx.size, dbc_idx, h, tol,
                                      int(with_collision), dHat2, kap, thickness, lin_rtol, max_newton,
                                      None if bf is None else bf.ctypes.data_as(C.c_void_p), C.byref(st))
+        return it, X.reshape(-1, 3), vel.reshape(-1, 3), kap, st.as_dict()
+
+    def advance_hinge_friction(self, X, vel, h, tol, dHat2, kappa, mu, epsv2, fric_iters=1, thickness=0.0,
+                               lin_rtol=1e-12, max_newton=0, dbc_idx=None, body_force=None):
+        """One Advance_One_Step_IE_Hinge step with lagged friction (DBC nodes stay)."""
+        X = _f(X).copy()
+        vel = _f(vel).copy()
+        bf = None if body_force is None else _f(body_force).copy()
+        kap = np.ascontiguousarray(kappa, dtype=np.float64).copy()
+        dbc_idx = np.zeros(0, np.int32) if dbc_idx is None else _i(dbc_idx)
+        st = Stats()
+        it = lib().orc_advance_shell_friction(
+            self.nV, X, self.X0, vel, self.mass, self.nE, self.tris, self.tvol, self.tIB, self.lam, self.mu,
+            self.nH, self.hinge, self.hrest, self.bend_k, self.bnode.size, self.bnode, self.bedge.shape[0],
+            _i(self.bedge), self.btri.shape[0], _i(self.btri), dbc_idx.size, dbc_idx, h, tol, dHat2, kap, thickness,
+            lin_rtol, max_newton, None if bf is None else bf.ctypes.data_as(C.c_void_p), mu, epsv2, int(fric_iters),
+            C.byref(st))
         return it, X.reshape(-1, 3), vel.reshape(-1, 3), kap, st.as_dict()
 
     def normal_flow_force(self, X, magnitude):

@@ -3,7 +3,8 @@ bending + inertia + barrier on a sheet draped over a sphere shell whose nodes
 are Dirichlet.  The Newton system (gradient within 1e-10, BSR matrix within
 1e-9 of its largest entry after the projections) and whole steps (same
 Newton counts, line-search trials within one, positions within 1e-6 of the
-bounding-box diagonal, intersection free)."""
+bounding-box diagonal, intersection free), also with lagged friction (same
+friction set sizes and basis updates as the oracle)."""
 import numpy as np
 import pytest
 
@@ -82,5 +83,41 @@ def test_shell_hinge_steps_match_oracle():
             d.ctx.set_state(Xo, vo)
         assert st["n_constraints"] > 0
         assert not exact.surface_self_intersects(Xg, m.bedge, m.btri)
+    finally:
+        d.close()
+
+
+@pytest.mark.parametrize("fric_iters", [1, 3])
+def test_shell_hinge_friction_steps_match_oracle(fric_iters):
+    """The draped sheet with lagged friction (mu 0.3) on the sphere: same
+    Newton counts, friction set sizes and basis updates as the oracle,
+    positions within 1e-6 of the bounding-box diagonal."""
+    X, T, dbc, m, d = _scene()
+    dHat2, kappa = d.dHat2, d.kappa.copy()
+    Xo, vo = X.copy(), np.zeros_like(X)
+    diag = float(np.linalg.norm(X.max(0) - X.min(0)))
+    nf = 0
+    try:
+        d.ctx.set_friction(0.3, 1e-6, fric_iters)
+        for s in range(8):
+            it_o, Xo, vo, kappa, st_o = m.advance_hinge_friction(Xo, vo, H, 1e-3 * H, dHat2, kappa, 0.3, 1e-6,
+                                                                 fric_iters=fric_iters, dbc_idx=dbc,
+                                                                 body_force=d.body_force)
+            say(f"[shell friction {fric_iters}] step {s}: gpu")
+            it, st = d.advance_one_time_step()
+            Xg, vg = d.state()
+            err = np.abs(Xg - Xo).max() / diag
+            say(f"[shell friction {fric_iters}] step {s}: newton {it} (oracle {it_o}), friction set "
+                f"{st['n_friction']} (oracle {st_o['n_friction']}), updates {st['friction_updates']} "
+                f"(oracle {st_o['friction_updates']}), err/diag {err:.2e}")
+            assert it == it_o
+            assert st["n_friction"] == st_o["n_friction"]
+            assert st["friction_updates"] == st_o["friction_updates"]
+            assert d.kappa[0] == kappa[0]
+            assert err < 1e-6
+            nf = max(nf, st["n_friction"])
+            Xo, vo = Xg.copy(), vg.copy()
+            d.ctx.set_state(Xo, vo)
+        assert nf > 0
     finally:
         d.close()

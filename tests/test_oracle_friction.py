@@ -6,7 +6,8 @@ contact normal, the normal force as the barrier's force on the contact, the
 gradient and Hessian as derivatives of the energy (central differences), the
 Coulomb magnitude mu N of a sliding contact with zero net force, the
 per-component coefficients, and a block on a tilted slab that slides with
-acceleration g (sin t - mu cos t) or sticks."""
+acceleration g (sin t - mu cos t) or sticks; the shell step with friction
+is the frictionless one as mu -> 0 and slows a sheet sliding over a sphere."""
 
 import numpy as np
 import pytest
@@ -244,3 +245,48 @@ def test_block_sticks_and_updates_stay_within_fric_iters(fric_iters):
     # the block rings elastically about rest (a few mm/s) instead of sliding
     assert abs(vx[-1]) < 0.05 * free and np.abs(vx).max() < 0.1 * free
     assert all(st["friction_updates"] <= fric_iters - 1 for st in sts)
+
+
+def _sheet_run(mu, steps=8, fric_iters=1):
+    RHO, YOUNG, NU, THICK, H = 100.0, 1e4, 0.3, 0.003, 0.01
+    X, T, dbc = scenes.draped_sheet()
+    m = pyoracle.ShellMesh(X, T, RHO, THICK, E=YOUNG, nu=NU, bend_mult=1.0)
+    m.set_dbc(dbc)
+    bf = m.mass[:, None] * np.array([0.0, -9.81, 0.0])[None, :]
+    dHat2, kappa = pyoracle.initialize_oipc(1e-3, m.mass)
+    Xs, v = X.copy(), np.zeros_like(X)
+    sts = []
+    for _ in range(steps):
+        if mu is None:
+            _, Xs, v, kappa, st = m.advance_hinge(Xs, v, H, 1e-3 * H, dHat2, kappa, dbc_idx=dbc, body_force=bf)
+        else:
+            _, Xs, v, kappa, st = m.advance_hinge_friction(Xs, v, H, 1e-3 * H, dHat2, kappa, mu, 1e-6,
+                                                           fric_iters=fric_iters, dbc_idx=dbc, body_force=bf)
+        sts.append(st)
+    return X, dbc, Xs, v, sts
+
+
+def test_shell_friction_vanishes_with_mu_and_slows_the_draped_sheet():
+    """The shell step with friction (Advance_One_Step_IE_Hinge with lagged
+    friction): with mu -> 0 it is the frictionless step, and with mu = 2 the
+    sheet's vertices on the sphere slide over it more slowly."""
+    X, dbc, X0, v0, _ = _sheet_run(None, steps=10)
+    _, _, Xe, ve, sts = _sheet_run(1e-12, steps=10)
+    assert sum(st["n_friction"] for st in sts) > 0
+    diag = float(np.linalg.norm(X.max(0) - X.min(0)))
+    assert np.abs(Xe - X0).max() <= 1e-9 * diag
+    _, _, X2, v2, sts2 = _sheet_run(2.0, steps=10)
+    assert sts2[-1]["n_friction"] > 0
+    c = np.array([0.5, -0.303, 0.5])  # the sphere's centre (radius 0.3)
+    free = np.setdiff1d(np.arange(X.shape[0]), dbc)
+
+    def tangential(Xs, v):
+        r = Xs[free] - c
+        d = np.linalg.norm(r, axis=1)
+        n = r / d[:, None]
+        vt = v[free] - (v[free] * n).sum(1)[:, None] * n
+        return np.linalg.norm(vt[d < 0.31], axis=1)
+
+    t0, t2 = tangential(X0, v0), tangential(X2, v2)
+    assert t0.size > 0 and t2.size == t0.size
+    assert t2.max() < 0.95 * t0.max() and t2.mean() < 0.95 * t0.mean()
