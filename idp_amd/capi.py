@@ -45,7 +45,7 @@ EXPORTED = [
     "idp_elastic_energy", "idp_elastic_gradient", "idp_elastic_hessian_local", "idp_newton_direction", "idp_get_bsr",
     "idp_profile_enable", "idp_profile_read", "idp_set_mesh_batch", "idp_batch_size", "idp_advance_batch",
     "idp_serve_configure", "idp_serve_start", "idp_serve_iterate", "idp_serve_result", "idp_set_launch_config",
-    "idp_get_launch_config", "idp_set_accd_max_steps", "idp_barrier_energy_noalloc", "idp_get_surface", "idp_device_memory", "idp_set_barrier_team",
+    "idp_get_launch_config", "idp_set_accd_max_steps", "idp_set_accd_certificate", "idp_ccd_last_stats", "idp_barrier_energy_noalloc", "idp_get_surface", "idp_device_memory", "idp_set_barrier_team",
     "idp_get_barrier_team", "idp_load_velocity", "idp_shell_init", "idp_shell_surface", "idp_set_mesh_shell",
     "idp_normal_flow_force", "idp_set_normal_flow", "idp_newton_direction_flow", "idp_shell_hinges",
     "idp_set_shell_elasticity", "idp_set_friction", "idp_friction_basis", "idp_get_friction", "idp_friction_energy",
@@ -142,6 +142,7 @@ class StepStats(C.Structure):
         ("ms_solve", C.c_double), ("dbc_alpha", C.c_double), ("dbc_stiff", C.c_double),
         ("pcg_unconverged", C.c_int), ("accd_capped", C.c_long), ("n_friction", C.c_long),
         ("friction_updates", C.c_int),
+        ("accd_certified", C.c_long),
     ]
 
     def as_dict(self):
@@ -200,6 +201,8 @@ def lib():
         L.idp_advance_one_step.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_double, dp, C.c_double, dp,
                                            C.c_int, C.POINTER(SolverOpts), C.POINTER(StepStats)]
         L.idp_set_accd_max_steps.argtypes = [vp, C.c_int]
+        L.idp_set_accd_certificate.argtypes = [vp, C.c_int]
+        L.idp_ccd_last_stats.argtypes = [vp, C.POINTER(C.c_long), C.POINTER(C.c_long)]
         L.idp_constraint_set.argtypes = [vp, dp, C.c_double, C.c_double, C.POINTER(C.c_long)]
         L.idp_get_constraints.argtypes = [vp, ip, C.c_long]
         L.idp_barrier_energy.argtypes = [vp, dp, C.c_long, ip, C.c_double, C.c_double, C.c_double, dp]
@@ -709,6 +712,16 @@ class Context:
     def set_accd_max_steps(self, n):
         """ACCD step cap of the stage calls (0 default, < 0 none)."""
         self._check(lib().idp_set_accd_max_steps(self._h, int(n)))
+
+    def set_accd_certificate(self, on):
+        """No-hit certificate of long ACCD pairs (on by default)."""
+        self._check(lib().idp_set_accd_certificate(self._h, int(bool(on))))
+
+    def ccd_last_stats(self):
+        """(capped, certified) ACCD pairs of the last CCD call."""
+        a, b = C.c_long(), C.c_long()
+        self._check(lib().idp_ccd_last_stats(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     # ---- stage entry points -------------------------------------------------
     def constraint_set(self, X, dHat2, thickness=0.0):

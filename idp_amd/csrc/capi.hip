@@ -2264,6 +2264,7 @@ int idp_advance_one_step(idp_ctx* c, double h, double newton_tol, int with_colli
                 S.ms_ccd += now_ms() - t0;
                 S.ccd_candidates += nc;
                 S.accd_capped += c->cw.last_capped;
+                S.accd_certified += c->cw.last_certified;
                 if (alpha == 0.0) {  // IPC.h:2014-2031 exits on a zero time of impact
                     c->err = "zero CCD step size";
                     if (st) *st = S;
@@ -2730,6 +2731,23 @@ int idp_set_accd_max_steps(idp_ctx* c, int n)
 {
     return guarded(c, [&]() -> int {
         c->cw.accd_max_steps = n == 0 ? IDP_ACCD_MAX_STEPS_DEFAULT : (n < 0 ? 0 : n);
+        return IDP_OK;
+    });
+}
+
+int idp_set_accd_certificate(idp_ctx* c, int on)
+{
+    return guarded(c, [&]() -> int {
+        c->cw.accd_cert = on != 0;
+        return IDP_OK;
+    });
+}
+
+int idp_ccd_last_stats(idp_ctx* c, long* capped, long* certified)
+{
+    return guarded(c, [&]() -> int {
+        if (capped) *capped = c->cw.last_capped;
+        if (certified) *certified = c->cw.last_certified;
         return IDP_OK;
     });
 }

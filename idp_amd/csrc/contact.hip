@@ -656,6 +656,7 @@ struct CCDArgs {
     const int* vinst;
     const unsigned char* active;  // null: every instance
     int nf;
+    int cert;  // long pairs try the no-hit certificate
     double th;
 };
 
@@ -781,6 +782,7 @@ struct AccdLane {
     double maxDisp, d2, dc, f, gap, toc, tmax, d0, tmax0;
     double lb;  // the next step's lower bound, computed as soon as dc is known
     int guard, inst;
+    int cert_at;  // step count at which the no-hit certificate is next tried
     int2 c;
     bool ee;
 };
@@ -791,7 +793,7 @@ constexpr int ACCD_RUN = 32;
 // pairs that ran at least this many ACCD steps are logged (diagnostics)
 constexpr int ACCD_LOG_STEPS = 1 << 16;
 constexpr int ACCD_LOG_MAX = 8;
-constexpr int ACCD_LOG_W = 10;  // doubles per entry
+constexpr int ACCD_LOG_W = 34;  // doubles per entry (10 figures, then the pair's start x[4], dx[4])
 
 __device__ inline double accd_dist(const AccdLane& L)
 {
@@ -807,6 +809,87 @@ __device__ inline void accd_fix_ee(AccdLane& L, double th)
         L.d2 = fmin(fmin(sqn(L.x[0] - L.x[2]), sqn(L.x[0] - L.x[3])), fmin(sqn(L.x[1] - L.x[2]), sqn(L.x[1] - L.x[3])));
         L.f = L.d2 - th * th;
     }
+}
+
+// No-hit certificate of a long pair.  The iteration ends with a hit only where
+// the distance it evaluates, minus the thickness, drops below gap = eta (d0 - th).
+// Its evaluations lie in [toc, T]: every one before the last is at or below
+// tmax, and the last one is at toc_k + lb_k with lb_k = (1 - eta)(d_k - th) /
+// maxDisp; if d_k >= d0 there, the distance can fall to at most
+// th + eta (d_k - th) (maxDisp bounds its rate), which is no hit, and otherwise
+// toc_k + lb_k <= tmax + (1 - eta)(d0 - th) / maxDisp = T.  Every distance the
+// iteration can evaluate (point-triangle: point-plane, point-edge or point-point;
+// edge-edge: line-line, point-edge or point-point) is at least the distance
+// between the point and the triangle's plane (PT) or between the two lines
+// (EE), |g(t)| / |c(t)| with c = u x v and g = c . w, where u, v, w move
+// linearly: g is a cubic and c a quadratic in t.  Over each of
+// ACCD_CERT_PIECES pieces of [toc, T] the Bernstein coefficients of g bound
+// |g| from below (when they share a sign) and the control points of c bound |c|
+// from above; the pair cannot hit when the bound stays above the gap with a
+// margin of 1e-3 of it, of the floating-point error of the coefficients (1e-12
+// of the product of the moving vectors' sizes) and of the rounding the
+// iteration's position updates can gather over the steps still to come (27
+// standard deviations of a random walk of half-ulp errors, per coordinate of
+// the four points).  Such a pair ends with no hit, as the steps would.
+constexpr int ACCD_CERT_STEPS = 4096;  // pairs expected to take more steps try the certificate
+constexpr int ACCD_CERT_PIECES = 64;
+
+__device__ inline double ulp_of(double a)
+{
+    int e;
+    frexp(a, &e);
+    return ldexp(1.0, e - 53);
+}
+
+__device__ inline bool accd_certify(const AccdLane& L, double eta, double th)
+{
+    if (!(L.gap > 0) || !(L.maxDisp > 0)) return false;
+    const double T = L.tmax * (1 + 1e-9) + (1 - eta) * (L.d0 - th) * (1 + 1e-6) / L.maxDisp + 1e-9;
+    const double Tr = T - L.toc;
+    if (!(Tr > 0)) return false;
+    // EE: u = a1 - a0, v = b1 - b0, w = b0 - a0; PT: u = t1 - t0, v = t2 - t0, w = p - t0
+    const int iu0 = L.ee ? 0 : 1, iu1 = L.ee ? 1 : 2, iv0 = L.ee ? 2 : 1, iv1 = 3, iw0 = L.ee ? 0 : 1,
+              iw1 = L.ee ? 2 : 0;
+    const d3 u0 = L.x[iu1] - L.x[iu0], u1 = L.dx[iu1] - L.dx[iu0];
+    const d3 v0 = L.x[iv1] - L.x[iv0], v1 = L.dx[iv1] - L.dx[iv0];
+    const d3 w0 = L.x[iw1] - L.x[iw0], w1 = L.dx[iw1] - L.dx[iw0];
+    const d3 c0 = cross(u0, v0), c1 = cross(u0, v1) + cross(u1, v0), c2 = cross(u1, v1);
+    const double g0 = dot(c0, w0), g1 = dot(c0, w1) + dot(c1, w0), g2 = dot(c1, w1) + dot(c2, w0),
+                 g3 = dot(c2, w1);
+    const double su = sqrt(sqn(u0)) + sqrt(sqn(u1)) * Tr, sv = sqrt(sqn(v0)) + sqrt(sqn(v1)) * Tr,
+                 sw = sqrt(sqn(w0)) + sqrt(sqn(w1)) * Tr;
+    const double errc = 1e-12 * su * sv, errg = errc * sw;
+    double xm = 0.0, dm = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        xm = fmax(xm, fmax(fabs(L.x[k].x), fmax(fabs(L.x[k].y), fabs(L.x[k].z))));
+        dm = fmax(dm, fmax(fabs(L.dx[k].x), fmax(fabs(L.dx[k].y), fabs(L.dx[k].z))));
+    }
+    const double ulp = ulp_of(xm + dm * Tr);
+    const double kb = Tr * L.maxDisp / ((1 - eta) * L.gap) + 1;
+    const double need = th + L.gap * (1 + 1e-3) + (32 * 1.7320508075688772 * sqrt(kb) + 64) * ulp;
+    const double h = Tr / ACCD_CERT_PIECES;
+    for (int i = 0; i < ACCD_CERT_PIECES; ++i) {
+        const double ta = i * h;
+        const double G0 = ((g3 * ta + g2) * ta + g1) * ta + g0;
+        const double G1 = ((3 * g3 * ta + 2 * g2) * ta + g1) * h;
+        const double G2 = (3 * g3 * ta + g2) * h * h;
+        const double G3 = g3 * h * h * h;
+        const double b0 = G0, b1 = G0 + G1 / 3, b2 = G0 + (2 * G1 + G2) / 3, b3 = G0 + G1 + G2 + G3;
+        const double bmin = fmin(fmin(b0, b1), fmin(b2, b3)), bmax = fmax(fmax(b0, b1), fmax(b2, b3));
+        double lo;
+        if (bmin - errg > 0)
+            lo = bmin - errg;
+        else if (bmax + errg < 0)
+            lo = -bmax - errg;
+        else
+            return false;
+        const d3 C0 = ta * (ta * c2 + c1) + c0, C1 = h * (2 * ta * c2 + c1), C2 = (h * h) * c2;
+        const d3 q1 = C0 + 0.5 * C1, q2 = (C0 + C1) + C2;
+        const double cmax = sqrt(fmax(sqn(C0), fmax(sqn(q1), sqn(q2)))) + errc;
+        if (!(lo > need * cmax)) return false;
+    }
+    return true;
 }
 
 // returns false when the pair has no motion (no hit)
@@ -849,6 +932,7 @@ __device__ inline bool accd_init(AccdLane& L, int2 c, const CCDArgs& A, double e
     L.d0 = L.dc;
     L.tmax0 = L.tmax;
     L.c = c;
+    L.cert_at = ACCD_CERT_STEPS;
     return true;
 }
 
@@ -878,7 +962,7 @@ __device__ inline int accd_step(AccdLane& L, double eta, double th, int max_step
 __global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, unsigned cap,
                                               const int2* __restrict__ cand, CCDArgs A, double eta,
                                               unsigned long long* __restrict__ amin, unsigned* __restrict__ next,
-                                              unsigned long long* __restrict__ stepstat /* total, max, capped */,
+                                              unsigned long long* __restrict__ stepstat /* total, max, capped, certified */,
                                               unsigned* __restrict__ nlog, double* __restrict__ lg, int budget,
                                               int max_steps, unsigned* __restrict__ capcnt,
                                               const AccdLane* __restrict__ res_in, unsigned nres,
@@ -917,6 +1001,12 @@ __global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, 
                     const int2 c = q < npt ? cand[q] : cand[cap - 1 - (q - npt)];
                     active = accd_init(L, c, A, eta, amin);
                     amin_pf = L.tmax;
+                    // a pair whose first step predicts a long run tries the certificate
+                    // after its first run of steps, every pair after ACCD_CERT_STEPS
+                    if (!A.cert)
+                        L.cert_at = 0x7fffffff;
+                    else if (L.tmax * L.maxDisp > ACCD_CERT_STEPS * (1 - eta) * (L.dc - A.th))
+                        L.cert_at = 0;
                 }
             }
         }
@@ -935,6 +1025,15 @@ __global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, 
             } while (r == 0 && k < room);
             steps += k;
             slice += k;
+            if (r == 0 && L.guard >= L.cert_at) {
+                // a long pair tries the certificate at 4x spaced step counts
+                if (accd_certify(L, eta, A.th)) {
+                    r = 2;
+                    atomicAdd(stepstat + 3, 1ull);
+                } else {
+                    L.cert_at = L.guard > (1 << 28) ? 0x7fffffff : max(4 * L.guard, 4 * ACCD_CERT_STEPS);
+                }
+            }
             if (r == 1 || r == 3) atomicMin(amin + L.inst, (unsigned long long)__double_as_longlong(L.toc));
             if (r == 3) {
                 atomicAdd(stepstat + 2, 1ull);
@@ -957,6 +1056,12 @@ __global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, 
                         e[7] = L.guard + 1;
                         e[8] = r;
                         e[9] = L.dc;
+                        AccdLane L0;
+                        if (accd_init(L0, L.c, A, eta, amin))
+                            for (int v = 0; v < 4; ++v) {
+                                e[10 + 3 * v] = L0.x[v].x, e[11 + 3 * v] = L0.x[v].y, e[12 + 3 * v] = L0.x[v].z;
+                                e[22 + 3 * v] = L0.dx[v].x, e[23 + 3 * v] = L0.dx[v].y, e[24 + 3 * v] = L0.dx[v].z;
+                            }
                     }
                 }
             } else if (budget > 0 && slice >= budget) {
@@ -1357,11 +1462,11 @@ void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const do
     double* dalpha = w.ialpha.ensure(B, s);
     unsigned long long* amin = w.iamin.ensure(B, s);
     unsigned* cnt = w.cnt.ensure(6, s);
-    unsigned long long* stepstat = (unsigned long long*)w.scal.ensure(6, s) + 2;  // total, max, capped
+    unsigned long long* stepstat = (unsigned long long*)w.scal.ensure(6, s) + 2;  // total, max, capped, certified
     IDP_LAUNCH(k_ccd_cap, grid_for(B, 64), 64, 0, s, B, ep, I.echunk.pbeg.p, I.d_ebeg.p, pp, I.d_nbeg.p, ain, dalpha,
                amin, active);
     IDP_LAUNCH(k_zero_u32, 1, 64, 0, s, cnt, 6);
-    IDP_LAUNCH(k_zero_u64, 1, 64, 0, s, stepstat, 3);
+    IDP_LAUNCH(k_zero_u64, 1, 64, 0, s, stepstat, 4);
     IDP_LAUNCH(k_zero_u8, grid_for(B, 256), 256, 0, s, pend, B);
     unsigned* capcnt = w.icap.ensure(B, s);
     IDP_LAUNCH(k_fill_u32, grid_for(B, 256), 256, 0, s, (size_t)B, capcnt, 0u);
@@ -1388,10 +1493,11 @@ void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const do
     A.vinst = I.vinst.p;
     A.active = active;
     A.nf = m.nf;
+    A.cert = w.accd_cert ? 1 : 0;
     A.th = thickness;
     if (w.cap_cand == 0) w.cap_cand = 16u * (m.nb + m.ne) + 4096;
     std::vector<unsigned long long> ha(B);
-    unsigned long long hsteps[3] = {0, 0, 0};
+    unsigned long long hsteps[4] = {0, 0, 0, 0};
     unsigned hn[6] = {0, 0, 0, 0, 0, 0};
     for (int attempt = 0; attempt < 2; ++attempt) {
         int2* cand = w.cand.ensure(w.cap_cand, s);
@@ -1399,7 +1505,7 @@ void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const do
         if (attempt) {
             IDP_LAUNCH(k_reset_amin, grid_for(B, 64), 64, 0, s, B, dalpha, amin, active);
             IDP_LAUNCH(k_zero_u32, 1, 64, 0, s, cnt, 6);
-            IDP_LAUNCH(k_zero_u64, 1, 64, 0, s, stepstat, 3);
+            IDP_LAUNCH(k_zero_u64, 1, 64, 0, s, stepstat, 4);
             IDP_LAUNCH(k_zero_u8, grid_for(B, 256), 256, 0, s, pend, B);
             IDP_LAUNCH(k_fill_u32, grid_for(B, 256), 256, 0, s, (size_t)B, capcnt, 0u);
         }
@@ -1426,14 +1532,16 @@ void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const do
         if (w.prof) w.prof->end(IDP_K_CCD, tok);
         if (log_time) {
             // kernel time against the longest pair's steps in this call
-            unsigned long long st[2];
+            unsigned long long st[4];
             float ms = 0.f;
             IDP_HIP(hipMemcpyAsync(st, stepstat, sizeof(st), hipMemcpyDeviceToHost, s));
             IDP_HIP(hipStreamSynchronize(s));
             IDP_HIP(hipEventElapsedTime(&ms, e0, e1));
             if (st[1] >= 4096)
-                std::fprintf(stderr, "[idp] accd call: %.3f ms, longest pair %llu steps (%.1f ns per step), %llu steps\n",
-                             ms, st[1], 1e6 * ms / (double)st[1], st[0]);
+                std::fprintf(stderr,
+                             "[idp] accd call: %.3f ms, longest pair %llu steps (%.1f ns per step), %llu steps, %llu "
+                             "certified\n",
+                             ms, st[1], 1e6 * ms / (double)st[1], st[0], st[3]);
             IDP_HIP(hipEventDestroy(e0));
             IDP_HIP(hipEventDestroy(e1));
         }
@@ -1461,6 +1569,9 @@ void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const do
                          "[idp] long ACCD pair (%d,%d) inst %d d0 %.3e maxDisp %.3e tmax %.3e toc %.3e steps %.0f "
                          "result %.0f d %.3e\n",
                          (int)q[0], (int)q[1], (int)q[2], q[3], q[4], q[5], q[6], q[7], q[8], q[9]);
+            std::fprintf(stderr, "[idp] long ACCD pair start");
+            for (int v = 10; v < ACCD_LOG_W; ++v) std::fprintf(stderr, " %.17g", q[v]);
+            std::fprintf(stderr, "\n");
         }
     }
     if (w.prof) {
@@ -1470,6 +1581,7 @@ void contact_ccd_batch(ContactWork& w, hipStream_t s, const MeshDev& m, const do
     }
     if (ncand_out) *ncand_out = (long)hn[0] + hn[1];
     w.last_capped = (long)hsteps[2];
+    w.last_certified = (long)hsteps[3];
     w.h_capped.resize(B);
     if (hsteps[2]) {
         IDP_HIP(hipMemcpyAsync(w.h_capped.data(), capcnt, B * sizeof(unsigned), hipMemcpyDeviceToHost, s));

@@ -160,7 +160,9 @@ struct ContactWork {
     // ACCD step cap per pair and call (<= 0: none, as in CCD.h:308-327); a
     // capped pair counts as a hit at the time of impact it reached
     int accd_max_steps = 1 << 20;
-    long last_capped = 0;  // pairs capped in the last CCD call
+    long last_capped = 0;     // pairs capped in the last CCD call
+    long last_certified = 0;  // pairs ended by the no-hit certificate in the last CCD call
+    bool accd_cert = true;    // long ACCD pairs try the no-hit certificate
     std::vector<unsigned> h_capped;  // ... per instance
     DBuf<unsigned> icap;
     Profiler* prof = nullptr;

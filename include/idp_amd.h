@@ -95,6 +95,9 @@ typedef struct idp_step_stats {
        (IMPLICIT_EULER.h:757-848) */
     long n_friction;
     int friction_updates;
+    /* ACCD pairs ended with no hit by the certificate that their distance
+       stays above the stopping gap up to their bound (no steps taken past it) */
+    long accd_certified;
 } idp_step_stats;
 
 /* Kernel-family profile: device time from HIP events recorded on the
@@ -366,6 +369,16 @@ int idp_ccd_stepsize(idp_ctx* ctx, const double* X, const double* p, double thic
 /* ACCD step cap of the context's stage calls: n > 0 steps, 0 the default
  * (IDP_ACCD_MAX_STEPS_DEFAULT), < 0 none (the reference's loop). */
 int idp_set_accd_max_steps(idp_ctx* ctx, int n);
+/* No-hit certificate of long ACCD pairs (on by default): a pair expected to
+ * take more than 4096 steps, and every pair again at 4096, 16384, ... steps,
+ * is ended with no hit when a Bernstein bound shows that its distance stays
+ * above the stopping gap up to its bound (the outcome its remaining steps
+ * would reach; pairs that hit are always stepped).  No counterpart in the
+ * reference, whose loop steps every pair; on = 0 steps them all here too. */
+int idp_set_accd_certificate(idp_ctx* ctx, int on);
+/* ACCD pairs stopped by the step cap and ended by the certificate in the
+ * context's last CCD call (diagnostics; no counterpart in the reference) */
+int idp_ccd_last_stats(idp_ctx* ctx, long* capped, long* certified);
 /* Neo-Hookean tet terms with weight w = h^2 (INC_POTENTIAL.h:98-109, 228-229, 355-363) */
 int idp_elastic_energy(idp_ctx* ctx, const double* X, double w, double* E, int* valid);
 int idp_elastic_gradient(idp_ctx* ctx, const double* X, double w, double* g);

@@ -447,6 +447,8 @@ class ShellMesh:
         self.hrest = rest[: 3 * nh].copy()
         self.bend_k = bend_mult * E * thickness ** 3 / (24 * (1 - nu * nu))
 
+    ccd = Mesh.ccd  # the same surface step size (nodes, edges and triangles of the shell)
+
     def set_dbc(self, idx):
         self.dbc[:] = 0
         self.dbc[np.asarray(idx, dtype=np.int64)] = 1
