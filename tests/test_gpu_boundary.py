@@ -121,6 +121,9 @@ def test_accd_step_cap_matches_oracle_bit_for_bit():
     try:
         ctx.set_mesh(sc.X, sc.tets, IB, vol, lam, mu, mass)
         out = {}
+        # every pair stepped: a pair the no-hit certificate would end is
+        # otherwise stopped by the cap and taken as a hit
+        ctx.set_accd_certificate(False)
         for cap in (200, 1000, -1):
             say(f"[boundary] ACCD cap {cap}")
             ctx.set_accd_max_steps(cap)
@@ -130,6 +133,10 @@ def test_accd_step_cap_matches_oracle_bit_for_bit():
             assert ag == ao
             out[cap] = ag
         assert 0 < out[200] < out[1000] < out[-1] <= 1.0
+        # with the certificate (the default) and no cap: the reference's loop
+        ctx.set_accd_certificate(True)
+        ag, _ = ctx.ccd(sc.X.ravel(), p.ravel(), 1.0)
+        assert ag == out[-1]
     finally:
         pyoracle.set_accd_max_steps(ORACLE_ACCD_CAP)
         ctx.close()
