@@ -6,7 +6,7 @@
 
 `solve` runs the first Newton iterations of the 64x16x16 twist-and-fold bar
 (one mesh, the default solver options).  `analyze` groups the trace's
-consecutive PCG kernels (SpMV, update and their partial-sum reductions) into
+consecutive PCG kernels (SpMV, update, their partial-sum reductions and the coarse level) into
 runs, one run per chunk of enqueued iterations, and reports the kernels'
 device time against the runs' span: the difference is the time between
 kernels inside a run, which is what capturing the chunk in a graph could
@@ -22,7 +22,7 @@ import sys
 
 import numpy as np
 
-PCG_KERNELS = ("k_pcg_spmv", "k_pcg_update", "k_pcg_red_spmv", "k_pcg_red_upd")
+PCG_KERNELS = ("k_pcg_spmv", "k_pcg_update", "k_pcg_red_spmv", "k_pcg_red_upd", "k_coarse_restrict", "k_coarse_solve")
 
 
 def _base(name):
