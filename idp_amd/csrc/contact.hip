@@ -787,8 +787,10 @@ struct AccdLane {
     bool ee;
 };
 
-// steps a lane takes between visits to the work queue
+// steps a lane takes between visits to the work queue (ACCD_RUN_TAIL when its
+// wavefront has at most two pairs left)
 constexpr int ACCD_RUN = 32;
+constexpr int ACCD_RUN_TAIL = 256;
 
 // pairs that ran at least this many ACCD steps are logged (diagnostics)
 constexpr int ACCD_LOG_STEPS = 1 << 16;
@@ -941,14 +943,22 @@ __device__ inline bool accd_init(AccdLane& L, int2 c, const CCDArgs& A, double e
 // next step's bound is computed before the gap test is resolved (the same
 // expression the loop would evaluate after it), so the two divisions overlap
 // instead of following each other across the branch.
+// The kind (EE or PT) is a template argument, so that a run of steps is one
+// loop without the kind's branch.
+template <bool EE>
 __device__ inline int accd_step(AccdLane& L, double eta, double th, int max_steps)
 {
     const double lb = L.lb;
 #pragma unroll
     for (int k = 0; k < 4; ++k) L.x[k] = L.x[k] + lb * L.dx[k];
-    L.d2 = accd_dist(L);
-    L.f = L.d2 - th * th;
-    accd_fix_ee(L, th);
+    if (EE) {
+        L.d2 = d2_ee_unclassified(L.x[0], L.x[1], L.x[2], L.x[3]);
+        L.f = L.d2 - th * th;
+        accd_fix_ee(L, th);
+    } else {
+        L.d2 = d2_pt_unclassified(L.x[0], L.x[1], L.x[2], L.x[3]);
+        L.f = L.d2 - th * th;
+    }
     L.dc = sqrt(L.d2);
     const double den = L.dc + th;
     L.lb = (1 - eta) * L.f / (den * L.maxDisp);
@@ -1017,12 +1027,22 @@ __global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, 
         if (active) {
             // a run of steps between visits to the queue (a long pair's lane does
             // little besides stepping); it stops at the pair's end or the budget
-            const int room = budget > 0 ? min(ACCD_RUN, budget - slice) : ACCD_RUN;
+            // a pair with at most one other left in its wavefront (the tail of a long
+            // pair) takes longer runs between visits to the queue
+            const int run = __popcll(__ballot(1)) <= 2 ? ACCD_RUN_TAIL : ACCD_RUN;
+            const int room = budget > 0 ? min(run, budget - slice) : run;
             int r = 0, k = 0;
-            do {
-                r = accd_step(L, eta, A.th, max_steps);
-                ++k;
-            } while (r == 0 && k < room);
+            if (L.ee) {
+                do {
+                    r = accd_step<true>(L, eta, A.th, max_steps);
+                    ++k;
+                } while (r == 0 && k < room);
+            } else {
+                do {
+                    r = accd_step<false>(L, eta, A.th, max_steps);
+                    ++k;
+                } while (r == 0 && k < room);
+            }
             steps += k;
             slice += k;
             if (r == 0 && L.guard >= L.cert_at) {
