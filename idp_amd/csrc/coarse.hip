@@ -189,7 +189,7 @@ __device__ inline void coarse_block_sum(double* acc, double (*sh)[N])
 // r_c = P^T r: one workgroup per aggregate over its vertex list
 __global__ void __launch_bounds__(COARSE_BLOCK) k_coarse_restrict(int K, const int* __restrict__ aptr,
                                                                   const int* __restrict__ avert,
-                                                                  const double* __restrict__ P,
+                                                                  const double* __restrict__ Ps,
                                                                   const double* __restrict__ r,
                                                                   const int* __restrict__ flag,
                                                                   double* __restrict__ rc)
@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(COARSE_BLOCK) k_coarse_restrict(int K, const i
     double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int j = aptr[a] + (int)threadIdx.x; j < aptr[a + 1]; j += COARSE_BLOCK) {
         const int v = avert[j];
-        const double* Pv = P + 18 * (size_t)v;
+        const double* Pv = Ps + 18 * (size_t)j;
         const double r0 = r[3 * v], r1 = r[3 * v + 1], r2 = r[3 * v + 2];
 #pragma unroll
         for (int m = 0; m < 6; ++m) acc[m] += Pv[m] * r0 + Pv[6 + m] * r1 + Pv[12 + m] * r2;
@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(COARSE_BLOCK) k_coarse_restrict(int K, const i
 // partial r_c . e_c of those rows, and z += P_v e for the vertices of a
 __global__ void __launch_bounds__(COARSE_BLOCK) k_coarse_solve(int K, int ncd, const int* __restrict__ aptr,
                                                                const int* __restrict__ avert,
-                                                               const double* __restrict__ P,
+                                                               const double* __restrict__ Ps,
                                                                const double* __restrict__ Ainv,
                                                                const double* __restrict__ rc,
                                                                const int* __restrict__ flag,
@@ -229,7 +229,24 @@ __global__ void __launch_bounds__(COARSE_BLOCK) k_coarse_solve(int K, int ncd, c
     if (*(volatile const int*)flag != 0) return;  // uniform over the grid
     double e[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const double* A0 = Ainv + (size_t)(6 * a) * ncd;  // symmetric: rows = columns
-    for (int j = threadIdx.x; j < ncd; j += COARSE_BLOCK) {
+    // eight columns' loads in flight at once; each thread still adds its
+    // columns in increasing order
+    constexpr int U = 8;
+    int j = threadIdx.x;
+    for (; j + (U - 1) * COARSE_BLOCK < ncd; j += U * COARSE_BLOCK) {
+        double rj[U], av[U][6];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            rj[u] = rc[j + u * COARSE_BLOCK];
+#pragma unroll
+            for (int m = 0; m < 6; ++m) av[u][m] = A0[(size_t)m * ncd + j + u * COARSE_BLOCK];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int m = 0; m < 6; ++m) e[m] += av[u][m] * rj[u];
+    }
+    for (; j < ncd; j += COARSE_BLOCK) {
         const double rj = rc[j];
 #pragma unroll
         for (int m = 0; m < 6; ++m) e[m] += A0[(size_t)m * ncd + j] * rj;
@@ -243,7 +260,7 @@ __global__ void __launch_bounds__(COARSE_BLOCK) k_coarse_solve(int K, int ncd, c
     }
     for (int j = aptr[a] + (int)threadIdx.x; j < aptr[a + 1]; j += COARSE_BLOCK) {
         const int v = avert[j];
-        const double* Pv = P + 18 * (size_t)v;
+        const double* Pv = Ps + 18 * (size_t)j;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             double s = 0.0;
@@ -413,6 +430,13 @@ bool coarse_setup(LinWork& L, hipStream_t s, const MeshDev& m, const double* X)
     C.aptr.upload(aptr.data(), aptr.size(), s);
     C.avert.upload(avert.data(), avert.size(), s);
     double* d_P = C.P.upload_get(P.data(), P.size(), s);
+    {
+        // the apply kernels read P by slot, so its loads do not wait for avert's
+        std::vector<double> Ps(18 * (size_t)nfree);
+        for (int j = 0; j < nfree; ++j)
+            for (int q = 0; q < 18; ++q) Ps[18 * (size_t)j + q] = P[18 * (size_t)avert[j] + q];
+        C.Ps.upload(Ps.data(), Ps.size(), s);
+    }
     // A_c = P^T A P, summed per aggregate pair in block order
     const long ne = m.nnzb_e, nc = L.ncb, nb = ne + nc;
     unsigned long long* k0 = C.key0.ensure(nb + 1, s);
@@ -471,8 +495,8 @@ void coarse_apply(LinWork& L, hipStream_t s, int nV, const double* r, double* z,
 {
     (void)nV;
     CoarseWork& C = L.coarse;
-    IDP_LAUNCH(k_coarse_restrict, C.K, COARSE_BLOCK, 0, s, C.K, C.aptr.p, C.avert.p, C.P.p, r, flag, C.rc.p);
-    IDP_LAUNCH(k_coarse_solve, C.K, COARSE_BLOCK, 0, s, C.K, C.ncd, C.aptr.p, C.avert.p, C.P.p, C.Ainv.p, C.rc.p,
+    IDP_LAUNCH(k_coarse_restrict, C.K, COARSE_BLOCK, 0, s, C.K, C.aptr.p, C.avert.p, C.Ps.p, r, flag, C.rc.p);
+    IDP_LAUNCH(k_coarse_solve, C.K, COARSE_BLOCK, 0, s, C.K, C.ncd, C.aptr.p, C.avert.p, C.Ps.p, C.Ainv.p, C.rc.p,
                flag, C.part.p + (size_t)parity * C.nsb, z);
 }
 

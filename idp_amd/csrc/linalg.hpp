@@ -17,6 +17,7 @@ struct CoarseWork {
     int K = 0, ncd = 0, nsb = 0;
     DBuf<int> agg, aptr, avert;
     DBuf<double> P, Ainv, rc, part;
+    DBuf<double> Ps;  // P in aggregate order (18 doubles per slot of avert)
     DBuf<unsigned long long> key0, key1;
     DBuf<unsigned> val0, val1, head, hs, seg;
     DBuf<unsigned char> tmp;
