@@ -46,12 +46,12 @@ def scene(pair):
     return X, np.array([[0, 1, 2], [3, 4, 5]], np.int32), P
 
 
-def sliding_point(normal_speed, d=1e-8):
-    """A point at distance d above a unit triangle, moving 1e-3 along it and
-    normal_speed * d along the normal over the step: no hit for
+def sliding_point(normal_speed, d=1e-8, thickness=0.0):
+    """A point at distance thickness + d above a unit triangle, moving 1e-3
+    along it and normal_speed * d along the normal over the step: no hit for
     normal_speed > -0.9, a hit near t = 0.9 / -normal_speed below."""
     t = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
-    q = np.array([0.25, 0.25, d])
+    q = np.array([0.25, 0.25, thickness + d])
     x = np.vstack([q, t])
     p = np.zeros((4, 3))
     p[0] = [1e-3, 0.5e-3, normal_speed * d]

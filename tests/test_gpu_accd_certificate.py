@@ -54,3 +54,27 @@ def test_long_pair_step_sizes_match_oracle_with_and_without_certificate(uncapped
         assert cert_off == 0
         certified += cert
     assert certified >= 5
+
+
+@pytest.mark.parametrize("speed", [0.0, -1.9])
+def test_sliding_point_with_thickness_matches_oracle(uncapped_oracle, speed):
+    """The certificate's gap with a thickness: the loop stops where the
+    distance less the thickness falls below a tenth of its start value."""
+    th = 1e-4
+    pr = long_pairs.sliding_point(speed, thickness=th)
+    X, T, P = long_pairs.scene(pr)
+    m = pyoracle.ShellMesh(X, T, 100.0, 0.003, E=1e4, nu=0.3)
+    a_o, nc_o = m.ccd(X, P, 1.0, th)
+    tris, mass, tvol, _, _ = capi.shell_init(X, T, 100.0, 1e4, 0.3, 0.003)
+    with capi.Context(0) as ctx:
+        ctx.set_mesh_shell(X, mass, tris, tvol)
+        ctx.set_accd_max_steps(-1)
+        say(f"[accd certificate] sliding point, thickness {th}, normal speed {speed}")
+        a_c, nc = ctx.ccd(X, P, 1.0, th)
+        _, cert = ctx.ccd_last_stats()
+        say(f"[accd certificate] alpha {a_c!r} (oracle {a_o!r}), certified {cert}")
+    assert nc == nc_o and a_c == a_o
+    if speed == 0.0:
+        assert a_c == 1.0 and cert >= 1
+    else:
+        assert 0.46 < a_c < 0.9 / 1.9
