@@ -833,7 +833,7 @@ __device__ inline void accd_fix_ee(AccdLane& L, double th)
 // iteration's position updates can gather over the steps still to come (27
 // standard deviations of a random walk of half-ulp errors, per coordinate of
 // the four points).  Such a pair ends with no hit, as the steps would.
-constexpr int ACCD_CERT_STEPS = 4096;  // pairs expected to take more steps try the certificate
+constexpr int ACCD_CERT_STEPS = 1024;  // pairs expected to take more steps try the certificate
 constexpr int ACCD_CERT_PIECES = 64;
 
 __device__ inline double ulp_of(double a)
@@ -1046,12 +1046,12 @@ __global__ void __launch_bounds__(256) k_accd(const unsigned* __restrict__ cnt, 
             steps += k;
             slice += k;
             if (r == 0 && L.guard >= L.cert_at) {
-                // a long pair tries the certificate at 4x spaced step counts
+                // a long pair tries the certificate again at 4x the steps it has taken
                 if (accd_certify(L, eta, A.th)) {
                     r = 2;
                     atomicAdd(stepstat + 3, 1ull);
                 } else {
-                    L.cert_at = L.guard > (1 << 28) ? 0x7fffffff : max(4 * L.guard, 4 * ACCD_CERT_STEPS);
+                    L.cert_at = L.guard > (1 << 28) ? 0x7fffffff : max(4 * L.guard, 256);
                 }
             }
             if (r == 1 || r == 3) atomicMin(amin + L.inst, (unsigned long long)__double_as_longlong(L.toc));

@@ -370,7 +370,8 @@ int idp_ccd_stepsize(idp_ctx* ctx, const double* X, const double* p, double thic
  * (IDP_ACCD_MAX_STEPS_DEFAULT), < 0 none (the reference's loop). */
 int idp_set_accd_max_steps(idp_ctx* ctx, int n);
 /* No-hit certificate of long ACCD pairs (on by default): a pair expected to
- * take more than 4096 steps, and every pair again at 4096, 16384, ... steps,
+ * take more than 1024 steps tries it after 32, any other at 1024, and each again
+ * at four times the steps it has taken (at least 256),
  * is ended with no hit when a Bernstein bound shows that its distance stays
  * above the stopping gap up to its bound (the outcome its remaining steps
  * would reach; pairs that hit are always stepped).  No counterpart in the
